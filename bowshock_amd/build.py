@@ -1,0 +1,66 @@
+"""Build recipe of the native library: hipcc for gfx950, in-tree.
+
+``python -m bowshock_amd.build`` compiles ``csrc/*.hip`` into
+``bowshock_amd/lib/libbowshock.so`` (the C ABI of include/bowshock.h) and
+the Node addon ``bowshock_amd/lib/bowshock_napi.node`` (host/addon) when
+the Node headers are present.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB_DIR = os.path.join(HERE, "lib")
+LIB = os.path.join(LIB_DIR, "libbowshock.so")
+ADDON = os.path.join(LIB_DIR, "bowshock_napi.node")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+# Packed FP32 instructions are disabled for every HIP source.
+HIP_FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-Wall", "-Wno-unused-function",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+SOURCES = ["bs_engine.hip"]
+
+
+def _run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_library(verbose=True):
+    os.makedirs(LIB_DIR, exist_ok=True)
+    srcs = [os.path.join(HERE, "csrc", s) for s in SOURCES]
+    tmp = LIB + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *srcs])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_addon():
+    """The N-API addon: plain C against Node's headers, linked to the
+    library by rpath."""
+    inc = "/usr/include/node"
+    src = os.path.join(ROOT, "host", "addon", "bowshock_napi.c")
+    if not (os.path.isdir(inc) and os.path.exists(src)):
+        return None
+    cc = shutil.which("gcc") or "cc"
+    tmp = ADDON + ".tmp"
+    _run([cc, "-O2", "-fPIC", "-shared", "-std=c11", "-Wall", f"-I{inc}", f"-I{os.path.join(ROOT, 'include')}",
+          "-o", tmp, src, f"-L{LIB_DIR}", "-lbowshock", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, ADDON)
+    return ADDON
+
+
+def build():
+    build_library()
+    build_addon()
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

@@ -1,0 +1,967 @@
+// C ABI of the gfx950 Euler engine (include/bowshock.h): solver objects,
+// device memory, kernel launches, batches of steps and their status checks.
+//
+// Host semantics follow the reference's GPU engine
+// (src/engine-gpu/euler2d-gpu.ts) and its time loop
+// (src/engine-gpu/time-loop.ts): steps run in batches of at most 256
+// between readbacks of a small control block (status bits, steps run,
+// FP64 clock); a set status bit or a short batch is an error.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bowshock.h"
+#include "bs_kernels.hpp"
+
+using namespace bs;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+bs_status fail(bs_status code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+bs_status hip_fail(hipError_t e, const char* what) {
+  bs_status code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? BS_ERR_MEMORY : BS_ERR_EXECUTION;
+  return fail(code, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what)                       \
+  do {                                            \
+    hipError_t _e = (expr);                       \
+    if (_e != hipSuccess) return hip_fail(_e, what); \
+  } while (0)
+
+constexpr int BATCH = 256;  // steps between status checks (time-loop.ts:13)
+constexpr int TX = 64, TY = 4;
+
+}  // namespace
+
+struct bs_ctx {
+  int device;
+};
+
+// Device arrays of one solver.
+struct DeviceBuffers {
+  float* state[3] = {nullptr, nullptr, nullptr};  // 4 * n each, SoA
+  float *inx = nullptr, *iny = nullptr, *ia = nullptr, *jnx = nullptr, *jny = nullptr, *ja = nullptr;
+  float *vol = nullptr, *planar = nullptr, *dt = nullptr, *dt_global = nullptr, *dt_partials = nullptr;
+  double* partials = nullptr;
+  double* scalar = nullptr;
+  Control* control = nullptr;
+  float* face_cons[4] = {nullptr, nullptr, nullptr, nullptr};
+  float* face_flux[4] = {nullptr, nullptr, nullptr, nullptr};
+  double* face_geo[4] = {nullptr, nullptr, nullptr, nullptr};
+  double* wall_out = nullptr;
+  // implicit work
+  float *res = nullptr, *diag = nullptr, *lam_i = nullptr, *lam_j = nullptr, *dq = nullptr;
+};
+
+struct Timing {
+  bool enabled = false;
+  std::vector<hipEvent_t> pool;
+  std::vector<int> kinds;  // kind per recorded pair
+  int used = 0;
+  double total_ms[4] = {0, 0, 0, 0};
+  long long launches[4] = {0, 0, 0, 0};
+};
+
+struct bs_solver {
+  bs_config cfg;
+  int device;
+  int ni, nj, n;
+  float gamma32;
+  hipStream_t stream = nullptr;
+  DeviceBuffers d;
+  int cur = 0;
+  bool imposed[4] = {false, false, false, false};
+  std::vector<double> x, y;
+  int tiles_x, tiles_y;
+  std::atomic<bool> busy{false};
+  Timing timing;
+  long long steps_accounted = 0;
+};
+
+namespace {
+
+struct Guard {
+  bs_solver* s;
+  bool ok;
+  explicit Guard(bs_solver* solver) : s(solver) {
+    bool expected = false;
+    ok = s->busy.compare_exchange_strong(expected, true);
+  }
+  ~Guard() {
+    if (ok) s->busy.store(false);
+  }
+};
+
+#define SOLVER_GUARD(s)                                                                        \
+  if ((s) == nullptr) return fail(BS_ERR_RANGE, "no solver");                                  \
+  Guard _guard(s);                                                                             \
+  if (!_guard.ok) return fail(BS_ERR_BUSY, "bs_solver: await the previous operation first"); \
+  (void)hipSetDevice((s)->device)
+
+int along(const bs_solver* s, int side) { return side < 2 ? s->nj : s->ni; }
+
+StateDev state_of(bs_solver* s, int b) {
+  StateDev st;
+  for (int q = 0; q < 4; q++) st.q[q] = s->d.state[b] + (size_t)q * s->n;
+  return st;
+}
+
+GridDev grid_of(bs_solver* s) {
+  GridDev G;
+  G.ni = s->ni;
+  G.nj = s->nj;
+  G.inx = s->d.inx; G.iny = s->d.iny; G.ia = s->d.ia;
+  G.jnx = s->d.jnx; G.jny = s->d.jny; G.ja = s->d.ja;
+  G.vol = s->d.vol; G.planar = s->d.planar;
+  return G;
+}
+
+BoundaryDev boundary_of(bs_solver* s) {
+  BoundaryDev B;
+  for (int k = 0; k < 4; k++) {
+    const bs_boundary& b = s->cfg.boundaries[k];
+    B.kind[k] = b.kind;
+    B.state[k][0] = (float)b.rho;
+    B.state[k][1] = (float)b.u;
+    B.state[k][2] = (float)b.v;
+    B.state[k][3] = (float)b.p;
+    B.face_cons[k] = s->d.face_cons[k];
+    B.face_flux[k] = s->imposed[k] ? s->d.face_flux[k] : nullptr;
+  }
+  return B;
+}
+
+// ---- kernel dispatch tables -------------------------------------------------
+
+using TileFn = void (*)(TileArgs);
+
+template <int REC, int LIM, int FLUX>
+TileFn pick_mode(int mode) {
+  if (mode == TILE_STAGE) return tile_kernel<TX, TY, REC, LIM, FLUX, TILE_STAGE>;
+  if (mode == TILE_IMPLICIT) return tile_kernel<TX, TY, REC, LIM, FLUX, TILE_IMPLICIT>;
+  return tile_kernel<TX, TY, REC, LIM, FLUX, TILE_RESIDUAL>;
+}
+
+template <int FLUX>
+TileFn pick_rec(int rec, int lim, int mode) {
+  if (rec == SHARED) return pick_mode<SHARED, VAN_ALBADA, FLUX>(mode);
+  if (rec == SHARED_SCALED) return pick_mode<SHARED_SCALED, VAN_ALBADA, FLUX>(mode);
+  switch (lim) {
+    case FIRST_ORDER: return pick_mode<PER_VARIABLE, FIRST_ORDER, FLUX>(mode);
+    case MINMOD: return pick_mode<PER_VARIABLE, MINMOD, FLUX>(mode);
+    case VAN_LEER: return pick_mode<PER_VARIABLE, VAN_LEER, FLUX>(mode);
+    case VAN_ALBADA: return pick_mode<PER_VARIABLE, VAN_ALBADA, FLUX>(mode);
+    case MONOTONIZED_CENTRAL: return pick_mode<PER_VARIABLE, MONOTONIZED_CENTRAL, FLUX>(mode);
+    default: return pick_mode<PER_VARIABLE, UNLIMITED_CENTRAL, FLUX>(mode);
+  }
+}
+
+TileFn tile_fn(const bs_config& c, int mode) {
+  return c.flux == BS_FLUX_RUSANOV ? pick_rec<RUSANOV>(c.reconstruction, c.limiter, mode)
+                                   : pick_rec<HLLC>(c.reconstruction, c.limiter, mode);
+}
+
+using WallFn = void (*)(WallArgs, WallBoundary);
+WallFn wall_fn(const bs_config& c) {
+  if (c.flux == BS_FLUX_RUSANOV) {
+    if (c.reconstruction == SHARED) return wall_flux_kernel<SHARED, RUSANOV>;
+    if (c.reconstruction == SHARED_SCALED) return wall_flux_kernel<SHARED_SCALED, RUSANOV>;
+    return wall_flux_kernel<PER_VARIABLE, RUSANOV>;
+  }
+  if (c.reconstruction == SHARED) return wall_flux_kernel<SHARED, HLLC>;
+  if (c.reconstruction == SHARED_SCALED) return wall_flux_kernel<SHARED_SCALED, HLLC>;
+  return wall_flux_kernel<PER_VARIABLE, HLLC>;
+}
+
+// ---- timing -------------------------------------------------------------------
+
+hipEvent_t* timing_begin(bs_solver* s, int kind) {
+  Timing& t = s->timing;
+  if (!t.enabled) return nullptr;
+  if ((size_t)(2 * t.used + 2) > t.pool.size()) {
+    for (int k = 0; k < 64; k++) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      t.pool.push_back(e);
+    }
+  }
+  hipEvent_t* pair = &t.pool[2 * t.used];
+  t.kinds.resize(t.used + 1);
+  t.kinds[t.used] = kind;
+  t.used++;
+  (void)hipEventRecord(pair[0], s->stream);
+  return pair;
+}
+
+void timing_end(bs_solver* s, hipEvent_t* pair) {
+  if (pair) (void)hipEventRecord(pair[1], s->stream);
+}
+
+// After a synchronisation: fold the recorded pairs into the totals.
+void timing_collect(bs_solver* s) {
+  Timing& t = s->timing;
+  for (int k = 0; k < t.used; k++) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, t.pool[2 * k], t.pool[2 * k + 1]) == hipSuccess) {
+      t.total_ms[t.kinds[k]] += ms;
+      t.launches[t.kinds[k]]++;
+    }
+  }
+  t.used = 0;
+}
+
+// ---- launches -----------------------------------------------------------------
+
+TileArgs tile_args(bs_solver* s) {
+  TileArgs A;
+  memset(&A, 0, sizeof A);
+  A.G = grid_of(s);
+  A.B = boundary_of(s);
+  A.gamma = s->gamma32;
+  A.axisymmetric = s->cfg.geometry == BS_GEOMETRY_AXISYMMETRIC;
+  A.limiter = s->cfg.limiter;
+  A.cfl = (float)s->cfg.cfl;
+  A.dt = s->d.dt;
+  A.dt_global = s->d.dt_global;
+  A.control = s->d.control;
+  return A;
+}
+
+bs_status launch_tile(bs_solver* s, int mode, const TileArgs& A, int kind) {
+  dim3 grid(s->tiles_x, s->tiles_y);
+  hipEvent_t* ev = timing_begin(s, kind);
+  hipLaunchKernelGGL(tile_fn(s->cfg, mode), grid, dim3(TX * TY), 0, s->stream, A);
+  timing_end(s, ev);
+  HIP_TRY(hipGetLastError(), "tile kernel launch");
+  return BS_OK;
+}
+
+// Runge-Kutta coefficients (runge-kutta.ts:17-28).
+int rk_table(int integrator, const float** a, const float** b) {
+  static const float ea[] = {0.f}, eb[] = {1.f};
+  static const float a2[] = {0.f, 0.5f}, b2[] = {1.f, 0.5f};
+  static const float a3[] = {0.f, 0.75f, (float)(1.0 / 3.0)}, b3[] = {1.f, 0.25f, (float)(2.0 / 3.0)};
+  if (integrator == BS_RK_SSPRK2) { *a = a2; *b = b2; return 2; }
+  if (integrator == BS_RK_SSPRK3) { *a = a3; *b = b3; return 3; }
+  *a = ea; *b = eb;
+  return 1;
+}
+
+// One explicit step: the time step (global stepping: per-cell steps, their
+// minimum clamped to t_end, then the clock), then the stages. Stage k reads
+// `src` with the stencil and u0 at its own cell only, so the last stage of a
+// multi-stage scheme writes u0 in place (euler2d.ts:1146-1157).
+bs_status encode_step(bs_solver* s) {
+  const float *ca, *cb;
+  const int ns = rk_table(s->cfg.integrator, &ca, &cb);
+  const bool global = s->cfg.time_stepping == BS_STEPPING_GLOBAL;
+  if (global) {
+    const int nb = (s->n + 255) / 256;
+    hipLaunchKernelGGL(time_step_kernel, dim3(nb), dim3(256), 0, s->stream, grid_of(s), state_of(s, s->cur),
+                       s->gamma32, (float)s->cfg.cfl, s->d.dt_partials);
+    hipLaunchKernelGGL(finalize_time_step, dim3(1), dim3(256), 0, s->stream, s->d.dt_partials, nb, s->d.control,
+                       s->d.dt_global);
+  }
+  const int cur = s->cur;
+  const int other[2] = {(cur + 1) % 3, (cur + 2) % 3};
+  int prev = cur;
+  int last = cur;
+  for (int k = 0; k < ns; k++) {
+    const int dst = (k == ns - 1 && k > 0) ? cur : other[k % 2];
+    TileArgs A = tile_args(s);
+    A.u0 = state_of(s, cur);
+    A.src = state_of(s, prev);
+    A.dst = state_of(s, dst);
+    A.a = ca[k];
+    A.b = cb[k];
+    A.dt_mode = global ? DT_GLOBAL : (k == 0 ? DT_COMPUTE_LOCAL : DT_READ);
+    A.count_step = (!global && k == ns - 1) ? 1 : 0;
+    bs_status st = launch_tile(s, TILE_STAGE, A, 0);
+    if (st) return st;
+    prev = dst;
+    last = dst;
+  }
+  if (global) hipLaunchKernelGGL(advance_clock, dim3(1), dim3(1), 0, s->stream, s->d.control);
+  s->cur = last;
+  return BS_OK;
+}
+
+bs_status read_control(bs_solver* s, Control* c) {
+  HIP_TRY(hipMemcpyAsync(c, s->d.control, sizeof(Control), hipMemcpyDeviceToHost, s->stream), "control readback");
+  HIP_TRY(hipStreamSynchronize(s->stream), "stream synchronize");
+  timing_collect(s);
+  return BS_OK;
+}
+
+void fill_report(const Control& c, bs_report* r) {
+  if (!r) return;
+  r->steps = c.steps;
+  r->status = c.status;
+  r->time = c.time;
+  r->dt = c.dt;
+}
+
+// StepLedger.account (time-loop.ts:125-150).
+bs_status account(bs_solver* s, const Control& c, long long submitted) {
+  if (c.status & STATUS_NON_PHYSICAL)
+    return fail(BS_ERR_NONPHYSICAL, "non-physical state on the GPU between steps %lld and %lld",
+                s->steps_accounted, c.steps);
+  if (submitted >= 0 && c.steps != s->steps_accounted + submitted)
+    return fail(BS_ERR_EXECUTION,
+                "the GPU ran %lld of the %lld steps submitted: it rejected the work, ran out of memory or lost the "
+                "device",
+                c.steps - s->steps_accounted, submitted);
+  s->steps_accounted = c.steps;
+  return BS_OK;
+}
+
+bs_status reset_control(bs_solver* s) {
+  Control c;
+  HIP_TRY(hipMemcpy(&c, s->d.control, sizeof c, hipMemcpyDeviceToHost), "control readback");
+  double cfl = c.cfl;
+  memset(&c, 0, sizeof c);
+  c.cfl = cfl > 0 ? cfl : 1.0;
+  c.t_end = 3e38;
+  HIP_TRY(hipMemcpy(s->d.control, &c, sizeof c, hipMemcpyHostToDevice), "control upload");
+  s->steps_accounted = 0;
+  return BS_OK;
+}
+
+template <typename T>
+bs_status dev_alloc(T** p, size_t count, const char* what) {
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T) > 0 ? count * sizeof(T) : 16);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(BS_ERR_MEMORY, "allocating %s (%zu bytes): %s", what, count * sizeof(T), hipGetErrorString(e));
+  }
+  return BS_OK;
+}
+
+void free_all(bs_solver* s) {
+  DeviceBuffers& d = s->d;
+  void* all[] = {d.state[0], d.state[1], d.state[2], d.inx, d.iny, d.ia, d.jnx, d.jny, d.ja, d.vol, d.planar,
+                 d.dt, d.dt_global, d.dt_partials, d.partials, d.scalar, d.control, d.wall_out, d.res, d.diag,
+                 d.lam_i, d.lam_j, d.dq};
+  for (void* p : all)
+    if (p) (void)hipFree(p);
+  for (int k = 0; k < 4; k++) {
+    if (d.face_cons[k]) (void)hipFree(d.face_cons[k]);
+    if (d.face_flux[k]) (void)hipFree(d.face_flux[k]);
+    if (d.face_geo[k]) (void)hipFree(d.face_geo[k]);
+  }
+  d = DeviceBuffers();
+  for (hipEvent_t e : s->timing.pool) (void)hipEventDestroy(e);
+  s->timing.pool.clear();
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  s->stream = nullptr;
+}
+
+// structured.ts:94-168 computeMetrics, FP64 on the host.
+bs_status metrics(int ni, int nj, const double* x, const double* y, int axisym, double* inx, double* iny,
+                  double* ia, double* jnx, double* jny, double* ja, double* vol, double* planar, double* cxo,
+                  double* cyo) {
+  const size_t nn = (size_t)(ni + 1) * (nj + 1);
+  if (axisym)
+    for (size_t k = 0; k < nn; k++)
+      if (y[k] < 0) return fail(BS_ERR_RANGE, "axisymmetric grids must have y >= 0 (y is the radius)");
+  auto node = [&](int i, int j) { return i + j * (ni + 1); };
+  auto face = [&](double* NX, double* NY, double* AR, size_t k, int a, int b) {
+    double dx = x[b] - x[a], dy = y[b] - y[a];
+    double len = std::hypot(dx, dy);
+    if (NX) NX[k] = len > 0 ? dy / len : 0.0;
+    if (NY) NY[k] = len > 0 ? -dx / len : 0.0;
+    if (AR) AR[k] = axisym ? len * 0.5 * (y[a] + y[b]) : len;
+  };
+  for (int j = 0; j < nj; j++)
+    for (int i = 0; i <= ni; i++) face(inx, iny, ia, i + (size_t)j * (ni + 1), node(i, j), node(i, j + 1));
+  for (int j = 0; j <= nj; j++)
+    for (int i = 0; i < ni; i++) face(jnx, jny, ja, i + (size_t)j * ni, node(i + 1, j), node(i, j));
+  for (int j = 0; j < nj; j++)
+    for (int i = 0; i < ni; i++) {
+      const int corners[4] = {node(i, j), node(i + 1, j), node(i + 1, j + 1), node(i, j + 1)};
+      double area = 0, cx = 0, cy = 0;
+      for (int c = 0; c < 4; c++) {
+        int p = corners[c], q = corners[(c + 1) % 4];
+        double cross = x[p] * y[q] - x[q] * y[p];
+        area += cross;
+        cx += (x[p] + x[q]) * cross;
+        cy += (y[p] + y[q]) * cross;
+      }
+      area *= 0.5;
+      if (!(area > 0)) return fail(BS_ERR_RANGE, "cell (%d, %d) has non-positive area %g", i, j, area);
+      size_t k = i + (size_t)j * ni;
+      double gx = cx / (6 * area), gy = cy / (6 * area);
+      if (planar) planar[k] = area;
+      if (cxo) cxo[k] = gx;
+      if (cyo) cyo[k] = gy;
+      if (vol) vol[k] = axisym ? area * gy : area;
+    }
+  return BS_OK;
+}
+
+bs_status upload_f32(float** dst, const std::vector<double>& src, const char* what) {
+  std::vector<float> f(src.begin(), src.end());
+  bs_status st = dev_alloc(dst, f.size(), what);
+  if (st) return st;
+  HIP_TRY(hipMemcpy(*dst, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice), what);
+  return BS_OK;
+}
+
+// Conserved state of a primitive one with the FP32 gamma, as the reference
+// GPU engine packs states (euler2d-gpu.ts:412-432).
+void pack_cons(const double* w, double g32, float out[4]) {
+  out[0] = (float)w[0];
+  out[1] = (float)(w[0] * w[1]);
+  out[2] = (float)(w[0] * w[2]);
+  out[3] = (float)(w[3] / (g32 - 1) + 0.5 * w[0] * (w[1] * w[1] + w[2] * w[2]));
+}
+
+bool physical(const double* w) {
+  return w[0] > 0 && w[3] > 0 && std::isfinite(w[1]) && std::isfinite(w[2]);
+}
+
+}  // namespace
+
+// =============================================================================
+// C ABI
+// =============================================================================
+
+extern "C" {
+
+const char* bs_last_error(void) { return g_last_error.c_str(); }
+
+const char* bs_version(void) { return "bowshock-mi355x 0.1 (gfx950)"; }
+
+bs_status bs_ctx_create(int device, bs_ctx** out) {
+  if (!out) return fail(BS_ERR_RANGE, "no output pointer");
+  *out = nullptr;
+  int count = 0;
+  HIP_TRY(hipGetDeviceCount(&count), "hipGetDeviceCount");
+  if (device < 0 || device >= count) return fail(BS_ERR_RANGE, "device %d out of range (%d devices)", device, count);
+  HIP_TRY(hipSetDevice(device), "hipSetDevice");
+  *out = new bs_ctx{device};
+  return BS_OK;
+}
+
+bs_status bs_ctx_destroy(bs_ctx* ctx) {
+  delete ctx;
+  return BS_OK;
+}
+
+bs_status bs_compute_metrics(int32_t ni, int32_t nj, const double* x, const double* y, int32_t geometry,
+                             double* i_nx, double* i_ny, double* i_area, double* j_nx, double* j_ny,
+                             double* j_area, double* volume, double* planar_area, double* centroid_x,
+                             double* centroid_y) {
+  if (ni < 1 || nj < 1 || !x || !y) return fail(BS_ERR_RANGE, "a grid needs at least one cell in each direction");
+  return metrics(ni, nj, x, y, geometry == BS_GEOMETRY_AXISYMMETRIC, i_nx, i_ny, i_area, j_nx, j_ny, j_area, volume,
+                 planar_area, centroid_x, centroid_y);
+}
+
+bs_status bs_solver_create(bs_ctx* ctx, const bs_config* c, bs_solver** out) {
+  if (!ctx || !c || !out) return fail(BS_ERR_RANGE, "null argument");
+  *out = nullptr;
+  // config.ts:35-55 validateGrid2DConfig and the GPU engine's own checks.
+  if (!(c->cfl > 0 && c->cfl <= 1)) return fail(BS_ERR_RANGE, "cfl must be in (0, 1]");
+  if (c->ni < 2 || c->nj < 2) return fail(BS_ERR_RANGE, "the grid needs at least 2 x 2 cells");
+  if ((c->reconstruction == BS_RECON_SHARED || c->reconstruction == BS_RECON_SHARED_SCALED) &&
+      c->limiter != BS_LIMITER_VAN_ALBADA)
+    return fail(BS_ERR_RANGE, "shared reconstruction requires the 'vanAlbada' limiter");
+  if (c->reconstruction < 0 || c->reconstruction > 2 || c->limiter < 0 || c->limiter > 5)
+    return fail(BS_ERR_RANGE, "unknown reconstruction or limiter");
+  if (c->flux != BS_FLUX_HLLC && c->flux != BS_FLUX_RUSANOV)
+    return fail(BS_ERR_RANGE, "flux %d is not available on the GPU", c->flux);
+  if (c->integrator < 0 || c->integrator > 2) return fail(BS_ERR_RANGE, "unknown time integrator");
+  for (int k = 0; k < 4; k++)
+    if (c->boundaries[k].kind < 0 || c->boundaries[k].kind > 5) return fail(BS_ERR_RANGE, "unknown boundary kind");
+  if ((c->boundaries[0].kind == BS_BC_PERIODIC) != (c->boundaries[1].kind == BS_BC_PERIODIC))
+    return fail(BS_ERR_RANGE, "periodic boundaries must be used on both iMin and iMax");
+  if ((c->boundaries[2].kind == BS_BC_PERIODIC) != (c->boundaries[3].kind == BS_BC_PERIODIC))
+    return fail(BS_ERR_RANGE, "periodic boundaries must be used on both jMin and jMax");
+  if (!c->x || !c->y) return fail(BS_ERR_RANGE, "the grid has no nodes");
+
+  bs_solver* s = new bs_solver();
+  s->cfg = *c;
+  s->device = ctx->device;
+  s->ni = c->ni;
+  s->nj = c->nj;
+  s->n = c->ni * c->nj;
+  s->gamma32 = (float)c->gamma;
+  const int ni = s->ni, nj = s->nj, n = s->n;
+  const size_t nn = (size_t)(ni + 1) * (nj + 1);
+  s->x.assign(c->x, c->x + nn);
+  s->y.assign(c->y, c->y + nn);
+  s->cfg.x = s->x.data();
+  s->cfg.y = s->y.data();
+  s->tiles_x = (ni + TX - 1) / TX;
+  s->tiles_y = (nj + TY - 1) / TY;
+
+  const size_t fi = (size_t)(ni + 1) * nj, fj = (size_t)ni * (nj + 1);
+  std::vector<double> inx(fi), iny(fi), ia(fi), jnx(fj), jny(fj), ja(fj), vol(n), planar(n);
+  bs_status st = metrics(ni, nj, s->x.data(), s->y.data(), c->geometry == BS_GEOMETRY_AXISYMMETRIC, inx.data(),
+                         iny.data(), ia.data(), jnx.data(), jny.data(), ja.data(), vol.data(), planar.data(), nullptr,
+                         nullptr);
+  if (st) {
+    delete s;
+    return st;
+  }
+  (void)hipSetDevice(s->device);
+  auto bail = [&](bs_status code) {
+    free_all(s);
+    delete s;
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(BS_ERR_EXECUTION, "hipStreamCreate failed"));
+  for (int b = 0; b < 3; b++)
+    if ((st = dev_alloc(&s->d.state[b], 4 * (size_t)n, "state"))) return bail(st);
+  if ((st = upload_f32(&s->d.inx, inx, "i-face normals")) || (st = upload_f32(&s->d.iny, iny, "i-face normals")) ||
+      (st = upload_f32(&s->d.ia, ia, "i-face areas")) || (st = upload_f32(&s->d.jnx, jnx, "j-face normals")) ||
+      (st = upload_f32(&s->d.jny, jny, "j-face normals")) || (st = upload_f32(&s->d.ja, ja, "j-face areas")) ||
+      (st = upload_f32(&s->d.vol, vol, "volumes")) || (st = upload_f32(&s->d.planar, planar, "planar areas")))
+    return bail(st);
+  const int blocks = s->tiles_x * s->tiles_y;
+  const int nb256 = (n + 255) / 256;
+  if ((st = dev_alloc(&s->d.dt, (size_t)n, "time steps")) || (st = dev_alloc(&s->d.dt_global, 1, "dt")) ||
+      (st = dev_alloc(&s->d.dt_partials, (size_t)nb256, "dt partials")) ||
+      (st = dev_alloc(&s->d.partials, (size_t)(blocks > nb256 ? blocks : nb256) + 64, "partials")) ||
+      (st = dev_alloc(&s->d.scalar, 4, "scalars")) || (st = dev_alloc(&s->d.control, 1, "control")) ||
+      (st = dev_alloc(&s->d.wall_out, 9 * (size_t)(ni > nj ? ni : nj), "wall fluxes")))
+    return bail(st);
+  HIP_TRY(hipMemset(s->d.control, 0, sizeof(Control)), "control reset");
+  // Per-side FP64 face geometry: midpoint, normal towards increasing
+  // index, area (euler2d.ts:1430-1469 boundaryFace).
+  for (int side = 0; side < 4; side++) {
+    const int m = along(s, side);
+    std::vector<double> geo(5 * (size_t)m);
+    for (int t = 0; t < m; t++) {
+      int n0, n1;
+      double fnx, fny, far;
+      if (side < 2) {
+        int i = side == 0 ? 0 : ni;
+        size_t f = i + (size_t)t * (ni + 1);
+        n0 = (int)f;
+        n1 = (int)f + ni + 1;
+        fnx = inx[f]; fny = iny[f]; far = ia[f];
+      } else {
+        int j = side == 2 ? 0 : nj;
+        size_t f = t + (size_t)j * ni;
+        n0 = t + j * (ni + 1);
+        n1 = n0 + 1;
+        fnx = jnx[f]; fny = jny[f]; far = ja[f];
+      }
+      geo[5 * t + 0] = 0.5 * (s->x[n0] + s->x[n1]);
+      geo[5 * t + 1] = 0.5 * (s->y[n0] + s->y[n1]);
+      geo[5 * t + 2] = fnx;
+      geo[5 * t + 3] = fny;
+      geo[5 * t + 4] = far;
+    }
+    if ((st = dev_alloc(&s->d.face_geo[side], geo.size(), "boundary geometry"))) return bail(st);
+    HIP_TRY(hipMemcpy(s->d.face_geo[side], geo.data(), geo.size() * sizeof(double), hipMemcpyHostToDevice),
+            "boundary geometry");
+    if (c->boundaries[side].kind == BS_BC_FACE_STATES) {
+      if ((st = dev_alloc(&s->d.face_cons[side], 4 * (size_t)m, "face states")) ||
+          (st = dev_alloc(&s->d.face_flux[side], 4 * (size_t)m, "face fluxes")))
+        return bail(st);
+      const double w[4] = {c->boundaries[side].rho, c->boundaries[side].u, c->boundaries[side].v,
+                           c->boundaries[side].p};
+      std::vector<float> packed(4 * (size_t)m);
+      for (int t = 0; t < m; t++) pack_cons(w, (double)s->gamma32, &packed[4 * t]);
+      HIP_TRY(hipMemcpy(s->d.face_cons[side], packed.data(), packed.size() * sizeof(float), hipMemcpyHostToDevice),
+              "face states");
+    }
+  }
+  Control init;
+  memset(&init, 0, sizeof init);
+  init.cfl = 1.0;
+  init.t_end = 3e38;
+  HIP_TRY(hipMemcpy(s->d.control, &init, sizeof init, hipMemcpyHostToDevice), "control init");
+  *out = s;
+  return BS_OK;
+}
+
+bs_status bs_solver_destroy(bs_solver* s) {
+  if (!s) return BS_OK;
+  (void)hipSetDevice(s->device);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  free_all(s);
+  delete s;
+  return BS_OK;
+}
+
+bs_status bs_set_primitives(bs_solver* s, const double* prim) {
+  SOLVER_GUARD(s);
+  if (!prim) return fail(BS_ERR_RANGE, "no states");
+  const int n = s->n;
+  std::vector<float> packed(4 * (size_t)n);
+  for (int c = 0; c < n; c++) {
+    const double* w = prim + 4 * (size_t)c;
+    if (!physical(w)) return fail(BS_ERR_NONPHYSICAL, "non-physical initial state in cell %d", c);
+    float q[4];
+    pack_cons(w, (double)s->gamma32, q);
+    for (int k = 0; k < 4; k++) packed[(size_t)k * n + c] = q[k];
+  }
+  s->cur = 0;
+  HIP_TRY(hipMemcpy(s->d.state[0], packed.data(), packed.size() * sizeof(float), hipMemcpyHostToDevice),
+          "state upload");
+  return reset_control(s);
+}
+
+bs_status bs_set_state(bs_solver* s, const float* rho, const float* mu, const float* mv, const float* energy) {
+  SOLVER_GUARD(s);
+  if (!rho || !mu || !mv || !energy) return fail(BS_ERR_RANGE, "four arrays are needed");
+  const size_t n = s->n;
+  s->cur = 0;
+  const float* src[4] = {rho, mu, mv, energy};
+  for (int q = 0; q < 4; q++)
+    HIP_TRY(hipMemcpy(s->d.state[0] + q * n, src[q], n * sizeof(float), hipMemcpyHostToDevice), "state upload");
+  return reset_control(s);
+}
+
+bs_status bs_read_state(bs_solver* s, float* rho, float* mu, float* mv, float* energy, bs_report* report) {
+  SOLVER_GUARD(s);
+  const size_t n = s->n;
+  float* dst[4] = {rho, mu, mv, energy};
+  for (int q = 0; q < 4; q++)
+    if (dst[q])
+      HIP_TRY(hipMemcpyAsync(dst[q], s->d.state[s->cur] + q * n, n * sizeof(float), hipMemcpyDeviceToHost, s->stream),
+              "state readback");
+  Control c;
+  bs_status st = read_control(s, &c);
+  if (st) return st;
+  fill_report(c, report);
+  if (c.status & STATUS_NON_PHYSICAL) return fail(BS_ERR_NONPHYSICAL, "the GPU solution is non-physical");
+  return BS_OK;
+}
+
+bs_status bs_set_face_states(bs_solver* s, int32_t side, const double* prim, int32_t count) {
+  SOLVER_GUARD(s);
+  if (side < 0 || side > 3 || s->cfg.boundaries[side].kind != BS_BC_FACE_STATES)
+    return fail(BS_ERR_RANGE, "the boundary does not take face states");
+  if (count != along(s, side)) return fail(BS_ERR_RANGE, "the boundary needs %d face states", along(s, side));
+  std::vector<float> packed(4 * (size_t)count);
+  for (int t = 0; t < count; t++) pack_cons(prim + 4 * (size_t)t, (double)s->gamma32, &packed[4 * t]);
+  HIP_TRY(hipMemcpy(s->d.face_cons[side], packed.data(), packed.size() * sizeof(float), hipMemcpyHostToDevice),
+          "face states");
+  return BS_OK;
+}
+
+bs_status bs_set_face_fluxes(bs_solver* s, int32_t side, const double* flux, int32_t count) {
+  SOLVER_GUARD(s);
+  if (side < 0 || side > 3 || s->cfg.boundaries[side].kind != BS_BC_FACE_STATES)
+    return fail(BS_ERR_RANGE, "the boundary does not take face fluxes");
+  if (!flux) {
+    s->imposed[side] = false;
+    return BS_OK;
+  }
+  if (count != 4 * along(s, side)) return fail(BS_ERR_RANGE, "the boundary needs %d flux values", 4 * along(s, side));
+  std::vector<float> f(flux, flux + count);
+  HIP_TRY(hipMemcpy(s->d.face_flux[side], f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice), "face fluxes");
+  s->imposed[side] = true;
+  return BS_OK;
+}
+
+bs_status bs_iterate(bs_solver* s, int32_t count, bs_report* report) {
+  SOLVER_GUARD(s);
+  if (count < 0) return fail(BS_ERR_RANGE, "the step count cannot be negative");
+  Control c;
+  if (s->cfg.time_stepping == BS_STEPPING_GLOBAL) {
+    // iterate() with global steps runs without an end time.
+    HIP_TRY(hipMemcpyAsync(&c, s->d.control, sizeof c, hipMemcpyDeviceToHost, s->stream), "control");
+    HIP_TRY(hipStreamSynchronize(s->stream), "sync");
+    c.t_end = 3e38;
+    HIP_TRY(hipMemcpyAsync(s->d.control, &c, sizeof c, hipMemcpyHostToDevice, s->stream), "control");
+  }
+  int remaining = count;
+  while (remaining > 0) {
+    const int batch = remaining < BATCH ? remaining : BATCH;
+    for (int k = 0; k < batch; k++) {
+      bs_status st = encode_step(s);
+      if (st) return st;
+    }
+    bs_status st = read_control(s, &c);
+    if (st) return st;
+    fill_report(c, report);
+    if ((st = account(s, c, batch))) return st;
+    remaining -= batch;
+  }
+  if (count == 0) {
+    bs_status st = read_control(s, &c);
+    if (st) return st;
+    fill_report(c, report);
+  }
+  return BS_OK;
+}
+
+bs_status bs_advance_to(bs_solver* s, double t_end, int32_t max_steps, bs_report* report) {
+  SOLVER_GUARD(s);
+  if (s->cfg.time_stepping != BS_STEPPING_GLOBAL)
+    return fail(BS_ERR_RANGE, "advanceTo needs global time stepping; use iterate() for local stepping");
+  Control c;
+  HIP_TRY(hipMemcpyAsync(&c, s->d.control, sizeof c, hipMemcpyDeviceToHost, s->stream), "control");
+  HIP_TRY(hipStreamSynchronize(s->stream), "sync");
+  c.t_end = t_end;
+  HIP_TRY(hipMemcpyAsync(s->d.control, &c, sizeof c, hipMemcpyHostToDevice, s->stream), "control");
+  const long long first = c.steps;
+  double dt_est = 0;
+  // advanceInBatches (time-loop.ts:158-177): one step first, then batches
+  // sized from the latest time step, until the clock arrives.
+  while (true) {
+    if (c.time >= t_end) break;
+    if (c.steps - first >= max_steps) return fail(BS_ERR_RANGE, "advanceTo(%g) needed more than %d steps", t_end, max_steps);
+    long long batch = 1;
+    if (dt_est > 0) {
+      double need = std::ceil((t_end - c.time) / dt_est) + 1;
+      batch = need < 1 ? 1 : (need > BATCH ? BATCH : (long long)need);
+    }
+    long long left = max_steps - (c.steps - first);
+    if (batch > left) batch = left;
+    for (long long k = 0; k < batch; k++) {
+      bs_status st = encode_step(s);
+      if (st) return st;
+    }
+    bs_status st = read_control(s, &c);
+    if (st) return st;
+    fill_report(c, report);
+    if (c.status & STATUS_NON_PHYSICAL)
+      return fail(BS_ERR_NONPHYSICAL, "non-physical state on the GPU between steps %lld and %lld",
+                  s->steps_accounted, c.steps);
+    if (c.steps == s->steps_accounted && c.time < t_end)
+      return fail(BS_ERR_EXECUTION, "the GPU clock stopped short of t_end");
+    s->steps_accounted = c.steps;
+    dt_est = c.dt;
+  }
+  fill_report(c, report);
+  return BS_OK;
+}
+
+bs_status bs_residual_norm(bs_solver* s, double* norm) {
+  SOLVER_GUARD(s);
+  if (!norm) return fail(BS_ERR_RANGE, "no output");
+  TileArgs A = tile_args(s);
+  A.src = state_of(s, s->cur);
+  A.partials = s->d.partials;
+  bs_status st = launch_tile(s, TILE_RESIDUAL, A, 0);
+  if (st) return st;
+  const int blocks = s->tiles_x * s->tiles_y;
+  hipLaunchKernelGGL(finalize_sum, dim3(1), dim3(256), 0, s->stream, s->d.partials, blocks, (double)s->n, 1,
+                     s->d.scalar);
+  HIP_TRY(hipMemcpyAsync(norm, s->d.scalar, sizeof(double), hipMemcpyDeviceToHost, s->stream), "norm readback");
+  HIP_TRY(hipStreamSynchronize(s->stream), "sync");
+  timing_collect(s);
+  return BS_OK;
+}
+
+bs_status bs_residual_vector(bs_solver* s, double* residual) {
+  SOLVER_GUARD(s);
+  if (!residual) return fail(BS_ERR_RANGE, "no output");
+  TileArgs A = tile_args(s);
+  A.src = state_of(s, s->cur);
+  // The spare buffer (neither the current state nor needed after) holds it.
+  const int spare = (s->cur + 1) % 3;
+  for (int q = 0; q < 4; q++) A.rvec[q] = s->d.state[spare] + (size_t)q * s->n;
+  bs_status st = launch_tile(s, TILE_RESIDUAL, A, 0);
+  if (st) return st;
+  std::vector<float> r(4 * (size_t)s->n);
+  HIP_TRY(hipMemcpyAsync(r.data(), s->d.state[spare], r.size() * sizeof(float), hipMemcpyDeviceToHost, s->stream),
+          "residual readback");
+  HIP_TRY(hipStreamSynchronize(s->stream), "sync");
+  timing_collect(s);
+  for (int c = 0; c < s->n; c++)
+    for (int q = 0; q < 4; q++) residual[4 * (size_t)c + q] = r[(size_t)q * s->n + c];
+  return BS_OK;
+}
+
+static bs_status wall_launch(bs_solver* s, int side, double* out_dev, double p_inf, double* partials, int* blocks) {
+  WallArgs A;
+  memset(&A, 0, sizeof A);
+  A.G = grid_of(s);
+  A.s = state_of(s, s->cur);
+  A.side = side;
+  A.count = along(s, side);
+  A.kind = s->cfg.boundaries[side].kind;
+  A.face_geo = s->d.face_geo[side];
+  A.gamma = s->cfg.gamma;
+  A.gamma32 = (double)s->gamma32;
+  A.rec = s->cfg.reconstruction;
+  A.limiter = s->cfg.limiter;
+  A.flux = s->cfg.flux;
+  A.out = out_dev;
+  A.p_inf = p_inf;
+  A.partials = partials;
+  WallBoundary WB;
+  for (int k = 0; k < 4; k++) {
+    const bs_boundary& b = s->cfg.boundaries[k];
+    WB.kind[k] = b.kind;
+    WB.state[k][0] = b.rho; WB.state[k][1] = b.u; WB.state[k][2] = b.v; WB.state[k][3] = b.p;
+    WB.cons[k] = s->d.face_cons[k];
+  }
+  *blocks = (A.count + 255) / 256;
+  hipLaunchKernelGGL(wall_fn(s->cfg), dim3(*blocks), dim3(256), 0, s->stream, A, WB);
+  HIP_TRY(hipGetLastError(), "wall flux launch");
+  return BS_OK;
+}
+
+bs_status bs_wall_fluxes(bs_solver* s, int32_t side, double* faces, int32_t count) {
+  SOLVER_GUARD(s);
+  if (side < 0 || side > 3) return fail(BS_ERR_RANGE, "no such side");
+  if (!faces || count != along(s, side)) return fail(BS_ERR_RANGE, "the side has %d faces", along(s, side));
+  if (s->imposed[side]) return fail(BS_ERR_UNSUPPORTED, "the side's fluxes are imposed");
+  int blocks;
+  bs_status st = wall_launch(s, side, s->d.wall_out, 0.0, nullptr, &blocks);
+  if (st) return st;
+  HIP_TRY(hipMemcpyAsync(faces, s->d.wall_out, 9 * (size_t)count * sizeof(double), hipMemcpyDeviceToHost, s->stream),
+          "wall flux readback");
+  HIP_TRY(hipStreamSynchronize(s->stream), "sync");
+  return BS_OK;
+}
+
+bs_status bs_forebody_drag(bs_solver* s, int32_t side, double p_inf, double q_inf, double base_radius, double* cd) {
+  SOLVER_GUARD(s);
+  if (side < 0 || side > 3 || !cd) return fail(BS_ERR_RANGE, "bad side or output");
+  int blocks;
+  bs_status st = wall_launch(s, side, nullptr, p_inf, s->d.partials, &blocks);
+  if (st) return st;
+  hipLaunchKernelGGL(finalize_sum, dim3(1), dim3(256), 0, s->stream, s->d.partials, blocks, 1.0, 0, s->d.scalar);
+  double force = 0;
+  HIP_TRY(hipMemcpyAsync(&force, s->d.scalar, sizeof(double), hipMemcpyDeviceToHost, s->stream), "drag readback");
+  HIP_TRY(hipStreamSynchronize(s->stream), "sync");
+  *cd = (2.0 * force) / (q_inf * base_radius * base_radius);
+  return BS_OK;
+}
+
+bs_status bs_set_implicit_cfl(bs_solver* s, double cfl) {
+  SOLVER_GUARD(s);
+  if (!(cfl > 0)) return fail(BS_ERR_RANGE, "the CFL number must be positive");
+  Control c;
+  HIP_TRY(hipMemcpyAsync(&c, s->d.control, sizeof c, hipMemcpyDeviceToHost, s->stream), "control");
+  HIP_TRY(hipStreamSynchronize(s->stream), "sync");
+  c.cfl = cfl;
+  HIP_TRY(hipMemcpyAsync(s->d.control, &c, sizeof c, hipMemcpyHostToDevice, s->stream), "control");
+  HIP_TRY(hipStreamSynchronize(s->stream), "sync");
+  return BS_OK;
+}
+
+static bs_status ensure_implicit(bs_solver* s) {
+  if (s->d.res) return BS_OK;
+  const size_t n = s->n;
+  bs_status st;
+  if ((st = dev_alloc(&s->d.res, 4 * n, "implicit residual")) || (st = dev_alloc(&s->d.diag, n, "implicit diagonal")) ||
+      (st = dev_alloc(&s->d.lam_i, (size_t)(s->ni + 1) * s->nj, "lambda")) ||
+      (st = dev_alloc(&s->d.lam_j, (size_t)s->ni * (s->nj + 1), "lambda")) ||
+      (st = dev_alloc(&s->d.dq, 4 * n, "implicit dq")))
+    return st;
+  return BS_OK;
+}
+
+bs_status bs_iterate_implicit(bs_solver* s, int32_t count, double cfl_max, int32_t sweeps, double omega, double growth,
+                              bs_report* report) {
+  SOLVER_GUARD(s);
+  if (!(cfl_max > 0 && omega >= 1 && growth >= 1))
+    return fail(BS_ERR_RANGE, "implicit steps need cfl > 0, omega >= 1 and growth >= 1");
+  if (sweeps < 1) return fail(BS_ERR_RANGE, "the relaxation needs a whole number of sweeps, at least one");
+  bs_status st = ensure_implicit(s);
+  if (st) return st;
+  const size_t n = s->n;
+  StateDev res, dq;
+  for (int q = 0; q < 4; q++) {
+    res.q[q] = s->d.res + q * n;
+    dq.q[q] = s->d.dq + q * n;
+  }
+  Control c;
+  int remaining = count;
+  while (remaining > 0) {
+    const int batch = remaining < 64 ? remaining : 64;
+    for (int k = 0; k < batch; k++) {
+      const int src = s->cur, dst = (s->cur + 1) % 3;
+      TileArgs A = tile_args(s);
+      A.src = state_of(s, src);
+      A.omega = (float)omega;
+      for (int q = 0; q < 4; q++) {
+        A.res[q] = res.q[q];
+        A.dq[q] = dq.q[q];
+      }
+      A.diag = s->d.diag;
+      A.lam_i = s->d.lam_i;
+      A.lam_j = s->d.lam_j;
+      if ((st = launch_tile(s, TILE_IMPLICIT, A, 1))) return st;
+      RelaxArgs R;
+      R.G = grid_of(s);
+      R.q = state_of(s, src);
+      for (int q = 0; q < 4; q++) {
+        R.res[q] = res.q[q];
+        R.dq[q] = dq.q[q];
+      }
+      R.diag = s->d.diag;
+      R.lam_i = s->d.lam_i;
+      R.lam_j = s->d.lam_j;
+      R.gamma = s->gamma32;
+      R.omega = (float)omega;
+      dim3 rgrid((s->ni / 2 + 1 + 63) / 64, (s->nj + 3) / 4);
+      // The prepare kernel did the first red half-sweep.
+      for (int sw = 0; sw < sweeps; sw++) {
+        for (int colour = (sw == 0 ? 1 : 0); colour < 2; colour++) {
+          R.colour = colour;
+          hipEvent_t* ev = timing_begin(s, 2);
+          hipLaunchKernelGGL(relax_kernel, rgrid, dim3(64, 4), 0, s->stream, R);
+          timing_end(s, ev);
+        }
+      }
+      hipEvent_t* ev = timing_begin(s, 3);
+      hipLaunchKernelGGL(implicit_update_kernel, dim3((s->n + 255) / 256), dim3(256), 0, s->stream, state_of(s, src),
+                         state_of(s, dst), dq, s->n, s->gamma32, s->d.control, cfl_max, growth);
+      timing_end(s, ev);
+      HIP_TRY(hipGetLastError(), "implicit launch");
+      s->cur = dst;
+    }
+    if ((st = read_control(s, &c))) return st;
+    fill_report(c, report);
+    if ((st = account(s, c, batch))) return st;
+    remaining -= batch;
+  }
+  if (count == 0) {
+    if ((st = read_control(s, &c))) return st;
+    fill_report(c, report);
+  }
+  return BS_OK;
+}
+
+bs_status bs_enable_timing(bs_solver* s, int32_t enable) {
+  SOLVER_GUARD(s);
+  s->timing.enabled = enable != 0;
+  for (int k = 0; k < 4; k++) {
+    s->timing.total_ms[k] = 0;
+    s->timing.launches[k] = 0;
+  }
+  return BS_OK;
+}
+
+bs_status bs_kernel_timing(bs_solver* s, int32_t kernel, double* ms_per_launch, int64_t* launches) {
+  SOLVER_GUARD(s);
+  if (kernel < 0 || kernel > 3) return fail(BS_ERR_RANGE, "no such kernel");
+  long long L = s->timing.launches[kernel];
+  if (ms_per_launch) *ms_per_launch = L > 0 ? s->timing.total_ms[kernel] / (double)L : 0.0;
+  if (launches) *launches = L;
+  s->timing.total_ms[kernel] = 0;
+  s->timing.launches[kernel] = 0;
+  return BS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,648 @@
+// HIP kernels of the Euler path for gfx950 (64-wide wavefronts).
+//
+// State: three FP32 structure-of-arrays buffers (rho, rho u, rho v, E), each
+// component a contiguous array of ni * nj floats indexed i + j * ni, so a
+// wavefront's loads along i are coalesced. Metrics: FP32 unit normals and
+// areas per face set and volume / planar area per cell, computed in FP64 on
+// the host and uploaded once.
+//
+// The explicit stage kernel works on a TX x TY tile of cells: it stages the
+// primitives of the tile and a two-cell halo (ghost cells evaluated on the
+// fly from the boundary conditions) in LDS, computes every face flux of the
+// tile exactly once (each thread its cell's low i- and j-faces, one wave
+// the tile's high edges), keeps flux times area in LDS, and then updates
+// each cell: residual, axisymmetric source, local time step (stage 1),
+// the SSP-RK combination, the physical check and, when asked, the FP64
+// partial sum of the squared density residual.
+#pragma once
+
+#include "bs_numerics.hpp"
+
+namespace bs {
+
+constexpr unsigned STATUS_NON_PHYSICAL = 1u;
+
+struct GridDev {
+  int ni, nj;
+  const float *inx, *iny, *ia;  // i-faces, (ni + 1) * nj
+  const float *jnx, *jny, *ja;  // j-faces, ni * (nj + 1)
+  const float *vol, *planar;    // cells
+};
+
+struct BoundaryDev {
+  int kind[4];
+  float state[4][4];         // primitive (rho, u, v, p) outside inflow / far-field sides
+  const float *face_cons[4]; // FACE_STATES sides: conserved (4 floats) per face
+  const float *face_flux[4]; // imposed flux per unit area (4 floats) per face, or null
+};
+
+struct Control {
+  unsigned status;
+  unsigned pad;
+  long long steps;
+  double time;
+  double dt;
+  double cfl;
+  double t_end;
+};
+
+struct StateDev {
+  float *q[4];
+};
+
+// The conserved state of cell (i, j), or of a ghost cell up to two layers
+// outside one side (never a corner): euler2d.wgsl load_cell, euler2d.ts
+// fillGhosts 1361-1403. Ghost layer g mirrors interior layer g (outflow,
+// wall with the momentum reflected about the boundary face normal), wraps
+// (periodic), or holds the inflow / face / far-field state.
+__device__ __forceinline__ Vec4<float> load_cell(const StateDev& s, const GridDev& G, const BoundaryDev& B, int i,
+                                                 int j, float g) {
+  const int ni = G.ni, nj = G.nj;
+  if (i >= 0 && i < ni && j >= 0 && j < nj) {
+    int c = i + j * ni;
+    return {s.q[0][c], s.q[1][c], s.q[2][c], s.q[3][c]};
+  }
+  int side, t, mi, mj, wi, wj;
+  float nx, ny;
+  if (i < 0) {
+    side = IMIN; t = j; mi = -1 - i; mj = j; wi = i + ni; wj = j;
+    int f = j * (ni + 1);
+    nx = G.inx[f]; ny = G.iny[f];
+  } else if (i >= ni) {
+    side = IMAX; t = j; mi = 2 * ni - 1 - i; mj = j; wi = i - ni; wj = j;
+    int f = ni + j * (ni + 1);
+    nx = G.inx[f]; ny = G.iny[f];
+  } else if (j < 0) {
+    side = JMIN; t = i; mi = i; mj = -1 - j; wi = i; wj = j + nj;
+    nx = G.jnx[i]; ny = G.jny[i];
+  } else {
+    side = JMAX; t = i; mi = i; mj = 2 * nj - 1 - j; wi = i; wj = j - nj;
+    int f = i + nj * ni;
+    nx = G.jnx[f]; ny = G.jny[f];
+  }
+  const int kind = B.kind[side];
+  if (kind == FACE_STATES) {
+    const float* b = B.face_cons[side] + 4 * t;
+    return {b[0], b[1], b[2], b[3]};
+  }
+  if (kind == INFLOW) {
+    Prim<float> w = {B.state[side][0], B.state[side][1], B.state[side][2], B.state[side][3]};
+    return to_cons(w, g);
+  }
+  if (kind == FAR_FIELD) {
+    int ii = side == IMIN ? 0 : side == IMAX ? ni - 1 : i;
+    int jj = side == JMIN ? 0 : side == JMAX ? nj - 1 : j;
+    int c = ii + jj * ni;
+    Prim<float> in = to_prim(s.q[0][c], s.q[1][c], s.q[2][c], s.q[3][c], g);
+    Prim<float> fr = {B.state[side][0], B.state[side][1], B.state[side][2], B.state[side][3]};
+    float o = (side == IMIN || side == JMIN) ? -1.f : 1.f;
+    return to_cons(far_field_state(in, fr, o * nx, o * ny, g), g);
+  }
+  int c = kind == PERIODIC ? wi + wj * ni : mi + mj * ni;
+  Vec4<float> v = {s.q[0][c], s.q[1][c], s.q[2][c], s.q[3][c]};
+  if (kind == WALL) {
+    float mn = v.y * nx + v.z * ny;
+    v.y -= 2.f * mn * nx;
+    v.z -= 2.f * mn * ny;
+  }
+  return v;
+}
+
+__device__ __forceinline__ bool is_physical(const Vec4<float>& q, float g) {
+  Prim<float> w = to_prim(q.x, q.y, q.z, q.w, g);
+  // Comparisons with NaN are false; the bounds reject infinities
+  // (euler2d.wgsl is_physical).
+  return w.rho > 0.f && w.rho < 3.0e38f && w.p > 0.f && w.p < 3.0e38f && bs_abs(w.u) < 3.0e38f &&
+         bs_abs(w.v) < 3.0e38f;
+}
+
+enum DtMode : int { DT_READ = 0, DT_COMPUTE_LOCAL = 1, DT_GLOBAL = 2 };
+enum TileMode : int { TILE_STAGE = 0, TILE_IMPLICIT = 1, TILE_RESIDUAL = 2 };
+
+struct TileArgs {
+  GridDev G;
+  BoundaryDev B;
+  StateDev u0, src, dst;
+  float gamma;
+  int axisymmetric;
+  int limiter;
+  // stage: dst = a u0 + b (src + dt / V R(src))
+  float a, b;
+  int dt_mode;
+  float cfl;
+  float* dt;                 // per cell
+  const float* dt_global;    // DT_GLOBAL
+  Control* control;
+  int count_step;            // this launch completes a step: control->steps++
+  double* partials;          // per block: sum of (R_rho / V)^2, or null
+  // implicit prepare: residual, diagonal, first red half-sweep
+  float* res[4];
+  float* diag;
+  float* lam_i;              // lambda * area per i-face
+  float* lam_j;              // lambda * area per j-face
+  float* dq[4];
+  float omega;
+  // residual vector output (TILE_RESIDUAL)
+  float* rvec[4];
+};
+
+// Block-wide FP64 sum: wave shuffles, then one LDS step across waves.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* scratch) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) scratch[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < NT / 64; w++) s += scratch[w];
+  return s;
+}
+
+template <int TX, int TY, int REC, int LIM, int FLUX, int MODE>
+__global__ __launch_bounds__(TX * TY) void tile_kernel(TileArgs A) {
+  constexpr int NT = TX * TY;
+  constexpr int WX = TX + 4, WY = TY + 4;
+  __shared__ float sp[4][WY][WX];        // primitives of the tile and its halo
+  __shared__ float fi[4][TY][TX + 1];    // i-face flux times area
+  __shared__ float fj[4][TY + 1][TX];    // j-face flux times area
+  __shared__ float xi[2][TY][TX + 1];    // i-face area vector (stage) or lambda A (implicit)
+  __shared__ float xj[2][TY + 1][TX];
+  __shared__ double red[NT / 64];
+
+  const GridDev& G = A.G;
+  const int ni = G.ni, nj = G.nj;
+  const float g = A.gamma;
+  const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+  const int tid = threadIdx.x;
+
+  // Phase 1: primitives of the window [i0-2, i0+TX+2) x [j0-2, j0+TY+2),
+  // corners excluded.
+  for (int k = tid; k < WX * WY; k += NT) {
+    const int li = k % WX, lj = k / WX;
+    const bool ci = li < 2 || li >= TX + 2, cj = lj < 2 || lj >= TY + 2;
+    if (ci && cj) continue;
+    const int gi = i0 + li - 2, gj = j0 + lj - 2;
+    const bool oi = gi < 0 || gi >= ni, oj = gj < 0 || gj >= nj;
+    if ((oi && oj) || gi > ni + 1 || gj > nj + 1) continue;
+    Vec4<float> q = load_cell(A.src, G, A.B, gi, gj, g);
+    Prim<float> w = to_prim(q.x, q.y, q.z, q.w, g);
+    sp[0][lj][li] = w.rho;
+    sp[1][lj][li] = w.u;
+    sp[2][lj][li] = w.v;
+    sp[3][lj][li] = w.p;
+  }
+  __syncthreads();
+
+  auto prim_at = [&](int lj, int li) -> Prim<float> {
+    return {sp[0][lj][li], sp[1][lj][li], sp[2][lj][li], sp[3][lj][li]};
+  };
+
+  // Phase 2: faces. Face f of direction d at local (fx, fy).
+  auto do_face = [&](int d, int fx, int fy) {
+    const int gi = i0 + fx, gj = j0 + fy;
+    float nx, ny, area;
+    Vec4<float> f;
+    float lam = 0.f;
+    if (d == 0) {
+      if (gi > ni || gj >= nj) return;
+      const int fidx = gi + gj * (ni + 1);
+      nx = G.inx[fidx]; ny = G.iny[fidx]; area = G.ia[fidx];
+      const int side = gi == 0 ? IMIN : gi == ni ? IMAX : -1;
+      const Prim<float> l0 = prim_at(fy + 2, fx + 1), r0 = prim_at(fy + 2, fx + 2);
+      if (side >= 0 && A.B.face_flux[side] != nullptr) {
+        const float* b = A.B.face_flux[side] + 4 * gj;
+        f = {b[0], b[1], b[2], b[3]};
+      } else {
+        const int wall = side >= 0 && A.B.kind[side] == WALL ? (side == IMIN ? -1 : 1) : 0;
+        f = face_flux<float, REC, FLUX>(LIM, prim_at(fy + 2, fx), l0, r0, prim_at(fy + 2, fx + 3), nx, ny,
+                                        wall, g);
+      }
+      if (MODE == TILE_IMPLICIT) {
+        float rl = gi > 0 ? bs_abs(l0.u * nx + l0.v * ny) + bs_sqrt(g * l0.p / l0.rho) : 0.f;
+        float rr = gi < ni ? bs_abs(r0.u * nx + r0.v * ny) + bs_sqrt(g * r0.p / r0.rho) : 0.f;
+        lam = bs_max(rl, rr) * area;
+        A.lam_i[fidx] = lam;
+      }
+      fi[0][fy][fx] = f.x * area;
+      fi[1][fy][fx] = f.y * area;
+      fi[2][fy][fx] = f.z * area;
+      fi[3][fy][fx] = f.w * area;
+      if (MODE == TILE_IMPLICIT) {
+        xi[0][fy][fx] = lam;
+      } else {
+        xi[0][fy][fx] = nx * area;
+        xi[1][fy][fx] = ny * area;
+      }
+    } else {
+      if (gi >= ni || gj > nj) return;
+      const int fidx = gi + gj * ni;
+      nx = G.jnx[fidx]; ny = G.jny[fidx]; area = G.ja[fidx];
+      const int side = gj == 0 ? JMIN : gj == nj ? JMAX : -1;
+      const Prim<float> l0 = prim_at(fy + 1, fx + 2), r0 = prim_at(fy + 2, fx + 2);
+      if (side >= 0 && A.B.face_flux[side] != nullptr) {
+        const float* b = A.B.face_flux[side] + 4 * gi;
+        f = {b[0], b[1], b[2], b[3]};
+      } else {
+        const int wall = side >= 0 && A.B.kind[side] == WALL ? (side == JMIN ? -1 : 1) : 0;
+        f = face_flux<float, REC, FLUX>(LIM, prim_at(fy, fx + 2), l0, r0, prim_at(fy + 3, fx + 2), nx, ny,
+                                        wall, g);
+      }
+      if (MODE == TILE_IMPLICIT) {
+        float rl = gj > 0 ? bs_abs(l0.u * nx + l0.v * ny) + bs_sqrt(g * l0.p / l0.rho) : 0.f;
+        float rr = gj < nj ? bs_abs(r0.u * nx + r0.v * ny) + bs_sqrt(g * r0.p / r0.rho) : 0.f;
+        lam = bs_max(rl, rr) * area;
+        A.lam_j[fidx] = lam;
+      }
+      fj[0][fy][fx] = f.x * area;
+      fj[1][fy][fx] = f.y * area;
+      fj[2][fy][fx] = f.z * area;
+      fj[3][fy][fx] = f.w * area;
+      if (MODE == TILE_IMPLICIT) {
+        xj[0][fy][fx] = lam;
+      } else {
+        xj[0][fy][fx] = nx * area;
+        xj[1][fy][fx] = ny * area;
+      }
+    }
+  };
+
+  const int li = tid % TX, lj = tid / TX;
+  do_face(0, li, lj);
+  do_face(1, li, lj);
+  if (tid < TY) do_face(0, TX, tid);
+  else if (tid < TY + TX) do_face(1, tid - TY, TY);
+  __syncthreads();
+
+  // Phase 3: cells.
+  const int gi = i0 + li, gj = j0 + lj;
+  const bool valid = gi < ni && gj < nj;
+  double r2 = 0.0;
+  if (valid) {
+    const int c = gi + gj * ni;
+    const Prim<float> w = prim_at(lj + 2, li + 2);
+    float r[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) r[q] = fi[q][lj][li] - fi[q][lj][li + 1] + fj[q][lj][li] - fj[q][lj + 1][li];
+    const float planar = G.planar[c];
+    if (A.axisymmetric) r[2] += w.p * planar;
+    const float vol = G.vol[c];
+    if (A.partials != nullptr) {
+      double rv = (double)r[0] / (double)vol;
+      r2 = rv * rv;
+    }
+    if (MODE == TILE_STAGE) {
+      float dt;
+      if (A.dt_mode == DT_COMPUTE_LOCAL) {
+        // euler2d.ts:1183-1205 computeTimeSteps: mean area vector of the
+        // two faces in each direction.
+        const float a = bs_sqrt(g * w.p / w.rho);
+        const float six = 0.5f * (xi[0][lj][li] + xi[0][lj][li + 1]);
+        const float siy = 0.5f * (xi[1][lj][li] + xi[1][lj][li + 1]);
+        const float sjx = 0.5f * (xj[0][lj][li] + xj[0][lj + 1][li]);
+        const float sjy = 0.5f * (xj[1][lj][li] + xj[1][lj + 1][li]);
+        const float lam = bs_abs(w.u * six + w.v * siy) + a * bs_sqrt(six * six + siy * siy) +
+                          bs_abs(w.u * sjx + w.v * sjy) + a * bs_sqrt(sjx * sjx + sjy * sjy);
+        dt = A.cfl * vol / lam;
+        A.dt[c] = dt;
+      } else if (A.dt_mode == DT_GLOBAL) {
+        dt = *A.dt_global;
+      } else {
+        dt = A.dt[c];
+      }
+      const float k = dt / vol;
+      Vec4<float> out;
+      float srcq[4] = {A.src.q[0][c], A.src.q[1][c], A.src.q[2][c], A.src.q[3][c]};
+      float o[4];
+      if (A.a != 0.f) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) o[q] = A.a * A.u0.q[q][c] + A.b * (srcq[q] + k * r[q]);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; q++) o[q] = A.b * (srcq[q] + k * r[q]);
+      }
+      if (A.dt_mode == DT_GLOBAL && dt <= 0.f) {
+        // A zero time step (global stepping already at t_end) leaves the
+        // state unchanged (euler2d.wgsl update_cells).
+#pragma unroll
+        for (int q = 0; q < 4; q++) o[q] = A.u0.q[q][c];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++) A.dst.q[q][c] = o[q];
+      out = {o[0], o[1], o[2], o[3]};
+      if (!is_physical(out, g)) atomicOr(&A.control->status, STATUS_NON_PHYSICAL);
+    } else if (MODE == TILE_IMPLICIT) {
+      // euler2d.ts:955-971 / euler2d-implicit.wgsl implicit_prepare: the
+      // scalar diagonal and, with dQ = 0, the first red half-sweep
+      // dQ = R / D on the cells with i + j even.
+      const double cfl = A.control->cfl;
+      const float scale = (float)(0.5 * ((double)A.omega + 1.0 / cfl));
+      float d = scale * (xi[0][lj][li] + xi[0][lj][li + 1] + xj[0][lj][li] + xj[0][lj + 1][li]);
+      if (A.axisymmetric) d += planar * (bs_abs(w.v) + bs_sqrt(g * w.p / w.rho));
+      A.diag[c] = d;
+      const bool red = ((gi + gj) & 1) == 0;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        A.res[q][c] = r[q];
+        A.dq[q][c] = red ? r[q] / d : 0.f;
+      }
+    } else if (A.rvec[0] != nullptr) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) A.rvec[q][c] = r[q];
+    }
+  }
+  if (A.partials != nullptr) {
+    double s = block_sum<NT>(r2, red);
+    if (tid == 0) A.partials[blockIdx.x + blockIdx.y * gridDim.x] = s;
+  }
+  if (A.count_step && blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) A.control->steps += 1;
+}
+
+// Deterministic FP64 sum of n partials by one block; writes sqrt(sum / cells)
+// when `rms` (euler2d.ts:480-491) or the plain sum.
+__global__ __launch_bounds__(256) void finalize_sum(const double* partials, int n, double cells, int rms,
+                                                   double* out) {
+  __shared__ double red[4];
+  double v = 0.0;
+  for (int k = threadIdx.x; k < n; k += 256) v += partials[k];
+  double s = block_sum<256>(v, red);
+  if (threadIdx.x == 0) *out = rms ? sqrt(s / cells) : s;
+}
+
+// Per-cell global time step and block minima (euler2d.wgsl
+// compute_time_steps); the finalize takes the minimum and clamps it to the
+// FP64 clock's remaining time (finalize_time_step).
+__global__ __launch_bounds__(256) void time_step_kernel(GridDev G, StateDev s, float g, float cfl,
+                                                        float* partial_min) {
+  __shared__ float red[256];
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  const int ni = G.ni;
+  float dt = 3.0e38f;
+  if (c < G.ni * G.nj) {
+    const int i = c % ni, j = c / ni;
+    Prim<float> w = to_prim(s.q[0][c], s.q[1][c], s.q[2][c], s.q[3][c], g);
+    float a = bs_sqrt(g * w.p / w.rho);
+    int f0 = i + j * (ni + 1), f1 = f0 + 1, h0 = c, h1 = c + ni;
+    float six = 0.5f * (G.inx[f0] * G.ia[f0] + G.inx[f1] * G.ia[f1]);
+    float siy = 0.5f * (G.iny[f0] * G.ia[f0] + G.iny[f1] * G.ia[f1]);
+    float sjx = 0.5f * (G.jnx[h0] * G.ja[h0] + G.jnx[h1] * G.ja[h1]);
+    float sjy = 0.5f * (G.jny[h0] * G.ja[h0] + G.jny[h1] * G.ja[h1]);
+    float lam = bs_abs(w.u * six + w.v * siy) + a * bs_sqrt(six * six + siy * siy) +
+                bs_abs(w.u * sjx + w.v * sjy) + a * bs_sqrt(sjx * sjx + sjy * sjy);
+    dt = cfl * G.vol[c] / lam;
+    if (!(dt == dt)) dt = -1.f;  // NaN marks the block
+  }
+  red[threadIdx.x] = dt;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] = fminf(red[threadIdx.x], red[threadIdx.x + st]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial_min[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void finalize_time_step(const float* partial_min, int n, Control* ctl,
+                                                          float* dt_out) {
+  __shared__ float red[256];
+  float v = 3.0e38f;
+  for (int k = threadIdx.x; k < n; k += 256) v = fminf(v, partial_min[k]);
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] = fminf(red[threadIdx.x], red[threadIdx.x + st]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    float m = red[0];
+    double dt;
+    if (!(m > 0.f && m < 3.0e38f)) {
+      atomicOr(&ctl->status, STATUS_NON_PHYSICAL);
+      dt = 0.0;
+    } else {
+      double remaining = ctl->t_end - ctl->time;
+      double limit = 1e-12 * (fabs(ctl->t_end) > 1.0 ? fabs(ctl->t_end) : 1.0);
+      dt = remaining <= limit ? 0.0 : (m < remaining ? (double)m : remaining);
+    }
+    ctl->dt = dt;
+    *dt_out = (float)dt;
+  }
+}
+
+// After the last stage of a global step: FP64 clock and step count; a step
+// of zero length (already at t_end) is not counted.
+__global__ void advance_clock(Control* ctl) {
+  if (ctl->dt > 0.0) {
+    ctl->time += ctl->dt;
+    double limit = 1e-12 * (fabs(ctl->t_end) > 1.0 ? fabs(ctl->t_end) : 1.0);
+    if (ctl->t_end - ctl->time < limit) ctl->time = ctl->t_end;
+    ctl->steps += 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Implicit red-black relaxation (euler2d.ts:1007-1056, euler2d-implicit.wgsl
+// implicit_relax / implicit_update).
+// ---------------------------------------------------------------------------
+
+struct RelaxArgs {
+  GridDev G;
+  StateDev q;
+  const float* res[4];
+  const float* diag;
+  const float* lam_i;
+  const float* lam_j;
+  float* dq[4];
+  float gamma, omega;
+  int colour;
+};
+
+__device__ __forceinline__ void neighbour_term(const RelaxArgs& A, int nb, float nx, float ny, float area,
+                                               float lam_a, float sum[4]) {
+  const float g = A.gamma;
+  const float rho = A.q.q[0][nb], mx = A.q.q[1][nb], my = A.q.q[2][nb], e = A.q.q[3][nb];
+  const float d0 = A.dq[0][nb], d1 = A.dq[1][nb], d2 = A.dq[2][nb], d3 = A.dq[3][nb];
+  const float p0 = (g - 1.f) * (e - 0.5f * (mx * mx + my * my) / rho);
+  const float un0 = (mx * nx + my * ny) / rho;
+  const float rho1 = rho + d0, mx1 = mx + d1, my1 = my + d2, e1 = e + d3;
+  const float p1 = (g - 1.f) * (e1 - 0.5f * (mx1 * mx1 + my1 * my1) / rho1);
+  const float un1 = (mx1 * nx + my1 * ny) / rho1;
+  const float half = 0.5f * area, damp = 0.5f * A.omega * lam_a;
+  sum[0] += half * (rho1 * un1 - rho * un0) - damp * d0;
+  sum[1] += half * (mx1 * un1 + p1 * nx - (mx * un0 + p0 * nx)) - damp * d1;
+  sum[2] += half * (my1 * un1 + p1 * ny - (my * un0 + p0 * ny)) - damp * d2;
+  sum[3] += half * ((e1 + p1) * un1 - (e + p0) * un0) - damp * d3;
+}
+
+// One half-sweep over the cells with (i + j) % 2 == colour; thread x of a
+// row handles the x-th cell of that colour in the row.
+__global__ __launch_bounds__(256) void relax_kernel(RelaxArgs A) {
+  const int ni = A.G.ni, nj = A.G.nj;
+  const int j = blockIdx.y * 4 + threadIdx.y;
+  const int i = 2 * (blockIdx.x * 64 + threadIdx.x) + ((j + A.colour) & 1);
+  if (i >= ni || j >= nj) return;
+  const GridDev& G = A.G;
+  const int c = i + j * ni;
+  float sum[4] = {0.f, 0.f, 0.f, 0.f};
+  if (i > 0) {
+    int f = i + j * (ni + 1);
+    neighbour_term(A, c - 1, -G.inx[f], -G.iny[f], G.ia[f], A.lam_i[f], sum);
+  }
+  if (i < ni - 1) {
+    int f = i + 1 + j * (ni + 1);
+    neighbour_term(A, c + 1, G.inx[f], G.iny[f], G.ia[f], A.lam_i[f], sum);
+  }
+  if (j > 0) neighbour_term(A, c - ni, -G.jnx[c], -G.jny[c], G.ja[c], A.lam_j[c], sum);
+  if (j < nj - 1) {
+    int f = c + ni;
+    neighbour_term(A, c + ni, G.jnx[f], G.jny[f], G.ja[f], A.lam_j[f], sum);
+  }
+  const float d = A.diag[c];
+#pragma unroll
+  for (int q = 0; q < 4; q++) A.dq[q][c] = (A.res[q][c] - sum[q]) / d;
+}
+
+// q + alpha dQ, alpha halved (at most 30 times) until density and pressure
+// stay within a factor of two (euler2d.ts:816-847); the CFL ramp and the
+// step count by one thread.
+__global__ __launch_bounds__(256) void implicit_update_kernel(StateDev src, StateDev dst, StateDev dq, int n,
+                                                              float g, Control* ctl, double cfl_max, double growth) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c < n) {
+    const float rho = src.q[0][c], mx = src.q[1][c], my = src.q[2][c], e = src.q[3][c];
+    const float d0 = dq.q[0][c], d1 = dq.q[1][c], d2 = dq.q[2][c], d3 = dq.q[3][c];
+    const float p = (g - 1.f) * (e - 0.5f * (mx * mx + my * my) / rho);
+    float alpha = 1.f;
+    for (int tries = 0; tries < 30; tries++) {
+      float rho1 = rho + alpha * d0, mx1 = mx + alpha * d1, my1 = my + alpha * d2, e1 = e + alpha * d3;
+      float p1 = (g - 1.f) * (e1 - 0.5f * (mx1 * mx1 + my1 * my1) / rho1);
+      if (rho1 > 0.5f * rho && rho1 < 2.f * rho && p1 > 0.5f * p && p1 < 2.f * p) break;
+      alpha *= 0.5f;
+    }
+    Vec4<float> o = {rho + alpha * d0, mx + alpha * d1, my + alpha * d2, e + alpha * d3};
+    dst.q[0][c] = o.x;
+    dst.q[1][c] = o.y;
+    dst.q[2][c] = o.z;
+    dst.q[3][c] = o.w;
+    if (!is_physical(o, g)) atomicOr(&ctl->status, STATUS_NON_PHYSICAL);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    double next = ctl->cfl * growth;
+    ctl->cfl = next < cfl_max ? next : cfl_max;
+    ctl->steps += 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 boundary fluxes and drag (euler2d.ts:499-525, forces.ts:19-29).
+// ---------------------------------------------------------------------------
+
+struct WallArgs {
+  GridDev G;
+  StateDev s;
+  int side;
+  int count;
+  int kind;                 // boundary kind of the side
+  double state[4];          // primitive outside an inflow / far-field side
+  const float* face_cons;   // FACE_STATES side
+  const double* face_geo;   // per face: x, y, nx, ny, area (FP64, normals towards increasing index)
+  double gamma;             // configured gamma (FP64)
+  double gamma32;           // the FP32 gamma the state was computed with
+  int rec, limiter, flux;
+  double* out;              // 9 per face, or null
+  double p_inf;
+  double* partials;         // per block: sum of (F_x - p_inf n_x) A, or null
+};
+
+template <typename T>
+__device__ __forceinline__ Prim<T> prim64(const StateDev& s, int c, T g32) {
+  T rho = (T)s.q[0][c], mu = (T)s.q[1][c], mv = (T)s.q[2][c], e = (T)s.q[3][c];
+  return {rho, mu / rho, mv / rho, (g32 - T(1)) * (e - (T(0.5) * (mu * mu + mv * mv)) / rho)};
+}
+
+// Primitive state of cell (i, j) or a ghost cell, in FP64, with the
+// boundary conditions of euler2d.ts:1361-1403 applied to primitives.
+__device__ Prim<double> wall_cell(const WallArgs& A, const int bkind[4], const double bstate[4][4],
+                                  const float* const bcons[4], int i, int j) {
+  const int ni = A.G.ni, nj = A.G.nj;
+  if (i >= 0 && i < ni && j >= 0 && j < nj) return prim64<double>(A.s, i + j * ni, A.gamma32);
+  int side, t, mi, mj, wi, wj;
+  double nx, ny;
+  if (i < 0) { side = IMIN; t = j; mi = -1 - i; mj = j; wi = i + ni; wj = j; }
+  else if (i >= ni) { side = IMAX; t = j; mi = 2 * ni - 1 - i; mj = j; wi = i - ni; wj = j; }
+  else if (j < 0) { side = JMIN; t = i; mi = i; mj = -1 - j; wi = i; wj = j + nj; }
+  else { side = JMAX; t = i; mi = i; mj = 2 * nj - 1 - j; wi = i; wj = j - nj; }
+  if (side < 2) {
+    int f = (side == IMIN ? 0 : ni) + t * (ni + 1);
+    nx = A.G.inx[f]; ny = A.G.iny[f];
+  } else {
+    int f = t + (side == JMIN ? 0 : nj) * ni;
+    nx = A.G.jnx[f]; ny = A.G.jny[f];
+  }
+  if (side == A.side) {
+    // The evaluated side's normals in FP64.
+    nx = A.face_geo[5 * t + 2];
+    ny = A.face_geo[5 * t + 3];
+  }
+  const int kind = bkind[side];
+  if (kind == INFLOW) return {bstate[side][0], bstate[side][1], bstate[side][2], bstate[side][3]};
+  if (kind == FACE_STATES) {
+    const float* b = bcons[side] + 4 * t;
+    double rho = b[0], mu = b[1], mv = b[2], e = b[3];
+    return {rho, mu / rho, mv / rho, (A.gamma32 - 1.0) * (e - (0.5 * (mu * mu + mv * mv)) / rho)};
+  }
+  if (kind == FAR_FIELD) {
+    int ii = side == IMIN ? 0 : side == IMAX ? ni - 1 : i;
+    int jj = side == JMIN ? 0 : side == JMAX ? nj - 1 : j;
+    Prim<double> in = prim64<double>(A.s, ii + jj * ni, A.gamma32);
+    Prim<double> fr = {bstate[side][0], bstate[side][1], bstate[side][2], bstate[side][3]};
+    double o = (side == IMIN || side == JMIN) ? -1.0 : 1.0;
+    return far_field_state<double>(in, fr, o * nx, o * ny, A.gamma);
+  }
+  Prim<double> w = prim64<double>(A.s, kind == PERIODIC ? wi + wj * ni : mi + mj * ni, A.gamma32);
+  if (kind == WALL) {
+    double un = w.u * nx + w.v * ny;
+    w.u -= 2.0 * un * nx;
+    w.v -= 2.0 * un * ny;
+  }
+  return w;
+}
+
+struct WallBoundary {
+  int kind[4];
+  double state[4][4];
+  const float* cons[4];
+};
+
+template <int REC, int FLUX>
+__global__ __launch_bounds__(256) void wall_flux_kernel(WallArgs A, WallBoundary WB) {
+  __shared__ double red[4];
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  double contrib = 0.0;
+  if (t < A.count) {
+    const int ni = A.G.ni, nj = A.G.nj;
+    int i, j, di, dj;
+    if (A.side < 2) { i = A.side == IMIN ? 0 : ni; j = t; di = 1; dj = 0; }
+    else { i = t; j = A.side == JMIN ? 0 : nj; di = 0; dj = 1; }
+    Prim<double> ll = wall_cell(A, WB.kind, WB.state, WB.cons, i - 2 * di, j - 2 * dj);
+    Prim<double> l0 = wall_cell(A, WB.kind, WB.state, WB.cons, i - di, j - dj);
+    Prim<double> r0 = wall_cell(A, WB.kind, WB.state, WB.cons, i, j);
+    Prim<double> rr = wall_cell(A, WB.kind, WB.state, WB.cons, i + di, j + dj);
+    const double* geo = A.face_geo + 5 * t;
+    const double nx = geo[2], ny = geo[3], area = geo[4];
+    const int wall = A.kind == WALL ? ((A.side == IMIN || A.side == JMIN) ? -1 : 1) : 0;
+    Vec4<double> f = face_flux<double, REC, FLUX>(A.limiter, ll, l0, r0, rr, nx, ny, wall, A.gamma);
+    const double o = (A.side == IMIN || A.side == JMIN) ? -1.0 : 1.0;
+    if (A.out != nullptr) {
+      double* w = A.out + 9 * t;
+      w[0] = geo[0]; w[1] = geo[1]; w[2] = o * nx; w[3] = o * ny; w[4] = area;
+      w[5] = o * f.x; w[6] = o * f.y; w[7] = o * f.z; w[8] = o * f.w;
+    }
+    contrib = (o * f.y - A.p_inf * o * nx) * area;
+  }
+  if (A.partials != nullptr) {
+    double s = block_sum<256>(contrib, red);
+    if (threadIdx.x == 0) A.partials[blockIdx.x] = s;
+  }
+}
+
+}  // namespace bs

@@ -1,0 +1,309 @@
+"""GpuEuler2D over the HIP engine: the reference's GPU engine interface
+(src/engine-gpu/euler2d-gpu.ts, class GpuEuler2D) for Python callers.
+
+Method names, argument meaning and errors follow the reference: a config
+object with the fields of Euler2DConfig (euler2d.ts:58-81), boundaries per
+side as ``{"kind": ..., "state": (rho, u, v, p)}``, primitive states as
+(rho, u, v, p), conserved state read back as FP32 arrays indexed
+``i + j * ni`` (GpuState2D, euler2d-gpu.ts:116-125). Calls are synchronous;
+a call made while another is running on the same solver raises BusyError,
+as the reference's ExclusiveRunner does (gpu-common.ts:122-140).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Callable, Mapping, Optional
+
+import numpy as np
+
+from . import _native as N
+
+SIDES = ("iMin", "iMax", "jMin", "jMax")
+BC_KINDS = {"inflow": 0, "outflow": 1, "wall": 2, "periodic": 3, "faceStates": 4, "farField": 5}
+LIMITERS = {"firstOrder": 0, "minmod": 1, "vanLeer": 2, "vanAlbada": 3, "monotonizedCentral": 4,
+            "unlimitedCentral": 5}
+RECONSTRUCTIONS = {"perVariable": 0, "shared": 1, "sharedScaled": 2}
+FLUXES = {"hllc": 0, "rusanov": 1}
+INTEGRATORS = {"forwardEuler": 0, "ssprk2": 1, "ssprk3": 2}
+KERNELS = {"stage": 0, "implicitPrepare": 1, "implicitRelax": 2, "implicitUpdate": 3}
+
+
+class RangeError(ValueError):
+    """Invalid configuration or argument (the reference's RangeError)."""
+
+
+class NonPhysicalStateError(RuntimeError):
+    """src/solver/errors.ts:2-7."""
+
+
+class GpuMemoryLimitError(MemoryError):
+    """src/engine-gpu/gpu-common.ts:7."""
+
+
+class GpuExecutionError(RuntimeError):
+    """src/engine-gpu/gpu-common.ts:10."""
+
+
+class GpuPrecisionError(RuntimeError):
+    """src/engine-gpu/gpu-common.ts:17."""
+
+
+class BusyError(RuntimeError):
+    """A call overlapped another on the same solver."""
+
+
+class UnsupportedError(NotImplementedError):
+    pass
+
+
+_ERRORS = {N.BS_ERR_RANGE: RangeError, N.BS_ERR_NONPHYSICAL: NonPhysicalStateError,
+           N.BS_ERR_MEMORY: GpuMemoryLimitError, N.BS_ERR_EXECUTION: GpuExecutionError,
+           N.BS_ERR_PRECISION: GpuPrecisionError, N.BS_ERR_BUSY: BusyError,
+           N.BS_ERR_UNSUPPORTED: UnsupportedError}
+
+
+def check(rc: int):
+    if rc != N.BS_OK:
+        raise _ERRORS.get(rc, RuntimeError)(N.last_error())
+
+
+@dataclass
+class GpuState2D:
+    """euler2d-gpu.ts:116-125."""
+
+    time: float
+    steps: int
+    rho: np.ndarray
+    mu: np.ndarray
+    mv: np.ndarray
+    energy: np.ndarray
+
+
+@dataclass
+class Euler2DConfig:
+    """euler2d.ts:58-81 / GpuEuler2DConfig euler2d-gpu.ts:82-92."""
+
+    grid: object  # .ni, .nj, .x, .y
+    geometry: str = "axisymmetric"
+    gamma: float = 1.4
+    flux: str = "hllc"
+    limiter: str = "minmod"
+    reconstruction: str = "perVariable"
+    timeIntegrator: str = "ssprk3"
+    cfl: float = 0.5
+    timeStepping: str = "local"
+    boundaries: Mapping[str, Mapping] = field(default_factory=dict)
+
+
+class Context:
+    """A HIP device (the reference's GPUDevice)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(N.lib().bs_ctx_create(device, C.byref(h)))
+        self._h = h
+        self.device = device
+
+    def close(self):
+        if self._h:
+            N.lib().bs_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _c_config(cfg: Euler2DConfig, keep: list) -> N.Config:
+    c = N.Config()
+    g = cfg.grid
+    x = np.ascontiguousarray(g.x, dtype=np.float64)
+    y = np.ascontiguousarray(g.y, dtype=np.float64)
+    keep += [x, y]
+    c.ni, c.nj = g.ni, g.nj
+    c.x, c.y = N.dptr(x), N.dptr(y)
+    c.geometry = 1 if cfg.geometry == "axisymmetric" else 0
+    c.gamma = cfg.gamma
+    if cfg.flux not in FLUXES:
+        raise RangeError(f"flux {cfg.flux} is not available on the GPU")
+    c.flux = FLUXES[cfg.flux]
+    c.limiter = LIMITERS[cfg.limiter]
+    c.reconstruction = RECONSTRUCTIONS[cfg.reconstruction]
+    c.integrator = INTEGRATORS[cfg.timeIntegrator]
+    c.cfl = cfg.cfl
+    c.time_stepping = 1 if cfg.timeStepping == "local" else 0
+    for k, side in enumerate(SIDES):
+        bc = cfg.boundaries[side]
+        c.boundaries[k].kind = BC_KINDS[bc["kind"]]
+        st = bc.get("state")
+        if st is None and bc["kind"] == "faceStates" and bc.get("states") is not None:
+            st = bc["states"][0]
+        st = st if st is not None else (0.0, 0.0, 0.0, 0.0)
+        c.boundaries[k].rho, c.boundaries[k].u, c.boundaries[k].v, c.boundaries[k].p = map(float, st)
+    return c
+
+
+def compute_metrics(grid, geometry="axisymmetric"):
+    """GridMetrics (structured.ts:28-50) in FP64, from the library."""
+    ni, nj = grid.ni, grid.nj
+    fi, fj, n = (ni + 1) * nj, ni * (nj + 1), ni * nj
+    out = {k: np.zeros(m) for k, m in (("iNx", fi), ("iNy", fi), ("iArea", fi), ("jNx", fj), ("jNy", fj),
+                                        ("jArea", fj), ("volume", n), ("planarArea", n), ("centroidX", n),
+                                        ("centroidY", n))}
+    x = np.ascontiguousarray(grid.x, dtype=np.float64)
+    y = np.ascontiguousarray(grid.y, dtype=np.float64)
+    check(N.lib().bs_compute_metrics(ni, nj, N.dptr(x), N.dptr(y), 1 if geometry == "axisymmetric" else 0,
+                                     *[N.dptr(v) for v in out.values()]))
+    return out
+
+
+class GpuEuler2D:
+    """The HIP twin of the reference's GpuEuler2D."""
+
+    def __init__(self, ctx: Context, config: Euler2DConfig):
+        self.config = config
+        self.ctx = ctx
+        keep: list = []
+        c = _c_config(config, keep)
+        h = C.c_void_p()
+        check(N.lib().bs_solver_create(ctx._h, C.byref(c), C.byref(h)))
+        self._h = h
+        self.ni, self.nj = config.grid.ni, config.grid.nj
+        self.metrics = compute_metrics(config.grid, config.geometry)
+        self.last_report = N.Report()
+        for side in SIDES:
+            bc = config.boundaries[side]
+            if bc["kind"] == "faceStates" and bc.get("states") is not None:
+                self.setFaceStates(side, bc["states"])
+
+    @staticmethod
+    def create(ctx: Context, config: Euler2DConfig) -> "GpuEuler2D":
+        """euler2d-gpu.ts:217-291."""
+        return GpuEuler2D(ctx, config)
+
+    @property
+    def n(self):
+        return self.ni * self.nj
+
+    def destroy(self):
+        """euler2d-gpu.ts:659."""
+        if getattr(self, "_h", None):
+            N.lib().bs_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def setCellPrimitives(self, cell):
+        """euler2d-gpu.ts:439-459: cell(i, j) -> (rho, u, v, p), or an
+        (n, 4) array of primitives in cell order."""
+        if callable(cell):
+            prim = np.array([cell(c % self.ni, c // self.ni) for c in range(self.n)], dtype=np.float64)
+        else:
+            prim = np.ascontiguousarray(np.broadcast_to(np.asarray(cell, dtype=np.float64), (self.n, 4)))
+        prim = np.ascontiguousarray(prim, dtype=np.float64)
+        check(N.lib().bs_set_primitives(self._h, N.dptr(prim)))
+
+    def setInitialCondition(self, field: Callable):
+        """euler2d-gpu.ts:412-432: field(x, y) at the cell centroids."""
+        cx, cy = self.metrics["centroidX"], self.metrics["centroidY"]
+        self.setCellPrimitives(np.array([field(a, b) for a, b in zip(cx, cy)], dtype=np.float64))
+
+    def setState(self, rho, mu, mv, energy):
+        arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (rho, mu, mv, energy)]
+        check(N.lib().bs_set_state(self._h, *[N.fptr(a) for a in arrs]))
+
+    def iterate(self, count: int):
+        """euler2d-gpu.ts:462-473."""
+        check(N.lib().bs_iterate(self._h, int(count), C.byref(self.last_report)))
+
+    def advanceTo(self, tEnd: float, maxSteps: int = 1_000_000):
+        """euler2d-gpu.ts:511-527."""
+        check(N.lib().bs_advance_to(self._h, float(tEnd), int(maxSteps), C.byref(self.last_report)))
+
+    def densityResidualNorm(self) -> float:
+        """euler2d-gpu.ts:530-546 (FP64 reduction on the GPU)."""
+        out = C.c_double()
+        check(N.lib().bs_residual_norm(self._h, C.byref(out)))
+        return out.value
+
+    def residualVector(self) -> np.ndarray:
+        """euler2d.ts:462-477: (n, 4)."""
+        r = np.zeros(4 * self.n)
+        check(N.lib().bs_residual_vector(self._h, N.dptr(r)))
+        return r.reshape(self.n, 4)
+
+    def readState(self) -> GpuState2D:
+        """euler2d-gpu.ts:549-572."""
+        a = [np.zeros(self.n, dtype=np.float32) for _ in range(4)]
+        rep = N.Report()
+        check(N.lib().bs_read_state(self._h, *[N.fptr(v) for v in a], C.byref(rep)))
+        return GpuState2D(rep.time, int(rep.steps), *a)
+
+    def setFaceStates(self, side: str, states):
+        """euler2d-gpu.ts:575-600."""
+        a = np.ascontiguousarray(states, dtype=np.float64).reshape(-1)
+        check(N.lib().bs_set_face_states(self._h, SIDES.index(side), N.dptr(a), len(a) // 4))
+
+    def setFaceFluxes(self, side: str, fluxes):
+        """euler2d-gpu.ts:606-630."""
+        k = SIDES.index(side)
+        if fluxes is None:
+            check(N.lib().bs_set_face_fluxes(self._h, k, None, 0))
+            return
+        a = np.ascontiguousarray(fluxes, dtype=np.float64).reshape(-1)
+        check(N.lib().bs_set_face_fluxes(self._h, k, N.dptr(a), len(a)))
+
+    def wallFluxes(self, side: str = "jMin") -> np.ndarray:
+        """Euler2D.boundaryFluxes (euler2d.ts:499-525) evaluated on the GPU
+        in FP64: (faces, 9) rows of x, y, nx, ny, area, flux[4]."""
+        k = SIDES.index(side)
+        m = self.nj if k < 2 else self.ni
+        out = np.zeros(9 * m)
+        check(N.lib().bs_wall_fluxes(self._h, k, N.dptr(out), m))
+        return out.reshape(m, 9)
+
+    def forebodyDrag(self, pressure: float, dynamicPressure: float, baseRadius: float, side: str = "jMin") -> float:
+        """forces.ts:19-29 forebodyPressureDrag over the side's faces, FP64 on
+        the GPU."""
+        out = C.c_double()
+        check(N.lib().bs_forebody_drag(self._h, SIDES.index(side), pressure, dynamicPressure, baseRadius,
+                                       C.byref(out)))
+        return out.value
+
+    def setImplicitCfl(self, cfl: float):
+        """euler2d-gpu.ts:668-681."""
+        check(N.lib().bs_set_implicit_cfl(self._h, float(cfl)))
+
+    def iterateImplicit(self, count: int, cfl: float, sweeps: int, omega: float = 1.5, growth: float = 1.2):
+        """euler2d-gpu.ts:687-723."""
+        check(N.lib().bs_iterate_implicit(self._h, int(count), float(cfl), int(sweeps), float(omega), float(growth),
+                                          C.byref(self.last_report)))
+
+    def enableTiming(self, on: bool = True):
+        check(N.lib().bs_enable_timing(self._h, 1 if on else 0))
+
+    def kernelTiming(self, kernel: str = "stage"):
+        """(ms per launch, launches) since the last call."""
+        ms = C.c_double()
+        L = C.c_int64()
+        check(N.lib().bs_kernel_timing(self._h, KERNELS[kernel], C.byref(ms), C.byref(L)))
+        return ms.value, L.value
+
+
+def gpu_primitives(state: GpuState2D, gamma: float) -> np.ndarray:
+    """gpuPrimitive2D (euler2d-gpu.ts:136-146): FP64 primitives of a state
+    read back, pressure with the FP32 gamma; (n, 4)."""
+    g32 = float(np.float32(gamma))
+    rho = state.rho.astype(np.float64)
+    mu = state.mu.astype(np.float64)
+    mv = state.mv.astype(np.float64)
+    e = state.energy.astype(np.float64)
+    p = (g32 - 1) * (e - 0.5 * (mu * mu + mv * mv) / rho)
+    return np.stack([rho, mu / rho, mv / rho, p], axis=1)

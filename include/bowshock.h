@@ -1,0 +1,197 @@
+/*
+ * bowshock-mi355x: C ABI of the MI355X (gfx950) axisymmetric Euler engine.
+ *
+ * One solver holds one structured grid block on one GPU, its state in FP32
+ * structure-of-arrays form in HBM, and runs bowshock's explicit SSP-RK and
+ * implicit red-black LU-SGS steps with hand-written HIP kernels. Residual
+ * norms and wall fluxes / drag are evaluated on the GPU in FP64.
+ *
+ * Every entry point replaces a method of the reference's GPU engine
+ * (src/engine-gpu/euler2d-gpu.ts, class GpuEuler2D) or of the CPU engine
+ * it shares its method set with (src/engine-cpu/euler2d.ts, class
+ * Euler2D); the citation sits beside each declaration. Plain pointers and
+ * sizes only: callers own every host buffer they pass, the solver owns its
+ * device memory, and no pointer escapes.
+ *
+ * Errors: each call returns a bs_status; bs_last_error() gives the message
+ * of the calling thread's last failure. A solver is not reentrant: a call
+ * made while another call on the same solver is running returns BS_ERR_BUSY
+ * (the reference's ExclusiveRunner, src/engine-gpu/gpu-common.ts:122-140).
+ *
+ * Conventions (src/grid/structured.ts:1-8): node (i, j) at i + j (ni + 1);
+ * cell (i, j) at i + j ni; sides in the order iMin, iMax, jMin, jMax.
+ */
+#ifndef BOWSHOCK_H
+#define BOWSHOCK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes; the host facades map each to the reference's error class
+ * (src/solver/errors.ts:2-7, src/engine-gpu/gpu-common.ts:7-17). */
+typedef enum {
+  BS_OK = 0,
+  BS_ERR_RANGE = 1,        /* RangeError: invalid configuration or argument */
+  BS_ERR_NONPHYSICAL = 2,  /* NonPhysicalStateError */
+  BS_ERR_MEMORY = 3,       /* GpuMemoryLimitError */
+  BS_ERR_EXECUTION = 4,    /* GpuExecutionError: the device lost or rejected work */
+  BS_ERR_PRECISION = 5,    /* GpuPrecisionError */
+  BS_ERR_BUSY = 6,         /* "await the previous operation first" */
+  BS_ERR_UNSUPPORTED = 7   /* a method of the reference this engine does not provide */
+} bs_status;
+
+/* Boundary kinds (src/engine-cpu/euler2d.ts:34-56). */
+enum { BS_BC_INFLOW = 0, BS_BC_OUTFLOW = 1, BS_BC_WALL = 2, BS_BC_PERIODIC = 3,
+       BS_BC_FACE_STATES = 4, BS_BC_FAR_FIELD = 5 };
+enum { BS_SIDE_IMIN = 0, BS_SIDE_IMAX = 1, BS_SIDE_JMIN = 2, BS_SIDE_JMAX = 3 };
+/* src/scheme/limiters.ts:87-94 */
+enum { BS_LIMITER_FIRST_ORDER = 0, BS_LIMITER_MINMOD = 1, BS_LIMITER_VAN_LEER = 2,
+       BS_LIMITER_VAN_ALBADA = 3, BS_LIMITER_MONOTONIZED_CENTRAL = 4,
+       BS_LIMITER_UNLIMITED_CENTRAL = 5 };
+/* src/scheme/reconstruction.ts:143 */
+enum { BS_RECON_PER_VARIABLE = 0, BS_RECON_SHARED = 1, BS_RECON_SHARED_SCALED = 2 };
+/* The GPU fluxes, src/engine-gpu/euler2d-gpu.ts:57-60 */
+enum { BS_FLUX_HLLC = 0, BS_FLUX_RUSANOV = 1 };
+/* src/solver/runge-kutta.ts:17-28 */
+enum { BS_RK_FORWARD_EULER = 0, BS_RK_SSPRK2 = 1, BS_RK_SSPRK3 = 2 };
+enum { BS_GEOMETRY_PLANAR = 0, BS_GEOMETRY_AXISYMMETRIC = 1 };
+enum { BS_STEPPING_GLOBAL = 0, BS_STEPPING_LOCAL = 1 };
+/* Status word bits read back after every batch of steps
+ * (src/engine-gpu/time-loop.ts:15 STATUS_NON_PHYSICAL). */
+enum { BS_STATUS_NON_PHYSICAL = 1u };
+
+typedef struct bs_ctx bs_ctx;
+typedef struct bs_solver bs_solver;
+
+/* A boundary condition: kind, and the primitive state (rho, u, v, p)
+ * outside an inflow side or the free stream of a far field; for a
+ * faceStates side the state given to every face until bs_set_face_states. */
+typedef struct {
+  int32_t kind;
+  double rho, u, v, p;
+} bs_boundary;
+
+/* Euler2DConfig / GpuEuler2DConfig (src/engine-cpu/euler2d.ts:58-81,
+ * src/engine-gpu/euler2d-gpu.ts:82-92), as plain data. */
+typedef struct {
+  int32_t ni, nj;
+  const double *x, *y;         /* (ni + 1) (nj + 1) node coordinates */
+  int32_t geometry;            /* BS_GEOMETRY_* */
+  double gamma;
+  int32_t flux, limiter, reconstruction, integrator;
+  double cfl;
+  int32_t time_stepping;       /* BS_STEPPING_* */
+  bs_boundary boundaries[4];   /* iMin, iMax, jMin, jMax */
+} bs_config;
+
+/* What a batch of steps leaves (src/engine-gpu/time-loop.ts:79-92
+ * ControlReading): steps taken since the last reset, status bits, the
+ * simulated time (global stepping, FP64 sum of the steps) and the last
+ * step's dt. */
+typedef struct {
+  int64_t steps;
+  uint32_t status;
+  double time;
+  double dt;
+} bs_report;
+
+/* Thread-local message of the last failure ("" if none). */
+const char *bs_last_error(void);
+
+/* Library version string. */
+const char *bs_version(void);
+
+/* A context on one HIP device (the reference's GPUDevice argument,
+ * euler2d-gpu.ts:217). */
+bs_status bs_ctx_create(int device, bs_ctx **out);
+bs_status bs_ctx_destroy(bs_ctx *ctx);
+
+/* GpuEuler2D.create (euler2d-gpu.ts:217-291): validates the configuration
+ * (src/solver/config.ts:35-55), computes the metrics in FP64 on the host
+ * (src/grid/structured.ts:94-168) and uploads them in FP32. */
+bs_status bs_solver_create(bs_ctx *ctx, const bs_config *config, bs_solver **out);
+/* GpuEuler2D.destroy (euler2d-gpu.ts:659); idempotent, NULL is accepted. */
+bs_status bs_solver_destroy(bs_solver *s);
+
+/* Grid metrics in FP64 (GridMetrics, src/grid/structured.ts:28-50). Any
+ * pointer may be NULL. i-faces (ni + 1) nj, j-faces ni (nj + 1), cells
+ * ni nj. Stateless: needs no GPU. */
+bs_status bs_compute_metrics(int32_t ni, int32_t nj, const double *x, const double *y,
+                             int32_t geometry, double *i_nx, double *i_ny, double *i_area,
+                             double *j_nx, double *j_ny, double *j_area, double *volume,
+                             double *planar_area, double *centroid_x, double *centroid_y);
+
+/* setInitialCondition / setCellPrimitives (euler2d-gpu.ts:412-459): one
+ * primitive state (rho, u, v, p) per cell, interleaved, 4 ni nj doubles;
+ * converted to conserved with the FP32 gamma; resets the clock. Returns
+ * BS_ERR_NONPHYSICAL before anything reaches the GPU if a state is not
+ * physical. */
+bs_status bs_set_primitives(bs_solver *s, const double *prim);
+/* Euler2D.setConserved without the step count: conserved variables as FP32
+ * structure of arrays (rho[n], mu[n], mv[n], energy[n]); resets the clock. */
+bs_status bs_set_state(bs_solver *s, const float *rho, const float *mu, const float *mv,
+                       const float *energy);
+/* GpuEuler2D.readState (euler2d-gpu.ts:549-572, GpuState2D 116-125):
+ * conserved FP32 arrays indexed i + j ni. Any pointer may be NULL. Returns
+ * BS_ERR_NONPHYSICAL if the status word has the non-physical bit. */
+bs_status bs_read_state(bs_solver *s, float *rho, float *mu, float *mv, float *energy,
+                        bs_report *report);
+
+/* setFaceStates (euler2d-gpu.ts:575-600): `count` primitive states, one per
+ * face along a faceStates side. */
+bs_status bs_set_face_states(bs_solver *s, int32_t side, const double *prim, int32_t count);
+/* setFaceFluxes (euler2d-gpu.ts:606-630): 4 values per face per unit area
+ * along the face normal, or NULL to go back to the Riemann flux. */
+bs_status bs_set_face_fluxes(bs_solver *s, int32_t side, const double *flux, int32_t count);
+
+/* iterate (euler2d-gpu.ts:462-473): `count` explicit steps, local or global
+ * time steps, in batches between status checks. Returns BS_ERR_NONPHYSICAL
+ * when a batch ends with the status bit set, BS_ERR_EXECUTION when the
+ * device ran fewer steps than submitted (time-loop.ts:125-150). */
+bs_status bs_iterate(bs_solver *s, int32_t count, bs_report *report);
+/* advanceTo (euler2d-gpu.ts:511-527, time-loop.ts:158-177): global steps
+ * to exactly t_end, at most max_steps. */
+bs_status bs_advance_to(bs_solver *s, double t_end, int32_t max_steps, bs_report *report);
+
+/* densityResidualNorm (euler2d-gpu.ts:530-546; euler2d.ts:480-491): RMS of
+ * the density residual over volume, reduced in FP64 on the GPU. */
+bs_status bs_residual_norm(bs_solver *s, double *norm);
+/* residualVector (euler2d.ts:462-477) of the current state, FP32 residual
+ * returned as FP64, 4 per cell interleaved. */
+bs_status bs_residual_vector(bs_solver *s, double *residual);
+
+/* boundaryFluxes (euler2d.ts:499-525) evaluated on the GPU in FP64 from the
+ * FP32 state (pressure with the FP32 gamma, gpuPrimitive2D,
+ * euler2d-gpu.ts:136-146): 9 doubles per face, x, y, nx, ny (outward),
+ * area, flux[4] (outward). `count` must be the faces along the side. */
+bs_status bs_wall_fluxes(bs_solver *s, int32_t side, double *faces, int32_t count);
+/* forebodyPressureDrag (src/analysis/forces.ts:19-29) over one side's faces,
+ * reduced in FP64 on the GPU. */
+bs_status bs_forebody_drag(bs_solver *s, int32_t side, double p_inf, double q_inf,
+                           double base_radius, double *cd);
+
+/* setImplicitCfl (euler2d-gpu.ts:668-681). */
+bs_status bs_set_implicit_cfl(bs_solver *s, double cfl);
+/* iterateImplicit (euler2d-gpu.ts:687-723; euler2d.ts:581-589 with the
+ * red-black relaxation): `count` steps, each with `sweeps` red-black
+ * sweeps; the CFL number starts from the value set, grows by `growth` per
+ * step up to cfl_max. */
+bs_status bs_iterate_implicit(bs_solver *s, int32_t count, double cfl_max, int32_t sweeps,
+                              double omega, double growth, bs_report *report);
+
+/* Mean device time per launch of the solver's kernels since the last call,
+ * for throughput reporting: kernel-name index (0 stage, 1 implicit prepare,
+ * 2 implicit relax, 3 implicit update), milliseconds per launch and the
+ * launches counted. Timing is taken with HIP events on the solver's stream
+ * when enabled with bs_enable_timing. */
+bs_status bs_enable_timing(bs_solver *s, int32_t enable);
+bs_status bs_kernel_timing(bs_solver *s, int32_t kernel, double *ms_per_launch, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BOWSHOCK_H */

@@ -1,0 +1,41 @@
+"""Implicit red-black LU-SGS on the GPU against the FP64 oracle's
+red-black relaxation (euler2d.ts:878-1105) step by step, and the
+converged drag against the explicit scheme's."""
+import numpy as np
+import pytest
+
+import cases
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("sweeps", [1, 2])
+def test_red_black_steps_match_oracle(ctx, sweeps):
+    grid, free, p_inf, q_inf, bcs = cases.cone_case(48, 24, mach=2.0)
+    orc, gpu = cases.make_pair(ctx, grid, bcs, limiter="vanAlbada", reconstruction="sharedScaled")
+    orc.set_primitives(np.array(free))
+    gpu.setCellPrimitives(np.array(free))
+    cfl = 1.0
+    for _ in range(8):
+        orc.iterate_implicit(1, cfl, 1.5, "redBlack", sweeps)
+        cfl = min(10.0, cfl * 1.2)
+    gpu.setImplicitCfl(1.0)
+    gpu.iterateImplicit(8, cfl=10.0, sweeps=sweeps)
+    l1, linf = cases.field_errors(cases.gpu_conserved(gpu), orc.conserved())
+    print("L1", l1, "Linf", linf)
+    assert l1.max() < 5e-6
+    assert linf.max() < 2e-4
+
+
+def test_implicit_converges_to_explicit_drag(ctx):
+    grid, free, p_inf, q_inf, bcs = cases.cone_case(48, 24, mach=2.0)
+    orc, gpu = cases.make_pair(ctx, grid, bcs, limiter="vanAlbada", reconstruction="sharedScaled")
+    orc.set_primitives(np.array(free))
+    orc.iterate(4000)
+    cd_explicit = orc.forebody_drag(p_inf, q_inf, 0.5)
+    gpu.setCellPrimitives(np.array(free))
+    gpu.setImplicitCfl(1.0)
+    gpu.iterateImplicit(1500, cfl=10.0, sweeps=2)
+    cd = gpu.forebodyDrag(p_inf, q_inf, 0.5)
+    print("explicit", cd_explicit, "gpu implicit", cd, cd / cd_explicit - 1)
+    assert abs(cd / cd_explicit - 1) < 1e-5
