@@ -47,7 +47,16 @@ bs_status hip_fail(hipError_t e, const char* what) {
   } while (0)
 
 constexpr int BATCH = 256;  // steps between status checks (time-loop.ts:13)
-constexpr int TX = 64, TY = 4;
+// Tile of the stencil kernels: TX cells along i (one wavefront row when 64)
+// by TY along j; set at build time.
+#ifndef BS_TILE_X
+#define BS_TILE_X 64
+#endif
+#ifndef BS_TILE_Y
+#define BS_TILE_Y 8
+#endif
+constexpr int TX = BS_TILE_X, TY = BS_TILE_Y;
+static_assert(TX * TY <= 1024 && TX * TY >= TX + TY && (TX * TY) % 64 == 0, "tile shape");
 
 }  // namespace
 
@@ -63,6 +72,7 @@ struct DeviceBuffers {
   double* partials = nullptr;
   double* scalar = nullptr;
   Control* control = nullptr;
+  BoundaryDev* bnd = nullptr;  // the boundary table the stencil kernels read
   float* face_cons[4] = {nullptr, nullptr, nullptr, nullptr};
   float* face_flux[4] = {nullptr, nullptr, nullptr, nullptr};
   double* face_geo[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -147,6 +157,16 @@ BoundaryDev boundary_of(bs_solver* s) {
     B.face_flux[k] = s->imposed[k] ? s->d.face_flux[k] : nullptr;
   }
   return B;
+}
+
+// The boundary table lives in device memory so kernels index it by a
+// runtime side; re-uploaded whenever an imposed flux is switched. The copy
+// is ordered on the solver's stream before any later launch.
+bs_status upload_boundary(bs_solver* s) {
+  BoundaryDev B = boundary_of(s);
+  HIP_TRY(hipMemcpyAsync(s->d.bnd, &B, sizeof B, hipMemcpyHostToDevice, s->stream), "boundary table");
+  HIP_TRY(hipStreamSynchronize(s->stream), "boundary table");
+  return BS_OK;
 }
 
 // ---- kernel dispatch tables -------------------------------------------------
@@ -234,7 +254,7 @@ TileArgs tile_args(bs_solver* s) {
   TileArgs A;
   memset(&A, 0, sizeof A);
   A.G = grid_of(s);
-  A.B = boundary_of(s);
+  A.B = s->d.bnd;
   A.gamma = s->gamma32;
   A.axisymmetric = s->cfg.geometry == BS_GEOMETRY_AXISYMMETRIC;
   A.limiter = s->cfg.limiter;
@@ -358,8 +378,8 @@ bs_status dev_alloc(T** p, size_t count, const char* what) {
 void free_all(bs_solver* s) {
   DeviceBuffers& d = s->d;
   void* all[] = {d.state[0], d.state[1], d.state[2], d.inx, d.iny, d.ia, d.jnx, d.jny, d.ja, d.vol, d.planar,
-                 d.dt, d.dt_global, d.dt_partials, d.partials, d.scalar, d.control, d.wall_out, d.res, d.diag,
-                 d.lam_i, d.lam_j, d.dq};
+                 d.dt, d.dt_global, d.dt_partials, d.partials, d.scalar, d.control, d.bnd, d.wall_out, d.res,
+                 d.diag, d.lam_i, d.lam_j, d.dq};
   for (void* p : all)
     if (p) (void)hipFree(p);
   for (int k = 0; k < 4; k++) {
@@ -543,6 +563,7 @@ bs_status bs_solver_create(bs_ctx* ctx, const bs_config* c, bs_solver** out) {
       (st = dev_alloc(&s->d.dt_partials, (size_t)nb256, "dt partials")) ||
       (st = dev_alloc(&s->d.partials, (size_t)(blocks > nb256 ? blocks : nb256) + 64, "partials")) ||
       (st = dev_alloc(&s->d.scalar, 4, "scalars")) || (st = dev_alloc(&s->d.control, 1, "control")) ||
+      (st = dev_alloc(&s->d.bnd, 1, "boundary table")) ||
       (st = dev_alloc(&s->d.wall_out, 9 * (size_t)(ni > nj ? ni : nj), "wall fluxes")))
     return bail(st);
   HIP_TRY(hipMemset(s->d.control, 0, sizeof(Control)), "control reset");
@@ -593,6 +614,7 @@ bs_status bs_solver_create(bs_ctx* ctx, const bs_config* c, bs_solver** out) {
   init.cfl = 1.0;
   init.t_end = 3e38;
   HIP_TRY(hipMemcpy(s->d.control, &init, sizeof init, hipMemcpyHostToDevice), "control init");
+  if ((st = upload_boundary(s))) return bail(st);
   *out = s;
   return BS_OK;
 }
@@ -669,13 +691,13 @@ bs_status bs_set_face_fluxes(bs_solver* s, int32_t side, const double* flux, int
     return fail(BS_ERR_RANGE, "the boundary does not take face fluxes");
   if (!flux) {
     s->imposed[side] = false;
-    return BS_OK;
+    return upload_boundary(s);
   }
   if (count != 4 * along(s, side)) return fail(BS_ERR_RANGE, "the boundary needs %d flux values", 4 * along(s, side));
   std::vector<float> f(flux, flux + count);
   HIP_TRY(hipMemcpy(s->d.face_flux[side], f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice), "face fluxes");
   s->imposed[side] = true;
-  return BS_OK;
+  return upload_boundary(s);
 }
 
 bs_status bs_iterate(bs_solver* s, int32_t count, bs_report* report) {

@@ -121,7 +121,7 @@ enum TileMode : int { TILE_STAGE = 0, TILE_IMPLICIT = 1, TILE_RESIDUAL = 2 };
 
 struct TileArgs {
   GridDev G;
-  BoundaryDev B;
+  const BoundaryDev* B;       // device-resident: indexed by a runtime side without spilling the arguments
   StateDev u0, src, dst;
   float gamma;
   int axisymmetric;
@@ -177,20 +177,45 @@ __global__ __launch_bounds__(TX * TY) void tile_kernel(TileArgs A) {
   const int tid = threadIdx.x;
 
   // Phase 1: primitives of the window [i0-2, i0+TX+2) x [j0-2, j0+TY+2),
-  // corners excluded.
-  for (int k = tid; k < WX * WY; k += NT) {
-    const int li = k % WX, lj = k / WX;
-    const bool ci = li < 2 || li >= TX + 2, cj = lj < 2 || lj >= TY + 2;
-    if (ci && cj) continue;
-    const int gi = i0 + li - 2, gj = j0 + lj - 2;
-    const bool oi = gi < 0 || gi >= ni, oj = gj < 0 || gj >= nj;
-    if ((oi && oj) || gi > ni + 1 || gj > nj + 1) continue;
-    Vec4<float> q = load_cell(A.src, G, A.B, gi, gj, g);
+  // corners excluded. Each thread first loads its own cell, whose conserved
+  // state it keeps in registers for the update; then the 4 (TX + TY) halo
+  // cells: two rows below and above the tile, two columns left and right.
+  const int li = tid % TX, lj = tid / TX;
+  const int gi = i0 + li, gj = j0 + lj;
+  const bool valid = gi < ni && gj < nj;
+  Vec4<float> own = {0.f, 0.f, 0.f, 0.f};
+  // On a tile at the high edge of the grid the thread's position may be a
+  // ghost cell (gi in [ni, ni + 1] or gj in [nj, nj + 1]) that the faces
+  // near the edge still need.
+  if (!(gi >= ni && gj >= nj) && gi <= ni + 1 && gj <= nj + 1) {
+    const Vec4<float> q = load_cell(A.src, G, *A.B, gi, gj, g);
+    if (valid) own = q;
     Prim<float> w = to_prim(q.x, q.y, q.z, q.w, g);
-    sp[0][lj][li] = w.rho;
-    sp[1][lj][li] = w.u;
-    sp[2][lj][li] = w.v;
-    sp[3][lj][li] = w.p;
+    sp[0][lj + 2][li + 2] = w.rho;
+    sp[1][lj + 2][li + 2] = w.u;
+    sp[2][lj + 2][li + 2] = w.v;
+    sp[3][lj + 2][li + 2] = w.p;
+  }
+  for (int h = tid; h < 4 * (TX + TY); h += NT) {
+    int wi, wj;
+    if (h < 4 * TX) {
+      const int r = h / TX;
+      wi = 2 + h % TX;
+      wj = r < 2 ? r : TY + r;
+    } else {
+      const int r = (h - 4 * TX) / TY;
+      wj = 2 + (h - 4 * TX) % TY;
+      wi = r < 2 ? r : TX + r;
+    }
+    const int hi = i0 + wi - 2, hj = j0 + wj - 2;
+    const bool oi = hi < 0 || hi >= ni, oj = hj < 0 || hj >= nj;
+    if ((oi && oj) || hi > ni + 1 || hj > nj + 1) continue;
+    Vec4<float> q = load_cell(A.src, G, *A.B, hi, hj, g);
+    Prim<float> w = to_prim(q.x, q.y, q.z, q.w, g);
+    sp[0][wj][wi] = w.rho;
+    sp[1][wj][wi] = w.u;
+    sp[2][wj][wi] = w.v;
+    sp[3][wj][wi] = w.p;
   }
   __syncthreads();
 
@@ -210,11 +235,11 @@ __global__ __launch_bounds__(TX * TY) void tile_kernel(TileArgs A) {
       nx = G.inx[fidx]; ny = G.iny[fidx]; area = G.ia[fidx];
       const int side = gi == 0 ? IMIN : gi == ni ? IMAX : -1;
       const Prim<float> l0 = prim_at(fy + 2, fx + 1), r0 = prim_at(fy + 2, fx + 2);
-      if (side >= 0 && A.B.face_flux[side] != nullptr) {
-        const float* b = A.B.face_flux[side] + 4 * gj;
+      if (side >= 0 && A.B->face_flux[side] != nullptr) {
+        const float* b = A.B->face_flux[side] + 4 * gj;
         f = {b[0], b[1], b[2], b[3]};
       } else {
-        const int wall = side >= 0 && A.B.kind[side] == WALL ? (side == IMIN ? -1 : 1) : 0;
+        const int wall = side >= 0 && A.B->kind[side] == WALL ? (side == IMIN ? -1 : 1) : 0;
         f = face_flux<float, REC, FLUX>(LIM, prim_at(fy + 2, fx), l0, r0, prim_at(fy + 2, fx + 3), nx, ny,
                                         wall, g);
       }
@@ -240,11 +265,11 @@ __global__ __launch_bounds__(TX * TY) void tile_kernel(TileArgs A) {
       nx = G.jnx[fidx]; ny = G.jny[fidx]; area = G.ja[fidx];
       const int side = gj == 0 ? JMIN : gj == nj ? JMAX : -1;
       const Prim<float> l0 = prim_at(fy + 1, fx + 2), r0 = prim_at(fy + 2, fx + 2);
-      if (side >= 0 && A.B.face_flux[side] != nullptr) {
-        const float* b = A.B.face_flux[side] + 4 * gi;
+      if (side >= 0 && A.B->face_flux[side] != nullptr) {
+        const float* b = A.B->face_flux[side] + 4 * gi;
         f = {b[0], b[1], b[2], b[3]};
       } else {
-        const int wall = side >= 0 && A.B.kind[side] == WALL ? (side == JMIN ? -1 : 1) : 0;
+        const int wall = side >= 0 && A.B->kind[side] == WALL ? (side == JMIN ? -1 : 1) : 0;
         f = face_flux<float, REC, FLUX>(LIM, prim_at(fy, fx + 2), l0, r0, prim_at(fy + 3, fx + 2), nx, ny,
                                         wall, g);
       }
@@ -267,7 +292,6 @@ __global__ __launch_bounds__(TX * TY) void tile_kernel(TileArgs A) {
     }
   };
 
-  const int li = tid % TX, lj = tid / TX;
   do_face(0, li, lj);
   do_face(1, li, lj);
   if (tid < TY) do_face(0, TX, tid);
@@ -275,8 +299,6 @@ __global__ __launch_bounds__(TX * TY) void tile_kernel(TileArgs A) {
   __syncthreads();
 
   // Phase 3: cells.
-  const int gi = i0 + li, gj = j0 + lj;
-  const bool valid = gi < ni && gj < nj;
   double r2 = 0.0;
   if (valid) {
     const int c = gi + gj * ni;
@@ -312,14 +334,21 @@ __global__ __launch_bounds__(TX * TY) void tile_kernel(TileArgs A) {
       }
       const float k = dt / vol;
       Vec4<float> out;
-      float srcq[4] = {A.src.q[0][c], A.src.q[1][c], A.src.q[2][c], A.src.q[3][c]};
+      const float srcq[4] = {own.x, own.y, own.z, own.w};
       float o[4];
+      // a u0 + b (src + k R) with a = 1 - b (every Shu-Osher stage here),
+      // written as u0 + b ((src - u0) + k R): the FP32 coefficients 1/3 and
+      // 2/3 do not sum to 1, and the plain form would add (a + b - 1) u0 per
+      // step, a source that shifts the steady state by up to 1e-5 in CD.
       if (A.a != 0.f) {
 #pragma unroll
-        for (int q = 0; q < 4; q++) o[q] = A.a * A.u0.q[q][c] + A.b * (srcq[q] + k * r[q]);
+        for (int q = 0; q < 4; q++) {
+          const float u0q = A.u0.q[q][c];
+          o[q] = u0q + A.b * ((srcq[q] - u0q) + k * r[q]);
+        }
       } else {
 #pragma unroll
-        for (int q = 0; q < 4; q++) o[q] = A.b * (srcq[q] + k * r[q]);
+        for (int q = 0; q < 4; q++) o[q] = srcq[q] + A.b * (k * r[q]);
       }
       if (A.dt_mode == DT_GLOBAL && dt <= 0.f) {
         // A zero time step (global stepping already at t_end) leaves the

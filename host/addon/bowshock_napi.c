@@ -1,0 +1,472 @@
+/*
+ * N-API addon: the thin JavaScript binding of include/bowshock.h.
+ *
+ * Every function takes and returns plain JS values and typed arrays and
+ * calls one C-ABI entry point; the long-running ones (iterate, advanceTo,
+ * iterateImplicit) run on a worker thread through napi_async_work and
+ * return a Promise, so Node's event loop is not blocked. A failure rejects
+ * or throws an Error whose `code` is the bs_status name; host/index.js maps
+ * it to the reference's error classes.
+ */
+#define NAPI_VERSION 8
+#include <node_api.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "bowshock.h"
+
+#define CHECK_NAPI(call)                                   \
+  do {                                                     \
+    if ((call) != napi_ok) {                               \
+      napi_throw_error(env, NULL, "N-API call failed");    \
+      return NULL;                                         \
+    }                                                      \
+  } while (0)
+
+static const char *status_name(bs_status st) {
+  switch (st) {
+    case BS_OK: return "BS_OK";
+    case BS_ERR_RANGE: return "BS_ERR_RANGE";
+    case BS_ERR_NONPHYSICAL: return "BS_ERR_NONPHYSICAL";
+    case BS_ERR_MEMORY: return "BS_ERR_MEMORY";
+    case BS_ERR_EXECUTION: return "BS_ERR_EXECUTION";
+    case BS_ERR_PRECISION: return "BS_ERR_PRECISION";
+    case BS_ERR_BUSY: return "BS_ERR_BUSY";
+    default: return "BS_ERR_UNSUPPORTED";
+  }
+}
+
+static napi_value make_error(napi_env env, bs_status st, const char *msg) {
+  napi_value code, text, err;
+  napi_create_string_utf8(env, status_name(st), NAPI_AUTO_LENGTH, &code);
+  napi_create_string_utf8(env, msg, NAPI_AUTO_LENGTH, &text);
+  napi_create_error(env, code, text, &err);
+  return err;
+}
+
+static napi_value throw_status(napi_env env, bs_status st) {
+  napi_throw(env, make_error(env, st, bs_last_error()));
+  return NULL;
+}
+
+static void *get_ptr(napi_env env, napi_value v) {
+  void *p = NULL;
+  napi_get_value_external(env, v, &p);
+  return p;
+}
+
+static int get_args(napi_env env, napi_callback_info info, size_t want, napi_value *argv) {
+  size_t argc = want;
+  if (napi_get_cb_info(env, info, &argc, argv, NULL, NULL) != napi_ok || argc < want) {
+    napi_throw_type_error(env, NULL, "missing arguments");
+    return 0;
+  }
+  return 1;
+}
+
+static double num(napi_env env, napi_value v) {
+  double d = 0;
+  napi_get_value_double(env, v, &d);
+  return d;
+}
+
+static int32_t i32(napi_env env, napi_value v) {
+  int32_t d = 0;
+  napi_get_value_int32(env, v, &d);
+  return d;
+}
+
+/* Data pointer and element count of a typed array of the given type. */
+static void *typed(napi_env env, napi_value v, napi_typedarray_type want, size_t *count) {
+  bool is = false;
+  napi_is_typedarray(env, v, &is);
+  if (!is) return NULL;
+  napi_typedarray_type type;
+  size_t length = 0, offset = 0;
+  void *data = NULL;
+  napi_value buffer;
+  napi_get_typedarray_info(env, v, &type, &length, &data, &buffer, &offset);
+  if (type != want) return NULL;
+  if (count) *count = length;
+  return data;
+}
+
+static napi_value report_object(napi_env env, const bs_report *r) {
+  napi_value o, v;
+  napi_create_object(env, &o);
+  napi_create_double(env, (double)r->steps, &v);
+  napi_set_named_property(env, o, "steps", v);
+  napi_create_uint32(env, r->status, &v);
+  napi_set_named_property(env, o, "status", v);
+  napi_create_double(env, r->time, &v);
+  napi_set_named_property(env, o, "time", v);
+  napi_create_double(env, r->dt, &v);
+  napi_set_named_property(env, o, "dt", v);
+  return o;
+}
+
+static napi_value js_version(napi_env env, napi_callback_info info) {
+  (void)info;
+  napi_value s;
+  napi_create_string_utf8(env, bs_version(), NAPI_AUTO_LENGTH, &s);
+  return s;
+}
+
+static napi_value js_ctx_create(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  bs_ctx *ctx = NULL;
+  bs_status st = bs_ctx_create(i32(env, argv[0]), &ctx);
+  if (st) return throw_status(env, st);
+  napi_value ext;
+  CHECK_NAPI(napi_create_external(env, ctx, NULL, NULL, &ext));
+  return ext;
+}
+
+static napi_value js_ctx_destroy(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  bs_ctx_destroy((bs_ctx *)get_ptr(env, argv[0]));
+  return NULL;
+}
+
+/* solverCreate(ctx, cfg): cfg = {ni, nj, x: Float64Array, y: Float64Array,
+ * geometry, gamma, flux, limiter, reconstruction, integrator, cfl,
+ * timeStepping, boundaries: [{kind, rho, u, v, p} x 4]} with integer codes. */
+static napi_value js_solver_create(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  bs_ctx *ctx = (bs_ctx *)get_ptr(env, argv[0]);
+  napi_value c = argv[1], v;
+  bs_config cfg;
+  memset(&cfg, 0, sizeof cfg);
+#define FIELD(name) (napi_get_named_property(env, c, name, &v), v)
+  cfg.ni = i32(env, FIELD("ni"));
+  cfg.nj = i32(env, FIELD("nj"));
+  size_t nx = 0, ny = 0;
+  cfg.x = (const double *)typed(env, FIELD("x"), napi_float64_array, &nx);
+  cfg.y = (const double *)typed(env, FIELD("y"), napi_float64_array, &ny);
+  size_t nodes = (size_t)(cfg.ni + 1) * (size_t)(cfg.nj + 1);
+  if (!cfg.x || !cfg.y || nx != nodes || ny != nodes) {
+    napi_throw(env, make_error(env, BS_ERR_RANGE, "x and y must be Float64Arrays of (ni + 1) (nj + 1) nodes"));
+    return NULL;
+  }
+  cfg.geometry = i32(env, FIELD("geometry"));
+  cfg.gamma = num(env, FIELD("gamma"));
+  cfg.flux = i32(env, FIELD("flux"));
+  cfg.limiter = i32(env, FIELD("limiter"));
+  cfg.reconstruction = i32(env, FIELD("reconstruction"));
+  cfg.integrator = i32(env, FIELD("integrator"));
+  cfg.cfl = num(env, FIELD("cfl"));
+  cfg.time_stepping = i32(env, FIELD("timeStepping"));
+  napi_value bcs = FIELD("boundaries");
+  for (uint32_t k = 0; k < 4; k++) {
+    napi_value b;
+    napi_get_element(env, bcs, k, &b);
+    napi_value f;
+    napi_get_named_property(env, b, "kind", &f);
+    cfg.boundaries[k].kind = i32(env, f);
+    napi_get_named_property(env, b, "rho", &f);
+    cfg.boundaries[k].rho = num(env, f);
+    napi_get_named_property(env, b, "u", &f);
+    cfg.boundaries[k].u = num(env, f);
+    napi_get_named_property(env, b, "v", &f);
+    cfg.boundaries[k].v = num(env, f);
+    napi_get_named_property(env, b, "p", &f);
+    cfg.boundaries[k].p = num(env, f);
+  }
+#undef FIELD
+  bs_solver *s = NULL;
+  bs_status st = bs_solver_create(ctx, &cfg, &s);
+  if (st) return throw_status(env, st);
+  napi_value ext;
+  CHECK_NAPI(napi_create_external(env, s, NULL, NULL, &ext));
+  return ext;
+}
+
+static napi_value js_solver_destroy(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  bs_solver_destroy((bs_solver *)get_ptr(env, argv[0]));
+  return NULL;
+}
+
+static napi_value new_f64(napi_env env, size_t n, double **data) {
+  napi_value buf, arr;
+  napi_create_arraybuffer(env, n * sizeof(double), (void **)data, &buf);
+  napi_create_typedarray(env, napi_float64_array, n, buf, 0, &arr);
+  return arr;
+}
+
+static napi_value new_f32(napi_env env, size_t n, float **data) {
+  napi_value buf, arr;
+  napi_create_arraybuffer(env, n * sizeof(float), (void **)data, &buf);
+  napi_create_typedarray(env, napi_float32_array, n, buf, 0, &arr);
+  return arr;
+}
+
+static napi_value js_compute_metrics(napi_env env, napi_callback_info info) {
+  napi_value argv[5];
+  if (!get_args(env, info, 5, argv)) return NULL;
+  int32_t ni = i32(env, argv[0]), nj = i32(env, argv[1]);
+  size_t nx = 0, ny = 0;
+  const double *x = (const double *)typed(env, argv[2], napi_float64_array, &nx);
+  const double *y = (const double *)typed(env, argv[3], napi_float64_array, &ny);
+  if (ni < 1 || nj < 1 || !x || !y || nx != (size_t)(ni + 1) * (nj + 1) || ny != nx) {
+    napi_throw(env, make_error(env, BS_ERR_RANGE, "bad grid"));
+    return NULL;
+  }
+  size_t fi = (size_t)(ni + 1) * nj, fj = (size_t)ni * (nj + 1), n = (size_t)ni * nj;
+  const char *names[10] = {"iNx", "iNy", "iArea", "jNx", "jNy", "jArea", "volume", "planarArea", "centroidX",
+                           "centroidY"};
+  size_t sizes[10] = {fi, fi, fi, fj, fj, fj, n, n, n, n};
+  double *ptr[10];
+  napi_value out, arr[10];
+  napi_create_object(env, &out);
+  for (int k = 0; k < 10; k++) {
+    arr[k] = new_f64(env, sizes[k], &ptr[k]);
+    napi_set_named_property(env, out, names[k], arr[k]);
+  }
+  bs_status st = bs_compute_metrics(ni, nj, x, y, i32(env, argv[4]), ptr[0], ptr[1], ptr[2], ptr[3], ptr[4],
+                                    ptr[5], ptr[6], ptr[7], ptr[8], ptr[9]);
+  if (st) return throw_status(env, st);
+  return out;
+}
+
+static napi_value js_set_primitives(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  const double *p = (const double *)typed(env, argv[1], napi_float64_array, NULL);
+  if (!p) return throw_status(env, BS_ERR_RANGE);
+  bs_status st = bs_set_primitives((bs_solver *)get_ptr(env, argv[0]), p);
+  if (st) return throw_status(env, st);
+  return NULL;
+}
+
+static napi_value js_set_state(napi_env env, napi_callback_info info) {
+  napi_value argv[5];
+  if (!get_args(env, info, 5, argv)) return NULL;
+  const float *q[4];
+  for (int k = 0; k < 4; k++) {
+    q[k] = (const float *)typed(env, argv[1 + k], napi_float32_array, NULL);
+    if (!q[k]) return throw_status(env, BS_ERR_RANGE);
+  }
+  bs_status st = bs_set_state((bs_solver *)get_ptr(env, argv[0]), q[0], q[1], q[2], q[3]);
+  if (st) return throw_status(env, st);
+  return NULL;
+}
+
+static napi_value js_read_state(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  size_t n = (size_t)num(env, argv[1]);
+  float *q[4];
+  napi_value out, arr[4];
+  napi_create_object(env, &out);
+  const char *names[4] = {"rho", "mu", "mv", "energy"};
+  for (int k = 0; k < 4; k++) {
+    arr[k] = new_f32(env, n, &q[k]);
+    napi_set_named_property(env, out, names[k], arr[k]);
+  }
+  bs_report r;
+  memset(&r, 0, sizeof r);
+  bs_status st = bs_read_state((bs_solver *)get_ptr(env, argv[0]), q[0], q[1], q[2], q[3], &r);
+  if (st) return throw_status(env, st);
+  napi_set_named_property(env, out, "report", report_object(env, &r));
+  return out;
+}
+
+static napi_value js_set_face_states(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return NULL;
+  size_t n = 0;
+  const double *p = (const double *)typed(env, argv[2], napi_float64_array, &n);
+  if (!p) return throw_status(env, BS_ERR_RANGE);
+  bs_status st = bs_set_face_states((bs_solver *)get_ptr(env, argv[0]), i32(env, argv[1]), p, (int32_t)(n / 4));
+  if (st) return throw_status(env, st);
+  return NULL;
+}
+
+static napi_value js_set_face_fluxes(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return NULL;
+  napi_valuetype t;
+  napi_typeof(env, argv[2], &t);
+  bs_status st;
+  if (t == napi_null || t == napi_undefined) {
+    st = bs_set_face_fluxes((bs_solver *)get_ptr(env, argv[0]), i32(env, argv[1]), NULL, 0);
+  } else {
+    size_t n = 0;
+    const double *p = (const double *)typed(env, argv[2], napi_float64_array, &n);
+    if (!p) return throw_status(env, BS_ERR_RANGE);
+    st = bs_set_face_fluxes((bs_solver *)get_ptr(env, argv[0]), i32(env, argv[1]), p, (int32_t)n);
+  }
+  if (st) return throw_status(env, st);
+  return NULL;
+}
+
+static napi_value js_residual_norm(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  double norm = 0;
+  bs_status st = bs_residual_norm((bs_solver *)get_ptr(env, argv[0]), &norm);
+  if (st) return throw_status(env, st);
+  napi_value v;
+  napi_create_double(env, norm, &v);
+  return v;
+}
+
+static napi_value js_residual_vector(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  size_t n = (size_t)num(env, argv[1]);
+  double *r;
+  napi_value arr = new_f64(env, 4 * n, &r);
+  bs_status st = bs_residual_vector((bs_solver *)get_ptr(env, argv[0]), r);
+  if (st) return throw_status(env, st);
+  return arr;
+}
+
+static napi_value js_wall_fluxes(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return NULL;
+  int32_t count = i32(env, argv[2]);
+  double *f;
+  napi_value arr = new_f64(env, 9 * (size_t)(count > 0 ? count : 0), &f);
+  bs_status st = bs_wall_fluxes((bs_solver *)get_ptr(env, argv[0]), i32(env, argv[1]), f, count);
+  if (st) return throw_status(env, st);
+  return arr;
+}
+
+static napi_value js_forebody_drag(napi_env env, napi_callback_info info) {
+  napi_value argv[5];
+  if (!get_args(env, info, 5, argv)) return NULL;
+  double cd = 0;
+  bs_status st = bs_forebody_drag((bs_solver *)get_ptr(env, argv[0]), i32(env, argv[1]), num(env, argv[2]),
+                                  num(env, argv[3]), num(env, argv[4]), &cd);
+  if (st) return throw_status(env, st);
+  napi_value v;
+  napi_create_double(env, cd, &v);
+  return v;
+}
+
+static napi_value js_set_implicit_cfl(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  bs_status st = bs_set_implicit_cfl((bs_solver *)get_ptr(env, argv[0]), num(env, argv[1]));
+  if (st) return throw_status(env, st);
+  return NULL;
+}
+
+/* ---- asynchronous steps ------------------------------------------------ */
+
+enum { OP_ITERATE, OP_ADVANCE, OP_IMPLICIT };
+
+typedef struct {
+  napi_async_work work;
+  napi_deferred deferred;
+  bs_solver *s;
+  int op;
+  int32_t count, sweeps, max_steps;
+  double t_end, cfl, omega, growth;
+  bs_status st;
+  char message[512];
+  bs_report report;
+} StepJob;
+
+static void step_execute(napi_env env, void *data) {
+  (void)env;
+  StepJob *j = (StepJob *)data;
+  memset(&j->report, 0, sizeof j->report);
+  if (j->op == OP_ITERATE) j->st = bs_iterate(j->s, j->count, &j->report);
+  else if (j->op == OP_ADVANCE) j->st = bs_advance_to(j->s, j->t_end, j->max_steps, &j->report);
+  else j->st = bs_iterate_implicit(j->s, j->count, j->cfl, j->sweeps, j->omega, j->growth, &j->report);
+  /* The message is thread-local to the worker: copy it here. */
+  strncpy(j->message, bs_last_error(), sizeof j->message - 1);
+  j->message[sizeof j->message - 1] = 0;
+}
+
+static void step_complete(napi_env env, napi_status status, void *data) {
+  StepJob *j = (StepJob *)data;
+  if (status != napi_ok) {
+    napi_reject_deferred(env, j->deferred, make_error(env, BS_ERR_EXECUTION, "the step job was cancelled"));
+  } else if (j->st != BS_OK) {
+    napi_reject_deferred(env, j->deferred, make_error(env, j->st, j->message));
+  } else {
+    napi_resolve_deferred(env, j->deferred, report_object(env, &j->report));
+  }
+  napi_delete_async_work(env, j->work);
+  free(j);
+}
+
+static napi_value queue_job(napi_env env, StepJob *j) {
+  napi_value promise, name;
+  napi_create_promise(env, &j->deferred, &promise);
+  napi_create_string_utf8(env, "bowshock-steps", NAPI_AUTO_LENGTH, &name);
+  napi_create_async_work(env, NULL, name, step_execute, step_complete, j, &j->work);
+  napi_queue_async_work(env, j->work);
+  return promise;
+}
+
+static napi_value js_iterate(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  StepJob *j = (StepJob *)calloc(1, sizeof *j);
+  j->s = (bs_solver *)get_ptr(env, argv[0]);
+  j->op = OP_ITERATE;
+  j->count = i32(env, argv[1]);
+  return queue_job(env, j);
+}
+
+static napi_value js_advance_to(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return NULL;
+  StepJob *j = (StepJob *)calloc(1, sizeof *j);
+  j->s = (bs_solver *)get_ptr(env, argv[0]);
+  j->op = OP_ADVANCE;
+  j->t_end = num(env, argv[1]);
+  j->max_steps = i32(env, argv[2]);
+  return queue_job(env, j);
+}
+
+static napi_value js_iterate_implicit(napi_env env, napi_callback_info info) {
+  napi_value argv[6];
+  if (!get_args(env, info, 6, argv)) return NULL;
+  StepJob *j = (StepJob *)calloc(1, sizeof *j);
+  j->s = (bs_solver *)get_ptr(env, argv[0]);
+  j->op = OP_IMPLICIT;
+  j->count = i32(env, argv[1]);
+  j->cfl = num(env, argv[2]);
+  j->sweeps = i32(env, argv[3]);
+  j->omega = num(env, argv[4]);
+  j->growth = num(env, argv[5]);
+  return queue_job(env, j);
+}
+
+static napi_value init(napi_env env, napi_value exports) {
+  napi_property_descriptor props[] = {
+      {"version", NULL, js_version, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"ctxCreate", NULL, js_ctx_create, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"ctxDestroy", NULL, js_ctx_destroy, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"solverCreate", NULL, js_solver_create, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"solverDestroy", NULL, js_solver_destroy, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"computeMetrics", NULL, js_compute_metrics, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"setPrimitives", NULL, js_set_primitives, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"setState", NULL, js_set_state, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"readState", NULL, js_read_state, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"setFaceStates", NULL, js_set_face_states, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"setFaceFluxes", NULL, js_set_face_fluxes, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"residualNorm", NULL, js_residual_norm, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"residualVector", NULL, js_residual_vector, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"wallFluxes", NULL, js_wall_fluxes, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"forebodyDrag", NULL, js_forebody_drag, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"setImplicitCfl", NULL, js_set_implicit_cfl, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"iterate", NULL, js_iterate, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"advanceTo", NULL, js_advance_to, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"iterateImplicit", NULL, js_iterate_implicit, NULL, NULL, NULL, napi_enumerable, NULL},
+  };
+  napi_define_properties(env, exports, sizeof props / sizeof props[0], props);
+  return exports;
+}
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, init)

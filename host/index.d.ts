@@ -1,0 +1,182 @@
+/**
+ * Types of the HIP engine's Node facade (host/index.js): the GpuEuler2D
+ * method set of the reference (src/engine-gpu/euler2d-gpu.ts) and the
+ * adapters its drivers consume (src/analysis/verified-drag.ts:22-37,
+ * src/analysis/transonic-drag.ts:46-87). A TypeScript caller of the
+ * reference imports these names from here instead.
+ */
+
+export type Side = 'iMin' | 'iMax' | 'jMin' | 'jMax';
+export type Geometry = 'planar' | 'axisymmetric';
+export type LimiterName =
+  | 'firstOrder'
+  | 'minmod'
+  | 'vanLeer'
+  | 'vanAlbada'
+  | 'monotonizedCentral'
+  | 'unlimitedCentral';
+export type Reconstruction = 'perVariable' | 'shared' | 'sharedScaled';
+export type GpuFluxName = 'hllc' | 'rusanov';
+export type TimeIntegrator = 'forwardEuler' | 'ssprk2' | 'ssprk3';
+
+export interface Primitive2D {
+  readonly rho: number;
+  readonly u: number;
+  readonly v: number;
+  readonly p: number;
+}
+
+export interface StructuredGrid {
+  readonly ni: number;
+  readonly nj: number;
+  readonly x: Float64Array | readonly number[];
+  readonly y: Float64Array | readonly number[];
+}
+
+export type BoundaryCondition2D =
+  | { readonly kind: 'inflow'; readonly state: Primitive2D }
+  | { readonly kind: 'faceStates'; readonly states: readonly Primitive2D[] }
+  | { readonly kind: 'outflow' }
+  | { readonly kind: 'farField'; readonly state: Primitive2D }
+  | { readonly kind: 'wall' }
+  | { readonly kind: 'periodic' };
+
+/** euler2d.ts:58-81 / euler2d-gpu.ts:82-92 (no fitted shock, no source). */
+export interface GpuEuler2DConfig {
+  readonly grid: StructuredGrid;
+  readonly geometry: Geometry;
+  readonly gamma: number;
+  readonly flux: GpuFluxName;
+  readonly limiter: LimiterName;
+  readonly reconstruction: Reconstruction;
+  readonly timeIntegrator: TimeIntegrator;
+  readonly cfl: number;
+  readonly timeStepping: 'global' | 'local';
+  readonly boundaries: Readonly<Record<Side, BoundaryCondition2D>>;
+}
+
+export interface FaceSet {
+  readonly nx: Float64Array;
+  readonly ny: Float64Array;
+  readonly area: Float64Array;
+}
+
+export interface GridMetrics {
+  readonly geometry: Geometry;
+  readonly ni: number;
+  readonly nj: number;
+  readonly iFaces: FaceSet;
+  readonly jFaces: FaceSet;
+  readonly volume: Float64Array;
+  readonly planarArea: Float64Array;
+  readonly centroidX: Float64Array;
+  readonly centroidY: Float64Array;
+}
+
+export interface GpuState2D {
+  readonly time: number;
+  readonly steps: number;
+  readonly rho: Float32Array;
+  readonly mu: Float32Array;
+  readonly mv: Float32Array;
+  readonly energy: Float32Array;
+}
+
+export interface BoundaryFaceFlux {
+  readonly x: number;
+  readonly y: number;
+  readonly nx: number;
+  readonly ny: number;
+  readonly area: number;
+  readonly flux: readonly [number, number, number, number];
+}
+
+export interface FreeStream {
+  readonly state: Primitive2D;
+  readonly pressure: number;
+  readonly dynamicPressure: number;
+}
+
+export class NonPhysicalStateError extends Error {}
+export class GpuMemoryLimitError extends Error {}
+export class GpuExecutionError extends Error {}
+export class GpuPrecisionError extends Error {}
+
+export class ExclusiveRunner {
+  constructor(name: string);
+  assertIdle(): void;
+  run<T>(operation: () => Promise<T> | T): Promise<T>;
+}
+
+/** A HIP device ordinal wrapped as the engine's device. */
+export class HipDevice {
+  constructor(ordinal?: number);
+  readonly ordinal: number;
+  destroy(): void;
+}
+
+export class GpuEuler2D {
+  static create(device: HipDevice | number, config: GpuEuler2DConfig): Promise<GpuEuler2D>;
+  readonly config: GpuEuler2DConfig;
+  readonly metrics: GridMetrics;
+  setInitialCondition(field: (x: number, y: number) => Primitive2D): void;
+  setCellPrimitives(cell: (i: number, j: number) => Primitive2D): void;
+  iterate(count: number): Promise<void>;
+  advanceTo(tEnd: number, maxSteps?: number): Promise<void>;
+  densityResidualNorm(): Promise<number>;
+  residualVector(): Promise<Float64Array>;
+  readState(): Promise<GpuState2D>;
+  setFaceStates(side: Side, states: readonly Primitive2D[]): void;
+  setFaceFluxes(side: Side, fluxes: Float64Array | null): void;
+  wallFluxes(side?: Side): Promise<BoundaryFaceFlux[]>;
+  forebodyDrag(freeStream: { readonly pressure: number; readonly dynamicPressure: number }, baseRadius: number,
+               side?: Side): Promise<number>;
+  setImplicitCfl(cfl: number): Promise<void>;
+  iterateImplicit(
+    count: number,
+    options: { readonly cfl: number; readonly sweeps: number; readonly omega?: number; readonly growth?: number },
+  ): Promise<void>;
+  destroy(): void;
+}
+
+export function computeMetrics(grid: StructuredGrid, geometry: Geometry): GridMetrics;
+export function gpuPrimitive2D(state: GpuState2D, c: number, gamma: number): Primitive2D;
+export function freeStreamAt(mach: number, gamma: number): FreeStream;
+export function forebodyPressureDrag(
+  wall: readonly BoundaryFaceFlux[],
+  freeStream: { readonly pressure: number; readonly dynamicPressure: number },
+  baseRadius: number,
+): number;
+
+/** SteadyEngine / EngineFactory of verified-drag.ts:22-37. */
+export interface SteadyEngine {
+  iterate(steps: number): Promise<void>;
+  wallFluxes(): Promise<readonly BoundaryFaceFlux[]>;
+  cells(): Promise<(i: number, j: number) => Primitive2D>;
+  destroy?(): void;
+}
+export type EngineFactory = (config: GpuEuler2DConfig, freeStream: Primitive2D) => Promise<SteadyEngine>;
+export function hipEngine(device: HipDevice | number): EngineFactory;
+
+/** TransonicGridSolver / TransonicEngine of transonic-drag.ts:46-87. */
+export interface TransonicGridSolver {
+  start(cells: (i: number, j: number) => Primitive2D, cfl: number): Promise<void>;
+  iterate(steps: number): Promise<void>;
+  drag(): Promise<number>;
+  cells(): Promise<(i: number, j: number) => Primitive2D>;
+  residual(): Promise<number>;
+  checkPrecision(): Promise<void>;
+  destroy(): void;
+}
+export interface TransonicEngine {
+  (problem: {
+    readonly config: GpuEuler2DConfig;
+    readonly freeStream: FreeStream;
+    readonly body: { readonly nose: { readonly baseRadius: number } };
+  }): Promise<TransonicGridSolver>;
+  readonly refuses?: (cells: { readonly ni: number; readonly nj: number }) => string | null;
+}
+export function hipTransonicEngine(
+  device: HipDevice | number,
+  options: { readonly sweeps: number; readonly cfl?: number },
+): TransonicEngine;

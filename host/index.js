@@ -1,0 +1,453 @@
+'use strict';
+/**
+ * The HIP engine behind the reference's TypeScript engine interface, for
+ * Node: GpuEuler2D with the method set of src/engine-gpu/euler2d-gpu.ts
+ * (class GpuEuler2D) and the engine adapters its drivers consume
+ * (src/analysis/verified-drag.ts:22-37 EngineFactory / SteadyEngine;
+ * src/analysis/transonic-drag.ts:46-87 TransonicEngine).
+ *
+ * This file is the erasure of index.d.ts (types in index.d.ts, plain
+ * CommonJS here) so that it runs on any Node with N-API 8, with or
+ * without a TypeScript toolchain. It reaches the GPU only through the
+ * N-API addon over include/bowshock.h; there is no other backend.
+ */
+
+const path = require('path');
+
+let addon = null;
+/** The native addon; throws when it was not built (no fallback). */
+function native() {
+  if (addon === null) {
+    const file = process.env.BOWSHOCK_ADDON || path.join(__dirname, '..', 'bowshock_amd', 'lib', 'bowshock_napi.node');
+    addon = require(file);
+  }
+  return addon;
+}
+
+// ---- errors (src/solver/errors.ts:2-7, src/engine-gpu/gpu-common.ts:7-17) ----
+
+class NonPhysicalStateError extends Error {
+  constructor(message) {
+    super(message);
+    this.name = 'NonPhysicalStateError';
+  }
+}
+class GpuMemoryLimitError extends Error {
+  constructor(message) {
+    super(message);
+    this.name = 'GpuMemoryLimitError';
+  }
+}
+class GpuExecutionError extends Error {
+  constructor(message) {
+    super(message);
+    this.name = 'GpuExecutionError';
+  }
+}
+class GpuPrecisionError extends Error {
+  constructor(message) {
+    super(message);
+    this.name = 'GpuPrecisionError';
+  }
+}
+
+/** Map an addon error (its `code` is the bs_status name) to the reference's class. */
+function translate(error) {
+  if (!error || typeof error.code !== 'string') return error;
+  switch (error.code) {
+    case 'BS_ERR_RANGE':
+      return new RangeError(error.message);
+    case 'BS_ERR_NONPHYSICAL':
+      return new NonPhysicalStateError(error.message);
+    case 'BS_ERR_MEMORY':
+      return new GpuMemoryLimitError(error.message);
+    case 'BS_ERR_EXECUTION':
+      return new GpuExecutionError(error.message);
+    case 'BS_ERR_PRECISION':
+      return new GpuPrecisionError(error.message);
+    case 'BS_ERR_BUSY':
+      return new Error(error.message);
+    default:
+      return error;
+  }
+}
+
+function call(fn) {
+  try {
+    return fn();
+  } catch (error) {
+    throw translate(error);
+  }
+}
+
+/**
+ * One operation at a time per engine (gpu-common.ts:122-140
+ * ExclusiveRunner): a call made while another is running throws.
+ */
+class ExclusiveRunner {
+  constructor(name) {
+    this.name = name;
+    this.running = false;
+  }
+  assertIdle() {
+    if (this.running) throw new Error(`${this.name}: await the previous operation first`);
+  }
+  async run(operation) {
+    this.assertIdle();
+    this.running = true;
+    try {
+      return await operation();
+    } catch (error) {
+      throw translate(error);
+    } finally {
+      this.running = false;
+    }
+  }
+}
+
+// ---- configuration codes (include/bowshock.h) ----
+
+const SIDES = ['iMin', 'iMax', 'jMin', 'jMax'];
+const BOUNDARY_KINDS = { inflow: 0, outflow: 1, wall: 2, periodic: 3, faceStates: 4, farField: 5 };
+const LIMITERS = { firstOrder: 0, minmod: 1, vanLeer: 2, vanAlbada: 3, monotonizedCentral: 4, unlimitedCentral: 5 };
+const RECONSTRUCTIONS = { perVariable: 0, shared: 1, sharedScaled: 2 };
+const FLUXES = { hllc: 0, rusanov: 1 };
+const INTEGRATORS = { forwardEuler: 0, ssprk2: 1, ssprk3: 2 };
+
+function code(table, value, what) {
+  if (!Object.prototype.hasOwnProperty.call(table, value)) throw new RangeError(`unknown ${what} ${value}`);
+  return table[value];
+}
+
+/** The addon's plain-data form of a GpuEuler2DConfig (euler2d-gpu.ts:82-92). */
+function nativeConfig(config) {
+  if (config.source !== undefined) {
+    throw new RangeError('manufactured-solution sources are only supported by the CPU engine');
+  }
+  if (config.fittedShock !== undefined) {
+    throw new RangeError('shock fitting stays on the host: a fitted shock is not supported by this engine');
+  }
+  const { grid } = config;
+  const boundaries = SIDES.map((side) => {
+    const bc = config.boundaries[side];
+    const state = bc.state || (bc.states && bc.states[0]) || { rho: 0, u: 0, v: 0, p: 0 };
+    return { kind: code(BOUNDARY_KINDS, bc.kind, 'boundary kind'), rho: state.rho, u: state.u, v: state.v, p: state.p };
+  });
+  return {
+    ni: grid.ni,
+    nj: grid.nj,
+    x: Float64Array.from(grid.x),
+    y: Float64Array.from(grid.y),
+    geometry: config.geometry === 'axisymmetric' ? 1 : 0,
+    gamma: config.gamma,
+    flux: code(FLUXES, config.flux, 'GPU flux'),
+    limiter: code(LIMITERS, config.limiter, 'limiter'),
+    reconstruction: code(RECONSTRUCTIONS, config.reconstruction, 'reconstruction'),
+    integrator: code(INTEGRATORS, config.timeIntegrator, 'time integrator'),
+    cfl: config.cfl,
+    timeStepping: config.timeStepping === 'local' ? 1 : 0,
+    boundaries,
+  };
+}
+
+/** Grid metrics in FP64 (GridMetrics, src/grid/structured.ts:28-50). */
+function computeMetrics(grid, geometry) {
+  const m = call(() =>
+    native().computeMetrics(grid.ni, grid.nj, Float64Array.from(grid.x), Float64Array.from(grid.y), geometry === 'axisymmetric' ? 1 : 0),
+  );
+  return {
+    geometry,
+    ni: grid.ni,
+    nj: grid.nj,
+    iFaces: { nx: m.iNx, ny: m.iNy, area: m.iArea },
+    jFaces: { nx: m.jNx, ny: m.jNy, area: m.jArea },
+    volume: m.volume,
+    planarArea: m.planarArea,
+    centroidX: m.centroidX,
+    centroidY: m.centroidY,
+  };
+}
+
+/** A HIP device (the reference's GPUDevice). */
+class HipDevice {
+  constructor(ordinal) {
+    this.ordinal = ordinal || 0;
+    this.handle = call(() => native().ctxCreate(this.ordinal));
+  }
+  destroy() {
+    if (this.handle !== null) native().ctxDestroy(this.handle);
+    this.handle = null;
+  }
+}
+
+function packPrimitives(n, at) {
+  const packed = new Float64Array(4 * n);
+  for (let c = 0; c < n; c++) {
+    const w = at(c);
+    packed[4 * c] = w.rho;
+    packed[4 * c + 1] = w.u;
+    packed[4 * c + 2] = w.v;
+    packed[4 * c + 3] = w.p;
+  }
+  return packed;
+}
+
+class GpuEuler2D {
+  /**
+   * euler2d-gpu.ts:217-291: validate, compute the metrics (FP64, host) and
+   * upload them. `device` is a HipDevice or a device ordinal.
+   */
+  static async create(device, config) {
+    const dev = device instanceof HipDevice ? device : new HipDevice(device);
+    return new GpuEuler2D(dev, config);
+  }
+
+  constructor(device, config) {
+    this.config = config;
+    this.device = device;
+    this.metrics = computeMetrics(config.grid, config.geometry);
+    this.exclusive = new ExclusiveRunner('GpuEuler2D');
+    this.handle = call(() => native().solverCreate(device.handle, nativeConfig(config)));
+    this.nCells = config.grid.ni * config.grid.nj;
+    for (const side of SIDES) {
+      const bc = config.boundaries[side];
+      if (bc.kind === 'faceStates' && bc.states !== undefined) this.setFaceStates(side, bc.states);
+    }
+  }
+
+  /** euler2d-gpu.ts:412-432: the field at each cell centroid; resets the clock. */
+  setInitialCondition(field) {
+    this.exclusive.assertIdle();
+    const { centroidX, centroidY } = this.metrics;
+    const packed = packPrimitives(this.nCells, (c) => field(centroidX[c], centroidY[c]));
+    call(() => native().setPrimitives(this.handle, packed));
+  }
+
+  /** euler2d-gpu.ts:439-459: cell(i, j) for every cell; resets the clock. */
+  setCellPrimitives(cell) {
+    this.exclusive.assertIdle();
+    const ni = this.config.grid.ni;
+    const packed = packPrimitives(this.nCells, (c) => cell(c % ni, Math.floor(c / ni)));
+    call(() => native().setPrimitives(this.handle, packed));
+  }
+
+  /** euler2d-gpu.ts:462-473. */
+  async iterate(count) {
+    await this.exclusive.run(() => native().iterate(this.handle, count));
+  }
+
+  /** euler2d-gpu.ts:511-527. */
+  async advanceTo(tEnd, maxSteps = 1000000) {
+    if (this.config.timeStepping !== 'global') {
+      throw new Error('advanceTo needs global time stepping; use iterate() for local stepping');
+    }
+    await this.exclusive.run(() => native().advanceTo(this.handle, tEnd, maxSteps));
+  }
+
+  /** euler2d-gpu.ts:530-546 (the reduction in FP64 on the GPU). */
+  async densityResidualNorm() {
+    return this.exclusive.run(async () => native().residualNorm(this.handle));
+  }
+
+  /** Euler2D.residualVector (euler2d.ts:462-477) of the current state. */
+  async residualVector() {
+    return this.exclusive.run(async () => native().residualVector(this.handle, this.nCells));
+  }
+
+  /** euler2d-gpu.ts:549-572: GpuState2D. */
+  async readState() {
+    return this.exclusive.run(async () => {
+      const s = native().readState(this.handle, this.nCells);
+      return { time: s.report.time, steps: s.report.steps, rho: s.rho, mu: s.mu, mv: s.mv, energy: s.energy };
+    });
+  }
+
+  /** euler2d-gpu.ts:575-600. */
+  setFaceStates(side, states) {
+    this.exclusive.assertIdle();
+    const packed = packPrimitives(states.length, (t) => states[t]);
+    call(() => native().setFaceStates(this.handle, SIDES.indexOf(side), packed));
+  }
+
+  /** euler2d-gpu.ts:606-630. */
+  setFaceFluxes(side, fluxes) {
+    this.exclusive.assertIdle();
+    call(() => native().setFaceFluxes(this.handle, SIDES.indexOf(side), fluxes === null ? null : Float64Array.from(fluxes)));
+  }
+
+  /**
+   * Euler2D.boundaryFluxes (euler2d.ts:499-525) of the current state,
+   * evaluated on the GPU in FP64 (BoundaryFaceFlux[]).
+   */
+  async wallFluxes(side = 'jMin') {
+    return this.exclusive.run(async () => {
+      const k = SIDES.indexOf(side);
+      const count = k < 2 ? this.config.grid.nj : this.config.grid.ni;
+      const f = native().wallFluxes(this.handle, k, count);
+      const faces = [];
+      for (let t = 0; t < count; t++) {
+        const o = 9 * t;
+        faces.push({ x: f[o], y: f[o + 1], nx: f[o + 2], ny: f[o + 3], area: f[o + 4], flux: [f[o + 5], f[o + 6], f[o + 7], f[o + 8]] });
+      }
+      return faces;
+    });
+  }
+
+  /** forces.ts:19-29 over the side's faces, reduced in FP64 on the GPU. */
+  async forebodyDrag(freeStream, baseRadius, side = 'jMin') {
+    return this.exclusive.run(async () =>
+      native().forebodyDrag(this.handle, SIDES.indexOf(side), freeStream.pressure, freeStream.dynamicPressure, baseRadius),
+    );
+  }
+
+  /** euler2d-gpu.ts:668-681. */
+  async setImplicitCfl(cfl) {
+    if (!(cfl > 0)) throw new RangeError('the CFL number must be positive');
+    await this.exclusive.run(async () => native().setImplicitCfl(this.handle, cfl));
+  }
+
+  /** euler2d-gpu.ts:687-723. */
+  async iterateImplicit(count, options) {
+    const { cfl, sweeps } = options;
+    const omega = options.omega === undefined ? 1.5 : options.omega;
+    const growth = options.growth === undefined ? 1.2 : options.growth;
+    if (!(cfl > 0 && omega >= 1 && growth >= 1)) {
+      throw new RangeError('implicit steps need cfl > 0, omega >= 1 and growth >= 1');
+    }
+    if (!(Number.isInteger(sweeps) && sweeps >= 1)) {
+      throw new RangeError('the relaxation needs a whole number of sweeps, at least one');
+    }
+    await this.exclusive.run(() => native().iterateImplicit(this.handle, count, cfl, sweeps, omega, growth));
+  }
+
+  /** The WebGPU buffer getters and grid motion of the reference engine belong to shock fitting, which stays on the host. */
+  get stateBuffer() {
+    throw new Error('GpuEuler2D (HIP): GPU buffers are not exposed');
+  }
+  moveGrid() {
+    throw new Error('GpuEuler2D (HIP): shock fitting stays on the host');
+  }
+  iterateFitted() {
+    throw new Error('GpuEuler2D (HIP): shock fitting stays on the host');
+  }
+  readShock() {
+    throw new Error('GpuEuler2D (HIP): shock fitting stays on the host');
+  }
+
+  /** euler2d-gpu.ts:659. */
+  destroy() {
+    if (this.handle !== null) native().solverDestroy(this.handle);
+    this.handle = null;
+  }
+}
+
+/**
+ * euler2d-gpu.ts:136-146 gpuPrimitive2D: primitives of cell c of a state
+ * read back, the pressure with the 32-bit gamma the GPU solved with.
+ */
+function gpuPrimitive2D(state, c, gamma) {
+  const rho = state.rho[c];
+  const mu = state.mu[c];
+  const mv = state.mv[c];
+  const g = Math.fround(gamma);
+  return { rho, u: mu / rho, v: mv / rho, p: (g - 1) * (state.energy[c] - (0.5 * (mu * mu + mv * mv)) / rho) };
+}
+
+/**
+ * The HIP engine as a SteadyEngine factory (src/analysis/gpu-engine.ts:14-41,
+ * verified-drag.ts:22-37): steps on the GPU in FP32; the wall fluxes in
+ * FP64 on the GPU from the FP32 state.
+ */
+function hipEngine(device) {
+  return async (config, freeStream) => {
+    const { flux } = config;
+    if (!(flux === 'hllc' || flux === 'rusanov')) {
+      throw new RangeError(`the GPU engine has no ${flux} flux (only hllc and rusanov)`);
+    }
+    if (config.source !== undefined) throw new RangeError('the GPU engine has no source terms');
+    const gpu = await GpuEuler2D.create(device, config);
+    gpu.setInitialCondition(() => freeStream);
+    const ni = config.grid.ni;
+    return {
+      iterate: (steps) => gpu.iterate(steps),
+      wallFluxes: () => gpu.wallFluxes('jMin'),
+      cells: async () => {
+        const state = await gpu.readState();
+        return (i, j) => gpuPrimitive2D(state, i + j * ni, config.gamma);
+      },
+      destroy: () => {
+        gpu.destroy();
+      },
+    };
+  };
+}
+
+/**
+ * The HIP engine as a TransonicEngine (transonic-drag.ts:46-87,
+ * gpu-transonic.ts:26-106): implicit red-black steps with a CFL ramp, the
+ * drag and residual from the GPU in FP64.
+ */
+function hipTransonicEngine(device, options) {
+  const sweeps = options.sweeps;
+  const cfl = options.cfl === undefined ? 1000 : options.cfl;
+  if (!(Number.isInteger(sweeps) && sweeps >= 1)) throw new RangeError('the relaxation needs a whole number of sweeps');
+  const solve = async (problem) => {
+    const gpu = await GpuEuler2D.create(device, problem.config);
+    const { ni } = problem.config.grid;
+    const gamma = problem.config.gamma;
+    return {
+      start: async (cells, startCfl) => {
+        gpu.setCellPrimitives(cells);
+        await gpu.setImplicitCfl(startCfl);
+      },
+      iterate: (steps) => gpu.iterateImplicit(steps, { cfl, sweeps }),
+      // The forebody drag of the body's wall (jMin), FP64 on the GPU.
+      drag: () => gpu.forebodyDrag(problem.freeStream, problem.body.nose.baseRadius),
+      residual: () => gpu.densityResidualNorm(),
+      // The residual is an FP64 reduction of the FP32 residual on the GPU; no
+      // split-float arithmetic is involved, so there are no low parts to lose.
+      // The check reads the state back, which throws if it is non-physical.
+      checkPrecision: async () => {
+        await gpu.readState();
+      },
+      cells: async () => {
+        const state = await gpu.readState();
+        return (i, j) => gpuPrimitive2D(state, i + j * ni, gamma);
+      },
+      destroy: () => gpu.destroy(),
+    };
+  };
+  return Object.assign(solve, { refuses: () => null });
+}
+
+/** verified-drag.ts:225-228 freeStreamAt. */
+function freeStreamAt(mach, gamma) {
+  const u = Math.fround(mach * Math.sqrt(gamma));
+  return { state: { rho: 1, u, v: 0, p: 1 }, pressure: 1, dynamicPressure: 0.5 * u * u };
+}
+
+/** forces.ts:19-29 forebodyPressureDrag, on faces from wallFluxes. */
+function forebodyPressureDrag(wall, freeStream, baseRadius) {
+  let force = 0;
+  for (const face of wall) force += (face.flux[1] - freeStream.pressure * face.nx) * face.area;
+  return (2 * force) / (freeStream.dynamicPressure * baseRadius * baseRadius);
+}
+
+module.exports = {
+  GpuEuler2D,
+  HipDevice,
+  ExclusiveRunner,
+  NonPhysicalStateError,
+  GpuMemoryLimitError,
+  GpuExecutionError,
+  GpuPrecisionError,
+  computeMetrics,
+  gpuPrimitive2D,
+  hipEngine,
+  hipTransonicEngine,
+  freeStreamAt,
+  forebodyPressureDrag,
+  translate,
+  nativeConfig,
+  native,
+};

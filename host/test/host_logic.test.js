@@ -1,0 +1,102 @@
+'use strict';
+// Host-side logic of the Node facade, runnable without a GPU: the addon
+// loads and exports its functions, errors map to the reference's classes,
+// the exclusive runner refuses overlapping calls, configurations are
+// checked before anything reaches the device, and the helpers restate
+// verified-drag.ts / forces.ts.
+const assert = require('assert');
+const bs = require('..');
+
+const results = [];
+async function test(name, fn) {
+  try {
+    await fn();
+    results.push(['ok', name]);
+  } catch (e) {
+    results.push(['FAIL', name, e && e.stack]);
+  }
+}
+
+(async () => {
+  await test('addon exports every binding', () => {
+    const names = ['version', 'ctxCreate', 'ctxDestroy', 'solverCreate', 'solverDestroy', 'computeMetrics',
+      'setPrimitives', 'setState', 'readState', 'setFaceStates', 'setFaceFluxes', 'residualNorm',
+      'residualVector', 'wallFluxes', 'forebodyDrag', 'setImplicitCfl', 'iterate', 'advanceTo', 'iterateImplicit'];
+    const a = bs.native();
+    for (const n of names) assert.strictEqual(typeof a[n], 'function', n);
+    assert.ok(/gfx950/.test(a.version()));
+  });
+
+  await test('error codes map to the reference classes', () => {
+    const mk = (code) => Object.assign(new Error('m'), { code });
+    assert.ok(bs.translate(mk('BS_ERR_RANGE')) instanceof RangeError);
+    assert.ok(bs.translate(mk('BS_ERR_NONPHYSICAL')) instanceof bs.NonPhysicalStateError);
+    assert.ok(bs.translate(mk('BS_ERR_MEMORY')) instanceof bs.GpuMemoryLimitError);
+    assert.ok(bs.translate(mk('BS_ERR_EXECUTION')) instanceof bs.GpuExecutionError);
+    assert.ok(bs.translate(mk('BS_ERR_PRECISION')) instanceof bs.GpuPrecisionError);
+    assert.strictEqual(bs.translate(mk('BS_ERR_NONPHYSICAL')).name, 'NonPhysicalStateError');
+  });
+
+  await test('the exclusive runner refuses overlapping operations', async () => {
+    const r = new bs.ExclusiveRunner('GpuEuler2D');
+    let release;
+    const first = r.run(() => new Promise((resolve) => { release = resolve; }));
+    await assert.rejects(r.run(async () => 1), /await the previous operation first/);
+    assert.throws(() => r.assertIdle(), /await the previous operation first/);
+    release(7);
+    assert.strictEqual(await first, 7);
+    assert.strictEqual(await r.run(async () => 2), 2);
+  });
+
+  const grid = { ni: 2, nj: 2, x: [0, 1, 2, 0, 1, 2, 0, 1, 2], y: [0, 0, 0, 1, 1, 1, 2, 2, 2] };
+  const free = { rho: 1, u: 2, v: 0, p: 1 };
+  const base = {
+    grid, geometry: 'axisymmetric', gamma: 1.4, flux: 'hllc', limiter: 'minmod', reconstruction: 'perVariable',
+    timeIntegrator: 'ssprk3', cfl: 0.5, timeStepping: 'local',
+    boundaries: { iMin: { kind: 'inflow', state: free }, iMax: { kind: 'outflow' }, jMin: { kind: 'wall' },
+      jMax: { kind: 'inflow', state: free } },
+  };
+
+  await test('configurations are checked on the host', () => {
+    const c = bs.nativeConfig(base);
+    assert.strictEqual(c.limiter, 1);
+    assert.strictEqual(c.boundaries[0].kind, 0);
+    assert.strictEqual(c.boundaries[2].kind, 2);
+    assert.strictEqual(c.x.length, 9);
+    assert.throws(() => bs.nativeConfig(Object.assign({}, base, { flux: 'godunov' })), RangeError);
+    assert.throws(() => bs.nativeConfig(Object.assign({}, base, { limiter: 'superbee' })), RangeError);
+    assert.throws(() => bs.nativeConfig(Object.assign({}, base, { source: new Float64Array(16) })), RangeError);
+    assert.throws(() => bs.nativeConfig(Object.assign({}, base, { fittedShock: { freeStream: free } })), RangeError);
+  });
+
+  await test('metrics come from the library without a GPU', () => {
+    const m = bs.computeMetrics(grid, 'axisymmetric');
+    assert.strictEqual(m.iFaces.nx.length, 6);
+    assert.strictEqual(m.volume[0], 0.5);  // unit cell, centroid radius 0.5
+    assert.strictEqual(m.jFaces.ny[0], 1);
+    assert.throws(() => bs.computeMetrics({ ni: 2, nj: 2, x: grid.x, y: grid.y.map((v) => v - 1) }, 'axisymmetric'),
+      RangeError);
+  });
+
+  await test('freeStreamAt and forebodyPressureDrag restate the reference', () => {
+    const f = bs.freeStreamAt(3, 1.4);
+    assert.strictEqual(f.state.u, Math.fround(3 * Math.sqrt(1.4)));
+    assert.strictEqual(f.dynamicPressure, 0.5 * f.state.u * f.state.u);
+    const wall = [{ x: 0, y: 0, nx: 1, ny: 0, area: 0.1, flux: [0, 2, 0, 0] },
+      { x: 0, y: 0, nx: 0.5, ny: 0.5, area: 0.2, flux: [0, 0.5, 0, 0] }];
+    assert.ok(Math.abs(bs.forebodyPressureDrag(wall, { pressure: 0, dynamicPressure: 1 }, 1) - 0.6) < 1e-15);
+  });
+
+  await test('a device that does not exist is an error, not a fallback', () => {
+    let threw = false;
+    try {
+      new bs.HipDevice(99);
+    } catch (e) {
+      threw = e instanceof RangeError || e instanceof bs.GpuExecutionError;
+    }
+    assert.ok(threw);
+  });
+
+  for (const r of results) console.log(r[0], r[1], r[2] || '');
+  process.exit(results.every((r) => r[0] === 'ok') ? 0 : 1);
+})();
