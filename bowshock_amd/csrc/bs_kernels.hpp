@@ -184,11 +184,21 @@ __global__ __launch_bounds__(TX * TY) void tile_kernel(TileArgs A) {
   const int gi = i0 + li, gj = j0 + lj;
   const bool valid = gi < ni && gj < nj;
   Vec4<float> own = {0.f, 0.f, 0.f, 0.f};
+  // Tiles whose whole window lies inside the grid (all but a ring of them)
+  // take the direct loads; the block-uniform test costs no divergence.
+  const bool inner = i0 >= 2 && j0 >= 2 && i0 + TX + 2 <= ni && j0 + TY + 2 <= nj;
+  auto fetch = [&](int i, int j) -> Vec4<float> {
+    if (inner) {
+      const int c = i + j * ni;
+      return {A.src.q[0][c], A.src.q[1][c], A.src.q[2][c], A.src.q[3][c]};
+    }
+    return load_cell(A.src, G, *A.B, i, j, g);
+  };
   // On a tile at the high edge of the grid the thread's position may be a
   // ghost cell (gi in [ni, ni + 1] or gj in [nj, nj + 1]) that the faces
   // near the edge still need.
   if (!(gi >= ni && gj >= nj) && gi <= ni + 1 && gj <= nj + 1) {
-    const Vec4<float> q = load_cell(A.src, G, *A.B, gi, gj, g);
+    const Vec4<float> q = fetch(gi, gj);
     if (valid) own = q;
     Prim<float> w = to_prim(q.x, q.y, q.z, q.w, g);
     sp[0][lj + 2][li + 2] = w.rho;
@@ -210,7 +220,7 @@ __global__ __launch_bounds__(TX * TY) void tile_kernel(TileArgs A) {
     const int hi = i0 + wi - 2, hj = j0 + wj - 2;
     const bool oi = hi < 0 || hi >= ni, oj = hj < 0 || hj >= nj;
     if ((oi && oj) || hi > ni + 1 || hj > nj + 1) continue;
-    Vec4<float> q = load_cell(A.src, G, *A.B, hi, hj, g);
+    Vec4<float> q = fetch(hi, hj);
     Prim<float> w = to_prim(q.x, q.y, q.z, q.w, g);
     sp[0][wj][wi] = w.rho;
     sp[1][wj][wi] = w.u;
@@ -244,8 +254,8 @@ __global__ __launch_bounds__(TX * TY) void tile_kernel(TileArgs A) {
                                         wall, g);
       }
       if (MODE == TILE_IMPLICIT) {
-        float rl = gi > 0 ? bs_abs(l0.u * nx + l0.v * ny) + bs_sqrt(g * l0.p / l0.rho) : 0.f;
-        float rr = gi < ni ? bs_abs(r0.u * nx + r0.v * ny) + bs_sqrt(g * r0.p / r0.rho) : 0.f;
+        float rl = gi > 0 ? bs_abs(l0.u * nx + l0.v * ny) + bs_sqrt(bs_div(g * l0.p, l0.rho)) : 0.f;
+        float rr = gi < ni ? bs_abs(r0.u * nx + r0.v * ny) + bs_sqrt(bs_div(g * r0.p, r0.rho)) : 0.f;
         lam = bs_max(rl, rr) * area;
         A.lam_i[fidx] = lam;
       }
@@ -274,8 +284,8 @@ __global__ __launch_bounds__(TX * TY) void tile_kernel(TileArgs A) {
                                         wall, g);
       }
       if (MODE == TILE_IMPLICIT) {
-        float rl = gj > 0 ? bs_abs(l0.u * nx + l0.v * ny) + bs_sqrt(g * l0.p / l0.rho) : 0.f;
-        float rr = gj < nj ? bs_abs(r0.u * nx + r0.v * ny) + bs_sqrt(g * r0.p / r0.rho) : 0.f;
+        float rl = gj > 0 ? bs_abs(l0.u * nx + l0.v * ny) + bs_sqrt(bs_div(g * l0.p, l0.rho)) : 0.f;
+        float rr = gj < nj ? bs_abs(r0.u * nx + r0.v * ny) + bs_sqrt(bs_div(g * r0.p, r0.rho)) : 0.f;
         lam = bs_max(rl, rr) * area;
         A.lam_j[fidx] = lam;
       }
@@ -318,21 +328,21 @@ __global__ __launch_bounds__(TX * TY) void tile_kernel(TileArgs A) {
       if (A.dt_mode == DT_COMPUTE_LOCAL) {
         // euler2d.ts:1183-1205 computeTimeSteps: mean area vector of the
         // two faces in each direction.
-        const float a = bs_sqrt(g * w.p / w.rho);
+        const float a = bs_sqrt(bs_div(g * w.p, w.rho));
         const float six = 0.5f * (xi[0][lj][li] + xi[0][lj][li + 1]);
         const float siy = 0.5f * (xi[1][lj][li] + xi[1][lj][li + 1]);
         const float sjx = 0.5f * (xj[0][lj][li] + xj[0][lj + 1][li]);
         const float sjy = 0.5f * (xj[1][lj][li] + xj[1][lj + 1][li]);
         const float lam = bs_abs(w.u * six + w.v * siy) + a * bs_sqrt(six * six + siy * siy) +
                           bs_abs(w.u * sjx + w.v * sjy) + a * bs_sqrt(sjx * sjx + sjy * sjy);
-        dt = A.cfl * vol / lam;
+        dt = A.cfl * vol * bs_rcp(lam);
         A.dt[c] = dt;
       } else if (A.dt_mode == DT_GLOBAL) {
         dt = *A.dt_global;
       } else {
         dt = A.dt[c];
       }
-      const float k = dt / vol;
+      const float k = dt * bs_rcp(vol);
       Vec4<float> out;
       const float srcq[4] = {own.x, own.y, own.z, own.w};
       float o[4];
@@ -367,13 +377,13 @@ __global__ __launch_bounds__(TX * TY) void tile_kernel(TileArgs A) {
       const double cfl = A.control->cfl;
       const float scale = (float)(0.5 * ((double)A.omega + 1.0 / cfl));
       float d = scale * (xi[0][lj][li] + xi[0][lj][li + 1] + xj[0][lj][li] + xj[0][lj + 1][li]);
-      if (A.axisymmetric) d += planar * (bs_abs(w.v) + bs_sqrt(g * w.p / w.rho));
+      if (A.axisymmetric) d += planar * (bs_abs(w.v) + bs_sqrt(bs_div(g * w.p, w.rho)));
       A.diag[c] = d;
       const bool red = ((gi + gj) & 1) == 0;
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         A.res[q][c] = r[q];
-        A.dq[q][c] = red ? r[q] / d : 0.f;
+        A.dq[q][c] = red ? bs_div(r[q], d) : 0.f;
       }
     } else if (A.rvec[0] != nullptr) {
 #pragma unroll
@@ -410,7 +420,7 @@ __global__ __launch_bounds__(256) void time_step_kernel(GridDev G, StateDev s, f
   if (c < G.ni * G.nj) {
     const int i = c % ni, j = c / ni;
     Prim<float> w = to_prim(s.q[0][c], s.q[1][c], s.q[2][c], s.q[3][c], g);
-    float a = bs_sqrt(g * w.p / w.rho);
+    float a = bs_sqrt(bs_div(g * w.p, w.rho));
     int f0 = i + j * (ni + 1), f1 = f0 + 1, h0 = c, h1 = c + ni;
     float six = 0.5f * (G.inx[f0] * G.ia[f0] + G.inx[f1] * G.ia[f1]);
     float siy = 0.5f * (G.iny[f0] * G.ia[f0] + G.iny[f1] * G.ia[f1]);
@@ -418,7 +428,7 @@ __global__ __launch_bounds__(256) void time_step_kernel(GridDev G, StateDev s, f
     float sjy = 0.5f * (G.jny[h0] * G.ja[h0] + G.jny[h1] * G.ja[h1]);
     float lam = bs_abs(w.u * six + w.v * siy) + a * bs_sqrt(six * six + siy * siy) +
                 bs_abs(w.u * sjx + w.v * sjy) + a * bs_sqrt(sjx * sjx + sjy * sjy);
-    dt = cfl * G.vol[c] / lam;
+    dt = cfl * G.vol[c] * bs_rcp(lam);
     if (!(dt == dt)) dt = -1.f;  // NaN marks the block
   }
   red[threadIdx.x] = dt;
@@ -490,11 +500,13 @@ __device__ __forceinline__ void neighbour_term(const RelaxArgs& A, int nb, float
   const float g = A.gamma;
   const float rho = A.q.q[0][nb], mx = A.q.q[1][nb], my = A.q.q[2][nb], e = A.q.q[3][nb];
   const float d0 = A.dq[0][nb], d1 = A.dq[1][nb], d2 = A.dq[2][nb], d3 = A.dq[3][nb];
-  const float p0 = (g - 1.f) * (e - 0.5f * (mx * mx + my * my) / rho);
-  const float un0 = (mx * nx + my * ny) / rho;
+  const float ir0 = bs_rcp(rho);
+  const float p0 = (g - 1.f) * (e - 0.5f * (mx * mx + my * my) * ir0);
+  const float un0 = (mx * nx + my * ny) * ir0;
   const float rho1 = rho + d0, mx1 = mx + d1, my1 = my + d2, e1 = e + d3;
-  const float p1 = (g - 1.f) * (e1 - 0.5f * (mx1 * mx1 + my1 * my1) / rho1);
-  const float un1 = (mx1 * nx + my1 * ny) / rho1;
+  const float ir1 = bs_rcp(rho1);
+  const float p1 = (g - 1.f) * (e1 - 0.5f * (mx1 * mx1 + my1 * my1) * ir1);
+  const float un1 = (mx1 * nx + my1 * ny) * ir1;
   const float half = 0.5f * area, damp = 0.5f * A.omega * lam_a;
   sum[0] += half * (rho1 * un1 - rho * un0) - damp * d0;
   sum[1] += half * (mx1 * un1 + p1 * nx - (mx * un0 + p0 * nx)) - damp * d1;
@@ -525,9 +537,9 @@ __global__ __launch_bounds__(256) void relax_kernel(RelaxArgs A) {
     int f = c + ni;
     neighbour_term(A, c + ni, G.jnx[f], G.jny[f], G.ja[f], A.lam_j[f], sum);
   }
-  const float d = A.diag[c];
+  const float id = bs_rcp(A.diag[c]);
 #pragma unroll
-  for (int q = 0; q < 4; q++) A.dq[q][c] = (A.res[q][c] - sum[q]) / d;
+  for (int q = 0; q < 4; q++) A.dq[q][c] = (A.res[q][c] - sum[q]) * id;
 }
 
 // q + alpha dQ, alpha halved (at most 30 times) until density and pressure
@@ -539,11 +551,11 @@ __global__ __launch_bounds__(256) void implicit_update_kernel(StateDev src, Stat
   if (c < n) {
     const float rho = src.q[0][c], mx = src.q[1][c], my = src.q[2][c], e = src.q[3][c];
     const float d0 = dq.q[0][c], d1 = dq.q[1][c], d2 = dq.q[2][c], d3 = dq.q[3][c];
-    const float p = (g - 1.f) * (e - 0.5f * (mx * mx + my * my) / rho);
+    const float p = (g - 1.f) * (e - 0.5f * (mx * mx + my * my) * bs_rcp(rho));
     float alpha = 1.f;
     for (int tries = 0; tries < 30; tries++) {
       float rho1 = rho + alpha * d0, mx1 = mx + alpha * d1, my1 = my + alpha * d2, e1 = e + alpha * d3;
-      float p1 = (g - 1.f) * (e1 - 0.5f * (mx1 * mx1 + my1 * my1) / rho1);
+      float p1 = (g - 1.f) * (e1 - 0.5f * (mx1 * mx1 + my1 * my1) * bs_rcp(rho1));
       if (rho1 > 0.5f * rho && rho1 < 2.f * rho && p1 > 0.5f * p && p1 < 2.f * p) break;
       alpha *= 0.5f;
     }

@@ -24,9 +24,23 @@ template <typename T> struct Prim { T rho, u, v, p; };
 template <typename T> struct FaceState { T rho, un, ut, p; };
 template <typename T> struct Vec4 { T x, y, z, w; };
 
+// Reciprocal, square root and division. FP32 (the stencil kernels): the
+// hardware v_rcp_f32 / v_rsq_f32 / v_sqrt_f32 (1 ulp), a / b as a * rcp(b);
+// the compiler's own division and square root wrap each in range-scaling
+// sequences of 6-10 instructions, and a face needs about a dozen. FP64 (the
+// wall-flux and drag evaluation): IEEE operations, as the oracle.
 template <typename T> __device__ __forceinline__ T bs_sqrt(T x);
-template <> __device__ __forceinline__ float bs_sqrt<float>(float x) { return __builtin_sqrtf(x); }
+template <> __device__ __forceinline__ float bs_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 template <> __device__ __forceinline__ double bs_sqrt<double>(double x) { return __builtin_sqrt(x); }
+template <typename T> __device__ __forceinline__ T bs_rcp(T x);
+template <> __device__ __forceinline__ float bs_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <> __device__ __forceinline__ double bs_rcp<double>(double x) { return 1.0 / x; }
+template <typename T> __device__ __forceinline__ T bs_rsqrt(T x);
+template <> __device__ __forceinline__ float bs_rsqrt<float>(float x) { return __builtin_amdgcn_rsqf(x); }
+template <> __device__ __forceinline__ double bs_rsqrt<double>(double x) { return 1.0 / __builtin_sqrt(x); }
+template <typename T> __device__ __forceinline__ T bs_div(T a, T b);
+template <> __device__ __forceinline__ float bs_div<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+template <> __device__ __forceinline__ double bs_div<double>(double a, double b) { return a / b; }
 template <typename T> __device__ __forceinline__ T bs_abs(T x) { return x < T(0) ? -x : x; }
 template <typename T> __device__ __forceinline__ T bs_min(T a, T b) { return a < b ? a : b; }
 template <typename T> __device__ __forceinline__ T bs_max(T a, T b) { return a > b ? a : b; }
@@ -34,7 +48,7 @@ template <typename T> __device__ __forceinline__ T bs_max(T a, T b) { return a >
 // gas.ts:43-51 pressureFromConserved2D; gas.wgsl to_primitive_2d.
 template <typename T>
 __device__ __forceinline__ Prim<T> to_prim(T rho, T mu, T mv, T e, T g) {
-  T inv = T(1) / rho;
+  T inv = bs_rcp(rho);
   Prim<T> w;
   w.rho = rho;
   w.u = mu * inv;
@@ -46,7 +60,7 @@ __device__ __forceinline__ Prim<T> to_prim(T rho, T mu, T mv, T e, T g) {
 // euler2d.ts:413-416: conserved from primitive.
 template <typename T>
 __device__ __forceinline__ Vec4<T> to_cons(Prim<T> w, T g) {
-  return {w.rho, w.rho * w.u, w.rho * w.v, w.p / (g - T(1)) + T(0.5) * w.rho * (w.u * w.u + w.v * w.v)};
+  return {w.rho, w.rho * w.u, w.rho * w.v, bs_div(w.p, g - T(1)) + T(0.5) * w.rho * (w.u * w.u + w.v * w.v)};
 }
 
 // limiters.ts:24-94.
@@ -55,10 +69,10 @@ __device__ __forceinline__ T limit(int kind, T a, T b) {
   switch (kind) {
     case FIRST_ORDER: return T(0);
     case MINMOD: return a * b <= T(0) ? T(0) : (bs_abs(a) < bs_abs(b) ? a : b);
-    case VAN_LEER: return a * b <= T(0) ? T(0) : (T(2) * a * b) / (a + b);
+    case VAN_LEER: return a * b <= T(0) ? T(0) : bs_div(T(2) * a * b, a + b);
     case VAN_ALBADA: {
       const T e = T(1e-6);
-      return (a * (b * b + e) + b * (a * a + e)) / (a * a + b * b + T(2) * e);
+      return bs_div(a * (b * b + e) + b * (a * a + e), a * a + b * b + T(2) * e);
     }
     case MONOTONIZED_CENTRAL: {
       if (a * b <= T(0)) return T(0);
@@ -88,7 +102,7 @@ __device__ __forceinline__ Prim<T> reconstruct(int lim, const Prim<T>& p, const 
   if (REC == SHARED_SCALED) {
     T rho = bs_min(bs_min(p.rho, c.rho), n.rho);
     T rhoa2 = g * bs_min(bs_min(p.p, c.p), n.p);
-    T sr = T(1) / rho, su = bs_sqrt(rho / rhoa2), sp = T(1) / rhoa2;
+    T sr = bs_rcp(rho), sp = bs_rcp(rhoa2), su = bs_sqrt(rho * sp);
     ar *= sr; au *= su; av *= su; ap *= sp;
     br *= sr; bu *= su; bv *= su; bp *= sp;
   }
@@ -96,7 +110,7 @@ __device__ __forceinline__ Prim<T> reconstruct(int lim, const Prim<T>& p, const 
   T ab = ar * br + au * bu + av * bv + ap * bp;
   T aa = ar * ar + au * au + av * av + ap * ap;
   T bb = br * br + bu * bu + bv * bv + bp * bp;
-  T f = (T(2) * (ab + e)) / (aa + bb + T(2) * e);
+  T f = bs_div(T(2) * (ab + e), aa + bb + T(2) * e);
   f = f < T(0) ? T(0) : f;
   T k = T(0.25) * side * f;
   o.rho = c.rho + k * (n.rho - p.rho);
@@ -109,7 +123,7 @@ __device__ __forceinline__ Prim<T> reconstruct(int lim, const Prim<T>& p, const 
 // flux.ts:34-36 energyOf.
 template <typename T>
 __device__ __forceinline__ T energy_of(const FaceState<T>& w, T g) {
-  return w.p / (g - T(1)) + T(0.5) * w.rho * (w.un * w.un + w.ut * w.ut);
+  return bs_div(w.p, g - T(1)) + T(0.5) * w.rho * (w.un * w.un + w.ut * w.ut);
 }
 
 // flux.ts:39-45 physicalFaceFlux.
@@ -123,10 +137,12 @@ __device__ __forceinline__ Vec4<T> physical_flux(const FaceState<T>& w, T e, T g
 template <typename T>
 __device__ __forceinline__ Vec4<T> hllc(const FaceState<T>& l, const FaceState<T>& r, T g) {
   T el = energy_of(l, g), er = energy_of(r, g);
-  T il = T(1) / l.rho, ir = T(1) / r.rho;
+  // 1 / rho and sqrt(rho) from one reciprocal square root per side.
+  T sl_ = bs_rsqrt(l.rho), sr_ = bs_rsqrt(r.rho);
+  T il = sl_ * sl_, ir = sr_ * sr_;
+  T wl = l.rho * sl_, wr = r.rho * sr_;
   T al = bs_sqrt(g * l.p * il), ar = bs_sqrt(g * r.p * ir);
-  T wl = bs_sqrt(l.rho), wr = bs_sqrt(r.rho);
-  T iw = T(1) / (wl + wr);
+  T iw = bs_rcp(wl + wr);
   T un = (wl * l.un + wr * r.un) * iw;
   T ut = (wl * l.ut + wr * r.ut) * iw;
   T hl = (el + l.p) * il, hr = (er + r.p) * ir;
@@ -138,15 +154,15 @@ __device__ __forceinline__ Vec4<T> hllc(const FaceState<T>& l, const FaceState<T
   if (sr <= T(0)) return physical_flux(r, er, g);
   T ml = l.rho * (sl - l.un);
   T mr = r.rho * (sr - r.un);
-  T ss = (r.p - l.p + l.un * ml - r.un * mr) / (ml - mr);
+  T ss = bs_div(r.p - l.p + l.un * ml - r.un * mr, ml - mr);
   bool left = ss >= T(0);
   FaceState<T> k = left ? l : r;
   T sk = left ? sl : sr;
   T ek = left ? el : er;
   T ik = left ? il : ir;
   T d = sk - k.un;
-  T srho = (k.rho * d) / (sk - ss);
-  T se = srho * (ek * ik + (ss - k.un) * (ss + k.p / (k.rho * d)));
+  T srho = bs_div(k.rho * d, sk - ss);
+  T se = srho * (ek * ik + (ss - k.un) * (ss + bs_div(k.p, k.rho * d)));
   T mass = k.rho * k.un;
   return {mass + sk * (srho - k.rho), mass * k.un + k.p + sk * (srho * ss - mass),
           mass * k.ut + sk * (srho - k.rho) * k.ut, k.un * (ek + k.p) + sk * (se - ek)};
@@ -155,7 +171,7 @@ __device__ __forceinline__ Vec4<T> hllc(const FaceState<T>& l, const FaceState<T
 // flux.ts:55-78 rusanovFaceFlux.
 template <typename T>
 __device__ __forceinline__ Vec4<T> rusanov(const FaceState<T>& l, const FaceState<T>& r, T g) {
-  T s = bs_max(bs_abs(l.un) + bs_sqrt(g * l.p / l.rho), bs_abs(r.un) + bs_sqrt(g * r.p / r.rho));
+  T s = bs_max(bs_abs(l.un) + bs_sqrt(bs_div(g * l.p, l.rho)), bs_abs(r.un) + bs_sqrt(bs_div(g * r.p, r.rho)));
   T el = energy_of(l, g), er = energy_of(r, g);
   T ml = l.rho * l.un, mr = r.rho * r.un;
   return {T(0.5) * (ml + mr) - T(0.5) * s * (r.rho - l.rho),
