@@ -57,6 +57,7 @@ SIGNATURES = {
     "bs_set_implicit_cfl": (C.c_int, [_vp, C.c_double]),
     "bs_iterate_implicit": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_int32, C.c_double, C.c_double,
                                       C.POINTER(Report)]),
+    "bs_enable_graphs": (C.c_int, [_vp, C.c_int32]),
     "bs_enable_timing": (C.c_int, [_vp, C.c_int32]),
     "bs_kernel_timing": (C.c_int, [_vp, C.c_int32, _dp, C.POINTER(C.c_int64)]),
 }

@@ -47,6 +47,7 @@ bs_status hip_fail(hipError_t e, const char* what) {
   } while (0)
 
 constexpr int BATCH = 256;  // steps between status checks (time-loop.ts:13)
+constexpr int GRAPH_STEPS = 12;  // steps per captured graph (a multiple of 3)
 // Tile of the stencil kernels: TX cells along i (one wavefront row when 64)
 // by TY along j; set at build time.
 #ifndef BS_TILE_X
@@ -90,6 +91,18 @@ struct Timing {
   long long launches[4] = {0, 0, 0, 0};
 };
 
+// A captured run of GRAPH_STEPS steps starting from state buffer `cur`;
+// the run returns to the same buffer (RK stages write the last stage in
+// place; single-stage and implicit steps cycle the three buffers, and
+// GRAPH_STEPS is a multiple of 3).
+struct StepGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int cur = -1;
+  int sweeps = 0;
+  double omega = 0, growth = 0, cfl_max = 0;
+};
+
 struct bs_solver {
   bs_config cfg;
   int device;
@@ -104,6 +117,8 @@ struct bs_solver {
   std::atomic<bool> busy{false};
   Timing timing;
   long long steps_accounted = 0;
+  bool graphs = true;
+  StepGraph explicit_graph, implicit_graph;
 };
 
 namespace {
@@ -324,6 +339,63 @@ bs_status encode_step(bs_solver* s) {
   return BS_OK;
 }
 
+void drop_graph(StepGraph& g) {
+  if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  if (g.graph) (void)hipGraphDestroy(g.graph);
+  g = StepGraph();
+}
+
+// Capture GRAPH_STEPS steps that `encode` enqueues into one graph. Nothing
+// runs during the capture, so the buffer index is restored afterwards.
+template <typename Encode>
+bs_status capture_steps(bs_solver* s, StepGraph& g, Encode encode) {
+  drop_graph(g);
+  const int cur0 = s->cur;
+  HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+  bs_status st = BS_OK;
+  for (int k = 0; k < GRAPH_STEPS && st == BS_OK; k++) st = encode();
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(s->stream, &graph);
+  const int cur_after = s->cur;
+  s->cur = cur0;
+  if (st != BS_OK || e != hipSuccess || cur_after != cur0) {
+    if (graph) (void)hipGraphDestroy(graph);
+    if (st != BS_OK) return st;
+    if (e != hipSuccess) return hip_fail(e, "graph capture");
+    return fail(BS_ERR_EXECUTION, "captured steps do not return to their state buffer");
+  }
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return hip_fail(e, "graph instantiate");
+  }
+  g.graph = graph;
+  g.exec = exec;
+  g.cur = cur0;
+  return BS_OK;
+}
+
+// `count` steps: whole captured graphs where possible (one launch per
+// GRAPH_STEPS steps), the rest launched kernel by kernel. Kernel timing
+// needs the events between launches, so it runs without graphs.
+template <typename Encode>
+bs_status run_steps(bs_solver* s, StepGraph& g, int count, Encode encode) {
+  int k = 0;
+  if (s->graphs && !s->timing.enabled && count >= GRAPH_STEPS) {
+    if (!(g.exec && g.cur == s->cur)) {
+      bs_status st = capture_steps(s, g, encode);
+      if (st) return st;
+    }
+    for (; k + GRAPH_STEPS <= count; k += GRAPH_STEPS) HIP_TRY(hipGraphLaunch(g.exec, s->stream), "graph launch");
+  }
+  for (; k < count; k++) {
+    bs_status st = encode();
+    if (st) return st;
+  }
+  return BS_OK;
+}
+
 bs_status read_control(bs_solver* s, Control* c) {
   HIP_TRY(hipMemcpyAsync(c, s->d.control, sizeof(Control), hipMemcpyDeviceToHost, s->stream), "control readback");
   HIP_TRY(hipStreamSynchronize(s->stream), "stream synchronize");
@@ -376,6 +448,8 @@ bs_status dev_alloc(T** p, size_t count, const char* what) {
 }
 
 void free_all(bs_solver* s) {
+  drop_graph(s->explicit_graph);
+  drop_graph(s->implicit_graph);
   DeviceBuffers& d = s->d;
   void* all[] = {d.state[0], d.state[1], d.state[2], d.inx, d.iny, d.ia, d.jnx, d.jny, d.ja, d.vol, d.planar,
                  d.dt, d.dt_global, d.dt_partials, d.partials, d.scalar, d.control, d.bnd, d.wall_out, d.res,
@@ -714,11 +788,9 @@ bs_status bs_iterate(bs_solver* s, int32_t count, bs_report* report) {
   int remaining = count;
   while (remaining > 0) {
     const int batch = remaining < BATCH ? remaining : BATCH;
-    for (int k = 0; k < batch; k++) {
-      bs_status st = encode_step(s);
-      if (st) return st;
-    }
-    bs_status st = read_control(s, &c);
+    bs_status st = run_steps(s, s->explicit_graph, batch, [s]() { return encode_step(s); });
+    if (st) return st;
+    st = read_control(s, &c);
     if (st) return st;
     fill_report(c, report);
     if ((st = account(s, c, batch))) return st;
@@ -907,52 +979,63 @@ bs_status bs_iterate_implicit(bs_solver* s, int32_t count, double cfl_max, int32
     res.q[q] = s->d.res + q * n;
     dq.q[q] = s->d.dq + q * n;
   }
+  // One implicit step: prepare (residual, lambda A, diagonal, first red
+  // half-sweep), the remaining half-sweeps, the update into the next buffer.
+  auto encode = [&]() -> bs_status {
+    const int src = s->cur, dst = (s->cur + 1) % 3;
+    TileArgs A = tile_args(s);
+    A.src = state_of(s, src);
+    A.omega = (float)omega;
+    for (int q = 0; q < 4; q++) {
+      A.res[q] = res.q[q];
+      A.dq[q] = dq.q[q];
+    }
+    A.diag = s->d.diag;
+    A.lam_i = s->d.lam_i;
+    A.lam_j = s->d.lam_j;
+    bs_status r = launch_tile(s, TILE_IMPLICIT, A, 1);
+    if (r) return r;
+    RelaxArgs R;
+    R.G = grid_of(s);
+    R.q = state_of(s, src);
+    for (int q = 0; q < 4; q++) {
+      R.res[q] = res.q[q];
+      R.dq[q] = dq.q[q];
+    }
+    R.diag = s->d.diag;
+    R.lam_i = s->d.lam_i;
+    R.lam_j = s->d.lam_j;
+    R.gamma = s->gamma32;
+    R.omega = (float)omega;
+    dim3 rgrid((s->ni / 2 + 1 + 63) / 64, (s->nj + 3) / 4);
+    for (int sw = 0; sw < sweeps; sw++) {
+      for (int colour = (sw == 0 ? 1 : 0); colour < 2; colour++) {
+        R.colour = colour;
+        hipEvent_t* ev = timing_begin(s, 2);
+        hipLaunchKernelGGL(relax_kernel, rgrid, dim3(64, 4), 0, s->stream, R);
+        timing_end(s, ev);
+      }
+    }
+    hipEvent_t* ev = timing_begin(s, 3);
+    hipLaunchKernelGGL(implicit_update_kernel, dim3((s->n + 255) / 256), dim3(256), 0, s->stream, state_of(s, src),
+                       state_of(s, dst), dq, s->n, s->gamma32, s->d.control, cfl_max, growth);
+    timing_end(s, ev);
+    HIP_TRY(hipGetLastError(), "implicit launch");
+    s->cur = dst;
+    return BS_OK;
+  };
+  StepGraph& g = s->implicit_graph;
+  if (g.exec && (g.sweeps != sweeps || g.omega != omega || g.growth != growth || g.cfl_max != cfl_max))
+    drop_graph(g);
   Control c;
   int remaining = count;
   while (remaining > 0) {
     const int batch = remaining < 64 ? remaining : 64;
-    for (int k = 0; k < batch; k++) {
-      const int src = s->cur, dst = (s->cur + 1) % 3;
-      TileArgs A = tile_args(s);
-      A.src = state_of(s, src);
-      A.omega = (float)omega;
-      for (int q = 0; q < 4; q++) {
-        A.res[q] = res.q[q];
-        A.dq[q] = dq.q[q];
-      }
-      A.diag = s->d.diag;
-      A.lam_i = s->d.lam_i;
-      A.lam_j = s->d.lam_j;
-      if ((st = launch_tile(s, TILE_IMPLICIT, A, 1))) return st;
-      RelaxArgs R;
-      R.G = grid_of(s);
-      R.q = state_of(s, src);
-      for (int q = 0; q < 4; q++) {
-        R.res[q] = res.q[q];
-        R.dq[q] = dq.q[q];
-      }
-      R.diag = s->d.diag;
-      R.lam_i = s->d.lam_i;
-      R.lam_j = s->d.lam_j;
-      R.gamma = s->gamma32;
-      R.omega = (float)omega;
-      dim3 rgrid((s->ni / 2 + 1 + 63) / 64, (s->nj + 3) / 4);
-      // The prepare kernel did the first red half-sweep.
-      for (int sw = 0; sw < sweeps; sw++) {
-        for (int colour = (sw == 0 ? 1 : 0); colour < 2; colour++) {
-          R.colour = colour;
-          hipEvent_t* ev = timing_begin(s, 2);
-          hipLaunchKernelGGL(relax_kernel, rgrid, dim3(64, 4), 0, s->stream, R);
-          timing_end(s, ev);
-        }
-      }
-      hipEvent_t* ev = timing_begin(s, 3);
-      hipLaunchKernelGGL(implicit_update_kernel, dim3((s->n + 255) / 256), dim3(256), 0, s->stream, state_of(s, src),
-                         state_of(s, dst), dq, s->n, s->gamma32, s->d.control, cfl_max, growth);
-      timing_end(s, ev);
-      HIP_TRY(hipGetLastError(), "implicit launch");
-      s->cur = dst;
-    }
+    if ((st = run_steps(s, g, batch, encode))) return st;
+    g.sweeps = sweeps;
+    g.omega = omega;
+    g.growth = growth;
+    g.cfl_max = cfl_max;
     if ((st = read_control(s, &c))) return st;
     fill_report(c, report);
     if ((st = account(s, c, batch))) return st;
@@ -961,6 +1044,16 @@ bs_status bs_iterate_implicit(bs_solver* s, int32_t count, double cfl_max, int32
   if (count == 0) {
     if ((st = read_control(s, &c))) return st;
     fill_report(c, report);
+  }
+  return BS_OK;
+}
+
+bs_status bs_enable_graphs(bs_solver* s, int32_t enable) {
+  SOLVER_GUARD(s);
+  s->graphs = enable != 0;
+  if (!s->graphs) {
+    drop_graph(s->explicit_graph);
+    drop_graph(s->implicit_graph);
   }
   return BS_OK;
 }

@@ -286,6 +286,10 @@ class GpuEuler2D:
         check(N.lib().bs_iterate_implicit(self._h, int(count), float(cfl), int(sweeps), float(omega), float(growth),
                                           C.byref(self.last_report)))
 
+    def enableGraphs(self, on: bool = True):
+        """Run steps as captured HIP graphs (default on)."""
+        check(N.lib().bs_enable_graphs(self._h, 1 if on else 0))
+
     def enableTiming(self, on: bool = True):
         check(N.lib().bs_enable_timing(self._h, 1 if on else 0))
 

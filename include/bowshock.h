@@ -183,6 +183,11 @@ bs_status bs_set_implicit_cfl(bs_solver *s, double cfl);
 bs_status bs_iterate_implicit(bs_solver *s, int32_t count, double cfl_max, int32_t sweeps,
                               double omega, double growth, bs_report *report);
 
+/* Steps run as captured HIP graphs of 12 steps each (one launch per graph
+ * instead of one per kernel), on by default; no counterpart in the
+ * reference, whose WebGPU engine encodes command buffers instead. */
+bs_status bs_enable_graphs(bs_solver *s, int32_t enable);
+
 /* Mean device time per launch of the solver's kernels since the last call,
  * for throughput reporting: kernel-name index (0 stage, 1 implicit prepare,
  * 2 implicit relax, 3 implicit update), milliseconds per launch and the
