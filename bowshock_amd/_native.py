@@ -60,6 +60,19 @@ SIGNATURES = {
     "bs_enable_graphs": (C.c_int, [_vp, C.c_int32]),
     "bs_enable_timing": (C.c_int, [_vp, C.c_int32]),
     "bs_kernel_timing": (C.c_int, [_vp, C.c_int32, _dp, C.POINTER(C.c_int64)]),
+    "bs_batch_create": (C.c_int, [_vp, C.POINTER(Config), C.c_int32, C.POINTER(_vp)]),
+    "bs_batch_destroy": (C.c_int, [_vp]),
+    "bs_batch_set_uniform": (C.c_int, [_vp, _dp]),
+    "bs_batch_set_primitives": (C.c_int, [_vp, C.c_int32, _dp]),
+    "bs_batch_read_state": (C.c_int, [_vp, C.c_int32, _fp, _fp, _fp, _fp]),
+    "bs_batch_set_active": (C.c_int, [_vp, C.POINTER(C.c_uint8)]),
+    "bs_batch_set_tiling": (C.c_int, [_vp, C.c_int32]),
+    "bs_batch_iterate": (C.c_int, [_vp, C.c_int32]),
+    "bs_batch_set_implicit_cfl": (C.c_int, [_vp, C.c_double]),
+    "bs_batch_iterate_implicit": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_int32, C.c_double, C.c_double]),
+    "bs_batch_residual_norms": (C.c_int, [_vp, _dp]),
+    "bs_batch_drags": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
+    "bs_batch_status": (C.c_int, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_int64)]),
 }
 
 _lib = None

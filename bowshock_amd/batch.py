@@ -1,0 +1,255 @@
+"""Batched solves over the HIP engine (bs_batch_* in include/bowshock.h).
+
+The reference solves its grids one after another, each through a
+GpuEuler2D (analysis/verified-drag.ts:239-300; the drivers of
+analysis/gpu-engine.ts:14-41 and analysis/gpu-transonic.ts:26-106).
+GpuBatch2D holds many cases of one scheme in one set of device arenas and
+advances every active case with one launch per stage; ``solve`` is the
+driver loop of verifiedForebodyDrag / solveTransonic (check the residual
+and the drag every chunk of steps, stop a case when it has converged)
+applied to all cases at once, retiring converged and failed cases from the
+launches.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import math
+import os
+import time
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _native as N
+from .engine import (Context, Euler2DConfig, GpuState2D, NonPhysicalStateError, RangeError, _c_config, check)
+
+STATUS_NON_PHYSICAL = 1
+
+
+class GpuBatch2D:
+    """Many GpuEuler2D cases advanced together. Every case shares the
+    scheme fields (geometry, gamma, flux, limiter, reconstruction,
+    integrator, cfl, local time stepping) and the boundary kinds per side;
+    grids and boundary states differ."""
+
+    def __init__(self, ctx: Context, configs: Sequence[Euler2DConfig]):
+        if len(configs) == 0:
+            raise RangeError("a batch needs at least one case")
+        keep: list = []
+        arr = (N.Config * len(configs))()
+        for k, cfg in enumerate(configs):
+            arr[k] = _c_config(cfg, keep)
+        h = C.c_void_p()
+        check(N.lib().bs_batch_create(ctx._h, arr, len(configs), C.byref(h)))
+        self._h = h
+        self.ctx = ctx
+        self.configs = list(configs)
+        self.count = len(configs)
+        self.shapes = [(c.grid.ni, c.grid.nj) for c in configs]
+        self.active = np.ones(self.count, dtype=np.uint8)
+        self.small_below = 0
+
+    def destroy(self):
+        if getattr(self, "_h", None):
+            N.lib().bs_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def setUniform(self, primitives):
+        """One (rho, u, v, p) per case (or one for all)."""
+        p = np.ascontiguousarray(np.broadcast_to(np.asarray(primitives, dtype=np.float64), (self.count, 4)))
+        check(N.lib().bs_batch_set_uniform(self._h, N.dptr(p)))
+
+    def setCellPrimitives(self, index: int, primitives):
+        ni, nj = self.shapes[index]
+        p = np.ascontiguousarray(np.broadcast_to(np.asarray(primitives, dtype=np.float64), (ni * nj, 4)))
+        check(N.lib().bs_batch_set_primitives(self._h, int(index), N.dptr(p)))
+
+    def readState(self, index: int) -> GpuState2D:
+        ni, nj = self.shapes[index]
+        out = [np.empty(ni * nj, dtype=np.float32) for _ in range(4)]
+        check(N.lib().bs_batch_read_state(self._h, int(index), *[N.fptr(a) for a in out]))
+        return GpuState2D(ni, nj, *out)
+
+    def setActive(self, mask):
+        m = np.ascontiguousarray(np.asarray(mask, dtype=np.uint8))
+        if m.shape != (self.count,):
+            raise RangeError("one flag per case")
+        check(N.lib().bs_batch_set_active(self._h, m.ctypes.data_as(C.POINTER(C.c_uint8))))
+        self.active = m.copy()
+
+    def setTiling(self, small_tile_below_cells: int):
+        check(N.lib().bs_batch_set_tiling(self._h, int(small_tile_below_cells)))
+        self.small_below = int(small_tile_below_cells)
+
+    def iterate(self, steps: int):
+        check(N.lib().bs_batch_iterate(self._h, int(steps)))
+
+    def setImplicitCfl(self, cfl: float):
+        check(N.lib().bs_batch_set_implicit_cfl(self._h, float(cfl)))
+
+    def iterateImplicit(self, steps: int, cfl: float, sweeps: int, omega: float = 1.5, growth: float = 1.2):
+        check(N.lib().bs_batch_iterate_implicit(self._h, int(steps), float(cfl), int(sweeps), float(omega),
+                                                float(growth)))
+
+    def densityResidualNorms(self) -> np.ndarray:
+        out = np.empty(self.count)
+        check(N.lib().bs_batch_residual_norms(self._h, N.dptr(out)))
+        return out
+
+    def forebodyDrags(self, pressure, dynamicPressure, baseRadius) -> np.ndarray:
+        args = [np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float64), (self.count,)))
+                for v in (pressure, dynamicPressure, baseRadius)]
+        out = np.empty(self.count)
+        check(N.lib().bs_batch_drags(self._h, *[N.dptr(a) for a in args], N.dptr(out)))
+        return out
+
+    def status(self):
+        st = np.empty(self.count, dtype=np.uint32)
+        steps = np.empty(self.count, dtype=np.int64)
+        check(N.lib().bs_batch_status(self._h, st.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                      steps.ctypes.data_as(C.POINTER(C.c_int64))))
+        return st, steps
+
+
+@dataclass
+class CaseResult:
+    cd: float = math.nan
+    residual: float = math.nan
+    steps: int = 0
+    converged: bool = False
+    reason: str = "running"
+    history: List[float] = field(default_factory=list)
+
+
+@dataclass
+class SolvePlan:
+    """When a case is done. The reference's drivers stop on a residual
+    drop or a settled drag (verified-drag.ts:258-300: batches of 500 r steps
+    until the drag changes by less than its tolerance; transonic-flow.ts:
+    residual target). A case converges when its density residual norm is at
+    most ``residual`` (absolute) or has fallen by ``drop`` from its first
+    reading, and its drag changed by at most ``cd_tol`` (relative) over the
+    last ``settle`` checks. It fails at ``max_steps`` or on a non-physical
+    state."""
+    implicit: bool = False
+    chunk: int = 120
+    max_steps: int = 20000
+    residual: float = 1e-6
+    drop: float = 1e-4
+    cd_tol: float = 1e-5
+    settle: int = 2
+    cfl: float = 10.0          # implicit: final CFL number
+    cfl_start: float = 1.0     # implicit: initial CFL number
+    sweeps: int = 2
+    omega: float = 1.5
+    growth: float = 1.2
+
+
+def solve(batch: GpuBatch2D, p_inf, q_inf, base_radius, plan: SolvePlan = SolvePlan()) -> List[CaseResult]:
+    """Run every case of the batch until it converges or fails; converged
+    and failed cases leave the launches at the next check."""
+    n = batch.count
+    res = [CaseResult() for _ in range(n)]
+    p_inf = np.broadcast_to(np.asarray(p_inf, dtype=np.float64), (n,))
+    q_inf = np.broadcast_to(np.asarray(q_inf, dtype=np.float64), (n,))
+    base_radius = np.broadcast_to(np.asarray(base_radius, dtype=np.float64), (n,))
+    active = np.ones(n, dtype=np.uint8)
+    batch.setActive(active)
+    first = np.full(n, math.nan)
+    cds: List[List[float]] = [[] for _ in range(n)]
+    if plan.implicit:
+        batch.setImplicitCfl(plan.cfl_start)
+    done = 0
+    while active.any():
+        if plan.implicit:
+            batch.iterateImplicit(plan.chunk, plan.cfl, plan.sweeps, plan.omega, plan.growth)
+        else:
+            batch.iterate(plan.chunk)
+        done += plan.chunk
+        status, steps = batch.status()
+        norms = batch.densityResidualNorms()
+        cd = batch.forebodyDrags(p_inf, q_inf, base_radius)
+        changed = False
+        for c in np.nonzero(active)[0]:
+            r = res[c]
+            r.steps = int(steps[c])
+            r.residual = float(norms[c])
+            r.cd = float(cd[c])
+            r.history.append(r.residual)
+            if status[c] & STATUS_NON_PHYSICAL or not math.isfinite(r.residual) or not math.isfinite(r.cd):
+                r.reason = "non-physical state"
+            else:
+                if math.isnan(first[c]):
+                    first[c] = r.residual
+                cds[c].append(r.cd)
+                settled = len(cds[c]) > plan.settle and all(
+                    abs(cds[c][-1] - v) <= plan.cd_tol * abs(cds[c][-1]) for v in cds[c][-1 - plan.settle:-1])
+                small = r.residual <= plan.residual or r.residual <= plan.drop * first[c]
+                if small and settled:
+                    r.converged = True
+                    r.reason = "converged"
+                elif done >= plan.max_steps:
+                    r.reason = "step limit"
+                else:
+                    continue
+            active[c] = 0
+            changed = True
+        if changed and active.any():
+            batch.setActive(active)
+    return res
+
+
+def _cache_path(path: Optional[str]) -> str:
+    if path:
+        return path
+    base = os.environ.get("XDG_CACHE_HOME") or os.path.join(os.path.expanduser("~"), ".cache")
+    return os.path.join(base, "bowshock_amd", "tiling.json")
+
+
+def autotune_tiling(batch: GpuBatch2D, steps: int = 24, candidates: Sequence[int] = (0, 2048, 8192, 32768),
+                    cache: Optional[str] = None, implicit: bool = False) -> int:
+    """Pick the cell count below which cases run on the small tile by
+    timing ``steps`` steps of the batch at each candidate, and cache the
+    choice by the batch's size histogram (a JSON file). The steps advance
+    the batch's state: tune on a throwaway start."""
+    shapes = sorted(batch.shapes)
+    hist = np.histogram([a * b for a, b in shapes], bins=[0, 1024, 2048, 4096, 8192, 16384, 32768, 65536, 1 << 40])[0]
+    key = f"{'implicit' if implicit else 'explicit'}:{','.join(map(str, hist.tolist()))}"
+    path = _cache_path(cache)
+    table = {}
+    if os.path.exists(path):
+        try:
+            with open(path) as f:
+                table = json.load(f)
+        except (OSError, ValueError):
+            table = {}
+    if key in table:
+        batch.setTiling(int(table[key]))
+        return int(table[key])
+    best, best_t = 0, math.inf
+    for cand in candidates:
+        batch.setTiling(cand)
+        run = (lambda k: batch.iterateImplicit(k, 10.0, 2)) if implicit else batch.iterate
+        run(12)  # capture the graphs outside the timing
+        t0 = time.perf_counter()
+        run(steps)
+        t = time.perf_counter() - t0
+        if t < best_t:
+            best, best_t = cand, t
+    batch.setTiling(best)
+    table[key] = best
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        with open(path, "w") as f:
+            json.dump(table, f, indent=1, sort_keys=True)
+    except OSError:
+        pass
+    return best

@@ -1,0 +1,827 @@
+// Batched solves: many grids of uneven size advanced together, one launch
+// per stage for every active case (bs_batch_* in include/bowshock.h).
+//
+// Layout: every per-cell and per-face array of all cases is one arena in
+// HBM, case c owning a contiguous range (its four state components
+// contiguous inside its range). The kernels are the single-grid stencil,
+// relaxation and update bodies; a block reads its work item (case, tile,
+// partial slot) and the case's argument record from device memory. Records
+// are role-major (a role being a stage of the scheme for a given current
+// state buffer), so a launch picks its role by the base pointer it passes.
+//
+// Cases are split into two size classes by cell count, with a large tile
+// (64 x 8) for big grids and a small one (32 x 4) for small grids where the
+// large tile would leave most of its threads idle; the threshold is set by
+// the caller's autotuner (bs_batch_set_tiling). Each step of the active set
+// is captured as HIP graphs; the work lists and graphs are rebuilt when
+// the active set or the tiling changes. Included by bs_engine.hip.
+
+namespace {
+
+constexpr int BTX_L = 64, BTY_L = 8;  // large tile
+constexpr int BTX_S = 32, BTY_S = 4;  // small tile
+
+struct BatchCase {
+  int ni, nj, n;
+  size_t cell, iface, jface, wall;  // arena offsets
+  int tiles_x[2], tiles_y[2];       // per tile class
+  int slot0, slots;                 // partial slots of its stencil launches
+  int wall_slot0, wall_blocks;
+  int cls;                          // tile class: 0 large, 1 small
+};
+
+enum BatchRole {
+  ROLE_STAGE0 = 0,   // + 3 * cur + k for stage k (9 roles)
+  ROLE_RESIDUAL = 9, // + cur
+  ROLE_PREPARE = 12, // + cur
+  N_TILE_ROLES = 15
+};
+
+template <int TX_, int TY_, int REC, int LIM, int FLUX>
+void* pick_batch_mode(int mode) {
+  if (mode == TILE_STAGE) return (void*)batch_tile_kernel<TX_, TY_, REC, LIM, FLUX, TILE_STAGE>;
+  if (mode == TILE_IMPLICIT) return (void*)batch_tile_kernel<TX_, TY_, REC, LIM, FLUX, TILE_IMPLICIT>;
+  return (void*)batch_tile_kernel<TX_, TY_, REC, LIM, FLUX, TILE_RESIDUAL>;
+}
+
+template <int TX_, int TY_, int FLUX>
+void* pick_batch_rec(int rec, int lim, int mode) {
+  if (rec == SHARED) return pick_batch_mode<TX_, TY_, SHARED, VAN_ALBADA, FLUX>(mode);
+  if (rec == SHARED_SCALED) return pick_batch_mode<TX_, TY_, SHARED_SCALED, VAN_ALBADA, FLUX>(mode);
+  switch (lim) {
+    case FIRST_ORDER: return pick_batch_mode<TX_, TY_, PER_VARIABLE, FIRST_ORDER, FLUX>(mode);
+    case MINMOD: return pick_batch_mode<TX_, TY_, PER_VARIABLE, MINMOD, FLUX>(mode);
+    case VAN_LEER: return pick_batch_mode<TX_, TY_, PER_VARIABLE, VAN_LEER, FLUX>(mode);
+    case VAN_ALBADA: return pick_batch_mode<TX_, TY_, PER_VARIABLE, VAN_ALBADA, FLUX>(mode);
+    case MONOTONIZED_CENTRAL: return pick_batch_mode<TX_, TY_, PER_VARIABLE, MONOTONIZED_CENTRAL, FLUX>(mode);
+    default: return pick_batch_mode<TX_, TY_, PER_VARIABLE, UNLIMITED_CENTRAL, FLUX>(mode);
+  }
+}
+
+using BatchTileFn = void (*)(const TileArgs*, const WorkItem*);
+
+BatchTileFn batch_tile_fn(const bs_config& c, int cls, int mode) {
+  void* f;
+  if (cls == 0)
+    f = c.flux == BS_FLUX_RUSANOV ? pick_batch_rec<BTX_L, BTY_L, RUSANOV>(c.reconstruction, c.limiter, mode)
+                                  : pick_batch_rec<BTX_L, BTY_L, HLLC>(c.reconstruction, c.limiter, mode);
+  else
+    f = c.flux == BS_FLUX_RUSANOV ? pick_batch_rec<BTX_S, BTY_S, RUSANOV>(c.reconstruction, c.limiter, mode)
+                                  : pick_batch_rec<BTX_S, BTY_S, HLLC>(c.reconstruction, c.limiter, mode);
+  return (BatchTileFn)f;
+}
+
+using BatchWallFn = void (*)(const WallArgs*, const WallBoundary*, const WorkItem*);
+
+template <int REC, int FLUX>
+__global__ __launch_bounds__(256) void batch_wall_kernel(const WallArgs* __restrict__ args,
+                                                         const WallBoundary* __restrict__ wb,
+                                                         const WorkItem* __restrict__ work) {
+  const WorkItem w = work[blockIdx.x];
+  wall_body<REC, FLUX>(args[w.args], wb[w.by], w.bx, w.slot);
+}
+
+BatchWallFn batch_wall_fn(const bs_config& c) {
+  if (c.flux == BS_FLUX_RUSANOV) {
+    if (c.reconstruction == SHARED) return batch_wall_kernel<SHARED, RUSANOV>;
+    if (c.reconstruction == SHARED_SCALED) return batch_wall_kernel<SHARED_SCALED, RUSANOV>;
+    return batch_wall_kernel<PER_VARIABLE, RUSANOV>;
+  }
+  if (c.reconstruction == SHARED) return batch_wall_kernel<SHARED, HLLC>;
+  if (c.reconstruction == SHARED_SCALED) return batch_wall_kernel<SHARED_SCALED, HLLC>;
+  return batch_wall_kernel<PER_VARIABLE, HLLC>;
+}
+
+// Sum of each case's range of partials, in a fixed order (one block per
+// case): out[c] = sum, or sqrt(sum / cells) when `rms`.
+__global__ __launch_bounds__(256) void segment_sums(const double* __restrict__ partials, const int* __restrict__ first,
+                                                    const int* __restrict__ count, const int* __restrict__ cells,
+                                                    const int* __restrict__ cases, int rms, double* __restrict__ out) {
+  __shared__ double red[4];
+  const int c = cases[blockIdx.x];
+  double v = 0.0;
+  for (int k = threadIdx.x; k < count[c]; k += 256) v += partials[first[c] + k];
+  double s = block_sum<256>(v, red);
+  if (threadIdx.x == 0) out[c] = rms ? sqrt(s / (double)cells[c]) : s;
+}
+
+template <typename T>
+T* dev_ptr(void* base, size_t offset_elems) {
+  return (T*)base + offset_elems;
+}
+
+}  // namespace
+
+struct bs_batch {
+  int device;
+  hipStream_t stream = nullptr;
+  bs_config scheme;
+  float gamma32;
+  int count;
+  std::vector<BatchCase> cases;
+  std::vector<bs_config> configs;
+  size_t cells = 0, ifaces = 0, jfaces = 0, walls = 0;
+  int slots = 0, wall_slots = 0;
+  // arenas
+  float* state[3] = {nullptr, nullptr, nullptr};
+  float *inx = nullptr, *iny = nullptr, *ia = nullptr, *jnx = nullptr, *jny = nullptr, *ja = nullptr;
+  float *vol = nullptr, *planar = nullptr, *dt = nullptr;
+  float *res = nullptr, *diag = nullptr, *lam_i = nullptr, *lam_j = nullptr, *dq = nullptr;
+  double* wall_geo = nullptr;
+  double* partials = nullptr;
+  double* wall_partials = nullptr;
+  double* sums = nullptr;
+  Control* control = nullptr;
+  BoundaryDev* bnd = nullptr;
+  WallBoundary* wall_bnd = nullptr;
+  // argument records and work lists (device)
+  TileArgs* tile_args = nullptr;     // N_TILE_ROLES x count
+  RelaxArgs* relax_args = nullptr;   // 3 (cur) x count
+  UpdateArgs* update_args = nullptr; // 3 (cur) x count
+  WallArgs* wall_args = nullptr;     // 3 (cur) x count
+  WorkItem* work_tiles[2] = {nullptr, nullptr};
+  WorkItem* work_relax = nullptr;
+  WorkItem* work_update = nullptr;
+  WorkItem* work_wall = nullptr;
+  int n_tiles[2] = {0, 0}, n_relax = 0, n_update = 0, n_wall = 0;
+  int *seg_first = nullptr, *seg_count = nullptr, *seg_cells = nullptr, *seg_cases = nullptr;
+  int *wall_first = nullptr, *wall_count = nullptr;
+  int n_seg = 0;
+  std::vector<char> active;
+  int small_below = 0;  // cases with fewer cells take the small tile
+  bool lists_dirty = true;
+  bool implicit_ready = false;
+  double upd_cfl_max = -1, upd_growth = -1;
+  int cur = 0;
+  StepGraph explicit_graph, implicit_graph;
+  std::atomic<bool> busy{false};
+};
+
+namespace {
+
+struct BatchGuard {
+  bs_batch* b;
+  bool ok;
+  explicit BatchGuard(bs_batch* batch) : b(batch) {
+    bool expected = false;
+    ok = b->busy.compare_exchange_strong(expected, true);
+  }
+  ~BatchGuard() {
+    if (ok) b->busy.store(false);
+  }
+};
+
+#define BATCH_GUARD(b)                                                                        \
+  if ((b) == nullptr) return fail(BS_ERR_RANGE, "no batch");                                  \
+  BatchGuard _bguard(b);                                                                      \
+  if (!_bguard.ok) return fail(BS_ERR_BUSY, "bs_batch: await the previous operation first"); \
+  (void)hipSetDevice((b)->device)
+
+StateDev batch_state(bs_batch* b, int buf, const BatchCase& k) {
+  StateDev s;
+  for (int q = 0; q < 4; q++) s.q[q] = b->state[buf] + 4 * k.cell + (size_t)q * k.n;
+  return s;
+}
+
+GridDev batch_grid(bs_batch* b, const BatchCase& k) {
+  GridDev G;
+  G.ni = k.ni;
+  G.nj = k.nj;
+  G.inx = b->inx + k.iface; G.iny = b->iny + k.iface; G.ia = b->ia + k.iface;
+  G.jnx = b->jnx + k.jface; G.jny = b->jny + k.jface; G.ja = b->ja + k.jface;
+  G.vol = b->vol + k.cell; G.planar = b->planar + k.cell;
+  return G;
+}
+
+void batch_free(bs_batch* b) {
+  drop_graph(b->explicit_graph);
+  drop_graph(b->implicit_graph);
+  void* all[] = {b->state[0], b->state[1], b->state[2], b->inx, b->iny, b->ia, b->jnx, b->jny, b->ja, b->vol,
+                 b->planar, b->dt, b->res, b->diag, b->lam_i, b->lam_j, b->dq, b->wall_geo, b->partials,
+                 b->wall_partials, b->sums, b->control, b->bnd, b->wall_bnd, b->tile_args, b->relax_args,
+                 b->update_args, b->wall_args, b->work_tiles[0], b->work_tiles[1], b->work_relax, b->work_update,
+                 b->work_wall, b->seg_first, b->seg_count, b->seg_cells, b->seg_cases, b->wall_first,
+                 b->wall_count};
+  for (void* p : all)
+    if (p) (void)hipFree(p);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  b->stream = nullptr;
+}
+
+// Argument records of every role for every case (uploaded once; the
+// implicit records when first needed).
+bs_status batch_upload_tile_args(bs_batch* b) {
+  const int n = b->count;
+  std::vector<TileArgs> recs((size_t)N_TILE_ROLES * n);
+  const float *ca, *cb;
+  const int ns = rk_table(b->scheme.integrator, &ca, &cb);
+  for (int c = 0; c < n; c++) {
+    const BatchCase& k = b->cases[c];
+    TileArgs base;
+    memset(&base, 0, sizeof base);
+    base.G = batch_grid(b, k);
+    base.B = b->bnd + c;
+    base.gamma = b->gamma32;
+    base.axisymmetric = b->scheme.geometry == BS_GEOMETRY_AXISYMMETRIC;
+    base.limiter = b->scheme.limiter;
+    base.cfl = (float)b->scheme.cfl;
+    base.dt = b->dt + k.cell;
+    base.control = b->control + c;
+    for (int cur = 0; cur < 3; cur++) {
+      const int other[2] = {(cur + 1) % 3, (cur + 2) % 3};
+      int prev = cur;
+      for (int s = 0; s < 3; s++) {
+        TileArgs A = base;
+        if (s < ns) {
+          const int dst = (s == ns - 1 && s > 0) ? cur : other[s % 2];
+          A.u0 = batch_state(b, cur, k);
+          A.src = batch_state(b, prev, k);
+          A.dst = batch_state(b, dst, k);
+          A.a = ca[s];
+          A.b = cb[s];
+          A.dt_mode = s == 0 ? DT_COMPUTE_LOCAL : DT_READ;
+          A.count_step = s == ns - 1;
+          prev = dst;
+        }
+        recs[(size_t)(ROLE_STAGE0 + 3 * cur + s) * n + c] = A;
+      }
+      TileArgs R = base;
+      R.src = batch_state(b, cur, k);
+      R.partials = b->partials;
+      recs[(size_t)(ROLE_RESIDUAL + cur) * n + c] = R;
+      TileArgs P = base;
+      P.src = batch_state(b, cur, k);
+      if (b->implicit_ready) {
+        for (int q = 0; q < 4; q++) {
+          P.res[q] = b->res + 4 * k.cell + (size_t)q * k.n;
+          P.dq[q] = b->dq + 4 * k.cell + (size_t)q * k.n;
+        }
+        P.diag = b->diag + k.cell;
+        P.lam_i = b->lam_i + k.iface;
+        P.lam_j = b->lam_j + k.jface;
+      }
+      recs[(size_t)(ROLE_PREPARE + cur) * n + c] = P;
+    }
+  }
+  HIP_TRY(hipMemcpy(b->tile_args, recs.data(), recs.size() * sizeof(TileArgs), hipMemcpyHostToDevice),
+          "batch records");
+  return BS_OK;
+}
+
+bs_status batch_upload_implicit_args(bs_batch* b, double omega, double cfl_max, double growth) {
+  const int n = b->count;
+  std::vector<RelaxArgs> rel((size_t)3 * n);
+  std::vector<UpdateArgs> upd((size_t)3 * n);
+  std::vector<TileArgs> prep((size_t)3 * n);
+  HIP_TRY(hipMemcpy(prep.data(), b->tile_args + (size_t)ROLE_PREPARE * n, prep.size() * sizeof(TileArgs),
+                    hipMemcpyDeviceToHost),
+          "batch records");
+  for (int c = 0; c < n; c++) {
+    const BatchCase& k = b->cases[c];
+    for (int cur = 0; cur < 3; cur++) {
+      RelaxArgs R;
+      R.G = batch_grid(b, k);
+      R.q = batch_state(b, cur, k);
+      for (int q = 0; q < 4; q++) {
+        R.res[q] = b->res + 4 * k.cell + (size_t)q * k.n;
+        R.dq[q] = b->dq + 4 * k.cell + (size_t)q * k.n;
+      }
+      R.diag = b->diag + k.cell;
+      R.lam_i = b->lam_i + k.iface;
+      R.lam_j = b->lam_j + k.jface;
+      R.gamma = b->gamma32;
+      R.omega = (float)omega;
+      R.colour = 0;
+      rel[(size_t)cur * n + c] = R;
+      UpdateArgs U;
+      U.src = batch_state(b, cur, k);
+      U.dst = batch_state(b, (cur + 1) % 3, k);
+      for (int q = 0; q < 4; q++) U.dq.q[q] = b->dq + 4 * k.cell + (size_t)q * k.n;
+      U.n = k.n;
+      U.gamma = b->gamma32;
+      U.control = b->control + c;
+      U.cfl_max = cfl_max;
+      U.growth = growth;
+      upd[(size_t)cur * n + c] = U;
+      prep[(size_t)cur * n + c].omega = (float)omega;
+    }
+  }
+  HIP_TRY(hipMemcpy(b->relax_args, rel.data(), rel.size() * sizeof(RelaxArgs), hipMemcpyHostToDevice), "records");
+  HIP_TRY(hipMemcpy(b->update_args, upd.data(), upd.size() * sizeof(UpdateArgs), hipMemcpyHostToDevice), "records");
+  HIP_TRY(hipMemcpy(b->tile_args + (size_t)ROLE_PREPARE * n, prep.data(), prep.size() * sizeof(TileArgs),
+                    hipMemcpyHostToDevice),
+          "records");
+  return BS_OK;
+}
+
+template <typename T>
+bs_status upload_vec(T** dst, const std::vector<T>& v, const char* what) {
+  if (*dst) {
+    (void)hipFree(*dst);
+    *dst = nullptr;
+  }
+  bs_status st = dev_alloc(dst, v.size() ? v.size() : 1, what);
+  if (st) return st;
+  if (!v.empty()) HIP_TRY(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), what);
+  return BS_OK;
+}
+
+// Work lists of the active cases: stencil tiles per class (ordered by case,
+// so neighbouring blocks share a grid), relaxation and update blocks, wall
+// blocks, and the segments of the per-case reductions.
+bs_status batch_build_lists(bs_batch* b) {
+  std::vector<WorkItem> tiles[2], relax, update, wall;
+  std::vector<int> seg_cases;
+  for (int c = 0; c < b->count; c++) {
+    if (!b->active[c]) continue;
+    BatchCase& k = b->cases[c];
+    k.cls = k.n < b->small_below ? 1 : 0;
+    seg_cases.push_back(c);
+    const int tx = k.tiles_x[k.cls], ty = k.tiles_y[k.cls];
+    for (int by = 0; by < ty; by++)
+      for (int bx = 0; bx < tx; bx++) tiles[k.cls].push_back({c, bx, by, k.slot0 + bx + by * tx});
+    const int rx = (k.ni / 2 + 1 + 63) / 64, ry = (k.nj + 3) / 4;
+    for (int by = 0; by < ry; by++)
+      for (int bx = 0; bx < rx; bx++) relax.push_back({c, bx, by, 0});
+    for (int blk = 0; blk < (k.n + 255) / 256; blk++) update.push_back({c, blk, 0, 0});
+    for (int blk = 0; blk < k.wall_blocks; blk++) wall.push_back({c, blk, c, k.wall_slot0 + blk});
+  }
+  // Slots and segments depend on the class of each case.
+  std::vector<int> first(b->count), cnt(b->count), cells(b->count), wfirst(b->count), wcnt(b->count);
+  for (int c = 0; c < b->count; c++) {
+    const BatchCase& k = b->cases[c];
+    first[c] = k.slot0;
+    cnt[c] = k.tiles_x[k.cls] * k.tiles_y[k.cls];
+    cells[c] = k.n;
+    wfirst[c] = k.wall_slot0;
+    wcnt[c] = k.wall_blocks;
+  }
+  bs_status st;
+  for (int cl = 0; cl < 2; cl++)
+    if ((st = upload_vec(&b->work_tiles[cl], tiles[cl], "work list"))) return st;
+  if ((st = upload_vec(&b->work_relax, relax, "work list")) || (st = upload_vec(&b->work_update, update, "work list")) ||
+      (st = upload_vec(&b->work_wall, wall, "work list")) || (st = upload_vec(&b->seg_first, first, "segments")) ||
+      (st = upload_vec(&b->seg_count, cnt, "segments")) || (st = upload_vec(&b->seg_cells, cells, "segments")) ||
+      (st = upload_vec(&b->seg_cases, seg_cases, "segments")) || (st = upload_vec(&b->wall_first, wfirst, "segments")) ||
+      (st = upload_vec(&b->wall_count, wcnt, "segments")))
+    return st;
+  b->n_tiles[0] = (int)tiles[0].size();
+  b->n_tiles[1] = (int)tiles[1].size();
+  b->n_relax = (int)relax.size();
+  b->n_update = (int)update.size();
+  b->n_wall = (int)wall.size();
+  b->n_seg = (int)seg_cases.size();
+  b->lists_dirty = false;
+  drop_graph(b->explicit_graph);
+  drop_graph(b->implicit_graph);
+  return BS_OK;
+}
+
+bs_status batch_launch_tiles(bs_batch* b, int mode, int role) {
+  const TileArgs* recs = b->tile_args + (size_t)role * b->count;
+  for (int cl = 0; cl < 2; cl++) {
+    if (b->n_tiles[cl] == 0) continue;
+    const int threads = cl == 0 ? BTX_L * BTY_L : BTX_S * BTY_S;
+    hipLaunchKernelGGL(batch_tile_fn(b->scheme, cl, mode), dim3(b->n_tiles[cl]), dim3(threads), 0, b->stream, recs,
+                       b->work_tiles[cl]);
+  }
+  HIP_TRY(hipGetLastError(), "batch stencil launch");
+  return BS_OK;
+}
+
+bs_status batch_encode_explicit(bs_batch* b) {
+  const float *ca, *cb;
+  const int ns = rk_table(b->scheme.integrator, &ca, &cb);
+  for (int s = 0; s < ns; s++) {
+    bs_status st = batch_launch_tiles(b, TILE_STAGE, ROLE_STAGE0 + 3 * b->cur + s);
+    if (st) return st;
+  }
+  if (ns == 1) b->cur = (b->cur + 1) % 3;
+  return BS_OK;
+}
+
+bs_status batch_encode_implicit(bs_batch* b, int sweeps) {
+  const int cur = b->cur;
+  bs_status st = batch_launch_tiles(b, TILE_IMPLICIT, ROLE_PREPARE + cur);
+  if (st) return st;
+  const RelaxArgs* rel = b->relax_args + (size_t)cur * b->count;
+  for (int sw = 0; sw < sweeps; sw++)
+    for (int colour = (sw == 0 ? 1 : 0); colour < 2; colour++)
+      if (b->n_relax)
+        hipLaunchKernelGGL(batch_relax_kernel, dim3(b->n_relax), dim3(64, 4), 0, b->stream, rel, b->work_relax, colour);
+  if (b->n_update)
+    hipLaunchKernelGGL(batch_update_kernel, dim3(b->n_update), dim3(256), 0, b->stream,
+                       b->update_args + (size_t)cur * b->count, b->work_update);
+  HIP_TRY(hipGetLastError(), "batch implicit launch");
+  b->cur = (cur + 1) % 3;
+  return BS_OK;
+}
+
+template <typename Encode>
+bs_status batch_run(bs_batch* b, StepGraph& g, int count, Encode encode) {
+  int k = 0;
+  if (count >= GRAPH_STEPS) {
+    if (!(g.exec && g.cur == b->cur)) {
+      drop_graph(g);
+      const int cur0 = b->cur;
+      HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+      bs_status st = BS_OK;
+      for (int s = 0; s < GRAPH_STEPS && st == BS_OK; s++) st = encode();
+      hipGraph_t graph = nullptr;
+      hipError_t e = hipStreamEndCapture(b->stream, &graph);
+      const int after = b->cur;
+      b->cur = cur0;
+      if (st != BS_OK || e != hipSuccess || after != cur0) {
+        if (graph) (void)hipGraphDestroy(graph);
+        if (st) return st;
+        if (e != hipSuccess) return hip_fail(e, "graph capture");
+        return fail(BS_ERR_EXECUTION, "captured steps do not return to their state buffer");
+      }
+      hipGraphExec_t exec = nullptr;
+      e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return hip_fail(e, "graph instantiate");
+      }
+      g.graph = graph;
+      g.exec = exec;
+      g.cur = cur0;
+    }
+    for (; k + GRAPH_STEPS <= count; k += GRAPH_STEPS) HIP_TRY(hipGraphLaunch(g.exec, b->stream), "graph launch");
+  }
+  for (; k < count; k++) {
+    bs_status st = encode();
+    if (st) return st;
+  }
+  HIP_TRY(hipStreamSynchronize(b->stream), "batch steps");
+  return BS_OK;
+}
+
+// Zero the status word and step count of cases [first, first + count),
+// keeping their implicit CFL number (a new start, as bs_set_primitives).
+bs_status batch_reset_control(bs_batch* b, int first, int count) {
+  std::vector<Control> ctl(count);
+  HIP_TRY(hipMemcpy(ctl.data(), b->control + first, count * sizeof(Control), hipMemcpyDeviceToHost), "control");
+  for (auto& c : ctl) {
+    const double cfl = c.cfl;
+    memset(&c, 0, sizeof c);
+    c.cfl = cfl > 0 ? cfl : 1.0;
+    c.t_end = 3e38;
+  }
+  HIP_TRY(hipMemcpy(b->control + first, ctl.data(), count * sizeof(Control), hipMemcpyHostToDevice), "control");
+  return BS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+bs_status bs_batch_create(bs_ctx* ctx, const bs_config* configs, int32_t count, bs_batch** out) {
+  if (!ctx || !configs || !out || count < 1) return fail(BS_ERR_RANGE, "a batch needs at least one case");
+  *out = nullptr;
+  const bs_config& s0 = configs[0];
+  for (int c = 0; c < count; c++) {
+    const bs_config& k = configs[c];
+    if (k.geometry != s0.geometry || k.gamma != s0.gamma || k.flux != s0.flux || k.limiter != s0.limiter ||
+        k.reconstruction != s0.reconstruction || k.integrator != s0.integrator || k.cfl != s0.cfl ||
+        k.time_stepping != s0.time_stepping)
+      return fail(BS_ERR_RANGE, "case %d: the cases of a batch share one scheme", c);
+    if (k.time_stepping != BS_STEPPING_LOCAL)
+      return fail(BS_ERR_RANGE, "batched solves use local time steps");
+    for (int sd = 0; sd < 4; sd++) {
+      if (k.boundaries[sd].kind == BS_BC_FACE_STATES)
+        return fail(BS_ERR_RANGE, "case %d: faceStates boundaries are not batched", c);
+      if (k.boundaries[sd].kind != s0.boundaries[sd].kind)
+        return fail(BS_ERR_RANGE, "case %d: the cases of a batch share their boundary kinds", c);
+    }
+    if (!(k.cfl > 0 && k.cfl <= 1)) return fail(BS_ERR_RANGE, "cfl must be in (0, 1]");
+    if (k.ni < 2 || k.nj < 2 || !k.x || !k.y) return fail(BS_ERR_RANGE, "case %d: the grid needs 2 x 2 cells", c);
+    if ((k.reconstruction == BS_RECON_SHARED || k.reconstruction == BS_RECON_SHARED_SCALED) &&
+        k.limiter != BS_LIMITER_VAN_ALBADA)
+      return fail(BS_ERR_RANGE, "shared reconstruction requires the 'vanAlbada' limiter");
+    if (k.flux != BS_FLUX_HLLC && k.flux != BS_FLUX_RUSANOV) return fail(BS_ERR_RANGE, "flux not available");
+  }
+  bs_batch* b = new bs_batch();
+  b->device = ctx->device;
+  b->scheme = s0;
+  b->scheme.x = b->scheme.y = nullptr;
+  b->gamma32 = (float)s0.gamma;
+  b->count = count;
+  b->cases.resize(count);
+  b->active.assign(count, 1);
+  (void)hipSetDevice(b->device);
+  auto bail = [&](bs_status code) {
+    batch_free(b);
+    delete b;
+    return code;
+  };
+  // Offsets.
+  for (int c = 0; c < count; c++) {
+    BatchCase& k = b->cases[c];
+    k.ni = configs[c].ni;
+    k.nj = configs[c].nj;
+    k.n = k.ni * k.nj;
+    k.cell = b->cells;
+    k.iface = b->ifaces;
+    k.jface = b->jfaces;
+    k.wall = b->walls;
+    b->cells += k.n;
+    b->ifaces += (size_t)(k.ni + 1) * k.nj;
+    b->jfaces += (size_t)k.ni * (k.nj + 1);
+    b->walls += k.ni;
+    k.tiles_x[0] = (k.ni + BTX_L - 1) / BTX_L;
+    k.tiles_y[0] = (k.nj + BTY_L - 1) / BTY_L;
+    k.tiles_x[1] = (k.ni + BTX_S - 1) / BTX_S;
+    k.tiles_y[1] = (k.nj + BTY_S - 1) / BTY_S;
+    k.slot0 = b->slots;
+    const int t0 = k.tiles_x[0] * k.tiles_y[0], t1 = k.tiles_x[1] * k.tiles_y[1];
+    k.slots = t0 > t1 ? t0 : t1;
+    b->slots += k.slots;
+    k.wall_slot0 = b->wall_slots;
+    k.wall_blocks = (k.ni + 255) / 256;
+    b->wall_slots += k.wall_blocks;
+    k.cls = 0;
+  }
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(BS_ERR_EXECUTION, "hipStreamCreate failed"));
+  bs_status st;
+  for (int q = 0; q < 3; q++)
+    if ((st = dev_alloc(&b->state[q], 4 * b->cells, "batch state"))) return bail(st);
+  if ((st = dev_alloc(&b->inx, b->ifaces, "metrics")) || (st = dev_alloc(&b->iny, b->ifaces, "metrics")) ||
+      (st = dev_alloc(&b->ia, b->ifaces, "metrics")) || (st = dev_alloc(&b->jnx, b->jfaces, "metrics")) ||
+      (st = dev_alloc(&b->jny, b->jfaces, "metrics")) || (st = dev_alloc(&b->ja, b->jfaces, "metrics")) ||
+      (st = dev_alloc(&b->vol, b->cells, "metrics")) || (st = dev_alloc(&b->planar, b->cells, "metrics")) ||
+      (st = dev_alloc(&b->dt, b->cells, "time steps")) || (st = dev_alloc(&b->wall_geo, 5 * b->walls, "wall")) ||
+      (st = dev_alloc(&b->partials, (size_t)b->slots, "partials")) ||
+      (st = dev_alloc(&b->wall_partials, (size_t)b->wall_slots, "partials")) ||
+      (st = dev_alloc(&b->sums, (size_t)count, "sums")) || (st = dev_alloc(&b->control, (size_t)count, "control")) ||
+      (st = dev_alloc(&b->bnd, (size_t)count, "boundaries")) ||
+      (st = dev_alloc(&b->wall_bnd, (size_t)count, "boundaries")) ||
+      (st = dev_alloc(&b->tile_args, (size_t)N_TILE_ROLES * count, "records")) ||
+      (st = dev_alloc(&b->relax_args, (size_t)3 * count, "records")) ||
+      (st = dev_alloc(&b->update_args, (size_t)3 * count, "records")) ||
+      (st = dev_alloc(&b->wall_args, (size_t)3 * count, "records")))
+    return bail(st);
+  // Metrics per case (FP64 on the host), boundary tables, controls.
+  std::vector<BoundaryDev> bnds(count);
+  std::vector<WallBoundary> wbs(count);
+  std::vector<Control> ctl(count);
+  std::vector<WallArgs> wargs((size_t)3 * count);
+  for (int c = 0; c < count; c++) {
+    const bs_config& cf = configs[c];
+    const BatchCase& k = b->cases[c];
+    const int ni = k.ni, nj = k.nj;
+    const size_t fi = (size_t)(ni + 1) * nj, fj = (size_t)ni * (nj + 1);
+    std::vector<double> inx(fi), iny(fi), ia(fi), jnx(fj), jny(fj), ja(fj), vol(k.n), planar(k.n);
+    st = metrics(ni, nj, cf.x, cf.y, cf.geometry == BS_GEOMETRY_AXISYMMETRIC, inx.data(), iny.data(), ia.data(),
+                 jnx.data(), jny.data(), ja.data(), vol.data(), planar.data(), nullptr, nullptr);
+    if (st) return bail(fail(st, "case %d: %s", c, bs_last_error()));
+    auto put = [&](float* base, size_t off, const std::vector<double>& v) -> bs_status {
+      std::vector<float> f(v.begin(), v.end());
+      HIP_TRY(hipMemcpyAsync(base + off, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice, b->stream),
+              "metrics upload");
+      HIP_TRY(hipStreamSynchronize(b->stream), "metrics upload");
+      return BS_OK;
+    };
+    if ((st = put(b->inx, k.iface, inx)) || (st = put(b->iny, k.iface, iny)) || (st = put(b->ia, k.iface, ia)) ||
+        (st = put(b->jnx, k.jface, jnx)) || (st = put(b->jny, k.jface, jny)) || (st = put(b->ja, k.jface, ja)) ||
+        (st = put(b->vol, k.cell, vol)) || (st = put(b->planar, k.cell, planar)))
+      return bail(st);
+    // The jMin side's FP64 face geometry (the body wall).
+    std::vector<double> geo(5 * (size_t)ni);
+    for (int t = 0; t < ni; t++) {
+      const int n0 = t, n1 = t + 1;
+      geo[5 * t + 0] = 0.5 * (cf.x[n0] + cf.x[n1]);
+      geo[5 * t + 1] = 0.5 * (cf.y[n0] + cf.y[n1]);
+      geo[5 * t + 2] = jnx[t];
+      geo[5 * t + 3] = jny[t];
+      geo[5 * t + 4] = ja[t];
+    }
+    HIP_TRY(hipMemcpy(b->wall_geo + 5 * k.wall, geo.data(), geo.size() * sizeof(double), hipMemcpyHostToDevice),
+            "wall geometry");
+    BoundaryDev& B = bnds[c];
+    WallBoundary& W = wbs[c];
+    for (int sd = 0; sd < 4; sd++) {
+      const bs_boundary& bb = cf.boundaries[sd];
+      B.kind[sd] = bb.kind;
+      B.state[sd][0] = (float)bb.rho; B.state[sd][1] = (float)bb.u;
+      B.state[sd][2] = (float)bb.v; B.state[sd][3] = (float)bb.p;
+      B.face_cons[sd] = nullptr;
+      B.face_flux[sd] = nullptr;
+      W.kind[sd] = bb.kind;
+      W.state[sd][0] = bb.rho; W.state[sd][1] = bb.u; W.state[sd][2] = bb.v; W.state[sd][3] = bb.p;
+      W.cons[sd] = nullptr;
+    }
+    memset(&ctl[c], 0, sizeof(Control));
+    ctl[c].cfl = 1.0;
+    ctl[c].t_end = 3e38;
+    for (int cur = 0; cur < 3; cur++) {
+      WallArgs& A = wargs[(size_t)cur * count + c];
+      memset(&A, 0, sizeof A);
+      A.G = batch_grid(b, k);
+      A.s = batch_state(b, cur, k);
+      A.side = JMIN;
+      A.count = ni;
+      A.kind = cf.boundaries[JMIN].kind;
+      A.face_geo = b->wall_geo + 5 * k.wall;
+      A.gamma = cf.gamma;
+      A.gamma32 = (double)b->gamma32;
+      A.rec = cf.reconstruction;
+      A.limiter = cf.limiter;
+      A.flux = cf.flux;
+      A.partials = b->wall_partials;
+    }
+  }
+  HIP_TRY(hipMemcpy(b->bnd, bnds.data(), bnds.size() * sizeof(BoundaryDev), hipMemcpyHostToDevice), "boundaries");
+  HIP_TRY(hipMemcpy(b->wall_bnd, wbs.data(), wbs.size() * sizeof(WallBoundary), hipMemcpyHostToDevice), "boundaries");
+  HIP_TRY(hipMemcpy(b->control, ctl.data(), ctl.size() * sizeof(Control), hipMemcpyHostToDevice), "control");
+  HIP_TRY(hipMemcpy(b->wall_args, wargs.data(), wargs.size() * sizeof(WallArgs), hipMemcpyHostToDevice), "records");
+  if ((st = batch_upload_tile_args(b))) return bail(st);
+  *out = b;
+  return BS_OK;
+}
+
+bs_status bs_batch_destroy(bs_batch* b) {
+  if (!b) return BS_OK;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_free(b);
+  delete b;
+  return BS_OK;
+}
+
+bs_status bs_batch_set_uniform(bs_batch* b, const double* states) {
+  BATCH_GUARD(b);
+  if (!states) return fail(BS_ERR_RANGE, "no states");
+  for (int c = 0; c < b->count; c++) {
+    const BatchCase& k = b->cases[c];
+    const double* w = states + 4 * (size_t)c;
+    if (!physical(w)) return fail(BS_ERR_NONPHYSICAL, "non-physical initial state of case %d", c);
+    float q[4];
+    pack_cons(w, (double)b->gamma32, q);
+    std::vector<float> v(4 * (size_t)k.n);
+    for (int qq = 0; qq < 4; qq++) std::fill(v.begin() + (size_t)qq * k.n, v.begin() + (size_t)(qq + 1) * k.n, q[qq]);
+    HIP_TRY(hipMemcpy(b->state[b->cur] + 4 * k.cell, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice),
+            "state upload");
+  }
+  return batch_reset_control(b, 0, b->count);
+}
+
+bs_status bs_batch_set_primitives(bs_batch* b, int32_t index, const double* prim) {
+  BATCH_GUARD(b);
+  if (index < 0 || index >= b->count || !prim) return fail(BS_ERR_RANGE, "no such case");
+  const BatchCase& k = b->cases[index];
+  std::vector<float> v(4 * (size_t)k.n);
+  for (int c = 0; c < k.n; c++) {
+    const double* w = prim + 4 * (size_t)c;
+    if (!physical(w)) return fail(BS_ERR_NONPHYSICAL, "non-physical state in cell %d of case %d", c, index);
+    float q[4];
+    pack_cons(w, (double)b->gamma32, q);
+    for (int qq = 0; qq < 4; qq++) v[(size_t)qq * k.n + c] = q[qq];
+  }
+  HIP_TRY(hipMemcpy(b->state[b->cur] + 4 * k.cell, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice),
+          "state upload");
+  return batch_reset_control(b, index, 1);
+}
+
+bs_status bs_batch_read_state(bs_batch* b, int32_t index, float* rho, float* mu, float* mv, float* energy) {
+  BATCH_GUARD(b);
+  if (index < 0 || index >= b->count) return fail(BS_ERR_RANGE, "no such case");
+  const BatchCase& k = b->cases[index];
+  float* dst[4] = {rho, mu, mv, energy};
+  for (int q = 0; q < 4; q++)
+    if (dst[q])
+      HIP_TRY(hipMemcpy(dst[q], b->state[b->cur] + 4 * k.cell + (size_t)q * k.n, k.n * sizeof(float),
+                        hipMemcpyDeviceToHost),
+              "state readback");
+  return BS_OK;
+}
+
+bs_status bs_batch_set_active(bs_batch* b, const uint8_t* active) {
+  BATCH_GUARD(b);
+  if (!active) return fail(BS_ERR_RANGE, "no mask");
+  bool changed = false;
+  for (int c = 0; c < b->count; c++) {
+    const char a = active[c] ? 1 : 0;
+    changed |= a != b->active[c];
+    b->active[c] = a;
+  }
+  if (changed) b->lists_dirty = true;
+  return BS_OK;
+}
+
+bs_status bs_batch_set_tiling(bs_batch* b, int32_t small_tile_below_cells) {
+  BATCH_GUARD(b);
+  if (small_tile_below_cells != b->small_below) b->lists_dirty = true;
+  b->small_below = small_tile_below_cells < 0 ? 0 : small_tile_below_cells;
+  return BS_OK;
+}
+
+bs_status bs_batch_iterate(bs_batch* b, int32_t steps) {
+  BATCH_GUARD(b);
+  if (steps < 0) return fail(BS_ERR_RANGE, "the step count cannot be negative");
+  bs_status st;
+  if (b->lists_dirty && (st = batch_build_lists(b))) return st;
+  return batch_run(b, b->explicit_graph, steps, [b]() { return batch_encode_explicit(b); });
+}
+
+bs_status bs_batch_set_implicit_cfl(bs_batch* b, double cfl) {
+  BATCH_GUARD(b);
+  if (!(cfl > 0)) return fail(BS_ERR_RANGE, "the CFL number must be positive");
+  std::vector<Control> ctl(b->count);
+  HIP_TRY(hipMemcpy(ctl.data(), b->control, ctl.size() * sizeof(Control), hipMemcpyDeviceToHost), "control");
+  for (auto& c : ctl) c.cfl = cfl;
+  HIP_TRY(hipMemcpy(b->control, ctl.data(), ctl.size() * sizeof(Control), hipMemcpyHostToDevice), "control");
+  return BS_OK;
+}
+
+bs_status bs_batch_iterate_implicit(bs_batch* b, int32_t steps, double cfl_max, int32_t sweeps, double omega,
+                                    double growth) {
+  BATCH_GUARD(b);
+  if (!(cfl_max > 0 && omega >= 1 && growth >= 1))
+    return fail(BS_ERR_RANGE, "implicit steps need cfl > 0, omega >= 1 and growth >= 1");
+  if (sweeps < 1) return fail(BS_ERR_RANGE, "the relaxation needs a whole number of sweeps, at least one");
+  bs_status st;
+  if (!b->implicit_ready) {
+    if ((st = dev_alloc(&b->res, 4 * b->cells, "implicit residual")) ||
+        (st = dev_alloc(&b->diag, b->cells, "implicit diagonal")) ||
+        (st = dev_alloc(&b->lam_i, b->ifaces, "lambda")) || (st = dev_alloc(&b->lam_j, b->jfaces, "lambda")) ||
+        (st = dev_alloc(&b->dq, 4 * b->cells, "implicit dq")))
+      return st;
+    b->implicit_ready = true;
+    if ((st = batch_upload_tile_args(b))) return st;
+    b->upd_cfl_max = -1;
+  }
+  StepGraph& g = b->implicit_graph;
+  if (b->upd_cfl_max != cfl_max || b->upd_growth != growth || g.omega != omega || g.sweeps != sweeps) {
+    if ((st = batch_upload_implicit_args(b, omega, cfl_max, growth))) return st;
+    b->upd_cfl_max = cfl_max;
+    b->upd_growth = growth;
+    drop_graph(g);
+  }
+  if (b->lists_dirty && (st = batch_build_lists(b))) return st;
+  st = batch_run(b, g, steps, [b, sweeps]() { return batch_encode_implicit(b, sweeps); });
+  g.sweeps = sweeps;
+  g.omega = omega;
+  return st;
+}
+
+bs_status bs_batch_residual_norms(bs_batch* b, double* norms) {
+  BATCH_GUARD(b);
+  if (!norms) return fail(BS_ERR_RANGE, "no output");
+  bs_status st;
+  if (b->lists_dirty && (st = batch_build_lists(b))) return st;
+  if ((st = batch_launch_tiles(b, TILE_RESIDUAL, ROLE_RESIDUAL + b->cur))) return st;
+  if (b->n_seg)
+    hipLaunchKernelGGL(segment_sums, dim3(b->n_seg), dim3(256), 0, b->stream, b->partials, b->seg_first, b->seg_count,
+                       b->seg_cells, b->seg_cases, 1, b->sums);
+  std::vector<double> all(b->count);
+  HIP_TRY(hipMemcpyAsync(all.data(), b->sums, all.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream),
+          "norm readback");
+  HIP_TRY(hipStreamSynchronize(b->stream), "sync");
+  for (int c = 0; c < b->count; c++) norms[c] = b->active[c] ? all[c] : NAN;
+  return BS_OK;
+}
+
+bs_status bs_batch_drags(bs_batch* b, const double* p_inf, const double* q_inf, const double* base_radius,
+                         double* cd) {
+  BATCH_GUARD(b);
+  if (!p_inf || !q_inf || !base_radius || !cd) return fail(BS_ERR_RANGE, "null argument");
+  bs_status st;
+  if (b->lists_dirty && (st = batch_build_lists(b))) return st;
+  // p_inf enters the wall kernel per case: set it in the records.
+  std::vector<WallArgs> wargs((size_t)b->count);
+  HIP_TRY(hipMemcpy(wargs.data(), b->wall_args + (size_t)b->cur * b->count, wargs.size() * sizeof(WallArgs),
+                    hipMemcpyDeviceToHost),
+          "records");
+  for (int c = 0; c < b->count; c++) wargs[c].p_inf = p_inf[c];
+  HIP_TRY(hipMemcpy(b->wall_args + (size_t)b->cur * b->count, wargs.data(), wargs.size() * sizeof(WallArgs),
+                    hipMemcpyHostToDevice),
+          "records");
+  if (b->n_wall)
+    hipLaunchKernelGGL(batch_wall_fn(b->scheme), dim3(b->n_wall), dim3(256), 0, b->stream,
+                       b->wall_args + (size_t)b->cur * b->count, b->wall_bnd, b->work_wall);
+  if (b->n_seg)
+    hipLaunchKernelGGL(segment_sums, dim3(b->n_seg), dim3(256), 0, b->stream, b->wall_partials, b->wall_first,
+                       b->wall_count, b->seg_cells, b->seg_cases, 0, b->sums);
+  std::vector<double> force(b->count);
+  HIP_TRY(hipMemcpyAsync(force.data(), b->sums, force.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream),
+          "drag readback");
+  HIP_TRY(hipStreamSynchronize(b->stream), "sync");
+  for (int c = 0; c < b->count; c++)
+    cd[c] = b->active[c] ? (2.0 * force[c]) / (q_inf[c] * base_radius[c] * base_radius[c]) : NAN;
+  return BS_OK;
+}
+
+bs_status bs_batch_status(bs_batch* b, uint32_t* status, int64_t* steps) {
+  BATCH_GUARD(b);
+  std::vector<Control> ctl(b->count);
+  HIP_TRY(hipMemcpy(ctl.data(), b->control, ctl.size() * sizeof(Control), hipMemcpyDeviceToHost), "control");
+  for (int c = 0; c < b->count; c++) {
+    if (status) status[c] = ctl[c].status;
+    if (steps) steps[c] = ctl[c].steps;
+  }
+  return BS_OK;
+}
+
+}  // extern "C"

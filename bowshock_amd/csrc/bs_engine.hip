@@ -1017,8 +1017,16 @@ bs_status bs_iterate_implicit(bs_solver* s, int32_t count, double cfl_max, int32
       }
     }
     hipEvent_t* ev = timing_begin(s, 3);
-    hipLaunchKernelGGL(implicit_update_kernel, dim3((s->n + 255) / 256), dim3(256), 0, s->stream, state_of(s, src),
-                       state_of(s, dst), dq, s->n, s->gamma32, s->d.control, cfl_max, growth);
+    UpdateArgs U;
+    U.src = state_of(s, src);
+    U.dst = state_of(s, dst);
+    U.dq = dq;
+    U.n = s->n;
+    U.gamma = s->gamma32;
+    U.control = s->d.control;
+    U.cfl_max = cfl_max;
+    U.growth = growth;
+    hipLaunchKernelGGL(implicit_update_kernel, dim3((s->n + 255) / 256), dim3(256), 0, s->stream, U);
     timing_end(s, ev);
     HIP_TRY(hipGetLastError(), "implicit launch");
     s->cur = dst;
@@ -1080,3 +1088,5 @@ bs_status bs_kernel_timing(bs_solver* s, int32_t kernel, double* ms_per_launch, 
 }
 
 }  // extern "C"
+
+#include "bs_batch.hpp"

@@ -159,8 +159,9 @@ __device__ __forceinline__ double block_sum(double v, double* scratch) {
   return s;
 }
 
+// The tile (bx, by) of one grid; its FP64 partial goes to A.partials[slot].
 template <int TX, int TY, int REC, int LIM, int FLUX, int MODE>
-__global__ __launch_bounds__(TX * TY) void tile_kernel(TileArgs A) {
+__device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int slot) {
   constexpr int NT = TX * TY;
   constexpr int WX = TX + 4, WY = TY + 4;
   __shared__ float sp[4][WY][WX];        // primitives of the tile and its halo
@@ -173,7 +174,7 @@ __global__ __launch_bounds__(TX * TY) void tile_kernel(TileArgs A) {
   const GridDev& G = A.G;
   const int ni = G.ni, nj = G.nj;
   const float g = A.gamma;
-  const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+  const int i0 = bx * TX, j0 = by * TY;
   const int tid = threadIdx.x;
 
   // Phase 1: primitives of the window [i0-2, i0+TX+2) x [j0-2, j0+TY+2),
@@ -392,9 +393,30 @@ __global__ __launch_bounds__(TX * TY) void tile_kernel(TileArgs A) {
   }
   if (A.partials != nullptr) {
     double s = block_sum<NT>(r2, red);
-    if (tid == 0) A.partials[blockIdx.x + blockIdx.y * gridDim.x] = s;
+    if (tid == 0) A.partials[slot] = s;
   }
-  if (A.count_step && blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) A.control->steps += 1;
+  if (A.count_step && bx == 0 && by == 0 && tid == 0) A.control->steps += 1;
+}
+
+template <int TX, int TY, int REC, int LIM, int FLUX, int MODE>
+__global__ __launch_bounds__(TX * TY) void tile_kernel(TileArgs A) {
+  tile_body<TX, TY, REC, LIM, FLUX, MODE>(A, blockIdx.x, blockIdx.y, blockIdx.x + blockIdx.y * gridDim.x);
+}
+
+// A batch of grids in one launch: block b takes work item b (the case's
+// argument record, the tile and the partial slot); the argument records
+// live in device memory, one per case and stage role.
+struct WorkItem {
+  int args;  // index of the case's TileArgs
+  int bx, by;
+  int slot;  // FP64 partial slot
+};
+
+template <int TX, int TY, int REC, int LIM, int FLUX, int MODE>
+__global__ __launch_bounds__(TX * TY) void batch_tile_kernel(const TileArgs* __restrict__ args,
+                                                             const WorkItem* __restrict__ work) {
+  const WorkItem w = work[blockIdx.x];
+  tile_body<TX, TY, REC, LIM, FLUX, MODE>(args[w.args], w.bx, w.by, w.slot);
 }
 
 // Deterministic FP64 sum of n partials by one block; writes sqrt(sum / cells)
@@ -515,11 +537,12 @@ __device__ __forceinline__ void neighbour_term(const RelaxArgs& A, int nb, float
 }
 
 // One half-sweep over the cells with (i + j) % 2 == colour; thread x of a
-// row handles the x-th cell of that colour in the row.
-__global__ __launch_bounds__(256) void relax_kernel(RelaxArgs A) {
+// row handles the x-th cell of that colour in the row; a block covers 64
+// cells of the colour along i by 4 rows.
+__device__ __forceinline__ void relax_body(const RelaxArgs& A, int bx, int by, int colour) {
   const int ni = A.G.ni, nj = A.G.nj;
-  const int j = blockIdx.y * 4 + threadIdx.y;
-  const int i = 2 * (blockIdx.x * 64 + threadIdx.x) + ((j + A.colour) & 1);
+  const int j = by * 4 + threadIdx.y;
+  const int i = 2 * (bx * 64 + threadIdx.x) + ((j + colour) & 1);
   if (i >= ni || j >= nj) return;
   const GridDev& G = A.G;
   const int c = i + j * ni;
@@ -542,12 +565,34 @@ __global__ __launch_bounds__(256) void relax_kernel(RelaxArgs A) {
   for (int q = 0; q < 4; q++) A.dq[q][c] = (A.res[q][c] - sum[q]) * id;
 }
 
+__global__ __launch_bounds__(256) void relax_kernel(RelaxArgs A) { relax_body(A, blockIdx.x, blockIdx.y, A.colour); }
+
+__global__ __launch_bounds__(256) void batch_relax_kernel(const RelaxArgs* __restrict__ args,
+                                                          const WorkItem* __restrict__ work, int colour) {
+  const WorkItem w = work[blockIdx.x];
+  relax_body(args[w.args], w.bx, w.by, colour);
+}
+
+struct UpdateArgs {
+  StateDev src, dst, dq;
+  int n;
+  float gamma;
+  Control* control;
+  double cfl_max, growth;
+};
+
 // q + alpha dQ, alpha halved (at most 30 times) until density and pressure
 // stay within a factor of two (euler2d.ts:816-847); the CFL ramp and the
-// step count by one thread.
-__global__ __launch_bounds__(256) void implicit_update_kernel(StateDev src, StateDev dst, StateDev dq, int n,
-                                                              float g, Control* ctl, double cfl_max, double growth) {
-  const int c = blockIdx.x * 256 + threadIdx.x;
+// step count by one thread. Block `block` covers cells 256 block ...
+__device__ __forceinline__ void update_body(const UpdateArgs& U, int block) {
+  const StateDev& src = U.src;
+  const StateDev& dst = U.dst;
+  const StateDev& dq = U.dq;
+  const float g = U.gamma;
+  Control* ctl = U.control;
+  const int n = U.n;
+  const double cfl_max = U.cfl_max, growth = U.growth;
+  const int c = block * 256 + threadIdx.x;
   if (c < n) {
     const float rho = src.q[0][c], mx = src.q[1][c], my = src.q[2][c], e = src.q[3][c];
     const float d0 = dq.q[0][c], d1 = dq.q[1][c], d2 = dq.q[2][c], d3 = dq.q[3][c];
@@ -566,11 +611,19 @@ __global__ __launch_bounds__(256) void implicit_update_kernel(StateDev src, Stat
     dst.q[3][c] = o.w;
     if (!is_physical(o, g)) atomicOr(&ctl->status, STATUS_NON_PHYSICAL);
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (block == 0 && threadIdx.x == 0) {
     double next = ctl->cfl * growth;
     ctl->cfl = next < cfl_max ? next : cfl_max;
     ctl->steps += 1;
   }
+}
+
+__global__ __launch_bounds__(256) void implicit_update_kernel(UpdateArgs U) { update_body(U, blockIdx.x); }
+
+__global__ __launch_bounds__(256) void batch_update_kernel(const UpdateArgs* __restrict__ args,
+                                                           const WorkItem* __restrict__ work) {
+  const WorkItem w = work[blockIdx.x];
+  update_body(args[w.args], w.bx);
 }
 
 // ---------------------------------------------------------------------------
@@ -654,10 +707,12 @@ struct WallBoundary {
   const float* cons[4];
 };
 
+// Block `block` evaluates faces 256 block ... of the side; its drag partial
+// goes to partials[slot].
 template <int REC, int FLUX>
-__global__ __launch_bounds__(256) void wall_flux_kernel(WallArgs A, WallBoundary WB) {
+__device__ __forceinline__ void wall_body(const WallArgs& A, const WallBoundary& WB, int block, int slot) {
   __shared__ double red[4];
-  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int t = block * 256 + threadIdx.x;
   double contrib = 0.0;
   if (t < A.count) {
     const int ni = A.G.ni, nj = A.G.nj;
@@ -682,8 +737,13 @@ __global__ __launch_bounds__(256) void wall_flux_kernel(WallArgs A, WallBoundary
   }
   if (A.partials != nullptr) {
     double s = block_sum<256>(contrib, red);
-    if (threadIdx.x == 0) A.partials[blockIdx.x] = s;
+    if (threadIdx.x == 0) A.partials[slot] = s;
   }
+}
+
+template <int REC, int FLUX>
+__global__ __launch_bounds__(256) void wall_flux_kernel(WallArgs A, WallBoundary WB) {
+  wall_body<REC, FLUX>(A, WB, blockIdx.x, blockIdx.x);
 }
 
 }  // namespace bs

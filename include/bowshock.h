@@ -196,6 +196,46 @@ bs_status bs_enable_graphs(bs_solver *s, int32_t enable);
 bs_status bs_enable_timing(bs_solver *s, int32_t enable);
 bs_status bs_kernel_timing(bs_solver *s, int32_t kernel, double *ms_per_launch, int64_t *launches);
 
+/* ------------------------------------------------------------------------
+ * Batched solves. The reference solves its grids one after another
+ * (analysis/verified-drag.ts:239-300, each through GpuEuler2D); these entry
+ * points advance many cases of one scheme together, every stage one launch
+ * over all active cases, with the same per-case kernels, status words and
+ * FP64 reductions as the single-solve entry points above. Cases share the
+ * scheme fields of bs_config and their boundary kinds (values may differ),
+ * use local time steps, and have no faceStates sides.
+ * ------------------------------------------------------------------------ */
+typedef struct bs_batch bs_batch;
+
+bs_status bs_batch_create(bs_ctx *ctx, const bs_config *cases, int32_t count, bs_batch **out);
+bs_status bs_batch_destroy(bs_batch *b);
+/* Uniform start per case (setInitialCondition, euler2d-gpu.ts:412-437):
+ * 4 primitives (rho, u, v, p) per case. */
+bs_status bs_batch_set_uniform(bs_batch *b, const double *primitives);
+/* setCellPrimitives (euler2d-gpu.ts:439-460) of one case: 4 per cell. */
+bs_status bs_batch_set_primitives(bs_batch *b, int32_t index, const double *primitives);
+/* readState (euler2d-gpu.ts:549-573) of one case; null outputs skipped. */
+bs_status bs_batch_read_state(bs_batch *b, int32_t index, float *rho, float *mu, float *mv, float *energy);
+/* Cases with active[c] == 0 are skipped by later steps and reductions. */
+bs_status bs_batch_set_active(bs_batch *b, const uint8_t *active);
+/* Cases of fewer cells than the threshold run on 32 x 4 tiles, the others
+ * on 64 x 8 tiles (set by the autotuner; 0 = all on 64 x 8). */
+bs_status bs_batch_set_tiling(bs_batch *b, int32_t small_tile_below_cells);
+/* `steps` explicit steps of every active case (iterate). */
+bs_status bs_batch_iterate(bs_batch *b, int32_t steps);
+/* setImplicitCfl and iterateImplicit (euler2d-gpu.ts:668-723) of every
+ * active case. */
+bs_status bs_batch_set_implicit_cfl(bs_batch *b, double cfl);
+bs_status bs_batch_iterate_implicit(bs_batch *b, int32_t steps, double cfl_max, int32_t sweeps, double omega,
+                                    double growth);
+/* densityResidualNorm per case (NaN for inactive cases). */
+bs_status bs_batch_residual_norms(bs_batch *b, double *norms);
+/* forebodyPressureDrag over each case's jMin side (NaN when inactive). */
+bs_status bs_batch_drags(bs_batch *b, const double *p_inf, const double *q_inf, const double *base_radius,
+                         double *cd);
+/* Status word (bit 0: non-physical) and step count per case. */
+bs_status bs_batch_status(bs_batch *b, uint32_t *status, int64_t *steps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,133 @@
+"""Batched solves (bs_batch_*) against the same cases solved one at a time
+by the single-grid engine, whose parity with the oracle the other GPU
+tests establish: the per-case kernels are the same, so every case's state
+is bitwise equal to its single solve, on either tile class, with cases
+retired in between; norms and drags agree to the reduction order."""
+import math
+
+import numpy as np
+import pytest
+
+import cases
+from bowshock_amd.batch import GpuBatch2D, SolvePlan, solve
+from bowshock_amd.engine import Euler2DConfig, GpuEuler2D, RangeError
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [((72, 36), 2.5), ((40, 20), 3.0), ((130, 64), 1.8), ((24, 12), 4.0)]
+
+
+def _cases(**scheme):
+    out = []
+    for (ni, nj), mach in SHAPES:
+        grid, free, p_inf, q_inf, bcs = cases.cone_case(ni, nj, mach=mach)
+        kw = dict(geometry="axisymmetric", gamma=cases.GAMMA, flux="hllc", limiter="minmod",
+                  reconstruction="perVariable", timeIntegrator="ssprk3", cfl=0.5, timeStepping="local")
+        kw.update(scheme)
+        out.append((Euler2DConfig(grid=grid, boundaries=bcs, **kw), free, p_inf, q_inf))
+    return out
+
+
+def _singles(ctx, cs):
+    solvers = []
+    for cfg, free, _, _ in cs:
+        s = GpuEuler2D.create(ctx, cfg)
+        s.setCellPrimitives(np.array(free))
+        solvers.append(s)
+    return solvers
+
+
+def _state(st):
+    return np.stack([st.rho, st.mu, st.mv, st.energy])
+
+
+@pytest.mark.parametrize("small_below", [0, 5000])
+def test_explicit_batch_matches_single_solves(ctx, small_below):
+    cs = _cases()
+    batch = GpuBatch2D(ctx, [c[0] for c in cs])
+    batch.setTiling(small_below)
+    batch.setUniform([c[1] for c in cs])
+    singles = _singles(ctx, cs)
+    for count in (37, 24):
+        batch.iterate(count)
+        for s in singles:
+            s.iterate(count)
+        for k, s in enumerate(singles):
+            assert np.array_equal(_state(batch.readState(k)), _state(s.readState())), k
+    norms = batch.densityResidualNorms()
+    p = np.array([c[2] for c in cs])
+    q = np.array([c[3] for c in cs])
+    drags = batch.forebodyDrags(p, q, 0.5)
+    for k, s in enumerate(singles):
+        assert norms[k] == pytest.approx(s.densityResidualNorm(), rel=1e-12, abs=0)
+        assert drags[k] == pytest.approx(s.forebodyDrag(p[k], q[k], 0.5), rel=1e-13, abs=1e-15)
+    status, steps = batch.status()
+    assert status.tolist() == [0] * len(cs) and steps.tolist() == [61] * len(cs)
+
+
+def test_retired_cases_stay_put(ctx):
+    cs = _cases()
+    batch = GpuBatch2D(ctx, [c[0] for c in cs])
+    batch.setUniform([c[1] for c in cs])
+    singles = _singles(ctx, cs)
+    batch.iterate(30)
+    for s in singles:
+        s.iterate(30)
+    frozen = _state(batch.readState(1))
+    batch.setActive([1, 0, 1, 1])
+    batch.iterate(25)
+    for k in (0, 2, 3):
+        singles[k].iterate(25)
+    assert np.array_equal(_state(batch.readState(1)), frozen)
+    for k in (0, 2, 3):
+        assert np.array_equal(_state(batch.readState(k)), _state(singles[k].readState())), k
+    norms = batch.densityResidualNorms()
+    assert math.isnan(norms[1]) and all(math.isfinite(norms[k]) for k in (0, 2, 3))
+    _, steps = batch.status()
+    assert steps.tolist() == [55, 30, 55, 55]
+    # Back in: it resumes where it stopped.
+    batch.setActive([1, 1, 1, 1])
+    batch.iterate(5)
+    singles[1].iterate(5)
+    assert np.array_equal(_state(batch.readState(1)), _state(singles[1].readState()))
+
+
+@pytest.mark.parametrize("small_below", [0, 5000])
+def test_implicit_batch_matches_single_solves(ctx, small_below):
+    cs = _cases(limiter="vanAlbada", reconstruction="sharedScaled")
+    batch = GpuBatch2D(ctx, [c[0] for c in cs])
+    batch.setTiling(small_below)
+    batch.setUniform([c[1] for c in cs])
+    batch.setImplicitCfl(1.0)
+    singles = _singles(ctx, cs)
+    for s in singles:
+        s.setImplicitCfl(1.0)
+    for count, sweeps in ((30, 2), (13, 2), (12, 1)):
+        batch.iterateImplicit(count, 10.0, sweeps)
+        for s in singles:
+            s.iterateImplicit(count, cfl=10.0, sweeps=sweeps)
+        for k, s in enumerate(singles):
+            assert np.array_equal(_state(batch.readState(k)), _state(s.readState())), (count, k)
+
+
+def test_batch_solve_matches_single_converged_drag(ctx):
+    """The driver loop retires each case once its residual and drag have
+    settled; the drags equal a single solve run for the same step count."""
+    cs = _cases()[:2]
+    batch = GpuBatch2D(ctx, [c[0] for c in cs])
+    batch.setUniform([c[1] for c in cs])
+    p = np.array([c[2] for c in cs])
+    q = np.array([c[3] for c in cs])
+    res = solve(batch, p, q, 0.5, SolvePlan(chunk=240, max_steps=24000, drop=1e-4, cd_tol=1e-6))
+    for k, r in enumerate(res):
+        assert r.converged, (k, r.reason, r.residual, r.steps)
+        s = _singles(ctx, [cs[k]])[0]
+        s.iterate(r.steps)
+        assert s.forebodyDrag(p[k], q[k], 0.5) == pytest.approx(r.cd, rel=1e-13)
+
+
+def test_batch_rejects_mixed_schemes(ctx):
+    a = _cases()[0][0]
+    b = _cases(limiter="vanLeer")[1][0]
+    with pytest.raises(RangeError):
+        GpuBatch2D(ctx, [a, b])
