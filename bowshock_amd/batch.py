@@ -135,10 +135,11 @@ class SolvePlan:
     drop or a settled drag (verified-drag.ts:258-300: batches of 500 r steps
     until the drag changes by less than its tolerance; transonic-flow.ts:
     residual target). A case converges when its density residual norm is at
-    most ``residual`` (absolute) or has fallen by ``drop`` from its first
-    reading, and its drag changed by at most ``cd_tol`` (relative) over the
-    last ``settle`` checks. It fails at ``max_steps`` or on a non-physical
-    state."""
+    most ``residual`` (absolute), has fallen by ``drop`` from its first
+    reading, or has reached the FP32 floor (no new minimum 10% lower in
+    ``stall`` checks), and its drag changed by at most ``cd_tol``
+    (relative) over the last ``settle`` checks. It fails at ``max_steps``
+    or on a non-physical state."""
     implicit: bool = False
     chunk: int = 120
     max_steps: int = 20000
@@ -146,6 +147,7 @@ class SolvePlan:
     drop: float = 1e-4
     cd_tol: float = 1e-5
     settle: int = 2
+    stall: int = 4
     cfl: float = 10.0          # implicit: final CFL number
     cfl_start: float = 1.0     # implicit: initial CFL number
     sweeps: int = 2
@@ -153,9 +155,10 @@ class SolvePlan:
     growth: float = 1.2
 
 
-def solve(batch: GpuBatch2D, p_inf, q_inf, base_radius, plan: SolvePlan = SolvePlan()) -> List[CaseResult]:
+def solve(batch: GpuBatch2D, p_inf, q_inf, base_radius, plan: Optional[SolvePlan] = None) -> List[CaseResult]:
     """Run every case of the batch until it converges or fails; converged
     and failed cases leave the launches at the next check."""
+    plan = plan or SolvePlan()
     n = batch.count
     res = [CaseResult() for _ in range(n)]
     p_inf = np.broadcast_to(np.asarray(p_inf, dtype=np.float64), (n,))
@@ -164,6 +167,8 @@ def solve(batch: GpuBatch2D, p_inf, q_inf, base_radius, plan: SolvePlan = SolveP
     active = np.ones(n, dtype=np.uint8)
     batch.setActive(active)
     first = np.full(n, math.nan)
+    best = np.full(n, math.inf)
+    since_best = np.zeros(n, dtype=np.int64)
     cds: List[List[float]] = [[] for _ in range(n)]
     if plan.implicit:
         batch.setImplicitCfl(plan.cfl_start)
@@ -189,13 +194,19 @@ def solve(batch: GpuBatch2D, p_inf, q_inf, base_radius, plan: SolvePlan = SolveP
             else:
                 if math.isnan(first[c]):
                     first[c] = r.residual
+                if r.residual < 0.9 * best[c]:
+                    best[c] = r.residual
+                    since_best[c] = 0
+                else:
+                    since_best[c] += 1
                 cds[c].append(r.cd)
                 settled = len(cds[c]) > plan.settle and all(
                     abs(cds[c][-1] - v) <= plan.cd_tol * abs(cds[c][-1]) for v in cds[c][-1 - plan.settle:-1])
                 small = r.residual <= plan.residual or r.residual <= plan.drop * first[c]
-                if small and settled:
+                stalled = since_best[c] >= plan.stall
+                if settled and (small or stalled):
                     r.converged = True
-                    r.reason = "converged"
+                    r.reason = "converged" if small else "converged (residual floor)"
                 elif done >= plan.max_steps:
                     r.reason = "step limit"
                 else:

@@ -549,3 +549,123 @@ def free_stream_at(mach: float, gamma: float):
     rounded to 32 bits; returns (state, pressure, dynamic pressure)."""
     u = float(np.float32(mach * math.sqrt(gamma)))
     return (1.0, u, 0.0, 1.0), 1.0, 0.5 * u * u
+
+
+# --------------------------------------------------------------------------
+# Vectorized generator for populations of bodies (the formulas of
+# body_grid_fixed above, evaluated on numpy arrays).
+# --------------------------------------------------------------------------
+
+def nose_radius_array(nose: Nose, x) -> np.ndarray:
+    """noseRadius (nose.ts:76-98) at many stations."""
+    L, R = nose.length, nose.base_radius
+    t = np.clip(np.asarray(x, dtype=np.float64) / L, 0.0, 1.0)
+    k = nose.kind
+    if k == "conical" or (k == "ogive" and nose.parameter < OGIVE_CONE_LIMIT):
+        return R * t
+    if k == "ogive":
+        stretch, _, rho, xc, y0 = _ogive_arc(nose.parameter, L, R)
+        s = t * L * stretch
+        r = (s * (2 * xc - s)) / (np.sqrt(np.maximum(0.0, rho * rho - (xc - s) ** 2)) + y0)
+        return np.where(s <= 0, 0.0, r)
+    if k == "ellipsoid":
+        return R * np.sqrt(np.maximum(0.0, 1 - (1 - t) * (1 - t)))
+    if k == "power":
+        return R * t ** nose.parameter
+    if k == "parabolic":
+        kk = nose.parameter
+        return (R * (2 * t - kk * t * t)) / (2 - kk)
+    if k == "haack":
+        theta = np.arccos(1 - 2 * t)
+        area = theta - np.sin(2 * theta) / 2 + nose.parameter * np.sin(theta) ** 3
+        return R * np.sqrt(np.maximum(0.0, area) / math.pi)
+    raise ValueError(f"unknown nose shape {k}")
+
+
+def body_radius_array(body: Body, x) -> np.ndarray:
+    x = np.asarray(x, dtype=np.float64)
+    r = nose_radius_array(body.nose, x)
+    return np.where(x <= 0, 0.0, np.where(x < body.nose.length, r, body.nose.base_radius))
+
+
+def _outer_geometry_array(body: Body, shock: float, nj_base: int):
+    R = body.nose.base_radius
+    x_end = body_length(body)
+    x_start = -R
+    max_strength = 2 * math.log(QUALITY_LIMITS["max_aspect_ratio"])
+    max_outer = (shock + math.pi / 2) / 2
+    samples = np.array(body_breakpoints(body)[1:] + [(x_end * k) / 1024 for k in range(1, 1024)])
+    wall_s = body_radius_array(body, samples)
+
+    def across_for(angle):
+        def across(x):
+            wall = body_radius_array(body, x)
+            h = 0.5 * R + (np.asarray(x) - x_start) * math.tan(angle) - wall
+            return wall, h, np.minimum(np.log1p(h / R), max_strength)
+        return across
+
+    def fraction_above_shock(angle):
+        h = 0.5 * R + (samples - x_start) * math.tan(angle) - wall_s
+        b = np.minimum(np.log1p(h / R), max_strength)
+        eta = np.log1p((np.expm1(b) * (samples * math.tan(shock) - wall_s)) / h) / b
+        return float(np.min(1 - eta))
+
+    base_nj = max(nj_base, math.ceil(OUTER_CELLS_ABOVE_SHOCK / fraction_above_shock(max_outer) - 1e-9))
+    outer = min(OUTER_ANGLE_FACTOR * shock, max_outer)
+    if base_nj * fraction_above_shock(outer) < OUTER_CELLS_ABOVE_SHOCK:
+        low, high = outer, max_outer
+        for _ in range(50):
+            mid = (low + high) / 2
+            if base_nj * fraction_above_shock(mid) < OUTER_CELLS_ABOVE_SHOCK:
+                low = mid
+            else:
+                high = mid
+        outer = high
+    return outer, across_for(outer), x_start
+
+
+def tip_shock_angle(body: Body, mach: float, gamma: float, tolerance: float = 1e-13) -> float:
+    """The conical shock angle at the nose tip (body-grid.ts:86-100):
+    raises ValueError for a blunt tip, a tip half-angle above the grid's
+    limit, or a detached shock."""
+    tip_slope = nose_tip_slope(body.nose)
+    if not math.isfinite(tip_slope):
+        raise ValueError("blunt noses (profile vertical at the tip) need a C grid")
+    tip_half = math.atan(tip_slope)
+    if tip_half > MAX_TIP_HALF_ANGLE:
+        raise ValueError(f"tip half-angles above {math.degrees(MAX_TIP_HALF_ANGLE):.1f} deg are not supported")
+    return taylor_maccoll(mach, tip_half, gamma, tolerance).shock_angle
+
+
+def body_grid_array(body: Body, ni: int, nj: int, shock: float) -> BodyGrid:
+    """body_grid_fixed on numpy arrays, given the tip shock angle: the
+    same outer boundary, row stretching and column placement."""
+    outer, across, x_start = _outer_geometry_array(body, shock, nj)
+    bps = [x_start] + body_breakpoints(body)
+    tables = []
+    for k in range(1, len(bps)):
+        a, b = bps[k - 1], bps[k]
+        at = np.linspace(a, b, 4097)
+        _, h, bb = across(at)
+        vals = (np.expm1(bb) * nj) / (h * bb * np.exp(bb / 2))
+        cum = np.concatenate([[0.0], np.cumsum(0.5 * (vals[1:] + vals[:-1]) * np.diff(at))])
+        tables.append((a, b, at, cum))
+    totals = np.array([t[3][-1] for t in tables])
+    share = totals / totals.sum() * ni
+    counts = np.maximum(1, np.floor(share).astype(int))
+    while counts.sum() < ni:
+        counts[np.argmax(share - counts)] += 1
+    while counts.sum() > ni:
+        counts[np.argmax(counts - share)] -= 1
+    xs = [bps[0]]
+    for (a, b, at, cum), count in zip(tables, counts):
+        targets = cum[-1] * np.arange(1, count) / count
+        xs.extend(np.interp(targets, cum, at).tolist())
+        xs.append(b)
+    xa = np.array(xs)
+    wall, h, bb = across(xa)
+    jj = np.arange(nj + 1)[:, None] / nj
+    y = wall[None, :] + (h[None, :] * np.expm1(bb[None, :] * jj)) / np.expm1(bb)[None, :]
+    x = np.broadcast_to(xa[None, :], y.shape)
+    grid = StructuredGrid(ni, nj, np.ascontiguousarray(x).ravel(), np.ascontiguousarray(y).ravel())
+    return BodyGrid(grid, xs.index(0.0), outer)

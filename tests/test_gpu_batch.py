@@ -118,7 +118,7 @@ def test_batch_solve_matches_single_converged_drag(ctx):
     batch.setUniform([c[1] for c in cs])
     p = np.array([c[2] for c in cs])
     q = np.array([c[3] for c in cs])
-    res = solve(batch, p, q, 0.5, SolvePlan(chunk=240, max_steps=24000, drop=1e-4, cd_tol=1e-6))
+    res = solve(batch, p, q, 0.5, SolvePlan(chunk=240, max_steps=24000))
     for k, r in enumerate(res):
         assert r.converged, (k, r.reason, r.residual, r.steps)
         s = _singles(ctx, [cs[k]])[0]

@@ -1,0 +1,60 @@
+"""Case generators for batched solves (CPU): the seeded draws, the grid
+sizing rules, refusals, and the vectorized grid generator against the
+scalar one."""
+import math
+
+import numpy as np
+import pytest
+
+from bowshock_amd import grids
+from bowshock_amd import population as P
+
+
+def test_mulberry32_is_a_32_bit_generator():
+    """Uniform in [0, 1), reproducible, period-free over a long draw, and
+    each draw a multiple of 2^-32."""
+    a = P.mulberry32(P.SEED)
+    b = P.mulberry32(P.SEED)
+    xs = np.array([a() for _ in range(20000)])
+    assert np.array_equal(xs, [b() for _ in range(20000)])
+    assert xs.min() >= 0 and xs.max() < 1
+    assert np.all(np.mod(xs * 2 ** 32, 1) == 0)
+    assert abs(xs.mean() - 0.5) < 0.01 and len(np.unique(xs)) == len(xs)
+    assert P.mulberry32(1)() != P.mulberry32(2)()
+
+
+def test_population_sizes_follow_the_rule():
+    cs = P.population_cases(50)
+    assert len(cs) == 200
+    for c in cs:
+        L = c.rocket.fineness + c.cylinder
+        assert 160 <= c.ni <= 640 and c.ni % 2 == 0 and c.nj == c.ni // 2
+        assert abs(c.ni - (160 + 480 * (L - 7) / 18)) <= 1
+        assert 1 <= c.rocket.fineness <= 5 and 6 <= c.cylinder <= 20 and 24 <= c.rocket.diameter_mm <= 100
+    sw = P.sweep_cases()
+    assert len(sw) == 39 and sw[0].mach == 1.2 and sw[-1].mach == 5.0
+    assert len({(c.rocket, c.ni) for c in sw}) == 1
+
+
+def test_refusals():
+    blunt = P.Case(P.Rocket(0, "ellipsoid", 2.0, 50, 8.0), 3.0, 160, 80, 8.0)
+    steep = P.Case(P.Rocket(1, "parabolic", 1.2, 50, 8.0), 3.0, 160, 80, 8.0)
+    detached = P.Case(P.Rocket(2, "tangent-ogive", 1.95, 50, 8.0), 1.2, 160, 80, 8.0)
+    fine = P.Case(P.Rocket(3, "conical", 3.0, 50, 8.0), 2.0, 40, 20, 8.0)
+    P.build_grids([blunt, steep, detached, fine], workers=1)
+    counts = P.refusal_counts([blunt, steep, detached, fine])
+    assert counts == {"blunt tip (needs a C grid)": 1, "tip half-angle above 30 deg": 1,
+                      "detached tip shock": 1, "accepted": 1}
+    assert fine.grid is not None and fine.grid.ni == 40 and fine.grid.nj == 20
+
+
+@pytest.mark.parametrize("body,mach", [(grids.ogive_cylinder(3.0, 15.0), 2.0), (grids.cone_cylinder(15, 4), 3.0),
+                                       (grids.Body(grids.Nose("parabolic", 2.5, 0.5, 1.0), 8.0), 1.5)])
+def test_array_generator_matches_the_scalar_one(body, mach):
+    a = grids.body_grid_fixed(body, 96, 48, mach, 1.4)
+    b = grids.body_grid_array(body, 96, 48, grids.tip_shock_angle(body, mach, 1.4))
+    assert a.i_tip == b.i_tip and a.outer_angle == b.outer_angle
+    assert np.allclose(a.grid.x, b.grid.x, rtol=0, atol=1e-13)
+    assert np.allclose(a.grid.y, b.grid.y, rtol=0, atol=1e-13)
+    for x in np.linspace(-0.5, grids.body_length(body) + 0.1, 57):
+        assert grids.body_radius_array(body, x) == pytest.approx(grids.body_radius(body, x), abs=1e-15)
