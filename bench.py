@@ -45,12 +45,13 @@ BYTES_PER_STAGE = BYTES_PER_CU_RK3 / 3.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--ni", type=int, default=1280)
     ap.add_argument("--nj", type=int, default=640)
     ap.add_argument("--mach", type=float, default=3.0)
-    ap.add_argument("--cpu-steps", type=int, default=8, help="oracle steps timed for the CPU baseline (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=40, help="oracle steps timed for the CPU baseline (0: skip)")
+    ap.add_argument("--no-graphs", action="store_true", help="launch kernel by kernel instead of HIP graphs")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "stage_kernel_1280x640_pmc.txt"))
     ap.add_argument("--workload", choices=("single", "sweep", "population"), default="single")
     ap.add_argument("--rockets", type=int, default=2000, help="population: rockets drawn (4 Mach numbers each)")
@@ -97,6 +98,8 @@ def main():
     ctx = Context(0)
     gpu = GpuEuler2D.create(ctx, Euler2DConfig(grid=bg.grid, boundaries=bcs, **scheme))
     gpu.setCellPrimitives(np.array(free))
+    if a.no_graphs:
+        gpu.enableGraphs(False)
     gpu.iterate(a.warmup)
     cells = a.ni * a.nj
 
@@ -111,9 +114,11 @@ def main():
     print(f"{a.steps} steps in {elapsed:.3f}s: {ms_per_step * 1e3:.1f} us/step, {value:.3e} CU/s",
           file=sys.stderr, flush=True)
 
-    # Kernel timing pass: HIP events around each stage-kernel launch.
-    gpu.enableTiming(True)
-    gpu.iterate(min(a.steps, 300))
+    # Kernel timing pass: HIP events on the solver's stream around each
+    # batch of steps (explicit local steps launch the stage kernel only),
+    # graphs on as in the timed region.
+    gpu.enableTiming("region")
+    gpu.iterate(min(a.steps, 1200))
     ms_launch, launches = gpu.kernelTiming("stage")
     gpu.enableTiming(False)
     bytes_per_launch = BYTES_PER_STAGE * cells

@@ -84,8 +84,11 @@ struct DeviceBuffers {
 
 struct Timing {
   bool enabled = false;
+  bool region = false;     // one pair per batch of explicit local steps
+  bool suppress = false;   // inside such a batch: no per-launch pairs
   std::vector<hipEvent_t> pool;
   std::vector<int> kinds;  // kind per recorded pair
+  std::vector<long long> weights;  // launches each pair brackets
   int used = 0;
   double total_ms[4] = {0, 0, 0, 0};
   long long launches[4] = {0, 0, 0, 0};
@@ -228,7 +231,7 @@ WallFn wall_fn(const bs_config& c) {
 
 // ---- timing -------------------------------------------------------------------
 
-hipEvent_t* timing_begin(bs_solver* s, int kind) {
+hipEvent_t* timing_pair(bs_solver* s, int kind, long long weight) {
   Timing& t = s->timing;
   if (!t.enabled) return nullptr;
   if ((size_t)(2 * t.used + 2) > t.pool.size()) {
@@ -240,10 +243,17 @@ hipEvent_t* timing_begin(bs_solver* s, int kind) {
   }
   hipEvent_t* pair = &t.pool[2 * t.used];
   t.kinds.resize(t.used + 1);
+  t.weights.resize(t.used + 1);
   t.kinds[t.used] = kind;
+  t.weights[t.used] = weight;
   t.used++;
   (void)hipEventRecord(pair[0], s->stream);
   return pair;
+}
+
+hipEvent_t* timing_begin(bs_solver* s, int kind) {
+  if (s->timing.suppress) return nullptr;
+  return timing_pair(s, kind, 1);
 }
 
 void timing_end(bs_solver* s, hipEvent_t* pair) {
@@ -257,7 +267,7 @@ void timing_collect(bs_solver* s) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, t.pool[2 * k], t.pool[2 * k + 1]) == hipSuccess) {
       t.total_ms[t.kinds[k]] += ms;
-      t.launches[t.kinds[k]]++;
+      t.launches[t.kinds[k]] += t.weights[k];
     }
   }
   t.used = 0;
@@ -377,12 +387,12 @@ bs_status capture_steps(bs_solver* s, StepGraph& g, Encode encode) {
 }
 
 // `count` steps: whole captured graphs where possible (one launch per
-// GRAPH_STEPS steps), the rest launched kernel by kernel. Kernel timing
-// needs the events between launches, so it runs without graphs.
+// GRAPH_STEPS steps), the rest launched kernel by kernel. Per-launch kernel
+// timing needs the events between launches, so it runs without graphs.
 template <typename Encode>
 bs_status run_steps(bs_solver* s, StepGraph& g, int count, Encode encode) {
   int k = 0;
-  if (s->graphs && !s->timing.enabled && count >= GRAPH_STEPS) {
+  if (s->graphs && (!s->timing.enabled || s->timing.suppress) && count >= GRAPH_STEPS) {
     if (!(g.exec && g.cur == s->cur)) {
       bs_status st = capture_steps(s, g, encode);
       if (st) return st;
@@ -785,10 +795,20 @@ bs_status bs_iterate(bs_solver* s, int32_t count, bs_report* report) {
     c.t_end = 3e38;
     HIP_TRY(hipMemcpyAsync(s->d.control, &c, sizeof c, hipMemcpyHostToDevice, s->stream), "control");
   }
+  // Region timing: a batch of explicit local steps launches nothing but
+  // the stage kernel, so one event pair around the batch, split over its
+  // stage launches, times that kernel with graphs left on.
+  const float *ca, *cb;
+  const int ns = rk_table(s->cfg.integrator, &ca, &cb);
+  const bool region = s->timing.enabled && s->timing.region && s->cfg.time_stepping == BS_STEPPING_LOCAL;
   int remaining = count;
   while (remaining > 0) {
     const int batch = remaining < BATCH ? remaining : BATCH;
+    hipEvent_t* pair = region ? timing_pair(s, 0, (long long)batch * ns) : nullptr;
+    s->timing.suppress = region;
     bs_status st = run_steps(s, s->explicit_graph, batch, [s]() { return encode_step(s); });
+    s->timing.suppress = false;
+    timing_end(s, pair);
     if (st) return st;
     st = read_control(s, &c);
     if (st) return st;
@@ -1069,6 +1089,7 @@ bs_status bs_enable_graphs(bs_solver* s, int32_t enable) {
 bs_status bs_enable_timing(bs_solver* s, int32_t enable) {
   SOLVER_GUARD(s);
   s->timing.enabled = enable != 0;
+  s->timing.region = enable == 2;
   for (int k = 0; k < 4; k++) {
     s->timing.total_ms[k] = 0;
     s->timing.launches[k] = 0;

@@ -290,8 +290,13 @@ class GpuEuler2D:
         """Run steps as captured HIP graphs (default on)."""
         check(N.lib().bs_enable_graphs(self._h, 1 if on else 0))
 
-    def enableTiming(self, on: bool = True):
-        check(N.lib().bs_enable_timing(self._h, 1 if on else 0))
+    def enableTiming(self, on=True):
+        """HIP-event timing of the solver's kernels: True (or "launch") times
+        every launch, without graphs; "region" times whole batches of
+        explicit local steps, which launch only the stage kernel, with graphs
+        on; False stops."""
+        mode = 2 if on == "region" else (1 if on else 0)
+        check(N.lib().bs_enable_timing(self._h, mode))
 
     def kernelTiming(self, kernel: str = "stage"):
         """(ms per launch, launches) since the last call."""

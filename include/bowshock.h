@@ -192,7 +192,11 @@ bs_status bs_enable_graphs(bs_solver *s, int32_t enable);
  * for throughput reporting: kernel-name index (0 stage, 1 implicit prepare,
  * 2 implicit relax, 3 implicit update), milliseconds per launch and the
  * launches counted. Timing is taken with HIP events on the solver's stream
- * when enabled with bs_enable_timing. */
+ * when enabled with bs_enable_timing: enable = 1 records a pair of events
+ * around every launch (steps then run without graphs); enable = 2 records
+ * one pair around each batch of explicit local-time steps, which launch the
+ * stage kernel only, and splits it over the batch's stage launches (graphs
+ * stay on; other steps are timed per launch). */
 bs_status bs_enable_timing(bs_solver *s, int32_t enable);
 bs_status bs_kernel_timing(bs_solver *s, int32_t kernel, double *ms_per_launch, int64_t *launches);
 

@@ -45,3 +45,20 @@ def test_implicit_graph_replays_match_eager(ctx):
     a.iterateImplicit(24, cfl=10.0, sweeps=1)
     b.iterateImplicit(24, cfl=10.0, sweeps=1)
     assert np.array_equal(cases.gpu_conserved(a), cases.gpu_conserved(b))
+
+
+def test_region_timing_counts_stage_launches(ctx):
+    """Region timing (events around whole batches of explicit local steps,
+    graphs on) counts every stage launch and agrees with per-launch events
+    to within the launch gaps."""
+    a, _, _, _ = _pair(ctx)
+    a.enableTiming("region")
+    a.iterate(300)
+    ms_region, n_region = a.kernelTiming("stage")
+    assert n_region == 900 and ms_region > 0
+    a.enableTiming(True)
+    a.iterate(60)
+    ms_launch, n_launch = a.kernelTiming("stage")
+    assert n_launch == 180 and ms_launch > 0
+    a.enableTiming(False)
+    assert 0.2 * ms_launch < ms_region < 5 * ms_launch
