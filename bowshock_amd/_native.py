@@ -73,6 +73,16 @@ SIGNATURES = {
     "bs_batch_residual_norms": (C.c_int, [_vp, _dp]),
     "bs_batch_drags": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
     "bs_batch_status": (C.c_int, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_int64)]),
+    "bs_mlp_create": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(_vp)]),
+    "bs_mlp_destroy": (C.c_int, [_vp]),
+    "bs_mlp_param_count": (C.c_int64, [_vp]),
+    "bs_mlp_set_params": (C.c_int, [_vp, _fp, C.c_int64]),
+    "bs_mlp_get_params": (C.c_int, [_vp, _fp, C.c_int64]),
+    "bs_mlp_load_data": (C.c_int, [_vp, _fp, _fp, C.c_int32]),
+    "bs_mlp_gradients": (C.c_int, [_vp, C.POINTER(C.c_int32), C.c_int32, _fp, _dp]),
+    "bs_mlp_train_batch": (C.c_int, [_vp, C.POINTER(C.c_int32), C.c_int32, C.c_double, _dp]),
+    "bs_mlp_predict": (C.c_int, [_vp, _fp, C.c_int32, _fp]),
+    "bs_mlp_time_gemm": (C.c_int, [_vp, C.c_int32, C.c_int32, _dp, _dp]),
 }
 
 _lib = None

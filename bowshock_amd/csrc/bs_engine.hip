@@ -1111,3 +1111,4 @@ bs_status bs_kernel_timing(bs_solver* s, int32_t kernel, double* ms_per_launch, 
 }  // extern "C"
 
 #include "bs_batch.hpp"
+#include "bs_mlp.hpp"

@@ -1,0 +1,582 @@
+// Learned warm start: a multilayer perceptron trained and evaluated with
+// hand-written BF16 MFMA GEMM kernels (FP32 accumulation), FP32 master
+// weights and Adam (bs_mlp_* in include/bowshock.h). No counterpart in the
+// reference, which contains no learned components.
+//
+// Network: inputs -> hidden (SiLU) x depth -> outputs (linear). Layer l maps
+// in_l to out_l with Z_l = A_{l-1} W_l + b_l, A_l = SiLU(Z_l); W_l is stored
+// in-major [in_l][out_l]. Every dimension the GEMMs see is a multiple of 64
+// (the input width is zero-padded to 64), so the kernels need no edge
+// handling. Included by bs_engine.hip.
+//
+// GEMM: C[M x N] = A[M x K] B[K x N], A and B BF16 with element strides
+// (row-major or transposed views), a 64 x 64 block tile in four wavefronts
+// of 32 x 32, K steps of 32 staged in LDS, v_mfma_f32_16x16x32_bf16 with
+// FP32 accumulators, and a fused epilogue (bias, SiLU, SiLU' for the
+// backward pass, FP32 and/or BF16 stores).
+
+namespace {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int GM = 64, GN = 64, GK = 32, GPAD = 8;
+
+enum Epilogue : int { EPI_STORE = 0, EPI_BIAS = 1, EPI_BIAS_SILU = 2, EPI_DSILU = 3 };
+
+struct GemmArgs {
+  const bf16* A;
+  long lda_m, lda_k;
+  const bf16* B;
+  long ldb_k, ldb_n;
+  float* C;          // FP32 result (EPI_STORE, EPI_BIAS: the output; EPI_BIAS_SILU: pre-activation; EPI_DSILU: gradient)
+  bf16* Cb;          // BF16 copy (EPI_BIAS_SILU: activation; EPI_DSILU: gradient), or null
+  long ldc;
+  const float* bias; // per column
+  const float* pre;  // EPI_DSILU: the forward pre-activation, ldc layout
+  int M, N, K;
+};
+
+__device__ __forceinline__ float sigmoid_f(float z) { return 1.f / (1.f + __expf(-z)); }
+
+// A: K contiguous (A_KC) or M contiguous; B: N contiguous (B_NC) or K
+// contiguous. LDS holds A as [m][k] and B as [n][k], so both MFMA operands
+// are 16-byte LDS reads of 8 consecutive k.
+template <int EPI, bool A_KC, bool B_NC>
+__global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
+  __shared__ __attribute__((aligned(16))) bf16 As[GM][GK + GPAD];
+  __shared__ __attribute__((aligned(16))) bf16 Bs[GN][GK + GPAD];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const long m0 = (long)blockIdx.y * GM, n0 = (long)blockIdx.x * GN;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  for (int k0 = 0; k0 < g.K; k0 += GK) {
+    if (A_KC) {
+      const int m = t >> 2, kc = (t & 3) * 8;
+      *(uint4*)&As[m][kc] = *(const uint4*)(g.A + (m0 + m) * g.lda_m + (k0 + kc));
+    } else {
+      const int k = t >> 3, mc = (t & 7) * 8;
+      uint4 v = *(const uint4*)(g.A + (m0 + mc) + (long)(k0 + k) * g.lda_k);
+      const bf16* e = (const bf16*)&v;
+#pragma unroll
+      for (int q = 0; q < 8; q++) As[mc + q][k] = e[q];
+    }
+    if (B_NC) {
+      const int k = t >> 3, nc = (t & 7) * 8;
+      uint4 v = *(const uint4*)(g.B + (long)(k0 + k) * g.ldb_k + (n0 + nc));
+      const bf16* e = (const bf16*)&v;
+#pragma unroll
+      for (int q = 0; q < 8; q++) Bs[nc + q][k] = e[q];
+    } else {
+      const int n = t >> 2, kc = (t & 3) * 8;
+      *(uint4*)&Bs[n][kc] = *(const uint4*)(g.B + (n0 + n) * g.ldb_n + (k0 + kc));
+    }
+    __syncthreads();
+    bf16x8 a[2], b[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) a[i] = *(const bf16x8*)&As[wm + i * 16 + (lane & 15)][8 * (lane >> 4)];
+#pragma unroll
+    for (int j = 0; j < 2; j++) b[j] = *(const bf16x8*)&Bs[wn + j * 16 + (lane & 15)][8 * (lane >> 4)];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+  // Accumulator r of lane l holds row 4 (l / 16) + r, column l % 16 of its
+  // 16 x 16 block.
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const long m = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
+        const long n = n0 + wn + j * 16 + (lane & 15);
+        const long o = m * g.ldc + n;
+        float v = acc[i][j][r];
+        if (EPI == EPI_STORE) {
+          g.C[o] = v;
+        } else if (EPI == EPI_BIAS) {
+          g.C[o] = v + g.bias[n];
+        } else if (EPI == EPI_BIAS_SILU) {
+          const float z = v + g.bias[n];
+          g.C[o] = z;
+          g.Cb[o] = (bf16)(z * sigmoid_f(z));
+        } else {
+          const float z = g.pre[o];
+          const float s = sigmoid_f(z);
+          const float d = v * (s * (1.f + z * (1.f - s)));
+          g.C[o] = d;
+          g.Cb[o] = (bf16)d;
+        }
+      }
+}
+
+// Rows idx[b] of the data set (or rows b of a staged block) into the padded
+// BF16 input; rows at or beyond `rows` are zero.
+__global__ void gather_inputs(const float* __restrict__ x, const int* __restrict__ idx, int rows, int inputs,
+                              int in_pad, int batch_pad, bf16* __restrict__ out) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)batch_pad * in_pad) return;
+  const int b = (int)(e / in_pad), k = (int)(e % in_pad);
+  float v = 0.f;
+  if (b < rows && k < inputs) {
+    const long row = idx ? idx[b] : b;
+    v = x[row * inputs + k];
+  }
+  out[e] = (bf16)v;
+}
+
+// dY = 2 (Y - T) / (rows * outputs) for the batch's rows (FP32 and BF16),
+// and per block the FP64 sum of squared errors.
+__global__ __launch_bounds__(256) void mse_grad(const float* __restrict__ y, const float* __restrict__ data_y,
+                                                const int* __restrict__ idx, int rows, int outputs, float scale,
+                                                float* __restrict__ dyf, bf16* __restrict__ dyb,
+                                                double* __restrict__ partials) {
+  __shared__ double red[4];
+  const long e = (long)blockIdx.x * 256 + threadIdx.x;
+  double sq = 0.0;
+  if (e < (long)rows * outputs) {
+    const int b = (int)(e / outputs), n = (int)(e % outputs);
+    const float d = y[e] - data_y[(long)idx[b] * outputs + n];
+    const float g = scale * d;
+    dyf[e] = g;
+    dyb[e] = (bf16)g;
+    sq = (double)d * (double)d;
+  }
+  double s = bs::block_sum<256>(sq, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// Column sums of a [rows x cols] FP32 matrix (bias gradients).
+__global__ void column_sums(const float* __restrict__ g, int rows, int cols, float* __restrict__ out) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= cols) return;
+  float s = 0.f;
+  for (int b = 0; b < rows; b++) s += g[(long)b * cols + n];
+  out[n] = s;
+}
+
+// Adam over the flat parameter vector; the BF16 mirror follows the FP32
+// master weights.
+__global__ void adam_step(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                          float* __restrict__ v, bf16* __restrict__ wb, long n, float lr, float b1, float b2,
+                          float eps, float c1, float c2) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const float gi = g[e];
+  const float mi = b1 * m[e] + (1.f - b1) * gi;
+  const float vi = b2 * v[e] + (1.f - b2) * gi * gi;
+  m[e] = mi;
+  v[e] = vi;
+  const float wi = w[e] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  w[e] = wi;
+  wb[e] = (bf16)wi;
+}
+
+__global__ void to_bf16(const float* __restrict__ w, bf16* __restrict__ wb, long n) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) wb[e] = (bf16)w[e];
+}
+
+int round64(int v) { return (v + 63) / 64 * 64; }
+
+}  // namespace
+
+struct bs_mlp {
+  int device;
+  hipStream_t stream = nullptr;
+  int inputs, in_pad, hidden, depth, outputs, max_batch;
+  int layers;                     // depth + 1
+  std::vector<int> lin, lout;     // padded widths per layer
+  std::vector<long> w_off, b_off; // offsets into the flat parameter vector
+  long n_params = 0;              // padded flat count
+  float *w = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr;
+  bf16* wb = nullptr;
+  bf16* x = nullptr;                      // [max_batch x in_pad]
+  std::vector<float*> z;                  // per hidden layer [max_batch x hidden], and the output
+  std::vector<bf16*> act;                 // per hidden layer
+  float *dzf[2] = {nullptr, nullptr};     // ping-pong gradients [max_batch x max(outputs, hidden)]
+  bf16* dzb[2] = {nullptr, nullptr};
+  double* partials = nullptr;
+  int* idx = nullptr;
+  float *data_x = nullptr, *data_y = nullptr;
+  int data_rows = 0;
+  float* stage_x = nullptr;               // predict: staged host block
+  long step = 0;
+  std::atomic<bool> busy{false};
+};
+
+namespace {
+
+struct MlpGuard {
+  bs_mlp* p;
+  bool ok;
+  explicit MlpGuard(bs_mlp* m) : p(m) {
+    bool expected = false;
+    ok = p->busy.compare_exchange_strong(expected, true);
+  }
+  ~MlpGuard() {
+    if (ok) p->busy.store(false);
+  }
+};
+
+#define MLP_GUARD(p)                                                                         \
+  if ((p) == nullptr) return fail(BS_ERR_RANGE, "no network");                               \
+  MlpGuard _mguard(p);                                                                       \
+  if (!_mguard.ok) return fail(BS_ERR_BUSY, "bs_mlp: await the previous operation first"); \
+  (void)hipSetDevice((p)->device)
+
+void mlp_free(bs_mlp* p) {
+  std::vector<void*> all = {p->w, p->grad, p->m, p->v, p->wb, p->x, p->dzf[0], p->dzf[1], p->dzb[0], p->dzb[1],
+                            p->partials, p->idx, p->data_x, p->data_y, p->stage_x};
+  for (float* q : p->z) all.push_back(q);
+  for (bf16* q : p->act) all.push_back(q);
+  for (void* q : all)
+    if (q) (void)hipFree(q);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
+  p->stream = nullptr;
+}
+
+template <int EPI>
+void launch_gemm(bs_mlp* p, const GemmArgs& g, bool a_kc, bool b_nc) {
+  dim3 grid(g.N / GN, g.M / GM);
+  if (a_kc && b_nc) hipLaunchKernelGGL((gemm_bf16_kernel<EPI, true, true>), grid, dim3(256), 0, p->stream, g);
+  else if (a_kc) hipLaunchKernelGGL((gemm_bf16_kernel<EPI, true, false>), grid, dim3(256), 0, p->stream, g);
+  else if (b_nc) hipLaunchKernelGGL((gemm_bf16_kernel<EPI, false, true>), grid, dim3(256), 0, p->stream, g);
+  else hipLaunchKernelGGL((gemm_bf16_kernel<EPI, false, false>), grid, dim3(256), 0, p->stream, g);
+}
+
+// Forward pass of `rows` staged input rows (p->x, padded to batch_pad).
+void mlp_forward(bs_mlp* p, int batch_pad) {
+  const bf16* a = p->x;
+  for (int l = 0; l < p->layers; l++) {
+    GemmArgs g{};
+    g.A = a;
+    g.lda_m = p->lin[l];
+    g.lda_k = 1;
+    g.B = p->wb + p->w_off[l];
+    g.ldb_k = p->lout[l];
+    g.ldb_n = 1;
+    g.C = p->z[l];
+    g.ldc = p->lout[l];
+    g.bias = p->w + p->b_off[l];
+    g.M = batch_pad;
+    g.N = p->lout[l];
+    g.K = p->lin[l];
+    if (l + 1 < p->layers) {
+      g.Cb = p->act[l];
+      launch_gemm<EPI_BIAS_SILU>(p, g, true, true);
+      a = p->act[l];
+    } else {
+      launch_gemm<EPI_BIAS>(p, g, true, true);
+    }
+  }
+}
+
+// Loss and gradients of the rows idx[0..batch) (batch a multiple of 64).
+bs_status mlp_backward(bs_mlp* p, int batch, double* loss) {
+  const int L = p->layers;
+  const long ne = (long)batch * p->outputs;
+  const int nb = (int)((ne + 255) / 256);
+  hipLaunchKernelGGL(mse_grad, dim3(nb), dim3(256), 0, p->stream, p->z[L - 1], p->data_y, p->idx, batch, p->outputs,
+                     2.f / (float)ne, p->dzf[0], p->dzb[0], p->partials);
+  hipLaunchKernelGGL(finalize_sum, dim3(1), dim3(256), 0, p->stream, p->partials, nb, 1.0, 0, p->partials + nb);
+  int cur = 0;
+  for (int l = L - 1; l >= 0; l--) {
+    const bf16* a_prev = l == 0 ? p->x : p->act[l - 1];
+    GemmArgs g{};  // dW_l = A_{l-1}^T dZ_l
+    g.A = a_prev;
+    g.lda_m = 1;
+    g.lda_k = p->lin[l];
+    g.B = p->dzb[cur];
+    g.ldb_k = p->lout[l];
+    g.ldb_n = 1;
+    g.C = p->grad + p->w_off[l];
+    g.ldc = p->lout[l];
+    g.M = p->lin[l];
+    g.N = p->lout[l];
+    g.K = batch;
+    launch_gemm<EPI_STORE>(p, g, false, true);
+    hipLaunchKernelGGL(column_sums, dim3((p->lout[l] + 255) / 256), dim3(256), 0, p->stream, p->dzf[cur], batch,
+                       p->lout[l], p->grad + p->b_off[l]);
+    if (l > 0) {
+      GemmArgs h{};  // dZ_{l-1} = (dZ_l W_l^T) * SiLU'(Z_{l-1})
+      h.A = p->dzb[cur];
+      h.lda_m = p->lout[l];
+      h.lda_k = 1;
+      h.B = p->wb + p->w_off[l];
+      h.ldb_k = 1;
+      h.ldb_n = p->lout[l];
+      h.C = p->dzf[1 - cur];
+      h.Cb = p->dzb[1 - cur];
+      h.ldc = p->lin[l];
+      h.pre = p->z[l - 1];
+      h.M = batch;
+      h.N = p->lin[l];
+      h.K = p->lout[l];
+      launch_gemm<EPI_DSILU>(p, h, true, false);
+      cur = 1 - cur;
+    }
+  }
+  HIP_TRY(hipGetLastError(), "network launch");
+  double out = 0;
+  HIP_TRY(hipMemcpyAsync(&out, p->partials + nb, sizeof out, hipMemcpyDeviceToHost, p->stream), "loss readback");
+  HIP_TRY(hipStreamSynchronize(p->stream), "network step");
+  if (loss) *loss = out / (double)ne;  // mean squared error
+  return BS_OK;
+}
+
+bs_status mlp_stage_indices(bs_mlp* p, const int32_t* idx, int batch) {
+  if (batch < 64 || batch > p->max_batch || batch % 64)
+    return fail(BS_ERR_RANGE, "the batch must be a multiple of 64 between 64 and %d", p->max_batch);
+  if (!p->data_x) return fail(BS_ERR_RANGE, "no data set loaded");
+  for (int b = 0; b < batch; b++)
+    if (idx[b] < 0 || idx[b] >= p->data_rows) return fail(BS_ERR_RANGE, "row index %d out of range", idx[b]);
+  HIP_TRY(hipMemcpyAsync(p->idx, idx, batch * sizeof(int), hipMemcpyHostToDevice, p->stream), "index upload");
+  const long ne = (long)batch * p->in_pad;
+  hipLaunchKernelGGL(gather_inputs, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, p->stream, p->data_x, p->idx,
+                     batch, p->inputs, p->in_pad, batch, p->x);
+  return BS_OK;
+}
+
+// External parameter layout: per layer W [in][out] (in = inputs for the
+// first layer, unpadded) then b [out].
+long external_count(const bs_mlp* p) {
+  long c = 0;
+  for (int l = 0; l < p->layers; l++) c += (long)(l == 0 ? p->inputs : p->lin[l]) * p->lout[l] + p->lout[l];
+  return c;
+}
+
+}  // namespace
+
+extern "C" {
+
+bs_status bs_mlp_create(bs_ctx* ctx, int32_t inputs, int32_t hidden, int32_t depth, int32_t outputs,
+                        int32_t max_batch, bs_mlp** out) {
+  if (!ctx || !out) return fail(BS_ERR_RANGE, "null argument");
+  *out = nullptr;
+  if (inputs < 1 || hidden < 64 || hidden % 64 || depth < 1 || outputs < 64 || outputs % 64 || max_batch < 64 ||
+      max_batch % 64)
+    return fail(BS_ERR_RANGE, "hidden width, outputs and batch must be positive multiples of 64");
+  bs_mlp* p = new bs_mlp();
+  p->device = ctx->device;
+  p->inputs = inputs;
+  p->in_pad = round64(inputs);
+  p->hidden = hidden;
+  p->depth = depth;
+  p->outputs = outputs;
+  p->max_batch = max_batch;
+  p->layers = depth + 1;
+  for (int l = 0; l < p->layers; l++) {
+    p->lin.push_back(l == 0 ? p->in_pad : hidden);
+    p->lout.push_back(l == depth ? outputs : hidden);
+    p->w_off.push_back(p->n_params);
+    p->n_params += (long)p->lin[l] * p->lout[l];
+    p->b_off.push_back(p->n_params);
+    p->n_params += p->lout[l];
+  }
+  (void)hipSetDevice(p->device);
+  auto bail = [&](bs_status code) {
+    mlp_free(p);
+    delete p;
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(BS_ERR_EXECUTION, "hipStreamCreate failed"));
+  const long wide = outputs > hidden ? outputs : hidden;
+  bs_status st;
+  if ((st = dev_alloc(&p->w, p->n_params, "weights")) || (st = dev_alloc(&p->grad, p->n_params, "gradients")) ||
+      (st = dev_alloc(&p->m, p->n_params, "moments")) || (st = dev_alloc(&p->v, p->n_params, "moments")) ||
+      (st = dev_alloc(&p->wb, p->n_params, "weights")) || (st = dev_alloc(&p->x, (size_t)max_batch * p->in_pad, "inputs")) ||
+      (st = dev_alloc(&p->dzf[0], (size_t)max_batch * wide, "gradients")) ||
+      (st = dev_alloc(&p->dzf[1], (size_t)max_batch * wide, "gradients")) ||
+      (st = dev_alloc(&p->dzb[0], (size_t)max_batch * wide, "gradients")) ||
+      (st = dev_alloc(&p->dzb[1], (size_t)max_batch * wide, "gradients")) ||
+      (st = dev_alloc(&p->partials, (size_t)((long)max_batch * outputs / 256 + 2), "partials")) ||
+      (st = dev_alloc(&p->idx, (size_t)max_batch, "indices")) ||
+      (st = dev_alloc(&p->stage_x, (size_t)max_batch * inputs, "inputs")))
+    return bail(st);
+  for (int l = 0; l < p->layers; l++) {
+    float* zl = nullptr;
+    if ((st = dev_alloc(&zl, (size_t)max_batch * p->lout[l], "activations"))) return bail(st);
+    p->z.push_back(zl);
+    if (l + 1 < p->layers) {
+      bf16* al = nullptr;
+      if ((st = dev_alloc(&al, (size_t)max_batch * p->lout[l], "activations"))) return bail(st);
+      p->act.push_back(al);
+    }
+  }
+  const size_t bytes = p->n_params * sizeof(float);
+  if (hipMemsetAsync(p->w, 0, bytes, p->stream) != hipSuccess || hipMemsetAsync(p->m, 0, bytes, p->stream) != hipSuccess ||
+      hipMemsetAsync(p->v, 0, bytes, p->stream) != hipSuccess ||
+      hipMemsetAsync(p->wb, 0, p->n_params * sizeof(bf16), p->stream) != hipSuccess ||
+      hipStreamSynchronize(p->stream) != hipSuccess)
+    return bail(fail(BS_ERR_EXECUTION, "network initialisation failed"));
+  *out = p;
+  return BS_OK;
+}
+
+bs_status bs_mlp_destroy(bs_mlp* p) {
+  if (!p) return BS_OK;
+  (void)hipSetDevice(p->device);
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  mlp_free(p);
+  delete p;
+  return BS_OK;
+}
+
+int64_t bs_mlp_param_count(const bs_mlp* p) { return p ? external_count(p) : 0; }
+
+bs_status bs_mlp_set_params(bs_mlp* p, const float* params, int64_t count) {
+  MLP_GUARD(p);
+  if (!params || count != external_count(p)) return fail(BS_ERR_RANGE, "expected %ld parameters", external_count(p));
+  std::vector<float> flat(p->n_params, 0.f);
+  const float* src = params;
+  for (int l = 0; l < p->layers; l++) {
+    const int in = l == 0 ? p->inputs : p->lin[l], out_w = p->lout[l];
+    for (int k = 0; k < in; k++)
+      for (int n = 0; n < out_w; n++) flat[p->w_off[l] + (long)k * out_w + n] = src[(long)k * out_w + n];
+    src += (long)in * out_w;
+    for (int n = 0; n < out_w; n++) flat[p->b_off[l] + n] = src[n];
+    src += out_w;
+  }
+  HIP_TRY(hipMemcpyAsync(p->w, flat.data(), flat.size() * sizeof(float), hipMemcpyHostToDevice, p->stream), "upload");
+  hipLaunchKernelGGL(to_bf16, dim3((unsigned)((p->n_params + 255) / 256)), dim3(256), 0, p->stream, p->w, p->wb,
+                     p->n_params);
+  HIP_TRY(hipMemsetAsync(p->m, 0, p->n_params * sizeof(float), p->stream), "reset");
+  HIP_TRY(hipMemsetAsync(p->v, 0, p->n_params * sizeof(float), p->stream), "reset");
+  HIP_TRY(hipStreamSynchronize(p->stream), "upload");
+  p->step = 0;
+  return BS_OK;
+}
+
+static bs_status mlp_read_flat(bs_mlp* p, const float* dev, float* params) {
+  std::vector<float> flat(p->n_params);
+  HIP_TRY(hipMemcpyAsync(flat.data(), dev, flat.size() * sizeof(float), hipMemcpyDeviceToHost, p->stream), "readback");
+  HIP_TRY(hipStreamSynchronize(p->stream), "readback");
+  float* dst = params;
+  for (int l = 0; l < p->layers; l++) {
+    const int in = l == 0 ? p->inputs : p->lin[l], out_w = p->lout[l];
+    for (int k = 0; k < in; k++)
+      for (int n = 0; n < out_w; n++) dst[(long)k * out_w + n] = flat[p->w_off[l] + (long)k * out_w + n];
+    dst += (long)in * out_w;
+    for (int n = 0; n < out_w; n++) dst[n] = flat[p->b_off[l] + n];
+    dst += out_w;
+  }
+  return BS_OK;
+}
+
+bs_status bs_mlp_get_params(bs_mlp* p, float* params, int64_t count) {
+  MLP_GUARD(p);
+  if (!params || count != external_count(p)) return fail(BS_ERR_RANGE, "expected %ld parameters", external_count(p));
+  return mlp_read_flat(p, p->w, params);
+}
+
+bs_status bs_mlp_load_data(bs_mlp* p, const float* x, const float* y, int32_t rows) {
+  MLP_GUARD(p);
+  if (!x || !y || rows < 1) return fail(BS_ERR_RANGE, "empty data set");
+  if (p->data_x) (void)hipFree(p->data_x);
+  if (p->data_y) (void)hipFree(p->data_y);
+  p->data_x = p->data_y = nullptr;
+  p->data_rows = 0;
+  bs_status st;
+  if ((st = dev_alloc(&p->data_x, (size_t)rows * p->inputs, "data set")) ||
+      (st = dev_alloc(&p->data_y, (size_t)rows * p->outputs, "data set")))
+    return st;
+  HIP_TRY(hipMemcpy(p->data_x, x, (size_t)rows * p->inputs * sizeof(float), hipMemcpyHostToDevice), "data upload");
+  HIP_TRY(hipMemcpy(p->data_y, y, (size_t)rows * p->outputs * sizeof(float), hipMemcpyHostToDevice), "data upload");
+  p->data_rows = rows;
+  return BS_OK;
+}
+
+bs_status bs_mlp_gradients(bs_mlp* p, const int32_t* idx, int32_t batch, float* grads, double* loss) {
+  MLP_GUARD(p);
+  if (!idx) return fail(BS_ERR_RANGE, "no indices");
+  bs_status st = mlp_stage_indices(p, idx, batch);
+  if (st) return st;
+  mlp_forward(p, batch);
+  if ((st = mlp_backward(p, batch, loss))) return st;
+  return grads ? mlp_read_flat(p, p->grad, grads) : BS_OK;
+}
+
+bs_status bs_mlp_train_batch(bs_mlp* p, const int32_t* idx, int32_t batch, double lr, double* loss) {
+  MLP_GUARD(p);
+  if (!idx || !(lr > 0)) return fail(BS_ERR_RANGE, "no indices or learning rate");
+  bs_status st = mlp_stage_indices(p, idx, batch);
+  if (st) return st;
+  mlp_forward(p, batch);
+  if ((st = mlp_backward(p, batch, loss))) return st;
+  p->step++;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - powf(b1, (float)p->step), c2 = 1.f - powf(b2, (float)p->step);
+  hipLaunchKernelGGL(adam_step, dim3((unsigned)((p->n_params + 255) / 256)), dim3(256), 0, p->stream, p->w, p->grad,
+                     p->m, p->v, p->wb, p->n_params, (float)lr, b1, b2, 1e-8f, c1, c2);
+  HIP_TRY(hipGetLastError(), "adam launch");
+  HIP_TRY(hipStreamSynchronize(p->stream), "adam step");
+  return BS_OK;
+}
+
+bs_status bs_mlp_predict(bs_mlp* p, const float* x, int32_t rows, float* y) {
+  MLP_GUARD(p);
+  if (!x || !y || rows < 0) return fail(BS_ERR_RANGE, "null argument");
+  for (int r0 = 0; r0 < rows; r0 += p->max_batch) {
+    const int n = rows - r0 < p->max_batch ? rows - r0 : p->max_batch;
+    const int pad = round64(n);
+    HIP_TRY(hipMemcpyAsync(p->stage_x, x + (long)r0 * p->inputs, (size_t)n * p->inputs * sizeof(float),
+                           hipMemcpyHostToDevice, p->stream),
+            "input upload");
+    const long ne = (long)pad * p->in_pad;
+    hipLaunchKernelGGL(gather_inputs, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, p->stream, p->stage_x,
+                       (const int*)nullptr, n, p->inputs, p->in_pad, pad, p->x);
+    mlp_forward(p, pad);
+    HIP_TRY(hipMemcpyAsync(y + (long)r0 * p->outputs, p->z[p->layers - 1], (size_t)n * p->outputs * sizeof(float),
+                           hipMemcpyDeviceToHost, p->stream),
+            "output readback");
+  }
+  HIP_TRY(hipGetLastError(), "network launch");
+  HIP_TRY(hipStreamSynchronize(p->stream), "predict");
+  return BS_OK;
+}
+
+bs_status bs_mlp_time_gemm(bs_mlp* p, int32_t layer, int32_t reps, double* ms_per_gemm, double* flops_per_gemm) {
+  MLP_GUARD(p);
+  if (layer < 0 || layer >= p->layers || reps < 1) return fail(BS_ERR_RANGE, "no such layer");
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0), "event");
+  HIP_TRY(hipEventCreate(&e1), "event");
+  GemmArgs g{};
+  g.A = layer == 0 ? p->x : p->act[layer - 1];
+  g.lda_m = p->lin[layer];
+  g.lda_k = 1;
+  g.B = p->wb + p->w_off[layer];
+  g.ldb_k = p->lout[layer];
+  g.ldb_n = 1;
+  g.C = p->z[layer];
+  g.ldc = p->lout[layer];
+  g.bias = p->w + p->b_off[layer];
+  g.M = p->max_batch;
+  g.N = p->lout[layer];
+  g.K = p->lin[layer];
+  launch_gemm<EPI_BIAS>(p, g, true, true);
+  (void)hipEventRecord(e0, p->stream);
+  for (int r = 0; r < reps; r++) launch_gemm<EPI_BIAS>(p, g, true, true);
+  (void)hipEventRecord(e1, p->stream);
+  HIP_TRY(hipEventSynchronize(e1), "gemm timing");
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ms_per_gemm) *ms_per_gemm = ms / reps;
+  if (flops_per_gemm) *flops_per_gemm = 2.0 * g.M * (double)g.N * g.K;
+  return BS_OK;
+}
+
+}  // extern "C"

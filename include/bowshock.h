@@ -240,6 +240,39 @@ bs_status bs_batch_drags(bs_batch *b, const double *p_inf, const double *q_inf, 
 /* Status word (bit 0: non-physical) and step count per case. */
 bs_status bs_batch_status(bs_batch *b, uint32_t *status, int64_t *steps);
 
+/* ------------------------------------------------------------------------
+ * Learned warm start. No counterpart in the reference, which contains no
+ * learned components: a multilayer perceptron, inputs -> hidden (SiLU) x
+ * depth -> outputs (linear), trained with BF16 MFMA GEMM kernels (FP32
+ * accumulation), FP32 master weights and Adam (beta1 0.9, beta2 0.999,
+ * eps 1e-8) on a mean-squared-error loss. hidden, outputs and the batch are
+ * multiples of 64. Parameters travel as FP32 in the order: per layer W
+ * [in][out] row-major (in = inputs for the first layer, else hidden) then
+ * b [out]. A new network has all parameters zero until bs_mlp_set_params.
+ * ------------------------------------------------------------------------ */
+typedef struct bs_mlp bs_mlp;
+
+bs_status bs_mlp_create(bs_ctx *ctx, int32_t inputs, int32_t hidden, int32_t depth, int32_t outputs,
+                        int32_t max_batch, bs_mlp **out);
+bs_status bs_mlp_destroy(bs_mlp *p);
+int64_t bs_mlp_param_count(const bs_mlp *p);
+/* Sets the parameters and resets Adam's moments and step count. */
+bs_status bs_mlp_set_params(bs_mlp *p, const float *params, int64_t count);
+bs_status bs_mlp_get_params(bs_mlp *p, float *params, int64_t count);
+/* Copies a data set to the device: rows x inputs and rows x outputs FP32. */
+bs_status bs_mlp_load_data(bs_mlp *p, const float *x, const float *y, int32_t rows);
+/* Loss (mean squared error) and gradients (parameter order) over the data
+ * rows idx[0 .. batch), without an update. */
+bs_status bs_mlp_gradients(bs_mlp *p, const int32_t *idx, int32_t batch, float *grads, double *loss);
+/* One Adam step on the data rows idx[0 .. batch); `loss` is the batch's
+ * loss before the step. */
+bs_status bs_mlp_train_batch(bs_mlp *p, const int32_t *idx, int32_t batch, double lr, double *loss);
+/* Outputs for rows x inputs host inputs, in blocks of the maximum batch. */
+bs_status bs_mlp_predict(bs_mlp *p, const float *x, int32_t rows, float *y);
+/* Device time of one forward GEMM of `layer` at the maximum batch (HIP
+ * events over `reps` launches) and its floating-point operations. */
+bs_status bs_mlp_time_gemm(bs_mlp *p, int32_t layer, int32_t reps, double *ms_per_gemm, double *flops_per_gemm);
+
 #ifdef __cplusplus
 }
 #endif
