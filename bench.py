@@ -19,7 +19,11 @@ seeded population of nose-plus-cylinder bodies at Mach 1.5, 2, 3 and 4
 (implicit red-black LU-SGS), every case run to a converged drag in one
 batch; the metric is converged solves per second and the aggregate
 cell-update rate. Their CPU baseline runs the oracle on all host cores,
-one case per worker.
+one case per worker. `--workload warmstart` trains the warm-start network
+on the converged fields of three quarters of the population's rockets and
+reports converged solves/s of the held-out quarter's cases started from
+the network's prediction against the same cases started from the free
+stream, with the network's dominant GEMM against the BF16 MFMA peak.
 
 Prints one JSON line.
 """
@@ -53,7 +57,8 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=40, help="oracle steps timed for the CPU baseline (0: skip)")
     ap.add_argument("--no-graphs", action="store_true", help="launch kernel by kernel instead of HIP graphs")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "stage_kernel_1280x640_pmc.txt"))
-    ap.add_argument("--workload", choices=("single", "sweep", "population"), default="single")
+    ap.add_argument("--workload", choices=("single", "sweep", "population", "warmstart"), default="single")
+    ap.add_argument("--train-steps", type=int, default=3000, help="warmstart: Adam steps of the network")
     ap.add_argument("--rockets", type=int, default=2000, help="population: rockets drawn (4 Mach numbers each)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="batched CPU baseline: seconds of oracle work")
     return ap.parse_args()
@@ -76,6 +81,8 @@ def main():
               "the batched solves would shard by case, one process per GPU, with no exchange.", file=sys.stderr)
         sys.exit(2)
 
+    if a.workload == "warmstart":
+        return run_warmstart(a)
     if a.workload != "single":
         return run_batch(a)
 
@@ -205,14 +212,76 @@ def _cpu_batch_job(args):
     return grid.ni * grid.nj * steps, time.perf_counter() - t
 
 
-def run_batch(a):
+def _setup_cases(cases, implicit):
+    """Scheme, stopping plan, free streams, boundaries and configs of the
+    accepted cases."""
+    from bowshock_amd import grids
+    from bowshock_amd.batch import SolvePlan
+    from bowshock_amd.engine import Euler2DConfig
+
+    ok = [c for c in cases if c.grid is not None]
+    if not implicit:
+        scheme = dict(flux="hllc", limiter="minmod", reconstruction="perVariable", timeIntegrator="ssprk3", cfl=0.5)
+        plan = SolvePlan(implicit=False, chunk=240, max_steps=20000)
+    else:
+        scheme = dict(flux="hllc", limiter="vanAlbada", reconstruction="sharedScaled", timeIntegrator="ssprk3",
+                      cfl=0.8)
+        plan = SolvePlan(implicit=True, chunk=60, max_steps=6000, cfl=10.0, cfl_start=1.0, sweeps=2)
+    out = dict(ok=ok, scheme=scheme, plan=plan, frees=[], p_inf=[], q_inf=[], bcs=[], configs=[])
+    for c in ok:
+        free, p, q = grids.free_stream_at(c.mach, 1.4)
+        bcs = {"iMin": {"kind": "inflow", "state": free}, "iMax": {"kind": "outflow"},
+               "jMin": {"kind": "wall"}, "jMax": {"kind": "inflow", "state": free}}
+        out["frees"].append(free)
+        out["p_inf"].append(p)
+        out["q_inf"].append(q)
+        out["bcs"].append(bcs)
+        out["configs"].append(Euler2DConfig(grid=c.grid, boundaries=bcs, geometry="axisymmetric", gamma=1.4,
+                                            timeStepping="local", **scheme))
+    return out
+
+
+def _cpu_baseline(S, implicit, seconds, cu_per_solve):
+    """The oracle on every host core, one case per worker, each for a
+    bounded number of steps; solves/s from the GPU run's cell-updates per
+    converged solve."""
     import numpy as np
     from concurrent.futures import ProcessPoolExecutor
 
-    from bowshock_amd import grids
+    ok = S["ok"]
+    workers = min(16, os.cpu_count() or 1)
+    per_cu = 1.0 / 1e6 if implicit else 1.0 / 6e5   # rough oracle seconds per cell-update, one core
+    pick = np.linspace(0, len(ok) - 1, workers).astype(int)
+    jobs = []
+    for k in pick:
+        st = max(2, int(seconds / (per_cu * ok[k].cells)))
+        jobs.append((ok[k].grid, S["bcs"][k], S["frees"][k], S["scheme"], implicit, st))
+    t = time.perf_counter()
+    with ProcessPoolExecutor(workers) as ex:
+        out = list(ex.map(_cpu_batch_job, jobs))
+    cpu_wall = time.perf_counter() - t
+    cpu_cu = sum(o[0] for o in out)
+    rate = cpu_cu / cpu_wall
+    print(f"cpu baseline {rate:.3e} CU/s on {workers} cores", file=sys.stderr, flush=True)
+    return {"value": rate / cu_per_solve, "unit": "solves/s", "cores": workers, "kind": "port",
+            "sample": f"FP64 C oracle, {workers} processes, one case each ({len(jobs)} cases spread over the "
+                      f"batch), {cpu_cu:.3e} cell-updates in {cpu_wall:.1f}s = {rate:.3e} CU/s; solves/s = "
+                      f"that rate / the GPU run's {cu_per_solve:.3e} cell-updates per converged solve"}
+
+
+def _outcomes(results):
+    reasons = {}
+    for r in results:
+        reasons[r.reason] = reasons.get(r.reason, 0) + 1
+    return reasons
+
+
+def run_batch(a):
+    import numpy as np
+
     from bowshock_amd import population as P
-    from bowshock_amd.batch import GpuBatch2D, SolvePlan, autotune_tiling, solve
-    from bowshock_amd.engine import Context, Euler2DConfig
+    from bowshock_amd.batch import GpuBatch2D, autotune_tiling, solve
+    from bowshock_amd.engine import Context
 
     sweep = a.workload == "sweep"
     cases = P.sweep_cases() if sweep else P.population_cases(a.rockets)
@@ -220,38 +289,21 @@ def run_batch(a):
     P.build_grids(cases)
     t_grids = time.time() - t0
     refusals = P.refusal_counts(cases)
-    ok = [c for c in cases if c.grid is not None]
     print(f"{len(cases)} cases, grids in {t_grids:.1f}s: {refusals}", file=sys.stderr, flush=True)
-    if sweep:
-        scheme = dict(flux="hllc", limiter="minmod", reconstruction="perVariable", timeIntegrator="ssprk3", cfl=0.5)
-        plan = SolvePlan(implicit=False, chunk=240, max_steps=20000)
-    else:
-        scheme = dict(flux="hllc", limiter="vanAlbada", reconstruction="sharedScaled", timeIntegrator="ssprk3",
-                      cfl=0.8)
-        plan = SolvePlan(implicit=True, chunk=60, max_steps=6000, cfl=10.0, cfl_start=1.0, sweeps=2)
-    frees, p_inf, q_inf, bcs_all, configs = [], [], [], [], []
-    for c in ok:
-        free, p, q = grids.free_stream_at(c.mach, 1.4)
-        bcs = {"iMin": {"kind": "inflow", "state": free}, "iMax": {"kind": "outflow"},
-               "jMin": {"kind": "wall"}, "jMax": {"kind": "inflow", "state": free}}
-        frees.append(free)
-        p_inf.append(p)
-        q_inf.append(q)
-        bcs_all.append(bcs)
-        configs.append(Euler2DConfig(grid=c.grid, boundaries=bcs, geometry="axisymmetric", gamma=1.4,
-                                     timeStepping="local", **scheme))
+    S = _setup_cases(cases, implicit=not sweep)
+    ok, plan = S["ok"], S["plan"]
     cells = np.array([c.cells for c in ok])
     ctx = Context(0)
     t0 = time.time()
-    batch = GpuBatch2D(ctx, configs)
+    batch = GpuBatch2D(ctx, S["configs"])
     t_create = time.time() - t0
-    batch.setUniform(frees)
+    batch.setUniform(S["frees"])
     tile = autotune_tiling(batch, steps=24 if sweep else 12, implicit=not sweep)
-    batch.setUniform(frees)
+    batch.setUniform(S["frees"])
     print(f"batch of {len(ok)} cases ({cells.sum():.3e} cells) created in {t_create:.1f}s; small tile below "
           f"{tile} cells", file=sys.stderr, flush=True)
     t = time.perf_counter()
-    results = solve(batch, p_inf, q_inf, 0.5, plan)
+    results = solve(batch, S["p_inf"], S["q_inf"], 0.5, plan)
     wall = time.perf_counter() - t
     steps = np.array([r.steps for r in results])
     converged = sum(r.converged for r in results)
@@ -259,36 +311,10 @@ def run_batch(a):
     solves = converged / wall
     bpc = BYTES_PER_CU_RK3 if sweep else bytes_per_cu_implicit(plan.sweeps)
     achieved = bpc * cu / wall
-    reasons = {}
-    for r in results:
-        reasons[r.reason] = reasons.get(r.reason, 0) + 1
+    reasons = _outcomes(results)
     print(f"{converged}/{len(ok)} converged in {wall:.2f}s: {solves:.2f} solves/s, {cu / wall:.3e} CU/s; "
           f"{reasons}; steps median {int(np.median(steps))}", file=sys.stderr, flush=True)
-
-    # CPU baseline: the oracle on every host core, one case per worker,
-    # each for a bounded number of steps; solves/s from the GPU run's
-    # cell-updates per converged solve.
-    cpu = None
-    if a.cpu_seconds > 0:
-        workers = min(16, os.cpu_count() or 1)
-        per_cu = 1.0 / 6e5 if sweep else 1.0 / 1e6   # rough oracle seconds per cell-update, one core
-        pick = np.linspace(0, len(ok) - 1, workers).astype(int)
-        jobs = []
-        for k in pick:
-            st = max(2, int(a.cpu_seconds / (per_cu * cells[k])))
-            jobs.append((ok[k].grid, bcs_all[k], frees[k], scheme, not sweep, st))
-        t = time.perf_counter()
-        with ProcessPoolExecutor(workers) as ex:
-            out = list(ex.map(_cpu_batch_job, jobs))
-        cpu_wall = time.perf_counter() - t
-        cpu_cu = sum(o[0] for o in out)
-        rate = cpu_cu / cpu_wall
-        cu_per_solve = cu / max(1, converged)
-        cpu = {"value": rate / cu_per_solve, "unit": "solves/s", "cores": workers, "kind": "port",
-               "sample": f"FP64 C oracle, {workers} processes, one case each ({len(jobs)} cases spread over the "
-                         f"batch), {cpu_cu:.3e} cell-updates in {cpu_wall:.1f}s = {rate:.3e} CU/s; solves/s = "
-                         f"that rate / the GPU run's {cu_per_solve:.3e} cell-updates per converged solve"}
-        print(f"cpu baseline {rate:.3e} CU/s on {workers} cores", file=sys.stderr, flush=True)
+    cpu = _cpu_baseline(S, not sweep, a.cpu_seconds, cu / max(1, converged)) if a.cpu_seconds > 0 else None
     cds = [r.cd for r in results if r.converged]
     line = {
         "metric": "Euler cell-updates/s, %HBM roofline; converged CD solves/s warm-start vs plain",
@@ -318,6 +344,153 @@ def run_batch(a):
     }
     print(json.dumps(line), flush=True)
     batch.destroy()
+
+
+def run_warmstart(a):
+    """Population solved from the free stream, the network trained on the
+    converged fields of the training rockets, then the held-out rockets'
+    cases solved twice: from the free stream and from the warm start
+    (inference, interpolation and the fallback check inside the timing)."""
+    import numpy as np
+
+    from bowshock_amd import population as P
+    from bowshock_amd import warmstart as W
+    from bowshock_amd.batch import GpuBatch2D, solve
+    from bowshock_amd.engine import Context
+
+    cases = P.population_cases(a.rockets)
+    t0 = time.time()
+    P.build_grids(cases)
+    print(f"{len(cases)} cases, grids in {time.time() - t0:.1f}s: {P.refusal_counts(cases)}", file=sys.stderr,
+          flush=True)
+    accepted = [c for c in cases if c.grid is not None]
+    held = [c for c in accepted if c.rocket.index % 4 == 3]
+    train_cases = [c for c in accepted if c.rocket.index % 4 != 3]
+    ctx = Context(0)
+
+    def features(cs):
+        return np.array([np.append(c.features(), c.cylinder) for c in cs])
+
+    def run(cs, starts=None):
+        S = _setup_cases(cs, implicit=True)
+        batch = GpuBatch2D(ctx, S["configs"])
+        t = time.perf_counter()
+        batch.setUniform(S["frees"])
+        free_norm = batch.densityResidualNorms()
+        used = np.zeros(len(cs), dtype=bool)
+        if starts is not None:
+            # Warm start, kept where its density residual beats the free stream's.
+            warm = starts()
+            for k, prim in enumerate(warm):
+                batch.setCellPrimitives(k, prim)
+            warm_norm = batch.densityResidualNorms()
+            used = warm_norm <= free_norm
+            for k in np.nonzero(~used)[0]:
+                batch.setCellPrimitives(k, np.broadcast_to(np.array(S["frees"][k]), (cs[k].cells, 4)))
+        # A case started from a predicted field starts at the final CFL
+        # number, as the reference's driver does for a given start.
+        plan = S["plan"]
+        cfl0 = np.where(used, plan.cfl, plan.cfl_start)
+        results = solve(batch, S["p_inf"], S["q_inf"], 0.5, plan, reference=free_norm, cfl_start=cfl0)
+        wall = time.perf_counter() - t
+        return S, batch, results, wall, used
+
+    def fields(batch, cs, results):
+        import math as _m
+
+        rows, targets = [], []
+        for k, (c, r) in enumerate(zip(cs, results)):
+            if not r.converged:
+                continue
+            prim = W.primitives_from_state(batch.readState(k), c.ni, c.nj)
+            targets.append(W.to_target(prim, float(np.float32(c.mach * _m.sqrt(1.4)))))
+            rows.append(k)
+        return rows, np.array(targets)
+
+    # 1. Training data: the training rockets from the free stream.
+    S, batch, res_train, wall_train, _ = run(train_cases)
+    rows, T = fields(batch, train_cases, res_train)
+    batch.destroy()
+    F = features([train_cases[k] for k in rows])
+    print(f"training set: {len(rows)}/{len(train_cases)} converged in {wall_train:.1f}s", file=sys.stderr, flush=True)
+
+    # 2. Training.
+    t = time.perf_counter()
+    ws = W.train(ctx, F, T, steps=a.train_steps, batch=256, lr=1e-3, seed=0,
+                 log=lambda s, l: print(f"  step {s} loss {l:.4e}", file=sys.stderr, flush=True))
+    t_train = time.perf_counter() - t
+    ms_gemm, flops_gemm = ws.net.time_gemm(ws.net.depth, reps=50)
+    tflops = flops_gemm / (ms_gemm * 1e-3) / 1e12
+    print(f"trained in {t_train:.1f}s; output-layer GEMM {ms_gemm * 1e3:.1f} us = {tflops:.1f} TFLOP/s",
+          file=sys.stderr, flush=True)
+
+    # 3. Held-out rockets from the free stream.
+    S_plain, batch, res_plain, wall_plain, _ = run(held)
+    rows_h, T_h = fields(batch, held, res_plain)
+    batch.destroy()
+
+    # 4. Held-out rockets from the warm start.
+    Fh = features(held)
+
+    def starts():
+        pred = ws.predict_targets(Fh)
+        return [W.from_target(pred[k], float(np.float32(c.mach * np.sqrt(1.4))), c.ni, c.nj)
+                for k, c in enumerate(held)]
+
+    S_warm, batch, res_warm, wall_warm, used = run(held, starts)
+    batch.destroy()
+    pred_h = ws.predict_targets(Fh[rows_h])
+    field_err = float(np.linalg.norm(pred_h - T_h) / np.linalg.norm(T_h))
+
+    conv_plain = sum(r.converged for r in res_plain)
+    conv_warm = sum(r.converged for r in res_warm)
+    sp, sw = conv_plain / wall_plain, conv_warm / wall_warm
+    steps_plain = np.array([r.steps for r in res_plain])
+    steps_warm = np.array([r.steps for r in res_warm])
+    both = [k for k in range(len(held)) if res_plain[k].converged and res_warm[k].converged]
+    dcd = max((abs(res_warm[k].cd - res_plain[k].cd) / abs(res_plain[k].cd) for k in both), default=float("nan"))
+    cells = np.array([c.cells for c in held])
+    cu_plain = float((cells * steps_plain).sum())
+    print(f"held-out: plain {conv_plain}/{len(held)} in {wall_plain:.2f}s ({sp:.2f}/s), warm {conv_warm}/{len(held)} "
+          f"in {wall_warm:.2f}s ({sw:.2f}/s); median steps {int(np.median(steps_plain))} -> "
+          f"{int(np.median(steps_warm))}; warm start used for {int(used.sum())}; field error {field_err:.3f}; "
+          f"max CD change {dcd:.2e}", file=sys.stderr, flush=True)
+    for m in sorted({c.mach for c in held}):
+        ks = [k for k, c in enumerate(held) if c.mach == m]
+        print(f"  M{m:g}: {len(ks)} cases, median steps plain {int(np.median(steps_plain[ks]))}, warm "
+              f"{int(np.median(steps_warm[ks]))}", file=sys.stderr, flush=True)
+    cpu = (_cpu_baseline(S_plain, True, a.cpu_seconds, cu_plain / max(1, conv_plain)) if a.cpu_seconds > 0
+           else None)
+    line = {
+        "metric": "Euler cell-updates/s, %HBM roofline; converged CD solves/s warm-start vs plain",
+        "value": sw,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": int(steps_warm.max()),
+        "warmup": 0,
+        "ms_per_step": 1e3 * wall_warm / max(1, int(steps_warm.max())),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 solver, bf16 network (fp32 accumulation)",
+        "data": "synthetic: seeded rockets (mulberry32, seed 20261020); the network trained on this run's converged "
+                "fields of the training rockets",
+        "config": {"workload": f"held-out rockets (index % 4 == 3) of a {a.rockets}-rocket population x M 1.5/2/3/4, "
+                               "LU-SGS red-black 2 sweeps; warm start from an MLP 9 -> 512 x 4 -> 4 x 80 x 40",
+                   "held_out_cases": len(held), "training_fields": len(rows), "train_steps": a.train_steps,
+                   "train_seconds": t_train, "solves_per_s_plain": sp, "solves_per_s_warm": sw,
+                   "speedup": sw / sp if sp > 0 else None, "median_steps_plain": int(np.median(steps_plain)),
+                   "median_steps_warm": int(np.median(steps_warm)), "warm_start_used": int(used.sum()),
+                   "held_out_field_rel_l2": field_err, "max_rel_cd_change_warm_vs_plain": dcd,
+                   "outcomes_plain": _outcomes(res_plain), "outcomes_warm": _outcomes(res_warm),
+                   "parallelism": "single GPU, one batch"},
+        "roofline": {"bound": "mfma", "achieved": tflops, "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": tflops / 2500.0, "traffic": None,
+                     "kernel": "gemm_bf16_kernel, output layer 256 x 512 x 12800 (forward)",
+                     "us_per_launch": ms_gemm * 1e3},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

@@ -69,6 +69,7 @@ SIGNATURES = {
     "bs_batch_set_tiling": (C.c_int, [_vp, C.c_int32]),
     "bs_batch_iterate": (C.c_int, [_vp, C.c_int32]),
     "bs_batch_set_implicit_cfl": (C.c_int, [_vp, C.c_double]),
+    "bs_batch_set_implicit_cfls": (C.c_int, [_vp, _dp]),
     "bs_batch_iterate_implicit": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_int32, C.c_double, C.c_double]),
     "bs_batch_residual_norms": (C.c_int, [_vp, _dp]),
     "bs_batch_drags": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),

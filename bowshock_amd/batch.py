@@ -76,7 +76,7 @@ class GpuBatch2D:
         ni, nj = self.shapes[index]
         out = [np.empty(ni * nj, dtype=np.float32) for _ in range(4)]
         check(N.lib().bs_batch_read_state(self._h, int(index), *[N.fptr(a) for a in out]))
-        return GpuState2D(ni, nj, *out)
+        return GpuState2D(0.0, int(self.status()[1][index]), *out)
 
     def setActive(self, mask):
         m = np.ascontiguousarray(np.asarray(mask, dtype=np.uint8))
@@ -94,6 +94,10 @@ class GpuBatch2D:
 
     def setImplicitCfl(self, cfl: float):
         check(N.lib().bs_batch_set_implicit_cfl(self._h, float(cfl)))
+
+    def setImplicitCfls(self, cfl):
+        c = np.ascontiguousarray(np.broadcast_to(np.asarray(cfl, dtype=np.float64), (self.count,)))
+        check(N.lib().bs_batch_set_implicit_cfls(self._h, N.dptr(c)))
 
     def iterateImplicit(self, steps: int, cfl: float, sweeps: int, omega: float = 1.5, growth: float = 1.2):
         check(N.lib().bs_batch_iterate_implicit(self._h, int(steps), float(cfl), int(sweeps), float(omega),
@@ -155,9 +159,14 @@ class SolvePlan:
     growth: float = 1.2
 
 
-def solve(batch: GpuBatch2D, p_inf, q_inf, base_radius, plan: Optional[SolvePlan] = None) -> List[CaseResult]:
+def solve(batch: GpuBatch2D, p_inf, q_inf, base_radius, plan: Optional[SolvePlan] = None, reference=None,
+          cfl_start=None) -> List[CaseResult]:
     """Run every case of the batch until it converges or fails; converged
-    and failed cases leave the launches at the next check."""
+    and failed cases leave the launches at the next check. ``reference``
+    (per case) replaces the first reading as the residual the ``drop`` is
+    measured from (the free stream's, so that differently started runs
+    share one target); ``cfl_start`` (per case) replaces the plan's initial
+    implicit CFL number."""
     plan = plan or SolvePlan()
     n = batch.count
     res = [CaseResult() for _ in range(n)]
@@ -166,12 +175,15 @@ def solve(batch: GpuBatch2D, p_inf, q_inf, base_radius, plan: Optional[SolvePlan
     base_radius = np.broadcast_to(np.asarray(base_radius, dtype=np.float64), (n,))
     active = np.ones(n, dtype=np.uint8)
     batch.setActive(active)
-    first = np.full(n, math.nan)
+    first = np.full(n, math.nan) if reference is None else np.array(reference, dtype=np.float64)
     best = np.full(n, math.inf)
     since_best = np.zeros(n, dtype=np.int64)
     cds: List[List[float]] = [[] for _ in range(n)]
     if plan.implicit:
-        batch.setImplicitCfl(plan.cfl_start)
+        if cfl_start is None:
+            batch.setImplicitCfl(plan.cfl_start)
+        else:
+            batch.setImplicitCfls(cfl_start)
     done = 0
     while active.any():
         if plan.implicit:

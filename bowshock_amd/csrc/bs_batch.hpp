@@ -735,6 +735,18 @@ bs_status bs_batch_set_implicit_cfl(bs_batch* b, double cfl) {
   return BS_OK;
 }
 
+bs_status bs_batch_set_implicit_cfls(bs_batch* b, const double* cfl) {
+  BATCH_GUARD(b);
+  if (!cfl) return fail(BS_ERR_RANGE, "no CFL numbers");
+  for (int c = 0; c < b->count; c++)
+    if (!(cfl[c] > 0)) return fail(BS_ERR_RANGE, "the CFL number of case %d must be positive", c);
+  std::vector<Control> ctl(b->count);
+  HIP_TRY(hipMemcpy(ctl.data(), b->control, ctl.size() * sizeof(Control), hipMemcpyDeviceToHost), "control");
+  for (int c = 0; c < b->count; c++) ctl[c].cfl = cfl[c];
+  HIP_TRY(hipMemcpy(b->control, ctl.data(), ctl.size() * sizeof(Control), hipMemcpyHostToDevice), "control");
+  return BS_OK;
+}
+
 bs_status bs_batch_iterate_implicit(bs_batch* b, int32_t steps, double cfl_max, int32_t sweeps, double omega,
                                     double growth) {
   BATCH_GUARD(b);

@@ -136,7 +136,7 @@ def resample(field: np.ndarray, ni: int, nj: int) -> np.ndarray:
     return rows[:, :, i0] * (1 - wi)[None, None, :] + rows[:, :, i1] * wi[None, None, :]
 
 
-def primitives_from_state(state, gamma: float = 1.4) -> np.ndarray:
+def primitives_from_state(state, ni: int, nj: int, gamma: float = 1.4) -> np.ndarray:
     """(4, nj, ni) primitives from a GpuState2D (pressure with the FP32
     gamma, gpuPrimitive2D, euler2d-gpu.ts:136-146)."""
     g = float(np.float32(gamma))
@@ -144,7 +144,7 @@ def primitives_from_state(state, gamma: float = 1.4) -> np.ndarray:
     u = state.mu / rho
     v = state.mv / rho
     p = (g - 1) * (state.energy - 0.5 * (state.mu.astype(np.float64) ** 2 + state.mv.astype(np.float64) ** 2) / rho)
-    return np.stack([rho, u, v, p]).reshape(4, state.nj, state.ni)
+    return np.stack([rho, u, v, p]).reshape(4, nj, ni)
 
 
 def to_target(prim: np.ndarray, u_inf: float) -> np.ndarray:

@@ -230,6 +230,9 @@ bs_status bs_batch_iterate(bs_batch *b, int32_t steps);
 /* setImplicitCfl and iterateImplicit (euler2d-gpu.ts:668-723) of every
  * active case. */
 bs_status bs_batch_set_implicit_cfl(bs_batch *b, double cfl);
+/* Per case (one value each): a case started from a converged or predicted
+ * field starts at the final CFL number (transonic-flow.ts:174). */
+bs_status bs_batch_set_implicit_cfls(bs_batch *b, const double *cfl);
 bs_status bs_batch_iterate_implicit(bs_batch *b, int32_t steps, double cfl_max, int32_t sweeps, double omega,
                                     double growth);
 /* densityResidualNorm per case (NaN for inactive cases). */
