@@ -16,6 +16,8 @@
 // partial sum of the squared density residual.
 #pragma once
 
+#include <type_traits>
+
 #include "bs_numerics.hpp"
 
 namespace bs {
@@ -159,6 +161,33 @@ __device__ __forceinline__ double block_sum(double v, double* scratch) {
   return s;
 }
 
+// Which threads of a TX x TY block compute the tile's high-edge faces. Every
+// wave computes two faces per lane (its cells' low i- and j-faces); the TX
+// j-faces of the top edge take one more wave's worth of work, the TY
+// i-faces of the right edge another. Wave w of a block runs on SIMD w % 4,
+// so each edge goes to the highest wave on the SIMD with the least work so
+// far: at 64 x 8 the two extra faces land on different SIMDs, at 64 x 7
+// both on the SIMD that holds one wave fewer.
+struct EdgePlan {
+  int jstart, istart;
+};
+
+constexpr EdgePlan edge_plan(int TX, int TY) {
+  const int waves = TX * TY / 64;
+  int load[4] = {0, 0, 0, 0};
+  for (int w = 0; w < waves; w++) load[w % 4] += 2;
+  int jw = waves - 1;
+  for (int w = waves - 1; w >= 0; w--)
+    if (load[w % 4] < load[jw % 4]) jw = w;
+  load[jw % 4] += 1;
+  int iw = waves - 1;
+  for (int w = waves - 1; w >= 0; w--)
+    if (load[w % 4] < load[iw % 4]) iw = w;
+  // In one wave, the i-edge lanes follow the j-edge lanes when they fit.
+  const int ioff = (iw == jw && TX % 64 + TY <= 64) ? TX % 64 : 0;
+  return {jw * 64, iw * 64 + ioff};
+}
+
 // The tile (bx, by) of one grid; its FP64 partial goes to A.partials[slot].
 template <int TX, int TY, int REC, int LIM, int FLUX, int MODE>
 __device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int slot) {
@@ -234,17 +263,20 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int
     return {sp[0][lj][li], sp[1][lj][li], sp[2][lj][li], sp[3][lj][li]};
   };
 
-  // Phase 2: faces. Face f of direction d at local (fx, fy).
-  auto do_face = [&](int d, int fx, int fy) {
+  // Phase 2: faces. Face f of direction d at local (fx, fy). Inner tiles
+  // (IN) have no boundary face and no face outside the grid, so their
+  // instance drops the side tests and the wall mirror.
+  auto do_face = [&](auto inner_tag, int d, int fx, int fy) {
+    constexpr bool IN = decltype(inner_tag)::value;
     const int gi = i0 + fx, gj = j0 + fy;
     float nx, ny, area;
     Vec4<float> f;
     float lam = 0.f;
     if (d == 0) {
-      if (gi > ni || gj >= nj) return;
+      if (!IN && (gi > ni || gj >= nj)) return;
       const int fidx = gi + gj * (ni + 1);
       nx = G.inx[fidx]; ny = G.iny[fidx]; area = G.ia[fidx];
-      const int side = gi == 0 ? IMIN : gi == ni ? IMAX : -1;
+      const int side = IN ? -1 : gi == 0 ? IMIN : gi == ni ? IMAX : -1;
       const Prim<float> l0 = prim_at(fy + 2, fx + 1), r0 = prim_at(fy + 2, fx + 2);
       if (side >= 0 && A.B->face_flux[side] != nullptr) {
         const float* b = A.B->face_flux[side] + 4 * gj;
@@ -271,10 +303,10 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int
         xi[1][fy][fx] = ny * area;
       }
     } else {
-      if (gi >= ni || gj > nj) return;
+      if (!IN && (gi >= ni || gj > nj)) return;
       const int fidx = gi + gj * ni;
       nx = G.jnx[fidx]; ny = G.jny[fidx]; area = G.ja[fidx];
-      const int side = gj == 0 ? JMIN : gj == nj ? JMAX : -1;
+      const int side = IN ? -1 : gj == 0 ? JMIN : gj == nj ? JMAX : -1;
       const Prim<float> l0 = prim_at(fy + 1, fx + 2), r0 = prim_at(fy + 2, fx + 2);
       if (side >= 0 && A.B->face_flux[side] != nullptr) {
         const float* b = A.B->face_flux[side] + 4 * gi;
@@ -303,10 +335,18 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int
     }
   };
 
-  do_face(0, li, lj);
-  do_face(1, li, lj);
-  if (tid < TY) do_face(0, TX, tid);
-  else if (tid < TY + TX) do_face(1, tid - TY, TY);
+  // The tile's high-edge faces (TX j-faces, TY i-faces) go to the waves
+  // that leave the SIMDs evenly loaded (edge_plan).
+  constexpr EdgePlan E = edge_plan(TX, TY);
+  static_assert(TX <= 64 && E.jstart + TX <= NT && E.istart + TY <= NT, "edge faces outside the block");
+  auto faces = [&](auto inner_tag) {
+    do_face(inner_tag, 0, li, lj);
+    do_face(inner_tag, 1, li, lj);
+    if (tid >= E.jstart && tid < E.jstart + TX) do_face(inner_tag, 1, tid - E.jstart, TY);
+    if (tid >= E.istart && tid < E.istart + TY) do_face(inner_tag, 0, TX, tid - E.istart);
+  };
+  if (inner) faces(std::integral_constant<bool, true>());
+  else faces(std::integral_constant<bool, false>());
   __syncthreads();
 
   // Phase 3: cells.

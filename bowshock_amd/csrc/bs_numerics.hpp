@@ -63,12 +63,21 @@ __device__ __forceinline__ Vec4<T> to_cons(Prim<T> w, T g) {
   return {w.rho, w.rho * w.u, w.rho * w.v, bs_div(w.p, g - T(1)) + T(0.5) * w.rho * (w.u * w.u + w.v * w.v)};
 }
 
+// minmod(a, b): 0 when the signs differ, else the argument of smaller
+// magnitude, which is the median of (a, b, 0); in FP32 one v_med3_f32.
+template <typename T> __device__ __forceinline__ T bs_minmod(T a, T b) {
+  return a * b <= T(0) ? T(0) : (bs_abs(a) < bs_abs(b) ? a : b);
+}
+template <> __device__ __forceinline__ float bs_minmod<float>(float a, float b) {
+  return __builtin_amdgcn_fmed3f(a, b, 0.f);
+}
+
 // limiters.ts:24-94.
 template <typename T>
 __device__ __forceinline__ T limit(int kind, T a, T b) {
   switch (kind) {
     case FIRST_ORDER: return T(0);
-    case MINMOD: return a * b <= T(0) ? T(0) : (bs_abs(a) < bs_abs(b) ? a : b);
+    case MINMOD: return bs_minmod(a, b);
     case VAN_LEER: return a * b <= T(0) ? T(0) : bs_div(T(2) * a * b, a + b);
     case VAN_ALBADA: {
       const T e = T(1e-6);
@@ -201,7 +210,12 @@ __device__ __forceinline__ Vec4<T> face_flux(int lim, const Prim<T>& ll, const P
                                              const Prim<T>& rr, T nx, T ny, int wall, T g) {
   Prim<T> left = reconstruct<T, REC>(lim, ll, l0, r0, T(1), g);
   Prim<T> right = reconstruct<T, REC>(lim, l0, r0, rr, T(-1), g);
-  if (!(left.rho > T(0) && left.p > T(0) && right.rho > T(0) && right.p > T(0))) {
+  // Per variable with minmod (or first order) the face value of a positive
+  // quantity lies between the cell's value and the mean of the cell and its
+  // neighbour, so positive density and pressure stay positive and the
+  // first-order fallback never triggers: the test is skipped there.
+  const bool keeps_positive = REC == PER_VARIABLE && (lim == MINMOD || lim == FIRST_ORDER);
+  if (!keeps_positive && !(left.rho > T(0) && left.p > T(0) && right.rho > T(0) && right.p > T(0))) {
     left = l0;
     right = r0;
   }
