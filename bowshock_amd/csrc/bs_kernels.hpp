@@ -214,6 +214,31 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int
   const int gi = i0 + li, gj = j0 + lj;
   const bool valid = gi < ni && gj < nj;
   Vec4<float> own = {0.f, 0.f, 0.f, 0.f};
+  // Loads that do not depend on the staged primitives are issued first, so
+  // their latency overlaps the staging and the faces: the metrics of the
+  // thread's own i- and j-face, and the cell's volume, planar area, stored
+  // time step and u0.
+  const bool has_i = gi <= ni && gj < nj, has_j = gi < ni && gj <= nj;
+  float mi[3] = {0.f, 0.f, 0.f}, mj[3] = {0.f, 0.f, 0.f};
+  if (has_i) {
+    const int f = gi + gj * (ni + 1);
+    mi[0] = G.inx[f]; mi[1] = G.iny[f]; mi[2] = G.ia[f];
+  }
+  if (has_j) {
+    const int f = gi + gj * ni;
+    mj[0] = G.jnx[f]; mj[1] = G.jny[f]; mj[2] = G.ja[f];
+  }
+  float c_vol = 1.f, c_planar = 0.f, c_dt = 0.f, c_u0[4] = {0.f, 0.f, 0.f, 0.f};
+  if (valid) {
+    const int c = gi + gj * ni;
+    c_vol = G.vol[c];
+    c_planar = G.planar[c];
+    if (MODE == TILE_STAGE && A.dt_mode == DT_READ) c_dt = A.dt[c];
+    if (MODE == TILE_STAGE && A.a != 0.f) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) c_u0[q] = A.u0.q[q][c];
+    }
+  }
   // Tiles whose whole window lies inside the grid (all but a ring of them)
   // take the direct loads; the block-uniform test costs no divergence.
   const bool inner = i0 >= 2 && j0 >= 2 && i0 + TX + 2 <= ni && j0 + TY + 2 <= nj;
@@ -266,7 +291,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int
   // Phase 2: faces. Face f of direction d at local (fx, fy). Inner tiles
   // (IN) have no boundary face and no face outside the grid, so their
   // instance drops the side tests and the wall mirror.
-  auto do_face = [&](auto inner_tag, int d, int fx, int fy) {
+  auto do_face = [&](auto inner_tag, int d, int fx, int fy, const float* pre) {
     constexpr bool IN = decltype(inner_tag)::value;
     const int gi = i0 + fx, gj = j0 + fy;
     float nx, ny, area;
@@ -275,7 +300,8 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int
     if (d == 0) {
       if (!IN && (gi > ni || gj >= nj)) return;
       const int fidx = gi + gj * (ni + 1);
-      nx = G.inx[fidx]; ny = G.iny[fidx]; area = G.ia[fidx];
+      if (pre) { nx = pre[0]; ny = pre[1]; area = pre[2]; }
+      else { nx = G.inx[fidx]; ny = G.iny[fidx]; area = G.ia[fidx]; }
       const int side = IN ? -1 : gi == 0 ? IMIN : gi == ni ? IMAX : -1;
       const Prim<float> l0 = prim_at(fy + 2, fx + 1), r0 = prim_at(fy + 2, fx + 2);
       if (side >= 0 && A.B->face_flux[side] != nullptr) {
@@ -305,7 +331,8 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int
     } else {
       if (!IN && (gi >= ni || gj > nj)) return;
       const int fidx = gi + gj * ni;
-      nx = G.jnx[fidx]; ny = G.jny[fidx]; area = G.ja[fidx];
+      if (pre) { nx = pre[0]; ny = pre[1]; area = pre[2]; }
+      else { nx = G.jnx[fidx]; ny = G.jny[fidx]; area = G.ja[fidx]; }
       const int side = IN ? -1 : gj == 0 ? JMIN : gj == nj ? JMAX : -1;
       const Prim<float> l0 = prim_at(fy + 1, fx + 2), r0 = prim_at(fy + 2, fx + 2);
       if (side >= 0 && A.B->face_flux[side] != nullptr) {
@@ -340,10 +367,10 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int
   constexpr EdgePlan E = edge_plan(TX, TY);
   static_assert(TX <= 64 && E.jstart + TX <= NT && E.istart + TY <= NT, "edge faces outside the block");
   auto faces = [&](auto inner_tag) {
-    do_face(inner_tag, 0, li, lj);
-    do_face(inner_tag, 1, li, lj);
-    if (tid >= E.jstart && tid < E.jstart + TX) do_face(inner_tag, 1, tid - E.jstart, TY);
-    if (tid >= E.istart && tid < E.istart + TY) do_face(inner_tag, 0, TX, tid - E.istart);
+    do_face(inner_tag, 0, li, lj, mi);
+    do_face(inner_tag, 1, li, lj, mj);
+    if (tid >= E.jstart && tid < E.jstart + TX) do_face(inner_tag, 1, tid - E.jstart, TY, nullptr);
+    if (tid >= E.istart && tid < E.istart + TY) do_face(inner_tag, 0, TX, tid - E.istart, nullptr);
   };
   if (inner) faces(std::integral_constant<bool, true>());
   else faces(std::integral_constant<bool, false>());
@@ -357,9 +384,9 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int
     float r[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) r[q] = fi[q][lj][li] - fi[q][lj][li + 1] + fj[q][lj][li] - fj[q][lj + 1][li];
-    const float planar = G.planar[c];
+    const float planar = c_planar;
     if (A.axisymmetric) r[2] += w.p * planar;
-    const float vol = G.vol[c];
+    const float vol = c_vol;
     if (A.partials != nullptr) {
       double rv = (double)r[0] / (double)vol;
       r2 = rv * rv;
@@ -381,7 +408,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int
       } else if (A.dt_mode == DT_GLOBAL) {
         dt = *A.dt_global;
       } else {
-        dt = A.dt[c];
+        dt = c_dt;
       }
       const float k = dt * bs_rcp(vol);
       Vec4<float> out;
@@ -394,7 +421,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int
       if (A.a != 0.f) {
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-          const float u0q = A.u0.q[q][c];
+          const float u0q = c_u0[q];
           o[q] = u0q + A.b * ((srcq[q] - u0q) + k * r[q]);
         }
       } else {
