@@ -340,7 +340,7 @@ bs_status batch_build_lists(bs_batch* b) {
     const int tx = k.tiles_x[k.cls], ty = k.tiles_y[k.cls];
     for (int by = 0; by < ty; by++)
       for (int bx = 0; bx < tx; bx++) tiles[k.cls].push_back({c, bx, by, k.slot0 + bx + by * tx});
-    const int rx = (k.ni / 2 + 1 + 63) / 64, ry = (k.nj + 3) / 4;
+    const int rx = (k.ni + RTX - 1) / RTX, ry = (k.nj + RTY - 1) / RTY;
     for (int by = 0; by < ry; by++)
       for (int bx = 0; bx < rx; bx++) relax.push_back({c, bx, by, 0});
     for (int blk = 0; blk < (k.n + 255) / 256; blk++) update.push_back({c, blk, 0, 0});
@@ -405,10 +405,16 @@ bs_status batch_encode_implicit(bs_batch* b, int sweeps) {
   bs_status st = batch_launch_tiles(b, TILE_IMPLICIT, ROLE_PREPARE + cur);
   if (st) return st;
   const RelaxArgs* rel = b->relax_args + (size_t)cur * b->count;
-  for (int sw = 0; sw < sweeps; sw++)
-    for (int colour = (sw == 0 ? 1 : 0); colour < 2; colour++)
-      if (b->n_relax)
-        hipLaunchKernelGGL(batch_relax_kernel, dim3(b->n_relax), dim3(64, 4), 0, b->stream, rel, b->work_relax, colour);
+  // The half-sweeps after the prepare's red one, up to three per launch.
+  for (int done = 0, colour = 1, halves = 2 * sweeps - 1; done < halves && b->n_relax;) {
+    const int nh = halves - done < 3 ? halves - done : 3;
+    const dim3 grid(b->n_relax), block(RTX * RTY);
+    if (nh == 3) hipLaunchKernelGGL(batch_relax_tile_kernel<3>, grid, block, 0, b->stream, rel, b->work_relax, colour);
+    else if (nh == 2) hipLaunchKernelGGL(batch_relax_tile_kernel<2>, grid, block, 0, b->stream, rel, b->work_relax, colour);
+    else hipLaunchKernelGGL(batch_relax_tile_kernel<1>, grid, block, 0, b->stream, rel, b->work_relax, colour);
+    colour = (colour + nh) & 1;
+    done += nh;
+  }
   if (b->n_update)
     hipLaunchKernelGGL(batch_update_kernel, dim3(b->n_update), dim3(256), 0, b->stream,
                        b->update_args + (size_t)cur * b->count, b->work_update);

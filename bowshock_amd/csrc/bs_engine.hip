@@ -1027,14 +1027,19 @@ bs_status bs_iterate_implicit(bs_solver* s, int32_t count, double cfl_max, int32
     R.lam_j = s->d.lam_j;
     R.gamma = s->gamma32;
     R.omega = (float)omega;
-    dim3 rgrid((s->ni / 2 + 1 + 63) / 64, (s->nj + 3) / 4);
-    for (int sw = 0; sw < sweeps; sw++) {
-      for (int colour = (sw == 0 ? 1 : 0); colour < 2; colour++) {
-        R.colour = colour;
-        hipEvent_t* ev = timing_begin(s, 2);
-        hipLaunchKernelGGL(relax_kernel, rgrid, dim3(64, 4), 0, s->stream, R);
-        timing_end(s, ev);
-      }
+    // The 2 sweeps - 1 half-sweeps after the prepare's first red one, up
+    // to three per launch (relax_tile_kernel), starting with black.
+    dim3 rgrid((s->ni + RTX - 1) / RTX, (s->nj + RTY - 1) / RTY);
+    R.colour = 0;
+    for (int done = 0, colour = 1, halves = 2 * sweeps - 1; done < halves;) {
+      const int nh = halves - done < 3 ? halves - done : 3;
+      hipEvent_t* ev = timing_begin(s, 2);
+      if (nh == 3) hipLaunchKernelGGL(relax_tile_kernel<3>, rgrid, dim3(RTX * RTY), 0, s->stream, R, colour);
+      else if (nh == 2) hipLaunchKernelGGL(relax_tile_kernel<2>, rgrid, dim3(RTX * RTY), 0, s->stream, R, colour);
+      else hipLaunchKernelGGL(relax_tile_kernel<1>, rgrid, dim3(RTX * RTY), 0, s->stream, R, colour);
+      timing_end(s, ev);
+      colour = (colour + nh) & 1;
+      done += nh;
     }
     hipEvent_t* ev = timing_begin(s, 3);
     UpdateArgs U;

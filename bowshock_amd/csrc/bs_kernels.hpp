@@ -640,6 +640,144 @@ __global__ __launch_bounds__(256) void batch_relax_kernel(const RelaxArgs* __res
   relax_body(args[w.args], w.bx, w.by, colour);
 }
 
+// NH consecutive half-sweeps (colours colour0, colour0 ^ 1, ...) of a
+// 64 x 8 tile in one launch. Gauss-Seidel by colours: a cell's update reads
+// only cells of the other colour, so half-sweep s is exact on the tile grown
+// by NH - 1 - s cells, given q and dQ on the tile grown by NH cells; those
+// are staged in LDS once and the tile's dQ written back at the end. The
+// cell arithmetic is relax_body's (neighbour_term in the same order), so
+// the result is bitwise that of NH single half-sweeps; the grid's own
+// faces carry no neighbour term, as there.
+constexpr int RTX = 64, RTY = 8;
+
+struct RelaxCtx {
+  int i0, j0;  // global index of LDS window cell (0, 0)
+};
+
+template <int NH>
+__device__ __forceinline__ void relax_tile_body(const RelaxArgs& A, int bx, int by, int colour0) {
+  constexpr int WX = RTX + 2 * NH, WY = RTY + 2 * NH, NT = RTX * RTY;
+  // Window cells [0, WX) x [0, WY) hold q and dQ; the cells updated by some
+  // half-sweep lie in [1, WX - 1) x [1, WY - 1), and their residual,
+  // diagonal and faces (normal, area, lambda A) are staged too, so the
+  // half-sweeps read LDS only.
+  constexpr int CX = WX - 2, CY = WY - 2;
+  __shared__ float sq[4][WY][WX];
+  __shared__ float sdq[4][WY][WX];
+  __shared__ float sres[4][CY][CX];
+  __shared__ float sdiag[CY][CX];
+  __shared__ float sfi[4][CY][CX + 1];  // i-face x (between cells x - 1 and x) of row y: [y - 1][x - 1]
+  __shared__ float sfj[4][CY + 1][CX];  // j-face y of column x: [y - 1][x - 1]
+  const GridDev& G = A.G;
+  const int ni = G.ni, nj = G.nj;
+  const int i0 = bx * RTX - NH, j0 = by * RTY - NH;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < WX * WY; e += NT) {
+    const int wx = e % WX, wy = e / WX;
+    const int gi = i0 + wx, gj = j0 + wy;
+    if (gi < 0 || gi >= ni || gj < 0 || gj >= nj) continue;
+    const int c = gi + gj * ni;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      sq[q][wy][wx] = A.q.q[q][c];
+      sdq[q][wy][wx] = A.dq[q][c];
+    }
+    if (wx >= 1 && wx < WX - 1 && wy >= 1 && wy < WY - 1) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) sres[q][wy - 1][wx - 1] = A.res[q][c];
+      sdiag[wy - 1][wx - 1] = A.diag[c];
+    }
+  }
+  for (int e = tid; e < (CX + 1) * CY; e += NT) {
+    const int fx = 1 + e % (CX + 1), fy = 1 + e / (CX + 1);
+    const int gi = i0 + fx, gj = j0 + fy;
+    if (gi < 0 || gi > ni || gj < 0 || gj >= nj) continue;
+    const int f = gi + gj * (ni + 1);
+    sfi[0][fy - 1][fx - 1] = G.inx[f];
+    sfi[1][fy - 1][fx - 1] = G.iny[f];
+    sfi[2][fy - 1][fx - 1] = G.ia[f];
+    sfi[3][fy - 1][fx - 1] = A.lam_i[f];
+  }
+  for (int e = tid; e < CX * (CY + 1); e += NT) {
+    const int fx = 1 + e % CX, fy = 1 + e / CX;
+    const int gi = i0 + fx, gj = j0 + fy;
+    if (gi < 0 || gi >= ni || gj < 0 || gj > nj) continue;
+    const int f = gi + gj * ni;
+    sfj[0][fy - 1][fx - 1] = G.jnx[f];
+    sfj[1][fy - 1][fx - 1] = G.jny[f];
+    sfj[2][fy - 1][fx - 1] = G.ja[f];
+    sfj[3][fy - 1][fx - 1] = A.lam_j[f];
+  }
+  __syncthreads();
+  const float g = A.gamma;
+  // neighbour_term on LDS values (same arithmetic, same order).
+  auto term = [&](int wx, int wy, float nx, float ny, float area, float lam_a, float sum[4]) {
+    const float rho = sq[0][wy][wx], mx = sq[1][wy][wx], my = sq[2][wy][wx], e = sq[3][wy][wx];
+    const float d0 = sdq[0][wy][wx], d1 = sdq[1][wy][wx], d2 = sdq[2][wy][wx], d3 = sdq[3][wy][wx];
+    const float ir0 = bs_rcp(rho);
+    const float p0 = (g - 1.f) * (e - 0.5f * (mx * mx + my * my) * ir0);
+    const float un0 = (mx * nx + my * ny) * ir0;
+    const float rho1 = rho + d0, mx1 = mx + d1, my1 = my + d2, e1 = e + d3;
+    const float ir1 = bs_rcp(rho1);
+    const float p1 = (g - 1.f) * (e1 - 0.5f * (mx1 * mx1 + my1 * my1) * ir1);
+    const float un1 = (mx1 * nx + my1 * ny) * ir1;
+    const float half = 0.5f * area, damp = 0.5f * A.omega * lam_a;
+    sum[0] += half * (rho1 * un1 - rho * un0) - damp * d0;
+    sum[1] += half * (mx1 * un1 + p1 * nx - (mx * un0 + p0 * nx)) - damp * d1;
+    sum[2] += half * (my1 * un1 + p1 * ny - (my * un0 + p0 * ny)) - damp * d2;
+    sum[3] += half * ((e1 + p1) * un1 - (e + p0) * un0) - damp * d3;
+  };
+#pragma unroll
+  for (int s = 0; s < NH; s++) {
+    const int colour = (colour0 + s) & 1;
+    const int r = NH - 1 - s;                 // halo of this half-sweep's region
+    const int rw = RTX + 2 * r, rh = RTY + 2 * r;
+    const int half_w = rw / 2;
+    for (int t = tid; t < half_w * rh; t += NT) {
+      const int row = t / half_w, k = t % half_w;
+      const int wy = NH - r + row;
+      const int gj = j0 + wy;
+      const int gx0 = i0 + NH - r;
+      const int gi = gx0 + 2 * k + ((colour - gx0 - gj) & 1);
+      const int wx = gi - i0;
+      if (gi < 0 || gi >= ni || gj < 0 || gj >= nj) continue;
+      float sum[4] = {0.f, 0.f, 0.f, 0.f};
+      const int cx = wx - 1, cy = wy - 1;
+      if (gi > 0) term(wx - 1, wy, -sfi[0][cy][cx], -sfi[1][cy][cx], sfi[2][cy][cx], sfi[3][cy][cx], sum);
+      if (gi < ni - 1)
+        term(wx + 1, wy, sfi[0][cy][cx + 1], sfi[1][cy][cx + 1], sfi[2][cy][cx + 1], sfi[3][cy][cx + 1], sum);
+      if (gj > 0) term(wx, wy - 1, -sfj[0][cy][cx], -sfj[1][cy][cx], sfj[2][cy][cx], sfj[3][cy][cx], sum);
+      if (gj < nj - 1)
+        term(wx, wy + 1, sfj[0][cy + 1][cx], sfj[1][cy + 1][cx], sfj[2][cy + 1][cx], sfj[3][cy + 1][cx], sum);
+      const float id = bs_rcp(sdiag[cy][cx]);
+#pragma unroll
+      for (int q = 0; q < 4; q++) sdq[q][wy][wx] = (sres[q][cy][cx] - sum[q]) * id;
+    }
+    __syncthreads();
+  }
+  // The tile's own cells back to global memory.
+  for (int e = tid; e < RTX * RTY; e += NT) {
+    const int lx = e % RTX, ly = e / RTX;
+    const int gi = bx * RTX + lx, gj = by * RTY + ly;
+    if (gi >= ni || gj >= nj) continue;
+    const int c = gi + gj * ni;
+#pragma unroll
+    for (int q = 0; q < 4; q++) A.dq[q][c] = sdq[q][ly + NH][lx + NH];
+  }
+}
+
+template <int NH>
+__global__ __launch_bounds__(RTX * RTY) void relax_tile_kernel(RelaxArgs A, int colour0) {
+  relax_tile_body<NH>(A, blockIdx.x, blockIdx.y, colour0);
+}
+
+template <int NH>
+__global__ __launch_bounds__(RTX * RTY) void batch_relax_tile_kernel(const RelaxArgs* __restrict__ args,
+                                                                     const WorkItem* __restrict__ work, int colour0) {
+  const WorkItem w = work[blockIdx.x];
+  relax_tile_body<NH>(args[w.args], w.bx, w.by, colour0);
+}
+
 struct UpdateArgs {
   StateDev src, dst, dq;
   int n;
