@@ -291,7 +291,6 @@ bs_status batch_upload_implicit_args(bs_batch* b, double omega, double cfl_max, 
       R.lam_j = b->lam_j + k.jface;
       R.gamma = b->gamma32;
       R.omega = (float)omega;
-      R.colour = 0;
       rel[(size_t)cur * n + c] = R;
       UpdateArgs U;
       U.src = batch_state(b, cur, k);
@@ -405,13 +404,18 @@ bs_status batch_encode_implicit(bs_batch* b, int sweeps) {
   bs_status st = batch_launch_tiles(b, TILE_IMPLICIT, ROLE_PREPARE + cur);
   if (st) return st;
   const RelaxArgs* rel = b->relax_args + (size_t)cur * b->count;
-  // The half-sweeps after the prepare's red one, up to three per launch.
+  // The first red half-sweep (formed by the first launch) and the
+  // half-sweeps after it, up to three per launch.
   for (int done = 0, colour = 1, halves = 2 * sweeps - 1; done < halves && b->n_relax;) {
     const int nh = halves - done < 3 ? halves - done : 3;
+    const int init = done == 0;
     const dim3 grid(b->n_relax), block(RTX * RTY);
-    if (nh == 3) hipLaunchKernelGGL(batch_relax_tile_kernel<3>, grid, block, 0, b->stream, rel, b->work_relax, colour);
-    else if (nh == 2) hipLaunchKernelGGL(batch_relax_tile_kernel<2>, grid, block, 0, b->stream, rel, b->work_relax, colour);
-    else hipLaunchKernelGGL(batch_relax_tile_kernel<1>, grid, block, 0, b->stream, rel, b->work_relax, colour);
+    if (nh == 3)
+      hipLaunchKernelGGL(batch_relax_tile_kernel<3>, grid, block, 0, b->stream, rel, b->work_relax, colour, init);
+    else if (nh == 2)
+      hipLaunchKernelGGL(batch_relax_tile_kernel<2>, grid, block, 0, b->stream, rel, b->work_relax, colour, init);
+    else
+      hipLaunchKernelGGL(batch_relax_tile_kernel<1>, grid, block, 0, b->stream, rel, b->work_relax, colour, init);
     colour = (colour + nh) & 1;
     done += nh;
   }

@@ -1027,16 +1027,17 @@ bs_status bs_iterate_implicit(bs_solver* s, int32_t count, double cfl_max, int32
     R.lam_j = s->d.lam_j;
     R.gamma = s->gamma32;
     R.omega = (float)omega;
-    // The 2 sweeps - 1 half-sweeps after the prepare's first red one, up
-    // to three per launch (relax_tile_kernel), starting with black.
+    // The first red half-sweep (formed from R and D by the first launch)
+    // and the 2 sweeps - 1 half-sweeps after it, up to three per launch
+    // (relax_tile_kernel), starting with black.
     dim3 rgrid((s->ni + RTX - 1) / RTX, (s->nj + RTY - 1) / RTY);
-    R.colour = 0;
     for (int done = 0, colour = 1, halves = 2 * sweeps - 1; done < halves;) {
       const int nh = halves - done < 3 ? halves - done : 3;
+      const int init = done == 0;
       hipEvent_t* ev = timing_begin(s, 2);
-      if (nh == 3) hipLaunchKernelGGL(relax_tile_kernel<3>, rgrid, dim3(RTX * RTY), 0, s->stream, R, colour);
-      else if (nh == 2) hipLaunchKernelGGL(relax_tile_kernel<2>, rgrid, dim3(RTX * RTY), 0, s->stream, R, colour);
-      else hipLaunchKernelGGL(relax_tile_kernel<1>, rgrid, dim3(RTX * RTY), 0, s->stream, R, colour);
+      if (nh == 3) hipLaunchKernelGGL(relax_tile_kernel<3>, rgrid, dim3(RTX * RTY), 0, s->stream, R, colour, init);
+      else if (nh == 2) hipLaunchKernelGGL(relax_tile_kernel<2>, rgrid, dim3(RTX * RTY), 0, s->stream, R, colour, init);
+      else hipLaunchKernelGGL(relax_tile_kernel<1>, rgrid, dim3(RTX * RTY), 0, s->stream, R, colour, init);
       timing_end(s, ev);
       colour = (colour + nh) & 1;
       done += nh;

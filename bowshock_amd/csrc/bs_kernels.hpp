@@ -137,7 +137,7 @@ struct TileArgs {
   Control* control;
   int count_step;            // this launch completes a step: control->steps++
   double* partials;          // per block: sum of (R_rho / V)^2, or null
-  // implicit prepare: residual, diagonal, first red half-sweep
+  // implicit prepare: residual, diagonal, lambda A per face
   float* res[4];
   float* diag;
   float* lam_i;              // lambda * area per i-face
@@ -440,19 +440,17 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int
       if (!is_physical(out, g)) atomicOr(&A.control->status, STATUS_NON_PHYSICAL);
     } else if (MODE == TILE_IMPLICIT) {
       // euler2d.ts:955-971 / euler2d-implicit.wgsl implicit_prepare: the
-      // scalar diagonal and, with dQ = 0, the first red half-sweep
+      // scalar diagonal and the residual
       // dQ = R / D on the cells with i + j even.
       const double cfl = A.control->cfl;
       const float scale = (float)(0.5 * ((double)A.omega + 1.0 / cfl));
       float d = scale * (xi[0][lj][li] + xi[0][lj][li + 1] + xj[0][lj][li] + xj[0][lj + 1][li]);
       if (A.axisymmetric) d += planar * (bs_abs(w.v) + bs_sqrt(bs_div(g * w.p, w.rho)));
       A.diag[c] = d;
-      const bool red = ((gi + gj) & 1) == 0;
+      // The first red half-sweep (dQ = R / D on red cells, 0 on black) is
+      // formed by the relaxation's first launch from R and D.
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        A.res[q][c] = r[q];
-        A.dq[q][c] = red ? bs_div(r[q], d) : 0.f;
-      }
+      for (int q = 0; q < 4; q++) A.res[q][c] = r[q];
     } else if (A.rvec[0] != nullptr) {
 #pragma unroll
       for (int q = 0; q < 4; q++) A.rvec[q][c] = r[q];
@@ -581,64 +579,7 @@ struct RelaxArgs {
   const float* lam_j;
   float* dq[4];
   float gamma, omega;
-  int colour;
 };
-
-__device__ __forceinline__ void neighbour_term(const RelaxArgs& A, int nb, float nx, float ny, float area,
-                                               float lam_a, float sum[4]) {
-  const float g = A.gamma;
-  const float rho = A.q.q[0][nb], mx = A.q.q[1][nb], my = A.q.q[2][nb], e = A.q.q[3][nb];
-  const float d0 = A.dq[0][nb], d1 = A.dq[1][nb], d2 = A.dq[2][nb], d3 = A.dq[3][nb];
-  const float ir0 = bs_rcp(rho);
-  const float p0 = (g - 1.f) * (e - 0.5f * (mx * mx + my * my) * ir0);
-  const float un0 = (mx * nx + my * ny) * ir0;
-  const float rho1 = rho + d0, mx1 = mx + d1, my1 = my + d2, e1 = e + d3;
-  const float ir1 = bs_rcp(rho1);
-  const float p1 = (g - 1.f) * (e1 - 0.5f * (mx1 * mx1 + my1 * my1) * ir1);
-  const float un1 = (mx1 * nx + my1 * ny) * ir1;
-  const float half = 0.5f * area, damp = 0.5f * A.omega * lam_a;
-  sum[0] += half * (rho1 * un1 - rho * un0) - damp * d0;
-  sum[1] += half * (mx1 * un1 + p1 * nx - (mx * un0 + p0 * nx)) - damp * d1;
-  sum[2] += half * (my1 * un1 + p1 * ny - (my * un0 + p0 * ny)) - damp * d2;
-  sum[3] += half * ((e1 + p1) * un1 - (e + p0) * un0) - damp * d3;
-}
-
-// One half-sweep over the cells with (i + j) % 2 == colour; thread x of a
-// row handles the x-th cell of that colour in the row; a block covers 64
-// cells of the colour along i by 4 rows.
-__device__ __forceinline__ void relax_body(const RelaxArgs& A, int bx, int by, int colour) {
-  const int ni = A.G.ni, nj = A.G.nj;
-  const int j = by * 4 + threadIdx.y;
-  const int i = 2 * (bx * 64 + threadIdx.x) + ((j + colour) & 1);
-  if (i >= ni || j >= nj) return;
-  const GridDev& G = A.G;
-  const int c = i + j * ni;
-  float sum[4] = {0.f, 0.f, 0.f, 0.f};
-  if (i > 0) {
-    int f = i + j * (ni + 1);
-    neighbour_term(A, c - 1, -G.inx[f], -G.iny[f], G.ia[f], A.lam_i[f], sum);
-  }
-  if (i < ni - 1) {
-    int f = i + 1 + j * (ni + 1);
-    neighbour_term(A, c + 1, G.inx[f], G.iny[f], G.ia[f], A.lam_i[f], sum);
-  }
-  if (j > 0) neighbour_term(A, c - ni, -G.jnx[c], -G.jny[c], G.ja[c], A.lam_j[c], sum);
-  if (j < nj - 1) {
-    int f = c + ni;
-    neighbour_term(A, c + ni, G.jnx[f], G.jny[f], G.ja[f], A.lam_j[f], sum);
-  }
-  const float id = bs_rcp(A.diag[c]);
-#pragma unroll
-  for (int q = 0; q < 4; q++) A.dq[q][c] = (A.res[q][c] - sum[q]) * id;
-}
-
-__global__ __launch_bounds__(256) void relax_kernel(RelaxArgs A) { relax_body(A, blockIdx.x, blockIdx.y, A.colour); }
-
-__global__ __launch_bounds__(256) void batch_relax_kernel(const RelaxArgs* __restrict__ args,
-                                                          const WorkItem* __restrict__ work, int colour) {
-  const WorkItem w = work[blockIdx.x];
-  relax_body(args[w.args], w.bx, w.by, colour);
-}
 
 // NH consecutive half-sweeps (colours colour0, colour0 ^ 1, ...) of a
 // 64 x 8 tile in one launch. Gauss-Seidel by colours: a cell's update reads
@@ -655,17 +596,19 @@ struct RelaxCtx {
 };
 
 template <int NH>
-__device__ __forceinline__ void relax_tile_body(const RelaxArgs& A, int bx, int by, int colour0) {
+__device__ __forceinline__ void relax_tile_body(const RelaxArgs& A, int bx, int by, int colour0, int init_red) {
   constexpr int WX = RTX + 2 * NH, WY = RTY + 2 * NH, NT = RTX * RTY;
-  // Window cells [0, WX) x [0, WY) hold q and dQ; the cells updated by some
-  // half-sweep lie in [1, WX - 1) x [1, WY - 1), and their residual,
-  // diagonal and faces (normal, area, lambda A) are staged too, so the
-  // half-sweeps read LDS only.
+  // Window cells [0, WX) x [0, WY) hold q, dQ, the residual and the
+  // diagonal; the cells updated by some half-sweep lie in [1, WX - 1) x
+  // [1, WY - 1), and their faces (normal, area, lambda A) are staged too,
+  // so the half-sweeps read LDS only. With init_red the first red
+  // half-sweep (dQ = R / D on red cells, 0 on black, as the prepare
+  // kernel defines it) is formed here instead of read.
   constexpr int CX = WX - 2, CY = WY - 2;
   __shared__ float sq[4][WY][WX];
   __shared__ float sdq[4][WY][WX];
-  __shared__ float sres[4][CY][CX];
-  __shared__ float sdiag[CY][CX];
+  __shared__ float sres[4][WY][WX];
+  __shared__ float sdiag[WY][WX];
   __shared__ float sfi[4][CY][CX + 1];  // i-face x (between cells x - 1 and x) of row y: [y - 1][x - 1]
   __shared__ float sfj[4][CY + 1][CX];  // j-face y of column x: [y - 1][x - 1]
   const GridDev& G = A.G;
@@ -677,15 +620,22 @@ __device__ __forceinline__ void relax_tile_body(const RelaxArgs& A, int bx, int 
     const int gi = i0 + wx, gj = j0 + wy;
     if (gi < 0 || gi >= ni || gj < 0 || gj >= nj) continue;
     const int c = gi + gj * ni;
+    float r[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       sq[q][wy][wx] = A.q.q[q][c];
-      sdq[q][wy][wx] = A.dq[q][c];
+      r[q] = A.res[q][c];
+      sres[q][wy][wx] = r[q];
     }
-    if (wx >= 1 && wx < WX - 1 && wy >= 1 && wy < WY - 1) {
+    const float d = A.diag[c];
+    sdiag[wy][wx] = d;
+    if (init_red) {
+      const bool red = ((gi + gj) & 1) == 0;
 #pragma unroll
-      for (int q = 0; q < 4; q++) sres[q][wy - 1][wx - 1] = A.res[q][c];
-      sdiag[wy - 1][wx - 1] = A.diag[c];
+      for (int q = 0; q < 4; q++) sdq[q][wy][wx] = red ? bs_div(r[q], d) : 0.f;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; q++) sdq[q][wy][wx] = A.dq[q][c];
     }
   }
   for (int e = tid; e < (CX + 1) * CY; e += NT) {
@@ -749,9 +699,9 @@ __device__ __forceinline__ void relax_tile_body(const RelaxArgs& A, int bx, int 
       if (gj > 0) term(wx, wy - 1, -sfj[0][cy][cx], -sfj[1][cy][cx], sfj[2][cy][cx], sfj[3][cy][cx], sum);
       if (gj < nj - 1)
         term(wx, wy + 1, sfj[0][cy + 1][cx], sfj[1][cy + 1][cx], sfj[2][cy + 1][cx], sfj[3][cy + 1][cx], sum);
-      const float id = bs_rcp(sdiag[cy][cx]);
+      const float id = bs_rcp(sdiag[wy][wx]);
 #pragma unroll
-      for (int q = 0; q < 4; q++) sdq[q][wy][wx] = (sres[q][cy][cx] - sum[q]) * id;
+      for (int q = 0; q < 4; q++) sdq[q][wy][wx] = (sres[q][wy][wx] - sum[q]) * id;
     }
     __syncthreads();
   }
@@ -767,15 +717,16 @@ __device__ __forceinline__ void relax_tile_body(const RelaxArgs& A, int bx, int 
 }
 
 template <int NH>
-__global__ __launch_bounds__(RTX * RTY) void relax_tile_kernel(RelaxArgs A, int colour0) {
-  relax_tile_body<NH>(A, blockIdx.x, blockIdx.y, colour0);
+__global__ __launch_bounds__(RTX * RTY) void relax_tile_kernel(RelaxArgs A, int colour0, int init_red) {
+  relax_tile_body<NH>(A, blockIdx.x, blockIdx.y, colour0, init_red);
 }
 
 template <int NH>
 __global__ __launch_bounds__(RTX * RTY) void batch_relax_tile_kernel(const RelaxArgs* __restrict__ args,
-                                                                     const WorkItem* __restrict__ work, int colour0) {
+                                                                     const WorkItem* __restrict__ work, int colour0,
+                                                                     int init_red) {
   const WorkItem w = work[blockIdx.x];
-  relax_tile_body<NH>(args[w.args], w.bx, w.by, colour0);
+  relax_tile_body<NH>(args[w.args], w.bx, w.by, colour0, init_red);
 }
 
 struct UpdateArgs {
