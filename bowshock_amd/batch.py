@@ -159,6 +159,47 @@ class SolvePlan:
     growth: float = 1.2
 
 
+class _Tracker:
+    """The stopping rule of SolvePlan for one case."""
+
+    def __init__(self, plan: SolvePlan, reference: float = math.nan):
+        self.plan = plan
+        self.first = reference
+        self.best = math.inf
+        self.since_best = 0
+        self.cds: List[float] = []
+
+    def update(self, r: CaseResult, status: int, steps: int, residual: float, cd: float, done: int) -> bool:
+        """Record a check; True when the case is finished."""
+        plan = self.plan
+        r.steps, r.residual, r.cd = int(steps), float(residual), float(cd)
+        r.history.append(r.residual)
+        if status & STATUS_NON_PHYSICAL or not math.isfinite(r.residual) or not math.isfinite(r.cd):
+            r.reason = "non-physical state"
+            return True
+        if math.isnan(self.first):
+            self.first = r.residual
+        if r.residual < 0.9 * self.best:
+            self.best = r.residual
+            self.since_best = 0
+        else:
+            self.since_best += 1
+        self.cds.append(r.cd)
+        cds = self.cds
+        settled = len(cds) > plan.settle and all(
+            abs(cds[-1] - v) <= plan.cd_tol * abs(cds[-1]) for v in cds[-1 - plan.settle:-1])
+        small = r.residual <= plan.residual or r.residual <= plan.drop * self.first
+        stalled = self.since_best >= plan.stall
+        if settled and (small or stalled):
+            r.converged = True
+            r.reason = "converged" if small else "converged (residual floor)"
+            return True
+        if done >= plan.max_steps:
+            r.reason = "step limit"
+            return True
+        return False
+
+
 def solve(batch: GpuBatch2D, p_inf, q_inf, base_radius, plan: Optional[SolvePlan] = None, reference=None,
           cfl_start=None) -> List[CaseResult]:
     """Run every case of the batch until it converges or fails; converged
@@ -175,10 +216,8 @@ def solve(batch: GpuBatch2D, p_inf, q_inf, base_radius, plan: Optional[SolvePlan
     base_radius = np.broadcast_to(np.asarray(base_radius, dtype=np.float64), (n,))
     active = np.ones(n, dtype=np.uint8)
     batch.setActive(active)
-    first = np.full(n, math.nan) if reference is None else np.array(reference, dtype=np.float64)
-    best = np.full(n, math.inf)
-    since_best = np.zeros(n, dtype=np.int64)
-    cds: List[List[float]] = [[] for _ in range(n)]
+    refs = np.full(n, math.nan) if reference is None else np.asarray(reference, dtype=np.float64)
+    trackers = [_Tracker(plan, float(refs[c])) for c in range(n)]
     if plan.implicit:
         if cfl_start is None:
             batch.setImplicitCfl(plan.cfl_start)
@@ -196,38 +235,41 @@ def solve(batch: GpuBatch2D, p_inf, q_inf, base_radius, plan: Optional[SolvePlan
         cd = batch.forebodyDrags(p_inf, q_inf, base_radius)
         changed = False
         for c in np.nonzero(active)[0]:
-            r = res[c]
-            r.steps = int(steps[c])
-            r.residual = float(norms[c])
-            r.cd = float(cd[c])
-            r.history.append(r.residual)
-            if status[c] & STATUS_NON_PHYSICAL or not math.isfinite(r.residual) or not math.isfinite(r.cd):
-                r.reason = "non-physical state"
-            else:
-                if math.isnan(first[c]):
-                    first[c] = r.residual
-                if r.residual < 0.9 * best[c]:
-                    best[c] = r.residual
-                    since_best[c] = 0
-                else:
-                    since_best[c] += 1
-                cds[c].append(r.cd)
-                settled = len(cds[c]) > plan.settle and all(
-                    abs(cds[c][-1] - v) <= plan.cd_tol * abs(cds[c][-1]) for v in cds[c][-1 - plan.settle:-1])
-                small = r.residual <= plan.residual or r.residual <= plan.drop * first[c]
-                stalled = since_best[c] >= plan.stall
-                if settled and (small or stalled):
-                    r.converged = True
-                    r.reason = "converged" if small else "converged (residual floor)"
-                elif done >= plan.max_steps:
-                    r.reason = "step limit"
-                else:
-                    continue
-            active[c] = 0
-            changed = True
+            if trackers[c].update(res[c], int(status[c]), int(steps[c]), norms[c], cd[c], done):
+                active[c] = 0
+                changed = True
         if changed and active.any():
             batch.setActive(active)
     return res
+
+
+def solve_single(gpu, p_inf: float, q_inf: float, base_radius: float, plan: Optional[SolvePlan] = None,
+                 reference: Optional[float] = None) -> CaseResult:
+    """The same stopping rule for one GpuEuler2D solve."""
+    from .engine import NonPhysicalStateError
+
+    plan = plan or SolvePlan()
+    r = CaseResult()
+    tracker = _Tracker(plan, math.nan if reference is None else float(reference))
+    if plan.implicit:
+        gpu.setImplicitCfl(plan.cfl_start)
+    done = 0
+    while True:
+        status = 0
+        try:
+            if plan.implicit:
+                gpu.iterateImplicit(plan.chunk, cfl=plan.cfl, sweeps=plan.sweeps, omega=plan.omega,
+                                    growth=plan.growth)
+            else:
+                gpu.iterate(plan.chunk)
+        except NonPhysicalStateError:
+            status = STATUS_NON_PHYSICAL
+        done += plan.chunk
+        rep = gpu.last_report
+        residual = gpu.densityResidualNorm() if not status else math.nan
+        cd = gpu.forebodyDrag(p_inf, q_inf, base_radius) if not status else math.nan
+        if tracker.update(r, status, rep.steps, residual, cd, done):
+            return r
 
 
 def _cache_path(path: Optional[str]) -> str:
