@@ -177,9 +177,15 @@ constexpr EdgePlan edge_plan(int TX, int TY) {
   int load[4] = {0, 0, 0, 0};
   for (int w = 0; w < waves; w++) load[w % 4] += 2;
   int jw = waves - 1;
-  for (int w = waves - 1; w >= 0; w--)
-    if (load[w % 4] < load[jw % 4]) jw = w;
-  load[jw % 4] += 1;
+  if (TX > 64) {
+    // A j-edge wider than a wave takes the last TX / 64 waves.
+    jw = waves - (TX + 63) / 64;
+    for (int w = jw; w < waves; w++) load[w % 4] += 1;
+  } else {
+    for (int w = waves - 1; w >= 0; w--)
+      if (load[w % 4] < load[jw % 4]) jw = w;
+    load[jw % 4] += 1;
+  }
   int iw = waves - 1;
   for (int w = waves - 1; w >= 0; w--)
     if (load[w % 4] < load[iw % 4]) iw = w;
@@ -365,7 +371,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int
   // The tile's high-edge faces (TX j-faces, TY i-faces) go to the waves
   // that leave the SIMDs evenly loaded (edge_plan).
   constexpr EdgePlan E = edge_plan(TX, TY);
-  static_assert(TX <= 64 && E.jstart + TX <= NT && E.istart + TY <= NT, "edge faces outside the block");
+  static_assert(E.jstart + TX <= NT && E.istart + TY <= NT, "edge faces outside the block");
   auto faces = [&](auto inner_tag) {
     do_face(inner_tag, 0, li, lj, mi);
     do_face(inner_tag, 1, li, lj, mj);

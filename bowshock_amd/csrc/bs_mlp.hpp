@@ -40,6 +40,21 @@ struct GemmArgs {
 
 __device__ __forceinline__ float sigmoid_f(float z) { return 1.f / (1.f + __expf(-z)); }
 
+// The eight BF16 values of a 16-byte vector to eight LDS places `stride`
+// elements apart (a transposed tile), by bit fields: taking the vector's
+// address would put it in scratch memory.
+__device__ __forceinline__ void scatter8(bf16* dst, int stride, const uint4& v) {
+  unsigned short* d = (unsigned short*)dst;
+  d[0] = (unsigned short)(v.x & 0xFFFFu);
+  d[stride] = (unsigned short)(v.x >> 16);
+  d[2 * stride] = (unsigned short)(v.y & 0xFFFFu);
+  d[3 * stride] = (unsigned short)(v.y >> 16);
+  d[4 * stride] = (unsigned short)(v.z & 0xFFFFu);
+  d[5 * stride] = (unsigned short)(v.z >> 16);
+  d[6 * stride] = (unsigned short)(v.w & 0xFFFFu);
+  d[7 * stride] = (unsigned short)(v.w >> 16);
+}
+
 // A: K contiguous (A_KC) or M contiguous; B: N contiguous (B_NC) or K
 // contiguous. LDS holds A as [m][k] and B as [n][k], so both MFMA operands
 // are 16-byte LDS reads of 8 consecutive k.
@@ -62,17 +77,13 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
       *(uint4*)&As[m][kc] = *(const uint4*)(g.A + (m0 + m) * g.lda_m + (k0 + kc));
     } else {
       const int k = t >> 3, mc = (t & 7) * 8;
-      uint4 v = *(const uint4*)(g.A + (m0 + mc) + (long)(k0 + k) * g.lda_k);
-      const bf16* e = (const bf16*)&v;
-#pragma unroll
-      for (int q = 0; q < 8; q++) As[mc + q][k] = e[q];
+      const uint4 v = *(const uint4*)(g.A + (m0 + mc) + (long)(k0 + k) * g.lda_k);
+      scatter8(&As[mc][k], GK + GPAD, v);
     }
     if (B_NC) {
       const int k = t >> 3, nc = (t & 7) * 8;
-      uint4 v = *(const uint4*)(g.B + (long)(k0 + k) * g.ldb_k + (n0 + nc));
-      const bf16* e = (const bf16*)&v;
-#pragma unroll
-      for (int q = 0; q < 8; q++) Bs[nc + q][k] = e[q];
+      const uint4 v = *(const uint4*)(g.B + (long)(k0 + k) * g.ldb_k + (n0 + nc));
+      scatter8(&Bs[nc][k], GK + GPAD, v);
     } else {
       const int n = t >> 2, kc = (t & 3) * 8;
       *(uint4*)&Bs[n][kc] = *(const uint4*)(g.B + (n0 + n) * g.ldb_n + (k0 + kc));
@@ -117,6 +128,153 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
           g.Cb[o] = (bf16)d;
         }
       }
+}
+
+// The epilogue of one output element (shared by the GEMM kernels and the
+// split-K reduction).
+template <int EPI>
+__device__ __forceinline__ void gemm_store(const GemmArgs& g, long m, long n, float v) {
+  const long o = m * g.ldc + n;
+  if (EPI == EPI_STORE) {
+    g.C[o] = v;
+  } else if (EPI == EPI_BIAS) {
+    g.C[o] = v + g.bias[n];
+  } else if (EPI == EPI_BIAS_SILU) {
+    const float z = v + g.bias[n];
+    g.C[o] = z;
+    g.Cb[o] = (bf16)(z * sigmoid_f(z));
+  } else {
+    const float z = g.pre[o];
+    const float s = sigmoid_f(z);
+    const float d = v * (s * (1.f + z * (1.f - s)));
+    g.C[o] = d;
+    g.Cb[o] = (bf16)d;
+  }
+}
+
+// 128 x 128 block tile: four wavefronts of 64 x 64, each 2 x 2 blocks of
+// v_mfma_f32_32x32x16_bf16 (16 FP32 accumulators per lane per block),
+// K steps of 32 staged in LDS, the next step's operands loaded into
+// registers while the current one is multiplied. With `split` > 1 the
+// z-th block of the grid covers K range [z K / split, (z + 1) K / split)
+// and stores raw sums to ws[z] for gemm_split_reduce.
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+constexpr int BM = 128, BN = 128, BK = 32, BPAD = 8;
+
+template <int EPI, bool A_KC, bool B_NC>
+__global__ __launch_bounds__(256) void gemm128_kernel(GemmArgs g, int split, float* __restrict__ ws) {
+  __shared__ __attribute__((aligned(16))) bf16 As[BM][BK + BPAD];
+  __shared__ __attribute__((aligned(16))) bf16 Bs[BN][BK + BPAD];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const long m0 = (long)blockIdx.y * BM, n0 = (long)blockIdx.x * BN;
+  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+  const int kc = g.K / split;
+  const int kbeg = blockIdx.z * kc, kend = kbeg + kc;
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[i][j][r] = 0.f;
+  // Each thread moves two 16-byte vectors of A and two of B per K step:
+  // rows (or k) `t / 4` and `t / 4 + 64` (K-contiguous sources), or k
+  // `t / 16` and `t / 16 + 16` (transposed sources).
+  const int rq = t >> 2, kq = (t & 3) * 8;   // K-contiguous: row, k offset
+  const int kt = t >> 4, xq = (t & 15) * 8;  // transposed: k, row offset
+  const bf16* a0;
+  const bf16* a1;
+  long a_step;
+  if (A_KC) {
+    a0 = g.A + (m0 + rq) * g.lda_m + kq;
+    a1 = a0 + 64 * g.lda_m;
+    a_step = BK;
+  } else {
+    a0 = g.A + (m0 + xq) + (long)kt * g.lda_k;
+    a1 = a0 + 16 * g.lda_k;
+    a_step = (long)BK * g.lda_k;
+  }
+  const bf16* b0;
+  const bf16* b1;
+  long b_step;
+  if (B_NC) {
+    b0 = g.B + (long)kt * g.ldb_k + (n0 + xq);
+    b1 = b0 + 16 * g.ldb_k;
+    b_step = (long)BK * g.ldb_k;
+  } else {
+    b0 = g.B + (n0 + rq) * g.ldb_n + kq;
+    b1 = b0 + 64 * g.ldb_n;
+    b_step = BK;
+  }
+  const long first = kbeg / BK;
+  a0 += first * a_step;
+  a1 += first * a_step;
+  b0 += first * b_step;
+  b1 += first * b_step;
+  uint4 ra0 = *(const uint4*)a0, ra1 = *(const uint4*)a1, rb0 = *(const uint4*)b0, rb1 = *(const uint4*)b1;
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+    if (A_KC) {
+      *(uint4*)&As[rq][kq] = ra0;
+      *(uint4*)&As[rq + 64][kq] = ra1;
+    } else {
+      scatter8(&As[xq][kt], BK + BPAD, ra0);
+      scatter8(&As[xq][kt + 16], BK + BPAD, ra1);
+    }
+    if (B_NC) {
+      scatter8(&Bs[xq][kt], BK + BPAD, rb0);
+      scatter8(&Bs[xq][kt + 16], BK + BPAD, rb1);
+    } else {
+      *(uint4*)&Bs[rq][kq] = rb0;
+      *(uint4*)&Bs[rq + 64][kq] = rb1;
+    }
+    __syncthreads();
+    if (k0 + BK < kend) {
+      a0 += a_step;
+      a1 += a_step;
+      b0 += b_step;
+      b1 += b_step;
+      ra0 = *(const uint4*)a0;
+      ra1 = *(const uint4*)a1;
+      rb0 = *(const uint4*)b0;
+      rb1 = *(const uint4*)b1;
+    }
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 16) {
+      const bf16x8 fa0 = *(const bf16x8*)&As[wm + (lane & 31)][kk + 8 * (lane >> 5)];
+      const bf16x8 fa1 = *(const bf16x8*)&As[wm + 32 + (lane & 31)][kk + 8 * (lane >> 5)];
+      const bf16x8 fb0 = *(const bf16x8*)&Bs[wn + (lane & 31)][kk + 8 * (lane >> 5)];
+      const bf16x8 fb1 = *(const bf16x8*)&Bs[wn + 32 + (lane & 31)][kk + 8 * (lane >> 5)];
+      acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa0, fb0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa0, fb1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa1, fb0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa1, fb1, acc[1][1], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // Accumulator r of lane l holds row 8 (r / 4) + 4 (l / 32) + r % 4,
+  // column l % 32 of its 32 x 32 block.
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const long m = m0 + wm + 32 * i + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const long n = n0 + wn + 32 * j + (lane & 31);
+        if (split > 1) ws[(long)blockIdx.z * g.M * g.N + m * g.N + n] = acc[i][j][r];
+        else gemm_store<EPI>(g, m, n, acc[i][j][r]);
+      }
+}
+
+// Sum of the split partial products in a fixed order, then the epilogue.
+template <int EPI>
+__global__ __launch_bounds__(256) void gemm_split_reduce(GemmArgs g, int split, const float* __restrict__ ws) {
+  const long e = (long)blockIdx.x * 256 + threadIdx.x;
+  const long mn = (long)g.M * g.N;
+  if (e >= mn) return;
+  float v = 0.f;
+  for (int z = 0; z < split; z++) v += ws[z * mn + e];
+  gemm_store<EPI>(g, e / g.N, e % g.N, v);
 }
 
 // Rows idx[b] of the data set (or rows b of a staged block) into the padded
@@ -210,6 +368,8 @@ struct bs_mlp {
   float *data_x = nullptr, *data_y = nullptr;
   int data_rows = 0;
   float* stage_x = nullptr;               // predict: staged host block
+  float* ws = nullptr;                    // split-K partial products
+  long ws_floats = 0;
   long step = 0;
   std::atomic<bool> busy{false};
 };
@@ -236,7 +396,7 @@ struct MlpGuard {
 
 void mlp_free(bs_mlp* p) {
   std::vector<void*> all = {p->w, p->grad, p->m, p->v, p->wb, p->x, p->dzf[0], p->dzf[1], p->dzb[0], p->dzb[1],
-                            p->partials, p->idx, p->data_x, p->data_y, p->stage_x};
+                            p->partials, p->idx, p->data_x, p->data_y, p->stage_x, p->ws};
   for (float* q : p->z) all.push_back(q);
   for (bf16* q : p->act) all.push_back(q);
   for (void* q : all)
@@ -245,8 +405,38 @@ void mlp_free(bs_mlp* p) {
   p->stream = nullptr;
 }
 
+constexpr int MAX_SPLIT = 16;
+
+template <int EPI, bool AK, bool BN_>
+void launch_gemm128(bs_mlp* p, const GemmArgs& g) {
+  // Split K while the tile grid leaves CUs idle and each part keeps at
+  // least 256 of K.
+  const int tiles = (g.M / BM) * (g.N / BN);
+  int split = 1;
+  while (split < MAX_SPLIT && tiles * split < 512 && g.K % (2 * split * BK) == 0 && g.K / (2 * split) >= 256 &&
+         (long)2 * split * g.M * g.N <= p->ws_floats)
+    split *= 2;
+  dim3 grid(g.N / BN, g.M / BM, split);
+  hipLaunchKernelGGL((gemm128_kernel<EPI, AK, BN_>), grid, dim3(256), 0, p->stream, g, split, p->ws);
+  if (split > 1) {
+    const long mn = (long)g.M * g.N;
+    hipLaunchKernelGGL(gemm_split_reduce<EPI>, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, p->stream, g, split,
+                       (const float*)p->ws);
+  }
+}
+
 template <int EPI>
 void launch_gemm(bs_mlp* p, const GemmArgs& g, bool a_kc, bool b_nc) {
+  // The 128 x 128 tile where it fills the chip (or its K is long enough to
+  // split); otherwise the 64 x 64 tile, which gives four times the blocks.
+  const long tiles128 = (long)(g.M / BM) * (g.N / BN);
+  if (g.M % BM == 0 && g.N % BN == 0 && g.K % BK == 0 && (tiles128 >= 256 || g.K >= 4096)) {
+    if (a_kc && b_nc) launch_gemm128<EPI, true, true>(p, g);
+    else if (a_kc) launch_gemm128<EPI, true, false>(p, g);
+    else if (b_nc) launch_gemm128<EPI, false, true>(p, g);
+    else launch_gemm128<EPI, false, false>(p, g);
+    return;
+  }
   dim3 grid(g.N / GN, g.M / GM);
   if (a_kc && b_nc) hipLaunchKernelGGL((gemm_bf16_kernel<EPI, true, true>), grid, dim3(256), 0, p->stream, g);
   else if (a_kc) hipLaunchKernelGGL((gemm_bf16_kernel<EPI, true, false>), grid, dim3(256), 0, p->stream, g);
@@ -404,6 +594,9 @@ bs_status bs_mlp_create(bs_ctx* ctx, int32_t inputs, int32_t hidden, int32_t dep
       (st = dev_alloc(&p->idx, (size_t)max_batch, "indices")) ||
       (st = dev_alloc(&p->stage_x, (size_t)max_batch * inputs, "inputs")))
     return bail(st);
+  // Split-K products: up to MAX_SPLIT parts of the batch x hidden GEMMs.
+  p->ws_floats = (long)MAX_SPLIT * max_batch * hidden;
+  if ((st = dev_alloc(&p->ws, (size_t)p->ws_floats, "split-K workspace"))) return bail(st);
   for (int l = 0; l < p->layers; l++) {
     float* zl = nullptr;
     if ((st = dev_alloc(&zl, (size_t)max_batch * p->lout[l], "activations"))) return bail(st);

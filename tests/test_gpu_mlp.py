@@ -13,6 +13,11 @@ from bowshock_amd.warmstart import Mlp
 pytestmark = pytest.mark.gpu
 
 INPUTS, HIDDEN, DEPTH, OUTPUTS, BATCH = 9, 128, 2, 192, 128
+# Shapes that reach every GEMM path: the first runs on the 64 x 64 tile;
+# in the second, the output layer's weight gradient (256 x 16384, 256 tiles
+# of 128 x 128, transposed A) and its input gradient (K = 16384, split
+# along K) take the 128 x 128 tile.
+SHAPES = [(9, 128, 2, 192, 128), (9, 256, 1, 16384, 128)]
 
 
 def bf16(a):
@@ -22,20 +27,21 @@ def bf16(a):
     return r.view(np.float32).astype(np.float64)
 
 
-def _setup(ctx, seed=0):
+def _setup(ctx, seed=0, shape=SHAPES[0]):
+    inputs, hidden, depth, outputs, batch = shape
     rng = np.random.default_rng(seed)
-    net = Mlp(ctx, INPUTS, HIDDEN, DEPTH, OUTPUTS, max_batch=BATCH)
+    net = Mlp(ctx, inputs, hidden, depth, outputs, max_batch=batch)
     net.init_params(seed)
     params = net.get_params().astype(np.float64)
     # Non-zero biases so the bias path is exercised.
-    layers = M.unpack(params, INPUTS, HIDDEN, DEPTH, OUTPUTS)
+    layers = M.unpack(params, inputs, hidden, depth, outputs)
     layers = [(W, rng.normal(0, 0.1, b.shape)) for W, b in layers]
     params = M.pack(layers).astype(np.float32).astype(np.float64)
     net.set_params(params)
-    x = rng.normal(size=(300, INPUTS))
-    t = rng.normal(size=(300, OUTPUTS))
+    x = rng.normal(size=(300, inputs))
+    t = rng.normal(size=(300, outputs))
     net.load_data(x, t)
-    return net, M.unpack(params, INPUTS, HIDDEN, DEPTH, OUTPUTS), x, t, rng
+    return net, M.unpack(params, inputs, hidden, depth, outputs), x, t, rng
 
 
 def _emulated(layers, x, t):
@@ -67,8 +73,9 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def test_forward_matches_oracle(ctx):
-    net, layers, x, t, _ = _setup(ctx)
+@pytest.mark.parametrize("shape", SHAPES)
+def test_forward_matches_oracle(ctx, shape):
+    net, layers, x, t, _ = _setup(ctx, shape=shape)
     y = net.predict(x[:200]).astype(np.float64)  # 200 rows: two blocks, the second padded
     y_emu = _emulated(layers, x[:200], t[:200])[0]
     y64 = M.forward(layers, x[:200])[0]
@@ -76,8 +83,9 @@ def test_forward_matches_oracle(ctx):
     assert _rel(y, y64) < 2e-2
 
 
-def test_gradients_match_oracle(ctx):
-    net, layers, x, t, rng = _setup(ctx, 1)
+@pytest.mark.parametrize("shape", SHAPES)
+def test_gradients_match_oracle(ctx, shape):
+    net, layers, x, t, rng = _setup(ctx, 1, shape)
     idx = rng.choice(300, BATCH, replace=False)
     loss, g = net.gradients(idx)
     _, loss_emu, g_emu = _emulated(layers, x[idx], t[idx])
