@@ -110,6 +110,13 @@ __device__ __forceinline__ Vec4<float> load_cell(const StateDev& s, const GridDe
   return v;
 }
 
+// Sets the status word's non-physical bit with one atomic per wavefront
+// (the lowest flagging lane's), however many of its lanes flag a cell.
+__device__ __forceinline__ void flag_non_physical(bool bad, Control* ctl) {
+  const unsigned long long m = __ballot(bad);
+  if (bad && (int)(__lane_id()) == __ffsll((long long)m) - 1) atomicOr(&ctl->status, STATUS_NON_PHYSICAL);
+}
+
 __device__ __forceinline__ bool is_physical(const Vec4<float>& q, float g) {
   Prim<float> w = to_prim(q.x, q.y, q.z, q.w, g);
   // Comparisons with NaN are false; the bounds reject infinities
@@ -310,11 +317,11 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int
       else { nx = G.inx[fidx]; ny = G.iny[fidx]; area = G.ia[fidx]; }
       const int side = IN ? -1 : gi == 0 ? IMIN : gi == ni ? IMAX : -1;
       const Prim<float> l0 = prim_at(fy + 2, fx + 1), r0 = prim_at(fy + 2, fx + 2);
-      if (side >= 0 && A.B->face_flux[side] != nullptr) {
+      if (side >= 0 && A.B->face_flux[side < 0 ? 0 : side] != nullptr) {
         const float* b = A.B->face_flux[side] + 4 * gj;
         f = {b[0], b[1], b[2], b[3]};
       } else {
-        const int wall = side >= 0 && A.B->kind[side] == WALL ? (side == IMIN ? -1 : 1) : 0;
+        const int wall = side >= 0 && A.B->kind[side < 0 ? 0 : side] == WALL ? (side == IMIN ? -1 : 1) : 0;
         f = face_flux<float, REC, FLUX>(LIM, prim_at(fy + 2, fx), l0, r0, prim_at(fy + 2, fx + 3), nx, ny,
                                         wall, g);
       }
@@ -341,11 +348,11 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int
       else { nx = G.jnx[fidx]; ny = G.jny[fidx]; area = G.ja[fidx]; }
       const int side = IN ? -1 : gj == 0 ? JMIN : gj == nj ? JMAX : -1;
       const Prim<float> l0 = prim_at(fy + 1, fx + 2), r0 = prim_at(fy + 2, fx + 2);
-      if (side >= 0 && A.B->face_flux[side] != nullptr) {
+      if (side >= 0 && A.B->face_flux[side < 0 ? 0 : side] != nullptr) {
         const float* b = A.B->face_flux[side] + 4 * gi;
         f = {b[0], b[1], b[2], b[3]};
       } else {
-        const int wall = side >= 0 && A.B->kind[side] == WALL ? (side == JMIN ? -1 : 1) : 0;
+        const int wall = side >= 0 && A.B->kind[side < 0 ? 0 : side] == WALL ? (side == JMIN ? -1 : 1) : 0;
         f = face_flux<float, REC, FLUX>(LIM, prim_at(fy, fx + 2), l0, r0, prim_at(fy + 3, fx + 2), nx, ny,
                                         wall, g);
       }
@@ -443,7 +450,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int
 #pragma unroll
       for (int q = 0; q < 4; q++) A.dst.q[q][c] = o[q];
       out = {o[0], o[1], o[2], o[3]};
-      if (!is_physical(out, g)) atomicOr(&A.control->status, STATUS_NON_PHYSICAL);
+      flag_non_physical(!is_physical(out, g), A.control);
     } else if (MODE == TILE_IMPLICIT) {
       // euler2d.ts:955-971 / euler2d-implicit.wgsl implicit_prepare: the
       // scalar diagonal and the residual
@@ -771,7 +778,7 @@ __device__ __forceinline__ void update_body(const UpdateArgs& U, int block) {
     dst.q[1][c] = o.y;
     dst.q[2][c] = o.z;
     dst.q[3][c] = o.w;
-    if (!is_physical(o, g)) atomicOr(&ctl->status, STATUS_NON_PHYSICAL);
+    flag_non_physical(!is_physical(o, g), ctl);
   }
   if (block == 0 && threadIdx.x == 0) {
     double next = ctl->cfl * growth;
