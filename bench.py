@@ -316,8 +316,11 @@ def run_batch(a):
     bpc = BYTES_PER_CU_RK3 if sweep else bytes_per_cu_implicit(plan.sweeps)
     achieved = bpc * cu / wall
     reasons = _outcomes(results)
+    done_t = np.sort([r.seconds for r in results if r.converged])
+    pct = {f"p{q}": float(np.percentile(done_t, q)) for q in (50, 90, 99)} if len(done_t) else {}
     print(f"{converged}/{len(ok)} converged in {wall:.2f}s: {solves:.2f} solves/s, {cu / wall:.3e} CU/s; "
-          f"{reasons}; steps median {int(np.median(steps))}", file=sys.stderr, flush=True)
+          f"{reasons}; steps median {int(np.median(steps))}, p99 {int(np.percentile(steps, 99))}; "
+          f"converged by (s) {pct}", file=sys.stderr, flush=True)
     cpu = _cpu_baseline(S, not sweep, a.cpu_seconds, cu / max(1, converged)) if a.cpu_seconds > 0 else None
     cds = [r.cd for r in results if r.converged]
     line = {
@@ -339,7 +342,9 @@ def run_batch(a):
                    "cases": len(cases), "accepted": len(ok), "refusals": refusals, "converged": converged,
                    "outcomes": reasons, "cells": int(cells.sum()), "cell_updates": cu,
                    "cell_updates_per_s": cu / wall, "grid_seconds": t_grids, "create_seconds": t_create,
-                   "small_tile_below": tile, "parallelism": "single GPU, one batch"},
+                   "small_tile_below": tile, "converged_by_seconds": pct,
+                   "steps_percentiles": {f"p{q}": float(np.percentile(steps, q)) for q in (50, 90, 99, 100)},
+                   "parallelism": "single GPU, one batch"},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None, "kernel": "whole batched step (all kernels)",
                      "bytes_per_cell_update": bpc},

@@ -131,6 +131,7 @@ class CaseResult:
     converged: bool = False
     reason: str = "running"
     history: List[float] = field(default_factory=list)
+    seconds: float = math.nan  # wall time from the start of the solve to the check that finished the case
 
 
 @dataclass
@@ -224,6 +225,7 @@ def solve(batch: GpuBatch2D, p_inf, q_inf, base_radius, plan: Optional[SolvePlan
         else:
             batch.setImplicitCfls(cfl_start)
     done = 0
+    start = time.perf_counter()
     while active.any():
         if plan.implicit:
             batch.iterateImplicit(plan.chunk, plan.cfl, plan.sweeps, plan.omega, plan.growth)
@@ -234,8 +236,10 @@ def solve(batch: GpuBatch2D, p_inf, q_inf, base_radius, plan: Optional[SolvePlan
         norms = batch.densityResidualNorms()
         cd = batch.forebodyDrags(p_inf, q_inf, base_radius)
         changed = False
+        now = time.perf_counter() - start
         for c in np.nonzero(active)[0]:
             if trackers[c].update(res[c], int(status[c]), int(steps[c]), norms[c], cd[c], done):
+                res[c].seconds = now
                 active[c] = 0
                 changed = True
         if changed and active.any():

@@ -291,7 +291,7 @@ TileArgs tile_args(bs_solver* s) {
 }
 
 bs_status launch_tile(bs_solver* s, int mode, const TileArgs& A, int kind) {
-  dim3 grid(s->tiles_x, s->tiles_y);
+  dim3 grid(s->tiles_x * s->tiles_y);
   hipEvent_t* ev = timing_begin(s, kind);
   hipLaunchKernelGGL(tile_fn(s->cfg, mode), grid, dim3(TX * TY), 0, s->stream, A);
   timing_end(s, ev);
@@ -1030,7 +1030,7 @@ bs_status bs_iterate_implicit(bs_solver* s, int32_t count, double cfl_max, int32
     // The first red half-sweep (formed from R and D by the first launch)
     // and the 2 sweeps - 1 half-sweeps after it, up to three per launch
     // (relax_tile_kernel), starting with black.
-    dim3 rgrid((s->ni + RTX - 1) / RTX, (s->nj + RTY - 1) / RTY);
+    dim3 rgrid(((s->ni + RTX - 1) / RTX) * ((s->nj + RTY - 1) / RTY));
     for (int done = 0, colour = 1, halves = 2 * sweeps - 1; done < halves;) {
       const int nh = halves - done < 3 ? halves - done : 3;
       const int init = done == 0;

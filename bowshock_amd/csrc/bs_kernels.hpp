@@ -476,9 +476,23 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int
   if (A.count_step && bx == 0 && by == 0 && tid == 0) A.control->steps += 1;
 }
 
+// Workgroups go to the 8 XCDs round-robin by their linear index, each XCD
+// with its own L2. Block b becomes tile t so that every XCD gets one
+// contiguous run of tiles in row-major order: vertically adjacent tiles,
+// which share two halo rows, then run on the same XCD about one tile row
+// apart, and the second read of those rows hits its L2. A bijection of
+// [0, n) for any n.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  const int q = n >> 3, r = n & 7, x = b & 7, k = b >> 3;
+  return x * q + (x < r ? x : r) + k;
+}
+
 template <int TX, int TY, int REC, int LIM, int FLUX, int MODE>
 __global__ __launch_bounds__(TX * TY) void tile_kernel(TileArgs A) {
-  tile_body<TX, TY, REC, LIM, FLUX, MODE>(A, blockIdx.x, blockIdx.y, blockIdx.x + blockIdx.y * gridDim.x);
+  // One-dimensional grid of tiles_x * tiles_y blocks.
+  const int tiles_x = (A.G.ni + TX - 1) / TX;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  tile_body<TX, TY, REC, LIM, FLUX, MODE>(A, t % tiles_x, t / tiles_x, t);
 }
 
 // A batch of grids in one launch: block b takes work item b (the case's
@@ -493,7 +507,9 @@ struct WorkItem {
 template <int TX, int TY, int REC, int LIM, int FLUX, int MODE>
 __global__ __launch_bounds__(TX * TY) void batch_tile_kernel(const TileArgs* __restrict__ args,
                                                              const WorkItem* __restrict__ work) {
-  const WorkItem w = work[blockIdx.x];
+  // Work items are ordered by case and tile row; each XCD takes one
+  // contiguous run of them (xcd_tile).
+  const WorkItem w = work[xcd_tile(blockIdx.x, gridDim.x)];
   tile_body<TX, TY, REC, LIM, FLUX, MODE>(args[w.args], w.bx, w.by, w.slot);
 }
 
@@ -731,14 +747,17 @@ __device__ __forceinline__ void relax_tile_body(const RelaxArgs& A, int bx, int 
 
 template <int NH>
 __global__ __launch_bounds__(RTX * RTY) void relax_tile_kernel(RelaxArgs A, int colour0, int init_red) {
-  relax_tile_body<NH>(A, blockIdx.x, blockIdx.y, colour0, init_red);
+  // One-dimensional grid, tiles placed per XCD as in tile_kernel.
+  const int tiles_x = (A.G.ni + RTX - 1) / RTX;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  relax_tile_body<NH>(A, t % tiles_x, t / tiles_x, colour0, init_red);
 }
 
 template <int NH>
 __global__ __launch_bounds__(RTX * RTY) void batch_relax_tile_kernel(const RelaxArgs* __restrict__ args,
                                                                      const WorkItem* __restrict__ work, int colour0,
                                                                      int init_red) {
-  const WorkItem w = work[blockIdx.x];
+  const WorkItem w = work[xcd_tile(blockIdx.x, gridDim.x)];
   relax_tile_body<NH>(args[w.args], w.bx, w.by, colour0, init_red);
 }
 
