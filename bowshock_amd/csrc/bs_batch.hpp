@@ -141,9 +141,8 @@ struct bs_batch {
   WallArgs* wall_args = nullptr;     // 3 (cur) x count
   WorkItem* work_tiles[2] = {nullptr, nullptr};
   WorkItem* work_relax = nullptr;
-  WorkItem* work_update = nullptr;
   WorkItem* work_wall = nullptr;
-  int n_tiles[2] = {0, 0}, n_relax = 0, n_update = 0, n_wall = 0;
+  int n_tiles[2] = {0, 0}, n_relax = 0, n_wall = 0;
   int *seg_first = nullptr, *seg_count = nullptr, *seg_cells = nullptr, *seg_cases = nullptr;
   int *wall_first = nullptr, *wall_count = nullptr;
   int n_seg = 0;
@@ -199,7 +198,7 @@ void batch_free(bs_batch* b) {
   void* all[] = {b->state[0], b->state[1], b->state[2], b->inx, b->iny, b->ia, b->jnx, b->jny, b->ja, b->vol,
                  b->planar, b->dt, b->res, b->diag, b->lam_i, b->lam_j, b->dq, b->wall_geo, b->partials,
                  b->wall_partials, b->sums, b->control, b->bnd, b->wall_bnd, b->tile_args, b->relax_args,
-                 b->update_args, b->wall_args, b->work_tiles[0], b->work_tiles[1], b->work_relax, b->work_update,
+                 b->update_args, b->wall_args, b->work_tiles[0], b->work_tiles[1], b->work_relax,
                  b->work_wall, b->seg_first, b->seg_count, b->seg_cells, b->seg_cases, b->wall_first,
                  b->wall_count};
   for (void* p : all)
@@ -326,10 +325,11 @@ bs_status upload_vec(T** dst, const std::vector<T>& v, const char* what) {
 }
 
 // Work lists of the active cases: stencil tiles per class (ordered by case,
-// so neighbouring blocks share a grid), relaxation and update blocks, wall
+// so neighbouring blocks share a grid), relaxation tiles (whose last
+// launch of a step also writes the updated state), wall
 // blocks, and the segments of the per-case reductions.
 bs_status batch_build_lists(bs_batch* b) {
-  std::vector<WorkItem> tiles[2], relax, update, wall;
+  std::vector<WorkItem> tiles[2], relax, wall;
   std::vector<int> seg_cases;
   for (int c = 0; c < b->count; c++) {
     if (!b->active[c]) continue;
@@ -342,7 +342,6 @@ bs_status batch_build_lists(bs_batch* b) {
     const int rx = (k.ni + RTX - 1) / RTX, ry = (k.nj + RTY - 1) / RTY;
     for (int by = 0; by < ry; by++)
       for (int bx = 0; bx < rx; bx++) relax.push_back({c, bx, by, 0});
-    for (int blk = 0; blk < (k.n + 255) / 256; blk++) update.push_back({c, blk, 0, 0});
     for (int blk = 0; blk < k.wall_blocks; blk++) wall.push_back({c, blk, c, k.wall_slot0 + blk});
   }
   // Slots and segments depend on the class of each case.
@@ -358,7 +357,7 @@ bs_status batch_build_lists(bs_batch* b) {
   bs_status st;
   for (int cl = 0; cl < 2; cl++)
     if ((st = upload_vec(&b->work_tiles[cl], tiles[cl], "work list"))) return st;
-  if ((st = upload_vec(&b->work_relax, relax, "work list")) || (st = upload_vec(&b->work_update, update, "work list")) ||
+  if ((st = upload_vec(&b->work_relax, relax, "work list")) ||
       (st = upload_vec(&b->work_wall, wall, "work list")) || (st = upload_vec(&b->seg_first, first, "segments")) ||
       (st = upload_vec(&b->seg_count, cnt, "segments")) || (st = upload_vec(&b->seg_cells, cells, "segments")) ||
       (st = upload_vec(&b->seg_cases, seg_cases, "segments")) || (st = upload_vec(&b->wall_first, wfirst, "segments")) ||
@@ -367,7 +366,6 @@ bs_status batch_build_lists(bs_batch* b) {
   b->n_tiles[0] = (int)tiles[0].size();
   b->n_tiles[1] = (int)tiles[1].size();
   b->n_relax = (int)relax.size();
-  b->n_update = (int)update.size();
   b->n_wall = (int)wall.size();
   b->n_seg = (int)seg_cases.size();
   b->lists_dirty = false;
@@ -405,23 +403,23 @@ bs_status batch_encode_implicit(bs_batch* b, int sweeps) {
   if (st) return st;
   const RelaxArgs* rel = b->relax_args + (size_t)cur * b->count;
   // The first red half-sweep (formed by the first launch) and the
-  // half-sweeps after it, up to three per launch.
+  // half-sweeps after it, up to three per launch; the last launch writes
+  // the updated states.
+  const UpdateArgs* upd = b->update_args + (size_t)cur * b->count;
   for (int done = 0, colour = 1, halves = 2 * sweeps - 1; done < halves && b->n_relax;) {
     const int nh = halves - done < 3 ? halves - done : 3;
     const int init = done == 0;
+    const UpdateArgs* u = done + nh == halves ? upd : nullptr;
     const dim3 grid(b->n_relax), block(RTX * RTY);
     if (nh == 3)
-      hipLaunchKernelGGL(batch_relax_tile_kernel<3>, grid, block, 0, b->stream, rel, b->work_relax, colour, init);
+      hipLaunchKernelGGL(batch_relax_tile_kernel<3>, grid, block, 0, b->stream, rel, b->work_relax, colour, init, u);
     else if (nh == 2)
-      hipLaunchKernelGGL(batch_relax_tile_kernel<2>, grid, block, 0, b->stream, rel, b->work_relax, colour, init);
+      hipLaunchKernelGGL(batch_relax_tile_kernel<2>, grid, block, 0, b->stream, rel, b->work_relax, colour, init, u);
     else
-      hipLaunchKernelGGL(batch_relax_tile_kernel<1>, grid, block, 0, b->stream, rel, b->work_relax, colour, init);
+      hipLaunchKernelGGL(batch_relax_tile_kernel<1>, grid, block, 0, b->stream, rel, b->work_relax, colour, init, u);
     colour = (colour + nh) & 1;
     done += nh;
   }
-  if (b->n_update)
-    hipLaunchKernelGGL(batch_update_kernel, dim3(b->n_update), dim3(256), 0, b->stream,
-                       b->update_args + (size_t)cur * b->count, b->work_update);
   HIP_TRY(hipGetLastError(), "batch implicit launch");
   b->cur = (cur + 1) % 3;
   return BS_OK;

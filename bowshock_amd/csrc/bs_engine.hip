@@ -1027,22 +1027,6 @@ bs_status bs_iterate_implicit(bs_solver* s, int32_t count, double cfl_max, int32
     R.lam_j = s->d.lam_j;
     R.gamma = s->gamma32;
     R.omega = (float)omega;
-    // The first red half-sweep (formed from R and D by the first launch)
-    // and the 2 sweeps - 1 half-sweeps after it, up to three per launch
-    // (relax_tile_kernel), starting with black.
-    dim3 rgrid(((s->ni + RTX - 1) / RTX) * ((s->nj + RTY - 1) / RTY));
-    for (int done = 0, colour = 1, halves = 2 * sweeps - 1; done < halves;) {
-      const int nh = halves - done < 3 ? halves - done : 3;
-      const int init = done == 0;
-      hipEvent_t* ev = timing_begin(s, 2);
-      if (nh == 3) hipLaunchKernelGGL(relax_tile_kernel<3>, rgrid, dim3(RTX * RTY), 0, s->stream, R, colour, init);
-      else if (nh == 2) hipLaunchKernelGGL(relax_tile_kernel<2>, rgrid, dim3(RTX * RTY), 0, s->stream, R, colour, init);
-      else hipLaunchKernelGGL(relax_tile_kernel<1>, rgrid, dim3(RTX * RTY), 0, s->stream, R, colour, init);
-      timing_end(s, ev);
-      colour = (colour + nh) & 1;
-      done += nh;
-    }
-    hipEvent_t* ev = timing_begin(s, 3);
     UpdateArgs U;
     U.src = state_of(s, src);
     U.dst = state_of(s, dst);
@@ -1052,8 +1036,25 @@ bs_status bs_iterate_implicit(bs_solver* s, int32_t count, double cfl_max, int32
     U.control = s->d.control;
     U.cfl_max = cfl_max;
     U.growth = growth;
-    hipLaunchKernelGGL(implicit_update_kernel, dim3((s->n + 255) / 256), dim3(256), 0, s->stream, U);
-    timing_end(s, ev);
+    // The first red half-sweep (formed from R and D by the first launch)
+    // and the 2 sweeps - 1 half-sweeps after it, up to three per launch
+    // (relax_tile_kernel), starting with black; the last launch writes the
+    // updated state (q + alpha dQ) instead of dQ.
+    dim3 rgrid(((s->ni + RTX - 1) / RTX) * ((s->nj + RTY - 1) / RTY));
+    for (int done = 0, colour = 1, halves = 2 * sweeps - 1; done < halves;) {
+      const int nh = halves - done < 3 ? halves - done : 3;
+      const int init = done == 0, last = done + nh == halves;
+      hipEvent_t* ev = timing_begin(s, 2);
+      if (nh == 3)
+        hipLaunchKernelGGL(relax_tile_kernel<3>, rgrid, dim3(RTX * RTY), 0, s->stream, R, colour, init, U, last);
+      else if (nh == 2)
+        hipLaunchKernelGGL(relax_tile_kernel<2>, rgrid, dim3(RTX * RTY), 0, s->stream, R, colour, init, U, last);
+      else
+        hipLaunchKernelGGL(relax_tile_kernel<1>, rgrid, dim3(RTX * RTY), 0, s->stream, R, colour, init, U, last);
+      timing_end(s, ev);
+      colour = (colour + nh) & 1;
+      done += nh;
+    }
     HIP_TRY(hipGetLastError(), "implicit launch");
     s->cur = dst;
     return BS_OK;

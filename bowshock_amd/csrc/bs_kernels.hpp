@@ -610,6 +610,37 @@ struct RelaxArgs {
   float gamma, omega;
 };
 
+struct UpdateArgs {
+  StateDev src, dst, dq;
+  int n;
+  float gamma;
+  Control* control;
+  double cfl_max, growth;
+};
+
+// q + alpha dQ, alpha halved (at most 30 times) until density and pressure
+// stay within a factor of two (euler2d.ts:816-847).
+__device__ __forceinline__ Vec4<float> update_cell(float rho, float mx, float my, float e, float d0, float d1,
+                                                   float d2, float d3, float g) {
+  const float p = (g - 1.f) * (e - 0.5f * (mx * mx + my * my) * bs_rcp(rho));
+  float alpha = 1.f;
+  for (int tries = 0; tries < 30; tries++) {
+    float rho1 = rho + alpha * d0, mx1 = mx + alpha * d1, my1 = my + alpha * d2, e1 = e + alpha * d3;
+    float p1 = (g - 1.f) * (e1 - 0.5f * (mx1 * mx1 + my1 * my1) * bs_rcp(rho1));
+    if (rho1 > 0.5f * rho && rho1 < 2.f * rho && p1 > 0.5f * p && p1 < 2.f * p) break;
+    alpha *= 0.5f;
+  }
+  return {rho + alpha * d0, mx + alpha * d1, my + alpha * d2, e + alpha * d3};
+}
+
+// The CFL ramp and the step count, once per step (transonic-flow.ts:208).
+__device__ __forceinline__ void advance_control(const UpdateArgs& U) {
+  Control* ctl = U.control;
+  double next = ctl->cfl * U.growth;
+  ctl->cfl = next < U.cfl_max ? next : U.cfl_max;
+  ctl->steps += 1;
+}
+
 // NH consecutive half-sweeps (colours colour0, colour0 ^ 1, ...) of a
 // 64 x 8 tile in one launch. Gauss-Seidel by colours: a cell's update reads
 // only cells of the other colour, so half-sweep s is exact on the tile grown
@@ -625,7 +656,8 @@ struct RelaxCtx {
 };
 
 template <int NH>
-__device__ __forceinline__ void relax_tile_body(const RelaxArgs& A, int bx, int by, int colour0, int init_red) {
+__device__ __forceinline__ void relax_tile_body(const RelaxArgs& A, int bx, int by, int colour0, int init_red,
+                                                const UpdateArgs& U, bool fuse) {
   constexpr int WX = RTX + 2 * NH, WY = RTY + 2 * NH, NT = RTX * RTY;
   // Window cells [0, WX) x [0, WY) hold q, dQ, the residual and the
   // diagonal; the cells updated by some half-sweep lie in [1, WX - 1) x
@@ -734,85 +766,49 @@ __device__ __forceinline__ void relax_tile_body(const RelaxArgs& A, int bx, int 
     }
     __syncthreads();
   }
-  // The tile's own cells back to global memory.
+  // The tile's own cells back to global memory: dQ, or, after a step's
+  // last half-sweep (fuse), the updated state itself (update_cell), with
+  // the control advanced once.
   for (int e = tid; e < RTX * RTY; e += NT) {
     const int lx = e % RTX, ly = e / RTX;
     const int gi = bx * RTX + lx, gj = by * RTY + ly;
     if (gi >= ni || gj >= nj) continue;
     const int c = gi + gj * ni;
+    const int wx = lx + NH, wy = ly + NH;
+    if (fuse) {
+      const Vec4<float> o = update_cell(sq[0][wy][wx], sq[1][wy][wx], sq[2][wy][wx], sq[3][wy][wx], sdq[0][wy][wx],
+                                        sdq[1][wy][wx], sdq[2][wy][wx], sdq[3][wy][wx], U.gamma);
+      U.dst.q[0][c] = o.x;
+      U.dst.q[1][c] = o.y;
+      U.dst.q[2][c] = o.z;
+      U.dst.q[3][c] = o.w;
+      flag_non_physical(!is_physical(o, U.gamma), U.control);
+    } else {
 #pragma unroll
-    for (int q = 0; q < 4; q++) A.dq[q][c] = sdq[q][ly + NH][lx + NH];
+      for (int q = 0; q < 4; q++) A.dq[q][c] = sdq[q][wy][wx];
+    }
   }
+  if (fuse && bx == 0 && by == 0 && tid == 0) advance_control(U);
 }
 
 template <int NH>
-__global__ __launch_bounds__(RTX * RTY) void relax_tile_kernel(RelaxArgs A, int colour0, int init_red) {
+__global__ __launch_bounds__(RTX * RTY) void relax_tile_kernel(RelaxArgs A, int colour0, int init_red, UpdateArgs U,
+                                                             int fuse_update) {
   // One-dimensional grid, tiles placed per XCD as in tile_kernel.
   const int tiles_x = (A.G.ni + RTX - 1) / RTX;
   const int t = xcd_tile(blockIdx.x, gridDim.x);
-  relax_tile_body<NH>(A, t % tiles_x, t / tiles_x, colour0, init_red);
+  relax_tile_body<NH>(A, t % tiles_x, t / tiles_x, colour0, init_red, U, fuse_update != 0);
 }
 
 template <int NH>
 __global__ __launch_bounds__(RTX * RTY) void batch_relax_tile_kernel(const RelaxArgs* __restrict__ args,
                                                                      const WorkItem* __restrict__ work, int colour0,
-                                                                     int init_red) {
+                                                                     int init_red, const UpdateArgs* upd) {
   const WorkItem w = work[xcd_tile(blockIdx.x, gridDim.x)];
-  relax_tile_body<NH>(args[w.args], w.bx, w.by, colour0, init_red);
+  if (upd) relax_tile_body<NH>(args[w.args], w.bx, w.by, colour0, init_red, upd[w.args], true);
+  else relax_tile_body<NH>(args[w.args], w.bx, w.by, colour0, init_red, UpdateArgs{}, false);
 }
 
-struct UpdateArgs {
-  StateDev src, dst, dq;
-  int n;
-  float gamma;
-  Control* control;
-  double cfl_max, growth;
-};
-
-// q + alpha dQ, alpha halved (at most 30 times) until density and pressure
-// stay within a factor of two (euler2d.ts:816-847); the CFL ramp and the
-// step count by one thread. Block `block` covers cells 256 block ...
-__device__ __forceinline__ void update_body(const UpdateArgs& U, int block) {
-  const StateDev& src = U.src;
-  const StateDev& dst = U.dst;
-  const StateDev& dq = U.dq;
-  const float g = U.gamma;
-  Control* ctl = U.control;
-  const int n = U.n;
-  const double cfl_max = U.cfl_max, growth = U.growth;
-  const int c = block * 256 + threadIdx.x;
-  if (c < n) {
-    const float rho = src.q[0][c], mx = src.q[1][c], my = src.q[2][c], e = src.q[3][c];
-    const float d0 = dq.q[0][c], d1 = dq.q[1][c], d2 = dq.q[2][c], d3 = dq.q[3][c];
-    const float p = (g - 1.f) * (e - 0.5f * (mx * mx + my * my) * bs_rcp(rho));
-    float alpha = 1.f;
-    for (int tries = 0; tries < 30; tries++) {
-      float rho1 = rho + alpha * d0, mx1 = mx + alpha * d1, my1 = my + alpha * d2, e1 = e + alpha * d3;
-      float p1 = (g - 1.f) * (e1 - 0.5f * (mx1 * mx1 + my1 * my1) * bs_rcp(rho1));
-      if (rho1 > 0.5f * rho && rho1 < 2.f * rho && p1 > 0.5f * p && p1 < 2.f * p) break;
-      alpha *= 0.5f;
-    }
-    Vec4<float> o = {rho + alpha * d0, mx + alpha * d1, my + alpha * d2, e + alpha * d3};
-    dst.q[0][c] = o.x;
-    dst.q[1][c] = o.y;
-    dst.q[2][c] = o.z;
-    dst.q[3][c] = o.w;
-    flag_non_physical(!is_physical(o, g), ctl);
-  }
-  if (block == 0 && threadIdx.x == 0) {
-    double next = ctl->cfl * growth;
-    ctl->cfl = next < cfl_max ? next : cfl_max;
-    ctl->steps += 1;
-  }
-}
-
-__global__ __launch_bounds__(256) void implicit_update_kernel(UpdateArgs U) { update_body(U, blockIdx.x); }
-
-__global__ __launch_bounds__(256) void batch_update_kernel(const UpdateArgs* __restrict__ args,
-                                                           const WorkItem* __restrict__ work) {
-  const WorkItem w = work[blockIdx.x];
-  update_body(args[w.args], w.bx);
-}
 
 // ---------------------------------------------------------------------------
 // FP64 boundary fluxes and drag (euler2d.ts:499-525, forces.ts:19-29).

@@ -190,8 +190,9 @@ bs_status bs_enable_graphs(bs_solver *s, int32_t enable);
 
 /* Mean device time per launch of the solver's kernels since the last call,
  * for throughput reporting: kernel-name index (0 stage, 1 implicit prepare,
- * 2 implicit relax, 3 implicit update), milliseconds per launch and the
- * launches counted. Timing is taken with HIP events on the solver's stream
+ * 2 implicit relaxation, whose last launch of a step also applies the
+ * update; 3 a separate update launch, none in this build), milliseconds per
+ * launch and the launches counted. Timing is taken with HIP events on the solver's stream
  * when enabled with bs_enable_timing: enable = 1 records a pair of events
  * around every launch (steps then run without graphs); enable = 2 records
  * one pair around each batch of explicit local-time steps, which launch the
