@@ -57,7 +57,7 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=40, help="oracle steps timed for the CPU baseline (0: skip)")
     ap.add_argument("--no-graphs", action="store_true", help="launch kernel by kernel instead of HIP graphs")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "stage_kernel_1280x640_pmc.txt"))
-    ap.add_argument("--workload", choices=("single", "implicit", "sweep", "population", "warmstart", "ladder"), default="single")
+    ap.add_argument("--workload", choices=("single", "implicit", "sweep", "population", "warmstart", "ladder", "verified"), default="single")
     ap.add_argument("--train-steps", type=int, default=3000, help="warmstart: Adam steps of the network")
     ap.add_argument("--rockets", type=int, default=2000, help="population: rockets drawn (4 Mach numbers each)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="batched CPU baseline: seconds of oracle work")
@@ -87,6 +87,8 @@ def main():
         return run_ladder(a)
     if a.workload == "implicit":
         return run_implicit(a)
+    if a.workload == "verified":
+        return run_verified(a)
     if a.workload != "single":
         return run_batch(a)
 
@@ -557,6 +559,46 @@ def run_implicit(a):
                      "us_per_launch": ms_relax * 1e3},
         "cpu_baseline": None,
     }
+    print(json.dumps(line), flush=True)
+
+
+def run_verified(a):
+    """The forebody drag of the 15 and 20 deg cones (4-caliber cylinders)
+    with its measured discretisation error: the verified-drag driver on
+    nested bodyGrid families (20 cells across, refinement 1 to 8) on the
+    HIP engine."""
+    import math
+
+    from bowshock_amd import grids
+    from bowshock_amd.engine import Context
+    from bowshock_amd.verified_drag import verified_forebody_drag
+
+    ctx = Context(0)
+    rows = []
+    t_all = time.perf_counter()
+    for half_angle, mach in ((15.0, 2.0), (20.0, 3.0)):
+        body = grids.Body(grids.Nose("conical", 0.5 / math.tan(math.radians(half_angle)), 0.5), 4.0)
+        t = time.perf_counter()
+        res = verified_forebody_drag(ctx, body, mach=mach, gamma=1.4, tolerance=1e-4, max_refinement=8)
+        wall = time.perf_counter() - t
+        est = res.result.estimate if res.result else None
+        row = {"cone_deg": half_angle, "mach": mach, "kind": res.kind, "seconds": wall,
+               "levels": [[r.refinement, r.ni, r.nj, r.iterations, r.drag] for r in res.reports],
+               "estimate": est.kind if est else None, "value": est.value if est else None,
+               "extrapolated": est.extrapolated if est else None, "order": est.order if est else None,
+               "band": res.result.uncertainty if res.result else None,
+               "verified": res.result.verified if res.result else False, "reason": res.reason or
+               (est.reason if est else "")}
+        print(json.dumps(row), file=sys.stderr, flush=True)
+        rows.append(row)
+    line = {"metric": "Euler cell-updates/s, %HBM roofline; converged CD solves/s warm-start vs plain",
+            "value": len([r for r in rows if r["verified"]]) / (time.perf_counter() - t_all), "unit": "verified drags/s",
+            "n_gpus": 1, "steps": None, "warmup": 0, "ms_per_step": None, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "fp32 solve, fp64 drag",
+            "data": "synthetic: generated cone-cylinder grid families",
+            "config": {"workload": "verified forebody drag, nested grids nj = 20..160", "cases": rows,
+                       "parallelism": "single GPU"},
+            "roofline": None, "cpu_baseline": None}
     print(json.dumps(line), flush=True)
 
 
