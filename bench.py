@@ -56,7 +56,9 @@ def parse():
     ap.add_argument("--mach", type=float, default=3.0)
     ap.add_argument("--cpu-steps", type=int, default=40, help="oracle steps timed for the CPU baseline (0: skip)")
     ap.add_argument("--no-graphs", action="store_true", help="launch kernel by kernel instead of HIP graphs")
-    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "stage_kernel_1280x640_pmc.txt"))
+    ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32",
+                    help="single: the FP32 solver, or the FP64 one (state, metrics and arithmetic in FP64)")
+    ap.add_argument("--traffic", default=None, help="rocprofv3 PMC summary of the stage kernel")
     ap.add_argument("--workload", choices=("single", "implicit", "sweep", "population", "warmstart", "ladder", "verified"), default="single")
     ap.add_argument("--train-steps", type=int, default=3000, help="warmstart: Adam steps of the network")
     ap.add_argument("--rockets", type=int, default=2000, help="population: rockets drawn (4 Mach numbers each)")
@@ -109,7 +111,7 @@ def main():
     print(f"grid {a.ni}x{a.nj} built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
 
     ctx = Context(0)
-    gpu = GpuEuler2D.create(ctx, Euler2DConfig(grid=bg.grid, boundaries=bcs, **scheme))
+    gpu = GpuEuler2D.create(ctx, Euler2DConfig(grid=bg.grid, boundaries=bcs, **scheme), a.precision)
     gpu.setCellPrimitives(np.array(free))
     if a.no_graphs:
         gpu.enableGraphs(False)
@@ -134,9 +136,11 @@ def main():
     gpu.iterate(min(a.steps, 1200))
     ms_launch, launches = gpu.kernelTiming("stage")
     gpu.enableTiming(False)
-    bytes_per_launch = BYTES_PER_STAGE * cells
+    width = 2 if a.precision == "fp64" else 1   # every algorithmic byte doubles in FP64
+    bytes_per_launch = width * BYTES_PER_STAGE * cells
     achieved = bytes_per_launch / (ms_launch * 1e-3) if ms_launch > 0 else 0.0
-    traffic = read_traffic(a.traffic)
+    tag = "_fp64" if a.precision == "fp64" else ""
+    traffic = read_traffic(a.traffic or os.path.join(ROOT, "profiles", f"stage_kernel{tag}_{a.ni}x{a.nj}_pmc.txt"))
     cd = gpu.forebodyDrag(p_inf, q_inf, 0.5)
     res = gpu.densityResidualNorm()
     print(f"stage kernel {ms_launch * 1e3:.2f} us/launch over {launches} launches; "
@@ -171,15 +175,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": a.precision,
         "data": "synthetic: generated cone-cylinder grid, free-stream start",
         "config": {"workload": f"single solve, 15 deg cone + 4 cal cylinder, M{a.mach:g}, {a.ni}x{a.nj}, "
-                               "MUSCL minmod per-variable, HLLC, SSP-RK3 CFL 0.5, local dt",
+                               "MUSCL minmod per-variable, HLLC, SSP-RK3 CFL 0.5, local dt"
+                               + (", FP64 solver" if a.precision == "fp64" else ""),
                    "cells": cells, "grid": [a.ni, a.nj], "scheme": "ssprk3", "reconstruction": "perVariable/minmod",
                    "flux": "hllc", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "kernel": "tile_kernel (explicit stage)", "bytes_per_cell_update": BYTES_PER_CU_RK3,
+                     "kernel": "tile_kernel (explicit stage)", "bytes_per_cell_update": width * BYTES_PER_CU_RK3,
                      "us_per_launch": ms_launch * 1e3},
         "cpu_baseline": cpu,
         "drag": cd,

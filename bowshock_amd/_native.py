@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libbowshock.so")
 
 BS_OK, BS_ERR_RANGE, BS_ERR_NONPHYSICAL, BS_ERR_MEMORY, BS_ERR_EXECUTION, BS_ERR_PRECISION, BS_ERR_BUSY, \
     BS_ERR_UNSUPPORTED = range(8)
+BS_PRECISION_FP32, BS_PRECISION_FP64 = 0, 1
 
 # Every function include/bowshock.h declares, with its ctypes signature.
 _dp = C.POINTER(C.c_double)
@@ -41,11 +42,15 @@ SIGNATURES = {
     "bs_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
     "bs_ctx_destroy": (C.c_int, [_vp]),
     "bs_solver_create": (C.c_int, [_vp, C.POINTER(Config), C.POINTER(_vp)]),
+    "bs_solver_create_fp64": (C.c_int, [_vp, C.POINTER(Config), C.POINTER(_vp)]),
+    "bs_solver_precision": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
     "bs_solver_destroy": (C.c_int, [_vp]),
     "bs_compute_metrics": (C.c_int, [C.c_int32, C.c_int32, _dp, _dp, C.c_int32] + [_dp] * 10),
     "bs_set_primitives": (C.c_int, [_vp, _dp]),
     "bs_set_state": (C.c_int, [_vp, _fp, _fp, _fp, _fp]),
     "bs_read_state": (C.c_int, [_vp, _fp, _fp, _fp, _fp, C.POINTER(Report)]),
+    "bs_set_state_f64": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
+    "bs_read_state_f64": (C.c_int, [_vp, _dp, _dp, _dp, _dp, C.POINTER(Report)]),
     "bs_set_face_states": (C.c_int, [_vp, C.c_int32, _dp, C.c_int32]),
     "bs_set_face_fluxes": (C.c_int, [_vp, C.c_int32, _dp, C.c_int32]),
     "bs_iterate": (C.c_int, [_vp, C.c_int32, C.POINTER(Report)]),

@@ -78,7 +78,7 @@ __global__ __launch_bounds__(256) void batch_wall_kernel(const WallArgs* __restr
                                                          const WallBoundary* __restrict__ wb,
                                                          const WorkItem* __restrict__ work) {
   const WorkItem w = work[blockIdx.x];
-  wall_body<REC, FLUX>(args[w.args], wb[w.by], w.bx, w.slot);
+  wall_body<float, REC, FLUX>(args[w.args], wb[w.by], w.bx, w.slot);
 }
 
 BatchWallFn batch_wall_fn(const bs_config& c) {

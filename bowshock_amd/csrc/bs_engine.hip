@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/bowshock.h"
@@ -59,26 +60,38 @@ constexpr int GRAPH_STEPS = 12;  // steps per captured graph (a multiple of 3)
 constexpr int TX = BS_TILE_X, TY = BS_TILE_Y;
 static_assert(TX * TY <= 1024 && TX * TY >= TX + TY && (TX * TY) % 64 == 0, "tile shape");
 
+// Tile per state precision: FP64 halves the tile height so the doubled
+// register footprint of a thread still fits without spilling.
+template <typename T>
+struct Tile {
+  static constexpr int x = TX, y = TY;
+};
+template <>
+struct Tile<double> {
+  static constexpr int x = 64, y = 4;
+};
+
 }  // namespace
 
 struct bs_ctx {
   int device;
 };
 
-// Device arrays of one solver.
+// Device arrays of one solver. The untyped arrays hold values of the
+// solver's state precision (float, or double for an FP64 solver).
 struct DeviceBuffers {
-  float* state[3] = {nullptr, nullptr, nullptr};  // 4 * n each, SoA
-  float *inx = nullptr, *iny = nullptr, *ia = nullptr, *jnx = nullptr, *jny = nullptr, *ja = nullptr;
-  float *vol = nullptr, *planar = nullptr, *dt = nullptr, *dt_global = nullptr, *dt_partials = nullptr;
+  void* state[3] = {nullptr, nullptr, nullptr};  // 4 * n each, SoA
+  void *inx = nullptr, *iny = nullptr, *ia = nullptr, *jnx = nullptr, *jny = nullptr, *ja = nullptr;
+  void *vol = nullptr, *planar = nullptr, *dt = nullptr, *dt_global = nullptr, *dt_partials = nullptr;
   double* partials = nullptr;
   double* scalar = nullptr;
   Control* control = nullptr;
-  BoundaryDev* bnd = nullptr;  // the boundary table the stencil kernels read
-  float* face_cons[4] = {nullptr, nullptr, nullptr, nullptr};
-  float* face_flux[4] = {nullptr, nullptr, nullptr, nullptr};
+  void* bnd = nullptr;  // the BoundaryDevT table the stencil kernels read
+  void* face_cons[4] = {nullptr, nullptr, nullptr, nullptr};
+  void* face_flux[4] = {nullptr, nullptr, nullptr, nullptr};
   double* face_geo[4] = {nullptr, nullptr, nullptr, nullptr};
   double* wall_out = nullptr;
-  // implicit work
+  // implicit work (FP32 solvers only)
   float *res = nullptr, *diag = nullptr, *lam_i = nullptr, *lam_j = nullptr, *dq = nullptr;
 };
 
@@ -111,6 +124,7 @@ struct bs_solver {
   int device;
   int ni, nj, n;
   float gamma32;
+  bool fp64 = false;  // state, metrics and stencil arithmetic in FP64
   hipStream_t stream = nullptr;
   DeviceBuffers d;
   int cur = 0;
@@ -146,33 +160,47 @@ struct Guard {
 
 int along(const bs_solver* s, int side) { return side < 2 ? s->nj : s->ni; }
 
-StateDev state_of(bs_solver* s, int b) {
-  StateDev st;
-  for (int q = 0; q < 4; q++) st.q[q] = s->d.state[b] + (size_t)q * s->n;
+// Bytes of one state value of the solver.
+size_t real_size(const bs_solver* s) { return s->fp64 ? sizeof(double) : sizeof(float); }
+
+// The gamma the state's conserved variables are formed with: the FP32
+// gamma for an FP32 state (euler2d-gpu.ts:412-432), the configured one for
+// an FP64 state.
+double state_gamma(const bs_solver* s) { return s->fp64 ? s->cfg.gamma : (double)s->gamma32; }
+
+template <typename T>
+StateDevT<T> state_of(bs_solver* s, int b) {
+  StateDevT<T> st;
+  T* base = static_cast<T*>(s->d.state[b]);
+  for (int q = 0; q < 4; q++) st.q[q] = base + (size_t)q * s->n;
   return st;
 }
 
-GridDev grid_of(bs_solver* s) {
-  GridDev G;
+template <typename T>
+GridDevT<T> grid_of(bs_solver* s) {
+  GridDevT<T> G;
   G.ni = s->ni;
   G.nj = s->nj;
-  G.inx = s->d.inx; G.iny = s->d.iny; G.ia = s->d.ia;
-  G.jnx = s->d.jnx; G.jny = s->d.jny; G.ja = s->d.ja;
-  G.vol = s->d.vol; G.planar = s->d.planar;
+  G.inx = static_cast<const T*>(s->d.inx); G.iny = static_cast<const T*>(s->d.iny);
+  G.ia = static_cast<const T*>(s->d.ia);
+  G.jnx = static_cast<const T*>(s->d.jnx); G.jny = static_cast<const T*>(s->d.jny);
+  G.ja = static_cast<const T*>(s->d.ja);
+  G.vol = static_cast<const T*>(s->d.vol); G.planar = static_cast<const T*>(s->d.planar);
   return G;
 }
 
-BoundaryDev boundary_of(bs_solver* s) {
-  BoundaryDev B;
+template <typename T>
+BoundaryDevT<T> boundary_of(bs_solver* s) {
+  BoundaryDevT<T> B;
   for (int k = 0; k < 4; k++) {
     const bs_boundary& b = s->cfg.boundaries[k];
     B.kind[k] = b.kind;
-    B.state[k][0] = (float)b.rho;
-    B.state[k][1] = (float)b.u;
-    B.state[k][2] = (float)b.v;
-    B.state[k][3] = (float)b.p;
-    B.face_cons[k] = s->d.face_cons[k];
-    B.face_flux[k] = s->imposed[k] ? s->d.face_flux[k] : nullptr;
+    B.state[k][0] = (T)b.rho;
+    B.state[k][1] = (T)b.u;
+    B.state[k][2] = (T)b.v;
+    B.state[k][3] = (T)b.p;
+    B.face_cons[k] = static_cast<const T*>(s->d.face_cons[k]);
+    B.face_flux[k] = s->imposed[k] ? static_cast<const T*>(s->d.face_flux[k]) : nullptr;
   }
   return B;
 }
@@ -180,53 +208,67 @@ BoundaryDev boundary_of(bs_solver* s) {
 // The boundary table lives in device memory so kernels index it by a
 // runtime side; re-uploaded whenever an imposed flux is switched. The copy
 // is ordered on the solver's stream before any later launch.
-bs_status upload_boundary(bs_solver* s) {
-  BoundaryDev B = boundary_of(s);
+template <typename T>
+bs_status upload_boundary_t(bs_solver* s) {
+  BoundaryDevT<T> B = boundary_of<T>(s);
   HIP_TRY(hipMemcpyAsync(s->d.bnd, &B, sizeof B, hipMemcpyHostToDevice, s->stream), "boundary table");
   HIP_TRY(hipStreamSynchronize(s->stream), "boundary table");
   return BS_OK;
 }
 
+bs_status upload_boundary(bs_solver* s) {
+  return s->fp64 ? upload_boundary_t<double>(s) : upload_boundary_t<float>(s);
+}
+
 // ---- kernel dispatch tables -------------------------------------------------
 
-using TileFn = void (*)(TileArgs);
+template <typename T>
+using TileFnT = void (*)(TileArgsT<T>);
 
-template <int REC, int LIM, int FLUX>
-TileFn pick_mode(int mode) {
-  if (mode == TILE_STAGE) return tile_kernel<TX, TY, REC, LIM, FLUX, TILE_STAGE>;
-  if (mode == TILE_IMPLICIT) return tile_kernel<TX, TY, REC, LIM, FLUX, TILE_IMPLICIT>;
-  return tile_kernel<TX, TY, REC, LIM, FLUX, TILE_RESIDUAL>;
+// FP64 solvers take explicit steps and residuals only; their implicit
+// mode is never instantiated (the entry point refuses it first).
+template <typename T, int REC, int LIM, int FLUX>
+TileFnT<T> pick_mode(int mode) {
+  constexpr int X = Tile<T>::x, Y = Tile<T>::y;
+  if (mode == TILE_STAGE) return tile_kernel<T, X, Y, REC, LIM, FLUX, TILE_STAGE>;
+  if constexpr (std::is_same<T, float>::value)
+    if (mode == TILE_IMPLICIT) return tile_kernel<T, X, Y, REC, LIM, FLUX, TILE_IMPLICIT>;
+  return tile_kernel<T, X, Y, REC, LIM, FLUX, TILE_RESIDUAL>;
 }
 
-template <int FLUX>
-TileFn pick_rec(int rec, int lim, int mode) {
-  if (rec == SHARED) return pick_mode<SHARED, VAN_ALBADA, FLUX>(mode);
-  if (rec == SHARED_SCALED) return pick_mode<SHARED_SCALED, VAN_ALBADA, FLUX>(mode);
+template <typename T, int FLUX>
+TileFnT<T> pick_rec(int rec, int lim, int mode) {
+  if (rec == SHARED) return pick_mode<T, SHARED, VAN_ALBADA, FLUX>(mode);
+  if (rec == SHARED_SCALED) return pick_mode<T, SHARED_SCALED, VAN_ALBADA, FLUX>(mode);
   switch (lim) {
-    case FIRST_ORDER: return pick_mode<PER_VARIABLE, FIRST_ORDER, FLUX>(mode);
-    case MINMOD: return pick_mode<PER_VARIABLE, MINMOD, FLUX>(mode);
-    case VAN_LEER: return pick_mode<PER_VARIABLE, VAN_LEER, FLUX>(mode);
-    case VAN_ALBADA: return pick_mode<PER_VARIABLE, VAN_ALBADA, FLUX>(mode);
-    case MONOTONIZED_CENTRAL: return pick_mode<PER_VARIABLE, MONOTONIZED_CENTRAL, FLUX>(mode);
-    default: return pick_mode<PER_VARIABLE, UNLIMITED_CENTRAL, FLUX>(mode);
+    case FIRST_ORDER: return pick_mode<T, PER_VARIABLE, FIRST_ORDER, FLUX>(mode);
+    case MINMOD: return pick_mode<T, PER_VARIABLE, MINMOD, FLUX>(mode);
+    case VAN_LEER: return pick_mode<T, PER_VARIABLE, VAN_LEER, FLUX>(mode);
+    case VAN_ALBADA: return pick_mode<T, PER_VARIABLE, VAN_ALBADA, FLUX>(mode);
+    case MONOTONIZED_CENTRAL: return pick_mode<T, PER_VARIABLE, MONOTONIZED_CENTRAL, FLUX>(mode);
+    default: return pick_mode<T, PER_VARIABLE, UNLIMITED_CENTRAL, FLUX>(mode);
   }
 }
 
-TileFn tile_fn(const bs_config& c, int mode) {
-  return c.flux == BS_FLUX_RUSANOV ? pick_rec<RUSANOV>(c.reconstruction, c.limiter, mode)
-                                   : pick_rec<HLLC>(c.reconstruction, c.limiter, mode);
+template <typename T>
+TileFnT<T> tile_fn(const bs_config& c, int mode) {
+  return c.flux == BS_FLUX_RUSANOV ? pick_rec<T, RUSANOV>(c.reconstruction, c.limiter, mode)
+                                   : pick_rec<T, HLLC>(c.reconstruction, c.limiter, mode);
 }
 
-using WallFn = void (*)(WallArgs, WallBoundary);
-WallFn wall_fn(const bs_config& c) {
+template <typename S>
+using WallFnT = void (*)(WallArgsT<S>, WallBoundaryT<S>);
+
+template <typename S>
+WallFnT<S> wall_fn(const bs_config& c) {
   if (c.flux == BS_FLUX_RUSANOV) {
-    if (c.reconstruction == SHARED) return wall_flux_kernel<SHARED, RUSANOV>;
-    if (c.reconstruction == SHARED_SCALED) return wall_flux_kernel<SHARED_SCALED, RUSANOV>;
-    return wall_flux_kernel<PER_VARIABLE, RUSANOV>;
+    if (c.reconstruction == SHARED) return wall_flux_kernel<S, SHARED, RUSANOV>;
+    if (c.reconstruction == SHARED_SCALED) return wall_flux_kernel<S, SHARED_SCALED, RUSANOV>;
+    return wall_flux_kernel<S, PER_VARIABLE, RUSANOV>;
   }
-  if (c.reconstruction == SHARED) return wall_flux_kernel<SHARED, HLLC>;
-  if (c.reconstruction == SHARED_SCALED) return wall_flux_kernel<SHARED_SCALED, HLLC>;
-  return wall_flux_kernel<PER_VARIABLE, HLLC>;
+  if (c.reconstruction == SHARED) return wall_flux_kernel<S, SHARED, HLLC>;
+  if (c.reconstruction == SHARED_SCALED) return wall_flux_kernel<S, SHARED_SCALED, HLLC>;
+  return wall_flux_kernel<S, PER_VARIABLE, HLLC>;
 }
 
 // ---- timing -------------------------------------------------------------------
@@ -275,35 +317,38 @@ void timing_collect(bs_solver* s) {
 
 // ---- launches -----------------------------------------------------------------
 
-TileArgs tile_args(bs_solver* s) {
-  TileArgs A;
+template <typename T>
+TileArgsT<T> tile_args(bs_solver* s) {
+  TileArgsT<T> A;
   memset(&A, 0, sizeof A);
-  A.G = grid_of(s);
-  A.B = s->d.bnd;
-  A.gamma = s->gamma32;
+  A.G = grid_of<T>(s);
+  A.B = static_cast<const BoundaryDevT<T>*>(s->d.bnd);
+  A.gamma = (T)state_gamma(s);
   A.axisymmetric = s->cfg.geometry == BS_GEOMETRY_AXISYMMETRIC;
   A.limiter = s->cfg.limiter;
-  A.cfl = (float)s->cfg.cfl;
-  A.dt = s->d.dt;
-  A.dt_global = s->d.dt_global;
+  A.cfl = (T)s->cfg.cfl;
+  A.dt = static_cast<T*>(s->d.dt);
+  A.dt_global = static_cast<const T*>(s->d.dt_global);
   A.control = s->d.control;
   return A;
 }
 
-bs_status launch_tile(bs_solver* s, int mode, const TileArgs& A, int kind) {
+template <typename T>
+bs_status launch_tile(bs_solver* s, int mode, const TileArgsT<T>& A, int kind) {
   dim3 grid(s->tiles_x * s->tiles_y);
   hipEvent_t* ev = timing_begin(s, kind);
-  hipLaunchKernelGGL(tile_fn(s->cfg, mode), grid, dim3(TX * TY), 0, s->stream, A);
+  hipLaunchKernelGGL(tile_fn<T>(s->cfg, mode), grid, dim3(Tile<T>::x * Tile<T>::y), 0, s->stream, A);
   timing_end(s, ev);
   HIP_TRY(hipGetLastError(), "tile kernel launch");
   return BS_OK;
 }
 
-// Runge-Kutta coefficients (runge-kutta.ts:17-28).
-int rk_table(int integrator, const float** a, const float** b) {
-  static const float ea[] = {0.f}, eb[] = {1.f};
-  static const float a2[] = {0.f, 0.5f}, b2[] = {1.f, 0.5f};
-  static const float a3[] = {0.f, 0.75f, (float)(1.0 / 3.0)}, b3[] = {1.f, 0.25f, (float)(2.0 / 3.0)};
+// Runge-Kutta coefficients (runge-kutta.ts:17-28) in the state's precision.
+template <typename T>
+int rk_table(int integrator, const T** a, const T** b) {
+  static const T ea[] = {T(0)}, eb[] = {T(1)};
+  static const T a2[] = {T(0), T(0.5)}, b2[] = {T(1), T(0.5)};
+  static const T a3[] = {T(0), T(0.75), T(1.0 / 3.0)}, b3[] = {T(1), T(0.25), T(2.0 / 3.0)};
   if (integrator == BS_RK_SSPRK2) { *a = a2; *b = b2; return 2; }
   if (integrator == BS_RK_SSPRK3) { *a = a3; *b = b3; return 3; }
   *a = ea; *b = eb;
@@ -314,16 +359,17 @@ int rk_table(int integrator, const float** a, const float** b) {
 // minimum clamped to t_end, then the clock), then the stages. Stage k reads
 // `src` with the stencil and u0 at its own cell only, so the last stage of a
 // multi-stage scheme writes u0 in place (euler2d.ts:1146-1157).
-bs_status encode_step(bs_solver* s) {
-  const float *ca, *cb;
+template <typename T>
+bs_status encode_step_t(bs_solver* s) {
+  const T *ca, *cb;
   const int ns = rk_table(s->cfg.integrator, &ca, &cb);
   const bool global = s->cfg.time_stepping == BS_STEPPING_GLOBAL;
   if (global) {
     const int nb = (s->n + 255) / 256;
-    hipLaunchKernelGGL(time_step_kernel, dim3(nb), dim3(256), 0, s->stream, grid_of(s), state_of(s, s->cur),
-                       s->gamma32, (float)s->cfg.cfl, s->d.dt_partials);
-    hipLaunchKernelGGL(finalize_time_step, dim3(1), dim3(256), 0, s->stream, s->d.dt_partials, nb, s->d.control,
-                       s->d.dt_global);
+    hipLaunchKernelGGL(time_step_kernel<T>, dim3(nb), dim3(256), 0, s->stream, grid_of<T>(s),
+                       state_of<T>(s, s->cur), (T)state_gamma(s), (T)s->cfg.cfl, static_cast<T*>(s->d.dt_partials));
+    hipLaunchKernelGGL(finalize_time_step<T>, dim3(1), dim3(256), 0, s->stream,
+                       static_cast<const T*>(s->d.dt_partials), nb, s->d.control, static_cast<T*>(s->d.dt_global));
   }
   const int cur = s->cur;
   const int other[2] = {(cur + 1) % 3, (cur + 2) % 3};
@@ -331,10 +377,10 @@ bs_status encode_step(bs_solver* s) {
   int last = cur;
   for (int k = 0; k < ns; k++) {
     const int dst = (k == ns - 1 && k > 0) ? cur : other[k % 2];
-    TileArgs A = tile_args(s);
-    A.u0 = state_of(s, cur);
-    A.src = state_of(s, prev);
-    A.dst = state_of(s, dst);
+    TileArgsT<T> A = tile_args<T>(s);
+    A.u0 = state_of<T>(s, cur);
+    A.src = state_of<T>(s, prev);
+    A.dst = state_of<T>(s, dst);
     A.a = ca[k];
     A.b = cb[k];
     A.dt_mode = global ? DT_GLOBAL : (k == 0 ? DT_COMPUTE_LOCAL : DT_READ);
@@ -348,6 +394,8 @@ bs_status encode_step(bs_solver* s) {
   s->cur = last;
   return BS_OK;
 }
+
+bs_status encode_step(bs_solver* s) { return s->fp64 ? encode_step_t<double>(s) : encode_step_t<float>(s); }
 
 void drop_graph(StepGraph& g) {
   if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -521,21 +569,43 @@ bs_status metrics(int ni, int nj, const double* x, const double* y, int axisym, 
   return BS_OK;
 }
 
-bs_status upload_f32(float** dst, const std::vector<double>& src, const char* what) {
-  std::vector<float> f(src.begin(), src.end());
-  bs_status st = dev_alloc(dst, f.size(), what);
-  if (st) return st;
-  HIP_TRY(hipMemcpy(*dst, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice), what);
+// `count` values of the solver's state precision.
+bs_status real_alloc(const bs_solver* s, void** p, size_t count, const char* what) {
+  const size_t bytes = count * real_size(s);
+  hipError_t e = hipMalloc(p, bytes > 0 ? bytes : 16);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(BS_ERR_MEMORY, "allocating %s (%zu bytes): %s", what, bytes, hipGetErrorString(e));
+  }
   return BS_OK;
 }
 
-// Conserved state of a primitive one with the FP32 gamma, as the reference
-// GPU engine packs states (euler2d-gpu.ts:412-432).
-void pack_cons(const double* w, double g32, float out[4]) {
-  out[0] = (float)w[0];
-  out[1] = (float)(w[0] * w[1]);
-  out[2] = (float)(w[0] * w[2]);
-  out[3] = (float)(w[3] / (g32 - 1) + 0.5 * w[0] * (w[1] * w[1] + w[2] * w[2]));
+// Copies FP64 host values to `dst` in the solver's state precision.
+bs_status copy_real(const bs_solver* s, void* dst, const double* src, size_t count, const char* what) {
+  if (s->fp64) {
+    HIP_TRY(hipMemcpy(dst, src, count * sizeof(double), hipMemcpyHostToDevice), what);
+    return BS_OK;
+  }
+  std::vector<float> f(src, src + count);
+  HIP_TRY(hipMemcpy(dst, f.data(), count * sizeof(float), hipMemcpyHostToDevice), what);
+  return BS_OK;
+}
+
+bs_status upload_real(const bs_solver* s, void** dst, const std::vector<double>& src, const char* what) {
+  bs_status st = real_alloc(s, dst, src.size(), what);
+  if (st) return st;
+  return copy_real(s, *dst, src.data(), src.size(), what);
+}
+
+// Conserved state of a primitive one with the gamma `g` the state is
+// formed with (the FP32 gamma for an FP32 state, as the reference GPU
+// engine packs states, euler2d-gpu.ts:412-432), rounded to T at the end.
+template <typename T>
+void pack_cons(const double* w, double g, T out[4]) {
+  out[0] = (T)w[0];
+  out[1] = (T)(w[0] * w[1]);
+  out[2] = (T)(w[0] * w[2]);
+  out[3] = (T)(w[3] / (g - 1) + 0.5 * w[0] * (w[1] * w[1] + w[2] * w[2]));
 }
 
 bool physical(const double* w) {
@@ -579,7 +649,7 @@ bs_status bs_compute_metrics(int32_t ni, int32_t nj, const double* x, const doub
                  planar_area, centroid_x, centroid_y);
 }
 
-bs_status bs_solver_create(bs_ctx* ctx, const bs_config* c, bs_solver** out) {
+static bs_status solver_create(bs_ctx* ctx, const bs_config* c, bool fp64, bs_solver** out) {
   if (!ctx || !c || !out) return fail(BS_ERR_RANGE, "null argument");
   *out = nullptr;
   // config.ts:35-55 validateGrid2DConfig and the GPU engine's own checks.
@@ -608,14 +678,16 @@ bs_status bs_solver_create(bs_ctx* ctx, const bs_config* c, bs_solver** out) {
   s->nj = c->nj;
   s->n = c->ni * c->nj;
   s->gamma32 = (float)c->gamma;
+  s->fp64 = fp64;
   const int ni = s->ni, nj = s->nj, n = s->n;
   const size_t nn = (size_t)(ni + 1) * (nj + 1);
   s->x.assign(c->x, c->x + nn);
   s->y.assign(c->y, c->y + nn);
   s->cfg.x = s->x.data();
   s->cfg.y = s->y.data();
-  s->tiles_x = (ni + TX - 1) / TX;
-  s->tiles_y = (nj + TY - 1) / TY;
+  const int tx = fp64 ? Tile<double>::x : Tile<float>::x, ty = fp64 ? Tile<double>::y : Tile<float>::y;
+  s->tiles_x = (ni + tx - 1) / tx;
+  s->tiles_y = (nj + ty - 1) / ty;
 
   const size_t fi = (size_t)(ni + 1) * nj, fj = (size_t)ni * (nj + 1);
   std::vector<double> inx(fi), iny(fi), ia(fi), jnx(fj), jny(fj), ja(fj), vol(n), planar(n);
@@ -635,19 +707,21 @@ bs_status bs_solver_create(bs_ctx* ctx, const bs_config* c, bs_solver** out) {
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(BS_ERR_EXECUTION, "hipStreamCreate failed"));
   for (int b = 0; b < 3; b++)
-    if ((st = dev_alloc(&s->d.state[b], 4 * (size_t)n, "state"))) return bail(st);
-  if ((st = upload_f32(&s->d.inx, inx, "i-face normals")) || (st = upload_f32(&s->d.iny, iny, "i-face normals")) ||
-      (st = upload_f32(&s->d.ia, ia, "i-face areas")) || (st = upload_f32(&s->d.jnx, jnx, "j-face normals")) ||
-      (st = upload_f32(&s->d.jny, jny, "j-face normals")) || (st = upload_f32(&s->d.ja, ja, "j-face areas")) ||
-      (st = upload_f32(&s->d.vol, vol, "volumes")) || (st = upload_f32(&s->d.planar, planar, "planar areas")))
+    if ((st = real_alloc(s, &s->d.state[b], 4 * (size_t)n, "state"))) return bail(st);
+  if ((st = upload_real(s, &s->d.inx, inx, "i-face normals")) ||
+      (st = upload_real(s, &s->d.iny, iny, "i-face normals")) || (st = upload_real(s, &s->d.ia, ia, "i-face areas")) ||
+      (st = upload_real(s, &s->d.jnx, jnx, "j-face normals")) ||
+      (st = upload_real(s, &s->d.jny, jny, "j-face normals")) || (st = upload_real(s, &s->d.ja, ja, "j-face areas")) ||
+      (st = upload_real(s, &s->d.vol, vol, "volumes")) || (st = upload_real(s, &s->d.planar, planar, "planar areas")))
     return bail(st);
   const int blocks = s->tiles_x * s->tiles_y;
   const int nb256 = (n + 255) / 256;
-  if ((st = dev_alloc(&s->d.dt, (size_t)n, "time steps")) || (st = dev_alloc(&s->d.dt_global, 1, "dt")) ||
-      (st = dev_alloc(&s->d.dt_partials, (size_t)nb256, "dt partials")) ||
+  if ((st = real_alloc(s, &s->d.dt, (size_t)n, "time steps")) || (st = real_alloc(s, &s->d.dt_global, 1, "dt")) ||
+      (st = real_alloc(s, &s->d.dt_partials, (size_t)nb256, "dt partials")) ||
       (st = dev_alloc(&s->d.partials, (size_t)(blocks > nb256 ? blocks : nb256) + 64, "partials")) ||
       (st = dev_alloc(&s->d.scalar, 4, "scalars")) || (st = dev_alloc(&s->d.control, 1, "control")) ||
-      (st = dev_alloc(&s->d.bnd, 1, "boundary table")) ||
+      (st = dev_alloc((char**)&s->d.bnd, fp64 ? sizeof(BoundaryDevT<double>) : sizeof(BoundaryDevT<float>),
+                      "boundary table")) ||
       (st = dev_alloc(&s->d.wall_out, 9 * (size_t)(ni > nj ? ni : nj), "wall fluxes")))
     return bail(st);
   HIP_TRY(hipMemset(s->d.control, 0, sizeof(Control)), "control reset");
@@ -682,15 +756,14 @@ bs_status bs_solver_create(bs_ctx* ctx, const bs_config* c, bs_solver** out) {
     HIP_TRY(hipMemcpy(s->d.face_geo[side], geo.data(), geo.size() * sizeof(double), hipMemcpyHostToDevice),
             "boundary geometry");
     if (c->boundaries[side].kind == BS_BC_FACE_STATES) {
-      if ((st = dev_alloc(&s->d.face_cons[side], 4 * (size_t)m, "face states")) ||
-          (st = dev_alloc(&s->d.face_flux[side], 4 * (size_t)m, "face fluxes")))
+      if ((st = real_alloc(s, &s->d.face_cons[side], 4 * (size_t)m, "face states")) ||
+          (st = real_alloc(s, &s->d.face_flux[side], 4 * (size_t)m, "face fluxes")))
         return bail(st);
       const double w[4] = {c->boundaries[side].rho, c->boundaries[side].u, c->boundaries[side].v,
                            c->boundaries[side].p};
-      std::vector<float> packed(4 * (size_t)m);
-      for (int t = 0; t < m; t++) pack_cons(w, (double)s->gamma32, &packed[4 * t]);
-      HIP_TRY(hipMemcpy(s->d.face_cons[side], packed.data(), packed.size() * sizeof(float), hipMemcpyHostToDevice),
-              "face states");
+      std::vector<double> packed(4 * (size_t)m);
+      for (int t = 0; t < m; t++) pack_cons(w, state_gamma(s), &packed[4 * t]);
+      if ((st = copy_real(s, s->d.face_cons[side], packed.data(), packed.size(), "face states"))) return bail(st);
     }
   }
   Control init;
@@ -700,6 +773,20 @@ bs_status bs_solver_create(bs_ctx* ctx, const bs_config* c, bs_solver** out) {
   HIP_TRY(hipMemcpy(s->d.control, &init, sizeof init, hipMemcpyHostToDevice), "control init");
   if ((st = upload_boundary(s))) return bail(st);
   *out = s;
+  return BS_OK;
+}
+
+bs_status bs_solver_create(bs_ctx* ctx, const bs_config* c, bs_solver** out) {
+  return solver_create(ctx, c, false, out);
+}
+
+bs_status bs_solver_create_fp64(bs_ctx* ctx, const bs_config* c, bs_solver** out) {
+  return solver_create(ctx, c, true, out);
+}
+
+bs_status bs_solver_precision(bs_solver* s, int32_t* precision) {
+  if (!s || !precision) return fail(BS_ERR_RANGE, "null argument");
+  *precision = s->fp64 ? BS_PRECISION_FP64 : BS_PRECISION_FP32;
   return BS_OK;
 }
 
@@ -716,45 +803,99 @@ bs_status bs_set_primitives(bs_solver* s, const double* prim) {
   SOLVER_GUARD(s);
   if (!prim) return fail(BS_ERR_RANGE, "no states");
   const int n = s->n;
-  std::vector<float> packed(4 * (size_t)n);
+  std::vector<double> packed(4 * (size_t)n);
   for (int c = 0; c < n; c++) {
     const double* w = prim + 4 * (size_t)c;
     if (!physical(w)) return fail(BS_ERR_NONPHYSICAL, "non-physical initial state in cell %d", c);
-    float q[4];
-    pack_cons(w, (double)s->gamma32, q);
+    double q[4];
+    pack_cons(w, state_gamma(s), q);
     for (int k = 0; k < 4; k++) packed[(size_t)k * n + c] = q[k];
   }
   s->cur = 0;
-  HIP_TRY(hipMemcpy(s->d.state[0], packed.data(), packed.size() * sizeof(float), hipMemcpyHostToDevice),
-          "state upload");
+  // An FP32 state rounds the FP64 conserved values once.
+  bs_status st = copy_real(s, s->d.state[0], packed.data(), packed.size(), "state upload");
+  if (st) return st;
   return reset_control(s);
 }
+
+// C++ helpers of the entry points below.
+extern "C++" {
+// Uploads four arrays of n values of type V as the current state.
+template <typename V>
+static bs_status set_state_from(bs_solver* s, const V* const src[4]) {
+  const size_t n = s->n;
+  s->cur = 0;
+  const bool same = (sizeof(V) == sizeof(double)) == s->fp64;
+  char* base = static_cast<char*>(s->d.state[0]);
+  for (int q = 0; q < 4; q++) {
+    void* dst = base + q * n * real_size(s);
+    if (same) {
+      HIP_TRY(hipMemcpy(dst, src[q], n * sizeof(V), hipMemcpyHostToDevice), "state upload");
+    } else if (s->fp64) {
+      std::vector<double> v(src[q], src[q] + n);
+      HIP_TRY(hipMemcpy(dst, v.data(), n * sizeof(double), hipMemcpyHostToDevice), "state upload");
+    } else {
+      std::vector<float> v(src[q], src[q] + n);
+      HIP_TRY(hipMemcpy(dst, v.data(), n * sizeof(float), hipMemcpyHostToDevice), "state upload");
+    }
+  }
+  return reset_control(s);
+}
+
+// Reads the current state into four arrays of type V (null ones skipped).
+template <typename V>
+static bs_status read_state_into(bs_solver* s, V* const dst[4], bs_report* report) {
+  const size_t n = s->n;
+  const bool same = (sizeof(V) == sizeof(double)) == s->fp64;
+  std::vector<char> stage(same ? 0 : 4 * n * real_size(s));
+  const char* base = static_cast<const char*>(s->d.state[s->cur]);
+  for (int q = 0; q < 4; q++)
+    if (dst[q]) {
+      void* to = same ? (void*)dst[q] : (void*)(stage.data() + q * n * real_size(s));
+      HIP_TRY(hipMemcpyAsync(to, base + q * n * real_size(s), n * real_size(s), hipMemcpyDeviceToHost, s->stream),
+              "state readback");
+    }
+  Control c;
+  bs_status st = read_control(s, &c);
+  if (st) return st;
+  if (!same)
+    for (int q = 0; q < 4; q++)
+      if (dst[q]) {
+        const char* from = stage.data() + q * n * real_size(s);
+        for (size_t k = 0; k < n; k++)
+          dst[q][k] = s->fp64 ? (V)reinterpret_cast<const double*>(from)[k] : (V)reinterpret_cast<const float*>(from)[k];
+      }
+  fill_report(c, report);
+  if (c.status & STATUS_NON_PHYSICAL) return fail(BS_ERR_NONPHYSICAL, "the GPU solution is non-physical");
+  return BS_OK;
+}
+}  // extern "C++"
 
 bs_status bs_set_state(bs_solver* s, const float* rho, const float* mu, const float* mv, const float* energy) {
   SOLVER_GUARD(s);
   if (!rho || !mu || !mv || !energy) return fail(BS_ERR_RANGE, "four arrays are needed");
-  const size_t n = s->n;
-  s->cur = 0;
-  const float* src[4] = {rho, mu, mv, energy};
-  for (int q = 0; q < 4; q++)
-    HIP_TRY(hipMemcpy(s->d.state[0] + q * n, src[q], n * sizeof(float), hipMemcpyHostToDevice), "state upload");
-  return reset_control(s);
+  const float* const src[4] = {rho, mu, mv, energy};
+  return set_state_from(s, src);
+}
+
+bs_status bs_set_state_f64(bs_solver* s, const double* rho, const double* mu, const double* mv,
+                           const double* energy) {
+  SOLVER_GUARD(s);
+  if (!rho || !mu || !mv || !energy) return fail(BS_ERR_RANGE, "four arrays are needed");
+  const double* const src[4] = {rho, mu, mv, energy};
+  return set_state_from(s, src);
 }
 
 bs_status bs_read_state(bs_solver* s, float* rho, float* mu, float* mv, float* energy, bs_report* report) {
   SOLVER_GUARD(s);
-  const size_t n = s->n;
-  float* dst[4] = {rho, mu, mv, energy};
-  for (int q = 0; q < 4; q++)
-    if (dst[q])
-      HIP_TRY(hipMemcpyAsync(dst[q], s->d.state[s->cur] + q * n, n * sizeof(float), hipMemcpyDeviceToHost, s->stream),
-              "state readback");
-  Control c;
-  bs_status st = read_control(s, &c);
-  if (st) return st;
-  fill_report(c, report);
-  if (c.status & STATUS_NON_PHYSICAL) return fail(BS_ERR_NONPHYSICAL, "the GPU solution is non-physical");
-  return BS_OK;
+  float* const dst[4] = {rho, mu, mv, energy};
+  return read_state_into(s, dst, report);
+}
+
+bs_status bs_read_state_f64(bs_solver* s, double* rho, double* mu, double* mv, double* energy, bs_report* report) {
+  SOLVER_GUARD(s);
+  double* const dst[4] = {rho, mu, mv, energy};
+  return read_state_into(s, dst, report);
 }
 
 bs_status bs_set_face_states(bs_solver* s, int32_t side, const double* prim, int32_t count) {
@@ -762,11 +903,9 @@ bs_status bs_set_face_states(bs_solver* s, int32_t side, const double* prim, int
   if (side < 0 || side > 3 || s->cfg.boundaries[side].kind != BS_BC_FACE_STATES)
     return fail(BS_ERR_RANGE, "the boundary does not take face states");
   if (count != along(s, side)) return fail(BS_ERR_RANGE, "the boundary needs %d face states", along(s, side));
-  std::vector<float> packed(4 * (size_t)count);
-  for (int t = 0; t < count; t++) pack_cons(prim + 4 * (size_t)t, (double)s->gamma32, &packed[4 * t]);
-  HIP_TRY(hipMemcpy(s->d.face_cons[side], packed.data(), packed.size() * sizeof(float), hipMemcpyHostToDevice),
-          "face states");
-  return BS_OK;
+  std::vector<double> packed(4 * (size_t)count);
+  for (int t = 0; t < count; t++) pack_cons(prim + 4 * (size_t)t, state_gamma(s), &packed[4 * t]);
+  return copy_real(s, s->d.face_cons[side], packed.data(), packed.size(), "face states");
 }
 
 bs_status bs_set_face_fluxes(bs_solver* s, int32_t side, const double* flux, int32_t count) {
@@ -778,8 +917,8 @@ bs_status bs_set_face_fluxes(bs_solver* s, int32_t side, const double* flux, int
     return upload_boundary(s);
   }
   if (count != 4 * along(s, side)) return fail(BS_ERR_RANGE, "the boundary needs %d flux values", 4 * along(s, side));
-  std::vector<float> f(flux, flux + count);
-  HIP_TRY(hipMemcpy(s->d.face_flux[side], f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice), "face fluxes");
+  bs_status st = copy_real(s, s->d.face_flux[side], flux, (size_t)count, "face fluxes");
+  if (st) return st;
   s->imposed[side] = true;
   return upload_boundary(s);
 }
@@ -798,7 +937,7 @@ bs_status bs_iterate(bs_solver* s, int32_t count, bs_report* report) {
   // Region timing: a batch of explicit local steps launches nothing but
   // the stage kernel, so one event pair around the batch, split over its
   // stage launches, times that kernel with graphs left on.
-  const float *ca, *cb;
+  const double *ca, *cb;
   const int ns = rk_table(s->cfg.integrator, &ca, &cb);
   const bool region = s->timing.enabled && s->timing.region && s->cfg.time_stepping == BS_STEPPING_LOCAL;
   int remaining = count;
@@ -866,11 +1005,12 @@ bs_status bs_advance_to(bs_solver* s, double t_end, int32_t max_steps, bs_report
   return BS_OK;
 }
 
-bs_status bs_residual_norm(bs_solver* s, double* norm) {
-  SOLVER_GUARD(s);
-  if (!norm) return fail(BS_ERR_RANGE, "no output");
-  TileArgs A = tile_args(s);
-  A.src = state_of(s, s->cur);
+// C++ helpers of the entry points below.
+extern "C++" {
+template <typename T>
+static bs_status residual_norm_t(bs_solver* s, double* norm) {
+  TileArgsT<T> A = tile_args<T>(s);
+  A.src = state_of<T>(s, s->cur);
   A.partials = s->d.partials;
   bs_status st = launch_tile(s, TILE_RESIDUAL, A, 0);
   if (st) return st;
@@ -882,19 +1022,28 @@ bs_status bs_residual_norm(bs_solver* s, double* norm) {
   timing_collect(s);
   return BS_OK;
 }
+}  // extern "C++"
 
-bs_status bs_residual_vector(bs_solver* s, double* residual) {
+bs_status bs_residual_norm(bs_solver* s, double* norm) {
   SOLVER_GUARD(s);
-  if (!residual) return fail(BS_ERR_RANGE, "no output");
-  TileArgs A = tile_args(s);
-  A.src = state_of(s, s->cur);
+  if (!norm) return fail(BS_ERR_RANGE, "no output");
+  return s->fp64 ? residual_norm_t<double>(s, norm) : residual_norm_t<float>(s, norm);
+}
+
+// C++ helpers of the entry points below.
+extern "C++" {
+template <typename T>
+static bs_status residual_vector_t(bs_solver* s, double* residual) {
+  TileArgsT<T> A = tile_args<T>(s);
+  A.src = state_of<T>(s, s->cur);
   // The spare buffer (neither the current state nor needed after) holds it.
   const int spare = (s->cur + 1) % 3;
-  for (int q = 0; q < 4; q++) A.rvec[q] = s->d.state[spare] + (size_t)q * s->n;
+  StateDevT<T> out = state_of<T>(s, spare);
+  for (int q = 0; q < 4; q++) A.rvec[q] = out.q[q];
   bs_status st = launch_tile(s, TILE_RESIDUAL, A, 0);
   if (st) return st;
-  std::vector<float> r(4 * (size_t)s->n);
-  HIP_TRY(hipMemcpyAsync(r.data(), s->d.state[spare], r.size() * sizeof(float), hipMemcpyDeviceToHost, s->stream),
+  std::vector<T> r(4 * (size_t)s->n);
+  HIP_TRY(hipMemcpyAsync(r.data(), s->d.state[spare], r.size() * sizeof(T), hipMemcpyDeviceToHost, s->stream),
           "residual readback");
   HIP_TRY(hipStreamSynchronize(s->stream), "sync");
   timing_collect(s);
@@ -902,35 +1051,51 @@ bs_status bs_residual_vector(bs_solver* s, double* residual) {
     for (int q = 0; q < 4; q++) residual[4 * (size_t)c + q] = r[(size_t)q * s->n + c];
   return BS_OK;
 }
+}  // extern "C++"
 
-static bs_status wall_launch(bs_solver* s, int side, double* out_dev, double p_inf, double* partials, int* blocks) {
-  WallArgs A;
+bs_status bs_residual_vector(bs_solver* s, double* residual) {
+  SOLVER_GUARD(s);
+  if (!residual) return fail(BS_ERR_RANGE, "no output");
+  return s->fp64 ? residual_vector_t<double>(s, residual) : residual_vector_t<float>(s, residual);
+}
+
+// C++ helpers of the entry points below.
+extern "C++" {
+template <typename S>
+static bs_status wall_launch_t(bs_solver* s, int side, double* out_dev, double p_inf, double* partials, int* blocks) {
+  WallArgsT<S> A;
   memset(&A, 0, sizeof A);
-  A.G = grid_of(s);
-  A.s = state_of(s, s->cur);
+  A.G = grid_of<S>(s);
+  A.s = state_of<S>(s, s->cur);
   A.side = side;
   A.count = along(s, side);
   A.kind = s->cfg.boundaries[side].kind;
   A.face_geo = s->d.face_geo[side];
   A.gamma = s->cfg.gamma;
-  A.gamma32 = (double)s->gamma32;
+  A.gamma32 = state_gamma(s);
   A.rec = s->cfg.reconstruction;
   A.limiter = s->cfg.limiter;
   A.flux = s->cfg.flux;
   A.out = out_dev;
   A.p_inf = p_inf;
   A.partials = partials;
-  WallBoundary WB;
+  WallBoundaryT<S> WB;
   for (int k = 0; k < 4; k++) {
     const bs_boundary& b = s->cfg.boundaries[k];
     WB.kind[k] = b.kind;
     WB.state[k][0] = b.rho; WB.state[k][1] = b.u; WB.state[k][2] = b.v; WB.state[k][3] = b.p;
-    WB.cons[k] = s->d.face_cons[k];
+    WB.cons[k] = static_cast<const S*>(s->d.face_cons[k]);
   }
   *blocks = (A.count + 255) / 256;
-  hipLaunchKernelGGL(wall_fn(s->cfg), dim3(*blocks), dim3(256), 0, s->stream, A, WB);
+  hipLaunchKernelGGL(wall_fn<S>(s->cfg), dim3(*blocks), dim3(256), 0, s->stream, A, WB);
   HIP_TRY(hipGetLastError(), "wall flux launch");
   return BS_OK;
+}
+}  // extern "C++"
+
+static bs_status wall_launch(bs_solver* s, int side, double* out_dev, double p_inf, double* partials, int* blocks) {
+  return s->fp64 ? wall_launch_t<double>(s, side, out_dev, p_inf, partials, blocks)
+                 : wall_launch_t<float>(s, side, out_dev, p_inf, partials, blocks);
 }
 
 bs_status bs_wall_fluxes(bs_solver* s, int32_t side, double* faces, int32_t count) {
@@ -991,10 +1156,11 @@ bs_status bs_iterate_implicit(bs_solver* s, int32_t count, double cfl_max, int32
   if (!(cfl_max > 0 && omega >= 1 && growth >= 1))
     return fail(BS_ERR_RANGE, "implicit steps need cfl > 0, omega >= 1 and growth >= 1");
   if (sweeps < 1) return fail(BS_ERR_RANGE, "the relaxation needs a whole number of sweeps, at least one");
+  if (s->fp64) return fail(BS_ERR_UNSUPPORTED, "implicit steps run on FP32 solvers only");
   bs_status st = ensure_implicit(s);
   if (st) return st;
   const size_t n = s->n;
-  StateDev res, dq;
+  StateDevT<float> res, dq;
   for (int q = 0; q < 4; q++) {
     res.q[q] = s->d.res + q * n;
     dq.q[q] = s->d.dq + q * n;
@@ -1003,8 +1169,8 @@ bs_status bs_iterate_implicit(bs_solver* s, int32_t count, double cfl_max, int32
   // half-sweep), the remaining half-sweeps, the update into the next buffer.
   auto encode = [&]() -> bs_status {
     const int src = s->cur, dst = (s->cur + 1) % 3;
-    TileArgs A = tile_args(s);
-    A.src = state_of(s, src);
+    TileArgsT<float> A = tile_args<float>(s);
+    A.src = state_of<float>(s, src);
     A.omega = (float)omega;
     for (int q = 0; q < 4; q++) {
       A.res[q] = res.q[q];
@@ -1016,8 +1182,8 @@ bs_status bs_iterate_implicit(bs_solver* s, int32_t count, double cfl_max, int32
     bs_status r = launch_tile(s, TILE_IMPLICIT, A, 1);
     if (r) return r;
     RelaxArgs R;
-    R.G = grid_of(s);
-    R.q = state_of(s, src);
+    R.G = grid_of<float>(s);
+    R.q = state_of<float>(s, src);
     for (int q = 0; q < 4; q++) {
       R.res[q] = res.q[q];
       R.dq[q] = dq.q[q];
@@ -1028,8 +1194,8 @@ bs_status bs_iterate_implicit(bs_solver* s, int32_t count, double cfl_max, int32
     R.gamma = s->gamma32;
     R.omega = (float)omega;
     UpdateArgs U;
-    U.src = state_of(s, src);
-    U.dst = state_of(s, dst);
+    U.src = state_of<float>(s, src);
+    U.dst = state_of<float>(s, dst);
     U.dq = dq;
     U.n = s->n;
     U.gamma = s->gamma32;

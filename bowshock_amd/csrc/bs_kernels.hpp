@@ -24,19 +24,25 @@ namespace bs {
 
 constexpr unsigned STATUS_NON_PHYSICAL = 1u;
 
-struct GridDev {
+// Device structures are templates on the state's floating-point type:
+// FP32 (the default) or FP64 (solvers created with BS_PRECISION_FP64).
+template <typename T>
+struct GridDevT {
   int ni, nj;
-  const float *inx, *iny, *ia;  // i-faces, (ni + 1) * nj
-  const float *jnx, *jny, *ja;  // j-faces, ni * (nj + 1)
-  const float *vol, *planar;    // cells
+  const T *inx, *iny, *ia;  // i-faces, (ni + 1) * nj
+  const T *jnx, *jny, *ja;  // j-faces, ni * (nj + 1)
+  const T *vol, *planar;    // cells
 };
+using GridDev = GridDevT<float>;
 
-struct BoundaryDev {
+template <typename T>
+struct BoundaryDevT {
   int kind[4];
-  float state[4][4];         // primitive (rho, u, v, p) outside inflow / far-field sides
-  const float *face_cons[4]; // FACE_STATES sides: conserved (4 floats) per face
-  const float *face_flux[4]; // imposed flux per unit area (4 floats) per face, or null
+  T state[4][4];         // primitive (rho, u, v, p) outside inflow / far-field sides
+  const T *face_cons[4]; // FACE_STATES sides: conserved (4 values) per face
+  const T *face_flux[4]; // imposed flux per unit area (4 values) per face, or null
 };
+using BoundaryDev = BoundaryDevT<float>;
 
 struct Control {
   unsigned status;
@@ -48,24 +54,27 @@ struct Control {
   double t_end;
 };
 
-struct StateDev {
-  float *q[4];
+template <typename T>
+struct StateDevT {
+  T *q[4];
 };
+using StateDev = StateDevT<float>;
 
 // The conserved state of cell (i, j), or of a ghost cell up to two layers
 // outside one side (never a corner): euler2d.wgsl load_cell, euler2d.ts
 // fillGhosts 1361-1403. Ghost layer g mirrors interior layer g (outflow,
 // wall with the momentum reflected about the boundary face normal), wraps
 // (periodic), or holds the inflow / face / far-field state.
-__device__ __forceinline__ Vec4<float> load_cell(const StateDev& s, const GridDev& G, const BoundaryDev& B, int i,
-                                                 int j, float g) {
+template <typename T>
+__device__ __forceinline__ Vec4<T> load_cell(const StateDevT<T>& s, const GridDevT<T>& G, const BoundaryDevT<T>& B,
+                                             int i, int j, T g) {
   const int ni = G.ni, nj = G.nj;
   if (i >= 0 && i < ni && j >= 0 && j < nj) {
     int c = i + j * ni;
     return {s.q[0][c], s.q[1][c], s.q[2][c], s.q[3][c]};
   }
   int side, t, mi, mj, wi, wj;
-  float nx, ny;
+  T nx, ny;
   if (i < 0) {
     side = IMIN; t = j; mi = -1 - i; mj = j; wi = i + ni; wj = j;
     int f = j * (ni + 1);
@@ -84,28 +93,28 @@ __device__ __forceinline__ Vec4<float> load_cell(const StateDev& s, const GridDe
   }
   const int kind = B.kind[side];
   if (kind == FACE_STATES) {
-    const float* b = B.face_cons[side] + 4 * t;
+    const T* b = B.face_cons[side] + 4 * t;
     return {b[0], b[1], b[2], b[3]};
   }
   if (kind == INFLOW) {
-    Prim<float> w = {B.state[side][0], B.state[side][1], B.state[side][2], B.state[side][3]};
+    Prim<T> w = {B.state[side][0], B.state[side][1], B.state[side][2], B.state[side][3]};
     return to_cons(w, g);
   }
   if (kind == FAR_FIELD) {
     int ii = side == IMIN ? 0 : side == IMAX ? ni - 1 : i;
     int jj = side == JMIN ? 0 : side == JMAX ? nj - 1 : j;
     int c = ii + jj * ni;
-    Prim<float> in = to_prim(s.q[0][c], s.q[1][c], s.q[2][c], s.q[3][c], g);
-    Prim<float> fr = {B.state[side][0], B.state[side][1], B.state[side][2], B.state[side][3]};
-    float o = (side == IMIN || side == JMIN) ? -1.f : 1.f;
+    Prim<T> in = to_prim(s.q[0][c], s.q[1][c], s.q[2][c], s.q[3][c], g);
+    Prim<T> fr = {B.state[side][0], B.state[side][1], B.state[side][2], B.state[side][3]};
+    T o = (side == IMIN || side == JMIN) ? -T(1.) : T(1.);
     return to_cons(far_field_state(in, fr, o * nx, o * ny, g), g);
   }
   int c = kind == PERIODIC ? wi + wj * ni : mi + mj * ni;
-  Vec4<float> v = {s.q[0][c], s.q[1][c], s.q[2][c], s.q[3][c]};
+  Vec4<T> v = {s.q[0][c], s.q[1][c], s.q[2][c], s.q[3][c]};
   if (kind == WALL) {
-    float mn = v.y * nx + v.z * ny;
-    v.y -= 2.f * mn * nx;
-    v.z -= 2.f * mn * ny;
+    T mn = v.y * nx + v.z * ny;
+    v.y -= T(2.) * mn * nx;
+    v.z -= T(2.) * mn * ny;
   }
   return v;
 }
@@ -117,43 +126,47 @@ __device__ __forceinline__ void flag_non_physical(bool bad, Control* ctl) {
   if (bad && (int)(__lane_id()) == __ffsll((long long)m) - 1) atomicOr(&ctl->status, STATUS_NON_PHYSICAL);
 }
 
-__device__ __forceinline__ bool is_physical(const Vec4<float>& q, float g) {
-  Prim<float> w = to_prim(q.x, q.y, q.z, q.w, g);
+template <typename T>
+__device__ __forceinline__ bool is_physical(const Vec4<T>& q, T g) {
+  Prim<T> w = to_prim(q.x, q.y, q.z, q.w, g);
   // Comparisons with NaN are false; the bounds reject infinities
   // (euler2d.wgsl is_physical).
-  return w.rho > 0.f && w.rho < 3.0e38f && w.p > 0.f && w.p < 3.0e38f && bs_abs(w.u) < 3.0e38f &&
-         bs_abs(w.v) < 3.0e38f;
+  return w.rho > T(0) && w.rho < T(3.0e38) && w.p > T(0) && w.p < T(3.0e38) && bs_abs(w.u) < T(3.0e38) &&
+         bs_abs(w.v) < T(3.0e38);
 }
 
 enum DtMode : int { DT_READ = 0, DT_COMPUTE_LOCAL = 1, DT_GLOBAL = 2 };
 enum TileMode : int { TILE_STAGE = 0, TILE_IMPLICIT = 1, TILE_RESIDUAL = 2 };
 
-struct TileArgs {
-  GridDev G;
-  const BoundaryDev* B;       // device-resident: indexed by a runtime side without spilling the arguments
-  StateDev u0, src, dst;
-  float gamma;
+template <typename T>
+struct TileArgsT {
+  GridDevT<T> G;
+  const BoundaryDevT<T>* B;       // device-resident: indexed by a runtime side without spilling the arguments
+  StateDevT<T> u0, src, dst;
+  T gamma;
   int axisymmetric;
   int limiter;
   // stage: dst = a u0 + b (src + dt / V R(src))
-  float a, b;
+  T a, b;
   int dt_mode;
-  float cfl;
-  float* dt;                 // per cell
-  const float* dt_global;    // DT_GLOBAL
+  T cfl;
+  T* dt;                 // per cell
+  const T* dt_global;    // DT_GLOBAL
   Control* control;
   int count_step;            // this launch completes a step: control->steps++
   double* partials;          // per block: sum of (R_rho / V)^2, or null
   // implicit prepare: residual, diagonal, lambda A per face
-  float* res[4];
-  float* diag;
-  float* lam_i;              // lambda * area per i-face
-  float* lam_j;              // lambda * area per j-face
-  float* dq[4];
-  float omega;
+  T* res[4];
+  T* diag;
+  T* lam_i;              // lambda * area per i-face
+  T* lam_j;              // lambda * area per j-face
+  T* dq[4];
+  T omega;
   // residual vector output (TILE_RESIDUAL)
-  float* rvec[4];
+  T* rvec[4];
 };
+using TileArgs = TileArgsT<float>;
+
 
 // Block-wide FP64 sum: wave shuffles, then one LDS step across waves.
 template <int NT>
@@ -202,20 +215,20 @@ constexpr EdgePlan edge_plan(int TX, int TY) {
 }
 
 // The tile (bx, by) of one grid; its FP64 partial goes to A.partials[slot].
-template <int TX, int TY, int REC, int LIM, int FLUX, int MODE>
-__device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int slot) {
+template <typename T, int TX, int TY, int REC, int LIM, int FLUX, int MODE>
+__device__ __forceinline__ void tile_body(const TileArgsT<T>& A, int bx, int by, int slot) {
   constexpr int NT = TX * TY;
   constexpr int WX = TX + 4, WY = TY + 4;
-  __shared__ float sp[4][WY][WX];        // primitives of the tile and its halo
-  __shared__ float fi[4][TY][TX + 1];    // i-face flux times area
-  __shared__ float fj[4][TY + 1][TX];    // j-face flux times area
-  __shared__ float xi[2][TY][TX + 1];    // i-face area vector (stage) or lambda A (implicit)
-  __shared__ float xj[2][TY + 1][TX];
+  __shared__ T sp[4][WY][WX];        // primitives of the tile and its halo
+  __shared__ T fi[4][TY][TX + 1];    // i-face flux times area
+  __shared__ T fj[4][TY + 1][TX];    // j-face flux times area
+  __shared__ T xi[2][TY][TX + 1];    // i-face area vector (stage) or lambda A (implicit)
+  __shared__ T xj[2][TY + 1][TX];
   __shared__ double red[NT / 64];
 
-  const GridDev& G = A.G;
+  const GridDevT<T>& G = A.G;
   const int ni = G.ni, nj = G.nj;
-  const float g = A.gamma;
+  const T g = A.gamma;
   const int i0 = bx * TX, j0 = by * TY;
   const int tid = threadIdx.x;
 
@@ -226,13 +239,13 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int
   const int li = tid % TX, lj = tid / TX;
   const int gi = i0 + li, gj = j0 + lj;
   const bool valid = gi < ni && gj < nj;
-  Vec4<float> own = {0.f, 0.f, 0.f, 0.f};
+  Vec4<T> own = {T(0.), T(0.), T(0.), T(0.)};
   // Loads that do not depend on the staged primitives are issued first, so
   // their latency overlaps the staging and the faces: the metrics of the
   // thread's own i- and j-face, and the cell's volume, planar area, stored
   // time step and u0.
   const bool has_i = gi <= ni && gj < nj, has_j = gi < ni && gj <= nj;
-  float mi[3] = {0.f, 0.f, 0.f}, mj[3] = {0.f, 0.f, 0.f};
+  T mi[3] = {T(0.), T(0.), T(0.)}, mj[3] = {T(0.), T(0.), T(0.)};
   if (has_i) {
     const int f = gi + gj * (ni + 1);
     mi[0] = G.inx[f]; mi[1] = G.iny[f]; mi[2] = G.ia[f];
@@ -241,13 +254,13 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int
     const int f = gi + gj * ni;
     mj[0] = G.jnx[f]; mj[1] = G.jny[f]; mj[2] = G.ja[f];
   }
-  float c_vol = 1.f, c_planar = 0.f, c_dt = 0.f, c_u0[4] = {0.f, 0.f, 0.f, 0.f};
+  T c_vol = T(1.), c_planar = T(0.), c_dt = T(0.), c_u0[4] = {T(0.), T(0.), T(0.), T(0.)};
   if (valid) {
     const int c = gi + gj * ni;
     c_vol = G.vol[c];
     c_planar = G.planar[c];
     if (MODE == TILE_STAGE && A.dt_mode == DT_READ) c_dt = A.dt[c];
-    if (MODE == TILE_STAGE && A.a != 0.f) {
+    if (MODE == TILE_STAGE && A.a != T(0.)) {
 #pragma unroll
       for (int q = 0; q < 4; q++) c_u0[q] = A.u0.q[q][c];
     }
@@ -255,7 +268,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int
   // Tiles whose whole window lies inside the grid (all but a ring of them)
   // take the direct loads; the block-uniform test costs no divergence.
   const bool inner = i0 >= 2 && j0 >= 2 && i0 + TX + 2 <= ni && j0 + TY + 2 <= nj;
-  auto fetch = [&](int i, int j) -> Vec4<float> {
+  auto fetch = [&](int i, int j) -> Vec4<T> {
     if (inner) {
       const int c = i + j * ni;
       return {A.src.q[0][c], A.src.q[1][c], A.src.q[2][c], A.src.q[3][c]};
@@ -266,9 +279,9 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int
   // ghost cell (gi in [ni, ni + 1] or gj in [nj, nj + 1]) that the faces
   // near the edge still need.
   if (!(gi >= ni && gj >= nj) && gi <= ni + 1 && gj <= nj + 1) {
-    const Vec4<float> q = fetch(gi, gj);
+    const Vec4<T> q = fetch(gi, gj);
     if (valid) own = q;
-    Prim<float> w = to_prim(q.x, q.y, q.z, q.w, g);
+    Prim<T> w = to_prim(q.x, q.y, q.z, q.w, g);
     sp[0][lj + 2][li + 2] = w.rho;
     sp[1][lj + 2][li + 2] = w.u;
     sp[2][lj + 2][li + 2] = w.v;
@@ -288,8 +301,8 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int
     const int hi = i0 + wi - 2, hj = j0 + wj - 2;
     const bool oi = hi < 0 || hi >= ni, oj = hj < 0 || hj >= nj;
     if ((oi && oj) || hi > ni + 1 || hj > nj + 1) continue;
-    Vec4<float> q = fetch(hi, hj);
-    Prim<float> w = to_prim(q.x, q.y, q.z, q.w, g);
+    Vec4<T> q = fetch(hi, hj);
+    Prim<T> w = to_prim(q.x, q.y, q.z, q.w, g);
     sp[0][wj][wi] = w.rho;
     sp[1][wj][wi] = w.u;
     sp[2][wj][wi] = w.v;
@@ -297,37 +310,37 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int
   }
   __syncthreads();
 
-  auto prim_at = [&](int lj, int li) -> Prim<float> {
+  auto prim_at = [&](int lj, int li) -> Prim<T> {
     return {sp[0][lj][li], sp[1][lj][li], sp[2][lj][li], sp[3][lj][li]};
   };
 
   // Phase 2: faces. Face f of direction d at local (fx, fy). Inner tiles
   // (IN) have no boundary face and no face outside the grid, so their
   // instance drops the side tests and the wall mirror.
-  auto do_face = [&](auto inner_tag, int d, int fx, int fy, const float* pre) {
+  auto do_face = [&](auto inner_tag, int d, int fx, int fy, const T* pre) {
     constexpr bool IN = decltype(inner_tag)::value;
     const int gi = i0 + fx, gj = j0 + fy;
-    float nx, ny, area;
-    Vec4<float> f;
-    float lam = 0.f;
+    T nx, ny, area;
+    Vec4<T> f;
+    T lam = T(0.);
     if (d == 0) {
       if (!IN && (gi > ni || gj >= nj)) return;
       const int fidx = gi + gj * (ni + 1);
       if (pre) { nx = pre[0]; ny = pre[1]; area = pre[2]; }
       else { nx = G.inx[fidx]; ny = G.iny[fidx]; area = G.ia[fidx]; }
       const int side = IN ? -1 : gi == 0 ? IMIN : gi == ni ? IMAX : -1;
-      const Prim<float> l0 = prim_at(fy + 2, fx + 1), r0 = prim_at(fy + 2, fx + 2);
+      const Prim<T> l0 = prim_at(fy + 2, fx + 1), r0 = prim_at(fy + 2, fx + 2);
       if (side >= 0 && A.B->face_flux[side < 0 ? 0 : side] != nullptr) {
-        const float* b = A.B->face_flux[side] + 4 * gj;
+        const T* b = A.B->face_flux[side] + 4 * gj;
         f = {b[0], b[1], b[2], b[3]};
       } else {
         const int wall = side >= 0 && A.B->kind[side < 0 ? 0 : side] == WALL ? (side == IMIN ? -1 : 1) : 0;
-        f = face_flux<float, REC, FLUX>(LIM, prim_at(fy + 2, fx), l0, r0, prim_at(fy + 2, fx + 3), nx, ny,
+        f = face_flux<T, REC, FLUX>(LIM, prim_at(fy + 2, fx), l0, r0, prim_at(fy + 2, fx + 3), nx, ny,
                                         wall, g);
       }
       if (MODE == TILE_IMPLICIT) {
-        float rl = gi > 0 ? bs_abs(l0.u * nx + l0.v * ny) + bs_sqrt(bs_div(g * l0.p, l0.rho)) : 0.f;
-        float rr = gi < ni ? bs_abs(r0.u * nx + r0.v * ny) + bs_sqrt(bs_div(g * r0.p, r0.rho)) : 0.f;
+        T rl = gi > 0 ? bs_abs(l0.u * nx + l0.v * ny) + bs_sqrt(bs_div(g * l0.p, l0.rho)) : T(0.);
+        T rr = gi < ni ? bs_abs(r0.u * nx + r0.v * ny) + bs_sqrt(bs_div(g * r0.p, r0.rho)) : T(0.);
         lam = bs_max(rl, rr) * area;
         A.lam_i[fidx] = lam;
       }
@@ -347,18 +360,18 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int
       if (pre) { nx = pre[0]; ny = pre[1]; area = pre[2]; }
       else { nx = G.jnx[fidx]; ny = G.jny[fidx]; area = G.ja[fidx]; }
       const int side = IN ? -1 : gj == 0 ? JMIN : gj == nj ? JMAX : -1;
-      const Prim<float> l0 = prim_at(fy + 1, fx + 2), r0 = prim_at(fy + 2, fx + 2);
+      const Prim<T> l0 = prim_at(fy + 1, fx + 2), r0 = prim_at(fy + 2, fx + 2);
       if (side >= 0 && A.B->face_flux[side < 0 ? 0 : side] != nullptr) {
-        const float* b = A.B->face_flux[side] + 4 * gi;
+        const T* b = A.B->face_flux[side] + 4 * gi;
         f = {b[0], b[1], b[2], b[3]};
       } else {
         const int wall = side >= 0 && A.B->kind[side < 0 ? 0 : side] == WALL ? (side == JMIN ? -1 : 1) : 0;
-        f = face_flux<float, REC, FLUX>(LIM, prim_at(fy, fx + 2), l0, r0, prim_at(fy + 3, fx + 2), nx, ny,
+        f = face_flux<T, REC, FLUX>(LIM, prim_at(fy, fx + 2), l0, r0, prim_at(fy + 3, fx + 2), nx, ny,
                                         wall, g);
       }
       if (MODE == TILE_IMPLICIT) {
-        float rl = gj > 0 ? bs_abs(l0.u * nx + l0.v * ny) + bs_sqrt(bs_div(g * l0.p, l0.rho)) : 0.f;
-        float rr = gj < nj ? bs_abs(r0.u * nx + r0.v * ny) + bs_sqrt(bs_div(g * r0.p, r0.rho)) : 0.f;
+        T rl = gj > 0 ? bs_abs(l0.u * nx + l0.v * ny) + bs_sqrt(bs_div(g * l0.p, l0.rho)) : T(0.);
+        T rr = gj < nj ? bs_abs(r0.u * nx + r0.v * ny) + bs_sqrt(bs_div(g * r0.p, r0.rho)) : T(0.);
         lam = bs_max(rl, rr) * area;
         A.lam_j[fidx] = lam;
       }
@@ -393,28 +406,28 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int
   double r2 = 0.0;
   if (valid) {
     const int c = gi + gj * ni;
-    const Prim<float> w = prim_at(lj + 2, li + 2);
-    float r[4];
+    const Prim<T> w = prim_at(lj + 2, li + 2);
+    T r[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) r[q] = fi[q][lj][li] - fi[q][lj][li + 1] + fj[q][lj][li] - fj[q][lj + 1][li];
-    const float planar = c_planar;
+    const T planar = c_planar;
     if (A.axisymmetric) r[2] += w.p * planar;
-    const float vol = c_vol;
+    const T vol = c_vol;
     if (A.partials != nullptr) {
       double rv = (double)r[0] / (double)vol;
       r2 = rv * rv;
     }
     if (MODE == TILE_STAGE) {
-      float dt;
+      T dt;
       if (A.dt_mode == DT_COMPUTE_LOCAL) {
         // euler2d.ts:1183-1205 computeTimeSteps: mean area vector of the
         // two faces in each direction.
-        const float a = bs_sqrt(bs_div(g * w.p, w.rho));
-        const float six = 0.5f * (xi[0][lj][li] + xi[0][lj][li + 1]);
-        const float siy = 0.5f * (xi[1][lj][li] + xi[1][lj][li + 1]);
-        const float sjx = 0.5f * (xj[0][lj][li] + xj[0][lj + 1][li]);
-        const float sjy = 0.5f * (xj[1][lj][li] + xj[1][lj + 1][li]);
-        const float lam = bs_abs(w.u * six + w.v * siy) + a * bs_sqrt(six * six + siy * siy) +
+        const T a = bs_sqrt(bs_div(g * w.p, w.rho));
+        const T six = T(0.5) * (xi[0][lj][li] + xi[0][lj][li + 1]);
+        const T siy = T(0.5) * (xi[1][lj][li] + xi[1][lj][li + 1]);
+        const T sjx = T(0.5) * (xj[0][lj][li] + xj[0][lj + 1][li]);
+        const T sjy = T(0.5) * (xj[1][lj][li] + xj[1][lj + 1][li]);
+        const T lam = bs_abs(w.u * six + w.v * siy) + a * bs_sqrt(six * six + siy * siy) +
                           bs_abs(w.u * sjx + w.v * sjy) + a * bs_sqrt(sjx * sjx + sjy * sjy);
         dt = A.cfl * vol * bs_rcp(lam);
         A.dt[c] = dt;
@@ -423,25 +436,25 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int
       } else {
         dt = c_dt;
       }
-      const float k = dt * bs_rcp(vol);
-      Vec4<float> out;
-      const float srcq[4] = {own.x, own.y, own.z, own.w};
-      float o[4];
+      const T k = dt * bs_rcp(vol);
+      Vec4<T> out;
+      const T srcq[4] = {own.x, own.y, own.z, own.w};
+      T o[4];
       // a u0 + b (src + k R) with a = 1 - b (every Shu-Osher stage here),
       // written as u0 + b ((src - u0) + k R): the FP32 coefficients 1/3 and
       // 2/3 do not sum to 1, and the plain form would add (a + b - 1) u0 per
       // step, a source that shifts the steady state by up to 1e-5 in CD.
-      if (A.a != 0.f) {
+      if (A.a != T(0.)) {
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-          const float u0q = c_u0[q];
+          const T u0q = c_u0[q];
           o[q] = u0q + A.b * ((srcq[q] - u0q) + k * r[q]);
         }
       } else {
 #pragma unroll
         for (int q = 0; q < 4; q++) o[q] = srcq[q] + A.b * (k * r[q]);
       }
-      if (A.dt_mode == DT_GLOBAL && dt <= 0.f) {
+      if (A.dt_mode == DT_GLOBAL && dt <= T(0.)) {
         // A zero time step (global stepping already at t_end) leaves the
         // state unchanged (euler2d.wgsl update_cells).
 #pragma unroll
@@ -456,8 +469,8 @@ __device__ __forceinline__ void tile_body(const TileArgs& A, int bx, int by, int
       // scalar diagonal and the residual
       // dQ = R / D on the cells with i + j even.
       const double cfl = A.control->cfl;
-      const float scale = (float)(0.5 * ((double)A.omega + 1.0 / cfl));
-      float d = scale * (xi[0][lj][li] + xi[0][lj][li + 1] + xj[0][lj][li] + xj[0][lj + 1][li]);
+      const T scale = (T)(0.5 * ((double)A.omega + 1.0 / cfl));
+      T d = scale * (xi[0][lj][li] + xi[0][lj][li + 1] + xj[0][lj][li] + xj[0][lj + 1][li]);
       if (A.axisymmetric) d += planar * (bs_abs(w.v) + bs_sqrt(bs_div(g * w.p, w.rho)));
       A.diag[c] = d;
       // The first red half-sweep (dQ = R / D on red cells, 0 on black) is
@@ -487,12 +500,12 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
   return x * q + (x < r ? x : r) + k;
 }
 
-template <int TX, int TY, int REC, int LIM, int FLUX, int MODE>
-__global__ __launch_bounds__(TX * TY) void tile_kernel(TileArgs A) {
+template <typename T, int TX, int TY, int REC, int LIM, int FLUX, int MODE>
+__global__ __launch_bounds__(TX * TY) void tile_kernel(TileArgsT<T> A) {
   // One-dimensional grid of tiles_x * tiles_y blocks.
   const int tiles_x = (A.G.ni + TX - 1) / TX;
   const int t = xcd_tile(blockIdx.x, gridDim.x);
-  tile_body<TX, TY, REC, LIM, FLUX, MODE>(A, t % tiles_x, t / tiles_x, t);
+  tile_body<T, TX, TY, REC, LIM, FLUX, MODE>(A, t % tiles_x, t / tiles_x, t);
 }
 
 // A batch of grids in one launch: block b takes work item b (the case's
@@ -510,7 +523,7 @@ __global__ __launch_bounds__(TX * TY) void batch_tile_kernel(const TileArgs* __r
   // Work items are ordered by case and tile row; each XCD takes one
   // contiguous run of them (xcd_tile).
   const WorkItem w = work[xcd_tile(blockIdx.x, gridDim.x)];
-  tile_body<TX, TY, REC, LIM, FLUX, MODE>(args[w.args], w.bx, w.by, w.slot);
+  tile_body<float, TX, TY, REC, LIM, FLUX, MODE>(args[w.args], w.bx, w.by, w.slot);
 }
 
 // Deterministic FP64 sum of n partials by one block; writes sqrt(sum / cells)
@@ -527,50 +540,52 @@ __global__ __launch_bounds__(256) void finalize_sum(const double* partials, int 
 // Per-cell global time step and block minima (euler2d.wgsl
 // compute_time_steps); the finalize takes the minimum and clamps it to the
 // FP64 clock's remaining time (finalize_time_step).
-__global__ __launch_bounds__(256) void time_step_kernel(GridDev G, StateDev s, float g, float cfl,
-                                                        float* partial_min) {
-  __shared__ float red[256];
+template <typename T>
+__global__ __launch_bounds__(256) void time_step_kernel(GridDevT<T> G, StateDevT<T> s, T g, T cfl,
+                                                        T* partial_min) {
+  __shared__ T red[256];
   const int c = blockIdx.x * 256 + threadIdx.x;
   const int ni = G.ni;
-  float dt = 3.0e38f;
+  T dt = T(3.0e38);
   if (c < G.ni * G.nj) {
     const int i = c % ni, j = c / ni;
-    Prim<float> w = to_prim(s.q[0][c], s.q[1][c], s.q[2][c], s.q[3][c], g);
-    float a = bs_sqrt(bs_div(g * w.p, w.rho));
+    Prim<T> w = to_prim(s.q[0][c], s.q[1][c], s.q[2][c], s.q[3][c], g);
+    T a = bs_sqrt(bs_div(g * w.p, w.rho));
     int f0 = i + j * (ni + 1), f1 = f0 + 1, h0 = c, h1 = c + ni;
-    float six = 0.5f * (G.inx[f0] * G.ia[f0] + G.inx[f1] * G.ia[f1]);
-    float siy = 0.5f * (G.iny[f0] * G.ia[f0] + G.iny[f1] * G.ia[f1]);
-    float sjx = 0.5f * (G.jnx[h0] * G.ja[h0] + G.jnx[h1] * G.ja[h1]);
-    float sjy = 0.5f * (G.jny[h0] * G.ja[h0] + G.jny[h1] * G.ja[h1]);
-    float lam = bs_abs(w.u * six + w.v * siy) + a * bs_sqrt(six * six + siy * siy) +
+    T six = T(0.5) * (G.inx[f0] * G.ia[f0] + G.inx[f1] * G.ia[f1]);
+    T siy = T(0.5) * (G.iny[f0] * G.ia[f0] + G.iny[f1] * G.ia[f1]);
+    T sjx = T(0.5) * (G.jnx[h0] * G.ja[h0] + G.jnx[h1] * G.ja[h1]);
+    T sjy = T(0.5) * (G.jny[h0] * G.ja[h0] + G.jny[h1] * G.ja[h1]);
+    T lam = bs_abs(w.u * six + w.v * siy) + a * bs_sqrt(six * six + siy * siy) +
                 bs_abs(w.u * sjx + w.v * sjy) + a * bs_sqrt(sjx * sjx + sjy * sjy);
     dt = cfl * G.vol[c] * bs_rcp(lam);
-    if (!(dt == dt)) dt = -1.f;  // NaN marks the block
+    if (!(dt == dt)) dt = -T(1.);  // NaN marks the block
   }
   red[threadIdx.x] = dt;
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st) red[threadIdx.x] = fminf(red[threadIdx.x], red[threadIdx.x + st]);
+    if (threadIdx.x < st) red[threadIdx.x] = bs_fmin(red[threadIdx.x], red[threadIdx.x + st]);
     __syncthreads();
   }
   if (threadIdx.x == 0) partial_min[blockIdx.x] = red[0];
 }
 
-__global__ __launch_bounds__(256) void finalize_time_step(const float* partial_min, int n, Control* ctl,
-                                                          float* dt_out) {
-  __shared__ float red[256];
-  float v = 3.0e38f;
-  for (int k = threadIdx.x; k < n; k += 256) v = fminf(v, partial_min[k]);
+template <typename T>
+__global__ __launch_bounds__(256) void finalize_time_step(const T* partial_min, int n, Control* ctl,
+                                                          T* dt_out) {
+  __shared__ T red[256];
+  T v = T(3.0e38);
+  for (int k = threadIdx.x; k < n; k += 256) v = bs_fmin(v, partial_min[k]);
   red[threadIdx.x] = v;
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st) red[threadIdx.x] = fminf(red[threadIdx.x], red[threadIdx.x + st]);
+    if (threadIdx.x < st) red[threadIdx.x] = bs_fmin(red[threadIdx.x], red[threadIdx.x + st]);
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    float m = red[0];
+    T m = red[0];
     double dt;
-    if (!(m > 0.f && m < 3.0e38f)) {
+    if (!(m > T(0.) && m < T(3.0e38))) {
       atomicOr(&ctl->status, STATUS_NON_PHYSICAL);
       dt = 0.0;
     } else {
@@ -579,7 +594,7 @@ __global__ __launch_bounds__(256) void finalize_time_step(const float* partial_m
       dt = remaining <= limit ? 0.0 : (m < remaining ? (double)m : remaining);
     }
     ctl->dt = dt;
-    *dt_out = (float)dt;
+    *dt_out = (T)dt;
   }
 }
 
@@ -814,33 +829,36 @@ __global__ __launch_bounds__(RTX * RTY) void batch_relax_tile_kernel(const Relax
 // FP64 boundary fluxes and drag (euler2d.ts:499-525, forces.ts:19-29).
 // ---------------------------------------------------------------------------
 
-struct WallArgs {
-  GridDev G;
-  StateDev s;
+template <typename S>
+struct WallArgsT {
+  GridDevT<S> G;
+  StateDevT<S> s;
   int side;
   int count;
   int kind;                 // boundary kind of the side
   double state[4];          // primitive outside an inflow / far-field side
-  const float* face_cons;   // FACE_STATES side
+  const S* face_cons;       // FACE_STATES side
   const double* face_geo;   // per face: x, y, nx, ny, area (FP64, normals towards increasing index)
   double gamma;             // configured gamma (FP64)
-  double gamma32;           // the FP32 gamma the state was computed with
+  double gamma32;           // the gamma the state was computed with (FP32 for an FP32 state)
   int rec, limiter, flux;
   double* out;              // 9 per face, or null
   double p_inf;
   double* partials;         // per block: sum of (F_x - p_inf n_x) A, or null
 };
+using WallArgs = WallArgsT<float>;
 
-template <typename T>
-__device__ __forceinline__ Prim<T> prim64(const StateDev& s, int c, T g32) {
+template <typename T, typename S>
+__device__ __forceinline__ Prim<T> prim64(const StateDevT<S>& s, int c, T g32) {
   T rho = (T)s.q[0][c], mu = (T)s.q[1][c], mv = (T)s.q[2][c], e = (T)s.q[3][c];
   return {rho, mu / rho, mv / rho, (g32 - T(1)) * (e - (T(0.5) * (mu * mu + mv * mv)) / rho)};
 }
 
 // Primitive state of cell (i, j) or a ghost cell, in FP64, with the
 // boundary conditions of euler2d.ts:1361-1403 applied to primitives.
-__device__ Prim<double> wall_cell(const WallArgs& A, const int bkind[4], const double bstate[4][4],
-                                  const float* const bcons[4], int i, int j) {
+template <typename S>
+__device__ Prim<double> wall_cell(const WallArgsT<S>& A, const int bkind[4], const double bstate[4][4],
+                                  const S* const bcons[4], int i, int j) {
   const int ni = A.G.ni, nj = A.G.nj;
   if (i >= 0 && i < ni && j >= 0 && j < nj) return prim64<double>(A.s, i + j * ni, A.gamma32);
   int side, t, mi, mj, wi, wj;
@@ -864,7 +882,7 @@ __device__ Prim<double> wall_cell(const WallArgs& A, const int bkind[4], const d
   const int kind = bkind[side];
   if (kind == INFLOW) return {bstate[side][0], bstate[side][1], bstate[side][2], bstate[side][3]};
   if (kind == FACE_STATES) {
-    const float* b = bcons[side] + 4 * t;
+    const S* b = bcons[side] + 4 * t;
     double rho = b[0], mu = b[1], mv = b[2], e = b[3];
     return {rho, mu / rho, mv / rho, (A.gamma32 - 1.0) * (e - (0.5 * (mu * mu + mv * mv)) / rho)};
   }
@@ -885,16 +903,18 @@ __device__ Prim<double> wall_cell(const WallArgs& A, const int bkind[4], const d
   return w;
 }
 
-struct WallBoundary {
+template <typename S>
+struct WallBoundaryT {
   int kind[4];
   double state[4][4];
-  const float* cons[4];
+  const S* cons[4];
 };
+using WallBoundary = WallBoundaryT<float>;
 
 // Block `block` evaluates faces 256 block ... of the side; its drag partial
 // goes to partials[slot].
-template <int REC, int FLUX>
-__device__ __forceinline__ void wall_body(const WallArgs& A, const WallBoundary& WB, int block, int slot) {
+template <typename S, int REC, int FLUX>
+__device__ __forceinline__ void wall_body(const WallArgsT<S>& A, const WallBoundaryT<S>& WB, int block, int slot) {
   __shared__ double red[4];
   const int t = block * 256 + threadIdx.x;
   double contrib = 0.0;
@@ -925,9 +945,9 @@ __device__ __forceinline__ void wall_body(const WallArgs& A, const WallBoundary&
   }
 }
 
-template <int REC, int FLUX>
-__global__ __launch_bounds__(256) void wall_flux_kernel(WallArgs A, WallBoundary WB) {
-  wall_body<REC, FLUX>(A, WB, blockIdx.x, blockIdx.x);
+template <typename S, int REC, int FLUX>
+__global__ __launch_bounds__(256) void wall_flux_kernel(WallArgsT<S> A, WallBoundaryT<S> WB) {
+  wall_body<S, REC, FLUX>(A, WB, blockIdx.x, blockIdx.x);
 }
 
 }  // namespace bs

@@ -44,6 +44,9 @@ template <> __device__ __forceinline__ double bs_div<double>(double a, double b)
 template <typename T> __device__ __forceinline__ T bs_abs(T x) { return x < T(0) ? -x : x; }
 template <typename T> __device__ __forceinline__ T bs_min(T a, T b) { return a < b ? a : b; }
 template <typename T> __device__ __forceinline__ T bs_max(T a, T b) { return a > b ? a : b; }
+// Minimum that ignores a NaN operand (IEEE minNum), for reductions.
+__device__ __forceinline__ float bs_fmin(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ double bs_fmin(double a, double b) { return fmin(a, b); }
 
 // gas.ts:43-51 pressureFromConserved2D; gas.wgsl to_primitive_2d.
 template <typename T>

@@ -161,15 +161,22 @@ def compute_metrics(grid, geometry="axisymmetric"):
 
 
 class GpuEuler2D:
-    """The HIP twin of the reference's GpuEuler2D."""
+    """The HIP twin of the reference's GpuEuler2D. precision="fp64" gives
+    the precision extension: state, metrics and stencil arithmetic in FP64
+    (in place of the reference's two-float engine, euler2d-two-float.ts),
+    explicit steps only."""
 
-    def __init__(self, ctx: Context, config: Euler2DConfig):
+    def __init__(self, ctx: Context, config: Euler2DConfig, precision: str = "fp32"):
+        if precision not in ("fp32", "fp64"):
+            raise RangeError(f"unknown precision {precision!r}")
         self.config = config
         self.ctx = ctx
+        self.precision = precision
         keep: list = []
         c = _c_config(config, keep)
         h = C.c_void_p()
-        check(N.lib().bs_solver_create(ctx._h, C.byref(c), C.byref(h)))
+        create = N.lib().bs_solver_create_fp64 if precision == "fp64" else N.lib().bs_solver_create
+        check(create(ctx._h, C.byref(c), C.byref(h)))
         self._h = h
         self.ni, self.nj = config.grid.ni, config.grid.nj
         self.metrics = compute_metrics(config.grid, config.geometry)
@@ -180,9 +187,9 @@ class GpuEuler2D:
                 self.setFaceStates(side, bc["states"])
 
     @staticmethod
-    def create(ctx: Context, config: Euler2DConfig) -> "GpuEuler2D":
-        """euler2d-gpu.ts:217-291."""
-        return GpuEuler2D(ctx, config)
+    def create(ctx: Context, config: Euler2DConfig, precision: str = "fp32") -> "GpuEuler2D":
+        """euler2d-gpu.ts:217-291 (FP32); euler2d-two-float.ts:151 (FP64)."""
+        return GpuEuler2D(ctx, config, precision)
 
     @property
     def n(self):
@@ -216,6 +223,12 @@ class GpuEuler2D:
         self.setCellPrimitives(np.array([field(a, b) for a, b in zip(cx, cy)], dtype=np.float64))
 
     def setState(self, rho, mu, mv, energy):
+        """Conserved arrays; float64 arrays go through the FP64 entry
+        (exact on an FP64 solver, rounded once on an FP32 one)."""
+        if any(np.asarray(a).dtype == np.float64 for a in (rho, mu, mv, energy)):
+            arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, mu, mv, energy)]
+            check(N.lib().bs_set_state_f64(self._h, *[N.dptr(a) for a in arrs]))
+            return
         arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (rho, mu, mv, energy)]
         check(N.lib().bs_set_state(self._h, *[N.fptr(a) for a in arrs]))
 
@@ -240,10 +253,15 @@ class GpuEuler2D:
         return r.reshape(self.n, 4)
 
     def readState(self) -> GpuState2D:
-        """euler2d-gpu.ts:549-572."""
-        a = [np.zeros(self.n, dtype=np.float32) for _ in range(4)]
+        """euler2d-gpu.ts:549-572: float32 arrays, or float64 ones from an
+        FP64 solver (readConserved, euler2d-two-float.ts:486-501)."""
         rep = N.Report()
-        check(N.lib().bs_read_state(self._h, *[N.fptr(v) for v in a], C.byref(rep)))
+        if self.precision == "fp64":
+            a = [np.zeros(self.n, dtype=np.float64) for _ in range(4)]
+            check(N.lib().bs_read_state_f64(self._h, *[N.dptr(v) for v in a], C.byref(rep)))
+        else:
+            a = [np.zeros(self.n, dtype=np.float32) for _ in range(4)]
+            check(N.lib().bs_read_state(self._h, *[N.fptr(v) for v in a], C.byref(rep)))
         return GpuState2D(rep.time, int(rep.steps), *a)
 
     def setFaceStates(self, side: str, states):
@@ -308,8 +326,9 @@ class GpuEuler2D:
 
 def gpu_primitives(state: GpuState2D, gamma: float) -> np.ndarray:
     """gpuPrimitive2D (euler2d-gpu.ts:136-146): FP64 primitives of a state
-    read back, pressure with the FP32 gamma; (n, 4)."""
-    g32 = float(np.float32(gamma))
+    read back, pressure with the FP32 gamma for an FP32 state (the gamma
+    itself for an FP64 state); (n, 4)."""
+    g32 = float(gamma) if state.rho.dtype == np.float64 else float(np.float32(gamma))
     rho = state.rho.astype(np.float64)
     mu = state.mu.astype(np.float64)
     mv = state.mv.astype(np.float64)

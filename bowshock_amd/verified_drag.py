@@ -9,7 +9,7 @@ local steps) in batches of 500 r steps until the drag itself settles
 (SettlingValue), then checked: the flow must leave the grid supersonic
 and the outer boundary must still see the free stream. The levels feed
 refine_until_verified. The drag is integrated in FP64 on the GPU from the
-FP32 state.
+FP32 state, or from the FP64 state of the FP64 solver.
 """
 from __future__ import annotations
 
@@ -20,7 +20,7 @@ from typing import Callable, List, Optional
 import numpy as np
 
 from . import grids
-from .engine import Context, Euler2DConfig, GpuEuler2D
+from .engine import Context, Euler2DConfig, GpuEuler2D, gpu_primitives
 from .verification import SettlingValue, VerifiedValue, refine_until_verified
 
 # verified-drag.ts:102-200
@@ -63,13 +63,8 @@ class VerifiedDrag:
 
 def _primitives(gpu: GpuEuler2D, gamma: float) -> np.ndarray:
     """(nj, ni, 4) primitives of the current state (gpuPrimitive2D,
-    euler2d-gpu.ts:136-146: pressure with the FP32 gamma)."""
-    s = gpu.readState()
-    g = float(np.float32(gamma))
-    rho = s.rho.astype(np.float64)
-    mu, mv, e = s.mu.astype(np.float64), s.mv.astype(np.float64), s.energy.astype(np.float64)
-    p = (g - 1) * (e - 0.5 * (mu * mu + mv * mv) / rho)
-    return np.stack([rho, mu / rho, mv / rho, p], axis=1).reshape(gpu.nj, gpu.ni, 4)
+    euler2d-gpu.ts:136-146)."""
+    return gpu_primitives(gpu.readState(), gamma).reshape(gpu.nj, gpu.ni, 4)
 
 
 def outflow_mach(prim: np.ndarray, gamma: float) -> float:
@@ -92,7 +87,10 @@ def outer_deviation(prim: np.ndarray, free, gamma: float) -> float:
 def verified_forebody_drag(ctx: Context, body: grids.Body, *, mach: float, gamma: float = 1.4,
                            tolerance: float = 1e-4, max_refinement: int = 4, noise: float = GPU_NOISE,
                            floor: float = DRAG_FLOOR, order_tolerance: float = ORDER_TOLERANCE,
-                           on_level: Optional[Callable[[LevelReport], None]] = None) -> VerifiedDrag:
+                           on_level: Optional[Callable[[LevelReport], None]] = None,
+                           precision: str = "fp32") -> VerifiedDrag:
+    """verified-drag.ts:230-319 on the HIP engine. precision="fp64" solves
+    on the FP64 solver, which settles at the CPU engine's noise (1e-12)."""
     free, p_inf, q_inf = grids.free_stream_at(mach, gamma)
     reports: List[LevelReport] = []
     last = [1]
@@ -105,7 +103,7 @@ def verified_forebody_drag(ctx: Context, body: grids.Body, *, mach: float, gamma
         cfg = Euler2DConfig(grid=bg.grid, boundaries=bcs, geometry="axisymmetric", gamma=gamma, flux="hllc",
                             limiter="vanAlbada", reconstruction="sharedScaled", timeIntegrator="ssprk2", cfl=0.8,
                             timeStepping="local")
-        gpu = GpuEuler2D.create(ctx, cfg)
+        gpu = GpuEuler2D.create(ctx, cfg, precision)
         try:
             gpu.setCellPrimitives(np.array(free))
             settling = SettlingValue(noise)

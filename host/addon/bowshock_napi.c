@@ -130,12 +130,18 @@ static napi_value js_ctx_destroy(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
-/* solverCreate(ctx, cfg): cfg = {ni, nj, x: Float64Array, y: Float64Array,
- * geometry, gamma, flux, limiter, reconstruction, integrator, cfl,
- * timeStepping, boundaries: [{kind, rho, u, v, p} x 4]} with integer codes. */
+/* solverCreate(ctx, cfg[, precision]): cfg = {ni, nj, x: Float64Array,
+ * y: Float64Array, geometry, gamma, flux, limiter, reconstruction,
+ * integrator, cfl, timeStepping, boundaries: [{kind, rho, u, v, p} x 4]}
+ * with integer codes; precision BS_PRECISION_FP32 (default) or _FP64. */
 static napi_value js_solver_create(napi_env env, napi_callback_info info) {
-  napi_value argv[2];
-  if (!get_args(env, info, 2, argv)) return NULL;
+  napi_value argv[3];
+  size_t argc = 3;
+  if (napi_get_cb_info(env, info, &argc, argv, NULL, NULL) != napi_ok || argc < 2) {
+    napi_throw_type_error(env, NULL, "missing arguments");
+    return NULL;
+  }
+  const int precision = argc > 2 ? i32(env, argv[2]) : BS_PRECISION_FP32;
   bs_ctx *ctx = (bs_ctx *)get_ptr(env, argv[0]);
   napi_value c = argv[1], v;
   bs_config cfg;
@@ -177,7 +183,7 @@ static napi_value js_solver_create(napi_env env, napi_callback_info info) {
   }
 #undef FIELD
   bs_solver *s = NULL;
-  bs_status st = bs_solver_create(ctx, &cfg, &s);
+  bs_status st = precision == BS_PRECISION_FP64 ? bs_solver_create_fp64(ctx, &cfg, &s) : bs_solver_create(ctx, &cfg, &s);
   if (st) return throw_status(env, st);
   napi_value ext;
   CHECK_NAPI(napi_create_external(env, s, NULL, NULL, &ext));
@@ -271,6 +277,27 @@ static napi_value js_read_state(napi_env env, napi_callback_info info) {
   bs_report r;
   memset(&r, 0, sizeof r);
   bs_status st = bs_read_state((bs_solver *)get_ptr(env, argv[0]), q[0], q[1], q[2], q[3], &r);
+  if (st) return throw_status(env, st);
+  napi_set_named_property(env, out, "report", report_object(env, &r));
+  return out;
+}
+
+/* readState64(solver, n): the state as Float64Arrays (bs_read_state_f64). */
+static napi_value js_read_state_f64(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  size_t n = (size_t)num(env, argv[1]);
+  double *q[4];
+  napi_value out, arr[4];
+  napi_create_object(env, &out);
+  const char *names[4] = {"rho", "mu", "mv", "energy"};
+  for (int k = 0; k < 4; k++) {
+    arr[k] = new_f64(env, n, &q[k]);
+    napi_set_named_property(env, out, names[k], arr[k]);
+  }
+  bs_report r;
+  memset(&r, 0, sizeof r);
+  bs_status st = bs_read_state_f64((bs_solver *)get_ptr(env, argv[0]), q[0], q[1], q[2], q[3], &r);
   if (st) return throw_status(env, st);
   napi_set_named_property(env, out, "report", report_object(env, &r));
   return out;
@@ -454,6 +481,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"setPrimitives", NULL, js_set_primitives, NULL, NULL, NULL, napi_enumerable, NULL},
       {"setState", NULL, js_set_state, NULL, NULL, NULL, napi_enumerable, NULL},
       {"readState", NULL, js_read_state, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"readState64", NULL, js_read_state_f64, NULL, NULL, NULL, napi_enumerable, NULL},
       {"setFaceStates", NULL, js_set_face_states, NULL, NULL, NULL, napi_enumerable, NULL},
       {"setFaceFluxes", NULL, js_set_face_fluxes, NULL, NULL, NULL, napi_enumerable, NULL},
       {"residualNorm", NULL, js_residual_norm, NULL, NULL, NULL, napi_enumerable, NULL},

@@ -76,10 +76,11 @@ export interface GridMetrics {
 export interface GpuState2D {
   readonly time: number;
   readonly steps: number;
-  readonly rho: Float32Array;
-  readonly mu: Float32Array;
-  readonly mv: Float32Array;
-  readonly energy: Float32Array;
+  /** Float64Array from an FP64 solver. */
+  readonly rho: Float32Array | Float64Array;
+  readonly mu: Float32Array | Float64Array;
+  readonly mv: Float32Array | Float64Array;
+  readonly energy: Float32Array | Float64Array;
 }
 
 export interface BoundaryFaceFlux {
@@ -116,8 +117,14 @@ export class HipDevice {
 }
 
 export class GpuEuler2D {
-  static create(device: HipDevice | number, config: GpuEuler2DConfig): Promise<GpuEuler2D>;
+  /** options.precision 'fp64': state, metrics and arithmetic in FP64 (explicit steps only). */
+  static create(
+    device: HipDevice | number,
+    config: GpuEuler2DConfig,
+    options?: { readonly precision?: 'fp32' | 'fp64' },
+  ): Promise<GpuEuler2D>;
   readonly config: GpuEuler2DConfig;
+  readonly precision: 'fp32' | 'fp64';
   readonly metrics: GridMetrics;
   setInitialCondition(field: (x: number, y: number) => Primitive2D): void;
   setCellPrimitives(cell: (i: number, j: number) => Primitive2D): void;

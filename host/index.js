@@ -197,17 +197,26 @@ class GpuEuler2D {
    * euler2d-gpu.ts:217-291: validate, compute the metrics (FP64, host) and
    * upload them. `device` is a HipDevice or a device ordinal.
    */
-  static async create(device, config) {
+  static async create(device, config, options = {}) {
     const dev = device instanceof HipDevice ? device : new HipDevice(device);
-    return new GpuEuler2D(dev, config);
+    return new GpuEuler2D(dev, config, options);
   }
 
-  constructor(device, config) {
+  /**
+   * options.precision: 'fp32' (default) or 'fp64', the solver with its
+   * state, metrics and arithmetic in FP64 (explicit steps only), in place
+   * of the two-float engine (euler2d-two-float.ts:151).
+   */
+  constructor(device, config, options = {}) {
+    const precision = options.precision === undefined ? 'fp32' : options.precision;
+    if (precision !== 'fp32' && precision !== 'fp64') throw new RangeError(`unknown precision ${precision}`);
     this.config = config;
     this.device = device;
+    this.precision = precision;
     this.metrics = computeMetrics(config.grid, config.geometry);
     this.exclusive = new ExclusiveRunner('GpuEuler2D');
-    this.handle = call(() => native().solverCreate(device.handle, nativeConfig(config)));
+    this.handle = call(() =>
+      native().solverCreate(device.handle, nativeConfig(config), precision === 'fp64' ? 1 : 0));
     this.nCells = config.grid.ni * config.grid.nj;
     for (const side of SIDES) {
       const bc = config.boundaries[side];
@@ -257,7 +266,10 @@ class GpuEuler2D {
   /** euler2d-gpu.ts:549-572: GpuState2D. */
   async readState() {
     return this.exclusive.run(async () => {
-      const s = native().readState(this.handle, this.nCells);
+      const s =
+        this.precision === 'fp64'
+          ? native().readState64(this.handle, this.nCells)
+          : native().readState(this.handle, this.nCells);
       return { time: s.report.time, steps: s.report.steps, rho: s.rho, mu: s.mu, mv: s.mv, energy: s.energy };
     });
   }
@@ -343,13 +355,14 @@ class GpuEuler2D {
 
 /**
  * euler2d-gpu.ts:136-146 gpuPrimitive2D: primitives of cell c of a state
- * read back, the pressure with the 32-bit gamma the GPU solved with.
+ * read back, the pressure with the 32-bit gamma the GPU solved with (the
+ * gamma itself for the FP64 solver's state).
  */
 function gpuPrimitive2D(state, c, gamma) {
   const rho = state.rho[c];
   const mu = state.mu[c];
   const mv = state.mv[c];
-  const g = Math.fround(gamma);
+  const g = state.rho instanceof Float64Array ? gamma : Math.fround(gamma);
   return { rho, u: mu / rho, v: mv / rho, p: (g - 1) * (state.energy[c] - (0.5 * (mu * mu + mv * mv)) / rho) };
 }
 

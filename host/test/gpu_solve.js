@@ -40,6 +40,14 @@ const bs = require('..');
   const state = await gpu.readState();
   out.steps = state.steps;
   out.rhoSum = state.rho.reduce((a, b) => a + b, 0);
+  // The FP64 solver: the same steps, its state read back in FP64.
+  const gpu64 = await bs.GpuEuler2D.create(device, config, { precision: 'fp64' });
+  gpu64.setInitialCondition(() => free.state);
+  await gpu64.iterate(c.steps);
+  out.drag64 = await gpu64.forebodyDrag(free, 0.5);
+  const state64 = await gpu64.readState();
+  out.state64 = state64.rho instanceof Float64Array ? 'Float64Array' : 'other';
+  gpu64.destroy();
   // Implicit steps through the transonic adapter's solver.
   const solve = bs.hipTransonicEngine(device, { sweeps: 2, cfl: 10 });
   const grid = await solve({ config, freeStream: free, body: { nose: { baseRadius: 0.5 } } });

@@ -60,6 +60,8 @@ enum { BS_FLUX_HLLC = 0, BS_FLUX_RUSANOV = 1 };
 enum { BS_RK_FORWARD_EULER = 0, BS_RK_SSPRK2 = 1, BS_RK_SSPRK3 = 2 };
 enum { BS_GEOMETRY_PLANAR = 0, BS_GEOMETRY_AXISYMMETRIC = 1 };
 enum { BS_STEPPING_GLOBAL = 0, BS_STEPPING_LOCAL = 1 };
+/* Precision of a solver's state, metrics and stencil arithmetic. */
+enum { BS_PRECISION_FP32 = 0, BS_PRECISION_FP64 = 1 };
 /* Status word bits read back after every batch of steps
  * (src/engine-gpu/time-loop.ts:15 STATUS_NON_PHYSICAL). */
 enum { BS_STATUS_NON_PHYSICAL = 1u };
@@ -114,6 +116,17 @@ bs_status bs_ctx_destroy(bs_ctx *ctx);
  * (src/solver/config.ts:35-55), computes the metrics in FP64 on the host
  * (src/grid/structured.ts:94-168) and uploads them in FP32. */
 bs_status bs_solver_create(bs_ctx *ctx, const bs_config *config, bs_solver **out);
+/* The precision extension of the path: the same solver with its state,
+ * metrics, time steps and stencil arithmetic in FP64, for convergence
+ * below the FP32 residual floor. It replaces the reference's two-float
+ * engine (GpuEuler2DTwoFloat.create, src/engine-gpu/euler2d-two-float.ts:151,
+ * whose ~48-bit arithmetic WebGPU needs for lack of FP64) with native FP64.
+ * Explicit steps, residuals, state input/output, boundary fluxes and drag
+ * run as on an FP32 solver; bs_iterate_implicit returns BS_ERR_UNSUPPORTED.
+ * Initial states are formed with the configured (FP64) gamma. */
+bs_status bs_solver_create_fp64(bs_ctx *ctx, const bs_config *config, bs_solver **out);
+/* BS_PRECISION_FP32 or BS_PRECISION_FP64. */
+bs_status bs_solver_precision(bs_solver *s, int32_t *precision);
 /* GpuEuler2D.destroy (euler2d-gpu.ts:659); idempotent, NULL is accepted. */
 bs_status bs_solver_destroy(bs_solver *s);
 
@@ -140,6 +153,14 @@ bs_status bs_set_state(bs_solver *s, const float *rho, const float *mu, const fl
  * BS_ERR_NONPHYSICAL if the status word has the non-physical bit. */
 bs_status bs_read_state(bs_solver *s, float *rho, float *mu, float *mv, float *energy,
                         bs_report *report);
+/* bs_set_state and bs_read_state with FP64 arrays: exact on an FP64
+ * solver; rounded to (or widened from) FP32 on an FP32 solver. The read
+ * replaces GpuEuler2DTwoFloat.readConserved (euler2d-two-float.ts:486-501),
+ * which returns its ~48-bit state as FP64. */
+bs_status bs_set_state_f64(bs_solver *s, const double *rho, const double *mu, const double *mv,
+                           const double *energy);
+bs_status bs_read_state_f64(bs_solver *s, double *rho, double *mu, double *mv, double *energy,
+                            bs_report *report);
 
 /* setFaceStates (euler2d-gpu.ts:575-600): `count` primitive states, one per
  * face along a faceStates side. */

@@ -46,6 +46,10 @@ def test_node_solve_matches_oracle(tmp_path):
     cd = orc.forebody_drag(p_inf, q_inf, 0.5)
     assert abs(out["drag"] / cd - 1) < 1e-5
     assert abs(out["gpuDrag"] - out["drag"]) < 1e-12
+    # The FP64 solver through the facade: the oracle's drag within FP64
+    # round-off of 60 steps (the FP64 parity bar of the Python tests).
+    assert abs(out["drag64"] / cd - 1) < 1e-11
+    assert out["state64"] == "Float64Array"
     assert "await the previous operation first" in out["overlap"]
     assert out["steps"] == 60
     assert out["nonPhysical"] == "NonPhysicalStateError"
