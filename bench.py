@@ -57,7 +57,7 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=40, help="oracle steps timed for the CPU baseline (0: skip)")
     ap.add_argument("--no-graphs", action="store_true", help="launch kernel by kernel instead of HIP graphs")
     ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32",
-                    help="single: the FP32 solver, or the FP64 one (state, metrics and arithmetic in FP64)")
+                    help="single, implicit: the FP32 solver, or the FP64 one (state, metrics and residual in FP64)")
     ap.add_argument("--traffic", default=None, help="rocprofv3 PMC summary of the stage kernel")
     ap.add_argument("--workload", choices=("single", "implicit", "sweep", "population", "warmstart", "ladder", "verified"), default="single")
     ap.add_argument("--train-steps", type=int, default=3000, help="warmstart: Adam steps of the network")
@@ -528,7 +528,7 @@ def run_implicit(a):
                         limiter="vanAlbada", reconstruction="sharedScaled", timeIntegrator="ssprk3", cfl=0.8,
                         timeStepping="local")
     ctx = Context(0)
-    gpu = GpuEuler2D.create(ctx, cfg)
+    gpu = GpuEuler2D.create(ctx, cfg, a.precision)
     gpu.setCellPrimitives(np.array(free))
     gpu.setImplicitCfl(1.0)
     steps = max(12, a.steps // 10)
@@ -553,14 +553,17 @@ def run_implicit(a):
         "metric": "Euler cell-updates/s, %HBM roofline; converged CD solves/s warm-start vs plain",
         "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": steps, "warmup": max(a.warmup // 2, 12),
         "ms_per_step": 1e3 * elapsed / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32", "data": "synthetic: generated cone-cylinder grid, free-stream start",
+        "dtype": a.precision, "data": "synthetic: generated cone-cylinder grid, free-stream start",
         "config": {"workload": f"single solve, 15 deg cone + 4 cal cylinder, M{a.mach:g}, {a.ni}x{a.nj}, "
-                               "implicit red-black LU-SGS (2 sweeps, CFL 10), MUSCL van Albada shared-scaled, HLLC",
+                               "implicit red-black LU-SGS (2 sweeps, CFL 10), MUSCL van Albada shared-scaled, HLLC"
+                               + (", FP64 solver (FP32 relaxation)" if a.precision == "fp64" else ""),
                    "cells": cells, "kernel_us": {k: v[0] * 1e3 for k, v in timing.items()},
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None,
                      "kernel": "relax_tile_kernel<3> (three red-black half-sweeps)", "bytes_per_cell": 174.0,
+                     "bytes_model": "FP32 operands" + (" (a lower bound: the FP64 state and metrics are read "
+                                                       "at twice the width)" if a.precision == "fp64" else ""),
                      "us_per_launch": ms_relax * 1e3},
         "cpu_baseline": None,
     }

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libbowshock.so")
 BS_OK, BS_ERR_RANGE, BS_ERR_NONPHYSICAL, BS_ERR_MEMORY, BS_ERR_EXECUTION, BS_ERR_PRECISION, BS_ERR_BUSY, \
     BS_ERR_UNSUPPORTED = range(8)
 BS_PRECISION_FP32, BS_PRECISION_FP64 = 0, 1
+BS_RELAX_FLUX_DIFFERENCE, BS_RELAX_LINEARIZED = 0, 1
 
 # Every function include/bowshock.h declares, with its ctypes signature.
 _dp = C.POINTER(C.c_double)
@@ -63,6 +64,7 @@ SIGNATURES = {
     "bs_iterate_implicit": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_int32, C.c_double, C.c_double,
                                       C.POINTER(Report)]),
     "bs_enable_graphs": (C.c_int, [_vp, C.c_int32]),
+    "bs_set_relaxation": (C.c_int, [_vp, C.c_int32]),
     "bs_enable_timing": (C.c_int, [_vp, C.c_int32]),
     "bs_kernel_timing": (C.c_int, [_vp, C.c_int32, _dp, C.POINTER(C.c_int64)]),
     "bs_batch_create": (C.c_int, [_vp, C.POINTER(Config), C.c_int32, C.POINTER(_vp)]),
@@ -72,6 +74,7 @@ SIGNATURES = {
     "bs_batch_read_state": (C.c_int, [_vp, C.c_int32, _fp, _fp, _fp, _fp]),
     "bs_batch_set_active": (C.c_int, [_vp, C.POINTER(C.c_uint8)]),
     "bs_batch_set_tiling": (C.c_int, [_vp, C.c_int32]),
+    "bs_batch_set_relaxation": (C.c_int, [_vp, C.c_int32]),
     "bs_batch_iterate": (C.c_int, [_vp, C.c_int32]),
     "bs_batch_set_implicit_cfl": (C.c_int, [_vp, C.c_double]),
     "bs_batch_set_implicit_cfls": (C.c_int, [_vp, _dp]),

@@ -23,7 +23,8 @@ from typing import List, Optional, Sequence
 import numpy as np
 
 from . import _native as N
-from .engine import (Context, Euler2DConfig, GpuState2D, NonPhysicalStateError, RangeError, _c_config, check)
+from .engine import (RELAXATION_FORMS, Context, Euler2DConfig, GpuState2D, NonPhysicalStateError, RangeError,
+                     _c_config, check)
 
 STATUS_NON_PHYSICAL = 1
 
@@ -84,6 +85,11 @@ class GpuBatch2D:
             raise RangeError("one flag per case")
         check(N.lib().bs_batch_set_active(self._h, m.ctypes.data_as(C.POINTER(C.c_uint8))))
         self.active = m.copy()
+
+    def setRelaxation(self, form: str):
+        """"fluxDifference" (the reference's, default) or "linearized"
+        (bs_batch_set_relaxation)."""
+        check(N.lib().bs_batch_set_relaxation(self._h, RELAXATION_FORMS[form]))
 
     def setTiling(self, small_tile_below_cells: int):
         check(N.lib().bs_batch_set_tiling(self._h, int(small_tile_below_cells)))
@@ -158,6 +164,7 @@ class SolvePlan:
     sweeps: int = 2
     omega: float = 1.5
     growth: float = 1.2
+    relaxation: str = "fluxDifference"   # implicit: or "linearized" (bs_set_relaxation)
 
 
 class _Tracker:
@@ -220,6 +227,7 @@ def solve(batch: GpuBatch2D, p_inf, q_inf, base_radius, plan: Optional[SolvePlan
     refs = np.full(n, math.nan) if reference is None else np.asarray(reference, dtype=np.float64)
     trackers = [_Tracker(plan, float(refs[c])) for c in range(n)]
     if plan.implicit:
+        batch.setRelaxation(plan.relaxation)
         if cfl_start is None:
             batch.setImplicitCfl(plan.cfl_start)
         else:
@@ -256,6 +264,8 @@ def solve_single(gpu, p_inf: float, q_inf: float, base_radius: float, plan: Opti
     r = CaseResult()
     tracker = _Tracker(plan, math.nan if reference is None else float(reference))
     if plan.implicit:
+        if gpu.precision == "fp32":
+            gpu.setRelaxation(plan.relaxation)
         gpu.setImplicitCfl(plan.cfl_start)
     done = 0
     while True:

@@ -143,6 +143,7 @@ struct bs_batch {
   WorkItem* work_relax = nullptr;
   WorkItem* work_wall = nullptr;
   int n_tiles[2] = {0, 0}, n_relax = 0, n_wall = 0;
+  bool relax_linear = false;  // BS_RELAX_LINEARIZED
   int *seg_first = nullptr, *seg_count = nullptr, *seg_cells = nullptr, *seg_cases = nullptr;
   int *wall_first = nullptr, *wall_count = nullptr;
   int n_seg = 0;
@@ -411,12 +412,12 @@ bs_status batch_encode_implicit(bs_batch* b, int sweeps) {
     const int init = done == 0;
     const UpdateArgs* u = done + nh == halves ? upd : nullptr;
     const dim3 grid(b->n_relax), block(RTX * RTY);
-    if (nh == 3)
-      hipLaunchKernelGGL(batch_relax_tile_kernel<3>, grid, block, 0, b->stream, rel, b->work_relax, colour, init, u);
-    else if (nh == 2)
-      hipLaunchKernelGGL(batch_relax_tile_kernel<2>, grid, block, 0, b->stream, rel, b->work_relax, colour, init, u);
-    else
-      hipLaunchKernelGGL(batch_relax_tile_kernel<1>, grid, block, 0, b->stream, rel, b->work_relax, colour, init, u);
+    using K = void (*)(const RelaxArgs*, const WorkItem*, int, int, const UpdateArgs*);
+    static const K kernels[2][3] = {
+        {batch_relax_tile_kernel<1, false>, batch_relax_tile_kernel<2, false>, batch_relax_tile_kernel<3, false>},
+        {batch_relax_tile_kernel<1, true>, batch_relax_tile_kernel<2, true>, batch_relax_tile_kernel<3, true>}};
+    hipLaunchKernelGGL(kernels[b->relax_linear ? 1 : 0][nh - 1], grid, block, 0, b->stream, rel, b->work_relax,
+                       colour, init, u);
     colour = (colour + nh) & 1;
     done += nh;
   }
@@ -740,6 +741,16 @@ bs_status bs_batch_set_implicit_cfl(bs_batch* b, double cfl) {
   HIP_TRY(hipMemcpy(ctl.data(), b->control, ctl.size() * sizeof(Control), hipMemcpyDeviceToHost), "control");
   for (auto& c : ctl) c.cfl = cfl;
   HIP_TRY(hipMemcpy(b->control, ctl.data(), ctl.size() * sizeof(Control), hipMemcpyHostToDevice), "control");
+  return BS_OK;
+}
+
+bs_status bs_batch_set_relaxation(bs_batch* b, int32_t form) {
+  BATCH_GUARD(b);
+  if (form != BS_RELAX_FLUX_DIFFERENCE && form != BS_RELAX_LINEARIZED)
+    return fail(BS_ERR_RANGE, "unknown relaxation form %d", form);
+  const bool linear = form == BS_RELAX_LINEARIZED;
+  if (linear != b->relax_linear) drop_graph(b->implicit_graph);
+  b->relax_linear = linear;
   return BS_OK;
 }
 

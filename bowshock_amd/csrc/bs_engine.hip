@@ -125,6 +125,7 @@ struct bs_solver {
   int ni, nj, n;
   float gamma32;
   bool fp64 = false;  // state, metrics and stencil arithmetic in FP64
+  bool relax_linear = false;  // BS_RELAX_LINEARIZED (always on an FP64 solver)
   hipStream_t stream = nullptr;
   DeviceBuffers d;
   int cur = 0;
@@ -225,14 +226,11 @@ bs_status upload_boundary(bs_solver* s) {
 template <typename T>
 using TileFnT = void (*)(TileArgsT<T>);
 
-// FP64 solvers take explicit steps and residuals only; their implicit
-// mode is never instantiated (the entry point refuses it first).
 template <typename T, int REC, int LIM, int FLUX>
 TileFnT<T> pick_mode(int mode) {
   constexpr int X = Tile<T>::x, Y = Tile<T>::y;
   if (mode == TILE_STAGE) return tile_kernel<T, X, Y, REC, LIM, FLUX, TILE_STAGE>;
-  if constexpr (std::is_same<T, float>::value)
-    if (mode == TILE_IMPLICIT) return tile_kernel<T, X, Y, REC, LIM, FLUX, TILE_IMPLICIT>;
+  if (mode == TILE_IMPLICIT) return tile_kernel<T, X, Y, REC, LIM, FLUX, TILE_IMPLICIT>;
   return tile_kernel<T, X, Y, REC, LIM, FLUX, TILE_RESIDUAL>;
 }
 
@@ -679,6 +677,7 @@ static bs_status solver_create(bs_ctx* ctx, const bs_config* c, bool fp64, bs_so
   s->n = c->ni * c->nj;
   s->gamma32 = (float)c->gamma;
   s->fp64 = fp64;
+  s->relax_linear = fp64;
   const int ni = s->ni, nj = s->nj, n = s->n;
   const size_t nn = (size_t)(ni + 1) * (nj + 1);
   s->x.assign(c->x, c->x + nn);
@@ -1150,13 +1149,97 @@ static bs_status ensure_implicit(bs_solver* s) {
   return BS_OK;
 }
 
+// C++ helpers of the implicit entry point.
+extern "C++" {
+// One implicit step: prepare (residual, lambda A, diagonal), the
+// red-black half-sweeps (the first red one formed from R and D by the
+// first launch), the update into the next buffer. On an FP32 state the
+// last relaxation launch writes q + alpha dQ itself; on an FP64 state
+// the relaxation runs in FP32 on the rounded state and metrics, and an
+// FP64 update kernel follows.
+template <typename T>
+static bs_status encode_implicit(bs_solver* s, const StateDevT<float>& res, const StateDevT<float>& dq,
+                                 double cfl_max, int sweeps, double omega, double growth) {
+  const int src = s->cur, dst = (s->cur + 1) % 3;
+  TileArgsT<T> A = tile_args<T>(s);
+  A.src = state_of<T>(s, src);
+  A.omega = (T)omega;
+  for (int q = 0; q < 4; q++) {
+    A.res[q] = res.q[q];
+    A.dq[q] = dq.q[q];
+  }
+  A.diag = s->d.diag;
+  A.lam_i = s->d.lam_i;
+  A.lam_j = s->d.lam_j;
+  bs_status r = launch_tile(s, TILE_IMPLICIT, A, 1);
+  if (r) return r;
+  RelaxArgsT<T> R;
+  R.G = grid_of<T>(s);
+  R.q = state_of<T>(s, src);
+  for (int q = 0; q < 4; q++) {
+    R.res[q] = res.q[q];
+    R.dq[q] = dq.q[q];
+  }
+  R.diag = s->d.diag;
+  R.lam_i = s->d.lam_i;
+  R.lam_j = s->d.lam_j;
+  R.gamma = s->gamma32;
+  R.omega = (float)omega;
+  UpdateArgsT<T> U;
+  U.src = state_of<T>(s, src);
+  U.dst = state_of<T>(s, dst);
+  U.dq = dq;
+  U.n = s->n;
+  U.gamma = (T)state_gamma(s);
+  U.control = s->d.control;
+  U.cfl_max = cfl_max;
+  U.growth = growth;
+  constexpr bool fused = std::is_same<T, float>::value;
+  UpdateArgs Uf;
+  if constexpr (fused) Uf = U;
+  else memset(&Uf, 0, sizeof Uf);
+  // The 2 sweeps - 1 half-sweeps after the first red one, up to three per
+  // launch (relax_tile_kernel), starting with black.
+  using K = void (*)(RelaxArgsT<T>, int, int, UpdateArgs, int);
+  K kernels[2][3];
+  if constexpr (fused) {
+    kernels[0][0] = relax_tile_kernel<T, 1, false>;
+    kernels[0][1] = relax_tile_kernel<T, 2, false>;
+    kernels[0][2] = relax_tile_kernel<T, 3, false>;
+  } else {
+    kernels[0][0] = kernels[0][1] = kernels[0][2] = nullptr;  // an FP64 state relaxes linearized only
+  }
+  kernels[1][0] = relax_tile_kernel<T, 1, true>;
+  kernels[1][1] = relax_tile_kernel<T, 2, true>;
+  kernels[1][2] = relax_tile_kernel<T, 3, true>;
+  const int form = (s->relax_linear || !fused) ? 1 : 0;
+  dim3 rgrid(((s->ni + RTX - 1) / RTX) * ((s->nj + RTY - 1) / RTY));
+  for (int done = 0, colour = 1, halves = 2 * sweeps - 1; done < halves;) {
+    const int nh = halves - done < 3 ? halves - done : 3;
+    const int init = done == 0, last = fused && done + nh == halves;
+    hipEvent_t* ev = timing_begin(s, 2);
+    hipLaunchKernelGGL(kernels[form][nh - 1], rgrid, dim3(RTX * RTY), 0, s->stream, R, colour, init, Uf, last);
+    timing_end(s, ev);
+    colour = (colour + nh) & 1;
+    done += nh;
+  }
+  if constexpr (!fused) {
+    hipEvent_t* ev = timing_begin(s, 3);
+    hipLaunchKernelGGL(update_kernel<T>, dim3((s->n + 255) / 256), dim3(256), 0, s->stream, U);
+    timing_end(s, ev);
+  }
+  HIP_TRY(hipGetLastError(), "implicit launch");
+  s->cur = dst;
+  return BS_OK;
+}
+}  // extern "C++"
+
 bs_status bs_iterate_implicit(bs_solver* s, int32_t count, double cfl_max, int32_t sweeps, double omega, double growth,
                               bs_report* report) {
   SOLVER_GUARD(s);
   if (!(cfl_max > 0 && omega >= 1 && growth >= 1))
     return fail(BS_ERR_RANGE, "implicit steps need cfl > 0, omega >= 1 and growth >= 1");
   if (sweeps < 1) return fail(BS_ERR_RANGE, "the relaxation needs a whole number of sweeps, at least one");
-  if (s->fp64) return fail(BS_ERR_UNSUPPORTED, "implicit steps run on FP32 solvers only");
   bs_status st = ensure_implicit(s);
   if (st) return st;
   const size_t n = s->n;
@@ -1165,65 +1248,9 @@ bs_status bs_iterate_implicit(bs_solver* s, int32_t count, double cfl_max, int32
     res.q[q] = s->d.res + q * n;
     dq.q[q] = s->d.dq + q * n;
   }
-  // One implicit step: prepare (residual, lambda A, diagonal, first red
-  // half-sweep), the remaining half-sweeps, the update into the next buffer.
   auto encode = [&]() -> bs_status {
-    const int src = s->cur, dst = (s->cur + 1) % 3;
-    TileArgsT<float> A = tile_args<float>(s);
-    A.src = state_of<float>(s, src);
-    A.omega = (float)omega;
-    for (int q = 0; q < 4; q++) {
-      A.res[q] = res.q[q];
-      A.dq[q] = dq.q[q];
-    }
-    A.diag = s->d.diag;
-    A.lam_i = s->d.lam_i;
-    A.lam_j = s->d.lam_j;
-    bs_status r = launch_tile(s, TILE_IMPLICIT, A, 1);
-    if (r) return r;
-    RelaxArgs R;
-    R.G = grid_of<float>(s);
-    R.q = state_of<float>(s, src);
-    for (int q = 0; q < 4; q++) {
-      R.res[q] = res.q[q];
-      R.dq[q] = dq.q[q];
-    }
-    R.diag = s->d.diag;
-    R.lam_i = s->d.lam_i;
-    R.lam_j = s->d.lam_j;
-    R.gamma = s->gamma32;
-    R.omega = (float)omega;
-    UpdateArgs U;
-    U.src = state_of<float>(s, src);
-    U.dst = state_of<float>(s, dst);
-    U.dq = dq;
-    U.n = s->n;
-    U.gamma = s->gamma32;
-    U.control = s->d.control;
-    U.cfl_max = cfl_max;
-    U.growth = growth;
-    // The first red half-sweep (formed from R and D by the first launch)
-    // and the 2 sweeps - 1 half-sweeps after it, up to three per launch
-    // (relax_tile_kernel), starting with black; the last launch writes the
-    // updated state (q + alpha dQ) instead of dQ.
-    dim3 rgrid(((s->ni + RTX - 1) / RTX) * ((s->nj + RTY - 1) / RTY));
-    for (int done = 0, colour = 1, halves = 2 * sweeps - 1; done < halves;) {
-      const int nh = halves - done < 3 ? halves - done : 3;
-      const int init = done == 0, last = done + nh == halves;
-      hipEvent_t* ev = timing_begin(s, 2);
-      if (nh == 3)
-        hipLaunchKernelGGL(relax_tile_kernel<3>, rgrid, dim3(RTX * RTY), 0, s->stream, R, colour, init, U, last);
-      else if (nh == 2)
-        hipLaunchKernelGGL(relax_tile_kernel<2>, rgrid, dim3(RTX * RTY), 0, s->stream, R, colour, init, U, last);
-      else
-        hipLaunchKernelGGL(relax_tile_kernel<1>, rgrid, dim3(RTX * RTY), 0, s->stream, R, colour, init, U, last);
-      timing_end(s, ev);
-      colour = (colour + nh) & 1;
-      done += nh;
-    }
-    HIP_TRY(hipGetLastError(), "implicit launch");
-    s->cur = dst;
-    return BS_OK;
+    return s->fp64 ? encode_implicit<double>(s, res, dq, cfl_max, sweeps, omega, growth)
+                   : encode_implicit<float>(s, res, dq, cfl_max, sweeps, omega, growth);
   };
   StepGraph& g = s->implicit_graph;
   if (g.exec && (g.sweeps != sweeps || g.omega != omega || g.growth != growth || g.cfl_max != cfl_max))
@@ -1246,6 +1273,18 @@ bs_status bs_iterate_implicit(bs_solver* s, int32_t count, double cfl_max, int32
     if ((st = read_control(s, &c))) return st;
     fill_report(c, report);
   }
+  return BS_OK;
+}
+
+bs_status bs_set_relaxation(bs_solver* s, int32_t form) {
+  SOLVER_GUARD(s);
+  if (form != BS_RELAX_FLUX_DIFFERENCE && form != BS_RELAX_LINEARIZED)
+    return fail(BS_ERR_RANGE, "unknown relaxation form %d", form);
+  if (s->fp64 && form != BS_RELAX_LINEARIZED)
+    return fail(BS_ERR_UNSUPPORTED, "an FP64 solver relaxes with the linearized neighbour term only");
+  const bool linear = form == BS_RELAX_LINEARIZED;
+  if (linear != s->relax_linear) drop_graph(s->implicit_graph);
+  s->relax_linear = linear;
   return BS_OK;
 }
 

@@ -155,12 +155,13 @@ struct TileArgsT {
   Control* control;
   int count_step;            // this launch completes a step: control->steps++
   double* partials;          // per block: sum of (R_rho / V)^2, or null
-  // implicit prepare: residual, diagonal, lambda A per face
-  T* res[4];
-  T* diag;
-  T* lam_i;              // lambda * area per i-face
-  T* lam_j;              // lambda * area per j-face
-  T* dq[4];
+  // implicit prepare: residual, diagonal, lambda A per face, stored in
+  // FP32 for the FP32 relaxation whatever the state's precision
+  float* res[4];
+  float* diag;
+  float* lam_i;              // lambda * area per i-face
+  float* lam_j;              // lambda * area per j-face
+  float* dq[4];
   T omega;
   // residual vector output (TILE_RESIDUAL)
   T* rvec[4];
@@ -614,9 +615,15 @@ __global__ void advance_clock(Control* ctl) {
 // implicit_relax / implicit_update).
 // ---------------------------------------------------------------------------
 
-struct RelaxArgs {
-  GridDev G;
-  StateDev q;
+// The relaxation reads the state and the metrics in their own precision
+// and relaxes in FP32: an FP64 solver's implicit step is FP64 residual,
+// diagonal and update around an FP32 relaxation, as the reference's
+// two-float engine solves its implicit system in 32 bits
+// (euler2d-two-float.ts:1-14).
+template <typename T>
+struct RelaxArgsT {
+  GridDevT<T> G;
+  StateDevT<T> q;
   const float* res[4];
   const float* diag;
   const float* lam_i;
@@ -624,36 +631,60 @@ struct RelaxArgs {
   float* dq[4];
   float gamma, omega;
 };
+using RelaxArgs = RelaxArgsT<float>;
 
-struct UpdateArgs {
-  StateDev src, dst, dq;
+template <typename S>
+struct UpdateArgsT {
+  StateDevT<S> src, dst;
+  StateDevT<float> dq;
   int n;
-  float gamma;
+  S gamma;
   Control* control;
   double cfl_max, growth;
 };
+using UpdateArgs = UpdateArgsT<float>;
 
 // q + alpha dQ, alpha halved (at most 30 times) until density and pressure
 // stay within a factor of two (euler2d.ts:816-847).
-__device__ __forceinline__ Vec4<float> update_cell(float rho, float mx, float my, float e, float d0, float d1,
-                                                   float d2, float d3, float g) {
-  const float p = (g - 1.f) * (e - 0.5f * (mx * mx + my * my) * bs_rcp(rho));
-  float alpha = 1.f;
+template <typename S>
+__device__ __forceinline__ Vec4<S> update_cell(S rho, S mx, S my, S e, S d0, S d1, S d2, S d3, S g) {
+  const S p = (g - S(1)) * (e - S(0.5) * (mx * mx + my * my) * bs_rcp(rho));
+  S alpha = S(1);
   for (int tries = 0; tries < 30; tries++) {
-    float rho1 = rho + alpha * d0, mx1 = mx + alpha * d1, my1 = my + alpha * d2, e1 = e + alpha * d3;
-    float p1 = (g - 1.f) * (e1 - 0.5f * (mx1 * mx1 + my1 * my1) * bs_rcp(rho1));
-    if (rho1 > 0.5f * rho && rho1 < 2.f * rho && p1 > 0.5f * p && p1 < 2.f * p) break;
-    alpha *= 0.5f;
+    S rho1 = rho + alpha * d0, mx1 = mx + alpha * d1, my1 = my + alpha * d2, e1 = e + alpha * d3;
+    S p1 = (g - S(1)) * (e1 - S(0.5) * (mx1 * mx1 + my1 * my1) * bs_rcp(rho1));
+    if (rho1 > S(0.5) * rho && rho1 < S(2) * rho && p1 > S(0.5) * p && p1 < S(2) * p) break;
+    alpha *= S(0.5);
   }
   return {rho + alpha * d0, mx + alpha * d1, my + alpha * d2, e + alpha * d3};
 }
 
 // The CFL ramp and the step count, once per step (transonic-flow.ts:208).
-__device__ __forceinline__ void advance_control(const UpdateArgs& U) {
+template <typename S>
+__device__ __forceinline__ void advance_control(const UpdateArgsT<S>& U) {
   Control* ctl = U.control;
   double next = ctl->cfl * U.growth;
   ctl->cfl = next < U.cfl_max ? next : U.cfl_max;
   ctl->steps += 1;
+}
+
+// The update as a kernel of its own, one thread per cell (the FP64
+// solver's; FP32 steps fuse it into the last relaxation launch).
+template <typename S>
+__global__ __launch_bounds__(256) void update_kernel(UpdateArgsT<S> U) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  bool bad = false;
+  if (c < U.n) {
+    const Vec4<S> o = update_cell<S>(U.src.q[0][c], U.src.q[1][c], U.src.q[2][c], U.src.q[3][c], (S)U.dq.q[0][c],
+                                     (S)U.dq.q[1][c], (S)U.dq.q[2][c], (S)U.dq.q[3][c], U.gamma);
+    U.dst.q[0][c] = o.x;
+    U.dst.q[1][c] = o.y;
+    U.dst.q[2][c] = o.z;
+    U.dst.q[3][c] = o.w;
+    bad = !is_physical(o, U.gamma);
+  }
+  flag_non_physical(bad, U.control);
+  if (blockIdx.x == 0 && threadIdx.x == 0) advance_control(U);
 }
 
 // NH consecutive half-sweeps (colours colour0, colour0 ^ 1, ...) of a
@@ -670,9 +701,12 @@ struct RelaxCtx {
   int i0, j0;  // global index of LDS window cell (0, 0)
 };
 
-template <int NH>
-__device__ __forceinline__ void relax_tile_body(const RelaxArgs& A, int bx, int by, int colour0, int init_red,
-                                                const UpdateArgs& U, bool fuse) {
+template <typename T, int NH, bool LIN>
+__device__ __forceinline__ void relax_tile_body(const RelaxArgsT<T>& A, int bx, int by, int colour0, int init_red,
+                                                const UpdateArgs& U, bool fuse_update) {
+  static_assert(LIN || std::is_same<T, float>::value, "an FP64 state relaxes with the linearized term");
+  // Only an FP32 state is updated in place of the last dQ write.
+  const bool fuse = std::is_same<T, float>::value && fuse_update;
   constexpr int WX = RTX + 2 * NH, WY = RTY + 2 * NH, NT = RTX * RTY;
   // Window cells [0, WX) x [0, WY) hold q, dQ, the residual and the
   // diagonal; the cells updated by some half-sweep lie in [1, WX - 1) x
@@ -687,7 +721,7 @@ __device__ __forceinline__ void relax_tile_body(const RelaxArgs& A, int bx, int 
   __shared__ float sdiag[WY][WX];
   __shared__ float sfi[4][CY][CX + 1];  // i-face x (between cells x - 1 and x) of row y: [y - 1][x - 1]
   __shared__ float sfj[4][CY + 1][CX];  // j-face y of column x: [y - 1][x - 1]
-  const GridDev& G = A.G;
+  const GridDevT<T>& G = A.G;
   const int ni = G.ni, nj = G.nj;
   const int i0 = bx * RTX - NH, j0 = by * RTY - NH;
   const int tid = threadIdx.x;
@@ -699,7 +733,7 @@ __device__ __forceinline__ void relax_tile_body(const RelaxArgs& A, int bx, int 
     float r[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      sq[q][wy][wx] = A.q.q[q][c];
+      sq[q][wy][wx] = (float)A.q.q[q][c];
       r[q] = A.res[q][c];
       sres[q][wy][wx] = r[q];
     }
@@ -719,9 +753,9 @@ __device__ __forceinline__ void relax_tile_body(const RelaxArgs& A, int bx, int 
     const int gi = i0 + fx, gj = j0 + fy;
     if (gi < 0 || gi > ni || gj < 0 || gj >= nj) continue;
     const int f = gi + gj * (ni + 1);
-    sfi[0][fy - 1][fx - 1] = G.inx[f];
-    sfi[1][fy - 1][fx - 1] = G.iny[f];
-    sfi[2][fy - 1][fx - 1] = G.ia[f];
+    sfi[0][fy - 1][fx - 1] = (float)G.inx[f];
+    sfi[1][fy - 1][fx - 1] = (float)G.iny[f];
+    sfi[2][fy - 1][fx - 1] = (float)G.ia[f];
     sfi[3][fy - 1][fx - 1] = A.lam_i[f];
   }
   for (int e = tid; e < CX * (CY + 1); e += NT) {
@@ -729,17 +763,36 @@ __device__ __forceinline__ void relax_tile_body(const RelaxArgs& A, int bx, int 
     const int gi = i0 + fx, gj = j0 + fy;
     if (gi < 0 || gi >= ni || gj < 0 || gj > nj) continue;
     const int f = gi + gj * ni;
-    sfj[0][fy - 1][fx - 1] = G.jnx[f];
-    sfj[1][fy - 1][fx - 1] = G.jny[f];
-    sfj[2][fy - 1][fx - 1] = G.ja[f];
+    sfj[0][fy - 1][fx - 1] = (float)G.jnx[f];
+    sfj[1][fy - 1][fx - 1] = (float)G.jny[f];
+    sfj[2][fy - 1][fx - 1] = (float)G.ja[f];
     sfj[3][fy - 1][fx - 1] = A.lam_j[f];
   }
   __syncthreads();
   const float g = A.gamma;
-  // neighbour_term on LDS values (same arithmetic, same order).
+  // neighbour_term on LDS values (same arithmetic, same order). With LIN
+  // the term is the flux Jacobian applied to dQ instead
+  // (euler2d-two-float.wgsl:574-598): linear in dQ, so its FP32 rounding
+  // shrinks with dQ, where F(Q + dQ) - F(Q) in FP32 keeps a noise of the
+  // size of F that does not (and that would set a converged FP64 state
+  // wandering again). The two agree to first order in dQ.
   auto term = [&](int wx, int wy, float nx, float ny, float area, float lam_a, float sum[4]) {
     const float rho = sq[0][wy][wx], mx = sq[1][wy][wx], my = sq[2][wy][wx], e = sq[3][wy][wx];
     const float d0 = sdq[0][wy][wx], d1 = sdq[1][wy][wx], d2 = sdq[2][wy][wx], d3 = sdq[3][wy][wx];
+    if constexpr (LIN) {
+      const float ir = bs_rcp(rho);
+      const float u = mx * ir, v = my * ir;
+      const float un = u * nx + v * ny;
+      const float p = (g - 1.f) * (e - 0.5f * (mx * mx + my * my) * ir);
+      const float dp = (g - 1.f) * (d3 - (u * d1 + v * d2) + 0.5f * (u * u + v * v) * d0);
+      const float dun = (d1 * nx + d2 * ny - un * d0) * ir;
+      const float half = 0.5f * area, damp = 0.5f * A.omega * lam_a;
+      sum[0] += half * (d1 * nx + d2 * ny) - damp * d0;
+      sum[1] += half * (d1 * un + mx * dun + dp * nx) - damp * d1;
+      sum[2] += half * (d2 * un + my * dun + dp * ny) - damp * d2;
+      sum[3] += half * ((d3 + dp) * un + (e + p) * dun) - damp * d3;
+      return;
+    }
     const float ir0 = bs_rcp(rho);
     const float p0 = (g - 1.f) * (e - 0.5f * (mx * mx + my * my) * ir0);
     const float un0 = (mx * nx + my * ny) * ir0;
@@ -791,8 +844,9 @@ __device__ __forceinline__ void relax_tile_body(const RelaxArgs& A, int bx, int 
     const int c = gi + gj * ni;
     const int wx = lx + NH, wy = ly + NH;
     if (fuse) {
-      const Vec4<float> o = update_cell(sq[0][wy][wx], sq[1][wy][wx], sq[2][wy][wx], sq[3][wy][wx], sdq[0][wy][wx],
-                                        sdq[1][wy][wx], sdq[2][wy][wx], sdq[3][wy][wx], U.gamma);
+      const Vec4<float> o = update_cell<float>(sq[0][wy][wx], sq[1][wy][wx], sq[2][wy][wx], sq[3][wy][wx],
+                                               sdq[0][wy][wx], sdq[1][wy][wx], sdq[2][wy][wx], sdq[3][wy][wx],
+                                               U.gamma);
       U.dst.q[0][c] = o.x;
       U.dst.q[1][c] = o.y;
       U.dst.q[2][c] = o.z;
@@ -806,22 +860,22 @@ __device__ __forceinline__ void relax_tile_body(const RelaxArgs& A, int bx, int 
   if (fuse && bx == 0 && by == 0 && tid == 0) advance_control(U);
 }
 
-template <int NH>
-__global__ __launch_bounds__(RTX * RTY) void relax_tile_kernel(RelaxArgs A, int colour0, int init_red, UpdateArgs U,
-                                                             int fuse_update) {
+template <typename T, int NH, bool LIN>
+__global__ __launch_bounds__(RTX * RTY) void relax_tile_kernel(RelaxArgsT<T> A, int colour0, int init_red,
+                                                             UpdateArgs U, int fuse_update) {
   // One-dimensional grid, tiles placed per XCD as in tile_kernel.
   const int tiles_x = (A.G.ni + RTX - 1) / RTX;
   const int t = xcd_tile(blockIdx.x, gridDim.x);
-  relax_tile_body<NH>(A, t % tiles_x, t / tiles_x, colour0, init_red, U, fuse_update != 0);
+  relax_tile_body<T, NH, LIN>(A, t % tiles_x, t / tiles_x, colour0, init_red, U, fuse_update != 0);
 }
 
-template <int NH>
+template <int NH, bool LIN>
 __global__ __launch_bounds__(RTX * RTY) void batch_relax_tile_kernel(const RelaxArgs* __restrict__ args,
                                                                      const WorkItem* __restrict__ work, int colour0,
                                                                      int init_red, const UpdateArgs* upd) {
   const WorkItem w = work[xcd_tile(blockIdx.x, gridDim.x)];
-  if (upd) relax_tile_body<NH>(args[w.args], w.bx, w.by, colour0, init_red, upd[w.args], true);
-  else relax_tile_body<NH>(args[w.args], w.bx, w.by, colour0, init_red, UpdateArgs{}, false);
+  if (upd) relax_tile_body<float, NH, LIN>(args[w.args], w.bx, w.by, colour0, init_red, upd[w.args], true);
+  else relax_tile_body<float, NH, LIN>(args[w.args], w.bx, w.by, colour0, init_red, UpdateArgs{}, false);
 }
 
 

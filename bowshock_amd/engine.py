@@ -27,6 +27,7 @@ RECONSTRUCTIONS = {"perVariable": 0, "shared": 1, "sharedScaled": 2}
 FLUXES = {"hllc": 0, "rusanov": 1}
 INTEGRATORS = {"forwardEuler": 0, "ssprk2": 1, "ssprk3": 2}
 KERNELS = {"stage": 0, "implicitPrepare": 1, "implicitRelax": 2, "implicitUpdate": 3}
+RELAXATION_FORMS = {"fluxDifference": N.BS_RELAX_FLUX_DIFFERENCE, "linearized": N.BS_RELAX_LINEARIZED}
 
 
 class RangeError(ValueError):
@@ -163,8 +164,8 @@ def compute_metrics(grid, geometry="axisymmetric"):
 class GpuEuler2D:
     """The HIP twin of the reference's GpuEuler2D. precision="fp64" gives
     the precision extension: state, metrics and stencil arithmetic in FP64
-    (in place of the reference's two-float engine, euler2d-two-float.ts),
-    explicit steps only."""
+    (in place of the reference's two-float engine, euler2d-two-float.ts);
+    its implicit steps relax in FP32 with the linearized neighbour term."""
 
     def __init__(self, ctx: Context, config: Euler2DConfig, precision: str = "fp32"):
         if precision not in ("fp32", "fp64"):
@@ -294,6 +295,13 @@ class GpuEuler2D:
         check(N.lib().bs_forebody_drag(self._h, SIDES.index(side), pressure, dynamicPressure, baseRadius,
                                        C.byref(out)))
         return out.value
+
+    def setRelaxation(self, form: str):
+        """The relaxation's neighbour term: "fluxDifference" (the
+        reference's, euler2d.ts:981-1005; default of FP32 solvers) or
+        "linearized" (the two-float engine's, euler2d-two-float.wgsl:
+        574-598; the only form of FP64 solvers)."""
+        check(N.lib().bs_set_relaxation(self._h, RELAXATION_FORMS[form]))
 
     def setImplicitCfl(self, cfl: float):
         """euler2d-gpu.ts:668-681."""

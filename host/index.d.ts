@@ -117,7 +117,7 @@ export class HipDevice {
 }
 
 export class GpuEuler2D {
-  /** options.precision 'fp64': state, metrics and arithmetic in FP64 (explicit steps only). */
+  /** options.precision 'fp64': state, metrics and residual in FP64 (implicit steps relax in FP32). */
   static create(
     device: HipDevice | number,
     config: GpuEuler2DConfig,

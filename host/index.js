@@ -204,7 +204,7 @@ class GpuEuler2D {
 
   /**
    * options.precision: 'fp32' (default) or 'fp64', the solver with its
-   * state, metrics and arithmetic in FP64 (explicit steps only), in place
+   * state, metrics and residual in FP64 (implicit steps relax in FP32), in place
    * of the two-float engine (euler2d-two-float.ts:151).
    */
   constructor(device, config, options = {}) {

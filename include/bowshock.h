@@ -122,7 +122,10 @@ bs_status bs_solver_create(bs_ctx *ctx, const bs_config *config, bs_solver **out
  * engine (GpuEuler2DTwoFloat.create, src/engine-gpu/euler2d-two-float.ts:151,
  * whose ~48-bit arithmetic WebGPU needs for lack of FP64) with native FP64.
  * Explicit steps, residuals, state input/output, boundary fluxes and drag
- * run as on an FP32 solver; bs_iterate_implicit returns BS_ERR_UNSUPPORTED.
+ * run as on an FP32 solver. Implicit steps take the FP64 residual,
+ * diagonal and update around a relaxation in FP32, as the two-float
+ * engine's iterateImplicit (euler2d-two-float.ts:379-448) solves its
+ * implicit system in 32 bits: the fixed point is the FP64 residual's.
  * Initial states are formed with the configured (FP64) gamma. */
 bs_status bs_solver_create_fp64(bs_ctx *ctx, const bs_config *config, bs_solver **out);
 /* BS_PRECISION_FP32 or BS_PRECISION_FP64. */
@@ -204,6 +207,18 @@ bs_status bs_set_implicit_cfl(bs_solver *s, double cfl);
 bs_status bs_iterate_implicit(bs_solver *s, int32_t count, double cfl_max, int32_t sweeps,
                               double omega, double growth, bs_report *report);
 
+/* The neighbour term of the red-black relaxation. BS_RELAX_FLUX_DIFFERENCE
+ * is the reference's A (1/2 (F(Q + dQ) - F(Q)) - 1/2 omega lambda dQ)
+ * (euler2d.ts:981-1005, euler2d-implicit.wgsl), the default of an FP32
+ * solver; BS_RELAX_LINEARIZED applies the flux Jacobian to dQ instead
+ * (euler2d-two-float.wgsl:574-598), the form of the two-float engine and
+ * the only one of an FP64 solver. The two agree to first order in dQ and
+ * share the fixed point (the residual's zero); in FP32 the difference of
+ * fluxes keeps a rounding noise of the size of F that does not shrink with
+ * dQ, the linearized term's shrinks with it. */
+enum { BS_RELAX_FLUX_DIFFERENCE = 0, BS_RELAX_LINEARIZED = 1 };
+bs_status bs_set_relaxation(bs_solver *s, int32_t form);
+
 /* Steps run as captured HIP graphs of 12 steps each (one launch per graph
  * instead of one per kernel), on by default; no counterpart in the
  * reference, whose WebGPU engine encodes command buffers instead. */
@@ -212,7 +227,7 @@ bs_status bs_enable_graphs(bs_solver *s, int32_t enable);
 /* Mean device time per launch of the solver's kernels since the last call,
  * for throughput reporting: kernel-name index (0 stage, 1 implicit prepare,
  * 2 implicit relaxation, whose last launch of a step also applies the
- * update; 3 a separate update launch, none in this build), milliseconds per
+ * update on an FP32 solver; 3 the FP64 solver's update launch), milliseconds per
  * launch and the launches counted. Timing is taken with HIP events on the solver's stream
  * when enabled with bs_enable_timing: enable = 1 records a pair of events
  * around every launch (steps then run without graphs); enable = 2 records
@@ -247,6 +262,9 @@ bs_status bs_batch_set_active(bs_batch *b, const uint8_t *active);
 /* Cases of fewer cells than the threshold run on 32 x 4 tiles, the others
  * on 64 x 8 tiles (set by the autotuner; 0 = all on 64 x 8). */
 bs_status bs_batch_set_tiling(bs_batch *b, int32_t small_tile_below_cells);
+/* bs_set_relaxation for every case of the batch (default
+ * BS_RELAX_FLUX_DIFFERENCE). */
+bs_status bs_batch_set_relaxation(bs_batch *b, int32_t form);
 /* `steps` explicit steps of every active case (iterate). */
 bs_status bs_batch_iterate(bs_batch *b, int32_t steps);
 /* setImplicitCfl and iterateImplicit (euler2d-gpu.ts:668-723) of every

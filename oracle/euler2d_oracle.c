@@ -822,6 +822,32 @@ static void neighbour_term(const orc_solver *s, int c, double nx, double ny, dou
   sum[3] += half * ((e1 + p1) * un1 - (e + p0) * un0) - damp * d[3];
 }
 
+/* The same term with dF_n the flux Jacobian along n applied to d: linear
+ * in d, so its rounding is relative to the term itself (the two-float
+ * engine's relaxation, euler2d-two-float.wgsl:574-598). It agrees with
+ * neighbour_term to first order in d. */
+static void neighbour_term_linear(const double *q, const double *d, double g, double nx, double ny,
+                                  double area, double lam_a, double omega, double *sum) {
+  double rho = q[0], mx = q[1], my = q[2], e = q[3];
+  double u = mx / rho, v = my / rho;
+  double un = u * nx + v * ny;
+  double p = (g - 1.0) * (e - (0.5 * (mx * mx + my * my)) / rho);
+  double dp = (g - 1.0) * (d[3] - (u * d[1] + v * d[2]) + 0.5 * (u * u + v * v) * d[0]);
+  double dun = (d[1] * nx + d[2] * ny - un * d[0]) / rho;
+  double df[4] = {d[1] * nx + d[2] * ny, d[1] * un + mx * dun + dp * nx, d[2] * un + my * dun + dp * ny,
+                  (d[3] + dp) * un + (e + p) * dun};
+  double half = 0.5 * area, damp = 0.5 * omega * lam_a;
+  for (int k = 0; k < 4; k++) sum[k] += half * df[k] - damp * d[k];
+}
+
+static void neighbour_of(const orc_solver *s, int linear, int c, double nx, double ny, double area,
+                         double lam_a, double omega, const double *from, double *sum) {
+  if (linear)
+    neighbour_term_linear(s->u + 4 * c, from + 4 * c, s->cfg.gamma, nx, ny, area, lam_a, omega, sum);
+  else
+    neighbour_term(s, c, nx, ny, area, lam_a, omega, from, sum);
+}
+
 static double radius_of(const orc_solver *s, int c, double nx, double ny) {
   const double *q = s->u + 4 * c;
   return fabs((q[1] * nx + q[2] * ny) / q[0]) + s->ac[c];
@@ -831,8 +857,11 @@ static double radius_of(const orc_solver *s, int c, double nx, double ny) {
 int orc_implicit_correction(orc_solver *s, double cfl, double omega, int relaxation,
                             int sweeps, double *dq_out) {
   if (!(cfl > 0.0 && omega >= 1.0)) return fail(ORC_ERR_RANGE, "LU-SGS needs cfl > 0 and omega >= 1");
+  if (relaxation < ORC_RELAX_SWEEPS || relaxation > ORC_RELAX_RED_BLACK_LINEAR)
+    return fail(ORC_ERR_RANGE, "unknown relaxation");
   if (relaxation != ORC_RELAX_SWEEPS && !(sweeps >= 1))
     return fail(ORC_ERR_RANGE, "a relaxation needs a whole number of sweeps, at least one");
+  const int linear = relaxation == ORC_RELAX_RED_BLACK_LINEAR;
   int ni = s->ni, nj = s->nj;
   size_t n = (size_t)ni * nj;
   double g = s->cfg.gamma;
@@ -889,17 +918,17 @@ int orc_implicit_correction(orc_solver *s, double cfl, double omega, int relaxat
             double sum[4] = {0, 0, 0, 0};
             if (i > 0) {
               int f = i + j * (ni + 1);
-              neighbour_term(s, c - 1, -s->inx[f], -s->iny[f], s->iarea[f], s->lam_i[f], omega, from, sum);
+              neighbour_of(s, linear, c - 1, -s->inx[f], -s->iny[f], s->iarea[f], s->lam_i[f], omega, from, sum);
             }
             if (i < ni - 1) {
               int f = i + 1 + j * (ni + 1);
-              neighbour_term(s, c + 1, s->inx[f], s->iny[f], s->iarea[f], s->lam_i[f], omega, from, sum);
+              neighbour_of(s, linear, c + 1, s->inx[f], s->iny[f], s->iarea[f], s->lam_i[f], omega, from, sum);
             }
             if (j > 0)
-              neighbour_term(s, c - ni, -s->jnx[c], -s->jny[c], s->jarea[c], s->lam_j[c], omega, from, sum);
+              neighbour_of(s, linear, c - ni, -s->jnx[c], -s->jny[c], s->jarea[c], s->lam_j[c], omega, from, sum);
             if (j < nj - 1) {
               int f = c + ni;
-              neighbour_term(s, c + ni, s->jnx[f], s->jny[f], s->jarea[f], s->lam_j[f], omega, from, sum);
+              neighbour_of(s, linear, c + ni, s->jnx[f], s->jny[f], s->jarea[f], s->lam_j[f], omega, from, sum);
             }
             double d = s->diag[c];
             for (int k = 0; k < 4; k++) dq[4 * c + k] = (res[4 * c + k] - sum[k]) / d;
@@ -984,6 +1013,12 @@ int orc_iterate_implicit(orc_solver *s, int count, double cfl, double omega, int
 /* ---------------------------------------------------------------------- */
 /* Scalar exports for property tests                                       */
 /* ---------------------------------------------------------------------- */
+
+void orc_neighbour_term_linear(const double *q, const double *d, double g, double nx, double ny, double area,
+                               double lam_a, double omega, double *out) {
+  out[0] = out[1] = out[2] = out[3] = 0.0;
+  neighbour_term_linear(q, d, g, nx, ny, area, lam_a, omega, out);
+}
 
 void orc_face_frame_flux(int flux, const double *l, const double *r, double g, double *out) {
   face_t a = {l[0], l[1], l[2], l[3]}, b = {r[0], r[1], r[2], r[3]};

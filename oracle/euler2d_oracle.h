@@ -32,7 +32,9 @@ enum { ORC_FIRST_ORDER = 0, ORC_MINMOD = 1, ORC_VAN_LEER = 2, ORC_VAN_ALBADA = 3
 enum { ORC_PER_VARIABLE = 0, ORC_SHARED = 1, ORC_SHARED_SCALED = 2 };
 enum { ORC_HLLC = 0, ORC_RUSANOV = 1, ORC_HLL = 2 };
 enum { ORC_FORWARD_EULER = 0, ORC_SSPRK2 = 1, ORC_SSPRK3 = 2 };
-enum { ORC_RELAX_SWEEPS = 0, ORC_RELAX_JACOBI = 1, ORC_RELAX_RED_BLACK = 2 };
+/* RED_BLACK_LINEAR: red-black with the linearized neighbour term of the
+ * two-float engine (euler2d-two-float.wgsl:574-598). */
+enum { ORC_RELAX_SWEEPS = 0, ORC_RELAX_JACOBI = 1, ORC_RELAX_RED_BLACK = 2, ORC_RELAX_RED_BLACK_LINEAR = 3 };
 
 /* Return codes. */
 enum { ORC_OK = 0, ORC_ERR_RANGE = 1, ORC_ERR_NONPHYSICAL = 2 };
@@ -103,6 +105,10 @@ void orc_face_frame_flux(int flux, const double *left, const double *right, doub
 void orc_face_flux_2d(int flux, const double *left, const double *right, double nx, double ny,
                       double gamma, double *out);
 double orc_limiter(int limiter, double a, double b);
+/* The linearized neighbour term (4 values) of a cell with conserved state
+ * q and change d across a face (normal, area, lambda A). */
+void orc_neighbour_term_linear(const double *q, const double *d, double g, double nx, double ny, double area,
+                               double lam_a, double omega, double *out);
 void orc_reconstruct(int reconstruction, int limiter, const double *prev, const double *centre,
                      const double *next, int side, double gamma, double *out);
 void orc_far_field_state(const double *inside, const double *free_state, double nx, double ny,

@@ -21,7 +21,7 @@ FIRST_ORDER, MINMOD, VAN_LEER, VAN_ALBADA, MONOTONIZED_CENTRAL, UNLIMITED_CENTRA
 PER_VARIABLE, SHARED, SHARED_SCALED = range(3)
 HLLC, RUSANOV, HLL = range(3)
 FORWARD_EULER, SSPRK2, SSPRK3 = range(3)
-RELAX_SWEEPS, RELAX_JACOBI, RELAX_RED_BLACK = range(3)
+RELAX_SWEEPS, RELAX_JACOBI, RELAX_RED_BLACK, RELAX_RED_BLACK_LINEAR = range(4)
 
 BC_KINDS = {"inflow": INFLOW, "outflow": OUTFLOW, "wall": WALL, "periodic": PERIODIC,
             "faceStates": FACE_STATES, "farField": FAR_FIELD}
@@ -31,7 +31,8 @@ LIMITERS = {"firstOrder": FIRST_ORDER, "minmod": MINMOD, "vanLeer": VAN_LEER,
 RECONSTRUCTIONS = {"perVariable": PER_VARIABLE, "shared": SHARED, "sharedScaled": SHARED_SCALED}
 FLUXES = {"hllc": HLLC, "rusanov": RUSANOV, "hll": HLL}
 INTEGRATORS = {"forwardEuler": FORWARD_EULER, "ssprk2": SSPRK2, "ssprk3": SSPRK3}
-RELAXATIONS = {"sweeps": RELAX_SWEEPS, "jacobi": RELAX_JACOBI, "redBlack": RELAX_RED_BLACK}
+RELAXATIONS = {"sweeps": RELAX_SWEEPS, "jacobi": RELAX_JACOBI, "redBlack": RELAX_RED_BLACK,
+               "redBlackLinear": RELAX_RED_BLACK_LINEAR}
 
 
 class OracleConfig(C.Structure):
@@ -93,6 +94,9 @@ def lib():
         L.orc_face_flux_2d.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, C.c_double, dp]
         L.orc_limiter.argtypes = [C.c_int, C.c_double, C.c_double]
         L.orc_limiter.restype = C.c_double
+        L.orc_neighbour_term_linear.argtypes = [dp, dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                                C.c_double, dp]
+        L.orc_neighbour_term_linear.restype = None
         L.orc_reconstruct.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_int, C.c_double, dp]
         L.orc_far_field_state.argtypes = [dp, dp, C.c_double, C.c_double, C.c_double, dp]
         _lib = L
@@ -294,4 +298,12 @@ def reconstruct(reconstruction, limiter_name, prev, centre, nxt, side, gamma):
 def far_field_state(inside, free, nx, ny, gamma):
     out = np.zeros(4)
     lib().orc_far_field_state(_p(np.asarray(inside, float)), _p(np.asarray(free, float)), nx, ny, gamma, _p(out))
+    return out
+
+
+def neighbour_term_linear(q, d, gamma, nx, ny, area, lam_a, omega):
+    """The linearized neighbour term (euler2d-two-float.wgsl:574-598)."""
+    out = np.zeros(4)
+    lib().orc_neighbour_term_linear(_p(np.asarray(q, float)), _p(np.asarray(d, float)), gamma, nx, ny, area, lam_a,
+                                    omega, _p(out))
     return out

@@ -92,15 +92,17 @@ def test_retired_cases_stay_put(ctx):
     assert np.array_equal(_state(batch.readState(1)), _state(singles[1].readState()))
 
 
-@pytest.mark.parametrize("small_below", [0, 5000])
-def test_implicit_batch_matches_single_solves(ctx, small_below):
+@pytest.mark.parametrize("small_below,form", [(0, "fluxDifference"), (5000, "fluxDifference"), (0, "linearized")])
+def test_implicit_batch_matches_single_solves(ctx, small_below, form):
     cs = _cases(limiter="vanAlbada", reconstruction="sharedScaled")
     batch = GpuBatch2D(ctx, [c[0] for c in cs])
     batch.setTiling(small_below)
+    batch.setRelaxation(form)
     batch.setUniform([c[1] for c in cs])
     batch.setImplicitCfl(1.0)
     singles = _singles(ctx, cs)
     for s in singles:
+        s.setRelaxation(form)
         s.setImplicitCfl(1.0)
     for count, sweeps in ((30, 2), (13, 2), (12, 1)):
         batch.iterateImplicit(count, 10.0, sweeps)

@@ -339,7 +339,8 @@ def test_implicit_relaxations_reach_the_explicit_drag():
     explicit.iterate(6000)
     assert explicit.density_residual_norm() < 1e-12
     ref = explicit.forebody_drag(p_inf, q_inf, 0.5)
-    for relaxation, sweeps, cap in (("sweeps", 1, 600), ("jacobi", 4, 3000), ("redBlack", 2, 3000)):
+    for relaxation, sweeps, cap in (("sweeps", 1, 600), ("jacobi", 4, 3000), ("redBlack", 2, 3000),
+                                    ("redBlackLinear", 2, 3000)):
         s, _, _ = make()
         cfl = 1.0
         for _ in range(cap):
@@ -349,3 +350,42 @@ def test_implicit_relaxations_reach_the_explicit_drag():
             cfl = min(1000.0, cfl * 1.2)
         assert s.density_residual_norm() < 1e-12, relaxation
         assert abs(s.forebody_drag(p_inf, q_inf, 0.5) / ref - 1) < 1e-10, relaxation
+
+
+def _normal_flux(q, nx, ny, g):
+    """The Euler flux along (nx, ny) of conserved q (textbook form)."""
+    rho, mx, my, e = q
+    p = (g - 1) * (e - 0.5 * (mx * mx + my * my) / rho)
+    un = (mx * nx + my * ny) / rho
+    return np.array([rho * un, mx * un + p * nx, my * un + p * ny, (e + p) * un])
+
+
+def test_linear_neighbour_term_is_the_flux_jacobian():
+    """euler2d-two-float.wgsl:574-598: A (1/2 J_n d - 1/2 omega lambda d)
+    with J_n the flux Jacobian along n. Pinned against central differences
+    of the textbook flux (error O(eps^2)), linearity, and the damping term
+    alone when the normal flux does not change."""
+    rng = np.random.default_rng(11)
+    g = 1.4
+    for _ in range(50):
+        rho = 0.5 + rng.random()
+        u, v = rng.normal(size=2)
+        p = 0.5 + rng.random()
+        q = np.array([rho, rho * u, rho * v, p / (g - 1) + 0.5 * rho * (u * u + v * v)])
+        d = rng.normal(size=4)
+        t = rng.random() * 2 * math.pi
+        nx, ny = math.cos(t), math.sin(t)
+        area, lam, omega = 0.7, 1.3, 1.5
+        lin = O.neighbour_term_linear(q, d, g, nx, ny, area, lam, omega)
+        eps = 1e-5
+        jd = (_normal_flux(q + eps * d, nx, ny, g) - _normal_flux(q - eps * d, nx, ny, g)) / (2 * eps)
+        expect = 0.5 * area * jd - 0.5 * omega * lam * d
+        assert np.abs(lin - expect).max() < 1e-8 * max(1.0, np.abs(expect).max())
+        lin2 = O.neighbour_term_linear(q, 2 * d, g, nx, ny, area, lam, omega)
+        assert np.allclose(lin2, 2 * lin, rtol=1e-14, atol=1e-14)
+        assert not np.any(O.neighbour_term_linear(q, np.zeros(4), g, nx, ny, area, lam, omega))
+    # A change along the state itself scales the flux: J_n q = F_n(q) for
+    # the homogeneous Euler flux (degree 1).
+    q = np.array([1.2, 0.6, -0.3, 2.9])
+    lin = O.neighbour_term_linear(q, q, 1.4, 0.6, 0.8, 1.0, 0.0, 1.0)
+    assert np.allclose(lin, 0.5 * _normal_flux(q, 0.6, 0.8, 1.4), rtol=1e-14, atol=1e-14)
