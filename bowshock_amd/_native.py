@@ -65,6 +65,13 @@ SIGNATURES = {
                                       C.POINTER(Report)]),
     "bs_enable_graphs": (C.c_int, [_vp, C.c_int32]),
     "bs_set_relaxation": (C.c_int, [_vp, C.c_int32]),
+    "bs_anderson_create": (C.c_int, [_vp, C.c_int64, C.c_int32, C.POINTER(_vp)]),
+    "bs_anderson_destroy": (C.c_int, [_vp]),
+    "bs_anderson_reset": (C.c_int, [_vp]),
+    "bs_anderson_next": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "bs_anderson_state": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32), _dp]),
+    "bs_iterate_anderson": (C.c_int, [_vp, _vp, C.c_int32, C.c_double, C.c_int32, C.c_double, C.c_double,
+                                      C.POINTER(Report)]),
     "bs_enable_timing": (C.c_int, [_vp, C.c_int32]),
     "bs_kernel_timing": (C.c_int, [_vp, C.c_int32, _dp, C.POINTER(C.c_int64)]),
     "bs_batch_create": (C.c_int, [_vp, C.POINTER(Config), C.c_int32, C.POINTER(_vp)]),

@@ -19,6 +19,7 @@
 
 #include "../../include/bowshock.h"
 #include "bs_kernels.hpp"
+#include "bs_anderson.hpp"
 
 using namespace bs;
 
@@ -136,7 +137,8 @@ struct bs_solver {
   Timing timing;
   long long steps_accounted = 0;
   bool graphs = true;
-  StepGraph explicit_graph, implicit_graph;
+  StepGraph explicit_graph, implicit_graph, anderson_graph;
+  long long anderson_for = 0;  // id of the history the anderson graph was captured with
 };
 
 namespace {
@@ -506,6 +508,7 @@ bs_status dev_alloc(T** p, size_t count, const char* what) {
 void free_all(bs_solver* s) {
   drop_graph(s->explicit_graph);
   drop_graph(s->implicit_graph);
+  drop_graph(s->anderson_graph);
   DeviceBuffers& d = s->d;
   void* all[] = {d.state[0], d.state[1], d.state[2], d.inx, d.iny, d.ia, d.jnx, d.jny, d.ja, d.vol, d.planar,
                  d.dt, d.dt_global, d.dt_partials, d.partials, d.scalar, d.control, d.bnd, d.wall_out, d.res,
@@ -1159,7 +1162,7 @@ extern "C++" {
 // FP64 update kernel follows.
 template <typename T>
 static bs_status encode_implicit(bs_solver* s, const StateDevT<float>& res, const StateDevT<float>& dq,
-                                 double cfl_max, int sweeps, double omega, double growth) {
+                                 double cfl_max, int sweeps, double omega, double growth, bool dq_only = false) {
   const int src = s->cur, dst = (s->cur + 1) % 3;
   TileArgsT<T> A = tile_args<T>(s);
   A.src = state_of<T>(s, src);
@@ -1194,15 +1197,16 @@ static bs_status encode_implicit(bs_solver* s, const StateDevT<float>& res, cons
   U.control = s->d.control;
   U.cfl_max = cfl_max;
   U.growth = growth;
-  constexpr bool fused = std::is_same<T, float>::value;
+  // dq_only: the correction dQ alone (the accelerated steps mix it first).
+  const bool fused = std::is_same<T, float>::value && !dq_only;
   UpdateArgs Uf;
-  if constexpr (fused) Uf = U;
+  if constexpr (std::is_same<T, float>::value) Uf = U;
   else memset(&Uf, 0, sizeof Uf);
   // The 2 sweeps - 1 half-sweeps after the first red one, up to three per
   // launch (relax_tile_kernel), starting with black.
   using K = void (*)(RelaxArgsT<T>, int, int, UpdateArgs, int);
   K kernels[2][3];
-  if constexpr (fused) {
+  if constexpr (std::is_same<T, float>::value) {
     kernels[0][0] = relax_tile_kernel<T, 1, false>;
     kernels[0][1] = relax_tile_kernel<T, 2, false>;
     kernels[0][2] = relax_tile_kernel<T, 3, false>;
@@ -1212,7 +1216,7 @@ static bs_status encode_implicit(bs_solver* s, const StateDevT<float>& res, cons
   kernels[1][0] = relax_tile_kernel<T, 1, true>;
   kernels[1][1] = relax_tile_kernel<T, 2, true>;
   kernels[1][2] = relax_tile_kernel<T, 3, true>;
-  const int form = (s->relax_linear || !fused) ? 1 : 0;
+  const int form = (s->relax_linear || !std::is_same<T, float>::value) ? 1 : 0;
   dim3 rgrid(((s->ni + RTX - 1) / RTX) * ((s->nj + RTY - 1) / RTY));
   for (int done = 0, colour = 1, halves = 2 * sweeps - 1; done < halves;) {
     const int nh = halves - done < 3 ? halves - done : 3;
@@ -1223,7 +1227,11 @@ static bs_status encode_implicit(bs_solver* s, const StateDevT<float>& res, cons
     colour = (colour + nh) & 1;
     done += nh;
   }
-  if constexpr (!fused) {
+  if (dq_only) {
+    HIP_TRY(hipGetLastError(), "implicit launch");
+    return BS_OK;
+  }
+  if (!fused) {
     hipEvent_t* ev = timing_begin(s, 3);
     hipLaunchKernelGGL(update_kernel<T>, dim3((s->n + 255) / 256), dim3(256), 0, s->stream, U);
     timing_end(s, ev);
@@ -1276,6 +1284,253 @@ bs_status bs_iterate_implicit(bs_solver* s, int32_t count, double cfl_max, int32
   return BS_OK;
 }
 
+// ---- Anderson acceleration ---------------------------------------------------
+
+}  // extern "C"
+
+struct bs_anderson {
+  long long id;  // unique per history: a captured graph names its buffers
+  int device;
+  long long size;
+  int depth;
+  int blocks;  // of the push and update kernels (fixed: the partials' order)
+  hipStream_t stream = nullptr;
+  double *dg = nullptr, *df = nullptr, *last_x = nullptr, *last_f = nullptr, *partials = nullptr;
+  AndersonCtl* ctl = nullptr;
+  double *x_dev = nullptr, *f_dev = nullptr, *out_dev = nullptr;  // bs_anderson_next staging
+  std::atomic<bool> busy{false};
+};
+
+namespace {
+
+// The push kernel whose accumulators cover the depth.
+template <typename TX, typename TF>
+void (*push_kernel(int depth))(const TX*, const TF*, AndersonArgs) {
+  if (depth <= 1) return anderson_push<TX, TF, 1>;
+  if (depth <= 2) return anderson_push<TX, TF, 2>;
+  if (depth <= 4) return anderson_push<TX, TF, 4>;
+  if (depth <= 8) return anderson_push<TX, TF, 8>;
+  return anderson_push<TX, TF, 16>;
+}
+
+AndersonArgs anderson_args(bs_anderson* a) {
+  AndersonArgs A;
+  A.dg = a->dg;
+  A.df = a->df;
+  A.last_x = a->last_x;
+  A.last_f = a->last_f;
+  A.partials = a->partials;
+  A.ctl = a->ctl;
+  A.size = a->size;
+  A.depth = a->depth;
+  return A;
+}
+
+void anderson_free(bs_anderson* a) {
+  void* all[] = {a->dg, a->df, a->last_x, a->last_f, a->partials, a->ctl, a->x_dev, a->f_dev, a->out_dev};
+  for (void* p : all)
+    if (p) (void)hipFree(p);
+  if (a->stream) (void)hipStreamDestroy(a->stream);
+}
+
+struct AndersonGuard {
+  bs_anderson* a;
+  bool ok;
+  explicit AndersonGuard(bs_anderson* x) : a(x) {
+    bool expected = false;
+    ok = a->busy.compare_exchange_strong(expected, true);
+  }
+  ~AndersonGuard() {
+    if (ok) a->busy.store(false);
+  }
+};
+
+// One accelerated implicit step (transonic-flow.ts:184-208): the
+// correction f at x, the reset rule, the mixed iterate x + f - (dX + dF)
+// gamma, the plain step where that is not physical, the CFL ramp.
+template <typename T>
+bs_status encode_anderson_step(bs_solver* s, bs_anderson* a, const StateDevT<float>& res,
+                               const StateDevT<float>& dq, double cfl_max, int sweeps, double omega, double growth) {
+  bs_status st = encode_implicit<T>(s, res, dq, cfl_max, sweeps, omega, growth, true);
+  if (st) return st;
+  const int src = s->cur, dst = (s->cur + 1) % 3;
+  const T* x = static_cast<const T*>(s->d.state[src]);
+  T* out = static_cast<T*>(s->d.state[dst]);
+  const float* f = s->d.dq;
+  const AndersonArgs A = anderson_args(a);
+  hipEvent_t* ev = timing_begin(s, 3);
+  hipLaunchKernelGGL((push_kernel<T, float>(a->depth)), dim3(a->blocks), dim3(256), 0, s->stream, x, f, A);
+  hipLaunchKernelGGL(anderson_solve, dim3(1), dim3(AA_SOLVE_THREADS), 0, s->stream, A, a->blocks, 1);
+  const int cb = (s->n + 255) / 256;
+  hipLaunchKernelGGL(anderson_update_cells<T>, dim3(cb), dim3(256), 0, s->stream, x, f, out, s->n,
+                     (T)state_gamma(s), A);
+  UpdateArgsT<T> U;
+  memset(&U, 0, sizeof U);
+  U.n = s->n;
+  U.gamma = (T)state_gamma(s);
+  U.control = s->d.control;
+  U.cfl_max = cfl_max;
+  U.growth = growth;
+  hipLaunchKernelGGL(anderson_fallback<T>, dim3(cb), dim3(256), 0, s->stream, x, f, out, s->n, U, A);
+  timing_end(s, ev);
+  HIP_TRY(hipGetLastError(), "anderson launch");
+  s->cur = dst;
+  return BS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+bs_status bs_anderson_create(bs_ctx* ctx, int64_t size, int32_t depth, bs_anderson** out) {
+  if (!ctx || !out) return fail(BS_ERR_RANGE, "null argument");
+  *out = nullptr;
+  if (size < 1) return fail(BS_ERR_RANGE, "the vectors need at least one component");
+  if (depth < 1 || depth > AA_MAX_DEPTH) return fail(BS_ERR_RANGE, "depth must be 1 to %d", AA_MAX_DEPTH);
+  (void)hipSetDevice(ctx->device);
+  static std::atomic<long long> next_id{1};
+  bs_anderson* a = new bs_anderson();
+  a->id = next_id++;
+  a->device = ctx->device;
+  a->size = size;
+  a->depth = depth;
+  const long long want = (size + 255) / 256;
+  a->blocks = (int)(want < 2 * AA_SOLVE_THREADS ? want : 2 * AA_SOLVE_THREADS);  // anderson_solve reads two per thread
+  auto bail = [&](bs_status code) {
+    anderson_free(a);
+    delete a;
+    return code;
+  };
+  bs_status st;
+  if (hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(BS_ERR_EXECUTION, "hipStreamCreate failed"));
+  const size_t n = (size_t)size;
+  if ((st = dev_alloc(&a->dg, (size_t)depth * n, "anderson dX + dF")) ||
+      (st = dev_alloc(&a->df, (size_t)depth * n, "anderson dF")) ||
+      (st = dev_alloc(&a->last_x, n, "anderson x")) || (st = dev_alloc(&a->last_f, n, "anderson f")) ||
+      (st = dev_alloc(&a->partials, (size_t)a->blocks * AA_VALUES, "anderson partials")) ||
+      (st = dev_alloc(&a->ctl, 1, "anderson control")))
+    return bail(st);
+  AndersonCtl c;
+  memset(&c, 0, sizeof c);
+  c.best = 1.0e308;
+  if (hipMemcpy(a->ctl, &c, sizeof c, hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail(BS_ERR_EXECUTION, "anderson control upload"));
+  *out = a;
+  return BS_OK;
+}
+
+bs_status bs_anderson_destroy(bs_anderson* a) {
+  if (!a) return BS_OK;
+  (void)hipSetDevice(a->device);
+  if (a->stream) (void)hipStreamSynchronize(a->stream);
+  anderson_free(a);
+  delete a;
+  return BS_OK;
+}
+
+bs_status bs_anderson_reset(bs_anderson* a) {
+  if (!a) return fail(BS_ERR_RANGE, "no history");
+  AndersonGuard g(a);
+  if (!g.ok) return fail(BS_ERR_BUSY, "bs_anderson: await the previous operation first");
+  (void)hipSetDevice(a->device);
+  hipLaunchKernelGGL(anderson_reset_kernel, dim3(1), dim3(1), 0, a->stream, a->ctl, 1);
+  HIP_TRY(hipStreamSynchronize(a->stream), "anderson reset");
+  return BS_OK;
+}
+
+bs_status bs_anderson_next(bs_anderson* a, const double* x, const double* f, double* out) {
+  if (!a || !x || !f || !out) return fail(BS_ERR_RANGE, "null argument");
+  AndersonGuard g(a);
+  if (!g.ok) return fail(BS_ERR_BUSY, "bs_anderson: await the previous operation first");
+  (void)hipSetDevice(a->device);
+  const size_t n = (size_t)a->size;
+  bs_status st;
+  if (!a->x_dev && ((st = dev_alloc(&a->x_dev, n, "anderson staging")) ||
+                    (st = dev_alloc(&a->f_dev, n, "anderson staging")) ||
+                    (st = dev_alloc(&a->out_dev, n, "anderson staging"))))
+    return st;
+  HIP_TRY(hipMemcpyAsync(a->x_dev, x, n * sizeof(double), hipMemcpyHostToDevice, a->stream), "anderson x");
+  HIP_TRY(hipMemcpyAsync(a->f_dev, f, n * sizeof(double), hipMemcpyHostToDevice, a->stream), "anderson f");
+  const AndersonArgs A = anderson_args(a);
+  hipLaunchKernelGGL((push_kernel<double, double>(a->depth)), dim3(a->blocks), dim3(256), 0, a->stream, a->x_dev, a->f_dev,
+                     A);
+  hipLaunchKernelGGL(anderson_solve, dim3(1), dim3(AA_SOLVE_THREADS), 0, a->stream, A, a->blocks, 0);
+  hipLaunchKernelGGL((anderson_update<double, double, double>), dim3(a->blocks), dim3(256), 0, a->stream, a->x_dev,
+                     a->f_dev, a->out_dev, A);
+  HIP_TRY(hipGetLastError(), "anderson launch");
+  HIP_TRY(hipMemcpyAsync(out, a->out_dev, n * sizeof(double), hipMemcpyDeviceToHost, a->stream), "anderson out");
+  HIP_TRY(hipStreamSynchronize(a->stream), "anderson next");
+  return BS_OK;
+}
+
+bs_status bs_anderson_state(bs_anderson* a, int32_t* columns, int32_t* used, int32_t* restarts, double* gamma) {
+  if (!a) return fail(BS_ERR_RANGE, "no history");
+  AndersonGuard g(a);
+  if (!g.ok) return fail(BS_ERR_BUSY, "bs_anderson: await the previous operation first");
+  (void)hipSetDevice(a->device);
+  AndersonCtl c;
+  HIP_TRY(hipMemcpy(&c, a->ctl, sizeof c, hipMemcpyDeviceToHost), "anderson state");
+  if (columns) *columns = c.count;
+  if (used) *used = c.used;
+  if (restarts) *restarts = c.restarts;
+  if (gamma)  // oldest column first
+    for (int k = 0; k < a->depth; k++) gamma[k] = k < c.count ? c.gamma[(c.oldest + k) % a->depth] : 0.0;
+  return BS_OK;
+}
+
+bs_status bs_iterate_anderson(bs_solver* s, bs_anderson* a, int32_t count, double cfl_max, int32_t sweeps,
+                              double omega, double growth, bs_report* report) {
+  SOLVER_GUARD(s);
+  if (!a) return fail(BS_ERR_RANGE, "no history");
+  AndersonGuard ag(a);
+  if (!ag.ok) return fail(BS_ERR_BUSY, "bs_anderson: await the previous operation first");
+  if (a->size != 4LL * s->n) return fail(BS_ERR_RANGE, "the history holds %lld values, the state %lld", a->size,
+                                         4LL * s->n);
+  if (a->device != s->device) return fail(BS_ERR_RANGE, "the history is on another device");
+  if (!(cfl_max > 0 && omega >= 1 && growth >= 1))
+    return fail(BS_ERR_RANGE, "implicit steps need cfl > 0, omega >= 1 and growth >= 1");
+  if (sweeps < 1) return fail(BS_ERR_RANGE, "the relaxation needs a whole number of sweeps, at least one");
+  bs_status st = ensure_implicit(s);
+  if (st) return st;
+  // The history's own stream may still run a reset or a next.
+  HIP_TRY(hipStreamSynchronize(a->stream), "anderson stream");
+  const size_t n = s->n;
+  StateDevT<float> res, dq;
+  for (int q = 0; q < 4; q++) {
+    res.q[q] = s->d.res + q * n;
+    dq.q[q] = s->d.dq + q * n;
+  }
+  auto encode = [&]() -> bs_status {
+    return s->fp64 ? encode_anderson_step<double>(s, a, res, dq, cfl_max, sweeps, omega, growth)
+                   : encode_anderson_step<float>(s, a, res, dq, cfl_max, sweeps, omega, growth);
+  };
+  StepGraph& g = s->anderson_graph;
+  if (g.exec && (g.sweeps != sweeps || g.omega != omega || g.growth != growth || g.cfl_max != cfl_max ||
+                 s->anderson_for != a->id))
+    drop_graph(g);
+  Control c;
+  int remaining = count;
+  while (remaining > 0) {
+    const int batch = remaining < 64 ? remaining : 64;
+    if ((st = run_steps(s, g, batch, encode))) return st;
+    g.sweeps = sweeps;
+    g.omega = omega;
+    g.growth = growth;
+    g.cfl_max = cfl_max;
+    s->anderson_for = a->id;
+    if ((st = read_control(s, &c))) return st;
+    fill_report(c, report);
+    if ((st = account(s, c, batch))) return st;
+    remaining -= batch;
+  }
+  if (count == 0) {
+    if ((st = read_control(s, &c))) return st;
+    fill_report(c, report);
+  }
+  return BS_OK;
+}
+
 bs_status bs_set_relaxation(bs_solver* s, int32_t form) {
   SOLVER_GUARD(s);
   if (form != BS_RELAX_FLUX_DIFFERENCE && form != BS_RELAX_LINEARIZED)
@@ -1283,7 +1538,10 @@ bs_status bs_set_relaxation(bs_solver* s, int32_t form) {
   if (s->fp64 && form != BS_RELAX_LINEARIZED)
     return fail(BS_ERR_UNSUPPORTED, "an FP64 solver relaxes with the linearized neighbour term only");
   const bool linear = form == BS_RELAX_LINEARIZED;
-  if (linear != s->relax_linear) drop_graph(s->implicit_graph);
+  if (linear != s->relax_linear) {
+    drop_graph(s->implicit_graph);
+    drop_graph(s->anderson_graph);
+  }
   s->relax_linear = linear;
   return BS_OK;
 }
@@ -1294,6 +1552,7 @@ bs_status bs_enable_graphs(bs_solver* s, int32_t enable) {
   if (!s->graphs) {
     drop_graph(s->explicit_graph);
     drop_graph(s->implicit_graph);
+    drop_graph(s->anderson_graph);
   }
   return BS_OK;
 }

@@ -312,6 +312,13 @@ class GpuEuler2D:
         check(N.lib().bs_iterate_implicit(self._h, int(count), float(cfl), int(sweeps), float(omega), float(growth),
                                           C.byref(self.last_report)))
 
+    def iterateAnderson(self, anderson: "AndersonMixing", count: int, cfl: float, sweeps: int, omega: float = 1.5,
+                        growth: float = 1.2):
+        """Implicit steps accelerated by `anderson` (size 4 ni nj), the inner
+        loop of solveTransonic (transonic-flow.ts:184-208) on the device."""
+        check(N.lib().bs_iterate_anderson(self._h, anderson._h, int(count), float(cfl), int(sweeps), float(omega),
+                                          float(growth), C.byref(self.last_report)))
+
     def enableGraphs(self, on: bool = True):
         """Run steps as captured HIP graphs (default on)."""
         check(N.lib().bs_enable_graphs(self._h, 1 if on else 0))
@@ -330,6 +337,47 @@ class GpuEuler2D:
         L = C.c_int64()
         check(N.lib().bs_kernel_timing(self._h, KERNELS[kernel], C.byref(ms), C.byref(L)))
         return ms.value, L.value
+
+
+class AndersonMixing:
+    """anderson.ts:20-123 AndersonMixing with its history on the GPU."""
+
+    def __init__(self, ctx: Context, size: int, depth: int):
+        h = C.c_void_p()
+        check(N.lib().bs_anderson_create(ctx._h, int(size), int(depth), C.byref(h)))
+        self._h = h
+        self.size, self.depth = int(size), int(depth)
+
+    def reset(self):
+        check(N.lib().bs_anderson_reset(self._h))
+
+    def next(self, x, f) -> np.ndarray:
+        """The next iterate from x and its correction f (both unchanged)."""
+        xa = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        fa = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+        if xa.size != self.size or fa.size != self.size:
+            raise RangeError("wrong vector size")
+        out = np.empty(self.size)
+        check(N.lib().bs_anderson_next(self._h, N.dptr(xa), N.dptr(fa), N.dptr(out)))
+        return out
+
+    def state(self):
+        """(columns held, columns used, restarts, gamma oldest first)."""
+        cols, used, restarts = C.c_int32(), C.c_int32(), C.c_int32()
+        gamma = np.zeros(self.depth)
+        check(N.lib().bs_anderson_state(self._h, C.byref(cols), C.byref(used), C.byref(restarts), N.dptr(gamma)))
+        return cols.value, used.value, restarts.value, gamma
+
+    def destroy(self):
+        if getattr(self, "_h", None):
+            N.lib().bs_anderson_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 def gpu_primitives(state: GpuState2D, gamma: float) -> np.ndarray:

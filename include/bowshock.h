@@ -219,6 +219,32 @@ bs_status bs_iterate_implicit(bs_solver *s, int32_t count, double cfl_max, int32
 enum { BS_RELAX_FLUX_DIFFERENCE = 0, BS_RELAX_LINEARIZED = 1 };
 bs_status bs_set_relaxation(bs_solver *s, int32_t form);
 
+/* Anderson acceleration (src/solver/anderson.ts:20-123 AndersonMixing):
+ * the history of the last `depth` (1 to 16) differences of iterates and
+ * of their corrections, in FP64 in device memory. bs_anderson_next is
+ * AndersonMixing.next on host vectors of `size` values: the next iterate
+ * x + f - (dX + dF) gamma, gamma the least-squares fit of f by dF (the
+ * oldest columns dropped while a Cholesky pivot of dF^T dF is below 1e-7
+ * of its column's norm; the reference's modified Gram-Schmidt drops below
+ * 1e-10). bs_anderson_reset is AndersonMixing.reset. */
+typedef struct bs_anderson bs_anderson;
+bs_status bs_anderson_create(bs_ctx *ctx, int64_t size, int32_t depth, bs_anderson **out);
+bs_status bs_anderson_destroy(bs_anderson *a);
+bs_status bs_anderson_reset(bs_anderson *a);
+bs_status bs_anderson_next(bs_anderson *a, const double *x, const double *f, double *out);
+/* Columns held, columns the last solve used, resets by the solver loop,
+ * and gamma (depth values, oldest column first); any pointer may be NULL. */
+bs_status bs_anderson_state(bs_anderson *a, int32_t *columns, int32_t *used, int32_t *restarts, double *gamma);
+/* `count` accelerated implicit steps of solveTransonic's loop
+ * (src/analysis/transonic-flow.ts:184-208) on the device: the correction f
+ * of an implicit step (bs_iterate_implicit's, before its update), the
+ * history reset when |f|^2 exceeds 100 times its smallest since the last
+ * reset, the mixed iterate as the new state, or, where a cell of it is not
+ * physical, the plain implicit step and a reset; the CFL ramp as
+ * bs_iterate_implicit's. The history's size must be 4 ni nj. */
+bs_status bs_iterate_anderson(bs_solver *s, bs_anderson *a, int32_t count, double cfl_max, int32_t sweeps,
+                              double omega, double growth, bs_report *report);
+
 /* Steps run as captured HIP graphs of 12 steps each (one launch per graph
  * instead of one per kernel), on by default; no counterpart in the
  * reference, whose WebGPU engine encodes command buffers instead. */
@@ -227,7 +253,8 @@ bs_status bs_enable_graphs(bs_solver *s, int32_t enable);
 /* Mean device time per launch of the solver's kernels since the last call,
  * for throughput reporting: kernel-name index (0 stage, 1 implicit prepare,
  * 2 implicit relaxation, whose last launch of a step also applies the
- * update on an FP32 solver; 3 the FP64 solver's update launch), milliseconds per
+ * update on an FP32 solver; 3 the FP64 solver's update launch, or the
+ * Anderson kernels of an accelerated step), milliseconds per
  * launch and the launches counted. Timing is taken with HIP events on the solver's stream
  * when enabled with bs_enable_timing: enable = 1 records a pair of events
  * around every launch (steps then run without graphs); enable = 2 records
