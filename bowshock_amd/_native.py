@@ -65,6 +65,8 @@ SIGNATURES = {
                                       C.POINTER(Report)]),
     "bs_enable_graphs": (C.c_int, [_vp, C.c_int32]),
     "bs_set_relaxation": (C.c_int, [_vp, C.c_int32]),
+    "bs_set_fitted_shock": (C.c_int, [_vp, _dp]),
+    "bs_read_shock": (C.c_int, [_vp, _dp, _dp, _dp, C.c_int32]),
     "bs_anderson_create": (C.c_int, [_vp, C.c_int64, C.c_int32, C.POINTER(_vp)]),
     "bs_anderson_destroy": (C.c_int, [_vp]),
     "bs_anderson_reset": (C.c_int, [_vp]),

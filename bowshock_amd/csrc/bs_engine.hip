@@ -92,7 +92,8 @@ struct DeviceBuffers {
   void* face_flux[4] = {nullptr, nullptr, nullptr, nullptr};
   double* face_geo[4] = {nullptr, nullptr, nullptr, nullptr};
   double* wall_out = nullptr;
-  // implicit work (FP32 solvers only)
+  double* shock_out = nullptr;  // fitted shock: speed, Mach number, signal per jMax face
+  // implicit work
   float *res = nullptr, *diag = nullptr, *lam_i = nullptr, *lam_j = nullptr, *dq = nullptr;
 };
 
@@ -127,6 +128,8 @@ struct bs_solver {
   float gamma32;
   bool fp64 = false;  // state, metrics and stencil arithmetic in FP64
   bool relax_linear = false;  // BS_RELAX_LINEARIZED (always on an FP64 solver)
+  bool fitted = false;        // jMax is a fitted shock ahead of which the free stream is shock_free
+  double shock_free[4] = {0, 0, 0, 0};
   hipStream_t stream = nullptr;
   DeviceBuffers d;
   int cur = 0;
@@ -364,6 +367,21 @@ bs_status encode_step_t(bs_solver* s) {
   const T *ca, *cb;
   const int ns = rk_table(s->cfg.integrator, &ca, &cb);
   const bool global = s->cfg.time_stepping == BS_STEPPING_GLOBAL;
+  if (s->fitted) {
+    // The fitted shock's faces from the solution at the start of the step,
+    // held through its stages (euler2d-gpu.ts:906-912).
+    ShockArgsT<T> S;
+    S.ni = s->ni;
+    S.nj = s->nj;
+    S.s = state_of<T>(s, s->cur);
+    S.face_geo = s->d.face_geo[JMAX];
+    S.cons = static_cast<T*>(s->d.face_cons[JMAX]);
+    S.flux = static_cast<T*>(s->d.face_flux[JMAX]);
+    S.out = s->d.shock_out;
+    for (int k = 0; k < 4; k++) S.free[k] = s->shock_free[k];
+    S.gamma = state_gamma(s);
+    hipLaunchKernelGGL(shock_boundary_kernel<T>, dim3((s->ni + 255) / 256), dim3(256), 0, s->stream, S);
+  }
   if (global) {
     const int nb = (s->n + 255) / 256;
     hipLaunchKernelGGL(time_step_kernel<T>, dim3(nb), dim3(256), 0, s->stream, grid_of<T>(s),
@@ -511,7 +529,7 @@ void free_all(bs_solver* s) {
   drop_graph(s->anderson_graph);
   DeviceBuffers& d = s->d;
   void* all[] = {d.state[0], d.state[1], d.state[2], d.inx, d.iny, d.ia, d.jnx, d.jny, d.ja, d.vol, d.planar,
-                 d.dt, d.dt_global, d.dt_partials, d.partials, d.scalar, d.control, d.bnd, d.wall_out, d.res,
+                 d.dt, d.dt_global, d.dt_partials, d.partials, d.scalar, d.control, d.bnd, d.wall_out, d.shock_out, d.res,
                  d.diag, d.lam_i, d.lam_j, d.dq};
   for (void* p : all)
     if (p) (void)hipFree(p);
@@ -1248,6 +1266,7 @@ bs_status bs_iterate_implicit(bs_solver* s, int32_t count, double cfl_max, int32
   if (!(cfl_max > 0 && omega >= 1 && growth >= 1))
     return fail(BS_ERR_RANGE, "implicit steps need cfl > 0, omega >= 1 and growth >= 1");
   if (sweeps < 1) return fail(BS_ERR_RANGE, "the relaxation needs a whole number of sweeps, at least one");
+  if (s->fitted) return fail(BS_ERR_UNSUPPORTED, "implicit steps do not move a fitted shock");
   bs_status st = ensure_implicit(s);
   if (st) return st;
   const size_t n = s->n;
@@ -1491,6 +1510,7 @@ bs_status bs_iterate_anderson(bs_solver* s, bs_anderson* a, int32_t count, doubl
   if (!(cfl_max > 0 && omega >= 1 && growth >= 1))
     return fail(BS_ERR_RANGE, "implicit steps need cfl > 0, omega >= 1 and growth >= 1");
   if (sweeps < 1) return fail(BS_ERR_RANGE, "the relaxation needs a whole number of sweeps, at least one");
+  if (s->fitted) return fail(BS_ERR_UNSUPPORTED, "implicit steps do not move a fitted shock");
   bs_status st = ensure_implicit(s);
   if (st) return st;
   // The history's own stream may still run a reset or a next.
@@ -1527,6 +1547,42 @@ bs_status bs_iterate_anderson(bs_solver* s, bs_anderson* a, int32_t count, doubl
   if (count == 0) {
     if ((st = read_control(s, &c))) return st;
     fill_report(c, report);
+  }
+  return BS_OK;
+}
+
+bs_status bs_set_fitted_shock(bs_solver* s, const double* free_stream) {
+  SOLVER_GUARD(s);
+  if (free_stream) {
+    if (s->cfg.boundaries[JMAX].kind != BS_BC_FACE_STATES)
+      return fail(BS_ERR_RANGE, "a fitted shock needs a 'faceStates' jMax boundary");
+    if (s->nj < 2) return fail(BS_ERR_RANGE, "a fitted shock needs two rows of cells below it");
+    if (!physical(free_stream)) return fail(BS_ERR_NONPHYSICAL, "the free stream is not physical");
+    if (!s->d.shock_out) {
+      bs_status st = dev_alloc(&s->d.shock_out, 3 * (size_t)s->ni, "shock");
+      if (st) return st;
+      HIP_TRY(hipMemset(s->d.shock_out, 0, 3 * (size_t)s->ni * sizeof(double)), "shock");
+    }
+    for (int k = 0; k < 4; k++) s->shock_free[k] = free_stream[k];
+  }
+  s->fitted = free_stream != nullptr;
+  s->imposed[JMAX] = s->fitted;
+  drop_graph(s->explicit_graph);
+  return upload_boundary(s);
+}
+
+bs_status bs_read_shock(bs_solver* s, double* speed, double* mach, double* signal, int32_t count) {
+  SOLVER_GUARD(s);
+  if (!s->fitted) return fail(BS_ERR_RANGE, "there is no fitted shock");
+  if (count != s->ni) return fail(BS_ERR_RANGE, "the shock has %d faces", s->ni);
+  std::vector<double> out(3 * (size_t)s->ni);
+  HIP_TRY(hipMemcpyAsync(out.data(), s->d.shock_out, out.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream),
+          "shock readback");
+  HIP_TRY(hipStreamSynchronize(s->stream), "sync");
+  for (int i = 0; i < s->ni; i++) {
+    if (speed) speed[i] = out[3 * (size_t)i];
+    if (mach) mach[i] = out[3 * (size_t)i + 1];
+    if (signal) signal[i] = out[3 * (size_t)i + 2];
   }
   return BS_OK;
 }

@@ -1004,4 +1004,96 @@ __global__ __launch_bounds__(256) void wall_flux_kernel(WallArgsT<S> A, WallBoun
   wall_body<S, REC, FLUX>(A, WB, blockIdx.x, blockIdx.x);
 }
 
+// ---------------------------------------------------------------------------
+// Fitted shock on jMax (euler2d.wgsl:402-454 compute_shock_boundary;
+// shock-fitting.ts:54-113 shockState / compatibleShock, 172-197
+// shockBoundary), in FP64 whatever the state's precision.
+// ---------------------------------------------------------------------------
+
+struct ShockStateD {
+  Prim<double> w;
+  double speed, normal_velocity;
+};
+
+// The Rankine-Hugoniot state behind a shock of outward normal (nx, ny) that
+// the free stream meets at relative normal Mach number `mach`; the shock
+// moves at W = mach a + q_n along n.
+__device__ inline ShockStateD shock_state(const double free[4], double g, double mach, double nx, double ny) {
+  const double a = sqrt(g * free[3] / free[0]);
+  const double vn = free[1] * nx + free[2] * ny;
+  const double speed = mach * a + vn;
+  const double m2 = mach * mach;
+  const double density_ratio = ((g + 1.0) * m2) / ((g - 1.0) * m2 + 2.0);
+  const double pressure_ratio = 1.0 + ((2.0 * g) / (g + 1.0)) * (m2 - 1.0);
+  const double un = speed - (mach * a) / density_ratio;
+  return {{free[0] * density_ratio, free[1] + (un - vn) * nx, free[2] + (un - vn) * ny, free[3] * pressure_ratio},
+          speed,
+          un};
+}
+
+template <typename T>
+struct ShockArgsT {
+  int ni, nj;
+  StateDevT<T> s;
+  const double* face_geo;  // jMax: x, y, nx, ny, area per face (normal towards increasing j)
+  T* cons;                 // jMax face states, conserved (4 per face)
+  T* flux;                 // jMax imposed fluxes per unit area (4 per face)
+  double* out;             // per face: speed, relative normal Mach number, |u| + a of the cell below
+  double free[4];          // primitive free stream ahead of the shock
+  double gamma;            // the state's gamma
+};
+
+// One thread per jMax face: the shock consistent with the u_n + a
+// characteristic from the two cells below the face, extrapolated to it;
+// its state fills the ghost cells, its physical flux is the face flux.
+template <typename T>
+__global__ __launch_bounds__(256) void shock_boundary_kernel(ShockArgsT<T> A) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= A.ni) return;
+  const int ni = A.ni, nj = A.nj;
+  const double g = A.gamma;
+  const double nx = A.face_geo[5 * i + 2], ny = A.face_geo[5 * i + 3];
+  const Prim<double> c1 = prim64<double>(A.s, i + (nj - 1) * ni, g);
+  const Prim<double> c2 = prim64<double>(A.s, i + (nj - 2) * ni, g);
+  const double a1 = sqrt(g * c1.p / c1.rho);
+  const double impedance = c1.rho * a1;
+  const double target = (1.5 * c1.p - 0.5 * c2.p) +
+                        impedance * (1.5 * (c1.u * nx + c1.v * ny) - 0.5 * (c2.u * nx + c2.v * ny));
+  auto mismatch = [&](double m) {
+    const ShockStateD st = shock_state(A.free, g, m, nx, ny);
+    return st.w.p + impedance * st.normal_velocity - target;
+  };
+  double mach = 1.0;
+  if (mismatch(1.0) < 0.0) {
+    double low = 1.0, high = 2.0;
+    // Both terms grow with M, so the doubling ends; 1000 doublings would
+    // overflow first (a non-physical cell below the face).
+    for (int k = 0; k < 1000 && mismatch(high) < 0.0; k++) high *= 2.0;
+    for (int k = 0; k < 100 && high - low > 1e-15 * high; k++) {
+      const double mid = 0.5 * (low + high);
+      if (mismatch(mid) < 0.0) low = mid;
+      else high = mid;
+    }
+    mach = 0.5 * (low + high);
+  }
+  const ShockStateD sh = shock_state(A.free, g, mach, nx, ny);
+  const Prim<double> w = sh.w;
+  const double energy = w.p / (g - 1.0) + 0.5 * w.rho * (w.u * w.u + w.v * w.v);
+  const double un = w.u * nx + w.v * ny;
+  T* c = A.cons + 4 * i;
+  c[0] = (T)w.rho;
+  c[1] = (T)(w.rho * w.u);
+  c[2] = (T)(w.rho * w.v);
+  c[3] = (T)energy;
+  T* f = A.flux + 4 * i;
+  f[0] = (T)(w.rho * un);
+  f[1] = (T)(w.rho * w.u * un + w.p * nx);
+  f[2] = (T)(w.rho * w.v * un + w.p * ny);
+  f[3] = (T)(un * (energy + w.p));
+  double* o = A.out + 3 * i;
+  o[0] = sh.speed;
+  o[1] = mach;
+  o[2] = sqrt(c1.u * c1.u + c1.v * c1.v) + a1;
+}
+
 }  // namespace bs

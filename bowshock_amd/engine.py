@@ -296,6 +296,22 @@ class GpuEuler2D:
                                        C.byref(out)))
         return out.value
 
+    def setFittedShock(self, freeStream):
+        """GpuEuler2DConfig.fittedShock (euler2d-gpu.ts:85-91): jMax, a
+        faceStates side, becomes a fitted shock with this free stream ahead
+        of it (None turns it off)."""
+        if freeStream is None:
+            check(N.lib().bs_set_fitted_shock(self._h, None))
+            return
+        w = np.ascontiguousarray(freeStream, dtype=np.float64)
+        check(N.lib().bs_set_fitted_shock(self._h, N.dptr(w)))
+
+    def readShock(self):
+        """euler2d-gpu.ts:630-656: speed, Mach number and signal per jMax face."""
+        out = [np.zeros(self.ni) for _ in range(3)]
+        check(N.lib().bs_read_shock(self._h, *[N.dptr(a) for a in out], self.ni))
+        return {"speed": out[0], "mach": out[1], "signal": out[2]}
+
     def setRelaxation(self, form: str):
         """The relaxation's neighbour term: "fluxDifference" (the
         reference's, euler2d.ts:981-1005; default of FP32 solvers) or

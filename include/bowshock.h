@@ -207,6 +207,22 @@ bs_status bs_set_implicit_cfl(bs_solver *s, double cfl);
 bs_status bs_iterate_implicit(bs_solver *s, int32_t count, double cfl_max, int32_t sweeps,
                               double omega, double growth, bs_report *report);
 
+/* The fitted shock (GpuEuler2DConfig.fittedShock, euler2d-gpu.ts:85-91):
+ * the jMax boundary, which must be a faceStates side, is a shock with this
+ * free stream (rho, u, v, p) ahead of it. At the start of every explicit
+ * step the GPU sets each jMax face's ghost state and imposed flux to those
+ * of the shock consistent with the u_n + a characteristic from the two
+ * cells below it (euler2d.wgsl:402-454 compute_shock_boundary;
+ * src/solver/shock-fitting.ts:91-113 compatibleShock), in FP64, and records
+ * the shock's speed. NULL turns it off. Implicit steps are refused while it
+ * is on (euler2d-gpu.ts:705-707). */
+bs_status bs_set_fitted_shock(bs_solver *s, const double *free_stream);
+/* readShock (euler2d-gpu.ts:630-656): per jMax face (count = ni), the
+ * shock's speed along the face's outward normal as set at the start of the
+ * last step, its relative normal Mach number, and |u| + a of the cell
+ * below; any pointer may be NULL. */
+bs_status bs_read_shock(bs_solver *s, double *speed, double *mach, double *signal, int32_t count);
+
 /* The neighbour term of the red-black relaxation. BS_RELAX_FLUX_DIFFERENCE
  * is the reference's A (1/2 (F(Q + dQ) - F(Q)) - 1/2 omega lambda dQ)
  * (euler2d.ts:981-1005, euler2d-implicit.wgsl), the default of an FP32
