@@ -67,6 +67,11 @@ SIGNATURES = {
     "bs_set_relaxation": (C.c_int, [_vp, C.c_int32]),
     "bs_set_fitted_shock": (C.c_int, [_vp, _dp]),
     "bs_read_shock": (C.c_int, [_vp, _dp, _dp, _dp, C.c_int32]),
+    "bs_set_line_grid": (C.c_int, [_vp, _dp, _dp, C.c_int32]),
+    "bs_set_heights": (C.c_int, [_vp, _dp]),
+    "bs_move_grid": (C.c_int, [_vp, _dp]),
+    "bs_iterate_fitted": (C.c_int, [_vp, C.c_int32, C.c_int32, C.POINTER(Report)]),
+    "bs_read_heights": (C.c_int, [_vp, _dp]),
     "bs_anderson_create": (C.c_int, [_vp, C.c_int64, C.c_int32, C.POINTER(_vp)]),
     "bs_anderson_destroy": (C.c_int, [_vp]),
     "bs_anderson_reset": (C.c_int, [_vp]),

@@ -20,6 +20,7 @@
 #include "../../include/bowshock.h"
 #include "bs_kernels.hpp"
 #include "bs_anderson.hpp"
+#include "bs_linegrid.hpp"
 
 using namespace bs;
 
@@ -93,6 +94,8 @@ struct DeviceBuffers {
   double* face_geo[4] = {nullptr, nullptr, nullptr, nullptr};
   double* wall_out = nullptr;
   double* shock_out = nullptr;  // fitted shock: speed, Mach number, signal per jMax face
+  // the grid on lines (bs_set_line_grid): per line x, y, dx, dy; fractions; heights; last moves
+  double *line = nullptr, *eta = nullptr, *height = nullptr, *move = nullptr;
   // implicit work
   float *res = nullptr, *diag = nullptr, *lam_i = nullptr, *lam_j = nullptr, *dq = nullptr;
 };
@@ -129,6 +132,8 @@ struct bs_solver {
   bool fp64 = false;  // state, metrics and stencil arithmetic in FP64
   bool relax_linear = false;  // BS_RELAX_LINEARIZED (always on an FP64 solver)
   bool fitted = false;        // jMax is a fitted shock ahead of which the free stream is shock_free
+  bool lines = false;         // the grid lies on lines with moving heights
+  bool symmetry = false;      // the lines' first node is normal to the axis
   double shock_free[4] = {0, 0, 0, 0};
   hipStream_t stream = nullptr;
   DeviceBuffers d;
@@ -529,7 +534,8 @@ void free_all(bs_solver* s) {
   drop_graph(s->anderson_graph);
   DeviceBuffers& d = s->d;
   void* all[] = {d.state[0], d.state[1], d.state[2], d.inx, d.iny, d.ia, d.jnx, d.jny, d.ja, d.vol, d.planar,
-                 d.dt, d.dt_global, d.dt_partials, d.partials, d.scalar, d.control, d.bnd, d.wall_out, d.shock_out, d.res,
+                 d.dt, d.dt_global, d.dt_partials, d.partials, d.scalar, d.control, d.bnd, d.wall_out, d.shock_out,
+                 d.line, d.eta, d.height, d.move, d.res,
                  d.diag, d.lam_i, d.lam_j, d.dq};
   for (void* p : all)
     if (p) (void)hipFree(p);
@@ -1583,6 +1589,165 @@ bs_status bs_read_shock(bs_solver* s, double* speed, double* mach, double* signa
     if (speed) speed[i] = out[3 * (size_t)i];
     if (mach) mach[i] = out[3 * (size_t)i + 1];
     if (signal) signal[i] = out[3 * (size_t)i + 2];
+  }
+  return BS_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+LineGridDev line_grid_of(bs_solver* s) {
+  LineGridDev L;
+  L.ni = s->ni;
+  L.nj = s->nj;
+  L.axisymmetric = s->cfg.geometry == BS_GEOMETRY_AXISYMMETRIC;
+  L.symmetry = s->symmetry;
+  L.lines = s->d.line;
+  L.eta = s->d.eta;
+  L.h = s->d.height;
+  L.move = s->d.move;
+  return L;
+}
+
+template <typename T>
+void launch_line_metrics(bs_solver* s) {
+  MetricOut<T> M;
+  M.inx = static_cast<T*>(s->d.inx); M.iny = static_cast<T*>(s->d.iny); M.ia = static_cast<T*>(s->d.ia);
+  M.jnx = static_cast<T*>(s->d.jnx); M.jny = static_cast<T*>(s->d.jny); M.ja = static_cast<T*>(s->d.ja);
+  M.vol = static_cast<T*>(s->d.vol); M.planar = static_cast<T*>(s->d.planar);
+  for (int k = 0; k < 4; k++) M.geo[k] = s->d.face_geo[k];
+  const long long total = (long long)(s->ni + 1) * s->nj + (long long)s->ni * (s->nj + 1) + (long long)s->n;
+  hipLaunchKernelGGL(line_metrics_kernel<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s->stream,
+                     line_grid_of(s), M);
+}
+
+void line_metrics(bs_solver* s) { s->fp64 ? launch_line_metrics<double>(s) : launch_line_metrics<float>(s); }
+
+// The flow in the current buffer moved with the lines into the spare one.
+template <typename T>
+void launch_line_remap(bs_solver* s) {
+  const int spare = (s->cur + 1) % 3;
+  hipLaunchKernelGGL(line_remap_kernel<T>, dim3((s->n + 255) / 256), dim3(256), 0, s->stream, line_grid_of(s),
+                     state_of<T>(s, s->cur), state_of<T>(s, spare), state_gamma(s));
+  s->cur = spare;
+}
+
+void line_remap(bs_solver* s) { s->fp64 ? launch_line_remap<double>(s) : launch_line_remap<float>(s); }
+
+// One move of the fitted shock (GpuLineGrid.encodeShockMove): the nodes by
+// the last step's shock speeds, the flow remapped, the moved metrics.
+void encode_shock_move(bs_solver* s) {
+  hipLaunchKernelGGL(shock_moves_kernel, dim3(1), dim3(256), 0, s->stream, line_grid_of(s), s->d.shock_out);
+  line_remap(s);
+  line_metrics(s);
+}
+
+}  // namespace
+
+extern "C" {
+
+bs_status bs_set_line_grid(bs_solver* s, const double* lines, const double* fractions, int32_t first_node_symmetry) {
+  SOLVER_GUARD(s);
+  if (!lines || !fractions) return fail(BS_ERR_RANGE, "the lines and their fractions are needed");
+  const int ni = s->ni, nj = s->nj;
+  if (first_node_symmetry && ni < 2) return fail(BS_ERR_RANGE, "a first node normal to the axis needs three lines");
+  for (int i = 0; i <= ni; i++) {
+    const double* l = lines + 4 * i;
+    if (!(std::fabs(std::hypot(l[2], l[3]) - 1.0) < 1e-12)) return fail(BS_ERR_RANGE, "line %d's direction is not a unit vector", i);
+    for (int j = 0; j < nj; j++)
+      if (!(fractions[i + (size_t)(j + 1) * (ni + 1)] > fractions[i + (size_t)j * (ni + 1)]))
+        return fail(BS_ERR_RANGE, "the fractions of line %d do not increase", i);
+  }
+  bs_status st;
+  const size_t nl = (size_t)ni + 1;
+  if (!s->d.line && ((st = dev_alloc(&s->d.line, 4 * nl, "lines")) ||
+                     (st = dev_alloc(&s->d.eta, nl * (nj + 1), "fractions")) ||
+                     (st = dev_alloc(&s->d.height, nl, "heights")) || (st = dev_alloc(&s->d.move, nl, "moves"))))
+    return st;
+  HIP_TRY(hipMemcpy(s->d.line, lines, 4 * nl * sizeof(double), hipMemcpyHostToDevice), "lines");
+  HIP_TRY(hipMemcpy(s->d.eta, fractions, nl * (nj + 1) * sizeof(double), hipMemcpyHostToDevice), "fractions");
+  s->lines = true;
+  s->symmetry = first_node_symmetry != 0;
+  return BS_OK;
+}
+
+bs_status bs_set_heights(bs_solver* s, const double* heights) {
+  SOLVER_GUARD(s);
+  if (!s->lines) return fail(BS_ERR_RANGE, "set the lines first (bs_set_line_grid)");
+  if (!heights) return fail(BS_ERR_RANGE, "no heights");
+  const size_t nl = (size_t)s->ni + 1;
+  for (size_t i = 0; i < nl; i++)
+    if (!(heights[i] > 0)) return fail(BS_ERR_RANGE, "height %zu is not positive", i);
+  std::vector<double> zero(nl, 0.0);
+  HIP_TRY(hipMemcpy(s->d.height, heights, nl * sizeof(double), hipMemcpyHostToDevice), "heights");
+  HIP_TRY(hipMemcpy(s->d.move, zero.data(), nl * sizeof(double), hipMemcpyHostToDevice), "moves");
+  line_metrics(s);
+  HIP_TRY(hipGetLastError(), "line metrics");
+  HIP_TRY(hipStreamSynchronize(s->stream), "line metrics");
+  return BS_OK;
+}
+
+bs_status bs_read_heights(bs_solver* s, double* heights) {
+  SOLVER_GUARD(s);
+  if (!s->lines || !heights) return fail(BS_ERR_RANGE, "no lines or no output");
+  HIP_TRY(hipMemcpyAsync(heights, s->d.height, ((size_t)s->ni + 1) * sizeof(double), hipMemcpyDeviceToHost,
+                         s->stream), "heights");
+  HIP_TRY(hipStreamSynchronize(s->stream), "heights");
+  return BS_OK;
+}
+
+bs_status bs_move_grid(bs_solver* s, const double* heights) {
+  SOLVER_GUARD(s);
+  if (!s->lines) return fail(BS_ERR_RANGE, "set the lines first (bs_set_line_grid)");
+  if (!heights) return fail(BS_ERR_RANGE, "no heights");
+  const size_t nl = (size_t)s->ni + 1;
+  std::vector<double> now(nl), move(nl);
+  HIP_TRY(hipMemcpyAsync(now.data(), s->d.height, nl * sizeof(double), hipMemcpyDeviceToHost, s->stream), "heights");
+  HIP_TRY(hipStreamSynchronize(s->stream), "heights");
+  for (size_t i = 0; i < nl; i++) {
+    if (!(heights[i] > 0)) return fail(BS_ERR_RANGE, "height %zu is not positive", i);
+    move[i] = heights[i] - now[i];
+  }
+  HIP_TRY(hipMemcpy(s->d.height, heights, nl * sizeof(double), hipMemcpyHostToDevice), "heights");
+  HIP_TRY(hipMemcpy(s->d.move, move.data(), nl * sizeof(double), hipMemcpyHostToDevice), "moves");
+  line_remap(s);
+  line_metrics(s);
+  HIP_TRY(hipGetLastError(), "grid move");
+  HIP_TRY(hipStreamSynchronize(s->stream), "grid move");
+  return BS_OK;
+}
+
+bs_status bs_iterate_fitted(bs_solver* s, int32_t moves, int32_t steps_per_move, bs_report* report) {
+  SOLVER_GUARD(s);
+  if (!s->fitted) return fail(BS_ERR_RANGE, "there is no fitted shock");
+  if (!s->lines) return fail(BS_ERR_RANGE, "set the lines first (bs_set_line_grid)");
+  if (moves < 0) return fail(BS_ERR_RANGE, "the move count cannot be negative");
+  if (steps_per_move < 1 || steps_per_move > BATCH)
+    return fail(BS_ERR_RANGE, "steps per move must be an integer from 1 to %d", BATCH);
+  const int per_batch = BATCH / steps_per_move;
+  Control c;
+  int remaining = moves;
+  while (remaining > 0) {
+    const int batch = remaining < per_batch ? remaining : per_batch;
+    for (int m = 0; m < batch; m++) {
+      for (int k = 0; k < steps_per_move; k++) {
+        bs_status st = encode_step(s);
+        if (st) return st;
+      }
+      encode_shock_move(s);
+    }
+    HIP_TRY(hipGetLastError(), "fitted steps");
+    bs_status st = read_control(s, &c);
+    if (st) return st;
+    fill_report(c, report);
+    if ((st = account(s, c, (long long)batch * steps_per_move))) return st;
+    remaining -= batch;
+  }
+  if (moves == 0) {
+    bs_status st = read_control(s, &c);
+    if (st) return st;
+    fill_report(c, report);
   }
   return BS_OK;
 }

@@ -312,6 +312,34 @@ class GpuEuler2D:
         check(N.lib().bs_read_shock(self._h, *[N.dptr(a) for a in out], self.ni))
         return {"speed": out[0], "mach": out[1], "signal": out[2]}
 
+    def setLineGrid(self, lines, fractions, firstNode: str = "fixed"):
+        """GpuLineGrid.create (line-grid.ts:63-110) on this solver: lines
+        as (ni + 1, 4) rows x, y, dx, dy (unit directions), fractions as
+        (nj + 1, ni + 1) (row j, line i); firstNode "symmetry" or "fixed"."""
+        ln = np.ascontiguousarray(lines, dtype=np.float64).reshape(self.ni + 1, 4)
+        fr = np.ascontiguousarray(fractions, dtype=np.float64).reshape(self.nj + 1, self.ni + 1)
+        check(N.lib().bs_set_line_grid(self._h, N.dptr(ln), N.dptr(fr), 1 if firstNode == "symmetry" else 0))
+
+    def setHeights(self, heights):
+        """GpuLineGrid.setHeights: the nodes at these heights, metrics on the GPU."""
+        h = np.ascontiguousarray(heights, dtype=np.float64)
+        check(N.lib().bs_set_heights(self._h, N.dptr(h)))
+
+    def moveGrid(self, heights):
+        """euler2d-gpu.ts:400-405: the lines to these heights, the flow remapped."""
+        h = np.ascontiguousarray(heights, dtype=np.float64)
+        check(N.lib().bs_move_grid(self._h, N.dptr(h)))
+
+    def iterateFitted(self, moves: int, stepsPerMove: int = 1):
+        """euler2d-gpu.ts:481-505."""
+        check(N.lib().bs_iterate_fitted(self._h, int(moves), int(stepsPerMove), C.byref(self.last_report)))
+
+    def readHeights(self) -> np.ndarray:
+        """GpuLineGrid.readHeights (line-grid.ts:257-270)."""
+        h = np.zeros(self.ni + 1)
+        check(N.lib().bs_read_heights(self._h, N.dptr(h)))
+        return h
+
     def setRelaxation(self, form: str):
         """The relaxation's neighbour term: "fluxDifference" (the
         reference's, euler2d.ts:981-1005; default of FP32 solvers) or

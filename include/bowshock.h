@@ -223,6 +223,29 @@ bs_status bs_set_fitted_shock(bs_solver *s, const double *free_stream);
  * below; any pointer may be NULL. */
 bs_status bs_read_shock(bs_solver *s, double *speed, double *mach, double *signal, int32_t count);
 
+/* The moving grid of a fitted shock (GpuLineGrid, src/engine-gpu/
+ * line-grid.ts): the solver's grid lies on ni + 1 straight lines from the
+ * body, `lines` holding per line its origin and unit direction (x, y, dx,
+ * dy), node (i, j) at the origin plus h_i fractions[i + j (ni + 1)] times
+ * the direction (fixed fractions, blunt-grid.ts withFixedFractions), the
+ * first node normal to the axis when `first_node_symmetry` (else held).
+ * bs_set_heights (setHeights) places the nodes and recomputes the metrics
+ * on the GPU; bs_move_grid (moveGrid / moveTo) moves them by less than
+ * half the end cells' height and remaps the flow along the lines
+ * (remapAlongLines, src/solver/shock-fitting.ts:305-338);
+ * bs_iterate_fitted (iterateFitted, euler2d-gpu.ts:481-505) runs `moves`
+ * times `steps_per_move` explicit steps with the fitted shock moved after
+ * each group by the last step's shock speeds (fitShock,
+ * shock-fitting.ts:252-296); bs_read_heights reads the heights back. The
+ * initial metrics of bs_solver_create stay the solver's until
+ * bs_set_heights. */
+bs_status bs_set_line_grid(bs_solver *s, const double *lines, const double *fractions,
+                           int32_t first_node_symmetry);
+bs_status bs_set_heights(bs_solver *s, const double *heights);
+bs_status bs_move_grid(bs_solver *s, const double *heights);
+bs_status bs_iterate_fitted(bs_solver *s, int32_t moves, int32_t steps_per_move, bs_report *report);
+bs_status bs_read_heights(bs_solver *s, double *heights);
+
 /* The neighbour term of the red-black relaxation. BS_RELAX_FLUX_DIFFERENCE
  * is the reference's A (1/2 (F(Q + dQ) - F(Q)) - 1/2 omega lambda dQ)
  * (euler2d.ts:981-1005, euler2d-implicit.wgsl), the default of an FP32

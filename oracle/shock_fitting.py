@@ -102,3 +102,97 @@ def fitted_steps(orc: O.Euler2D, grid, free, gamma, count: int):
         orc.set_face_fluxes("jMax", last[1])
         orc.iterate(1)
     return last
+
+
+def grid_on_lines(lines, eta, heights):
+    """Nodes at each line's origin plus h_i eta[j, i] times its direction
+    (gridOnLines, blunt-grid.ts:602-620, with fixed fractions)."""
+    from bowshock_amd.grids import StructuredGrid
+
+    lines = np.asarray(lines, dtype=np.float64)
+    eta = np.asarray(eta, dtype=np.float64)
+    nj, ni = eta.shape[0] - 1, eta.shape[1] - 1
+    s = eta * np.asarray(heights, dtype=np.float64)[None, :]
+    x = lines[None, :, 0] + s * lines[None, :, 2]
+    y = lines[None, :, 1] + s * lines[None, :, 3]
+    return StructuredGrid(ni, nj, x.reshape(-1), y.reshape(-1))
+
+
+def remap_along_lines(lines, nj, old_prim, old_grid, new_grid):
+    """shock-fitting.ts:305-338 remapAlongLines: the old primitives (n, 4)
+    interpolated linearly between old cell centres along each column to
+    the new cells' centres, held at the end cells."""
+    ni = old_grid.ni
+
+    def along(g, i, j):
+        k = i + j * (ni + 1)
+        l = lines[i]
+        return (g.x[k] - l[0]) * l[2] + (g.y[k] - l[1]) * l[3]
+
+    def column(g, i, j):
+        return 0.5 * (along(g, i, j) + along(g, i + 1, j))
+
+    out = np.zeros_like(old_prim)
+    for j in range(nj):
+        for i in range(ni):
+            d = 0.5 * (column(new_grid, i, j) + column(new_grid, i, j + 1))
+
+            def centre(jj):
+                return 0.5 * (column(old_grid, i, jj) + column(old_grid, i, jj + 1))
+
+            jj = 0
+            while jj < nj - 2 and centre(jj + 1) < d:
+                jj += 1
+            t = min(max((d - centre(jj)) / (centre(jj + 1) - centre(jj)), 0.0), 1.0)
+            a = old_prim[i + jj * ni]
+            b = old_prim[i + (jj + 1) * ni]
+            out[i + j * ni] = a + t * (b - a)
+    return out
+
+
+def fitted_moves(make, lines, eta, heights, free, gamma, initial, moves: int, symmetry: bool):
+    """fitShock (shock-fitting.ts:215-296) with one step per move: the
+    boundary, one step, each node moved by the speed of the shock face on
+    its axis side times half its top cell's height per signal speed, the
+    first node normal to the axis (symmetry) or held, the flow remapped
+    onto the new grid. The signal speed is the one the boundary recorded
+    at the start of the step, as the GPU engine's move takes it
+    (line-grid.wgsl node_move); fitShock takes the cell's after the step.
+    `make(grid, states)` builds the oracle solver for a grid with these
+    jMax face states. Returns (heights, oracle, grid)."""
+    lines = np.asarray(lines, dtype=np.float64)
+    eta = np.asarray(eta, dtype=np.float64)
+    nj, ni = eta.shape[0] - 1, eta.shape[1] - 1
+    h = np.array(heights, dtype=np.float64)
+    grid = grid_on_lines(lines, eta, h)
+    orc = make(grid, np.tile(np.asarray(free, dtype=np.float64), (ni, 1)))
+    orc.set_primitives(initial)
+    for _ in range(moves):
+        states, fluxes, speeds, machs, signals = shock_boundary(orc, grid, free, gamma)
+        orc.set_face_states("jMax", states)
+        orc.set_face_fluxes("jMax", fluxes)
+        orc.iterate(1)
+        nxt = h.copy()
+        m = orc.metrics
+        for k in range(1, ni + 1):
+            f = k - 1
+            nx, ny = m["jnx"][f + nj * ni], m["jny"][f + nj * ni]
+            l = lines[k]
+            size = h[k] * (eta[nj, k] - eta[nj - 1, k]) * math.hypot(l[2], l[3])
+            dt = 0.5 * size / signals[min(k, ni - 1)]
+            nxt[k] = h[k] + speeds[f] * dt / (l[2] * nx + l[3] * ny)
+        if symmetry:
+            def node(k):
+                s = nxt[k] * eta[nj, k]
+                return lines[k, 0] + s * lines[k, 2], lines[k, 1] + s * lines[k, 3]
+
+            x1, y1 = node(1)
+            x2, y2 = node(2)
+            c = (x2 - x1) / (y2 * y2 - y1 * y1)
+            nxt[0] = (x1 - c * y1 * y1 - lines[0, 0]) / (lines[0, 2] * eta[nj, 0])
+        new_grid = grid_on_lines(lines, eta, nxt)
+        prim = remap_along_lines(lines, nj, orc.primitives(), grid, new_grid)
+        orc = make(new_grid, states)
+        orc.set_primitives(prim)
+        grid, h = new_grid, nxt
+    return h, orc, grid
