@@ -387,7 +387,7 @@ static napi_value js_set_implicit_cfl(napi_env env, napi_callback_info info) {
 
 /* ---- asynchronous steps ------------------------------------------------ */
 
-enum { OP_ITERATE, OP_ADVANCE, OP_IMPLICIT };
+enum { OP_ITERATE, OP_ADVANCE, OP_IMPLICIT, OP_FITTED };
 
 typedef struct {
   napi_async_work work;
@@ -407,6 +407,7 @@ static void step_execute(napi_env env, void *data) {
   memset(&j->report, 0, sizeof j->report);
   if (j->op == OP_ITERATE) j->st = bs_iterate(j->s, j->count, &j->report);
   else if (j->op == OP_ADVANCE) j->st = bs_advance_to(j->s, j->t_end, j->max_steps, &j->report);
+  else if (j->op == OP_FITTED) j->st = bs_iterate_fitted(j->s, j->count, j->sweeps, &j->report);
   else j->st = bs_iterate_implicit(j->s, j->count, j->cfl, j->sweeps, j->omega, j->growth, &j->report);
   /* The message is thread-local to the worker: copy it here. */
   strncpy(j->message, bs_last_error(), sizeof j->message - 1);
@@ -470,6 +471,94 @@ static napi_value js_iterate_implicit(napi_env env, napi_callback_info info) {
   return queue_job(env, j);
 }
 
+/* iterateFitted(solver, moves, stepsPerMove): a promise of the report. */
+static napi_value js_iterate_fitted(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return NULL;
+  StepJob *j = (StepJob *)calloc(1, sizeof *j);
+  j->s = (bs_solver *)get_ptr(env, argv[0]);
+  j->op = OP_FITTED;
+  j->count = i32(env, argv[1]);
+  j->sweeps = i32(env, argv[2]);
+  return queue_job(env, j);
+}
+
+/* setFittedShock(solver, Float64Array(4) | null). */
+static napi_value js_set_fitted_shock(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  napi_valuetype t;
+  napi_typeof(env, argv[1], &t);
+  const double *w = NULL;
+  if (t != napi_null && t != napi_undefined) {
+    size_t n = 0;
+    w = (const double *)typed(env, argv[1], napi_float64_array, &n);
+    if (!w || n != 4) return throw_status(env, BS_ERR_RANGE);
+  }
+  bs_status st = bs_set_fitted_shock((bs_solver *)get_ptr(env, argv[0]), w);
+  if (st) return throw_status(env, st);
+  return NULL;
+}
+
+/* readShock(solver, ni): {speed, mach, signal} as Float64Arrays. */
+static napi_value js_read_shock(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  int32_t ni = i32(env, argv[1]);
+  double *q[3];
+  napi_value out, arr[3];
+  napi_create_object(env, &out);
+  const char *names[3] = {"speed", "mach", "signal"};
+  for (int k = 0; k < 3; k++) {
+    arr[k] = new_f64(env, (size_t)(ni > 0 ? ni : 0), &q[k]);
+    napi_set_named_property(env, out, names[k], arr[k]);
+  }
+  bs_status st = bs_read_shock((bs_solver *)get_ptr(env, argv[0]), q[0], q[1], q[2], ni);
+  if (st) return throw_status(env, st);
+  return out;
+}
+
+/* setLineGrid(solver, lines Float64Array, fractions Float64Array, symmetry). */
+static napi_value js_set_line_grid(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  if (!get_args(env, info, 4, argv)) return NULL;
+  size_t nl = 0, nf = 0;
+  const double *l = (const double *)typed(env, argv[1], napi_float64_array, &nl);
+  const double *f = (const double *)typed(env, argv[2], napi_float64_array, &nf);
+  if (!l || !f) return throw_status(env, BS_ERR_RANGE);
+  bs_status st = bs_set_line_grid((bs_solver *)get_ptr(env, argv[0]), l, f, i32(env, argv[3]));
+  if (st) return throw_status(env, st);
+  return NULL;
+}
+
+/* setHeights(solver, heights) / moveGrid(solver, heights). */
+static napi_value heights_call(napi_env env, napi_callback_info info, int move) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  size_t n = 0;
+  const double *h = (const double *)typed(env, argv[1], napi_float64_array, &n);
+  if (!h) return throw_status(env, BS_ERR_RANGE);
+  bs_solver *s = (bs_solver *)get_ptr(env, argv[0]);
+  bs_status st = move ? bs_move_grid(s, h) : bs_set_heights(s, h);
+  if (st) return throw_status(env, st);
+  return NULL;
+}
+
+static napi_value js_set_heights(napi_env env, napi_callback_info info) { return heights_call(env, info, 0); }
+static napi_value js_move_grid(napi_env env, napi_callback_info info) { return heights_call(env, info, 1); }
+
+/* readHeights(solver, ni + 1): Float64Array. */
+static napi_value js_read_heights(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  int32_t n = i32(env, argv[1]);
+  double *h;
+  napi_value arr = new_f64(env, (size_t)(n > 0 ? n : 0), &h);
+  bs_status st = bs_read_heights((bs_solver *)get_ptr(env, argv[0]), h);
+  if (st) return throw_status(env, st);
+  return arr;
+}
+
 static napi_value init(napi_env env, napi_value exports) {
   napi_property_descriptor props[] = {
       {"version", NULL, js_version, NULL, NULL, NULL, napi_enumerable, NULL},
@@ -492,6 +581,13 @@ static napi_value init(napi_env env, napi_value exports) {
       {"iterate", NULL, js_iterate, NULL, NULL, NULL, napi_enumerable, NULL},
       {"advanceTo", NULL, js_advance_to, NULL, NULL, NULL, napi_enumerable, NULL},
       {"iterateImplicit", NULL, js_iterate_implicit, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"iterateFitted", NULL, js_iterate_fitted, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"setFittedShock", NULL, js_set_fitted_shock, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"readShock", NULL, js_read_shock, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"setLineGrid", NULL, js_set_line_grid, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"setHeights", NULL, js_set_heights, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"moveGrid", NULL, js_move_grid, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"readHeights", NULL, js_read_heights, NULL, NULL, NULL, napi_enumerable, NULL},
   };
   napi_define_properties(env, exports, sizeof props / sizeof props[0], props);
   return exports;

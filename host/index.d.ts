@@ -41,7 +41,7 @@ export type BoundaryCondition2D =
   | { readonly kind: 'wall' }
   | { readonly kind: 'periodic' };
 
-/** euler2d.ts:58-81 / euler2d-gpu.ts:82-92 (no fitted shock, no source). */
+/** euler2d.ts:58-81 / euler2d-gpu.ts:82-92 (no manufactured source). */
 export interface GpuEuler2DConfig {
   readonly grid: StructuredGrid;
   readonly geometry: Geometry;
@@ -53,6 +53,8 @@ export interface GpuEuler2DConfig {
   readonly cfl: number;
   readonly timeStepping: 'global' | 'local';
   readonly boundaries: Readonly<Record<Side, BoundaryCondition2D>>;
+  /** euler2d-gpu.ts:85-91: jMax ('faceStates') is a fitted shock with this free stream ahead of it. */
+  readonly fittedShock?: { readonly freeStream: Primitive2D };
 }
 
 export interface FaceSet {
@@ -116,6 +118,19 @@ export class HipDevice {
   destroy(): void;
 }
 
+/** GpuLineGrid (src/engine-gpu/line-grid.ts) on a HIP solver. */
+export class GpuLineGrid {
+  static create(
+    device: HipDevice | number,
+    lines: readonly { x: number; y: number; dx: number; dy: number; fractions?: readonly number[] }[],
+    nj: number,
+    options: { readonly target: object; readonly boundary?: object; readonly firstNode?: 'symmetry' | 'fixed' },
+  ): Promise<GpuLineGrid>;
+  setHeights(heights: readonly number[]): void;
+  readHeights(): Promise<number[]>;
+  destroy(): void;
+}
+
 export class GpuEuler2D {
   /** options.precision 'fp64': state, metrics and residual in FP64 (implicit steps relax in FP32). */
   static create(
@@ -125,6 +140,12 @@ export class GpuEuler2D {
   ): Promise<GpuEuler2D>;
   readonly config: GpuEuler2DConfig;
   readonly precision: 'fp32' | 'fp64';
+  /** Stand-ins naming this solver, for GpuLineGrid.create. */
+  readonly geometryBuffer: object;
+  readonly boundaryBuffer: object;
+  moveGrid(lineGrid: GpuLineGrid, heights: readonly number[]): void;
+  iterateFitted(lineGrid: GpuLineGrid, moves: number, stepsPerMove: number): Promise<void>;
+  readShock(): Promise<{ speed: Float64Array; mach: Float64Array; signal: Float64Array }>;
   readonly metrics: GridMetrics;
   setInitialCondition(field: (x: number, y: number) => Primitive2D): void;
   setCellPrimitives(cell: (i: number, j: number) => Primitive2D): void;

@@ -124,9 +124,6 @@ function nativeConfig(config) {
   if (config.source !== undefined) {
     throw new RangeError('manufactured-solution sources are only supported by the CPU engine');
   }
-  if (config.fittedShock !== undefined) {
-    throw new RangeError('shock fitting stays on the host: a fitted shock is not supported by this engine');
-  }
   const { grid } = config;
   const boundaries = SIDES.map((side) => {
     const bc = config.boundaries[side];
@@ -217,6 +214,11 @@ class GpuEuler2D {
     this.exclusive = new ExclusiveRunner('GpuEuler2D');
     this.handle = call(() =>
       native().solverCreate(device.handle, nativeConfig(config), precision === 'fp64' ? 1 : 0));
+    if (config.fittedShock !== undefined) {
+      // euler2d-gpu.ts:240-242: the fitted shock is the jMax faceStates side.
+      const w = config.fittedShock.freeStream;
+      call(() => native().setFittedShock(this.handle, Float64Array.from([w.rho, w.u, w.v, w.p])));
+    }
     this.nCells = config.grid.ni * config.grid.nj;
     for (const side of SIDES) {
       const bc = config.boundaries[side];
@@ -332,18 +334,43 @@ class GpuEuler2D {
     await this.exclusive.run(() => native().iterateImplicit(this.handle, count, cfl, sweeps, omega, growth));
   }
 
-  /** The WebGPU buffer getters and grid motion of the reference engine belong to shock fitting, which stays on the host. */
+  /** GPU state buffers stay inside the library. */
   get stateBuffer() {
     throw new Error('GpuEuler2D (HIP): GPU buffers are not exposed');
   }
-  moveGrid() {
-    throw new Error('GpuEuler2D (HIP): shock fitting stays on the host');
+
+  /**
+   * euler2d-gpu.ts:376-394: stand-ins for the metric and boundary buffers
+   * a GpuLineGrid is made on; they name this solver.
+   */
+  get geometryBuffer() {
+    return { solver: this, part: 'geometry' };
   }
-  iterateFitted() {
-    throw new Error('GpuEuler2D (HIP): shock fitting stays on the host');
+  get boundaryBuffer() {
+    return { solver: this, part: 'boundary' };
   }
-  readShock() {
-    throw new Error('GpuEuler2D (HIP): shock fitting stays on the host');
+
+  /** euler2d-gpu.ts:400-405: the lines to these heights, the flow remapped. */
+  moveGrid(lineGrid, heights) {
+    this.exclusive.assertIdle();
+    lineGrid.checkSolver(this);
+    call(() => native().moveGrid(this.handle, Float64Array.from(heights)));
+  }
+
+  /** euler2d-gpu.ts:481-505. */
+  async iterateFitted(lineGrid, moves, stepsPerMove) {
+    if (this.config.fittedShock === undefined) throw new Error('there is no fitted shock');
+    if (!(Number.isInteger(stepsPerMove) && stepsPerMove >= 1 && stepsPerMove <= 256)) {
+      throw new RangeError('steps per move must be an integer from 1 to 256');
+    }
+    lineGrid.checkSolver(this);
+    await this.exclusive.run(() => native().iterateFitted(this.handle, moves, stepsPerMove));
+  }
+
+  /** euler2d-gpu.ts:630-656 (Float64Arrays here). */
+  async readShock() {
+    if (this.config.fittedShock === undefined) throw new Error('there is no fitted shock');
+    return this.exclusive.run(async () => native().readShock(this.handle, this.config.grid.ni));
   }
 
   /** euler2d-gpu.ts:659. */
@@ -446,8 +473,62 @@ function forebodyPressureDrag(wall, freeStream, baseRadius) {
   return (2 * force) / (freeStream.dynamicPressure * baseRadius * baseRadius);
 }
 
+/**
+ * GpuLineGrid (src/engine-gpu/line-grid.ts): the grid of a HIP solver on
+ * straight lines with fixed fractions (withFixedFractions), its metrics
+ * recomputed on the GPU for new heights. Made on the solver's
+ * geometryBuffer; the moves themselves run in the solver's moveGrid and
+ * iterateFitted.
+ */
+class GpuLineGrid {
+  static async create(device, lines, nj, options) {
+    const target = options && options.target;
+    if (!target || !(target.solver instanceof GpuEuler2D)) {
+      throw new RangeError("the target must be a HIP GpuEuler2D's geometryBuffer");
+    }
+    return new GpuLineGrid(target.solver, lines, nj, options.firstNode === 'symmetry');
+  }
+
+  constructor(solver, lines, nj, symmetry) {
+    const ni = lines.length - 1;
+    if (ni !== solver.config.grid.ni || nj !== solver.config.grid.nj) {
+      throw new RangeError(`the lines make a ${ni} x ${nj} grid, the solver's is another`);
+    }
+    const packed = new Float64Array(4 * (ni + 1));
+    const fractions = new Float64Array((ni + 1) * (nj + 1));
+    lines.forEach((l, i) => {
+      if (!l.fractions || l.fractions.length !== nj + 1) {
+        throw new RangeError(`line ${i} needs ${nj + 1} fixed fractions (withFixedFractions)`);
+      }
+      packed.set([l.x, l.y, l.dx, l.dy], 4 * i);
+      for (let j = 0; j <= nj; j++) fractions[i + j * (ni + 1)] = l.fractions[j];
+    });
+    this.solver = solver;
+    this.ni = ni;
+    call(() => native().setLineGrid(solver.handle, packed, fractions, symmetry ? 1 : 0));
+  }
+
+  checkSolver(solver) {
+    if (solver !== this.solver) throw new RangeError('the line grid was made on another solver');
+  }
+
+  /** line-grid.ts:195-205: the metrics for these heights. */
+  setHeights(heights) {
+    this.solver.exclusive.assertIdle();
+    call(() => native().setHeights(this.solver.handle, Float64Array.from(heights)));
+  }
+
+  /** line-grid.ts:257-270. */
+  async readHeights() {
+    return Array.from(native().readHeights(this.solver.handle, this.ni + 1));
+  }
+
+  destroy() {}
+}
+
 module.exports = {
   GpuEuler2D,
+  GpuLineGrid,
   HipDevice,
   ExclusiveRunner,
   NonPhysicalStateError,

@@ -21,7 +21,9 @@ async function test(name, fn) {
   await test('addon exports every binding', () => {
     const names = ['version', 'ctxCreate', 'ctxDestroy', 'solverCreate', 'solverDestroy', 'computeMetrics',
       'setPrimitives', 'setState', 'readState', 'setFaceStates', 'setFaceFluxes', 'residualNorm',
-      'residualVector', 'wallFluxes', 'forebodyDrag', 'setImplicitCfl', 'iterate', 'advanceTo', 'iterateImplicit'];
+      'residualVector', 'wallFluxes', 'forebodyDrag', 'setImplicitCfl', 'iterate', 'advanceTo', 'iterateImplicit',
+      'readState64', 'iterateFitted', 'setFittedShock', 'readShock', 'setLineGrid', 'setHeights', 'moveGrid',
+      'readHeights'];
     const a = bs.native();
     for (const n of names) assert.strictEqual(typeof a[n], 'function', n);
     assert.ok(/gfx950/.test(a.version()));
@@ -66,7 +68,12 @@ async function test(name, fn) {
     assert.throws(() => bs.nativeConfig(Object.assign({}, base, { flux: 'godunov' })), RangeError);
     assert.throws(() => bs.nativeConfig(Object.assign({}, base, { limiter: 'superbee' })), RangeError);
     assert.throws(() => bs.nativeConfig(Object.assign({}, base, { source: new Float64Array(16) })), RangeError);
-    assert.throws(() => bs.nativeConfig(Object.assign({}, base, { fittedShock: { freeStream: free } })), RangeError);
+    // A fitted shock is set on the solver after it is made, not in the plain data.
+    assert.strictEqual(bs.nativeConfig(Object.assign({}, base, { fittedShock: { freeStream: free } })).ni, 2);
+  });
+
+  await test('a line grid is made on a HIP solver only', async () => {
+    await assert.rejects(() => bs.GpuLineGrid.create(0, [], 2, { target: {} }), RangeError);
   });
 
   await test('metrics come from the library without a GPU', () => {

@@ -60,3 +60,33 @@ def test_node_solve_matches_oracle(tmp_path):
         orc2.iterate_implicit(1, cfl, 1.5, "redBlack", 2)
         cfl = min(10.0, cfl * 1.2)
     assert abs(out["implicitDrag"] / orc2.forebody_drag(p_inf, q_inf, 0.5) - 1) < 1e-5
+
+
+@pytest.mark.gpu
+def test_node_fitted_shock_matches_the_python_facade(tmp_path, ctx):
+    """The fitted shock and its moving grid through the Node facade
+    (GpuLineGrid, iterateFitted, readShock) on the FP64 solver: the same
+    heights and shock speeds as the Python facade, which the oracle's
+    fitShock loop pins (test_gpu_shock_fitting.py)."""
+    from test_gpu_shock_fitting import _fitted_pair, _line_cone
+
+    lines, eta, h0, _ = _line_cone()
+    case = dict(lines=lines.tolist(), fractions=eta.T.tolist(), heights=h0.tolist(), nj=eta.shape[0] - 1,
+                mach=2.0, moves=30)
+    path = tmp_path / "fitted.json"
+    path.write_text(json.dumps(case))
+    r = subprocess.run([NODE, os.path.join(ROOT, "host", "test", "gpu_fitted.js"), str(path)], capture_output=True,
+                       text=True, timeout=300)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    grid, free, make, gpu = _fitted_pair(ctx, lines, eta, h0, "fp64", 2.0)
+    gpu.setLineGrid(lines, eta, "fixed")
+    gpu.setHeights(h0)
+    gpu.setCellPrimitives(np.array(free))
+    gpu.setFittedShock(free)
+    gpu.iterateFitted(30, 1)
+    assert np.abs(np.array(out["heights"]) - gpu.readHeights()).max() < 1e-13
+    assert np.abs(np.array(out["speed"]) - gpu.readShock()["speed"]).max() < 1e-12
+    assert out["steps"] == 30
+    assert "fitted shock" in out["implicit"]
