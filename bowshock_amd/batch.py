@@ -32,8 +32,8 @@ STATUS_NON_PHYSICAL = 1
 class GpuBatch2D:
     """Many GpuEuler2D cases advanced together. Every case shares the
     scheme fields (geometry, gamma, flux, limiter, reconstruction,
-    integrator, cfl, local time stepping) and the boundary kinds per side;
-    grids and boundary states differ."""
+    integrator, cfl, local time stepping); grids, boundary kinds and
+    boundary states differ."""
 
     def __init__(self, ctx: Context, configs: Sequence[Euler2DConfig]):
         if len(configs) == 0:

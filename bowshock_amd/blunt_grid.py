@@ -377,8 +377,11 @@ class _Nose:
         and pi / 2 beyond the nose (blunt-grid.ts:423-428)."""
         s = np.asarray(s, dtype=np.float64)
         a = np.abs(s)
-        _, _, nx, ny = self.surface(a)
-        ang = np.where(a >= self.s_nose, math.pi / 2, np.arctan2(ny, -nx))
+        ang = np.full_like(a, math.pi / 2)
+        nose = a < self.s_nose
+        if nose.any():
+            _, _, nx, ny = self.surface(a[nose])
+            ang[nose] = np.arctan2(ny, -nx)
         return np.where(s < 0, -ang, ang)
 
     def fan(self, s):

@@ -497,12 +497,17 @@ bs_status bs_batch_create(bs_ctx* ctx, const bs_config* configs, int32_t count, 
       return fail(BS_ERR_RANGE, "case %d: the cases of a batch share one scheme", c);
     if (k.time_stepping != BS_STEPPING_LOCAL)
       return fail(BS_ERR_RANGE, "batched solves use local time steps");
+    // Boundary kinds may differ between cases (an H grid's inflow iMin, a
+    // C grid's axis): every kernel reads them from the case's own record.
     for (int sd = 0; sd < 4; sd++) {
+      if (k.boundaries[sd].kind < 0 || k.boundaries[sd].kind > 5)
+        return fail(BS_ERR_RANGE, "case %d: unknown boundary kind", c);
       if (k.boundaries[sd].kind == BS_BC_FACE_STATES)
         return fail(BS_ERR_RANGE, "case %d: faceStates boundaries are not batched", c);
-      if (k.boundaries[sd].kind != s0.boundaries[sd].kind)
-        return fail(BS_ERR_RANGE, "case %d: the cases of a batch share their boundary kinds", c);
     }
+    if ((k.boundaries[0].kind == BS_BC_PERIODIC) != (k.boundaries[1].kind == BS_BC_PERIODIC) ||
+        (k.boundaries[2].kind == BS_BC_PERIODIC) != (k.boundaries[3].kind == BS_BC_PERIODIC))
+      return fail(BS_ERR_RANGE, "case %d: periodic boundaries come in pairs", c);
     if (!(k.cfl > 0 && k.cfl <= 1)) return fail(BS_ERR_RANGE, "cfl must be in (0, 1]");
     if (k.ni < 2 || k.nj < 2 || !k.x || !k.y) return fail(BS_ERR_RANGE, "case %d: the grid needs 2 x 2 cells", c);
     if ((k.reconstruction == BS_RECON_SHARED || k.reconstruction == BS_RECON_SHARED_SCALED) &&

@@ -18,10 +18,14 @@ body-grid.ts:76-181): nj = ni / 2, and
 - population: L = f + cylinder, ni = 160 + 480 (L - 7) / 18 rounded to
   even (160 ... 640).
 
-A case is refused, with its reason, where the H grid cannot carry it:
-blunt tips (ellipsoid, Von Karman; the reference uses a C grid,
-src/grid/blunt-grid.ts), tip half-angles above 30 degrees
-(body-grid.ts:74), and Mach numbers at which the tip shock detaches.
+Blunt tips (ellipsoid, Von Karman) take the C grid of
+src/grid/blunt-grid.ts (bowshock_amd/blunt_grid.py), the axis ahead of
+the tip a wall: 20 cells across refined by the power of two nearest
+ni / 80 (2, 4 or 8, so nj = 40 ... 160; ni follows from the generator's
+density of lines). A case is refused, with its reason, where its grid
+cannot carry it: a C grid that fails the generator's quality gates, tip
+half-angles above 30 degrees (body-grid.ts:74), and Mach numbers at which
+an H grid's tip shock detaches.
 """
 from __future__ import annotations
 
@@ -29,7 +33,7 @@ import math
 import os
 from concurrent.futures import ProcessPoolExecutor
 from dataclasses import dataclass
-from typing import List, Optional, Sequence, Tuple
+from typing import List, Optional, Sequence
 
 import numpy as np
 
@@ -38,6 +42,8 @@ from . import grids
 SEED = 20261020
 GAMMA = 1.4
 NOSE_KINDS = ("conical", "tangent-ogive", "parabolic", "ellipsoid", "von-karman")
+BLUNT_KINDS = ("ellipsoid", "von-karman")
+C_GRID_BASE_NJ = 20
 SWEEP_MACHS = tuple(round(1.2 + 0.1 * k, 10) for k in range(39))   # 1.2 : 0.1 : 5.0
 POPULATION_MACHS = (1.5, 2.0, 3.0, 4.0)
 
@@ -110,6 +116,7 @@ class Case:
     refused: Optional[str] = None
     grid: Optional[grids.StructuredGrid] = None
     i_tip: int = 0
+    topology: str = "H"   # "C": the blunt-body grid, iMin the axis ahead of the tip
 
     @property
     def body(self) -> grids.Body:
@@ -149,29 +156,59 @@ def population_cases(n_rockets: int = 2000, machs: Sequence[float] = POPULATION_
     return out
 
 
-def _grid_job(args) -> Tuple[Optional[str], Optional[Tuple[np.ndarray, np.ndarray]], int]:
-    body, ni, nj, mach, tol = args
+def c_grid_refinement(ni: int) -> int:
+    """The power of two nearest ni / 80, within 1 ... 8."""
+    return int(2 ** min(3, max(0, round(math.log2(ni / 80)))))
+
+
+def boundaries(case: Case, free) -> dict:
+    """The case's boundary conditions: an H grid's iMin is the inflow
+    ahead of the tip, a C grid's the axis (a wall; blunt-body.test.ts:35)."""
+    inflow = {"kind": "inflow", "state": free}
+    return {"iMin": inflow if case.topology == "H" else {"kind": "wall"}, "iMax": {"kind": "outflow"},
+            "jMin": {"kind": "wall"}, "jMax": inflow}
+
+
+def _grid_job(args):
+    body, ni, nj, mach, tol, blunt = args
+    if blunt:
+        from .blunt_grid import blunt_body_grid
+
+        try:
+            bg = blunt_body_grid(body, C_GRID_BASE_NJ, mach, refinement=c_grid_refinement(ni))
+        except ValueError as e:   # GridQualityError included
+            return "C grid: " + str(e), None, 0, "C"
+        return None, (bg.grid.x, bg.grid.y, bg.grid.ni, bg.grid.nj), 0, "C"
     try:
         shock = grids.tip_shock_angle(body, mach, GAMMA, tol)
     except ValueError as e:
-        return str(e), None, 0
+        return str(e), None, 0, "H"
     bg = grids.body_grid_array(body, ni, nj, shock)
-    return None, (bg.grid.x, bg.grid.y), bg.i_tip
+    return None, (bg.grid.x, bg.grid.y, ni, nj), bg.i_tip, "H"
 
 
 def build_grids(cases: List[Case], workers: int = 0, tolerance: float = 1e-10) -> List[Case]:
     """Generate every case's grid (or its refusal) in a process pool; the
-    Taylor-Maccoll tolerance only moves the outer boundary's angle."""
-    jobs = [(c.body, c.ni, c.nj, c.mach, tolerance) for c in cases]
+    Taylor-Maccoll tolerance only moves the outer boundary's angle. Blunt
+    tips get C grids, whose size the generator sets: ni and nj change."""
+    jobs = [(c.body, c.ni, c.nj, c.mach, tolerance, c.rocket.kind in BLUNT_KINDS) for c in cases]
     workers = workers or min(16, os.cpu_count() or 1)
     if workers > 1 and len(jobs) > 8:
+        # The C grids take about a second each, the H grids milliseconds:
+        # the C jobs first, in small chunks, so the pool stays balanced.
+        order = sorted(range(len(jobs)), key=lambda k: not jobs[k][5])
         with ProcessPoolExecutor(workers) as ex:
-            results = list(ex.map(_grid_job, jobs, chunksize=8))
+            done = list(ex.map(_grid_job, [jobs[k] for k in order], chunksize=2))
+        results = [None] * len(jobs)
+        for k, r in zip(order, done):
+            results[k] = r
     else:
         results = [_grid_job(j) for j in jobs]
-    for c, (refusal, xy, i_tip) in zip(cases, results):
+    for c, (refusal, xy, i_tip, topology) in zip(cases, results):
         c.refused = refusal
+        c.topology = topology
         if xy is not None:
+            c.ni, c.nj = int(xy[2]), int(xy[3])
             c.grid = grids.StructuredGrid(c.ni, c.nj, xy[0], xy[1])
             c.i_tip = i_tip
     return cases
@@ -180,8 +217,10 @@ def build_grids(cases: List[Case], workers: int = 0, tolerance: float = 1e-10) -
 def refusal_counts(cases: Sequence[Case]) -> dict:
     counts: dict = {}
     for c in cases:
-        key = c.refused or "accepted"
-        if "blunt" in key:
+        key = c.refused or ("accepted" if c.topology == "H" else "accepted (C grid)")
+        if key.startswith("C grid"):
+            key = "C grid fails its quality gates" if "quality" in key else "C grid refused"
+        elif "blunt" in key:
             key = "blunt tip (needs a C grid)"
         elif "tip half-angle" in key:
             key = "tip half-angle above 30 deg"

@@ -309,8 +309,9 @@ bs_status bs_kernel_timing(bs_solver *s, int32_t kernel, double *ms_per_launch, 
  * points advance many cases of one scheme together, every stage one launch
  * over all active cases, with the same per-case kernels, status words and
  * FP64 reductions as the single-solve entry points above. Cases share the
- * scheme fields of bs_config and their boundary kinds (values may differ),
- * use local time steps, and have no faceStates sides.
+ * scheme fields of bs_config; their grids, boundary kinds and boundary
+ * states may differ (H grids with an inflow iMin beside C grids whose iMin
+ * is the axis). Cases use local time steps and have no faceStates sides.
  * ------------------------------------------------------------------------ */
 typedef struct bs_batch bs_batch;
 

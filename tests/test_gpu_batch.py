@@ -133,3 +133,55 @@ def test_batch_rejects_mixed_schemes(ctx):
     b = _cases(limiter="vanLeer")[1][0]
     with pytest.raises(RangeError):
         GpuBatch2D(ctx, [a, b])
+
+
+def _mixed_cases(**scheme):
+    """Two H grids (cones, inflow iMin) and two blunt-body C grids (the
+    axis ahead of the nose a wall) in one batch."""
+    from bowshock_amd import grids
+    from bowshock_amd.blunt_grid import blunt_body_grid
+
+    out = _cases(**scheme)[:2]
+    kw = dict(geometry="axisymmetric", gamma=cases.GAMMA, flux="hllc", limiter="minmod",
+              reconstruction="perVariable", timeIntegrator="ssprk3", cfl=0.5, timeStepping="local")
+    kw.update(scheme)
+    for body, mach in ((grids.Body(grids.Nose("ellipsoid", 0.5, 0.5), 4.0), 3.0),
+                       (grids.Body(grids.Nose("ellipsoid", 1.5, 0.5), 6.0), 2.0)):
+        grid = blunt_body_grid(body, 16, mach).grid
+        free, p_inf, q_inf = grids.free_stream_at(mach, cases.GAMMA)
+        bcs = {"iMin": {"kind": "wall"}, "iMax": {"kind": "outflow"}, "jMin": {"kind": "wall"},
+               "jMax": {"kind": "inflow", "state": free}}
+        out.append((Euler2DConfig(grid=grid, boundaries=bcs, **kw), free, p_inf, q_inf))
+    return out
+
+
+@pytest.mark.parametrize("implicit", [False, True])
+def test_mixed_topologies_match_single_solves(ctx, implicit):
+    """H and C grids, whose boundary kinds differ, in one batch: each case
+    bitwise equal to its single solve, drags to the reduction order."""
+    scheme = dict(limiter="vanAlbada", reconstruction="sharedScaled") if implicit else {}
+    cs = _mixed_cases(**scheme)
+    batch = GpuBatch2D(ctx, [c[0] for c in cs])
+    batch.setUniform([c[1] for c in cs])
+    singles = _singles(ctx, cs)
+    if implicit:
+        batch.setImplicitCfl(1.0)
+        for s in singles:
+            s.setImplicitCfl(1.0)
+    for count in (25, 16):
+        if implicit:
+            batch.iterateImplicit(count, 10.0, 2)
+        else:
+            batch.iterate(count)
+        for s in singles:
+            if implicit:
+                s.iterateImplicit(count, cfl=10.0, sweeps=2)
+            else:
+                s.iterate(count)
+        for k, s in enumerate(singles):
+            assert np.array_equal(_state(batch.readState(k)), _state(s.readState())), (count, k)
+    p = np.array([c[2] for c in cs])
+    q = np.array([c[3] for c in cs])
+    drags = batch.forebodyDrags(p, q, 0.5)
+    for k, s in enumerate(singles):
+        assert drags[k] == pytest.approx(s.forebodyDrag(p[k], q[k], 0.5), rel=1e-13, abs=1e-15)

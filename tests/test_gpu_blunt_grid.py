@@ -8,9 +8,10 @@ src/analysis/fitted-flow.ts bluntFitFamily).
 Bars: the FP64 solver converges to round-off (density residual 1e-13 of
 its start, as the reference's CPU engine does) onto the oracle's converged
 field within 1e-10, and stands the shock off within 10 % of Billig's
-correlation and within 1e-4 of the 0.1078 the reference recorded; the FP32 solver follows the oracle's transient within its
-bar (L1 5e-6, Linf 2e-4); fitted moves as in the cone and sphere cases
-(FP64 heights 1e-11, fields L1 1e-11, Linf 1e-9).
+correlation and within 1e-4 of the 0.1078 the reference recorded; the
+FP32 solver follows the oracle's transient within its bar (L1 5e-6, Linf
+2e-4); fitted moves as in the cone and sphere cases (FP64 heights 1e-11,
+fields L1 1e-11, Linf 1e-9).
 """
 import math
 

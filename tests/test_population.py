@@ -38,14 +38,22 @@ def test_population_sizes_follow_the_rule():
 
 def test_refusals():
     blunt = P.Case(P.Rocket(0, "ellipsoid", 2.0, 50, 8.0), 3.0, 160, 80, 8.0)
+    gated = P.Case(P.Rocket(4, "von-karman", 4.85, 50, 6.4), 2.0, 274, 137, 6.4)
     steep = P.Case(P.Rocket(1, "parabolic", 1.2, 50, 8.0), 3.0, 160, 80, 8.0)
     detached = P.Case(P.Rocket(2, "tangent-ogive", 1.95, 50, 8.0), 1.2, 160, 80, 8.0)
     fine = P.Case(P.Rocket(3, "conical", 3.0, 50, 8.0), 2.0, 40, 20, 8.0)
-    P.build_grids([blunt, steep, detached, fine], workers=1)
-    counts = P.refusal_counts([blunt, steep, detached, fine])
-    assert counts == {"blunt tip (needs a C grid)": 1, "tip half-angle above 30 deg": 1,
+    all_ = [blunt, gated, steep, detached, fine]
+    P.build_grids(all_, workers=1)
+    counts = P.refusal_counts(all_)
+    assert counts == {"accepted (C grid)": 1, "C grid fails its quality gates": 1, "tip half-angle above 30 deg": 1,
                       "detached tip shock": 1, "accepted": 1}
-    assert fine.grid is not None and fine.grid.ni == 40 and fine.grid.nj == 20
+    assert fine.grid is not None and fine.grid.ni == 40 and fine.grid.nj == 20 and fine.topology == "H"
+    # The blunt tip's C grid: 20 cells across refined by 2 (ni 160 / 80).
+    assert blunt.topology == "C" and blunt.nj == 40 and blunt.grid.nj == 40 and blunt.grid.ni == blunt.ni
+    free = grids.free_stream_at(3.0, 1.4)[0]
+    assert P.boundaries(blunt, free)["iMin"] == {"kind": "wall"}
+    assert P.boundaries(fine, free)["iMin"]["kind"] == "inflow"
+    assert [P.c_grid_refinement(n) for n in (160, 226, 240, 320, 640)] == [2, 2, 4, 4, 8]
 
 
 @pytest.mark.parametrize("body,mach", [(grids.ogive_cylinder(3.0, 15.0), 2.0), (grids.cone_cylinder(15, 4), 3.0),
