@@ -509,6 +509,16 @@ __global__ __launch_bounds__(TX * TY) void tile_kernel(TileArgsT<T> A) {
   tile_body<T, TX, TY, REC, LIM, FLUX, MODE>(A, t % tiles_x, t / tiles_x, t);
 }
 
+// The FP32 stage tiles held to 64 VGPRs: eight waves per SIMD, so four
+// 512-thread tiles fit on a CU (their LDS allows four) instead of three.
+template <int TX, int TY, int REC, int LIM, int FLUX, int MODE>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(8, 8))) void tile_kernel_occ8(
+    TileArgsT<float> A) {
+  const int tiles_x = (A.G.ni + TX - 1) / TX;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  tile_body<float, TX, TY, REC, LIM, FLUX, MODE>(A, t % tiles_x, t / tiles_x, t);
+}
+
 // A batch of grids in one launch: block b takes work item b (the case's
 // argument record, the tile and the partial slot); the argument records
 // live in device memory, one per case and stage role.
