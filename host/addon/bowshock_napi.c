@@ -130,10 +130,55 @@ static napi_value js_ctx_destroy(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
-/* solverCreate(ctx, cfg[, precision]): cfg = {ni, nj, x: Float64Array,
- * y: Float64Array, geometry, gamma, flux, limiter, reconstruction,
- * integrator, cfl, timeStepping, boundaries: [{kind, rho, u, v, p} x 4]}
- * with integer codes; precision BS_PRECISION_FP32 (default) or _FP64. */
+/* The plain-data configuration {ni, nj, x: Float64Array, y: Float64Array,
+ * geometry, gamma, flux, limiter, reconstruction, integrator, cfl,
+ * timeStepping, boundaries: [{kind, rho, u, v, p} x 4]} with integer codes
+ * as bs_config; x and y stay owned by the JS values. 0 (thrown) on a bad
+ * grid. */
+static int parse_config(napi_env env, napi_value c, bs_config *cfg) {
+  napi_value v;
+  memset(cfg, 0, sizeof *cfg);
+#define FIELD(name) (napi_get_named_property(env, c, name, &v), v)
+  cfg->ni = i32(env, FIELD("ni"));
+  cfg->nj = i32(env, FIELD("nj"));
+  size_t nx = 0, ny = 0;
+  cfg->x = (const double *)typed(env, FIELD("x"), napi_float64_array, &nx);
+  cfg->y = (const double *)typed(env, FIELD("y"), napi_float64_array, &ny);
+  size_t nodes = (size_t)(cfg->ni + 1) * (size_t)(cfg->nj + 1);
+  if (!cfg->x || !cfg->y || nx != nodes || ny != nodes) {
+    napi_throw(env, make_error(env, BS_ERR_RANGE, "x and y must be Float64Arrays of (ni + 1) (nj + 1) nodes"));
+    return 0;
+  }
+  cfg->geometry = i32(env, FIELD("geometry"));
+  cfg->gamma = num(env, FIELD("gamma"));
+  cfg->flux = i32(env, FIELD("flux"));
+  cfg->limiter = i32(env, FIELD("limiter"));
+  cfg->reconstruction = i32(env, FIELD("reconstruction"));
+  cfg->integrator = i32(env, FIELD("integrator"));
+  cfg->cfl = num(env, FIELD("cfl"));
+  cfg->time_stepping = i32(env, FIELD("timeStepping"));
+  napi_value bcs = FIELD("boundaries");
+  for (uint32_t k = 0; k < 4; k++) {
+    napi_value b;
+    napi_get_element(env, bcs, k, &b);
+    napi_value f;
+    napi_get_named_property(env, b, "kind", &f);
+    cfg->boundaries[k].kind = i32(env, f);
+    napi_get_named_property(env, b, "rho", &f);
+    cfg->boundaries[k].rho = num(env, f);
+    napi_get_named_property(env, b, "u", &f);
+    cfg->boundaries[k].u = num(env, f);
+    napi_get_named_property(env, b, "v", &f);
+    cfg->boundaries[k].v = num(env, f);
+    napi_get_named_property(env, b, "p", &f);
+    cfg->boundaries[k].p = num(env, f);
+  }
+#undef FIELD
+  return 1;
+}
+
+/* solverCreate(ctx, cfg[, precision]): precision BS_PRECISION_FP32
+ * (default) or _FP64. */
 static napi_value js_solver_create(napi_env env, napi_callback_info info) {
   napi_value argv[3];
   size_t argc = 3;
@@ -143,45 +188,8 @@ static napi_value js_solver_create(napi_env env, napi_callback_info info) {
   }
   const int precision = argc > 2 ? i32(env, argv[2]) : BS_PRECISION_FP32;
   bs_ctx *ctx = (bs_ctx *)get_ptr(env, argv[0]);
-  napi_value c = argv[1], v;
   bs_config cfg;
-  memset(&cfg, 0, sizeof cfg);
-#define FIELD(name) (napi_get_named_property(env, c, name, &v), v)
-  cfg.ni = i32(env, FIELD("ni"));
-  cfg.nj = i32(env, FIELD("nj"));
-  size_t nx = 0, ny = 0;
-  cfg.x = (const double *)typed(env, FIELD("x"), napi_float64_array, &nx);
-  cfg.y = (const double *)typed(env, FIELD("y"), napi_float64_array, &ny);
-  size_t nodes = (size_t)(cfg.ni + 1) * (size_t)(cfg.nj + 1);
-  if (!cfg.x || !cfg.y || nx != nodes || ny != nodes) {
-    napi_throw(env, make_error(env, BS_ERR_RANGE, "x and y must be Float64Arrays of (ni + 1) (nj + 1) nodes"));
-    return NULL;
-  }
-  cfg.geometry = i32(env, FIELD("geometry"));
-  cfg.gamma = num(env, FIELD("gamma"));
-  cfg.flux = i32(env, FIELD("flux"));
-  cfg.limiter = i32(env, FIELD("limiter"));
-  cfg.reconstruction = i32(env, FIELD("reconstruction"));
-  cfg.integrator = i32(env, FIELD("integrator"));
-  cfg.cfl = num(env, FIELD("cfl"));
-  cfg.time_stepping = i32(env, FIELD("timeStepping"));
-  napi_value bcs = FIELD("boundaries");
-  for (uint32_t k = 0; k < 4; k++) {
-    napi_value b;
-    napi_get_element(env, bcs, k, &b);
-    napi_value f;
-    napi_get_named_property(env, b, "kind", &f);
-    cfg.boundaries[k].kind = i32(env, f);
-    napi_get_named_property(env, b, "rho", &f);
-    cfg.boundaries[k].rho = num(env, f);
-    napi_get_named_property(env, b, "u", &f);
-    cfg.boundaries[k].u = num(env, f);
-    napi_get_named_property(env, b, "v", &f);
-    cfg.boundaries[k].v = num(env, f);
-    napi_get_named_property(env, b, "p", &f);
-    cfg.boundaries[k].p = num(env, f);
-  }
-#undef FIELD
+  if (!parse_config(env, argv[1], &cfg)) return NULL;
   bs_solver *s = NULL;
   bs_status st = precision == BS_PRECISION_FP64 ? bs_solver_create_fp64(ctx, &cfg, &s) : bs_solver_create(ctx, &cfg, &s);
   if (st) return throw_status(env, st);
@@ -559,6 +567,253 @@ static napi_value js_read_heights(napi_env env, napi_callback_info info) {
   return arr;
 }
 
+/* ---- batches (bs_batch_*) ------------------------------------------- */
+
+/* batchCreate(ctx, [cfg, ...]): the configurations as for solverCreate. */
+static napi_value js_batch_create(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  bs_ctx *ctx = (bs_ctx *)get_ptr(env, argv[0]);
+  uint32_t count = 0;
+  napi_get_array_length(env, argv[1], &count);
+  if (count == 0) {
+    napi_throw(env, make_error(env, BS_ERR_RANGE, "a batch needs at least one case"));
+    return NULL;
+  }
+  bs_config *cfgs = (bs_config *)calloc(count, sizeof(bs_config));
+  if (!cfgs) {
+    napi_throw(env, make_error(env, BS_ERR_MEMORY, "out of host memory"));
+    return NULL;
+  }
+  for (uint32_t k = 0; k < count; k++) {
+    napi_value c;
+    napi_get_element(env, argv[1], k, &c);
+    if (!parse_config(env, c, &cfgs[k])) {
+      free(cfgs);
+      return NULL;
+    }
+  }
+  bs_batch *b = NULL;
+  bs_status st = bs_batch_create(ctx, cfgs, (int32_t)count, &b);
+  free(cfgs);
+  if (st) return throw_status(env, st);
+  napi_value ext;
+  CHECK_NAPI(napi_create_external(env, b, NULL, NULL, &ext));
+  return ext;
+}
+
+static napi_value js_batch_destroy(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  bs_batch_destroy((bs_batch *)get_ptr(env, argv[0]));
+  return NULL;
+}
+
+/* batchSetUniform(b, Float64Array 4 per case). */
+static napi_value js_batch_set_uniform(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  const double *w = (const double *)typed(env, argv[1], napi_float64_array, NULL);
+  if (!w) return throw_status(env, BS_ERR_RANGE);
+  bs_status st = bs_batch_set_uniform((bs_batch *)get_ptr(env, argv[0]), w);
+  if (st) return throw_status(env, st);
+  return NULL;
+}
+
+/* batchSetPrimitives(b, index, Float64Array 4 per cell). */
+static napi_value js_batch_set_primitives(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return NULL;
+  const double *w = (const double *)typed(env, argv[2], napi_float64_array, NULL);
+  if (!w) return throw_status(env, BS_ERR_RANGE);
+  bs_status st = bs_batch_set_primitives((bs_batch *)get_ptr(env, argv[0]), i32(env, argv[1]), w);
+  if (st) return throw_status(env, st);
+  return NULL;
+}
+
+/* batchReadState(b, index, cells) -> {rho, mu, mv, energy} Float32Arrays. */
+static napi_value js_batch_read_state(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return NULL;
+  const int32_t n = i32(env, argv[2]);
+  float *q[4];
+  napi_value arr[4], o;
+  for (int k = 0; k < 4; k++) arr[k] = new_f32(env, (size_t)(n > 0 ? n : 0), &q[k]);
+  bs_status st = bs_batch_read_state((bs_batch *)get_ptr(env, argv[0]), i32(env, argv[1]), q[0], q[1], q[2], q[3]);
+  if (st) return throw_status(env, st);
+  napi_create_object(env, &o);
+  napi_set_named_property(env, o, "rho", arr[0]);
+  napi_set_named_property(env, o, "mu", arr[1]);
+  napi_set_named_property(env, o, "mv", arr[2]);
+  napi_set_named_property(env, o, "energy", arr[3]);
+  return o;
+}
+
+/* batchSetActive(b, Uint8Array one per case). */
+static napi_value js_batch_set_active(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  const uint8_t *a = (const uint8_t *)typed(env, argv[1], napi_uint8_array, NULL);
+  if (!a) return throw_status(env, BS_ERR_RANGE);
+  bs_status st = bs_batch_set_active((bs_batch *)get_ptr(env, argv[0]), a);
+  if (st) return throw_status(env, st);
+  return NULL;
+}
+
+/* batchSetInt(b, what, value): 0 tiling threshold, 1 relaxation form. */
+static napi_value js_batch_set_int(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return NULL;
+  bs_batch *b = (bs_batch *)get_ptr(env, argv[0]);
+  const int32_t v = i32(env, argv[2]);
+  bs_status st = i32(env, argv[1]) == 0 ? bs_batch_set_tiling(b, v) : bs_batch_set_relaxation(b, v);
+  if (st) return throw_status(env, st);
+  return NULL;
+}
+
+/* batchSetImplicitCfl(b, number | Float64Array one per case). */
+static napi_value js_batch_set_implicit_cfl(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  bs_batch *b = (bs_batch *)get_ptr(env, argv[0]);
+  const double *per = (const double *)typed(env, argv[1], napi_float64_array, NULL);
+  bs_status st = per ? bs_batch_set_implicit_cfls(b, per) : bs_batch_set_implicit_cfl(b, num(env, argv[1]));
+  if (st) return throw_status(env, st);
+  return NULL;
+}
+
+typedef struct {
+  napi_async_work work;
+  napi_deferred deferred;
+  bs_batch *b;
+  int implicit;
+  int32_t steps, sweeps;
+  double cfl, omega, growth;
+  bs_status st;
+  char message[512];
+} BatchJob;
+
+static void batch_execute(napi_env env, void *data) {
+  (void)env;
+  BatchJob *j = (BatchJob *)data;
+  j->st = j->implicit ? bs_batch_iterate_implicit(j->b, j->steps, j->cfl, j->sweeps, j->omega, j->growth)
+                      : bs_batch_iterate(j->b, j->steps);
+  strncpy(j->message, bs_last_error(), sizeof j->message - 1);
+  j->message[sizeof j->message - 1] = 0;
+}
+
+static void batch_complete(napi_env env, napi_status status, void *data) {
+  BatchJob *j = (BatchJob *)data;
+  if (status != napi_ok) {
+    napi_reject_deferred(env, j->deferred, make_error(env, BS_ERR_EXECUTION, "the batch job was cancelled"));
+  } else if (j->st != BS_OK) {
+    napi_reject_deferred(env, j->deferred, make_error(env, j->st, j->message));
+  } else {
+    napi_value undefined;
+    napi_get_undefined(env, &undefined);
+    napi_resolve_deferred(env, j->deferred, undefined);
+  }
+  napi_delete_async_work(env, j->work);
+  free(j);
+}
+
+/* batchIterate(b, steps) and batchIterateImplicit(b, steps, cfl, sweeps,
+ * omega, growth): Promises, on a worker thread. */
+static napi_value batch_queue(napi_env env, napi_callback_info info, int implicit) {
+  napi_value argv[6];
+  if (!get_args(env, info, implicit ? 6 : 2, argv)) return NULL;
+  BatchJob *j = (BatchJob *)calloc(1, sizeof(BatchJob));
+  if (!j) {
+    napi_throw(env, make_error(env, BS_ERR_MEMORY, "out of host memory"));
+    return NULL;
+  }
+  j->b = (bs_batch *)get_ptr(env, argv[0]);
+  j->implicit = implicit;
+  j->steps = i32(env, argv[1]);
+  if (implicit) {
+    j->cfl = num(env, argv[2]);
+    j->sweeps = i32(env, argv[3]);
+    j->omega = num(env, argv[4]);
+    j->growth = num(env, argv[5]);
+  }
+  napi_value promise, name;
+  napi_create_promise(env, &j->deferred, &promise);
+  napi_create_string_utf8(env, "bowshock-batch", NAPI_AUTO_LENGTH, &name);
+  napi_create_async_work(env, NULL, name, batch_execute, batch_complete, j, &j->work);
+  napi_queue_async_work(env, j->work);
+  return promise;
+}
+
+static napi_value js_batch_iterate(napi_env env, napi_callback_info info) { return batch_queue(env, info, 0); }
+static napi_value js_batch_iterate_implicit(napi_env env, napi_callback_info info) {
+  return batch_queue(env, info, 1);
+}
+
+/* batchResidualNorms(b, count) -> Float64Array. */
+static napi_value js_batch_residual_norms(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  const int32_t n = i32(env, argv[1]);
+  double *r;
+  napi_value arr = new_f64(env, (size_t)(n > 0 ? n : 0), &r);
+  bs_status st = bs_batch_residual_norms((bs_batch *)get_ptr(env, argv[0]), r);
+  if (st) return throw_status(env, st);
+  return arr;
+}
+
+/* batchDrags(b, pInf, qInf, baseRadius) (Float64Arrays, one per case) ->
+ * Float64Array. */
+static napi_value js_batch_drags(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  if (!get_args(env, info, 4, argv)) return NULL;
+  size_t n = 0, n2 = 0, n3 = 0;
+  const double *p = (const double *)typed(env, argv[1], napi_float64_array, &n);
+  const double *q = (const double *)typed(env, argv[2], napi_float64_array, &n2);
+  const double *R = (const double *)typed(env, argv[3], napi_float64_array, &n3);
+  if (!p || !q || !R || n2 != n || n3 != n) return throw_status(env, BS_ERR_RANGE);
+  double *cd;
+  napi_value arr = new_f64(env, n, &cd);
+  bs_status st = bs_batch_drags((bs_batch *)get_ptr(env, argv[0]), p, q, R, cd);
+  if (st) return throw_status(env, st);
+  return arr;
+}
+
+/* batchStatus(b, count) -> {status: Uint32Array, steps: Float64Array}. */
+static napi_value js_batch_status(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  const int32_t n = i32(env, argv[1]);
+  const size_t m = (size_t)(n > 0 ? n : 0);
+  uint32_t *status = (uint32_t *)calloc(m ? m : 1, sizeof(uint32_t));
+  int64_t *steps = (int64_t *)calloc(m ? m : 1, sizeof(int64_t));
+  if (!status || !steps) {
+    free(status);
+    free(steps);
+    napi_throw(env, make_error(env, BS_ERR_MEMORY, "out of host memory"));
+    return NULL;
+  }
+  bs_status st = bs_batch_status((bs_batch *)get_ptr(env, argv[0]), status, steps);
+  if (st) {
+    free(status);
+    free(steps);
+    return throw_status(env, st);
+  }
+  napi_value o, sbuf, sarr, steps_arr;
+  void *sdata = NULL;
+  napi_create_arraybuffer(env, m * sizeof(uint32_t), &sdata, &sbuf);
+  memcpy(sdata, status, m * sizeof(uint32_t));
+  napi_create_typedarray(env, napi_uint32_array, m, sbuf, 0, &sarr);
+  double *sd;
+  steps_arr = new_f64(env, m, &sd);
+  for (size_t k = 0; k < m; k++) sd[k] = (double)steps[k];
+  free(status);
+  free(steps);
+  napi_create_object(env, &o);
+  napi_set_named_property(env, o, "status", sarr);
+  napi_set_named_property(env, o, "steps", steps_arr);
+  return o;
+}
+
 static napi_value init(napi_env env, napi_value exports) {
   napi_property_descriptor props[] = {
       {"version", NULL, js_version, NULL, NULL, NULL, napi_enumerable, NULL},
@@ -588,6 +843,19 @@ static napi_value init(napi_env env, napi_value exports) {
       {"setHeights", NULL, js_set_heights, NULL, NULL, NULL, napi_enumerable, NULL},
       {"moveGrid", NULL, js_move_grid, NULL, NULL, NULL, napi_enumerable, NULL},
       {"readHeights", NULL, js_read_heights, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"batchCreate", NULL, js_batch_create, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"batchDestroy", NULL, js_batch_destroy, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"batchSetUniform", NULL, js_batch_set_uniform, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"batchSetPrimitives", NULL, js_batch_set_primitives, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"batchReadState", NULL, js_batch_read_state, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"batchSetActive", NULL, js_batch_set_active, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"batchSetInt", NULL, js_batch_set_int, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"batchSetImplicitCfl", NULL, js_batch_set_implicit_cfl, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"batchIterate", NULL, js_batch_iterate, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"batchIterateImplicit", NULL, js_batch_iterate_implicit, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"batchResidualNorms", NULL, js_batch_residual_norms, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"batchDrags", NULL, js_batch_drags, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"batchStatus", NULL, js_batch_status, NULL, NULL, NULL, napi_enumerable, NULL},
   };
   napi_define_properties(env, exports, sizeof props / sizeof props[0], props);
   return exports;

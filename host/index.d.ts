@@ -167,6 +167,69 @@ export class GpuEuler2D {
   destroy(): void;
 }
 
+/**
+ * Batched solves (include/bowshock.h bs_batch_*): cases of one scheme in
+ * persistent launches; grids, boundary kinds and states differ. No
+ * faceStates sides; local time steps.
+ */
+export class GpuBatch2D {
+  static create(device: HipDevice | number, configs: readonly GpuEuler2DConfig[]): Promise<GpuBatch2D>;
+  readonly count: number;
+  readonly configs: readonly GpuEuler2DConfig[];
+  setUniform(states: Primitive2D | readonly Primitive2D[]): void;
+  setCellPrimitives(index: number, cell: (i: number, j: number) => Primitive2D): void;
+  readState(index: number): Promise<{ rho: Float32Array; mu: Float32Array; mv: Float32Array; energy: Float32Array }>;
+  setActive(active: readonly boolean[]): void;
+  setTiling(smallTileBelowCells: number): void;
+  setRelaxation(form: 'fluxDifference' | 'linearized'): void;
+  iterate(steps: number): Promise<void>;
+  setImplicitCfl(cfl: number | readonly number[]): Promise<void>;
+  iterateImplicit(
+    steps: number,
+    options: { readonly cfl: number; readonly sweeps: number; readonly omega?: number; readonly growth?: number },
+  ): Promise<void>;
+  densityResidualNorms(): Promise<Float64Array>;
+  forebodyDrags(
+    freeStreams: FreeStream | readonly FreeStream[],
+    baseRadius: number | readonly number[],
+  ): Promise<Float64Array>;
+  status(): Promise<{ status: Uint32Array; steps: Float64Array }>;
+  destroy(): void;
+}
+
+export interface BatchPlan {
+  readonly implicit?: boolean;
+  readonly chunk?: number;
+  readonly maxSteps?: number;
+  readonly residual?: number;
+  readonly drop?: number;
+  readonly cdTol?: number;
+  readonly settle?: number;
+  readonly stall?: number;
+  readonly cfl?: number;
+  readonly cflStart?: number;
+  readonly sweeps?: number;
+  readonly omega?: number;
+  readonly growth?: number;
+  readonly relaxation?: 'fluxDifference' | 'linearized';
+}
+
+export interface BatchResult {
+  readonly cd: number;
+  readonly residual: number;
+  readonly steps: number;
+  readonly converged: boolean;
+  readonly reason: 'converged' | 'converged (residual floor)' | 'step limit' | 'non-physical state';
+}
+
+/** Every case to its stopping rule; finished cases leave the launches. */
+export function solveBatch(
+  batch: GpuBatch2D,
+  freeStreams: FreeStream | readonly FreeStream[],
+  baseRadius: number | readonly number[],
+  plan?: BatchPlan,
+): Promise<BatchResult[]>;
+
 export function computeMetrics(grid: StructuredGrid, geometry: Geometry): GridMetrics;
 export function gpuPrimitive2D(state: GpuState2D, c: number, gamma: number): Primitive2D;
 export function freeStreamAt(mach: number, gamma: number): FreeStream;

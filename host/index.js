@@ -526,8 +526,207 @@ class GpuLineGrid {
   destroy() {}
 }
 
+/**
+ * Many GpuEuler2D cases advanced together in persistent launches
+ * (include/bowshock.h bs_batch_*): the cases share the scheme fields of
+ * their configs; grids, boundary kinds and states differ. The reference
+ * solves its grids one after another (verified-drag.ts:239-300); this is
+ * the batched counterpart of that loop's engine.
+ */
+class GpuBatch2D {
+  static async create(device, configs) {
+    const dev = device instanceof HipDevice ? device : new HipDevice(device);
+    return new GpuBatch2D(dev, configs);
+  }
+
+  constructor(device, configs) {
+    if (!configs || configs.length === 0) throw new RangeError('a batch needs at least one case');
+    for (const c of configs) {
+      for (const side of SIDES) {
+        if (c.boundaries[side].kind === 'faceStates') throw new RangeError('faceStates boundaries are not batched');
+      }
+    }
+    this.device = device;
+    this.configs = configs;
+    this.count = configs.length;
+    this.cells = configs.map((c) => c.grid.ni * c.grid.nj);
+    this.exclusive = new ExclusiveRunner('GpuBatch2D');
+    this.handle = call(() => native().batchCreate(device.handle, configs.map(nativeConfig)));
+  }
+
+  /** One uniform primitive state per case (or one for all). */
+  setUniform(states) {
+    this.exclusive.assertIdle();
+    const list = Array.isArray(states) ? states : Array(this.count).fill(states);
+    if (list.length !== this.count) throw new RangeError('one state per case');
+    call(() => native().batchSetUniform(this.handle, packPrimitives(this.count, (k) => list[k])));
+  }
+
+  /** setCellPrimitives of case k: cell(i, j) for every cell. */
+  setCellPrimitives(k, cell) {
+    this.exclusive.assertIdle();
+    const ni = this.configs[k].grid.ni;
+    const packed = packPrimitives(this.cells[k], (c) => cell(c % ni, Math.floor(c / ni)));
+    call(() => native().batchSetPrimitives(this.handle, k, packed));
+  }
+
+  /** GpuState2D of case k (Float32Arrays). */
+  async readState(k) {
+    return this.exclusive.run(async () => native().batchReadState(this.handle, k, this.cells[k]));
+  }
+
+  /** Cases with active[k] false are skipped by later steps and reductions. */
+  setActive(active) {
+    this.exclusive.assertIdle();
+    call(() => native().batchSetActive(this.handle, Uint8Array.from(active, (a) => (a ? 1 : 0))));
+  }
+
+  /** Cases of fewer cells than this run on the small tile. */
+  setTiling(smallTileBelowCells) {
+    call(() => native().batchSetInt(this.handle, 0, smallTileBelowCells));
+  }
+
+  /** 'fluxDifference' (default) or 'linearized' neighbour terms. */
+  setRelaxation(form) {
+    const code = { fluxDifference: 0, linearized: 1 }[form];
+    if (code === undefined) throw new RangeError(`unknown relaxation ${form}`);
+    call(() => native().batchSetInt(this.handle, 1, code));
+  }
+
+  async iterate(steps) {
+    await this.exclusive.run(() => native().batchIterate(this.handle, steps));
+  }
+
+  /** One CFL number for all cases, or one per case. */
+  async setImplicitCfl(cfl) {
+    const arg = typeof cfl === 'number' ? cfl : Float64Array.from(cfl);
+    await this.exclusive.run(async () => native().batchSetImplicitCfl(this.handle, arg));
+  }
+
+  async iterateImplicit(steps, options) {
+    const { cfl, sweeps } = options;
+    const omega = options.omega === undefined ? 1.5 : options.omega;
+    const growth = options.growth === undefined ? 1.2 : options.growth;
+    if (!(cfl > 0 && omega >= 1 && growth >= 1 && Number.isInteger(sweeps) && sweeps >= 1)) {
+      throw new RangeError('implicit steps need cfl > 0, omega >= 1, growth >= 1 and whole sweeps');
+    }
+    await this.exclusive.run(() => native().batchIterateImplicit(this.handle, steps, cfl, sweeps, omega, growth));
+  }
+
+  /** densityResidualNorm per case, FP64 on the GPU (NaN when inactive). */
+  async densityResidualNorms() {
+    return this.exclusive.run(async () => native().batchResidualNorms(this.handle, this.count));
+  }
+
+  /** forebodyPressureDrag per case over jMin, FP64 on the GPU. */
+  async forebodyDrags(freeStreams, baseRadius) {
+    const list = Array.isArray(freeStreams) ? freeStreams : Array(this.count).fill(freeStreams);
+    const R = Array.isArray(baseRadius) ? baseRadius : Array(this.count).fill(baseRadius);
+    return this.exclusive.run(async () =>
+      native().batchDrags(this.handle, Float64Array.from(list, (f) => f.pressure),
+        Float64Array.from(list, (f) => f.dynamicPressure), Float64Array.from(R)),
+    );
+  }
+
+  /** Status word (bit 0: non-physical) and step count per case. */
+  async status() {
+    return this.exclusive.run(async () => native().batchStatus(this.handle, this.count));
+  }
+
+  destroy() {
+    if (this.handle !== null) native().batchDestroy(this.handle);
+    this.handle = null;
+  }
+}
+
+/** The stopping rule of bowshock_amd/batch.py SolvePlan (same defaults). */
+const DEFAULT_PLAN = {
+  implicit: false, chunk: 120, maxSteps: 20000, residual: 1e-6, drop: 1e-4, cdTol: 1e-5, settle: 2, stall: 4,
+  cfl: 10, cflStart: 1, sweeps: 2, omega: 1.5, growth: 1.2, relaxation: 'fluxDifference',
+};
+
+/**
+ * Run every case of a batch until it converges or fails (the reference's
+ * drivers stop on a residual drop or a settled drag, verified-drag.ts:
+ * 258-300, transonic-flow.ts): a case converges when its density residual
+ * is at most plan.residual, or has fallen by plan.drop from its first
+ * reading, or has stopped falling (no new minimum 10 % lower in plan.stall
+ * checks), and its drag changed by at most plan.cdTol over the last
+ * plan.settle checks; it fails at plan.maxSteps or on a non-physical
+ * state. Finished cases leave the launches at the next check.
+ */
+async function solveBatch(batch, freeStreams, baseRadius, planIn = {}) {
+  const plan = { ...DEFAULT_PLAN, ...planIn };
+  const n = batch.count;
+  const results = Array.from({ length: n }, () => ({ cd: NaN, residual: NaN, steps: 0, converged: false, reason: 'running' }));
+  const track = Array.from({ length: n }, () => ({ first: NaN, best: Infinity, since: 0, cds: [] }));
+  const active = Array(n).fill(true);
+  batch.setActive(active);
+  if (plan.implicit) {
+    batch.setRelaxation(plan.relaxation);
+    await batch.setImplicitCfl(plan.cflStart);
+  }
+  let done = 0;
+  while (active.some((a) => a)) {
+    if (plan.implicit) {
+      await batch.iterateImplicit(plan.chunk, { cfl: plan.cfl, sweeps: plan.sweeps, omega: plan.omega, growth: plan.growth });
+    } else {
+      await batch.iterate(plan.chunk);
+    }
+    done += plan.chunk;
+    const { status, steps } = await batch.status();
+    const norms = await batch.densityResidualNorms();
+    const cds = await batch.forebodyDrags(freeStreams, baseRadius);
+    let changed = false;
+    for (let c = 0; c < n; c++) {
+      if (!active[c]) continue;
+      const r = results[c];
+      const t = track[c];
+      r.steps = steps[c];
+      r.residual = norms[c];
+      r.cd = cds[c];
+      let finished = false;
+      if (status[c] & 1 || !Number.isFinite(r.residual) || !Number.isFinite(r.cd)) {
+        r.reason = 'non-physical state';
+        finished = true;
+      } else {
+        if (Number.isNaN(t.first)) t.first = r.residual;
+        if (r.residual < 0.9 * t.best) {
+          t.best = r.residual;
+          t.since = 0;
+        } else {
+          t.since += 1;
+        }
+        t.cds.push(r.cd);
+        const last = t.cds[t.cds.length - 1];
+        const settled =
+          t.cds.length > plan.settle &&
+          t.cds.slice(-1 - plan.settle, -1).every((v) => Math.abs(last - v) <= plan.cdTol * Math.abs(last));
+        const small = r.residual <= plan.residual || r.residual <= plan.drop * t.first;
+        const stalled = t.since >= plan.stall;
+        if (settled && (small || stalled)) {
+          r.converged = true;
+          r.reason = small ? 'converged' : 'converged (residual floor)';
+          finished = true;
+        } else if (done >= plan.maxSteps) {
+          r.reason = 'step limit';
+          finished = true;
+        }
+      }
+      if (finished) {
+        active[c] = false;
+        changed = true;
+      }
+    }
+    if (changed && active.some((a) => a)) batch.setActive(active);
+  }
+  return results;
+}
+
 module.exports = {
   GpuEuler2D,
+  GpuBatch2D,
+  solveBatch,
   GpuLineGrid,
   HipDevice,
   ExclusiveRunner,

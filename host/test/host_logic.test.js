@@ -23,7 +23,9 @@ async function test(name, fn) {
       'setPrimitives', 'setState', 'readState', 'setFaceStates', 'setFaceFluxes', 'residualNorm',
       'residualVector', 'wallFluxes', 'forebodyDrag', 'setImplicitCfl', 'iterate', 'advanceTo', 'iterateImplicit',
       'readState64', 'iterateFitted', 'setFittedShock', 'readShock', 'setLineGrid', 'setHeights', 'moveGrid',
-      'readHeights'];
+      'readHeights', 'batchCreate', 'batchDestroy', 'batchSetUniform', 'batchSetPrimitives', 'batchReadState',
+      'batchSetActive', 'batchSetInt', 'batchSetImplicitCfl', 'batchIterate', 'batchIterateImplicit',
+      'batchResidualNorms', 'batchDrags', 'batchStatus'];
     const a = bs.native();
     for (const n of names) assert.strictEqual(typeof a[n], 'function', n);
     assert.ok(/gfx950/.test(a.version()));
@@ -92,6 +94,39 @@ async function test(name, fn) {
     const wall = [{ x: 0, y: 0, nx: 1, ny: 0, area: 0.1, flux: [0, 2, 0, 0] },
       { x: 0, y: 0, nx: 0.5, ny: 0.5, area: 0.2, flux: [0, 0.5, 0, 0] }];
     assert.ok(Math.abs(bs.forebodyPressureDrag(wall, { pressure: 0, dynamicPressure: 1 }, 1) - 0.6) < 1e-15);
+  });
+
+  await test('solveBatch applies the stopping rule case by case', async () => {
+    // A stand-in batch with scripted readings: case 0 reaches the residual
+    // target 1e-6 with a settled drag, case 1 stalls at a floor
+    // with a settled drag, case 2 never settles, case 3 turns non-physical.
+    const checks = [];
+    const fake = {
+      count: 4,
+      done: 0,
+      active: null,
+      setActive(a) { this.active = a.slice(); },
+      async iterate(n) { this.done += n; checks.push(this.active.slice()); },
+      async status() {
+        return { status: Uint32Array.from([0, 0, 0, this.done >= 240 ? 1 : 0]), steps: Float64Array.from([1, 1, 1, 1], () => this.done) };
+      },
+      async densityResidualNorms() {
+        const k = this.done / 120;
+        return Float64Array.from([10 ** (-2 * k), 1e-3, 1 / k, 1]);
+      },
+      async forebodyDrags() {
+        const k = this.done / 120;
+        return Float64Array.from([0.2, 0.3, 0.2 + 0.01 * k, 0.4]);
+      },
+    };
+    const res = await bs.solveBatch(fake, { pressure: 1, dynamicPressure: 1 }, 0.5, { chunk: 120, maxSteps: 1200 });
+    assert.deepStrictEqual(res.map((r) => r.reason), ['converged', 'converged (residual floor)', 'step limit', 'non-physical state']);
+    assert.strictEqual(res[0].steps, 360);      // 1e-6 at the 3rd check, the drag settled over 2
+    assert.strictEqual(res[1].steps, 600);      // no new minimum in 4 checks after the first
+    assert.strictEqual(res[2].steps, 1200);
+    assert.strictEqual(res[3].steps, 240);
+    // Finished cases leave the launches at the next check.
+    assert.deepStrictEqual(checks[checks.length - 1], [false, false, true, false]);
   });
 
   await test('a device that does not exist is an error, not a fallback', () => {

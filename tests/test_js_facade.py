@@ -90,3 +90,60 @@ def test_node_fitted_shock_matches_the_python_facade(tmp_path, ctx):
     assert np.abs(np.array(out["speed"]) - gpu.readShock()["speed"]).max() < 1e-12
     assert out["steps"] == 30
     assert "fitted shock" in out["implicit"]
+
+
+@pytest.mark.gpu
+def test_node_batch_matches_the_python_facade(tmp_path, ctx):
+    """GpuBatch2D and solveBatch through the Node facade: two H grids and
+    a blunt-body C grid in one batch; the states after implicit steps
+    equal the Python facade's batch bit for bit, and the stopping rule
+    gives the same outcomes and drags."""
+    from bowshock_amd import grids
+    from bowshock_amd.batch import GpuBatch2D, SolvePlan, solve
+    from bowshock_amd.blunt_grid import blunt_body_grid
+    from bowshock_amd.engine import Euler2DConfig
+
+    specs = []
+    for ni, nj, mach in ((48, 24, 2.0), (40, 20, 3.0)):
+        g = grids.body_grid_fixed(grids.cone_cylinder(15.0, 4.0), ni, nj, mach, 1.4).grid
+        specs.append((g, mach, False))
+    specs.append((blunt_body_grid(grids.Body(grids.Nose("ellipsoid", 0.5, 0.5), 4.0), 12, 2.0).grid, 2.0, True))
+    case = dict(steps=30, maxSteps=4800,
+                cases=[dict(ni=g.ni, nj=g.nj, x=g.x.tolist(), y=g.y.tolist(), mach=m, axis=a) for g, m, a in specs])
+    path = tmp_path / "batch.json"
+    path.write_text(json.dumps(case))
+    r = subprocess.run([NODE, os.path.join(ROOT, "host", "test", "gpu_batch.js"), str(path)], capture_output=True,
+                       text=True, timeout=300)
+    print(r.stderr[-2000:])
+    assert r.returncode == 0
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    cfgs, frees, ps, qs = [], [], [], []
+    for g, m, axis in specs:
+        free, p, q = grids.free_stream_at(m, 1.4)
+        inflow = {"kind": "inflow", "state": free}
+        bcs = {"iMin": {"kind": "wall"} if axis else inflow, "iMax": {"kind": "outflow"}, "jMin": {"kind": "wall"},
+               "jMax": inflow}
+        cfgs.append(Euler2DConfig(grid=g, geometry="axisymmetric", gamma=1.4, flux="hllc", limiter="vanAlbada",
+                                  reconstruction="sharedScaled", timeIntegrator="ssprk3", cfl=0.5,
+                                  timeStepping="local", boundaries=bcs))
+        frees.append(free)
+        ps.append(p)
+        qs.append(q)
+    batch = GpuBatch2D(ctx, cfgs)
+    batch.setUniform(frees)
+    batch.setImplicitCfl(1.0)
+    batch.iterateImplicit(30, 10.0, 2)
+    for k in range(len(specs)):
+        assert np.array_equal(np.array(out["rho"][k], dtype=np.float32), batch.readState(k).rho), k
+    assert np.array_equal(np.array(out["norms"]), batch.densityResidualNorms())
+    assert np.array_equal(np.array(out["drags"]), batch.forebodyDrags(np.array(ps), np.array(qs), 0.5))
+    assert "await the previous operation" in out["overlap"]
+    assert "faceStates" in out["faceStates"]
+    batch.destroy()
+    b2 = GpuBatch2D(ctx, cfgs)
+    b2.setUniform(frees)
+    res = solve(b2, ps, qs, 0.5, SolvePlan(chunk=240, max_steps=4800))
+    for k, rr in enumerate(res):
+        js = out["solve"][k]
+        assert js["reason"] == rr.reason and js["steps"] == rr.steps and js["converged"] == rr.converged, k
+        assert js["cd"] == rr.cd, k
