@@ -111,6 +111,8 @@ def grid_on_lines(lines: LineSet, heights, nj: int, base_radius: float, gated: b
         raise ValueError("one height per line is needed")
     if lines.fractions is not None:
         frac = lines.fractions
+        if frac.shape != (nj + 1, len(h)):
+            raise ValueError(f"fixed fractions for {frac.shape[0] - 1} cells across, not {nj}")
     else:
         j = np.arange(nj + 1, dtype=np.float64)[:, None]
         frac = stretched_fraction(line_strength(h, base_radius)[None, :], j, nj)

@@ -239,9 +239,11 @@ using TileFnT = void (*)(TileArgsT<T>);
 template <typename T, int REC, int LIM, int FLUX>
 TileFnT<T> pick_mode(int mode) {
   constexpr int X = Tile<T>::x, Y = Tile<T>::y;
-  if constexpr (std::is_same<T, float>::value)
+  if constexpr (std::is_same<T, float>::value) {
     if (mode == TILE_STAGE) return tile_kernel_occ8<X, Y, REC, LIM, FLUX, TILE_STAGE>;
-  if (mode == TILE_STAGE) return tile_kernel<T, X, Y, REC, LIM, FLUX, TILE_STAGE>;
+  } else {
+    if (mode == TILE_STAGE) return tile_kernel<T, X, Y, REC, LIM, FLUX, TILE_STAGE>;
+  }
   if (mode == TILE_IMPLICIT) return tile_kernel<T, X, Y, REC, LIM, FLUX, TILE_IMPLICIT>;
   return tile_kernel<T, X, Y, REC, LIM, FLUX, TILE_RESIDUAL>;
 }

@@ -150,6 +150,8 @@ def test_fixed_fractions_reproduce_and_scale():
     fixed = with_fixed_fractions(b.lines, 20, 0.5)
     same = grid_on_lines(fixed, fixed.h, 20, 0.5)
     assert np.array_equal(same.x, b.grid.x) and np.array_equal(same.y, b.grid.y)
+    with pytest.raises(ValueError, match="cells across"):
+        grid_on_lines(fixed, fixed.h, 16, 0.5)
     taller = grid_on_lines(fixed, 1.1 * fixed.h, 20, 0.5, gated=False)
     for i in (0, 40, len(fixed) - 1):
         for j in (5, 20):
