@@ -298,7 +298,9 @@ def taylor_maccoll(mach, cone_angle, gamma, tolerance=1e-13) -> ConeFlow:
 # --------------------------------------------------------------------------
 
 QUALITY_LIMITS = dict(min_angle=12 * math.pi / 180, min_wall_angle=60 * math.pi / 180,
-                      max_size_ratio=1.3, max_aspect_ratio=10.0)  # quality.ts:252-266
+                      max_size_ratio=1.3, max_aspect_ratio=10.0)  # quality.ts:24-38
+# quality.ts:64-68: the grids of a fitted bow shock, whose lines end on it.
+FITTED_QUALITY_LIMITS = dict(QUALITY_LIMITS, min_wall_angle=50 * math.pi / 180, max_aspect_ratio=13.0)
 OUTER_ANGLE_FACTOR = 1.2  # body-grid.ts:61
 OUTER_CELLS_ABOVE_SHOCK = 3  # body-grid.ts:68
 MAX_TIP_HALF_ANGLE = math.pi / 2 - QUALITY_LIMITS["min_wall_angle"]  # body-grid.ts:74
@@ -341,54 +343,63 @@ def _js_round(v):
 
 
 def grid_quality(grid: StructuredGrid):
-    """quality.ts:298-353 gridQuality (vectorised)."""
+    """quality.ts:70-125 gridQuality (vectorised)."""
     ni, nj = grid.ni, grid.nj
     X = grid.x.reshape(nj + 1, ni + 1)
     Y = grid.y.reshape(nj + 1, ni + 1)
     corners = [(X[:-1, :-1], Y[:-1, :-1]), (X[:-1, 1:], Y[:-1, 1:]), (X[1:, 1:], Y[1:, 1:]),
                (X[1:, :-1], Y[1:, :-1])]
     area = np.zeros((nj, ni))
+    # NaN (from a degenerate cell's 0 / 0) propagates through np.min and
+    # np.max, as through Math.min and Math.max, and then fails the gates.
     min_angle = math.pi
     min_wall = math.pi
-    for k in range(4):
-        ax, ay = corners[k]
-        bx, by = corners[(k + 1) % 4]
-        px, py = corners[(k + 3) % 4]
-        area += 0.5 * (ax * by - bx * ay)
-        cosang = ((bx - ax) * (px - ax) + (by - ay) * (py - ay)) / (
-            np.hypot(bx - ax, by - ay) * np.hypot(px - ax, py - ay))
-        ang = np.arccos(np.clip(cosang, -1, 1))
-        min_angle = min(min_angle, float(ang.min()))
-        min_wall = min(min_wall, float(ang[0].min()))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        for k in range(4):
+            ax, ay = corners[k]
+            bx, by = corners[(k + 1) % 4]
+            px, py = corners[(k + 3) % 4]
+            area += 0.5 * (ax * by - bx * ay)
+            cosang = ((bx - ax) * (px - ax) + (by - ay) * (py - ay)) / (
+                np.hypot(bx - ax, by - ay) * np.hypot(px - ax, py - ay))
+            ang = np.arccos(np.clip(cosang, -1, 1))
+            min_angle = float(np.min([min_angle, ang.min()]))
+            min_wall = float(np.min([min_wall, ang[0].min()]))
 
-    def side(a, b):
-        return np.hypot(corners[b][0] - corners[a][0], corners[b][1] - corners[a][1])
+        def side(a, b):
+            return np.hypot(corners[b][0] - corners[a][0], corners[b][1] - corners[a][1])
 
-    w = 0.5 * (side(0, 1) + side(3, 2))
-    h = 0.5 * (side(0, 3) + side(1, 2))
-    aspect = float(max(1.0, np.max(w / h), np.max(h / w)))
-    size = 1.0
-    if ni > 1:
-        size = max(size, float(np.max(np.maximum(w[:, 1:] / w[:, :-1], w[:, :-1] / w[:, 1:]))))
-    if nj > 1:
-        size = max(size, float(np.max(np.maximum(h[1:] / h[:-1], h[:-1] / h[1:]))))
+        w = 0.5 * (side(0, 1) + side(3, 2))
+        h = 0.5 * (side(0, 3) + side(1, 2))
+        aspect = float(np.max([1.0, np.max(w / h), np.max(h / w)]))
+        size = 1.0
+        if ni > 1:
+            size = float(np.max([size, np.max(np.maximum(w[:, 1:] / w[:, :-1], w[:, :-1] / w[:, 1:]))]))
+        if nj > 1:
+            size = float(np.max([size, np.max(np.maximum(h[1:] / h[:-1], h[:-1] / h[1:]))]))
     return dict(min_angle=min_angle, min_wall_angle=min_wall, max_size_ratio=size,
                 max_aspect_ratio=aspect, min_area=float(area.min()))
 
 
 def quality_failures(q, limits=QUALITY_LIMITS):
-    """quality.ts:356-379 qualityFailures."""
+    """quality.ts:128-151 qualityFailures: the limits the grid fails, as the
+    reference's messages; written as not(value within limit), so a NaN from
+    a degenerate cell fails."""
+    def deg(r):
+        return f"{math.degrees(r):.1f} deg"
+
     f = []
     if not q["min_area"] > 0:
         f.append("a cell has non-positive area")
     if not q["min_angle"] >= limits["min_angle"]:
-        f.append(f"cell angle {math.degrees(q['min_angle']):.1f} deg below limit")
+        f.append(f"cell angle {deg(q['min_angle'])} below {deg(limits['min_angle'])}")
     if not q["min_wall_angle"] >= limits["min_wall_angle"]:
-        f.append(f"wall cell angle {math.degrees(q['min_wall_angle']):.1f} deg below limit")
+        f.append(f"wall cell angle {deg(q['min_wall_angle'])} below {deg(limits['min_wall_angle'])}")
     if not q["max_size_ratio"] <= limits["max_size_ratio"]:
-        f.append(f"neighbouring cells differ in size by {q['max_size_ratio']:.2f}")
+        f.append(f"neighbouring cells differ in size by {q['max_size_ratio']:.2f} "
+                 f"(limit {limits['max_size_ratio']:g})")
     if not q["max_aspect_ratio"] <= limits["max_aspect_ratio"]:
-        f.append(f"cell aspect ratio {q['max_aspect_ratio']:.1f} above limit")
+        f.append(f"cell aspect ratio {q['max_aspect_ratio']:.1f} above {limits['max_aspect_ratio']:g}")
     return f
 
 

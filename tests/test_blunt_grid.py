@@ -203,3 +203,29 @@ def test_slope_limit_bounds_the_spacing():
     assert np.max(d[1:] / d[:-1]) < 1.2
     smooth = column_positions_array([0.0, 1.0], lambda s: 20.0 + 0 * s, 1, max_spacing_slope=0.15)
     assert smooth == column_positions_array([0.0, 1.0], lambda s: 20.0 + 0 * s, 1)
+
+
+def test_grid_quality_measures():
+    """tests/grid/quality.test.ts: a uniform square grid is perfect; a grid
+    sheared by 40 deg with rows growing by 1.5 measures 50 deg and 1.5 and
+    fails with the reference's message; a collapsed corner fails rather
+    than slipping through on a NaN angle. The fitted grids' limits relax
+    the wall angle to 50 deg and the aspect ratio to 13 (quality.ts:64-68)."""
+    q = grids.grid_quality(grids.rectangular_grid(4, 4, 0.0, 1.0, 0.0, 1.0))
+    assert abs(q["min_angle"] - math.pi / 2) < 1e-12 and abs(q["max_size_ratio"] - 1) < 1e-12
+    assert abs(q["max_aspect_ratio"] - 1) < 1e-12 and abs(q["min_area"] - 1 / 16) < 1e-12
+    assert grids.quality_failures(q) == []
+    shear = math.tan(math.radians(40))
+
+    def node(i, j):
+        y = 0.25 * (1.5 ** j - 1)
+        return 4 * i * 0.25 + y * shear, y
+
+    q = grids.grid_quality(grids.grid_from_function(3, 3, node))
+    assert abs(math.degrees(q["min_angle"]) - 50) < 1e-9 and abs(q["max_size_ratio"] - 1.5) < 1e-9
+    assert "differ in size by 1.50 (limit 1.3)" in " ".join(grids.quality_failures(q))
+    degenerate = grids.grid_from_function(2, 1, lambda i, j: (0.0, 1.0) if (i == 1 and j == 1) else (float(i), float(j)))
+    assert len(grids.quality_failures(grids.grid_quality(degenerate))) > 0
+    assert grids.FITTED_QUALITY_LIMITS["min_wall_angle"] == 50 * math.pi / 180
+    assert grids.FITTED_QUALITY_LIMITS["max_aspect_ratio"] == 13
+    assert grids.FITTED_QUALITY_LIMITS["max_size_ratio"] == grids.QUALITY_LIMITS["max_size_ratio"]
