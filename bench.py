@@ -288,6 +288,10 @@ class _CaseGroups:
         g, local = self.where[k]
         self.groups[g][1].setCellPrimitives(local, prim)
 
+    def setCoarse(self, coarse, scale, shape):
+        for ks, b in self.groups:
+            b.setCoarse(coarse[ks], scale[ks], shape=shape)
+
     def readState(self, k):
         g, local = self.where[k]
         return self.groups[g][1].readState(local)
@@ -471,10 +475,11 @@ def run_warmstart(a):
         free_norm = batch.densityResidualNorms()
         used = np.zeros(len(cs), dtype=bool)
         if starts is not None:
-            # Warm start, kept where its density residual beats the free stream's.
-            warm = starts()
-            for k, prim in enumerate(warm):
-                batch.setCellPrimitives(k, prim)
+            # Warm start, interpolated to every case's cells on the GPU in
+            # one call, kept where its density residual beats the free
+            # stream's.
+            targets, scale = starts()
+            batch.setCoarse(targets, scale, (W.COARSE_NJ, W.COARSE_NI))
             warm_norm = batch.densityResidualNorms()
             used = warm_norm <= free_norm
             for k in np.nonzero(~used)[0]:
@@ -526,8 +531,8 @@ def run_warmstart(a):
 
     def starts():
         pred = ws.predict_targets(Fh)
-        return [W.from_target(pred[k], float(np.float32(c.mach * np.sqrt(1.4))), c.ni, c.nj)
-                for k, c in enumerate(held)]
+        u = np.array([float(np.float32(c.mach * np.sqrt(1.4))) for c in held])
+        return pred, np.stack([np.ones_like(u), u, u, u * u], axis=1)
 
     S_warm, batch, res_warm, wall_warm, used = run(held, starts)
     batch.destroy()

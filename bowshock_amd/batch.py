@@ -73,6 +73,19 @@ class GpuBatch2D:
         p = np.ascontiguousarray(np.broadcast_to(np.asarray(primitives, dtype=np.float64), (ni * nj, 4)))
         check(N.lib().bs_batch_set_primitives(self._h, int(index), N.dptr(p)))
 
+    def setCoarse(self, coarse, scale, floor: float = 0.05, shape=(40, 80)):
+        """Every case from a coarse primitive field at once, interpolated on
+        the GPU (bs_batch_set_coarse): coarse (count, 4 * cnj * cni) fields
+        of shape (cnj, cni), rows j-major; scale (count, 4) multipliers. The
+        state equals setCellPrimitives of the same field resampled on the
+        host (warmstart.from_target)."""
+        coarse = np.ascontiguousarray(coarse, dtype=np.float64)
+        scale = np.ascontiguousarray(np.broadcast_to(np.asarray(scale, dtype=np.float64), (self.count, 4)))
+        cnj, cni = shape
+        if coarse.shape != (self.count, 4 * cnj * cni):
+            raise RangeError(f"coarse fields must be ({self.count}, {4 * cnj * cni})")
+        check(N.lib().bs_batch_set_coarse(self._h, N.dptr(coarse), cni, cnj, N.dptr(scale), float(floor)))
+
     def readState(self, index: int) -> GpuState2D:
         ni, nj = self.shapes[index]
         out = [np.empty(ni * nj, dtype=np.float32) for _ in range(4)]

@@ -483,6 +483,59 @@ bs_status batch_reset_control(bs_batch* b, int first, int count) {
 
 }  // namespace
 
+// The warm start's coarse fields to the cases' cells: case c's coarse
+// primitives (4 x cnj x cni, FP64, row j0 = 0 first) times the case's scale
+// per variable, interpolated linearly between the cell centres of the two
+// uniform grids in (i / ni, j / nj), clamped at the ends; density and
+// pressure held at least `floor` (NaN kept, and flagged); stored conserved
+// in FP32 with the gamma the solver uses. The arithmetic is FP64 in the
+// order of the host's resampling, without contraction, so the state equals
+// the one set from the same field cell by cell on the host.
+struct CoarseCase {
+  float* state;  // the case's 4 x n conserved values
+  int n, ni;
+  int nj;
+  const double* coarse;
+  double scale[4];
+};
+
+__device__ __forceinline__ void coarse_axis(int k, int nd, int ns, int& a0, int& a1, double& w) {
+#pragma clang fp contract(off)
+  double s = (k + 0.5) / nd * ns - 0.5;
+  s = fmin(fmax(s, 0.0), ns - 1.0);
+  a0 = min((int)floor(s), max(ns - 2, 0));
+  w = s - a0;
+  a1 = min(a0 + 1, ns - 1);
+}
+
+__global__ __launch_bounds__(256) void batch_coarse_kernel(CoarseCase K, int cni, int cnj, double floor_, double g,
+                                                           int* bad) {
+#pragma clang fp contract(off)
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= K.n) return;
+  const int i = c % K.ni, j = c / K.ni;
+  int i0, i1, j0, j1;
+  double wi, wj;
+  coarse_axis(i, K.ni, cni, i0, i1, wi);
+  coarse_axis(j, K.nj, cnj, j0, j1, wj);
+  double w[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const double* f = K.coarse + (size_t)q * cnj * cni;
+    const double sq = K.scale[q];
+    const double a = (f[j0 * cni + i0] * sq) * (1.0 - wj) + (f[j1 * cni + i0] * sq) * wj;
+    const double b = (f[j0 * cni + i1] * sq) * (1.0 - wj) + (f[j1 * cni + i1] * sq) * wj;
+    w[q] = a * (1.0 - wi) + b * wi;
+  }
+  if (w[0] < floor_) w[0] = floor_;  // a NaN stays, and fails the check below
+  if (w[3] < floor_) w[3] = floor_;
+  if (!(w[0] > 0.0 && w[3] > 0.0 && isfinite(w[1]) && isfinite(w[2]))) atomicOr(bad, 1);
+  K.state[c] = (float)w[0];
+  K.state[(size_t)K.n + c] = (float)(w[0] * w[1]);
+  K.state[2 * (size_t)K.n + c] = (float)(w[0] * w[2]);
+  K.state[3 * (size_t)K.n + c] = (float)(w[3] / (g - 1) + 0.5 * w[0] * (w[1] * w[1] + w[2] * w[2]));
+}
+
 extern "C" {
 
 bs_status bs_batch_create(bs_ctx* ctx, const bs_config* configs, int32_t count, bs_batch** out) {
@@ -696,6 +749,52 @@ bs_status bs_batch_set_primitives(bs_batch* b, int32_t index, const double* prim
   HIP_TRY(hipMemcpy(b->state[b->cur] + 4 * k.cell, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice),
           "state upload");
   return batch_reset_control(b, index, 1);
+}
+
+bs_status bs_batch_set_coarse(bs_batch* b, const double* coarse, int32_t coarse_ni, int32_t coarse_nj,
+                              const double* scale, double floor_value) {
+  BATCH_GUARD(b);
+  if (!coarse || !scale || coarse_ni < 1 || coarse_nj < 1) return fail(BS_ERR_RANGE, "no coarse fields");
+  if (!(floor_value > 0)) return fail(BS_ERR_RANGE, "the floor must be positive");
+  const size_t per = 4 * (size_t)coarse_ni * coarse_nj;
+  double* d_coarse = nullptr;
+  int* d_bad = nullptr;
+  auto release = [&]() {
+    if (d_coarse) (void)hipFree(d_coarse);
+    if (d_bad) (void)hipFree(d_bad);
+  };
+  if (hipMalloc((void**)&d_coarse, per * b->count * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&d_bad, sizeof(int)) != hipSuccess) {
+    release();
+    return fail(BS_ERR_MEMORY, "coarse fields: out of device memory");
+  }
+  bs_status st = BS_OK;
+  int bad = 0;
+  if (hipMemcpyAsync(d_coarse, coarse, per * b->count * sizeof(double), hipMemcpyHostToDevice, b->stream) !=
+          hipSuccess ||
+      hipMemsetAsync(d_bad, 0, sizeof(int), b->stream) != hipSuccess) {
+    st = fail(BS_ERR_EXECUTION, "coarse fields: upload failed");
+  }
+  for (int c = 0; st == BS_OK && c < b->count; c++) {
+    const BatchCase& k = b->cases[c];
+    CoarseCase K;
+    K.state = b->state[b->cur] + 4 * k.cell;
+    K.n = k.n;
+    K.ni = k.ni;
+    K.nj = k.nj;
+    K.coarse = d_coarse + per * c;
+    for (int q = 0; q < 4; q++) K.scale[q] = scale[4 * (size_t)c + q];
+    hipLaunchKernelGGL(batch_coarse_kernel, dim3((k.n + 255) / 256), dim3(256), 0, b->stream, K, (int)coarse_ni,
+                       (int)coarse_nj, floor_value, (double)b->gamma32, d_bad);
+    if (hipGetLastError() != hipSuccess) st = fail(BS_ERR_EXECUTION, "coarse fields: launch failed");
+  }
+  if (st == BS_OK && (hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+                      hipStreamSynchronize(b->stream) != hipSuccess))
+    st = fail(BS_ERR_EXECUTION, "coarse fields: kernel failed");
+  release();
+  if (st) return st;
+  if (bad) return fail(BS_ERR_NONPHYSICAL, "a coarse field gives a non-physical state");
+  return batch_reset_control(b, 0, b->count);
 }
 
 bs_status bs_batch_read_state(bs_batch* b, int32_t index, float* rho, float* mu, float* mv, float* energy) {

@@ -322,6 +322,17 @@ bs_status bs_batch_destroy(bs_batch *b);
 bs_status bs_batch_set_uniform(bs_batch *b, const double *primitives);
 /* setCellPrimitives (euler2d-gpu.ts:439-460) of one case: 4 per cell. */
 bs_status bs_batch_set_primitives(bs_batch *b, int32_t index, const double *primitives);
+/* The learned warm start's fields for every case at once (no counterpart
+ * in the reference: its drivers start from the free stream or a given
+ * field, transonic-flow.ts:174): per case 4 x coarse_nj x coarse_ni
+ * primitives (rho, u, v, p; row-major, j outer) on a grid uniform in
+ * (i / ni, j / nj), multiplied by the case's 4 scale factors, interpolated
+ * bilinearly between cell centres (clamped at the ends) to the case's
+ * cells, density and pressure held at least `floor_value`. Equal to
+ * setting the same interpolated field with bs_batch_set_primitives.
+ * BS_ERR_NONPHYSICAL if a field is not finite. */
+bs_status bs_batch_set_coarse(bs_batch *b, const double *coarse, int32_t coarse_ni, int32_t coarse_nj,
+                              const double *scale, double floor_value);
 /* readState (euler2d-gpu.ts:549-573) of one case; null outputs skipped. */
 bs_status bs_batch_read_state(bs_batch *b, int32_t index, float *rho, float *mu, float *mv, float *energy);
 /* Cases with active[c] == 0 are skipped by later steps and reductions. */

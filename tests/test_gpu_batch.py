@@ -185,3 +185,36 @@ def test_mixed_topologies_match_single_solves(ctx, implicit):
     drags = batch.forebodyDrags(p, q, 0.5)
     for k, s in enumerate(singles):
         assert drags[k] == pytest.approx(s.forebodyDrag(p[k], q[k], 0.5), rel=1e-13, abs=1e-15)
+
+
+def test_coarse_fields_set_on_the_gpu_equal_the_host_resampling(ctx):
+    """bs_batch_set_coarse against bs_batch_set_primitives of the host's
+    resampling (warmstart.from_target) of the same seeded fields: every
+    case's state bit for bit, H and C grids alike; a NaN field is refused
+    as non-physical."""
+    from bowshock_amd import warmstart as W
+    from bowshock_amd.engine import NonPhysicalStateError
+
+    cs = _mixed_cases()
+    cfgs = [c[0] for c in cs]
+    rng = np.random.default_rng(5)
+    n_coarse = W.COARSE_NI * W.COARSE_NJ
+    base = np.array([1.0, 0.9, 0.05, 0.6])
+    targets = np.stack([np.concatenate([b + 0.3 * rng.standard_normal(n_coarse) for b in base]) for _ in cs])
+    targets[0, :n_coarse] = -0.5          # density below the floor everywhere in case 0
+    u_inf = np.array([c[1][1] for c in cs])
+    scale = np.stack([np.ones_like(u_inf), u_inf, u_inf, u_inf * u_inf], axis=1)
+    gpu = GpuBatch2D(ctx, cfgs)
+    gpu.setCoarse(targets, scale, 0.05, shape=(W.COARSE_NJ, W.COARSE_NI))
+    host = GpuBatch2D(ctx, cfgs)
+    for k, cfg in enumerate(cfgs):
+        host.setCellPrimitives(k, W.from_target(targets[k], u_inf[k], cfg.grid.ni, cfg.grid.nj, floor=0.05))
+    for k in range(len(cs)):
+        assert np.array_equal(_state(gpu.readState(k)), _state(host.readState(k))), k
+    assert np.all(gpu.readState(0).rho == np.float32(0.05))
+    bad = targets.copy()
+    bad[1, 7] = np.nan
+    with pytest.raises(NonPhysicalStateError):
+        gpu.setCoarse(bad, scale, 0.05, shape=(W.COARSE_NJ, W.COARSE_NI))
+    with pytest.raises(RangeError):
+        gpu.setCoarse(targets[:, :-1], scale)
