@@ -147,3 +147,112 @@ def test_fitted_moves_on_the_generator_lines(ctx, precision):
         assert dh < 1e-11 and l1.max() < 1e-11 and linf.max() < 1e-9
     else:
         assert dh < 1e-5 and l1.max() < 5e-6 and linf.max() < 2e-4
+
+
+def _line_solver(ctx, lines, eta, heights, precision, free):
+    from shock_fitting import grid_on_lines
+
+    grid = grid_on_lines(lines, eta, heights)
+    ni = grid.ni
+    bcs = {"iMin": {"kind": "wall"}, "iMax": {"kind": "outflow"}, "jMin": {"kind": "wall"},
+           "jMax": {"kind": "inflow", "state": free}}
+    cfg = Euler2DConfig(grid=grid, geometry="axisymmetric", gamma=GAMMA, flux="hllc", limiter="vanAlbada",
+                        reconstruction="sharedScaled", timeIntegrator="ssprk2", cfl=0.5, timeStepping="local",
+                        boundaries=bcs)
+    return grid, bcs, GpuEuler2D.create(ctx, cfg, precision)
+
+
+def _residual_error(gpu, grid, bcs, heights_grid):
+    """The solver's residual of its own state against the oracle's on the
+    grid the metrics should describe: equal metrics give equal residuals."""
+    import pyoracle as O
+
+    s = gpu.readState()
+    orc = O.Euler2D(heights_grid, geometry="axisymmetric", gamma=GAMMA, flux="hllc", limiter="vanAlbada",
+                    reconstruction="sharedScaled", time_integrator="ssprk2", cfl=0.5, time_stepping="local",
+                    boundaries=bcs)
+    q = np.stack([s.rho, s.mu, s.mv, s.energy], axis=1).astype(np.float64)
+    if s.rho.dtype == np.float32:
+        from bowshock_amd.engine import gpu_primitives
+
+        orc.set_primitives(gpu_primitives(s, GAMMA))
+    else:
+        orc.set_conserved(q)
+    r_gpu, r_orc = gpu.residualVector(), orc.residual_vector()
+    return (np.abs(r_gpu - r_orc).max(axis=0) / np.abs(r_orc).max(axis=0)).max()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_line_metrics_on_the_c_grid_heights_moved(ctx, precision):
+    """line-grid-gpu.test.ts:113-127: the hemisphere-cylinder's C grid with
+    fixed fractions, heights moved 5 %: the metrics the GPU computes from
+    the heights give the oracle's residual on the moved nodes (FP64 1e-11;
+    FP32, whose metrics are rounded to FP32, 2e-4)."""
+    from shock_fitting import grid_on_lines
+
+    nj = 20
+    b = blunt_body_grid(HEMI, nj, MACH)
+    fixed = with_fixed_fractions(b.lines, nj, 0.5)
+    lines, eta = fixed.origins_and_directions(), fixed.fractions
+    free = grids.free_stream_at(MACH, GAMMA)[0]
+    grid, bcs, gpu = _line_solver(ctx, lines, eta, fixed.h, precision, free)
+    gpu.setLineGrid(lines, eta, "symmetry")
+    h1 = 1.05 * fixed.h
+    gpu.setHeights(h1)
+    rng = np.random.default_rng(4)
+    prim = np.array(free) * (1 + 0.05 * rng.standard_normal((grid.ni * grid.nj, 4)))
+    prim[:, 2] = 0.1 * rng.standard_normal(grid.ni * grid.nj)
+    gpu.setCellPrimitives(prim)
+    err = _residual_error(gpu, grid, bcs, grid_on_lines(lines, eta, h1))
+    print(precision, "residual error on the moved grid", err)
+    assert err < (1e-11 if precision == "fp64" else 2e-4)
+
+
+def test_line_metrics_behind_a_sharp_tip(ctx):
+    """line-grid-gpu.test.ts:128-141: vertical lines behind a 15 deg cone's
+    tip, spaced geometrically from 1e-3 of the nose (cells 1e3 times
+    smaller at the tip than at the base): FP64 metrics from the heights
+    give the oracle's residual within 1e-11."""
+    from shock_fitting import grid_on_lines
+
+    length = 0.5 / math.tan(math.radians(15.0))
+    ni, nj = 40, 12
+    xs = np.concatenate([[0.0], length * 1e-3 * (1e3 ** (np.arange(ni) / (ni - 1)))])
+    lines = np.stack([xs, xs * math.tan(math.radians(15.0)), np.zeros(ni + 1), np.ones(ni + 1)], axis=1)
+    eta = np.tile((np.arange(nj + 1) / nj)[:, None], (1, ni + 1))
+    h0 = 0.02 + xs * 0.3
+    free = grids.free_stream_at(2.0, GAMMA)[0]
+    grid, bcs, gpu = _line_solver(ctx, lines, eta, h0, "fp64", free)
+    gpu.setLineGrid(lines, eta, "fixed")
+    h1 = h0 * 1.03
+    gpu.setHeights(h1)
+    rng = np.random.default_rng(6)
+    prim = np.array(free) * (1 + 0.05 * rng.standard_normal((grid.ni * grid.nj, 4)))
+    gpu.setCellPrimitives(prim)
+    err = _residual_error(gpu, grid, bcs, grid_on_lines(lines, eta, h1))
+    print("residual error behind the sharp tip", err)
+    assert err < 1e-11
+
+
+def test_remap_on_the_c_grid(ctx):
+    """line-grid-gpu.test.ts:144-: the flow remapped as the C grid's lines
+    move 2 %, against the oracle's remapAlongLines of the same flow."""
+    from bowshock_amd.engine import gpu_primitives
+    from shock_fitting import grid_on_lines, remap_along_lines
+
+    nj = 20
+    b = blunt_body_grid(HEMI, nj, MACH)
+    fixed = with_fixed_fractions(b.lines, nj, 0.5)
+    lines, eta = fixed.origins_and_directions(), fixed.fractions
+    free = grids.free_stream_at(MACH, GAMMA)[0]
+    grid, bcs, gpu = _line_solver(ctx, lines, eta, fixed.h, "fp64", free)
+    gpu.setLineGrid(lines, eta, "symmetry")
+    gpu.setHeights(fixed.h)
+    gpu.setCellPrimitives(np.array(free))
+    gpu.iterate(200)
+    before = gpu_primitives(gpu.readState(), GAMMA)
+    h1 = fixed.h * (1 + 0.02 * np.cos(np.arange(len(fixed.h))))
+    gpu.moveGrid(h1)
+    after = gpu_primitives(gpu.readState(), GAMMA)
+    expect = remap_along_lines(lines, nj, before, grid, grid_on_lines(lines, eta, h1))
+    assert np.abs(after - expect).max() < 1e-12 * np.abs(expect).max()
