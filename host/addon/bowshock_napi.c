@@ -649,6 +649,20 @@ static napi_value js_batch_read_state(napi_env env, napi_callback_info info) {
   return o;
 }
 
+/* batchSetCoarse(b, Float64Array coarse, cni, cnj, Float64Array scale,
+ * floor): bs_batch_set_coarse. */
+static napi_value js_batch_set_coarse(napi_env env, napi_callback_info info) {
+  napi_value argv[6];
+  if (!get_args(env, info, 6, argv)) return NULL;
+  const double *coarse = (const double *)typed(env, argv[1], napi_float64_array, NULL);
+  const double *scale = (const double *)typed(env, argv[4], napi_float64_array, NULL);
+  if (!coarse || !scale) return throw_status(env, BS_ERR_RANGE);
+  bs_status st = bs_batch_set_coarse((bs_batch *)get_ptr(env, argv[0]), coarse, i32(env, argv[2]), i32(env, argv[3]),
+                                     scale, num(env, argv[5]));
+  if (st) return throw_status(env, st);
+  return NULL;
+}
+
 /* batchSetActive(b, Uint8Array one per case). */
 static napi_value js_batch_set_active(napi_env env, napi_callback_info info) {
   napi_value argv[2];
@@ -849,6 +863,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"batchSetPrimitives", NULL, js_batch_set_primitives, NULL, NULL, NULL, napi_enumerable, NULL},
       {"batchReadState", NULL, js_batch_read_state, NULL, NULL, NULL, napi_enumerable, NULL},
       {"batchSetActive", NULL, js_batch_set_active, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"batchSetCoarse", NULL, js_batch_set_coarse, NULL, NULL, NULL, napi_enumerable, NULL},
       {"batchSetInt", NULL, js_batch_set_int, NULL, NULL, NULL, napi_enumerable, NULL},
       {"batchSetImplicitCfl", NULL, js_batch_set_implicit_cfl, NULL, NULL, NULL, napi_enumerable, NULL},
       {"batchIterate", NULL, js_batch_iterate, NULL, NULL, NULL, napi_enumerable, NULL},

@@ -178,6 +178,13 @@ export class GpuBatch2D {
   readonly configs: readonly GpuEuler2DConfig[];
   setUniform(states: Primitive2D | readonly Primitive2D[]): void;
   setCellPrimitives(index: number, cell: (i: number, j: number) => Primitive2D): void;
+  /** Learned warm-start fields for every case, interpolated on the GPU (bs_batch_set_coarse). */
+  setCoarse(
+    coarse: Float64Array | readonly number[],
+    scales: readonly (readonly [number, number, number, number])[],
+    shape?: readonly [number, number],
+    floor?: number,
+  ): void;
   readState(index: number): Promise<{ rho: Float32Array; mu: Float32Array; mv: Float32Array; energy: Float32Array }>;
   setActive(active: readonly boolean[]): void;
   setTiling(smallTileBelowCells: number): void;

@@ -570,6 +570,24 @@ class GpuBatch2D {
     call(() => native().batchSetPrimitives(this.handle, k, packed));
   }
 
+  /**
+   * Every case from a coarse primitive field (rho, u, v, p on a cnj x cni
+   * grid uniform in (i / ni, j / nj), rows j-major), times its case's four
+   * scale factors, interpolated on the GPU; density and pressure held at
+   * least `floor` (bs_batch_set_coarse).
+   */
+  setCoarse(coarse, scales, shape = [40, 80], floor = 0.05) {
+    this.exclusive.assertIdle();
+    const [cnj, cni] = shape;
+    const per = 4 * cnj * cni;
+    if (coarse.length !== this.count * per || scales.length !== this.count) {
+      throw new RangeError(`coarse fields of ${per} values and 4 scale factors per case are needed`);
+    }
+    const s = new Float64Array(4 * this.count);
+    scales.forEach((f, k) => s.set(f, 4 * k));
+    call(() => native().batchSetCoarse(this.handle, Float64Array.from(coarse), cni, cnj, s, floor));
+  }
+
   /** GpuState2D of case k (Float32Arrays). */
   async readState(k) {
     return this.exclusive.run(async () => native().batchReadState(this.handle, k, this.cells[k]));

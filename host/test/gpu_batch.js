@@ -45,6 +45,12 @@ const bs = require('..');
     out.overlap = e.message;
   }
   await running;
+  // Coarse fields for every case, interpolated on the GPU.
+  if (c.coarse) {
+    batch.setCoarse(c.coarse, c.scales);
+    out.coarseRho = [];
+    for (let k = 0; k < batch.count; k++) out.coarseRho.push(Array.from((await batch.readState(k)).rho));
+  }
   batch.destroy();
   // The stopping rule, explicit steps from the free stream.
   const solveBatch = await bs.GpuBatch2D.create(device, configs);

@@ -24,7 +24,7 @@ async function test(name, fn) {
       'residualVector', 'wallFluxes', 'forebodyDrag', 'setImplicitCfl', 'iterate', 'advanceTo', 'iterateImplicit',
       'readState64', 'iterateFitted', 'setFittedShock', 'readShock', 'setLineGrid', 'setHeights', 'moveGrid',
       'readHeights', 'batchCreate', 'batchDestroy', 'batchSetUniform', 'batchSetPrimitives', 'batchReadState',
-      'batchSetActive', 'batchSetInt', 'batchSetImplicitCfl', 'batchIterate', 'batchIterateImplicit',
+      'batchSetActive', 'batchSetCoarse', 'batchSetInt', 'batchSetImplicitCfl', 'batchIterate', 'batchIterateImplicit',
       'batchResidualNorms', 'batchDrags', 'batchStatus'];
     const a = bs.native();
     for (const n of names) assert.strictEqual(typeof a[n], 'function', n);

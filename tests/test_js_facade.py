@@ -108,7 +108,15 @@ def test_node_batch_matches_the_python_facade(tmp_path, ctx):
         g = grids.body_grid_fixed(grids.cone_cylinder(15.0, 4.0), ni, nj, mach, 1.4).grid
         specs.append((g, mach, False))
     specs.append((blunt_body_grid(grids.Body(grids.Nose("ellipsoid", 0.5, 0.5), 4.0), 12, 2.0).grid, 2.0, True))
-    case = dict(steps=30, maxSteps=4800,
+    from bowshock_amd import warmstart as W
+
+    rng = np.random.default_rng(8)
+    n_coarse = W.COARSE_NI * W.COARSE_NJ
+    coarse = np.stack([np.concatenate([b + 0.2 * rng.standard_normal(n_coarse) for b in (1.0, 0.9, 0.0, 0.6)])
+                       for _ in specs])
+    u_inf = np.array([grids.free_stream_at(m, 1.4)[0][1] for _, m, _ in specs])
+    scales = np.stack([np.ones_like(u_inf), u_inf, u_inf, u_inf * u_inf], axis=1)
+    case = dict(steps=30, maxSteps=4800, coarse=coarse.ravel().tolist(), scales=scales.tolist(),
                 cases=[dict(ni=g.ni, nj=g.nj, x=g.x.tolist(), y=g.y.tolist(), mach=m, axis=a) for g, m, a in specs])
     path = tmp_path / "batch.json"
     path.write_text(json.dumps(case))
@@ -139,6 +147,9 @@ def test_node_batch_matches_the_python_facade(tmp_path, ctx):
     assert np.array_equal(np.array(out["drags"]), batch.forebodyDrags(np.array(ps), np.array(qs), 0.5))
     assert "await the previous operation" in out["overlap"]
     assert "faceStates" in out["faceStates"]
+    batch.setCoarse(coarse, scales)
+    for k in range(len(specs)):
+        assert np.array_equal(np.array(out["coarseRho"][k], dtype=np.float32), batch.readState(k).rho), k
     batch.destroy()
     b2 = GpuBatch2D(ctx, cfgs)
     b2.setUniform(frees)
