@@ -731,6 +731,21 @@ def run_ladder(a):
         row = {"body": body_name, "mach": mach, "grid": [ni, nj], "scheme": label,
                "steps": r.steps, "seconds": wall, "cell_updates_per_s": ni * nj * r.steps / wall, "cd": r.cd,
                "residual": r.residual, "outcome": r.reason}
+        # The FP32 result's error against FP64: the FP64 solver (pinned to
+        # the oracle at 1e-12) run for the same steps; its implicit steps
+        # relax with the linearized neighbour term, which changes the path
+        # but not the steady state.
+        g64 = GpuEuler2D.create(ctx, cfg, "fp64")
+        g64.setCellPrimitives(np.array(free))
+        if implicit:
+            g64.setImplicitCfl(plan.cfl_start)
+            g64.iterateImplicit(r.steps, cfl=plan.cfl, sweeps=plan.sweeps, omega=plan.omega, growth=plan.growth)
+        else:
+            g64.iterate(r.steps)
+        cd64 = g64.forebodyDrag(p_inf, q_inf, 0.5)
+        row.update({"cd_fp64_same_steps": cd64, "cd_fp32_vs_fp64": abs(r.cd / cd64 - 1),
+                    "residual_fp64": g64.densityResidualNorm()})
+        g64.destroy()
         if oracle_seconds > 0:
             sys.path.insert(0, os.path.join(ROOT, "oracle"))
             import pyoracle as O
