@@ -64,6 +64,8 @@ def parse():
     ap.add_argument("--traffic", default=None, help="rocprofv3 PMC summary of the stage kernel")
     ap.add_argument("--workload", choices=("single", "implicit", "sweep", "population", "warmstart", "ladder", "verified"), default="single")
     ap.add_argument("--train-steps", type=int, default=3000, help="warmstart: Adam steps of the network")
+    ap.add_argument("--warm-snap", type=float, default=0.0,
+                    help="warmstart: predicted cells within this of the free stream start as the free stream")
     ap.add_argument("--rockets", type=int, default=2000, help="population: rockets drawn (4 Mach numbers each)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="batched CPU baseline: seconds of oracle work")
     return ap.parse_args()
@@ -545,6 +547,8 @@ def run_warmstart(a):
     def starts():
         pred = ws.predict_targets(Fh)
         u = np.array([float(np.float32(c.mach * np.sqrt(1.4))) for c in held])
+        if a.warm_snap > 0:
+            pred = W.snap_to_free_stream(pred, u, a.warm_snap)
         return pred, np.stack([np.ones_like(u), u, u, u * u], axis=1)
 
     S_warm, batch, res_warm, wall_warm, used = run(held, starts)

@@ -154,6 +154,25 @@ def to_target(prim: np.ndarray, u_inf: float) -> np.ndarray:
     return (coarse / scale).ravel()
 
 
+def snap_to_free_stream(targets: np.ndarray, u_inf, tol: float) -> np.ndarray:
+    """Predicted coarse fields (count, 4 * COARSE_NJ * COARSE_NI) with every
+    coarse cell whose four normalized primitives all lie within `tol` of
+    the free stream's (rho = 1, u = 1, v = 0, p = 1 / u_inf^2 after the
+    scaling of to_target) set to the free stream exactly: the region ahead
+    of the shock then starts as a free-stream start does."""
+    t = np.array(targets, dtype=np.float64).reshape(len(targets), 4, -1)
+    u = np.asarray(u_inf, dtype=np.float64).reshape(-1, 1)
+    free = [np.ones_like(u), np.ones_like(u), np.zeros_like(u), 1.0 / (u * u)]
+    near = np.ones(t[:, 0].shape, dtype=bool)
+    for q in range(4):
+        ref = free[q]
+        scale = np.maximum(np.abs(ref), 1.0 / (u * u) if q == 3 else 1.0)
+        near &= np.abs(t[:, q] - ref) <= tol * scale
+    for q in range(4):
+        t[:, q] = np.where(near, free[q], t[:, q])
+    return t.reshape(len(targets), -1)
+
+
 def from_target(t: np.ndarray, u_inf: float, ni: int, nj: int, floor: float = 0.05) -> np.ndarray:
     """(ni nj, 4) primitives on a case's grid from a predicted coarse field,
     density and pressure clamped to at least `floor` of the free stream's
