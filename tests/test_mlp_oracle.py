@@ -95,3 +95,21 @@ def test_target_round_trip_and_clamp():
     t2 = t.copy()
     t2[:3200] = -1.0  # negative densities
     assert W.from_target(t2, 2.0, ni, nj)[:, 0].min() == pytest.approx(0.05)
+
+
+def test_snap_to_free_stream():
+    """Coarse cells within the tolerance of the free stream (normalized:
+    rho 1, u 1, v 0, p 1 / u_inf^2) become it exactly; any variable outside
+    keeps the cell as predicted."""
+    from bowshock_amd import warmstart as W
+
+    n = W.COARSE_NI * W.COARSE_NJ
+    u = np.array([2.0, 3.0])
+    t = np.concatenate([np.full(n, 1.005), np.full(n, 0.99), np.full(n, 0.003), np.full(n, 0.25 * 1.01)])
+    t = np.stack([t, t])
+    t[0, 7] = 1.5
+    s = W.snap_to_free_stream(t, u, 0.02).reshape(2, 4, n)
+    assert np.all(s[0, 0, :7] == 1.0) and np.all(s[0, 2, :7] == 0.0) and np.all(s[0, 3, :7] == 0.25)
+    assert s[0, 0, 7] == 1.5 and s[0, 3, 7] == t[0, 3 * n + 7]
+    # At Mach 3 (u_inf = 3) the pressure 0.2525 is far from 1/9: nothing snaps.
+    assert np.array_equal(s[1].ravel(), t[1])
