@@ -1,6 +1,6 @@
 """The forebody pressure drag of a body with its measured discretisation
 error, solved on the HIP engine: analysis/verified-drag.ts
-verifiedForebodyDrag (lines 230-319) over this build's GpuEuler2D.
+verifiedForebodyDrag (lines 230-319) over the HIP GpuEuler2D.
 
 Each grid of the nested bodyGrid family (20 cells across at refinement 1,
 doubled per level) is solved from the free stream with the reference's
