@@ -381,6 +381,39 @@ def _build_grids_logged(P, cases, chunk=1000):
               flush=True)
 
 
+def _fp64_sample(ctx, S, results, plan, count=16):
+    """The FP32 results' drag error against FP64 on a sample of the
+    converged cases spread over the batch: each case solved again by the
+    FP64 solver (pinned to the oracle at 1e-12) for the same steps."""
+    import numpy as np
+
+    from bowshock_amd.engine import GpuEuler2D
+
+    conv = [k for k, r in enumerate(results) if r.converged]
+    if not conv:
+        return None
+    pick = sorted({conv[int(t)] for t in np.linspace(0, len(conv) - 1, min(count, len(conv)))})
+    diffs = []
+    for k in pick:
+        g = GpuEuler2D.create(ctx, S["configs"][k], "fp64")
+        g.setCellPrimitives(np.array(S["frees"][k]))
+        steps = results[k].steps
+        if plan.implicit:
+            g.setImplicitCfl(plan.cfl_start)
+            g.iterateImplicit(steps, cfl=plan.cfl, sweeps=plan.sweeps, omega=plan.omega, growth=plan.growth)
+        else:
+            g.iterate(steps)
+        cd64 = g.forebodyDrag(S["p_inf"][k], S["q_inf"][k], 0.5)
+        diffs.append(abs(results[k].cd / cd64 - 1))
+        g.destroy()
+    d = np.array(diffs)
+    print(f"FP32 vs FP64 drag on {len(d)} sampled cases: median {np.median(d):.2e}, max {d.max():.2e}",
+          file=sys.stderr, flush=True)
+    return {"cases": len(d), "median_rel_cd_diff": float(np.median(d)), "max_rel_cd_diff": float(d.max()),
+            "method": "FP64 solver, same configuration and step count; its implicit steps relax with the "
+                      "linearized term (same steady state)"}
+
+
 def run_batch(a):
     import numpy as np
 
@@ -421,6 +454,7 @@ def run_batch(a):
     print(f"{converged}/{len(ok)} converged in {wall:.2f}s: {solves:.2f} solves/s, {cu / wall:.3e} CU/s; "
           f"{reasons}; steps median {int(np.median(steps))}, p99 {int(np.percentile(steps, 99))}; "
           f"converged by (s) {pct}", file=sys.stderr, flush=True)
+    fp64 = _fp64_sample(ctx, S, results, plan)
     cpu = _cpu_baseline(S, not sweep, a.cpu_seconds, cu / max(1, converged)) if a.cpu_seconds > 0 else None
     cds = [r.cd for r in results if r.converged]
     line = {
@@ -451,6 +485,7 @@ def run_batch(a):
                      "bytes_per_cell_update": bpc},
         "cpu_baseline": cpu,
         "drag_range": [min(cds), max(cds)] if cds else None,
+        "fp32_vs_fp64": fp64,
     }
     print(json.dumps(line), flush=True)
     batch.destroy()
