@@ -229,3 +229,37 @@ def test_grid_quality_measures():
     assert grids.FITTED_QUALITY_LIMITS["min_wall_angle"] == 50 * math.pi / 180
     assert grids.FITTED_QUALITY_LIMITS["max_aspect_ratio"] == 13
     assert grids.FITTED_QUALITY_LIMITS["max_size_ratio"] == grids.QUALITY_LIMITS["max_size_ratio"]
+
+
+def test_fitted_start_finds_a_planted_shock():
+    """bowshock_amd/fitted_start.py on a synthetic captured flow: pressure
+    10 below a planted shock height (a smooth function of the line) and 1
+    above it, linear across one cell: the heights come back within a cell
+    of the planted ones before smoothing, the fitted grid's top nodes lie
+    on them, and the start carries the post-shock state below and the
+    free stream nowhere inside."""
+    from bowshock_amd.fitted_start import _centres, fitted_start, shock_heights
+
+    nj = 20
+    b = blunt_body_grid(HEMI, nj, 3.0)
+    g, L = b.grid, b.lines
+    ni = g.ni
+    planted = 0.5 * L.h
+    cx, cy = _centres(g)
+    prim = np.zeros((nj, ni, 4))
+    prim[..., 0], prim[..., 1] = 1.0, 1.0
+    for i in range(ni):
+        d = (cx[:, i] - L.x[i]) * L.dx[i] + (cy[:, i] - L.y[i]) * L.dy[i]
+        hs = 0.5 * (planted[i] + planted[i + 1])
+        prim[:, i, 3] = np.where(d < hs, 10.0, 1.0)
+    prim = prim.reshape(-1, 4)
+    h0 = shock_heights(L, g, prim, smoothing=0)
+    X, Y = g.x.reshape(nj + 1, ni + 1)[:, 0], g.y.reshape(nj + 1, ni + 1)[:, 0]
+    spacing = np.max(np.diff((X - L.x[0]) * L.dx[0] + (Y - L.y[0]) * L.dy[0]))   # tallest cell on the axis line
+    assert np.all(np.abs(h0 - planted) < 0.25 * L.h)
+    assert abs(h0[0] - planted[0]) < 2 * spacing
+    st = fitted_start(L, g, prim, 0.5)
+    top = np.stack([st.grid.x, st.grid.y]).reshape(2, nj + 1, ni + 1)[:, nj]
+    along = (top[0] - L.x) * L.dx + (top[1] - L.y) * L.dy
+    assert np.allclose(along, st.heights, rtol=0, atol=1e-12)
+    assert st.primitives[:, 3].min() >= 1.0 and np.median(st.primitives[:, 3]) == 10.0

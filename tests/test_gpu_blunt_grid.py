@@ -263,18 +263,17 @@ def test_fitted_hemisphere_from_the_captured_start(ctx):
     end on the GPU as the reference's fitted flow runs it
     (src/analysis/fitted-flow.ts bluntStart, bluntFitFamily; shock-
     fitting.ts fitShock): the captured FP64 solution on the generator's C
-    grid, its shock height on each line where the pressure crosses half its
-    rise (smoothed twice, 1/4-1/2-1/4), the lines' node fractions fixed at
-    those heights (withFixedFractions), the flow started at the same
-    fraction of the shock layer, then 6,000 moves of two steps with the
-    shock fitted and the grid moving on the GPU. Bars: the fitted grid
-    meets the fitted grids' gates; the shock settles (speeds below 1e-9 of
-    the free stream's sound speed, the residual 1e-10 of its first
-    reading); the standoff within 1 % of the captured one and 10 % of
+    grid, the fitted start on its lines (bowshock_amd/fitted_start.py:
+    shock heights at half the pressure rise, smoothed; fixed fractions; the
+    flow at the same distance along each line), then 6,000 moves of two
+    steps with the shock fitted and the grid moving on the GPU. Bars: the
+    fitted grid meets the fitted grids' gates; the shock settles (speeds
+    below 1e-9 of the free stream's sound speed, the residual 1e-10 of its
+    first reading); the standoff within 1 % of the captured one and 10 % of
     Billig's; the drag within 1 % of the captured grid's."""
     import pyoracle as O
-    from bowshock_amd.blunt_grid import grid_on_lines as lines_grid
     from bowshock_amd.engine import gpu_primitives
+    from bowshock_amd.fitted_start import fitted_start
 
     nj = 20
     bg = blunt_body_grid(HEMI, nj, MACH)
@@ -288,41 +287,16 @@ def test_fitted_hemisphere_from_the_captured_start(ctx):
     cap.setCellPrimitives(np.array(free))
     cap.iterate(6000)
     cd_cap = cap.forebodyDrag(p_inf, q_inf, 0.5)
-    prim = gpu_primitives(cap.readState(), GAMMA).reshape(nj, ni, 4)
-    m = O.compute_metrics(grid, True)
-    cx, cy = m["cx"].reshape(nj, ni), m["cy"].reshape(nj, ni)
-    L = bg.lines
-    heights = np.zeros(ni + 1)
-    for i in range(ni + 1):
-        col = min(i, ni - 1)
-        d = (cx[:, col] - L.x[i]) * L.dx[i] + (cy[:, col] - L.y[i]) * L.dy[i]
-        p = prim[:, col, 3]
-        mid = 1 + 0.5 * (p.max() - 1)
-        j = nj - 1
-        while j > 0 and p[j] < mid:
-            j -= 1
-        heights[i] = d[-1] if j == nj - 1 else d[j] + (p[j] - mid) / (p[j] - p[j + 1]) * (d[j + 1] - d[j])
-    for _ in range(2):
-        c = heights.copy()
-        heights[0] = 0.5 * (c[0] + c[1])
-        heights[1:-1] = 0.25 * c[:-2] + 0.5 * c[1:-1] + 0.25 * c[2:]
-        heights[-1] = 0.5 * (c[-1] + c[-2])
-    fixed = with_fixed_fractions(L.with_heights(heights), nj, 0.5)
-    fg = lines_grid(fixed, heights, nj, 0.5, gated=False)
+    prim = gpu_primitives(cap.readState(), GAMMA)
+    st = fitted_start(bg.lines, grid, prim, 0.5)
+    fixed, heights, fg = st.lines, st.heights, st.grid
     assert grids.quality_failures(grids.grid_quality(fg), grids.FITTED_QUALITY_LIMITS) == []
-    fm = O.compute_metrics(fg, True)
-    fcx, fcy = fm["cx"].reshape(nj, ni), fm["cy"].reshape(nj, ni)
-    start = np.zeros((nj, ni, 4))
-    for i in range(ni):
-        d_cap = (cx[:, i] - L.x[i]) * L.dx[i] + (cy[:, i] - L.y[i]) * L.dy[i]
-        d_fit = (fcx[:, i] - L.x[i]) * L.dx[i] + (fcy[:, i] - L.y[i]) * L.dy[i]
-        for k in range(4):
-            start[:, i, k] = np.interp(d_fit, d_cap, prim[:, i, k])
+    m = O.compute_metrics(grid, True)
     fbcs = dict(bcs, jMax={"kind": "faceStates", "states": np.tile(np.array(free), (ni, 1))})
     fit = GpuEuler2D.create(ctx, Euler2DConfig(grid=fg, boundaries=fbcs, cfl=0.5, **scheme), "fp64")
     fit.setLineGrid(fixed.origins_and_directions(), fixed.fractions, "symmetry")
     fit.setHeights(heights)
-    fit.setCellPrimitives(start.reshape(-1, 4))
+    fit.setCellPrimitives(st.primitives)
     fit.setFittedShock(free)
     fit.iterateFitted(500, 2)
     r_first = fit.densityResidualNorm()
@@ -331,7 +305,7 @@ def test_fitted_hemisphere_from_the_captured_start(ctx):
     speed = np.abs(fit.readShock()["speed"]).max() / math.sqrt(GAMMA)
     res = fit.densityResidualNorm() / r_first
     cd = fit.forebodyDrag(p_inf, q_inf, 0.5)
-    standoff_cap = _standoff(grid, prim.reshape(-1, 4), m["cx"])
+    standoff_cap = _standoff(grid, prim, m["cx"])
     print("standoff", h[0], "captured", standoff_cap, "Billig", billig_standoff(MACH, 0.5), "cd", cd, "captured",
           cd_cap, "residual", res, "speed", speed)
     assert speed < 1e-9 and res < 1e-10
