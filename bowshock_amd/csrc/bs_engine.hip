@@ -145,8 +145,9 @@ struct bs_solver {
   Timing timing;
   long long steps_accounted = 0;
   bool graphs = true;
-  StepGraph explicit_graph, implicit_graph, anderson_graph;
+  StepGraph explicit_graph, implicit_graph, anderson_graph, fitted_graph;
   long long anderson_for = 0;  // id of the history the anderson graph was captured with
+  int fitted_for = 0;          // steps per move the fitted graph was captured with
 };
 
 namespace {
@@ -536,6 +537,7 @@ void free_all(bs_solver* s) {
   drop_graph(s->explicit_graph);
   drop_graph(s->implicit_graph);
   drop_graph(s->anderson_graph);
+  drop_graph(s->fitted_graph);
   DeviceBuffers& d = s->d;
   void* all[] = {d.state[0], d.state[1], d.state[2], d.inx, d.iny, d.ia, d.jnx, d.jny, d.ja, d.vol, d.planar,
                  d.dt, d.dt_global, d.dt_partials, d.partials, d.scalar, d.control, d.bnd, d.wall_out, d.shock_out,
@@ -1578,6 +1580,7 @@ bs_status bs_set_fitted_shock(bs_solver* s, const double* free_stream) {
   s->fitted = free_stream != nullptr;
   s->imposed[JMAX] = s->fitted;
   drop_graph(s->explicit_graph);
+  drop_graph(s->fitted_graph);
   return upload_boundary(s);
 }
 
@@ -1669,6 +1672,7 @@ bs_status bs_set_line_grid(bs_solver* s, const double* lines, const double* frac
                      (st = dev_alloc(&s->d.eta, nl * (nj + 1), "fractions")) ||
                      (st = dev_alloc(&s->d.height, nl, "heights")) || (st = dev_alloc(&s->d.move, nl, "moves"))))
     return st;
+  drop_graph(s->fitted_graph);  // the symmetry flag is captured with the moves
   HIP_TRY(hipMemcpy(s->d.line, lines, 4 * nl * sizeof(double), hipMemcpyHostToDevice), "lines");
   HIP_TRY(hipMemcpy(s->d.eta, fractions, nl * (nj + 1) * sizeof(double), hipMemcpyHostToDevice), "fractions");
   s->lines = true;
@@ -1730,17 +1734,27 @@ bs_status bs_iterate_fitted(bs_solver* s, int32_t moves, int32_t steps_per_move,
   if (steps_per_move < 1 || steps_per_move > BATCH)
     return fail(BS_ERR_RANGE, "steps per move must be an integer from 1 to %d", BATCH);
   const int per_batch = BATCH / steps_per_move;
+  // One move: its steps, then the shock move. Whole graphs of moves are
+  // replayed where they fit (each move advances the state buffer by the
+  // remap's one, so GRAPH_STEPS moves return to it).
+  auto encode_move = [s, steps_per_move]() -> bs_status {
+    for (int k = 0; k < steps_per_move; k++) {
+      bs_status st = encode_step(s);
+      if (st) return st;
+    }
+    encode_shock_move(s);
+    return BS_OK;
+  };
+  if (s->fitted_for != steps_per_move) {
+    drop_graph(s->fitted_graph);
+    s->fitted_for = steps_per_move;
+  }
   Control c;
   int remaining = moves;
   while (remaining > 0) {
     const int batch = remaining < per_batch ? remaining : per_batch;
-    for (int m = 0; m < batch; m++) {
-      for (int k = 0; k < steps_per_move; k++) {
-        bs_status st = encode_step(s);
-        if (st) return st;
-      }
-      encode_shock_move(s);
-    }
+    bs_status st0 = run_steps(s, s->fitted_graph, batch, encode_move);
+    if (st0) return st0;
     HIP_TRY(hipGetLastError(), "fitted steps");
     bs_status st = read_control(s, &c);
     if (st) return st;
@@ -1778,6 +1792,7 @@ bs_status bs_enable_graphs(bs_solver* s, int32_t enable) {
     drop_graph(s->explicit_graph);
     drop_graph(s->implicit_graph);
     drop_graph(s->anderson_graph);
+    drop_graph(s->fitted_graph);
   }
   return BS_OK;
 }
