@@ -311,3 +311,37 @@ def test_fitted_hemisphere_from_the_captured_start(ctx):
     assert speed < 1e-9 and res < 1e-10
     assert abs(h[0] / standoff_cap - 1) < 0.01 and abs(h[0] / billig_standoff(MACH, 0.5) - 1) < 0.1
     assert abs(cd / cd_cap - 1) < 0.01
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("kind", ["H", "C"])
+def test_free_stream_preserved_on_generated_grids(ctx, precision, kind):
+    """euler2d-gpu.test.ts:117-165: every side imposing the free stream
+    (the body too), the uniform stream is the exact solution on the
+    generated grids, skewed cells included: 50 steps keep it to FP32
+    round-off (1e-5, the reference's bar) or FP64 round-off (1e-12). The H
+    grid of a 3 deg cone at Mach 1.1 and the C grid of the
+    hemisphere-cylinder at Mach 3 (its iMin on the axis)."""
+    from bowshock_amd.engine import gpu_primitives
+
+    if kind == "H":
+        mach = 1.1
+        body = grids.Body(grids.Nose("conical", 0.5 / math.tan(math.radians(3.0)), 0.5), 4.0)
+        grid = grids.body_grid(body, 20, mach, GAMMA).grid
+    else:
+        mach = MACH
+        grid = blunt_body_grid(HEMI, 20, mach).grid
+    free = (1.0, mach * math.sqrt(GAMMA), 0.0, 1.0)
+    inflow = {"kind": "inflow", "state": free}
+    bcs = {"iMin": inflow, "iMax": {"kind": "outflow"}, "jMin": inflow, "jMax": inflow}
+    cfg = Euler2DConfig(grid=grid, geometry="axisymmetric", gamma=GAMMA, flux="hllc", limiter="vanAlbada",
+                        reconstruction="sharedScaled", timeIntegrator="ssprk2", cfl=0.8, timeStepping="local",
+                        boundaries=bcs)
+    gpu = GpuEuler2D.create(ctx, cfg, precision)
+    gpu.setCellPrimitives(np.array(free))
+    gpu.iterate(50)
+    w = gpu_primitives(gpu.readState(), GAMMA)
+    bar = 1e-12 if precision == "fp64" else 1e-5
+    err = max(np.abs(w[:, 0] - 1).max(), np.abs(w[:, 3] - 1).max(), np.abs(w[:, 2]).max() / free[1])
+    print(kind, precision, "largest deviation", err)
+    assert err < bar
