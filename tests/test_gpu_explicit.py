@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 import cases
+from bowshock_amd.engine import Euler2DConfig, GpuEuler2D
 from cases import GAMMA
 
 pytestmark = pytest.mark.gpu
@@ -163,3 +164,65 @@ def test_non_physical_state_is_reported(ctx):
 
     with pytest.raises(NonPhysicalStateError):
         gpu.setCellPrimitives(np.array((1.0, 0.0, 0.0, -1.0)))
+
+
+def _planar(grid, **over):
+    """The reference GPU tests' default configuration (euler2d-gpu.test.ts:
+    47-62): planar, HLLC, van Leer per variable, SSP-RK2, CFL 0.5, global
+    steps, outflow sides."""
+    kw = dict(geometry="planar", gamma=cases.GAMMA, flux="hllc", limiter="vanLeer", reconstruction="perVariable",
+              timeIntegrator="ssprk2", cfl=0.5, timeStepping="global",
+              boundaries={s: {"kind": "outflow"} for s in ("iMin", "iMax", "jMin", "jMax")})
+    kw.update(over)
+    return Euler2DConfig(grid=grid, **kw)
+
+
+def test_local_steps_follow_the_oracle_on_uneven_cells(ctx):
+    """euler2d-gpu.test.ts:421-448: a smooth pressure pulse on a randomly
+    distorted 16 x 12 grid, 20 local steps: the pressure of every cell
+    within 1e-5 of the FP64 oracle's."""
+    import pyoracle as O
+    from bowshock_amd.engine import gpu_primitives
+
+    grid = cases.distorted_grid(16, 12, 0.0, 2.0, 0.0, 1.5)
+    cfg = _planar(grid, timeStepping="local", cfl=0.6)
+    m = O.compute_metrics(grid, False)
+    x, y = m["cx"], m["cy"]
+    prim = np.stack([np.ones_like(x), np.full_like(x, 0.3), np.full_like(x, 0.1),
+                     1 + 0.3 * np.exp(-8 * ((x - 1) ** 2 + (y - 0.75) ** 2))], axis=1)
+    orc = O.Euler2D(grid, geometry="planar", gamma=cases.GAMMA, flux="hllc", limiter="vanLeer",
+                    reconstruction="perVariable", time_integrator="ssprk2", cfl=0.6, time_stepping="local",
+                    boundaries=cfg.boundaries)
+    orc.set_primitives(prim)
+    orc.iterate(20)
+    gpu = GpuEuler2D.create(ctx, cfg)
+    gpu.setCellPrimitives(prim)
+    gpu.iterate(20)
+    p = gpu_primitives(gpu.readState(), cases.GAMMA)[:, 3]
+    assert np.abs(p - orc.primitives()[:, 3]).max() < 1e-5
+
+
+def test_thin_cells_with_the_flow_along_the_short_side(ctx):
+    """euler2d-gpu.test.ts:451-474: a density pulse carried along the short
+    side of 10 x 100 cells of a periodic unit square (v = 1, CFL 0.9,
+    global steps) to t = 0.25 stays stable and where the exact solution
+    puts it, within 1e-2."""
+    from bowshock_amd import grids
+
+    grid = grids.rectangular_grid(10, 100, 0.0, 1.0, 0.0, 1.0)
+    cfg = _planar(grid, cfl=0.9, boundaries={s: {"kind": "periodic"} for s in ("iMin", "iMax", "jMin", "jMax")})
+    yc = (np.arange(100) + 0.5) / 100
+
+    def pulse(y):
+        return 1 + 0.2 * np.exp(-100 * (y - 0.5) ** 2)
+
+    prim = np.zeros((100, 10, 4))
+    prim[..., 0] = pulse(yc)[:, None]
+    prim[..., 2] = 1.0
+    prim[..., 3] = 1.0
+    gpu = GpuEuler2D.create(ctx, cfg)
+    gpu.setCellPrimitives(prim.reshape(-1, 4))
+    gpu.advanceTo(0.25)
+    rho = gpu.readState().rho.reshape(100, 10)[:, 5]
+    exact = pulse((yc - 0.25 + 1) % 1)
+    assert np.all(np.isfinite(rho)) and np.abs(rho - exact).max() < 1e-2
