@@ -389,3 +389,13 @@ def test_linear_neighbour_term_is_the_flux_jacobian():
     q = np.array([1.2, 0.6, -0.3, 2.9])
     lin = O.neighbour_term_linear(q, q, 1.4, 0.6, 0.8, 1.0, 0.0, 1.0)
     assert np.allclose(lin, 0.5 * _normal_flux(q, 0.6, 0.8, 1.4), rtol=1e-14, atol=1e-14)
+
+
+def test_taylor_maccoll_transonic_cone_value():
+    """tests/analysis/transonic-drag.test.ts:173-176: Taylor-Maccoll's
+    surface pressure coefficient of the 6 deg 59' cone at Mach 1.1 (its
+    nose drag coefficient, the cone's pressure being uniform) is 0.0944976
+    to six decimals."""
+    flow = grids.taylor_maccoll(1.1, (6 + 59 / 60) * DEG, GAMMA)
+    print("Cp", flow.surface_pressure_coefficient)
+    assert abs(flow.surface_pressure_coefficient - 0.0944976) < 5e-7
