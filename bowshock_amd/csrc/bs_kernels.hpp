@@ -226,9 +226,12 @@ __device__ __forceinline__ void tile_body(const TileArgsT<T>& A, int bx, int by,
   // The FP64 stage tiles read the time step's face area vectors from the
   // metric arrays instead of staging them: 9 KB less LDS, so four tiles
   // fit on a CU instead of three.
+  // The implicit tiles stage lambda A alone: one plane, which keeps the
+  // FP64 implicit tile at 40640 B of LDS, four to a CU.
   constexpr bool XLDS = MODE == TILE_IMPLICIT || !std::is_same<T, double>::value;
-  __shared__ T xi[2][TY][TX + 1];    // i-face area vector (stage) or lambda A (implicit)
-  __shared__ T xj[2][TY + 1][TX];
+  constexpr int XP = MODE == TILE_IMPLICIT ? 1 : 2;
+  __shared__ T xi[XP][TY][TX + 1];    // i-face area vector (stage) or lambda A (implicit)
+  __shared__ T xj[XP][TY + 1][TX];
   __shared__ double red[NT / 64];
 
   const GridDevT<T>& G = A.G;
@@ -353,7 +356,7 @@ __device__ __forceinline__ void tile_body(const TileArgsT<T>& A, int bx, int by,
       fi[1][fy][fx] = f.y * area;
       fi[2][fy][fx] = f.z * area;
       fi[3][fy][fx] = f.w * area;
-      if (MODE == TILE_IMPLICIT) {
+      if constexpr (MODE == TILE_IMPLICIT) {
         xi[0][fy][fx] = lam;
       } else if constexpr (XLDS) {
         xi[0][fy][fx] = nx * area;
@@ -384,7 +387,7 @@ __device__ __forceinline__ void tile_body(const TileArgsT<T>& A, int bx, int by,
       fj[1][fy][fx] = f.y * area;
       fj[2][fy][fx] = f.z * area;
       fj[3][fy][fx] = f.w * area;
-      if (MODE == TILE_IMPLICIT) {
+      if constexpr (MODE == TILE_IMPLICIT) {
         xj[0][fy][fx] = lam;
       } else if constexpr (XLDS) {
         xj[0][fy][fx] = nx * area;
@@ -422,7 +425,7 @@ __device__ __forceinline__ void tile_body(const TileArgsT<T>& A, int bx, int by,
       double rv = (double)r[0] / (double)vol;
       r2 = rv * rv;
     }
-    if (MODE == TILE_STAGE) {
+    if constexpr (MODE == TILE_STAGE) {
       T dt;
       if (A.dt_mode == DT_COMPUTE_LOCAL) {
         // euler2d.ts:1183-1205 computeTimeSteps: mean area vector of the
