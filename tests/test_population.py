@@ -1,7 +1,10 @@
 """Case generators for batched solves (CPU): the seeded draws, the grid
 sizing rules, refusals, and the vectorized grid generator against the
 scalar one."""
+import json
 import math
+import shutil
+import subprocess
 
 import numpy as np
 import pytest
@@ -21,6 +24,42 @@ def test_mulberry32_is_a_32_bit_generator():
     assert np.all(np.mod(xs * 2 ** 32, 1) == 0)
     assert abs(xs.mean() - 0.5) < 0.01 and len(np.unique(xs)) == len(xs)
     assert P.mulberry32(1)() != P.mulberry32(2)()
+
+
+JS_MULBERRY32 = """
+// Tommy Ettinger's published Mulberry32, in JavaScript's own 32-bit
+// integer operations (Math.imul, >>>, ^), which population.ts:64-72 uses.
+const seeds = JSON.parse(process.argv[1]);
+const out = seeds.map((seed) => {
+  let s = seed >>> 0;
+  const draws = [];
+  for (let k = 0; k < 2000; k++) {
+    s = (s + 0x6d2b79f5) | 0;
+    let z = Math.imul(s ^ (s >>> 15), 1 | s);
+    z = (z + Math.imul(z ^ (z >>> 7), 61 | z)) ^ z;
+    draws.push(((z ^ (z >>> 14)) >>> 0) / 4294967296);
+  }
+  return draws;
+});
+console.log(JSON.stringify(out));
+"""
+
+
+@pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")
+def test_mulberry32_matches_javascript_integer_semantics():
+    """The Python draw emulates JavaScript's signed 32-bit Math.imul, the
+    unsigned shifts and the double-valued sum folded back by ^=; Node runs
+    the generator with the real operations (the state kept as a signed
+    int32 here, as an unsigned one in population.ts: the same bits), for
+    seeds that cross the sign bit, wrap, and come in negative."""
+    seeds = [P.SEED, 0, 1, 0x7FFFFFFF, 0x80000000, 0xFFFFFFFF, -1, -123456789, 2 ** 31 + 12345]
+    r = subprocess.run([shutil.which("node"), "-e", JS_MULBERRY32, json.dumps(seeds)], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    expected = json.loads(r.stdout)
+    for seed, js in zip(seeds, expected):
+        g = P.mulberry32(seed)
+        assert [g() for _ in range(len(js))] == js, seed
 
 
 def test_population_sizes_follow_the_rule():
