@@ -1227,14 +1227,12 @@ static bs_status encode_implicit(bs_solver* s, const StateDevT<float>& res, cons
   U.control = s->d.control;
   U.cfl_max = cfl_max;
   U.growth = growth;
-  // dq_only: the correction dQ alone (the accelerated steps mix it first).
-  const bool fused = std::is_same<T, float>::value && !dq_only;
-  UpdateArgs Uf;
-  if constexpr (std::is_same<T, float>::value) Uf = U;
-  else memset(&Uf, 0, sizeof Uf);
+  // dq_only: the correction dQ alone (the accelerated steps mix it first);
+  // otherwise the last relaxation launch writes the updated state.
+  const bool fused = !dq_only;
   // The 2 sweeps - 1 half-sweeps after the first red one, up to three per
   // launch (relax_tile_kernel), starting with black.
-  using K = void (*)(RelaxArgsT<T>, int, int, UpdateArgs, int);
+  using K = void (*)(RelaxArgsT<T>, int, int, UpdateArgsT<T>, int);
   K kernels[2][3];
   if constexpr (std::is_same<T, float>::value) {
     kernels[0][0] = relax_tile_kernel<T, 1, false>;
@@ -1252,7 +1250,7 @@ static bs_status encode_implicit(bs_solver* s, const StateDevT<float>& res, cons
     const int nh = halves - done < 3 ? halves - done : 3;
     const int init = done == 0, last = fused && done + nh == halves;
     hipEvent_t* ev = timing_begin(s, 2);
-    hipLaunchKernelGGL(kernels[form][nh - 1], rgrid, dim3(RTX * RTY), 0, s->stream, R, colour, init, Uf, last);
+    hipLaunchKernelGGL(kernels[form][nh - 1], rgrid, dim3(RTX * RTY), 0, s->stream, R, colour, init, U, last);
     timing_end(s, ev);
     colour = (colour + nh) & 1;
     done += nh;
@@ -1260,11 +1258,6 @@ static bs_status encode_implicit(bs_solver* s, const StateDevT<float>& res, cons
   if (dq_only) {
     HIP_TRY(hipGetLastError(), "implicit launch");
     return BS_OK;
-  }
-  if (!fused) {
-    hipEvent_t* ev = timing_begin(s, 3);
-    hipLaunchKernelGGL(update_kernel<T>, dim3((s->n + 255) / 256), dim3(256), 0, s->stream, U);
-    timing_end(s, ev);
   }
   HIP_TRY(hipGetLastError(), "implicit launch");
   s->cur = dst;

@@ -697,24 +697,6 @@ __device__ __forceinline__ void advance_control(const UpdateArgsT<S>& U) {
   ctl->steps += 1;
 }
 
-// The update as a kernel of its own, one thread per cell (the FP64
-// solver's; FP32 steps fuse it into the last relaxation launch).
-template <typename S>
-__global__ __launch_bounds__(256) void update_kernel(UpdateArgsT<S> U) {
-  const int c = blockIdx.x * 256 + threadIdx.x;
-  bool bad = false;
-  if (c < U.n) {
-    const Vec4<S> o = update_cell<S>(U.src.q[0][c], U.src.q[1][c], U.src.q[2][c], U.src.q[3][c], (S)U.dq.q[0][c],
-                                     (S)U.dq.q[1][c], (S)U.dq.q[2][c], (S)U.dq.q[3][c], U.gamma);
-    U.dst.q[0][c] = o.x;
-    U.dst.q[1][c] = o.y;
-    U.dst.q[2][c] = o.z;
-    U.dst.q[3][c] = o.w;
-    bad = !is_physical(o, U.gamma);
-  }
-  flag_non_physical(bad, U.control);
-  if (blockIdx.x == 0 && threadIdx.x == 0) advance_control(U);
-}
 
 // NH consecutive half-sweeps (colours colour0, colour0 ^ 1, ...) of a
 // 64 x 8 tile in one launch. Gauss-Seidel by colours: a cell's update reads
@@ -732,10 +714,9 @@ struct RelaxCtx {
 
 template <typename T, int NH, bool LIN>
 __device__ __forceinline__ void relax_tile_body(const RelaxArgsT<T>& A, int bx, int by, int colour0, int init_red,
-                                                const UpdateArgs& U, bool fuse_update) {
+                                                const UpdateArgsT<T>& U, bool fuse_update) {
   static_assert(LIN || std::is_same<T, float>::value, "an FP64 state relaxes with the linearized term");
-  // Only an FP32 state is updated in place of the last dQ write.
-  const bool fuse = std::is_same<T, float>::value && fuse_update;
+  const bool fuse = fuse_update;
   constexpr int WX = RTX + 2 * NH, WY = RTY + 2 * NH, NT = RTX * RTY;
   // Window cells [0, WX) x [0, WY) hold q, dQ, the residual and the
   // diagonal; the cells updated by some half-sweep lie in [1, WX - 1) x
@@ -873,9 +854,15 @@ __device__ __forceinline__ void relax_tile_body(const RelaxArgsT<T>& A, int bx, 
     const int c = gi + gj * ni;
     const int wx = lx + NH, wy = ly + NH;
     if (fuse) {
-      const Vec4<float> o = update_cell<float>(sq[0][wy][wx], sq[1][wy][wx], sq[2][wy][wx], sq[3][wy][wx],
-                                               sdq[0][wy][wx], sdq[1][wy][wx], sdq[2][wy][wx], sdq[3][wy][wx],
-                                               U.gamma);
+      // An FP64 state is updated from its own FP64 values (the window
+      // holds them rounded to FP32).
+      Vec4<T> o;
+      if constexpr (std::is_same<T, float>::value)
+        o = update_cell<float>(sq[0][wy][wx], sq[1][wy][wx], sq[2][wy][wx], sq[3][wy][wx], sdq[0][wy][wx],
+                               sdq[1][wy][wx], sdq[2][wy][wx], sdq[3][wy][wx], U.gamma);
+      else
+        o = update_cell<T>(U.src.q[0][c], U.src.q[1][c], U.src.q[2][c], U.src.q[3][c], (T)sdq[0][wy][wx],
+                           (T)sdq[1][wy][wx], (T)sdq[2][wy][wx], (T)sdq[3][wy][wx], U.gamma);
       U.dst.q[0][c] = o.x;
       U.dst.q[1][c] = o.y;
       U.dst.q[2][c] = o.z;
@@ -891,7 +878,7 @@ __device__ __forceinline__ void relax_tile_body(const RelaxArgsT<T>& A, int bx, 
 
 template <typename T, int NH, bool LIN>
 __global__ __launch_bounds__(RTX * RTY) void relax_tile_kernel(RelaxArgsT<T> A, int colour0, int init_red,
-                                                             UpdateArgs U, int fuse_update) {
+                                                             UpdateArgsT<T> U, int fuse_update) {
   // One-dimensional grid, tiles placed per XCD as in tile_kernel.
   const int tiles_x = (A.G.ni + RTX - 1) / RTX;
   const int t = xcd_tile(blockIdx.x, gridDim.x);
