@@ -14,12 +14,18 @@
 // large tile would leave most of its threads idle; the threshold is set by
 // the caller's autotuner (bs_batch_set_tiling). Each step of the active set
 // is captured as HIP graphs; the work lists and graphs are rebuilt when
-// the active set or the tiling changes. Included by bs_engine.hip.
+// the active set or the tiling changes.
+
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "bs_batch_tiles.hpp"
+
+using namespace bs;
 
 namespace {
-
-constexpr int BTX_L = 64, BTY_L = 8;  // large tile
-constexpr int BTX_S = 32, BTY_S = 4;  // small tile
 
 struct BatchCase {
   int ni, nj, n;
@@ -36,40 +42,6 @@ enum BatchRole {
   ROLE_PREPARE = 12, // + cur
   N_TILE_ROLES = 15
 };
-
-template <int TX_, int TY_, int REC, int LIM, int FLUX>
-void* pick_batch_mode(int mode) {
-  if (mode == TILE_STAGE) return (void*)batch_tile_kernel<TX_, TY_, REC, LIM, FLUX, TILE_STAGE>;
-  if (mode == TILE_IMPLICIT) return (void*)batch_tile_kernel<TX_, TY_, REC, LIM, FLUX, TILE_IMPLICIT>;
-  return (void*)batch_tile_kernel<TX_, TY_, REC, LIM, FLUX, TILE_RESIDUAL>;
-}
-
-template <int TX_, int TY_, int FLUX>
-void* pick_batch_rec(int rec, int lim, int mode) {
-  if (rec == SHARED) return pick_batch_mode<TX_, TY_, SHARED, VAN_ALBADA, FLUX>(mode);
-  if (rec == SHARED_SCALED) return pick_batch_mode<TX_, TY_, SHARED_SCALED, VAN_ALBADA, FLUX>(mode);
-  switch (lim) {
-    case FIRST_ORDER: return pick_batch_mode<TX_, TY_, PER_VARIABLE, FIRST_ORDER, FLUX>(mode);
-    case MINMOD: return pick_batch_mode<TX_, TY_, PER_VARIABLE, MINMOD, FLUX>(mode);
-    case VAN_LEER: return pick_batch_mode<TX_, TY_, PER_VARIABLE, VAN_LEER, FLUX>(mode);
-    case VAN_ALBADA: return pick_batch_mode<TX_, TY_, PER_VARIABLE, VAN_ALBADA, FLUX>(mode);
-    case MONOTONIZED_CENTRAL: return pick_batch_mode<TX_, TY_, PER_VARIABLE, MONOTONIZED_CENTRAL, FLUX>(mode);
-    default: return pick_batch_mode<TX_, TY_, PER_VARIABLE, UNLIMITED_CENTRAL, FLUX>(mode);
-  }
-}
-
-using BatchTileFn = void (*)(const TileArgs*, const WorkItem*);
-
-BatchTileFn batch_tile_fn(const bs_config& c, int cls, int mode) {
-  void* f;
-  if (cls == 0)
-    f = c.flux == BS_FLUX_RUSANOV ? pick_batch_rec<BTX_L, BTY_L, RUSANOV>(c.reconstruction, c.limiter, mode)
-                                  : pick_batch_rec<BTX_L, BTY_L, HLLC>(c.reconstruction, c.limiter, mode);
-  else
-    f = c.flux == BS_FLUX_RUSANOV ? pick_batch_rec<BTX_S, BTY_S, RUSANOV>(c.reconstruction, c.limiter, mode)
-                                  : pick_batch_rec<BTX_S, BTY_S, HLLC>(c.reconstruction, c.limiter, mode);
-  return (BatchTileFn)f;
-}
 
 using BatchWallFn = void (*)(const WallArgs*, const WallBoundary*, const WorkItem*);
 
@@ -380,7 +352,7 @@ bs_status batch_launch_tiles(bs_batch* b, int mode, int role) {
   for (int cl = 0; cl < 2; cl++) {
     if (b->n_tiles[cl] == 0) continue;
     const int threads = cl == 0 ? BTX_L * BTY_L : BTX_S * BTY_S;
-    hipLaunchKernelGGL(batch_tile_fn(b->scheme, cl, mode), dim3(b->n_tiles[cl]), dim3(threads), 0, b->stream, recs,
+    hipLaunchKernelGGL(cl == 0 ? batch_tile_fn_large(b->scheme, mode) : batch_tile_fn_small(b->scheme, mode), dim3(b->n_tiles[cl]), dim3(threads), 0, b->stream, recs,
                        b->work_tiles[cl]);
   }
   HIP_TRY(hipGetLastError(), "batch stencil launch");

@@ -17,16 +17,17 @@
 #include <type_traits>
 #include <vector>
 
-#include "../../include/bowshock.h"
-#include "bs_kernels.hpp"
+#include "bs_internal.hpp"
 #include "bs_anderson.hpp"
 #include "bs_linegrid.hpp"
 
 using namespace bs;
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
+
+namespace bs {
 
 bs_status fail(bs_status code, const char* fmt, ...) {
   char buf[1024];
@@ -43,41 +44,8 @@ bs_status hip_fail(hipError_t e, const char* what) {
   return fail(code, "%s: %s", what, hipGetErrorString(e));
 }
 
-#define HIP_TRY(expr, what)                       \
-  do {                                            \
-    hipError_t _e = (expr);                       \
-    if (_e != hipSuccess) return hip_fail(_e, what); \
-  } while (0)
+}  // namespace bs
 
-constexpr int BATCH = 256;  // steps between status checks (time-loop.ts:13)
-constexpr int GRAPH_STEPS = 12;  // steps per captured graph (a multiple of 3)
-// Tile of the stencil kernels: TX cells along i (one wavefront row when 64)
-// by TY along j; set at build time.
-#ifndef BS_TILE_X
-#define BS_TILE_X 64
-#endif
-#ifndef BS_TILE_Y
-#define BS_TILE_Y 8
-#endif
-constexpr int TX = BS_TILE_X, TY = BS_TILE_Y;
-static_assert(TX * TY <= 1024 && TX * TY >= TX + TY && (TX * TY) % 64 == 0, "tile shape");
-
-// Tile per state precision: FP64 halves the tile height so the doubled
-// register footprint of a thread still fits without spilling.
-template <typename T>
-struct Tile {
-  static constexpr int x = TX, y = TY;
-};
-template <>
-struct Tile<double> {
-  static constexpr int x = 64, y = 4;
-};
-
-}  // namespace
-
-struct bs_ctx {
-  int device;
-};
 
 // Device arrays of one solver. The untyped arrays hold values of the
 // solver's state precision (float, or double for an FP64 solver).
@@ -110,18 +78,6 @@ struct Timing {
   int used = 0;
   double total_ms[4] = {0, 0, 0, 0};
   long long launches[4] = {0, 0, 0, 0};
-};
-
-// A captured run of GRAPH_STEPS steps starting from state buffer `cur`;
-// the run returns to the same buffer (RK stages write the last stage in
-// place; single-stage and implicit steps cycle the three buffers, and
-// GRAPH_STEPS is a multiple of 3).
-struct StepGraph {
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  int cur = -1;
-  int sweeps = 0;
-  double omega = 0, growth = 0, cfl_max = 0;
 };
 
 struct bs_solver {
@@ -232,58 +188,6 @@ bs_status upload_boundary(bs_solver* s) {
   return s->fp64 ? upload_boundary_t<double>(s) : upload_boundary_t<float>(s);
 }
 
-// ---- kernel dispatch tables -------------------------------------------------
-
-template <typename T>
-using TileFnT = void (*)(TileArgsT<T>);
-
-template <typename T, int REC, int LIM, int FLUX>
-TileFnT<T> pick_mode(int mode) {
-  constexpr int X = Tile<T>::x, Y = Tile<T>::y;
-  if constexpr (std::is_same<T, float>::value) {
-    if (mode == TILE_STAGE) return tile_kernel_occ8<X, Y, REC, LIM, FLUX, TILE_STAGE>;
-  } else {
-    if (mode == TILE_STAGE) return tile_kernel<T, X, Y, REC, LIM, FLUX, TILE_STAGE>;
-  }
-  if (mode == TILE_IMPLICIT) return tile_kernel<T, X, Y, REC, LIM, FLUX, TILE_IMPLICIT>;
-  return tile_kernel<T, X, Y, REC, LIM, FLUX, TILE_RESIDUAL>;
-}
-
-template <typename T, int FLUX>
-TileFnT<T> pick_rec(int rec, int lim, int mode) {
-  if (rec == SHARED) return pick_mode<T, SHARED, VAN_ALBADA, FLUX>(mode);
-  if (rec == SHARED_SCALED) return pick_mode<T, SHARED_SCALED, VAN_ALBADA, FLUX>(mode);
-  switch (lim) {
-    case FIRST_ORDER: return pick_mode<T, PER_VARIABLE, FIRST_ORDER, FLUX>(mode);
-    case MINMOD: return pick_mode<T, PER_VARIABLE, MINMOD, FLUX>(mode);
-    case VAN_LEER: return pick_mode<T, PER_VARIABLE, VAN_LEER, FLUX>(mode);
-    case VAN_ALBADA: return pick_mode<T, PER_VARIABLE, VAN_ALBADA, FLUX>(mode);
-    case MONOTONIZED_CENTRAL: return pick_mode<T, PER_VARIABLE, MONOTONIZED_CENTRAL, FLUX>(mode);
-    default: return pick_mode<T, PER_VARIABLE, UNLIMITED_CENTRAL, FLUX>(mode);
-  }
-}
-
-template <typename T>
-TileFnT<T> tile_fn(const bs_config& c, int mode) {
-  return c.flux == BS_FLUX_RUSANOV ? pick_rec<T, RUSANOV>(c.reconstruction, c.limiter, mode)
-                                   : pick_rec<T, HLLC>(c.reconstruction, c.limiter, mode);
-}
-
-template <typename S>
-using WallFnT = void (*)(WallArgsT<S>, WallBoundaryT<S>);
-
-template <typename S>
-WallFnT<S> wall_fn(const bs_config& c) {
-  if (c.flux == BS_FLUX_RUSANOV) {
-    if (c.reconstruction == SHARED) return wall_flux_kernel<S, SHARED, RUSANOV>;
-    if (c.reconstruction == SHARED_SCALED) return wall_flux_kernel<S, SHARED_SCALED, RUSANOV>;
-    return wall_flux_kernel<S, PER_VARIABLE, RUSANOV>;
-  }
-  if (c.reconstruction == SHARED) return wall_flux_kernel<S, SHARED, HLLC>;
-  if (c.reconstruction == SHARED_SCALED) return wall_flux_kernel<S, SHARED_SCALED, HLLC>;
-  return wall_flux_kernel<S, PER_VARIABLE, HLLC>;
-}
-
 // ---- timing -------------------------------------------------------------------
 
 hipEvent_t* timing_pair(bs_solver* s, int kind, long long weight) {
@@ -356,18 +260,6 @@ bs_status launch_tile(bs_solver* s, int mode, const TileArgsT<T>& A, int kind) {
   return BS_OK;
 }
 
-// Runge-Kutta coefficients (runge-kutta.ts:17-28) in the state's precision.
-template <typename T>
-int rk_table(int integrator, const T** a, const T** b) {
-  static const T ea[] = {T(0)}, eb[] = {T(1)};
-  static const T a2[] = {T(0), T(0.5)}, b2[] = {T(1), T(0.5)};
-  static const T a3[] = {T(0), T(0.75), T(1.0 / 3.0)}, b3[] = {T(1), T(0.25), T(2.0 / 3.0)};
-  if (integrator == BS_RK_SSPRK2) { *a = a2; *b = b2; return 2; }
-  if (integrator == BS_RK_SSPRK3) { *a = a3; *b = b3; return 3; }
-  *a = ea; *b = eb;
-  return 1;
-}
-
 // One explicit step: the time step (global stepping: per-cell steps, their
 // minimum clamped to t_end, then the clock), then the stages. Stage k reads
 // `src` with the stencil and u0 at its own cell only, so the last stage of a
@@ -424,12 +316,6 @@ bs_status encode_step_t(bs_solver* s) {
 }
 
 bs_status encode_step(bs_solver* s) { return s->fp64 ? encode_step_t<double>(s) : encode_step_t<float>(s); }
-
-void drop_graph(StepGraph& g) {
-  if (g.exec) (void)hipGraphExecDestroy(g.exec);
-  if (g.graph) (void)hipGraphDestroy(g.graph);
-  g = StepGraph();
-}
 
 // Capture GRAPH_STEPS steps that `encode` enqueues into one graph. Nothing
 // runs during the capture, so the buffer index is restored afterwards.
@@ -523,16 +409,6 @@ bs_status reset_control(bs_solver* s) {
   return BS_OK;
 }
 
-template <typename T>
-bs_status dev_alloc(T** p, size_t count, const char* what) {
-  hipError_t e = hipMalloc((void**)p, count * sizeof(T) > 0 ? count * sizeof(T) : 16);
-  if (e != hipSuccess) {
-    *p = nullptr;
-    return fail(BS_ERR_MEMORY, "allocating %s (%zu bytes): %s", what, count * sizeof(T), hipGetErrorString(e));
-  }
-  return BS_OK;
-}
-
 void free_all(bs_solver* s) {
   drop_graph(s->explicit_graph);
   drop_graph(s->implicit_graph);
@@ -557,7 +433,38 @@ void free_all(bs_solver* s) {
   s->stream = nullptr;
 }
 
-// structured.ts:94-168 computeMetrics, FP64 on the host.
+// `count` values of the solver's state precision.
+bs_status real_alloc(const bs_solver* s, void** p, size_t count, const char* what) {
+  const size_t bytes = count * real_size(s);
+  hipError_t e = hipMalloc(p, bytes > 0 ? bytes : 16);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(BS_ERR_MEMORY, "allocating %s (%zu bytes): %s", what, bytes, hipGetErrorString(e));
+  }
+  return BS_OK;
+}
+
+// Copies FP64 host values to `dst` in the solver's state precision.
+bs_status copy_real(const bs_solver* s, void* dst, const double* src, size_t count, const char* what) {
+  if (s->fp64) {
+    HIP_TRY(hipMemcpy(dst, src, count * sizeof(double), hipMemcpyHostToDevice), what);
+    return BS_OK;
+  }
+  std::vector<float> f(src, src + count);
+  HIP_TRY(hipMemcpy(dst, f.data(), count * sizeof(float), hipMemcpyHostToDevice), what);
+  return BS_OK;
+}
+
+bs_status upload_real(const bs_solver* s, void** dst, const std::vector<double>& src, const char* what) {
+  bs_status st = real_alloc(s, dst, src.size(), what);
+  if (st) return st;
+  return copy_real(s, *dst, src.data(), src.size(), what);
+}
+
+}  // namespace
+
+namespace bs {
+
 bs_status metrics(int ni, int nj, const double* x, const double* y, int axisym, double* inx, double* iny,
                   double* ia, double* jnx, double* jny, double* ja, double* vol, double* planar, double* cxo,
                   double* cyo) {
@@ -600,50 +507,7 @@ bs_status metrics(int ni, int nj, const double* x, const double* y, int axisym, 
   return BS_OK;
 }
 
-// `count` values of the solver's state precision.
-bs_status real_alloc(const bs_solver* s, void** p, size_t count, const char* what) {
-  const size_t bytes = count * real_size(s);
-  hipError_t e = hipMalloc(p, bytes > 0 ? bytes : 16);
-  if (e != hipSuccess) {
-    *p = nullptr;
-    return fail(BS_ERR_MEMORY, "allocating %s (%zu bytes): %s", what, bytes, hipGetErrorString(e));
-  }
-  return BS_OK;
-}
-
-// Copies FP64 host values to `dst` in the solver's state precision.
-bs_status copy_real(const bs_solver* s, void* dst, const double* src, size_t count, const char* what) {
-  if (s->fp64) {
-    HIP_TRY(hipMemcpy(dst, src, count * sizeof(double), hipMemcpyHostToDevice), what);
-    return BS_OK;
-  }
-  std::vector<float> f(src, src + count);
-  HIP_TRY(hipMemcpy(dst, f.data(), count * sizeof(float), hipMemcpyHostToDevice), what);
-  return BS_OK;
-}
-
-bs_status upload_real(const bs_solver* s, void** dst, const std::vector<double>& src, const char* what) {
-  bs_status st = real_alloc(s, dst, src.size(), what);
-  if (st) return st;
-  return copy_real(s, *dst, src.data(), src.size(), what);
-}
-
-// Conserved state of a primitive one with the gamma `g` the state is
-// formed with (the FP32 gamma for an FP32 state, as the reference GPU
-// engine packs states, euler2d-gpu.ts:412-432), rounded to T at the end.
-template <typename T>
-void pack_cons(const double* w, double g, T out[4]) {
-  out[0] = (T)w[0];
-  out[1] = (T)(w[0] * w[1]);
-  out[2] = (T)(w[0] * w[2]);
-  out[3] = (T)(w[3] / (g - 1) + 0.5 * w[0] * (w[1] * w[1] + w[2] * w[2]));
-}
-
-bool physical(const double* w) {
-  return w[0] > 0 && w[3] > 0 && std::isfinite(w[1]) && std::isfinite(w[2]);
-}
-
-}  // namespace
+}  // namespace bs
 
 // =============================================================================
 // C ABI
@@ -1814,5 +1678,3 @@ bs_status bs_kernel_timing(bs_solver* s, int32_t kernel, double* ms_per_launch, 
 
 }  // extern "C"
 
-#include "bs_batch.hpp"
-#include "bs_mlp.hpp"

@@ -558,7 +558,7 @@ __global__ __launch_bounds__(TX * TY) void batch_tile_kernel(const TileArgs* __r
 
 // Deterministic FP64 sum of n partials by one block; writes sqrt(sum / cells)
 // when `rms` (euler2d.ts:480-491) or the plain sum.
-__global__ __launch_bounds__(256) void finalize_sum(const double* partials, int n, double cells, int rms,
+static __global__ __launch_bounds__(256) void finalize_sum(const double* partials, int n, double cells, int rms,
                                                    double* out) {
   __shared__ double red[4];
   double v = 0.0;
@@ -630,7 +630,7 @@ __global__ __launch_bounds__(256) void finalize_time_step(const T* partial_min, 
 
 // After the last stage of a global step: FP64 clock and step count; a step
 // of zero length (already at t_end) is not counted.
-__global__ void advance_clock(Control* ctl) {
+static __global__ void advance_clock(Control* ctl) {
   if (ctl->dt > 0.0) {
     ctl->time += ctl->dt;
     double limit = 1e-12 * (fabs(ctl->t_end) > 1.0 ? fabs(ctl->t_end) : 1.0);

@@ -7,13 +7,22 @@
 // in_l to out_l with Z_l = A_{l-1} W_l + b_l, A_l = SiLU(Z_l); W_l is stored
 // in-major [in_l][out_l]. Every dimension the GEMMs see is a multiple of 64
 // (the input width is zero-padded to 64), so the kernels need no edge
-// handling. Included by bs_engine.hip.
+// handling.
 //
 // GEMM: C[M x N] = A[M x K] B[K x N], A and B BF16 with element strides
 // (row-major or transposed views), a 64 x 64 block tile in four wavefronts
 // of 32 x 32, K steps of 32 staged in LDS, v_mfma_f32_16x16x32_bf16 with
 // FP32 accumulators, and a fused epilogue (bias, SiLU, SiLU' for the
 // backward pass, FP32 and/or BF16 stores).
+
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "bs_internal.hpp"
+
+using namespace bs;
 
 namespace {
 
