@@ -43,6 +43,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # B/s, MI355X
+# Steps per captured HIP graph in the library (bs_engine.hip GRAPH_STEPS):
+# warm-up runs at least two graphs' worth so that the capture and
+# instantiation happen before the timed region, whatever --warmup is.
+GRAPH_STEPS = 12
+# Steps of the kernel-timing pass, fixed so the per-launch figure does not
+# depend on --steps.
+TIMING_STEPS = 1200
 # Algorithmic HBM bytes per cell-update of SSP-RK3 with local steps: stage 1
 # 60 B, stages 2 and 3 76 B each (state 16 B per copy, metrics 24 B, dt 4 B).
 BYTES_PER_CU_RK3 = 212.0
@@ -70,6 +77,20 @@ def parse():
     ap.add_argument("--rockets", type=int, default=2000, help="population: rockets drawn (4 Mach numbers each)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="batched CPU baseline: seconds of oracle work")
     return ap.parse_args()
+
+
+def host_info():
+    """The host's logical CPU count and CPU model (os.cpu_count and
+    /proc/cpuinfo, the figures lscpu reports)."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "model": model}
 
 
 def read_traffic(path):
@@ -123,12 +144,15 @@ def main():
     gpu.setCellPrimitives(np.array(free))
     if a.no_graphs:
         gpu.enableGraphs(False)
-    gpu.iterate(a.warmup)
+    # Warm-up: at least two graphs of steps, so the step graph is captured
+    # and instantiated before the timed region, then one call of the timed
+    # region's own length (its eager remainder's launches warmed too).
+    gpu.iterate(max(a.warmup, 2 * GRAPH_STEPS))
+    gpu.iterate(a.steps % GRAPH_STEPS + GRAPH_STEPS)
     cells = a.ni * a.nj
 
     # Timed region: exactly `steps` steps, synchronised on both sides
     # (bs_iterate returns after the device has finished its last batch).
-    gpu.iterate(1)
     t = time.perf_counter()
     gpu.iterate(a.steps)
     elapsed = time.perf_counter() - t
@@ -141,7 +165,7 @@ def main():
     # batch of steps (explicit local steps launch the stage kernel only),
     # graphs on as in the timed region.
     gpu.enableTiming("region")
-    gpu.iterate(min(a.steps, 1200))
+    gpu.iterate(TIMING_STEPS)
     ms_launch, launches = gpu.kernelTiming("stage")
     gpu.enableTiming(False)
     width = 2 if a.precision == "fp64" else 1   # every algorithmic byte doubles in FP64
@@ -169,7 +193,7 @@ def main():
         ce = time.perf_counter() - t
         cpu = {"value": cells * a.cpu_steps / ce, "unit": "cell-updates/s", "cores": 1, "kind": "port",
                "sample": f"{a.cpu_steps} SSP-RK3 steps of the same {a.ni}x{a.nj} case on one core, "
-                         f"FP64 C oracle (gcc -O3), {ce:.1f}s"}
+                         f"FP64 C oracle (gcc -O3), {ce:.1f}s", "host": host_info()}
         print(f"cpu baseline {cpu['value']:.3e} CU/s ({ce:.1f}s)", file=sys.stderr, flush=True)
 
     line = {
@@ -361,7 +385,7 @@ def _cpu_baseline(S, implicit, seconds, cu_per_solve):
     cpu_cu = sum(o[0] for o in out)
     rate = cpu_cu / cpu_wall
     print(f"cpu baseline {rate:.3e} CU/s on {workers} cores", file=sys.stderr, flush=True)
-    return {"value": rate / cu_per_solve, "unit": "solves/s", "cores": workers, "kind": "port",
+    return {"value": rate / cu_per_solve, "unit": "solves/s", "cores": workers, "kind": "port", "host": host_info(),
             "sample": f"FP64 C oracle, {workers} processes, one case each ({len(jobs)} cases spread over the "
                       f"batch), {cpu_cu:.3e} cell-updates in {cpu_wall:.1f}s = {rate:.3e} CU/s; solves/s = "
                       f"that rate / the GPU run's {cu_per_solve:.3e} cell-updates per converged solve"}
@@ -807,6 +831,28 @@ def run_implicit(a):
     achieved = relax_bytes / (ms_relax * 1e-3)
     print(f"{steps} implicit steps in {elapsed:.3f}s: {1e6 * elapsed / steps:.1f} us/step, {value:.3e} CU/s; "
           + ", ".join(f"{k} {v[0] * 1e3:.2f} us" for k, v in timing.items()), file=sys.stderr, flush=True)
+    cpu = None
+    if a.cpu_steps > 0:
+        # The oracle's red-black LU-SGS (2 sweeps, the same CFL ramp) on one
+        # core for a bounded number of steps of the same case.
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O
+
+        orc = O.Euler2D(bg.grid, geometry="axisymmetric", gamma=1.4, flux="hllc", limiter="vanAlbada",
+                        reconstruction="sharedScaled", time_integrator="ssprk3", cfl=0.8, time_stepping="local",
+                        boundaries=bcs)
+        orc.set_primitives(np.array(free))
+        cpu_steps = max(2, a.cpu_steps // 4)
+        cfl = 1.0
+        t = time.perf_counter()
+        for _ in range(cpu_steps):
+            orc.iterate_implicit(1, cfl, 1.5, relaxation="redBlack", sweeps=2)
+            cfl = min(10.0, cfl * 1.2)
+        ce = time.perf_counter() - t
+        cpu = {"value": cells * cpu_steps / ce, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+               "sample": f"{cpu_steps} red-black LU-SGS steps (2 sweeps, CFL 1 -> 10 x1.2) of the same "
+                         f"{a.ni}x{a.nj} case on one core, FP64 C oracle (gcc -O3), {ce:.1f}s", "host": host_info()}
+        print(f"cpu baseline {cpu['value']:.3e} CU/s ({ce:.1f}s)", file=sys.stderr, flush=True)
     line = {
         "metric": "Euler cell-updates/s, %HBM roofline; converged CD solves/s warm-start vs plain",
         "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": steps, "warmup": max(a.warmup // 2, 12),
@@ -823,7 +869,7 @@ def run_implicit(a):
                      "bytes_model": "FP32 operands" + (" (a lower bound: the FP64 state and metrics are read "
                                                        "at twice the width)" if a.precision == "fp64" else ""),
                      "us_per_launch": ms_relax * 1e3},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 

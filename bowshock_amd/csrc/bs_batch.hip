@@ -400,39 +400,14 @@ bs_status batch_encode_implicit(bs_batch* b, int sweeps) {
 
 template <typename Encode>
 bs_status batch_run(bs_batch* b, StepGraph& g, int count, Encode encode) {
-  int k = 0;
   if (count >= GRAPH_STEPS) {
-    if (!(g.exec && g.cur == b->cur)) {
-      drop_graph(g);
-      const int cur0 = b->cur;
-      HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "graph capture");
-      bs_status st = BS_OK;
-      for (int s = 0; s < GRAPH_STEPS && st == BS_OK; s++) st = encode();
-      hipGraph_t graph = nullptr;
-      hipError_t e = hipStreamEndCapture(b->stream, &graph);
-      const int after = b->cur;
-      b->cur = cur0;
-      if (st != BS_OK || e != hipSuccess || after != cur0) {
-        if (graph) (void)hipGraphDestroy(graph);
-        if (st) return st;
-        if (e != hipSuccess) return hip_fail(e, "graph capture");
-        return fail(BS_ERR_EXECUTION, "captured steps do not return to their state buffer");
-      }
-      hipGraphExec_t exec = nullptr;
-      e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-      if (e != hipSuccess) {
-        (void)hipGraphDestroy(graph);
-        return hip_fail(e, "graph instantiate");
-      }
-      g.graph = graph;
-      g.exec = exec;
-      g.cur = cur0;
-    }
-    for (; k + GRAPH_STEPS <= count; k += GRAPH_STEPS) HIP_TRY(hipGraphLaunch(g.exec, b->stream), "graph launch");
-  }
-  for (; k < count; k++) {
-    bs_status st = encode();
+    bs_status st = replay_steps(b->stream, b->cur, g, count, encode);
     if (st) return st;
+  } else {
+    for (int k = 0; k < count; k++) {
+      bs_status st = encode();
+      if (st) return st;
+    }
   }
   HIP_TRY(hipStreamSynchronize(b->stream), "batch steps");
   return BS_OK;

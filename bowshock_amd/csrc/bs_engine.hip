@@ -317,51 +317,15 @@ bs_status encode_step_t(bs_solver* s) {
 
 bs_status encode_step(bs_solver* s) { return s->fp64 ? encode_step_t<double>(s) : encode_step_t<float>(s); }
 
-// Capture GRAPH_STEPS steps that `encode` enqueues into one graph. Nothing
-// runs during the capture, so the buffer index is restored afterwards.
-template <typename Encode>
-bs_status capture_steps(bs_solver* s, StepGraph& g, Encode encode) {
-  drop_graph(g);
-  const int cur0 = s->cur;
-  HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal), "graph capture");
-  bs_status st = BS_OK;
-  for (int k = 0; k < GRAPH_STEPS && st == BS_OK; k++) st = encode();
-  hipGraph_t graph = nullptr;
-  hipError_t e = hipStreamEndCapture(s->stream, &graph);
-  const int cur_after = s->cur;
-  s->cur = cur0;
-  if (st != BS_OK || e != hipSuccess || cur_after != cur0) {
-    if (graph) (void)hipGraphDestroy(graph);
-    if (st != BS_OK) return st;
-    if (e != hipSuccess) return hip_fail(e, "graph capture");
-    return fail(BS_ERR_EXECUTION, "captured steps do not return to their state buffer");
-  }
-  hipGraphExec_t exec = nullptr;
-  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-  if (e != hipSuccess) {
-    (void)hipGraphDestroy(graph);
-    return hip_fail(e, "graph instantiate");
-  }
-  g.graph = graph;
-  g.exec = exec;
-  g.cur = cur0;
-  return BS_OK;
-}
-
-// `count` steps: whole captured graphs where possible (one launch per
-// GRAPH_STEPS steps), the rest launched kernel by kernel. Per-launch kernel
-// timing needs the events between launches, so it runs without graphs.
+// `count` steps: captured graphs (replay_steps) for a call of at least
+// GRAPH_STEPS steps, a shorter call launched kernel by kernel. Per-launch
+// kernel timing needs the events between launches, so it runs without
+// graphs.
 template <typename Encode>
 bs_status run_steps(bs_solver* s, StepGraph& g, int count, Encode encode) {
-  int k = 0;
-  if (s->graphs && (!s->timing.enabled || s->timing.suppress) && count >= GRAPH_STEPS) {
-    if (!(g.exec && g.cur == s->cur)) {
-      bs_status st = capture_steps(s, g, encode);
-      if (st) return st;
-    }
-    for (; k + GRAPH_STEPS <= count; k += GRAPH_STEPS) HIP_TRY(hipGraphLaunch(g.exec, s->stream), "graph launch");
-  }
-  for (; k < count; k++) {
+  if (s->graphs && (!s->timing.enabled || s->timing.suppress) && count >= GRAPH_STEPS)
+    return replay_steps(s->stream, s->cur, g, count, encode);
+  for (int k = 0; k < count; k++) {
     bs_status st = encode();
     if (st) return st;
   }
@@ -1050,10 +1014,10 @@ static bs_status ensure_implicit(bs_solver* s) {
 extern "C++" {
 // One implicit step: prepare (residual, lambda A, diagonal), the
 // red-black half-sweeps (the first red one formed from R and D by the
-// first launch), the update into the next buffer. On an FP32 state the
-// last relaxation launch writes q + alpha dQ itself; on an FP64 state
-// the relaxation runs in FP32 on the rounded state and metrics, and an
-// FP64 update kernel follows.
+// first launch), the update into the next buffer, which the last
+// relaxation launch writes itself: q + alpha dQ from the window's FP32
+// state, or, on an FP64 state (relaxed in FP32 on the rounded state and
+// metrics), from the FP64 state in HBM.
 template <typename T>
 static bs_status encode_implicit(bs_solver* s, const StateDevT<float>& res, const StateDevT<float>& dq,
                                  double cfl_max, int sweeps, double omega, double growth, bool dq_only = false) {

@@ -76,12 +76,92 @@ struct StepGraph {
   int cur = -1;
   int sweeps = 0;
   double omega = 0, growth = 0, cfl_max = 0;
+  // The remainder of a call after its whole graphs: `tail_len` steps from
+  // buffer `tail_cur`, ending on buffer `tail_end` (kept while calls of one
+  // length repeat, so a call of any length replays graphs only).
+  hipGraph_t tail_graph = nullptr;
+  hipGraphExec_t tail_exec = nullptr;
+  int tail_len = 0, tail_cur = -1, tail_end = -1;
 };
 
+inline void drop_tail(StepGraph& g) {
+  if (g.tail_exec) (void)hipGraphExecDestroy(g.tail_exec);
+  if (g.tail_graph) (void)hipGraphDestroy(g.tail_graph);
+  g.tail_exec = nullptr;
+  g.tail_graph = nullptr;
+  g.tail_len = 0;
+  g.tail_cur = g.tail_end = -1;
+}
+
 inline void drop_graph(StepGraph& g) {
+  drop_tail(g);
   if (g.exec) (void)hipGraphExecDestroy(g.exec);
   if (g.graph) (void)hipGraphDestroy(g.graph);
   g = StepGraph();
+}
+
+// Captures the `n` steps that `encode` enqueues on `stream` into one
+// instantiated graph. Nothing runs during the capture, so the current
+// buffer index `cur` is restored afterwards; `cur_end` is the buffer the
+// steps end on.
+template <typename Encode>
+bs_status capture_graph(hipStream_t stream, int& cur, int n, Encode encode, hipGraph_t* graph_out,
+                        hipGraphExec_t* exec_out, int* cur_end) {
+  const int cur0 = cur;
+  HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "graph capture");
+  bs_status st = BS_OK;
+  for (int k = 0; k < n && st == BS_OK; k++) st = encode();
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(stream, &graph);
+  *cur_end = cur;
+  cur = cur0;
+  if (st != BS_OK || e != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    if (st != BS_OK) return st;
+    return hip_fail(e, "graph capture");
+  }
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return hip_fail(e, "graph instantiate");
+  }
+  *graph_out = graph;
+  *exec_out = exec;
+  return BS_OK;
+}
+
+// `count` steps from buffer `cur` as whole captured graphs of GRAPH_STEPS
+// steps (which return to the buffer they start from) and one tail graph
+// for the rest, capturing what is not cached yet.
+template <typename Encode>
+bs_status replay_steps(hipStream_t stream, int& cur, StepGraph& g, int count, Encode encode) {
+  if (!(g.exec && g.cur == cur)) {
+    drop_graph(g);
+    int end = -1;
+    bs_status st = capture_graph(stream, cur, GRAPH_STEPS, encode, &g.graph, &g.exec, &end);
+    if (st) return st;
+    if (end != cur) {
+      drop_graph(g);
+      return fail(BS_ERR_EXECUTION, "captured steps do not return to their state buffer");
+    }
+    g.cur = cur;
+  }
+  int k = 0;
+  for (; k + GRAPH_STEPS <= count; k += GRAPH_STEPS) HIP_TRY(hipGraphLaunch(g.exec, stream), "graph launch");
+  const int rest = count - k;
+  if (rest > 0) {
+    if (!(g.tail_exec && g.tail_cur == cur && g.tail_len == rest)) {
+      drop_tail(g);
+      bs_status st = capture_graph(stream, cur, rest, encode, &g.tail_graph, &g.tail_exec, &g.tail_end);
+      if (st) return st;
+      g.tail_cur = cur;
+      g.tail_len = rest;
+    }
+    HIP_TRY(hipGraphLaunch(g.tail_exec, stream), "graph launch");
+    cur = g.tail_end;
+  }
+  return BS_OK;
 }
 
 // Runge-Kutta coefficients (runge-kutta.ts:17-28) in the state's precision.
