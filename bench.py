@@ -43,13 +43,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # B/s, MI355X
-# Steps per captured HIP graph in the library (bs_engine.hip GRAPH_STEPS):
-# warm-up runs at least two graphs' worth so that the capture and
-# instantiation happen before the timed region, whatever --warmup is.
-GRAPH_STEPS = 12
 # Steps of the kernel-timing pass, fixed so the per-launch figure does not
 # depend on --steps.
 TIMING_STEPS = 1200
+# Untimed steps before the warm-up (about 30 ms at 1280 x 640): the GPU
+# clocks down when idle and takes that long to reach its steady clock, and
+# the step graphs are captured. Measured: 20 steps right after 10 ms of
+# idle take 7 % longer per step than back to back.
+PREWARM_STEPS = 600
 # Algorithmic HBM bytes per cell-update of SSP-RK3 with local steps: stage 1
 # 60 B, stages 2 and 3 76 B each (state 16 B per copy, metrics 24 B, dt 4 B).
 BYTES_PER_CU_RK3 = 212.0
@@ -144,11 +145,11 @@ def main():
     gpu.setCellPrimitives(np.array(free))
     if a.no_graphs:
         gpu.enableGraphs(False)
-    # Warm-up: at least two graphs of steps, so the step graph is captured
-    # and instantiated before the timed region, then one call of the timed
-    # region's own length (its eager remainder's launches warmed too).
-    gpu.iterate(max(a.warmup, 2 * GRAPH_STEPS))
-    gpu.iterate(a.steps % GRAPH_STEPS + GRAPH_STEPS)
+    # Untimed: the pre-warm, one call of the timed region's length (which
+    # captures the graphs it replays), then the `--warmup` steps.
+    gpu.iterate(PREWARM_STEPS)
+    gpu.iterate(a.steps)
+    gpu.iterate(a.warmup)
     cells = a.ni * a.nj
 
     # Timed region: exactly `steps` steps, synchronised on both sides
@@ -213,7 +214,8 @@ def main():
                                "MUSCL minmod per-variable, HLLC, SSP-RK3 CFL 0.5, local dt"
                                + (", FP64 solver" if a.precision == "fp64" else ""),
                    "cells": cells, "grid": [a.ni, a.nj], "scheme": "ssprk3", "reconstruction": "perVariable/minmod",
-                   "flux": "hllc", "parallelism": "single GPU"},
+                   "flux": "hllc", "parallelism": "single GPU",
+                   "untimed_steps_before_warmup": PREWARM_STEPS + a.steps},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "kernel": "tile_kernel (explicit stage)", "bytes_per_cell_update": width * BYTES_PER_CU_RK3,

@@ -48,9 +48,10 @@ SIGNATURES = {
     "bs_solver_destroy": (C.c_int, [_vp]),
     "bs_compute_metrics": (C.c_int, [C.c_int32, C.c_int32, _dp, _dp, C.c_int32] + [_dp] * 10),
     "bs_set_primitives": (C.c_int, [_vp, _dp]),
-    "bs_set_state": (C.c_int, [_vp, _fp, _fp, _fp, _fp]),
+    "bs_set_state": (C.c_int, [_vp, _fp, _fp, _fp, _fp, C.c_int32]),
     "bs_read_state": (C.c_int, [_vp, _fp, _fp, _fp, _fp, C.POINTER(Report)]),
-    "bs_set_state_f64": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
+    "bs_set_state_f64": (C.c_int, [_vp, _dp, _dp, _dp, _dp, C.c_int32]),
+    "bs_check_precision": (C.c_int, [_vp, _dp]),
     "bs_read_state_f64": (C.c_int, [_vp, _dp, _dp, _dp, _dp, C.POINTER(Report)]),
     "bs_set_face_states": (C.c_int, [_vp, C.c_int32, _dp, C.c_int32]),
     "bs_set_face_fluxes": (C.c_int, [_vp, C.c_int32, _dp, C.c_int32]),
@@ -68,10 +69,10 @@ SIGNATURES = {
     "bs_set_fitted_shock": (C.c_int, [_vp, _dp]),
     "bs_read_shock": (C.c_int, [_vp, _dp, _dp, _dp, C.c_int32]),
     "bs_set_line_grid": (C.c_int, [_vp, _dp, _dp, C.c_int32]),
-    "bs_set_heights": (C.c_int, [_vp, _dp]),
-    "bs_move_grid": (C.c_int, [_vp, _dp]),
+    "bs_set_heights": (C.c_int, [_vp, _dp, C.c_int32]),
+    "bs_move_grid": (C.c_int, [_vp, _dp, C.c_int32]),
     "bs_iterate_fitted": (C.c_int, [_vp, C.c_int32, C.c_int32, C.POINTER(Report)]),
-    "bs_read_heights": (C.c_int, [_vp, _dp]),
+    "bs_read_heights": (C.c_int, [_vp, _dp, C.c_int32]),
     "bs_anderson_create": (C.c_int, [_vp, C.c_int64, C.c_int32, C.POINTER(_vp)]),
     "bs_anderson_destroy": (C.c_int, [_vp]),
     "bs_anderson_reset": (C.c_int, [_vp]),
@@ -87,7 +88,7 @@ SIGNATURES = {
     "bs_batch_set_primitives": (C.c_int, [_vp, C.c_int32, _dp]),
     "bs_batch_set_coarse": (C.c_int, [_vp, _dp, C.c_int32, C.c_int32, _dp, C.c_double]),
     "bs_batch_read_state": (C.c_int, [_vp, C.c_int32, _fp, _fp, _fp, _fp]),
-    "bs_batch_set_active": (C.c_int, [_vp, C.POINTER(C.c_uint8)]),
+    "bs_batch_set_active": (C.c_int, [_vp, C.POINTER(C.c_uint8), C.c_int32]),
     "bs_batch_set_tiling": (C.c_int, [_vp, C.c_int32]),
     "bs_batch_set_relaxation": (C.c_int, [_vp, C.c_int32]),
     "bs_batch_iterate": (C.c_int, [_vp, C.c_int32]),

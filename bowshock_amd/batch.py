@@ -96,7 +96,7 @@ class GpuBatch2D:
         m = np.ascontiguousarray(np.asarray(mask, dtype=np.uint8))
         if m.shape != (self.count,):
             raise RangeError("one flag per case")
-        check(N.lib().bs_batch_set_active(self._h, m.ctypes.data_as(C.POINTER(C.c_uint8))))
+        check(N.lib().bs_batch_set_active(self._h, m.ctypes.data_as(C.POINTER(C.c_uint8)), self.count))
         self.active = m.copy()
 
     def setRelaxation(self, form: str):

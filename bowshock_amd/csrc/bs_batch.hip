@@ -657,6 +657,9 @@ bs_status bs_batch_create(bs_ctx* ctx, const bs_config* configs, int32_t count, 
 
 bs_status bs_batch_destroy(bs_batch* b) {
   if (!b) return BS_OK;
+  bool idle = false;
+  if (!b->busy.compare_exchange_strong(idle, true))
+    return fail(BS_ERR_BUSY, "bs_batch: await the previous operation first (destroy while a call runs)");
   (void)hipSetDevice(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   batch_free(b);
@@ -757,9 +760,10 @@ bs_status bs_batch_read_state(bs_batch* b, int32_t index, float* rho, float* mu,
   return BS_OK;
 }
 
-bs_status bs_batch_set_active(bs_batch* b, const uint8_t* active) {
+bs_status bs_batch_set_active(bs_batch* b, const uint8_t* active, int32_t count) {
   BATCH_GUARD(b);
   if (!active) return fail(BS_ERR_RANGE, "no mask");
+  if (count != b->count) return fail(BS_ERR_RANGE, "the mask needs %d values (one per case), not %d", b->count, count);
   bool changed = false;
   for (int c = 0; c < b->count; c++) {
     const char a = active[c] ? 1 : 0;

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "bs_internal.hpp"
+#include "bs_ledger.hpp"
 #include "bs_anderson.hpp"
 #include "bs_linegrid.hpp"
 
@@ -347,18 +348,11 @@ void fill_report(const Control& c, bs_report* r) {
   r->dt = c.dt;
 }
 
-// StepLedger.account (time-loop.ts:125-150).
+// StepLedger.account (time-loop.ts:125-150), bs_ledger.hpp.
 bs_status account(bs_solver* s, const Control& c, long long submitted) {
-  if (c.status & STATUS_NON_PHYSICAL)
-    return fail(BS_ERR_NONPHYSICAL, "non-physical state on the GPU between steps %lld and %lld",
-                s->steps_accounted, c.steps);
-  if (submitted >= 0 && c.steps != s->steps_accounted + submitted)
-    return fail(BS_ERR_EXECUTION,
-                "the GPU ran %lld of the %lld steps submitted: it rejected the work, ran out of memory or lost the "
-                "device",
-                c.steps - s->steps_accounted, submitted);
-  s->steps_accounted = c.steps;
-  return BS_OK;
+  char msg[256];
+  const bs_status st = ledger_account(s->steps_accounted, c.status, c.steps, submitted, msg, sizeof msg);
+  return st ? fail(st, "%s", msg) : BS_OK;
 }
 
 bs_status reset_control(bs_solver* s) {
@@ -652,6 +646,10 @@ bs_status bs_solver_precision(bs_solver* s, int32_t* precision) {
 
 bs_status bs_solver_destroy(bs_solver* s) {
   if (!s) return BS_OK;
+  // Taken and never released: the solver is freed below.
+  bool idle = false;
+  if (!s->busy.compare_exchange_strong(idle, true))
+    return fail(BS_ERR_BUSY, "bs_solver: await the previous operation first (destroy while a call runs)");
   (void)hipSetDevice(s->device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   free_all(s);
@@ -731,17 +729,20 @@ static bs_status read_state_into(bs_solver* s, V* const dst[4], bs_report* repor
 }
 }  // extern "C++"
 
-bs_status bs_set_state(bs_solver* s, const float* rho, const float* mu, const float* mv, const float* energy) {
+bs_status bs_set_state(bs_solver* s, const float* rho, const float* mu, const float* mv, const float* energy,
+                       int32_t count) {
   SOLVER_GUARD(s);
   if (!rho || !mu || !mv || !energy) return fail(BS_ERR_RANGE, "four arrays are needed");
+  if (count != s->n) return fail(BS_ERR_RANGE, "the state needs %d values per array, not %d", s->n, count);
   const float* const src[4] = {rho, mu, mv, energy};
   return set_state_from(s, src);
 }
 
 bs_status bs_set_state_f64(bs_solver* s, const double* rho, const double* mu, const double* mv,
-                           const double* energy) {
+                           const double* energy, int32_t count) {
   SOLVER_GUARD(s);
   if (!rho || !mu || !mv || !energy) return fail(BS_ERR_RANGE, "four arrays are needed");
+  if (count != s->n) return fail(BS_ERR_RANGE, "the state needs %d values per array, not %d", s->n, count);
   const double* const src[4] = {rho, mu, mv, energy};
   return set_state_from(s, src);
 }
@@ -917,6 +918,88 @@ bs_status bs_residual_vector(bs_solver* s, double* residual) {
   SOLVER_GUARD(s);
   if (!residual) return fail(BS_ERR_RANGE, "no output");
   return s->fp64 ? residual_vector_t<double>(s, residual) : residual_vector_t<float>(s, residual);
+}
+
+// The bar of the precision check (gpu-transonic.ts:108-117 RESIDUAL_PRECISION):
+// the largest difference between the device's residual and a double one
+// on the same state, relative to the largest residual.
+static constexpr double RESIDUAL_PRECISION = 1e-8;
+
+// C++ helpers of the precision check.
+extern "C++" {
+// Copies `count` values of an FP32 device array into an FP64 one through
+// the host (a check made once per solve).
+static bs_status widen_copy(const void* src, void* dst, size_t count, const char* what) {
+  std::vector<float> f(count);
+  std::vector<double> d(count);
+  HIP_TRY(hipMemcpy(f.data(), src, count * sizeof(float), hipMemcpyDeviceToHost), what);
+  for (size_t k = 0; k < count; k++) d[k] = f[k];
+  HIP_TRY(hipMemcpy(dst, d.data(), count * sizeof(double), hipMemcpyHostToDevice), what);
+  return BS_OK;
+}
+
+// The FP32 solver's state, face states and imposed fluxes, widened, on an
+// FP64 solver of the same configuration; its residual vector (the FP64
+// stencil kernel, FP64 metrics) is the double one the check compares with.
+static bs_status double_residual(bs_solver* s, std::vector<double>& out) {
+  bs_ctx ctx{s->device};
+  bs_solver* twin = nullptr;
+  bs_status st = solver_create(&ctx, &s->cfg, true, &twin);
+  if (st) return st;
+  auto done = [&](bs_status code) {
+    bs_solver_destroy(twin);
+    (void)hipSetDevice(s->device);
+    return code;
+  };
+  HIP_TRY(hipStreamSynchronize(s->stream), "precision check");
+  const size_t n = s->n;
+  if ((st = widen_copy(s->d.state[s->cur], twin->d.state[0], 4 * n, "precision check state"))) return done(st);
+  twin->cur = 0;
+  for (int side = 0; side < 4; side++) {
+    if (s->cfg.boundaries[side].kind != BS_BC_FACE_STATES) continue;
+    const size_t m = 4 * (size_t)along(s, side);
+    if ((st = widen_copy(s->d.face_cons[side], twin->d.face_cons[side], m, "precision check faces")))
+      return done(st);
+    if (s->imposed[side]) {
+      if ((st = widen_copy(s->d.face_flux[side], twin->d.face_flux[side], m, "precision check fluxes")))
+        return done(st);
+      twin->imposed[side] = true;
+    }
+  }
+  if ((st = upload_boundary(twin))) return done(st);
+  out.assign(4 * n, 0.0);
+  st = residual_vector_t<double>(twin, out.data());
+  return done(st);
+}
+}  // extern "C++"
+
+bs_status bs_check_precision(bs_solver* s, double* difference) {
+  SOLVER_GUARD(s);
+  if (difference) *difference = 0.0;
+  Control c;
+  bs_status st = read_control(s, &c);
+  if (st) return st;
+  if (c.status & STATUS_NON_PHYSICAL) return fail(BS_ERR_NONPHYSICAL, "the GPU solution is non-physical");
+  // An FP64 solver's residual is the double one (transonic-drag.ts:60-62:
+  // the engine in double precision passes at once).
+  if (s->fp64) return BS_OK;
+  std::vector<double> gpu(4 * (size_t)s->n), ref;
+  if ((st = residual_vector_t<float>(s, gpu.data()))) return st;
+  if ((st = double_residual(s, ref))) return st;
+  // residualPrecision (gpu-transonic.ts:118-131).
+  double worst = 0.0, largest = 0.0;
+  for (size_t k = 0; k < ref.size(); k++) {
+    const double d = std::fabs(gpu[k] - ref[k]);
+    worst = std::isnan(d) ? INFINITY : (d > worst ? d : worst);
+    largest = std::fabs(ref[k]) > largest ? std::fabs(ref[k]) : largest;
+  }
+  const double rel = largest > 0 ? worst / largest : (worst > 0 ? INFINITY : 0.0);
+  if (difference) *difference = rel;
+  if (worst <= RESIDUAL_PRECISION * largest) return BS_OK;
+  return fail(BS_ERR_PRECISION,
+              "this GPU's residual differs from the double-precision one by %.2e of the largest (at most %g "
+              "allowed): an FP32 solver loses the low parts; use the FP64 solver",
+              rel, RESIDUAL_PRECISION);
 }
 
 // C++ helpers of the entry points below.
@@ -1274,6 +1357,9 @@ bs_status bs_anderson_create(bs_ctx* ctx, int64_t size, int32_t depth, bs_anders
 
 bs_status bs_anderson_destroy(bs_anderson* a) {
   if (!a) return BS_OK;
+  bool idle = false;
+  if (!a->busy.compare_exchange_strong(idle, true))
+    return fail(BS_ERR_BUSY, "bs_anderson: await the previous operation first (destroy while a call runs)");
   (void)hipSetDevice(a->device);
   if (a->stream) (void)hipStreamSynchronize(a->stream);
   anderson_free(a);
@@ -1501,10 +1587,11 @@ bs_status bs_set_line_grid(bs_solver* s, const double* lines, const double* frac
   return BS_OK;
 }
 
-bs_status bs_set_heights(bs_solver* s, const double* heights) {
+bs_status bs_set_heights(bs_solver* s, const double* heights, int32_t count) {
   SOLVER_GUARD(s);
   if (!s->lines) return fail(BS_ERR_RANGE, "set the lines first (bs_set_line_grid)");
   if (!heights) return fail(BS_ERR_RANGE, "no heights");
+  if (count != s->ni + 1) return fail(BS_ERR_RANGE, "expected %d heights, got %d", s->ni + 1, count);
   const size_t nl = (size_t)s->ni + 1;
   for (size_t i = 0; i < nl; i++)
     if (!(heights[i] > 0)) return fail(BS_ERR_RANGE, "height %zu is not positive", i);
@@ -1517,19 +1604,21 @@ bs_status bs_set_heights(bs_solver* s, const double* heights) {
   return BS_OK;
 }
 
-bs_status bs_read_heights(bs_solver* s, double* heights) {
+bs_status bs_read_heights(bs_solver* s, double* heights, int32_t count) {
   SOLVER_GUARD(s);
   if (!s->lines || !heights) return fail(BS_ERR_RANGE, "no lines or no output");
+  if (count != s->ni + 1) return fail(BS_ERR_RANGE, "expected room for %d heights, got %d", s->ni + 1, count);
   HIP_TRY(hipMemcpyAsync(heights, s->d.height, ((size_t)s->ni + 1) * sizeof(double), hipMemcpyDeviceToHost,
                          s->stream), "heights");
   HIP_TRY(hipStreamSynchronize(s->stream), "heights");
   return BS_OK;
 }
 
-bs_status bs_move_grid(bs_solver* s, const double* heights) {
+bs_status bs_move_grid(bs_solver* s, const double* heights, int32_t count) {
   SOLVER_GUARD(s);
   if (!s->lines) return fail(BS_ERR_RANGE, "set the lines first (bs_set_line_grid)");
   if (!heights) return fail(BS_ERR_RANGE, "no heights");
+  if (count != s->ni + 1) return fail(BS_ERR_RANGE, "expected %d heights, got %d", s->ni + 1, count);
   const size_t nl = (size_t)s->ni + 1;
   std::vector<double> now(nl), move(nl);
   HIP_TRY(hipMemcpyAsync(now.data(), s->d.height, nl * sizeof(double), hipMemcpyDeviceToHost, s->stream), "heights");

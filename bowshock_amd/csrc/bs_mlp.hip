@@ -628,6 +628,9 @@ bs_status bs_mlp_create(bs_ctx* ctx, int32_t inputs, int32_t hidden, int32_t dep
 
 bs_status bs_mlp_destroy(bs_mlp* p) {
   if (!p) return BS_OK;
+  bool idle = false;
+  if (!p->busy.compare_exchange_strong(idle, true))
+    return fail(BS_ERR_BUSY, "bs_mlp: await the previous operation first (destroy while a call runs)");
   (void)hipSetDevice(p->device);
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   mlp_free(p);

@@ -226,12 +226,15 @@ class GpuEuler2D:
     def setState(self, rho, mu, mv, energy):
         """Conserved arrays; float64 arrays go through the FP64 entry
         (exact on an FP64 solver, rounded once on an FP32 one)."""
+        sizes = {np.asarray(a).size for a in (rho, mu, mv, energy)}
+        if sizes != {self.n}:
+            raise RangeError(f"the state needs {self.n} values per array, got {sorted(sizes)}")
         if any(np.asarray(a).dtype == np.float64 for a in (rho, mu, mv, energy)):
-            arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, mu, mv, energy)]
-            check(N.lib().bs_set_state_f64(self._h, *[N.dptr(a) for a in arrs]))
+            arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (rho, mu, mv, energy)]
+            check(N.lib().bs_set_state_f64(self._h, *[N.dptr(a) for a in arrs], self.n))
             return
-        arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (rho, mu, mv, energy)]
-        check(N.lib().bs_set_state(self._h, *[N.fptr(a) for a in arrs]))
+        arrs = [np.ascontiguousarray(a, dtype=np.float32).reshape(-1) for a in (rho, mu, mv, energy)]
+        check(N.lib().bs_set_state(self._h, *[N.fptr(a) for a in arrs], self.n))
 
     def iterate(self, count: int):
         """euler2d-gpu.ts:462-473."""
@@ -245,6 +248,16 @@ class GpuEuler2D:
         """euler2d-gpu.ts:530-546 (FP64 reduction on the GPU)."""
         out = C.c_double()
         check(N.lib().bs_residual_norm(self._h, C.byref(out)))
+        return out.value
+
+    def checkPrecision(self) -> float:
+        """TransonicGridSolver.checkPrecision (transonic-drag.ts:56-66,
+        gpu-transonic.ts:108-131): raises GpuPrecisionError when the
+        residual differs from a double-precision one of the same state by
+        more than 1e-8 of the largest; returns the relative difference
+        (0 for an FP64 solver, whose residual is the double one)."""
+        out = C.c_double()
+        check(N.lib().bs_check_precision(self._h, C.byref(out)))
         return out.value
 
     def residualVector(self) -> np.ndarray:
@@ -322,13 +335,13 @@ class GpuEuler2D:
 
     def setHeights(self, heights):
         """GpuLineGrid.setHeights: the nodes at these heights, metrics on the GPU."""
-        h = np.ascontiguousarray(heights, dtype=np.float64)
-        check(N.lib().bs_set_heights(self._h, N.dptr(h)))
+        h = self._heights(heights)
+        check(N.lib().bs_set_heights(self._h, N.dptr(h), h.size))
 
     def moveGrid(self, heights):
         """euler2d-gpu.ts:400-405: the lines to these heights, the flow remapped."""
-        h = np.ascontiguousarray(heights, dtype=np.float64)
-        check(N.lib().bs_move_grid(self._h, N.dptr(h)))
+        h = self._heights(heights)
+        check(N.lib().bs_move_grid(self._h, N.dptr(h), h.size))
 
     def iterateFitted(self, moves: int, stepsPerMove: int = 1):
         """euler2d-gpu.ts:481-505."""
@@ -337,7 +350,14 @@ class GpuEuler2D:
     def readHeights(self) -> np.ndarray:
         """GpuLineGrid.readHeights (line-grid.ts:257-270)."""
         h = np.zeros(self.ni + 1)
-        check(N.lib().bs_read_heights(self._h, N.dptr(h)))
+        check(N.lib().bs_read_heights(self._h, N.dptr(h), h.size))
+        return h
+
+    def _heights(self, heights) -> np.ndarray:
+        """One height per line (line-grid.ts:301: RangeError otherwise)."""
+        h = np.ascontiguousarray(heights, dtype=np.float64).reshape(-1)
+        if h.size != self.ni + 1:
+            raise RangeError(f"expected {self.ni + 1} heights, got {h.size}")
         return h
 
     def setRelaxation(self, form: str):

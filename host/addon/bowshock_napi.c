@@ -10,6 +10,7 @@
  */
 #define NAPI_VERSION 8
 #include <node_api.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -201,7 +202,8 @@ static napi_value js_solver_create(napi_env env, napi_callback_info info) {
 static napi_value js_solver_destroy(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   if (!get_args(env, info, 1, argv)) return NULL;
-  bs_solver_destroy((bs_solver *)get_ptr(env, argv[0]));
+  bs_status st = bs_solver_destroy((bs_solver *)get_ptr(env, argv[0]));
+  if (st) return throw_status(env, st);
   return NULL;
 }
 
@@ -261,11 +263,19 @@ static napi_value js_set_state(napi_env env, napi_callback_info info) {
   napi_value argv[5];
   if (!get_args(env, info, 5, argv)) return NULL;
   const float *q[4];
+  size_t len[4] = {0, 0, 0, 0};
   for (int k = 0; k < 4; k++) {
-    q[k] = (const float *)typed(env, argv[1 + k], napi_float32_array, NULL);
+    q[k] = (const float *)typed(env, argv[1 + k], napi_float32_array, &len[k]);
     if (!q[k]) return throw_status(env, BS_ERR_RANGE);
   }
-  bs_status st = bs_set_state((bs_solver *)get_ptr(env, argv[0]), q[0], q[1], q[2], q[3]);
+  const size_t n = len[0];
+  for (int k = 1; k < 4; k++)
+    if (len[k] != n) {
+      napi_throw_range_error(env, NULL, "the four state arrays must have one length");
+      return NULL;
+    }
+  bs_status st = bs_set_state((bs_solver *)get_ptr(env, argv[0]), q[0], q[1], q[2], q[3],
+                              n > INT32_MAX ? -1 : (int32_t)n);
   if (st) return throw_status(env, st);
   return NULL;
 }
@@ -348,6 +358,19 @@ static napi_value js_residual_norm(napi_env env, napi_callback_info info) {
   if (st) return throw_status(env, st);
   napi_value v;
   napi_create_double(env, norm, &v);
+  return v;
+}
+
+/* checkPrecision(solver): the relative difference; throws
+ * GpuPrecisionError (BS_ERR_PRECISION) above the bar. */
+static napi_value js_check_precision(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  double diff = 0;
+  bs_status st = bs_check_precision((bs_solver *)get_ptr(env, argv[0]), &diff);
+  if (st) return throw_status(env, st);
+  napi_value v;
+  napi_create_double(env, diff, &v);
   return v;
 }
 
@@ -547,7 +570,8 @@ static napi_value heights_call(napi_env env, napi_callback_info info, int move) 
   const double *h = (const double *)typed(env, argv[1], napi_float64_array, &n);
   if (!h) return throw_status(env, BS_ERR_RANGE);
   bs_solver *s = (bs_solver *)get_ptr(env, argv[0]);
-  bs_status st = move ? bs_move_grid(s, h) : bs_set_heights(s, h);
+  const int32_t count = n > INT32_MAX ? -1 : (int32_t)n;
+  bs_status st = move ? bs_move_grid(s, h, count) : bs_set_heights(s, h, count);
   if (st) return throw_status(env, st);
   return NULL;
 }
@@ -562,7 +586,7 @@ static napi_value js_read_heights(napi_env env, napi_callback_info info) {
   int32_t n = i32(env, argv[1]);
   double *h;
   napi_value arr = new_f64(env, (size_t)(n > 0 ? n : 0), &h);
-  bs_status st = bs_read_heights((bs_solver *)get_ptr(env, argv[0]), h);
+  bs_status st = bs_read_heights((bs_solver *)get_ptr(env, argv[0]), h, n);
   if (st) return throw_status(env, st);
   return arr;
 }
@@ -605,7 +629,8 @@ static napi_value js_batch_create(napi_env env, napi_callback_info info) {
 static napi_value js_batch_destroy(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   if (!get_args(env, info, 1, argv)) return NULL;
-  bs_batch_destroy((bs_batch *)get_ptr(env, argv[0]));
+  bs_status st = bs_batch_destroy((bs_batch *)get_ptr(env, argv[0]));
+  if (st) return throw_status(env, st);
   return NULL;
 }
 
@@ -667,9 +692,10 @@ static napi_value js_batch_set_coarse(napi_env env, napi_callback_info info) {
 static napi_value js_batch_set_active(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!get_args(env, info, 2, argv)) return NULL;
-  const uint8_t *a = (const uint8_t *)typed(env, argv[1], napi_uint8_array, NULL);
+  size_t n = 0;
+  const uint8_t *a = (const uint8_t *)typed(env, argv[1], napi_uint8_array, &n);
   if (!a) return throw_status(env, BS_ERR_RANGE);
-  bs_status st = bs_batch_set_active((bs_batch *)get_ptr(env, argv[0]), a);
+  bs_status st = bs_batch_set_active((bs_batch *)get_ptr(env, argv[0]), a, n > INT32_MAX ? -1 : (int32_t)n);
   if (st) return throw_status(env, st);
   return NULL;
 }
@@ -843,6 +869,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"setFaceStates", NULL, js_set_face_states, NULL, NULL, NULL, napi_enumerable, NULL},
       {"setFaceFluxes", NULL, js_set_face_fluxes, NULL, NULL, NULL, napi_enumerable, NULL},
       {"residualNorm", NULL, js_residual_norm, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"checkPrecision", NULL, js_check_precision, NULL, NULL, NULL, napi_enumerable, NULL},
       {"residualVector", NULL, js_residual_vector, NULL, NULL, NULL, napi_enumerable, NULL},
       {"wallFluxes", NULL, js_wall_fluxes, NULL, NULL, NULL, napi_enumerable, NULL},
       {"forebodyDrag", NULL, js_forebody_drag, NULL, NULL, NULL, napi_enumerable, NULL},

@@ -152,6 +152,12 @@ export class GpuEuler2D {
   iterate(count: number): Promise<void>;
   advanceTo(tEnd: number, maxSteps?: number): Promise<void>;
   densityResidualNorm(): Promise<number>;
+  /**
+   * TransonicGridSolver.checkPrecision (transonic-drag.ts:56-66): throws
+   * GpuPrecisionError when the residual is not a double-precision one's
+   * within 1e-8 of the largest; resolves to the relative difference.
+   */
+  checkPrecision(): Promise<number>;
   residualVector(): Promise<Float64Array>;
   readState(): Promise<GpuState2D>;
   setFaceStates(side: Side, states: readonly Primitive2D[]): void;
@@ -276,5 +282,6 @@ export interface TransonicEngine {
 }
 export function hipTransonicEngine(
   device: HipDevice | number,
-  options: { readonly sweeps: number; readonly cfl?: number },
+  /** precision: the solver ('fp64' by default, as the reference's two-float engine). */
+  options: { readonly sweeps: number; readonly cfl?: number; readonly precision?: 'fp32' | 'fp64' },
 ): TransonicEngine;

@@ -354,7 +354,7 @@ class GpuEuler2D {
   moveGrid(lineGrid, heights) {
     this.exclusive.assertIdle();
     lineGrid.checkSolver(this);
-    call(() => native().moveGrid(this.handle, Float64Array.from(heights)));
+    call(() => native().moveGrid(this.handle, lineGrid.heightsOf(heights)));
   }
 
   /** euler2d-gpu.ts:481-505. */
@@ -373,9 +373,22 @@ class GpuEuler2D {
     return this.exclusive.run(async () => native().readShock(this.handle, this.config.grid.ni));
   }
 
-  /** euler2d-gpu.ts:659. */
+  /**
+   * TransonicGridSolver.checkPrecision (transonic-drag.ts:56-66): the
+   * residual against a double-precision one of the same state; throws
+   * GpuPrecisionError above 1e-8 of the largest (an FP32 solver does).
+   */
+  async checkPrecision() {
+    return this.exclusive.run(async () => native().checkPrecision(this.handle));
+  }
+
+  /**
+   * euler2d-gpu.ts:659. Refused while an operation is pending: its worker
+   * thread is still using the solver.
+   */
   destroy() {
-    if (this.handle !== null) native().solverDestroy(this.handle);
+    this.exclusive.assertIdle();
+    if (this.handle !== null) call(() => native().solverDestroy(this.handle));
     this.handle = null;
   }
 }
@@ -425,14 +438,18 @@ function hipEngine(device) {
 /**
  * The HIP engine as a TransonicEngine (transonic-drag.ts:46-87,
  * gpu-transonic.ts:26-106): implicit red-black steps with a CFL ramp, the
- * drag and residual from the GPU in FP64.
+ * drag and residual from the GPU in FP64. Like the reference's, whose
+ * engine is the two-float solver, it runs the FP64 solver (relaxing in
+ * FP32) unless options.precision says 'fp32'; checkPrecision compares the
+ * residual with a double-precision one, so an FP32 solver fails it.
  */
 function hipTransonicEngine(device, options) {
   const sweeps = options.sweeps;
   const cfl = options.cfl === undefined ? 1000 : options.cfl;
+  const precision = options.precision === undefined ? 'fp64' : options.precision;
   if (!(Number.isInteger(sweeps) && sweeps >= 1)) throw new RangeError('the relaxation needs a whole number of sweeps');
   const solve = async (problem) => {
-    const gpu = await GpuEuler2D.create(device, problem.config);
+    const gpu = await GpuEuler2D.create(device, problem.config, { precision });
     const { ni } = problem.config.grid;
     const gamma = problem.config.gamma;
     return {
@@ -444,11 +461,10 @@ function hipTransonicEngine(device, options) {
       // The forebody drag of the body's wall (jMin), FP64 on the GPU.
       drag: () => gpu.forebodyDrag(problem.freeStream, problem.body.nose.baseRadius),
       residual: () => gpu.densityResidualNorm(),
-      // The residual is an FP64 reduction of the FP32 residual on the GPU; no
-      // split-float arithmetic is involved, so there are no low parts to lose.
-      // The check reads the state back, which throws if it is non-physical.
+      // gpu-transonic.ts:75-80: GpuPrecisionError when the residual is not
+      // the double one within RESIDUAL_PRECISION (bs_check_precision).
       checkPrecision: async () => {
-        await gpu.readState();
+        await gpu.checkPrecision();
       },
       cells: async () => {
         const state = await gpu.readState();
@@ -512,10 +528,18 @@ class GpuLineGrid {
     if (solver !== this.solver) throw new RangeError('the line grid was made on another solver');
   }
 
+  /** One height per line (line-grid.ts:301: RangeError otherwise). */
+  heightsOf(heights) {
+    if (heights.length !== this.ni + 1) {
+      throw new RangeError(`expected ${this.ni + 1} heights, got ${heights.length}`);
+    }
+    return Float64Array.from(heights);
+  }
+
   /** line-grid.ts:195-205: the metrics for these heights. */
   setHeights(heights) {
     this.solver.exclusive.assertIdle();
-    call(() => native().setHeights(this.solver.handle, Float64Array.from(heights)));
+    call(() => native().setHeights(this.solver.handle, this.heightsOf(heights)));
   }
 
   /** line-grid.ts:257-270. */
@@ -596,6 +620,7 @@ class GpuBatch2D {
   /** Cases with active[k] false are skipped by later steps and reductions. */
   setActive(active) {
     this.exclusive.assertIdle();
+    if (active.length !== this.count) throw new RangeError(`expected ${this.count} flags (one per case), got ${active.length}`);
     call(() => native().batchSetActive(this.handle, Uint8Array.from(active, (a) => (a ? 1 : 0))));
   }
 
@@ -651,8 +676,10 @@ class GpuBatch2D {
     return this.exclusive.run(async () => native().batchStatus(this.handle, this.count));
   }
 
+  /** Refused while an operation is pending (its worker still runs). */
   destroy() {
-    if (this.handle !== null) native().batchDestroy(this.handle);
+    this.exclusive.assertIdle();
+    if (this.handle !== null) call(() => native().batchDestroy(this.handle));
     this.handle = null;
   }
 }

@@ -25,7 +25,7 @@ async function test(name, fn) {
       'readState64', 'iterateFitted', 'setFittedShock', 'readShock', 'setLineGrid', 'setHeights', 'moveGrid',
       'readHeights', 'batchCreate', 'batchDestroy', 'batchSetUniform', 'batchSetPrimitives', 'batchReadState',
       'batchSetActive', 'batchSetCoarse', 'batchSetInt', 'batchSetImplicitCfl', 'batchIterate', 'batchIterateImplicit',
-      'batchResidualNorms', 'batchDrags', 'batchStatus'];
+      'batchResidualNorms', 'batchDrags', 'batchStatus', 'checkPrecision'];
     const a = bs.native();
     for (const n of names) assert.strictEqual(typeof a[n], 'function', n);
     assert.ok(/gfx950/.test(a.version()));
@@ -137,6 +137,34 @@ async function test(name, fn) {
       threw = e instanceof RangeError || e instanceof bs.GpuExecutionError;
     }
     assert.ok(threw);
+  });
+
+  await test('destroy is refused while an operation is pending', async () => {
+    // Engines made without a device: the guard runs before any native call.
+    for (const Cls of [bs.GpuEuler2D, bs.GpuBatch2D]) {
+      const e = Object.create(Cls.prototype);
+      e.exclusive = new bs.ExclusiveRunner(Cls.name);
+      e.handle = null;
+      let release;
+      const pending = e.exclusive.run(() => new Promise((resolve) => { release = resolve; }));
+      assert.throws(() => e.destroy(), /await the previous operation first/);
+      release();
+      await pending;
+      e.destroy();
+      assert.strictEqual(e.handle, null);
+    }
+  });
+
+  await test('heights and activity masks of the wrong length are RangeErrors', () => {
+    const lg = Object.create(bs.GpuLineGrid.prototype);
+    lg.ni = 4;
+    assert.throws(() => lg.heightsOf([1, 1, 1, 1]), RangeError);
+    assert.strictEqual(lg.heightsOf([1, 1, 1, 1, 1]).length, 5);
+    const b = Object.create(bs.GpuBatch2D.prototype);
+    b.exclusive = new bs.ExclusiveRunner('GpuBatch2D');
+    b.count = 3;
+    b.handle = null;
+    assert.throws(() => b.setActive([true, false]), RangeError);
   });
 
   for (const r of results) console.log(r[0], r[1], r[2] || '');

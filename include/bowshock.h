@@ -16,7 +16,9 @@
  * Errors: each call returns a bs_status; bs_last_error() gives the message
  * of the calling thread's last failure. A solver is not reentrant: a call
  * made while another call on the same solver is running returns BS_ERR_BUSY
- * (the reference's ExclusiveRunner, src/engine-gpu/gpu-common.ts:122-140).
+ * (the reference's ExclusiveRunner, src/engine-gpu/gpu-common.ts:122-140);
+ * so do bs_solver_destroy, bs_batch_destroy, bs_anderson_destroy and
+ * bs_mlp_destroy, which then free nothing.
  *
  * Conventions (src/grid/structured.ts:1-8): node (i, j) at i + j (ni + 1);
  * cell (i, j) at i + j ni; sides in the order iMin, iMax, jMin, jMax.
@@ -130,7 +132,9 @@ bs_status bs_solver_create(bs_ctx *ctx, const bs_config *config, bs_solver **out
 bs_status bs_solver_create_fp64(bs_ctx *ctx, const bs_config *config, bs_solver **out);
 /* BS_PRECISION_FP32 or BS_PRECISION_FP64. */
 bs_status bs_solver_precision(bs_solver *s, int32_t *precision);
-/* GpuEuler2D.destroy (euler2d-gpu.ts:659); idempotent, NULL is accepted. */
+/* GpuEuler2D.destroy (euler2d-gpu.ts:659); NULL is accepted. A solver
+ * that another call is still running on is not freed: BS_ERR_BUSY (the
+ * reference's ExclusiveRunner refuses the overlapping call). */
 bs_status bs_solver_destroy(bs_solver *s);
 
 /* Grid metrics in FP64 (GridMetrics, src/grid/structured.ts:28-50). Any
@@ -148,9 +152,11 @@ bs_status bs_compute_metrics(int32_t ni, int32_t nj, const double *x, const doub
  * physical. */
 bs_status bs_set_primitives(bs_solver *s, const double *prim);
 /* Euler2D.setConserved without the step count: conserved variables as FP32
- * structure of arrays (rho[n], mu[n], mv[n], energy[n]); resets the clock. */
+ * structure of arrays (rho[n], mu[n], mv[n], energy[n]), `count` = n = ni nj
+ * values each (BS_ERR_RANGE otherwise, as the reference's RangeError for a
+ * field of the wrong size); resets the clock. */
 bs_status bs_set_state(bs_solver *s, const float *rho, const float *mu, const float *mv,
-                       const float *energy);
+                       const float *energy, int32_t count);
 /* GpuEuler2D.readState (euler2d-gpu.ts:549-572, GpuState2D 116-125):
  * conserved FP32 arrays indexed i + j ni. Any pointer may be NULL. Returns
  * BS_ERR_NONPHYSICAL if the status word has the non-physical bit. */
@@ -161,7 +167,7 @@ bs_status bs_read_state(bs_solver *s, float *rho, float *mu, float *mv, float *e
  * replaces GpuEuler2DTwoFloat.readConserved (euler2d-two-float.ts:486-501),
  * which returns its ~48-bit state as FP64. */
 bs_status bs_set_state_f64(bs_solver *s, const double *rho, const double *mu, const double *mv,
-                           const double *energy);
+                           const double *energy, int32_t count);
 bs_status bs_read_state_f64(bs_solver *s, double *rho, double *mu, double *mv, double *energy,
                             bs_report *report);
 
@@ -187,6 +193,18 @@ bs_status bs_residual_norm(bs_solver *s, double *norm);
 /* residualVector (euler2d.ts:462-477) of the current state, FP32 residual
  * returned as FP64, 4 per cell interleaved. */
 bs_status bs_residual_vector(bs_solver *s, double *residual);
+
+/* TransonicGridSolver.checkPrecision (src/analysis/transonic-drag.ts:56-66;
+ * gpu-transonic.ts:75-80, 108-131 residualPrecision): the solver's residual
+ * vector of its current state against a double-precision one of the same
+ * state, here the FP64 stencil kernel's on the state widened to FP64 (with
+ * FP64 metrics). BS_ERR_PRECISION when the largest difference exceeds 1e-8
+ * (RESIDUAL_PRECISION) of the largest residual, which an FP32 solver does
+ * (its 32-bit fluxes lose about 1e-7 of values far larger than the
+ * residuals); an FP64 solver's residual is the double one and passes at
+ * once. `difference` (may be NULL) receives the relative difference.
+ * BS_ERR_NONPHYSICAL if the status word has the non-physical bit. */
+bs_status bs_check_precision(bs_solver *s, double *difference);
 
 /* boundaryFluxes (euler2d.ts:499-525) evaluated on the GPU in FP64 from the
  * FP32 state (pressure with the FP32 gamma, gpuPrimitive2D,
@@ -236,15 +254,17 @@ bs_status bs_read_shock(bs_solver *s, double *speed, double *mach, double *signa
  * bs_iterate_fitted (iterateFitted, euler2d-gpu.ts:481-505) runs `moves`
  * times `steps_per_move` explicit steps with the fitted shock moved after
  * each group by the last step's shock speeds (fitShock,
- * shock-fitting.ts:252-296); bs_read_heights reads the heights back. The
+ * shock-fitting.ts:252-296); bs_read_heights reads the heights back.
+ * Heights come and go as `count` = ni + 1 values, one per line (BS_ERR_RANGE
+ * otherwise: line-grid.ts:301 throws RangeError for another length). The
  * initial metrics of bs_solver_create stay the solver's until
  * bs_set_heights. */
 bs_status bs_set_line_grid(bs_solver *s, const double *lines, const double *fractions,
                            int32_t first_node_symmetry);
-bs_status bs_set_heights(bs_solver *s, const double *heights);
-bs_status bs_move_grid(bs_solver *s, const double *heights);
+bs_status bs_set_heights(bs_solver *s, const double *heights, int32_t count);
+bs_status bs_move_grid(bs_solver *s, const double *heights, int32_t count);
 bs_status bs_iterate_fitted(bs_solver *s, int32_t moves, int32_t steps_per_move, bs_report *report);
-bs_status bs_read_heights(bs_solver *s, double *heights);
+bs_status bs_read_heights(bs_solver *s, double *heights, int32_t count);
 
 /* The neighbour term of the red-black relaxation. BS_RELAX_FLUX_DIFFERENCE
  * is the reference's A (1/2 (F(Q + dQ) - F(Q)) - 1/2 omega lambda dQ)
@@ -335,8 +355,9 @@ bs_status bs_batch_set_coarse(bs_batch *b, const double *coarse, int32_t coarse_
                               const double *scale, double floor_value);
 /* readState (euler2d-gpu.ts:549-573) of one case; null outputs skipped. */
 bs_status bs_batch_read_state(bs_batch *b, int32_t index, float *rho, float *mu, float *mv, float *energy);
-/* Cases with active[c] == 0 are skipped by later steps and reductions. */
-bs_status bs_batch_set_active(bs_batch *b, const uint8_t *active);
+/* Cases with active[c] == 0 are skipped by later steps and reductions;
+ * `count` must be the batch's case count. */
+bs_status bs_batch_set_active(bs_batch *b, const uint8_t *active, int32_t count);
 /* Cases of fewer cells than the threshold run on 32 x 4 tiles, the others
  * on 64 x 8 tiles (set by the autotuner; 0 = all on 64 x 8). */
 bs_status bs_batch_set_tiling(bs_batch *b, int32_t small_tile_below_cells);

@@ -81,3 +81,34 @@ def test_solver_create_rejects_null_arguments():
     rc = N.lib().bs_solver_create(None, None, C.byref(h))
     assert rc == N.BS_ERR_RANGE
     assert N.last_error()
+
+
+def test_wrappers_refuse_arrays_of_the_wrong_length_before_the_library():
+    """A short array would be read past its end by the C entry points, so
+    the facades refuse it (line-grid.ts:301 throws RangeError for heights
+    of another length); the C entry points check the count again."""
+    from bowshock_amd.batch import GpuBatch2D
+    from bowshock_amd.engine import GpuEuler2D, RangeError
+
+    g = object.__new__(GpuEuler2D)
+    g.ni, g.nj = 4, 3
+    g._h = None
+    with pytest.raises(RangeError, match="5 heights"):
+        g.setHeights(np.ones(4))
+    with pytest.raises(RangeError, match="5 heights"):
+        g.moveGrid(np.ones(6))
+    with pytest.raises(RangeError, match="12 values"):
+        g.setState(np.ones(12), np.ones(12), np.ones(11), np.ones(12))
+    b = object.__new__(GpuBatch2D)
+    b.count = 3
+    b._h = None
+    with pytest.raises(RangeError, match="one flag per case"):
+        b.setActive([1, 0])
+
+
+def test_entry_points_refuse_a_wrong_count_without_a_solver():
+    """The count arguments are checked after the handle: a null handle is
+    refused first, and no entry point dereferences the arrays."""
+    h = np.ones(4)
+    assert N.lib().bs_set_heights(None, N.dptr(h), 4) == N.BS_ERR_RANGE
+    assert N.lib().bs_batch_set_active(None, None, 0) == N.BS_ERR_RANGE
