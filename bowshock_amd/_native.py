@@ -87,6 +87,7 @@ SIGNATURES = {
     "bs_batch_set_uniform": (C.c_int, [_vp, _dp]),
     "bs_batch_set_primitives": (C.c_int, [_vp, C.c_int32, _dp]),
     "bs_batch_set_coarse": (C.c_int, [_vp, _dp, C.c_int32, C.c_int32, _dp, C.c_double]),
+    "bs_batch_set_coarse_noalloc": (C.c_int, [_vp, _dp, C.c_int32, C.c_int32, _dp, C.c_double]),
     "bs_batch_read_state": (C.c_int, [_vp, C.c_int32, _fp, _fp, _fp, _fp]),
     "bs_batch_set_active": (C.c_int, [_vp, C.POINTER(C.c_uint8), C.c_int32]),
     "bs_batch_set_tiling": (C.c_int, [_vp, C.c_int32]),

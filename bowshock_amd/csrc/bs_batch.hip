@@ -19,6 +19,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "bs_batch_tiles.hpp"
@@ -483,6 +484,83 @@ __global__ __launch_bounds__(256) void batch_coarse_kernel(CoarseCase K, int cni
   K.state[3 * (size_t)K.n + c] = (float)(w[3] / (g - 1) + 0.5 * w[0] * (w[1] * w[1] + w[2] * w[2]));
 }
 
+// ---- the allocation-free warm start (bs_batch_set_coarse_noalloc) --------
+
+// Its staging: a file-scope device array, one copy per device the library
+// runs on, held until the process ends. Bytes [0, 256) hold the flag of a
+// non-physical field, the rest the coarse fields of as many cases at a time
+// as fit. Every call on a device shares it, so calls that overlap on one
+// device race.
+static constexpr size_t COARSE_SCRATCH_BYTES = BS_COARSE_SCRATCH_BYTES;
+static constexpr size_t COARSE_FLAG_BYTES = 256;
+static __device__ __align__(256) unsigned char scratch[COARSE_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+unsigned char* g_scratch[MAX_DEVICES] = {};
+
+// The device address of `scratch` on `device` (which is current), taken once
+// per device: host code passes this address to kernels and copies, never
+// the array's name, which on the host is only a stand-in.
+bs_status scratch_address(int device, unsigned char** out) {
+  if (device < 0 || device >= MAX_DEVICES) return fail(BS_ERR_RANGE, "device %d has no scratch slot", device);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "scratch address");
+    if (p == nullptr) return fail(BS_ERR_EXECUTION, "scratch address: null");
+    g_scratch[device] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch[device];
+  return BS_OK;
+}
+
+__global__ void clear_flag_kernel(int* flag) { *flag = 0; }
+
+// The coarse fields of every case through a staging area of `room` bytes
+// (whole cases at a time, in order), the fields' flag in `bad` (cleared
+// first by a kernel), then the flag read back; allocates nothing.
+bs_status coarse_through(bs_batch* b, const double* coarse, int cni, int cnj, const double* scale,
+                         double floor_value, double* staging, size_t room, int* bad) {
+  const size_t per = 4 * (size_t)cni * cnj;
+  const size_t at_once = room / (per * sizeof(double));
+  if (at_once == 0)
+    return fail(BS_ERR_RANGE, "a coarse field of %zu bytes exceeds the %zu bytes of staging", per * sizeof(double),
+                room);
+  hipLaunchKernelGGL(clear_flag_kernel, dim3(1), dim3(1), 0, b->stream, bad);
+  for (int c0 = 0; c0 < b->count; c0 += (int)at_once) {
+    const size_t left = (size_t)(b->count - c0);
+    const int n = (int)(left < at_once ? left : at_once);
+    // Stream order: these fields overwrite the staging only after the
+    // previous group's kernels have read it.
+    HIP_TRY(hipMemcpyAsync(staging, coarse + per * c0, per * n * sizeof(double), hipMemcpyHostToDevice, b->stream),
+            "coarse fields upload");
+    for (int c = c0; c < c0 + n; c++) {
+      const BatchCase& k = b->cases[c];
+      CoarseCase K;
+      K.state = b->state[b->cur] + 4 * k.cell;
+      K.n = k.n;
+      K.ni = k.ni;
+      K.nj = k.nj;
+      K.coarse = staging + per * (c - c0);
+      for (int q = 0; q < 4; q++) K.scale[q] = scale[4 * (size_t)c + q];
+      hipLaunchKernelGGL(batch_coarse_kernel, dim3((k.n + 255) / 256), dim3(256), 0, b->stream, K, cni, cnj,
+                         floor_value, (double)b->gamma32, bad);
+    }
+    HIP_TRY(hipGetLastError(), "coarse fields launch");
+  }
+  int flag = 0;
+  HIP_TRY(hipMemcpyAsync(&flag, bad, sizeof(int), hipMemcpyDeviceToHost, b->stream), "coarse fields flag");
+  HIP_TRY(hipStreamSynchronize(b->stream), "coarse fields");
+  if (flag) return fail(BS_ERR_NONPHYSICAL, "a coarse field gives a non-physical state");
+  return batch_reset_control(b, 0, b->count);
+}
+
+}  // namespace
+
 extern "C" {
 
 bs_status bs_batch_create(bs_ctx* ctx, const bs_config* configs, int32_t count, bs_batch** out) {
@@ -745,6 +823,23 @@ bs_status bs_batch_set_coarse(bs_batch* b, const double* coarse, int32_t coarse_
   if (st) return st;
   if (bad) return fail(BS_ERR_NONPHYSICAL, "a coarse field gives a non-physical state");
   return batch_reset_control(b, 0, b->count);
+}
+
+bs_status bs_batch_set_coarse_noalloc(bs_batch* b, const double* coarse, int32_t coarse_ni, int32_t coarse_nj,
+                                      const double* scale, double floor_value) {
+  BATCH_GUARD(b);
+  if (!coarse || !scale || coarse_ni < 1 || coarse_nj < 1) return fail(BS_ERR_RANGE, "no coarse fields");
+  if (!(floor_value > 0)) return fail(BS_ERR_RANGE, "the floor must be positive");
+  const size_t per = 4 * (size_t)coarse_ni * coarse_nj * sizeof(double);
+  if (per > COARSE_SCRATCH_BYTES - COARSE_FLAG_BYTES)
+    return fail(BS_ERR_RANGE, "a %d x %d coarse field takes %zu bytes; the scratch holds at most %zu", coarse_ni,
+                coarse_nj, per, COARSE_SCRATCH_BYTES - COARSE_FLAG_BYTES);
+  unsigned char* base = nullptr;
+  bs_status st = scratch_address(b->device, &base);
+  if (st) return st;
+  return coarse_through(b, coarse, coarse_ni, coarse_nj, scale, floor_value,
+                        reinterpret_cast<double*>(base + COARSE_FLAG_BYTES), COARSE_SCRATCH_BYTES - COARSE_FLAG_BYTES,
+                        reinterpret_cast<int*>(base));
 }
 
 bs_status bs_batch_read_state(bs_batch* b, int32_t index, float* rho, float* mu, float* mv, float* energy) {

@@ -353,6 +353,19 @@ bs_status bs_batch_set_primitives(bs_batch *b, int32_t index, const double *prim
  * BS_ERR_NONPHYSICAL if a field is not finite. */
 bs_status bs_batch_set_coarse(bs_batch *b, const double *coarse, int32_t coarse_ni, int32_t coarse_nj,
                               const double *scale, double floor_value);
+/* bs_batch_set_coarse for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc); it has no counterpart in the reference. Same arguments and
+ * result. Nothing is allocated: the fields pass, as many cases at a time as
+ * fit, through a fixed device scratch of BS_COARSE_SCRATCH_BYTES per device
+ * that the library holds until the process ends. One case's field (4 x
+ * coarse_ni x coarse_nj doubles) may take at most BS_COARSE_SCRATCH_BYTES -
+ * 256 bytes; a larger one is refused (BS_ERR_RANGE) before anything runs.
+ * Every call on a device shares its scratch: calls that overlap on one
+ * device (on any batches) race. */
+#define BS_COARSE_SCRATCH_BYTES (8u << 20)
+bs_status bs_batch_set_coarse_noalloc(bs_batch *b, const double *coarse, int32_t coarse_ni, int32_t coarse_nj,
+                                      const double *scale, double floor_value);
 /* readState (euler2d-gpu.ts:549-573) of one case; null outputs skipped. */
 bs_status bs_batch_read_state(bs_batch *b, int32_t index, float *rho, float *mu, float *mv, float *energy);
 /* Cases with active[c] == 0 are skipped by later steps and reductions;
