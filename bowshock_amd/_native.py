@@ -83,12 +83,15 @@ SIGNATURES = {
     "bs_enable_timing": (C.c_int, [_vp, C.c_int32]),
     "bs_kernel_timing": (C.c_int, [_vp, C.c_int32, _dp, C.POINTER(C.c_int64)]),
     "bs_batch_create": (C.c_int, [_vp, C.POINTER(Config), C.c_int32, C.POINTER(_vp)]),
+    "bs_batch_create_fp64": (C.c_int, [_vp, C.POINTER(Config), C.c_int32, C.POINTER(_vp)]),
+    "bs_batch_precision": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
     "bs_batch_destroy": (C.c_int, [_vp]),
     "bs_batch_set_uniform": (C.c_int, [_vp, _dp]),
     "bs_batch_set_primitives": (C.c_int, [_vp, C.c_int32, _dp]),
     "bs_batch_set_coarse": (C.c_int, [_vp, _dp, C.c_int32, C.c_int32, _dp, C.c_double]),
     "bs_batch_set_coarse_noalloc": (C.c_int, [_vp, _dp, C.c_int32, C.c_int32, _dp, C.c_double]),
     "bs_batch_read_state": (C.c_int, [_vp, C.c_int32, _fp, _fp, _fp, _fp]),
+    "bs_batch_read_state_f64": (C.c_int, [_vp, C.c_int32, _dp, _dp, _dp, _dp]),
     "bs_batch_set_active": (C.c_int, [_vp, C.POINTER(C.c_uint8), C.c_int32]),
     "bs_batch_set_tiling": (C.c_int, [_vp, C.c_int32]),
     "bs_batch_set_relaxation": (C.c_int, [_vp, C.c_int32]),

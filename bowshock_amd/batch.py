@@ -33,17 +33,23 @@ class GpuBatch2D:
     """Many GpuEuler2D cases advanced together. Every case shares the
     scheme fields (geometry, gamma, flux, limiter, reconstruction,
     integrator, cfl, local time stepping); grids, boundary kinds and
-    boundary states differ."""
+    boundary states differ. ``precision`` "fp64" makes the batch of FP64
+    solvers (bs_batch_create_fp64): FP64 state, metrics and time steps, the
+    implicit steps relaxing in FP32 with the linearized neighbour term."""
 
-    def __init__(self, ctx: Context, configs: Sequence[Euler2DConfig]):
+    def __init__(self, ctx: Context, configs: Sequence[Euler2DConfig], precision: str = "fp32"):
         if len(configs) == 0:
             raise RangeError("a batch needs at least one case")
+        if precision not in ("fp32", "fp64"):
+            raise RangeError(f"unknown precision {precision}")
         keep: list = []
         arr = (N.Config * len(configs))()
         for k, cfg in enumerate(configs):
             arr[k] = _c_config(cfg, keep)
         h = C.c_void_p()
-        check(N.lib().bs_batch_create(ctx._h, arr, len(configs), C.byref(h)))
+        create = N.lib().bs_batch_create_fp64 if precision == "fp64" else N.lib().bs_batch_create
+        check(create(ctx._h, arr, len(configs), C.byref(h)))
+        self.precision = precision
         self._h = h
         self.ctx = ctx
         self.configs = list(configs)
@@ -87,9 +93,14 @@ class GpuBatch2D:
         check(N.lib().bs_batch_set_coarse(self._h, N.dptr(coarse), cni, cnj, N.dptr(scale), float(floor)))
 
     def readState(self, index: int) -> GpuState2D:
+        """Case `index`'s conserved state: FP32 arrays, FP64 on an FP64 batch."""
         ni, nj = self.shapes[index]
-        out = [np.empty(ni * nj, dtype=np.float32) for _ in range(4)]
-        check(N.lib().bs_batch_read_state(self._h, int(index), *[N.fptr(a) for a in out]))
+        if self.precision == "fp64":
+            out = [np.empty(ni * nj, dtype=np.float64) for _ in range(4)]
+            check(N.lib().bs_batch_read_state_f64(self._h, int(index), *[N.dptr(a) for a in out]))
+        else:
+            out = [np.empty(ni * nj, dtype=np.float32) for _ in range(4)]
+            check(N.lib().bs_batch_read_state(self._h, int(index), *[N.fptr(a) for a in out]))
         return GpuState2D(0.0, int(self.status()[1][index]), *out)
 
     def setActive(self, mask):
@@ -240,7 +251,8 @@ def solve(batch: GpuBatch2D, p_inf, q_inf, base_radius, plan: Optional[SolvePlan
     refs = np.full(n, math.nan) if reference is None else np.asarray(reference, dtype=np.float64)
     trackers = [_Tracker(plan, float(refs[c])) for c in range(n)]
     if plan.implicit:
-        batch.setRelaxation(plan.relaxation)
+        if batch.precision == "fp32":
+            batch.setRelaxation(plan.relaxation)
         if cfl_start is None:
             batch.setImplicitCfl(plan.cfl_start)
         else:

@@ -9,12 +9,18 @@
 // are role-major (a role being a stage of the scheme for a given current
 // state buffer), so a launch picks its role by the base pointer it passes.
 //
+// A batch holds its state, metrics and time steps in FP32, or in FP64
+// (bs_batch_create_fp64): the FP64 solver's scheme, whose implicit steps
+// form the residual, diagonal and update in FP64 around an FP32 relaxation
+// with the linearized neighbour term. The host functions below are
+// templates on the state type; the arenas are untyped.
+//
 // Cases are split into two size classes by cell count, with a large tile
-// (64 x 8) for big grids and a small one (32 x 4) for small grids where the
-// large tile would leave most of its threads idle; the threshold is set by
-// the caller's autotuner (bs_batch_set_tiling). Each step of the active set
-// is captured as HIP graphs; the work lists and graphs are rebuilt when
-// the active set or the tiling changes.
+// for big grids and a small one for small grids where the large tile would
+// leave most of its threads idle; the threshold is set by the caller's
+// autotuner (bs_batch_set_tiling). Each step of the active set is captured
+// as HIP graphs; the work lists and graphs are rebuilt when the active set
+// or the tiling changes.
 
 #include <atomic>
 #include <cmath>
@@ -44,25 +50,27 @@ enum BatchRole {
   N_TILE_ROLES = 15
 };
 
-using BatchWallFn = void (*)(const WallArgs*, const WallBoundary*, const WorkItem*);
+template <typename S>
+using BatchWallFnT = void (*)(const WallArgsT<S>*, const WallBoundaryT<S>*, const WorkItem*);
 
-template <int REC, int FLUX>
-__global__ __launch_bounds__(256) void batch_wall_kernel(const WallArgs* __restrict__ args,
-                                                         const WallBoundary* __restrict__ wb,
+template <typename S, int REC, int FLUX>
+__global__ __launch_bounds__(256) void batch_wall_kernel(const WallArgsT<S>* __restrict__ args,
+                                                         const WallBoundaryT<S>* __restrict__ wb,
                                                          const WorkItem* __restrict__ work) {
   const WorkItem w = work[blockIdx.x];
-  wall_body<float, REC, FLUX>(args[w.args], wb[w.by], w.bx, w.slot);
+  wall_body<S, REC, FLUX>(args[w.args], wb[w.by], w.bx, w.slot);
 }
 
-BatchWallFn batch_wall_fn(const bs_config& c) {
+template <typename S>
+BatchWallFnT<S> batch_wall_fn(const bs_config& c) {
   if (c.flux == BS_FLUX_RUSANOV) {
-    if (c.reconstruction == SHARED) return batch_wall_kernel<SHARED, RUSANOV>;
-    if (c.reconstruction == SHARED_SCALED) return batch_wall_kernel<SHARED_SCALED, RUSANOV>;
-    return batch_wall_kernel<PER_VARIABLE, RUSANOV>;
+    if (c.reconstruction == SHARED) return batch_wall_kernel<S, SHARED, RUSANOV>;
+    if (c.reconstruction == SHARED_SCALED) return batch_wall_kernel<S, SHARED_SCALED, RUSANOV>;
+    return batch_wall_kernel<S, PER_VARIABLE, RUSANOV>;
   }
-  if (c.reconstruction == SHARED) return batch_wall_kernel<SHARED, HLLC>;
-  if (c.reconstruction == SHARED_SCALED) return batch_wall_kernel<SHARED_SCALED, HLLC>;
-  return batch_wall_kernel<PER_VARIABLE, HLLC>;
+  if (c.reconstruction == SHARED) return batch_wall_kernel<S, SHARED, HLLC>;
+  if (c.reconstruction == SHARED_SCALED) return batch_wall_kernel<S, SHARED_SCALED, HLLC>;
+  return batch_wall_kernel<S, PER_VARIABLE, HLLC>;
 }
 
 // Sum of each case's range of partials, in a fixed order (one block per
@@ -78,11 +86,6 @@ __global__ __launch_bounds__(256) void segment_sums(const double* __restrict__ p
   if (threadIdx.x == 0) out[c] = rms ? sqrt(s / (double)cells[c]) : s;
 }
 
-template <typename T>
-T* dev_ptr(void* base, size_t offset_elems) {
-  return (T*)base + offset_elems;
-}
-
 }  // namespace
 
 struct bs_batch {
@@ -90,33 +93,35 @@ struct bs_batch {
   hipStream_t stream = nullptr;
   bs_config scheme;
   float gamma32;
+  bool fp64 = false;  // state, metrics and time steps in FP64
   int count;
   std::vector<BatchCase> cases;
-  std::vector<bs_config> configs;
   size_t cells = 0, ifaces = 0, jfaces = 0, walls = 0;
   int slots = 0, wall_slots = 0;
-  // arenas
-  float* state[3] = {nullptr, nullptr, nullptr};
-  float *inx = nullptr, *iny = nullptr, *ia = nullptr, *jnx = nullptr, *jny = nullptr, *ja = nullptr;
-  float *vol = nullptr, *planar = nullptr, *dt = nullptr;
+  // arenas of the state's precision
+  void* state[3] = {nullptr, nullptr, nullptr};
+  void *inx = nullptr, *iny = nullptr, *ia = nullptr, *jnx = nullptr, *jny = nullptr, *ja = nullptr;
+  void *vol = nullptr, *planar = nullptr, *dt = nullptr;
+  // implicit work, FP32 in both precisions (the relaxation's)
   float *res = nullptr, *diag = nullptr, *lam_i = nullptr, *lam_j = nullptr, *dq = nullptr;
   double* wall_geo = nullptr;
   double* partials = nullptr;
   double* wall_partials = nullptr;
   double* sums = nullptr;
   Control* control = nullptr;
-  BoundaryDev* bnd = nullptr;
-  WallBoundary* wall_bnd = nullptr;
-  // argument records and work lists (device)
-  TileArgs* tile_args = nullptr;     // N_TILE_ROLES x count
-  RelaxArgs* relax_args = nullptr;   // 3 (cur) x count
-  UpdateArgs* update_args = nullptr; // 3 (cur) x count
-  WallArgs* wall_args = nullptr;     // 3 (cur) x count
+  void* bnd = nullptr;       // BoundaryDevT per case
+  void* wall_bnd = nullptr;  // WallBoundaryT per case
+  // argument records (TileArgsT, RelaxArgsT, UpdateArgsT, WallArgsT of the
+  // state's type) and work lists (device)
+  void* tile_args = nullptr;    // N_TILE_ROLES x count
+  void* relax_args = nullptr;   // 3 (cur) x count
+  void* update_args = nullptr;  // 3 (cur) x count
+  void* wall_args = nullptr;    // 3 (cur) x count
   WorkItem* work_tiles[2] = {nullptr, nullptr};
   WorkItem* work_relax = nullptr;
   WorkItem* work_wall = nullptr;
   int n_tiles[2] = {0, 0}, n_relax = 0, n_wall = 0;
-  bool relax_linear = false;  // BS_RELAX_LINEARIZED
+  bool relax_linear = false;  // BS_RELAX_LINEARIZED (always on an FP64 batch)
   int *seg_first = nullptr, *seg_count = nullptr, *seg_cells = nullptr, *seg_cases = nullptr;
   int *wall_first = nullptr, *wall_count = nullptr;
   int n_seg = 0;
@@ -150,20 +155,51 @@ struct BatchGuard {
   if (!_bguard.ok) return fail(BS_ERR_BUSY, "bs_batch: await the previous operation first"); \
   (void)hipSetDevice((b)->device)
 
-StateDev batch_state(bs_batch* b, int buf, const BatchCase& k) {
-  StateDev s;
-  for (int q = 0; q < 4; q++) s.q[q] = b->state[buf] + 4 * k.cell + (size_t)q * k.n;
+// The gamma the state's conserved variables are formed with: FP32 for an
+// FP32 state (euler2d-gpu.ts:412-432), the configured one for FP64.
+double state_gamma(const bs_batch* b) { return b->fp64 ? b->scheme.gamma : (double)b->gamma32; }
+size_t real_size(const bs_batch* b) { return b->fp64 ? sizeof(double) : sizeof(float); }
+
+template <typename T>
+T* arena(void* base, size_t offset) {
+  return static_cast<T*>(base) + offset;
+}
+
+template <typename T>
+StateDevT<T> batch_state(bs_batch* b, int buf, const BatchCase& k) {
+  StateDevT<T> s;
+  for (int q = 0; q < 4; q++) s.q[q] = arena<T>(b->state[buf], 4 * k.cell + (size_t)q * k.n);
   return s;
 }
 
-GridDev batch_grid(bs_batch* b, const BatchCase& k) {
-  GridDev G;
+template <typename T>
+GridDevT<T> batch_grid(bs_batch* b, const BatchCase& k) {
+  GridDevT<T> G;
   G.ni = k.ni;
   G.nj = k.nj;
-  G.inx = b->inx + k.iface; G.iny = b->iny + k.iface; G.ia = b->ia + k.iface;
-  G.jnx = b->jnx + k.jface; G.jny = b->jny + k.jface; G.ja = b->ja + k.jface;
-  G.vol = b->vol + k.cell; G.planar = b->planar + k.cell;
+  G.inx = arena<T>(b->inx, k.iface); G.iny = arena<T>(b->iny, k.iface); G.ia = arena<T>(b->ia, k.iface);
+  G.jnx = arena<T>(b->jnx, k.jface); G.jny = arena<T>(b->jny, k.jface); G.ja = arena<T>(b->ja, k.jface);
+  G.vol = arena<T>(b->vol, k.cell); G.planar = arena<T>(b->planar, k.cell);
   return G;
+}
+
+bs_status real_alloc(const bs_batch* b, void** p, size_t count, const char* what) {
+  const size_t bytes = (count ? count : 1) * real_size(b);
+  const hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(BS_ERR_MEMORY, "allocating %s (%zu bytes): %s", what, bytes, hipGetErrorString(e));
+  }
+  return BS_OK;
+}
+
+bs_status byte_alloc(void** p, size_t bytes, const char* what) {
+  const hipError_t e = hipMalloc(p, bytes ? bytes : 16);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(BS_ERR_MEMORY, "allocating %s (%zu bytes): %s", what, bytes, hipGetErrorString(e));
+  }
+  return BS_OK;
 }
 
 void batch_free(bs_batch* b) {
@@ -183,33 +219,34 @@ void batch_free(bs_batch* b) {
 
 // Argument records of every role for every case (uploaded once; the
 // implicit records when first needed).
+template <typename T>
 bs_status batch_upload_tile_args(bs_batch* b) {
   const int n = b->count;
-  std::vector<TileArgs> recs((size_t)N_TILE_ROLES * n);
-  const float *ca, *cb;
+  std::vector<TileArgsT<T>> recs((size_t)N_TILE_ROLES * n);
+  const T *ca, *cb;
   const int ns = rk_table(b->scheme.integrator, &ca, &cb);
   for (int c = 0; c < n; c++) {
     const BatchCase& k = b->cases[c];
-    TileArgs base;
+    TileArgsT<T> base;
     memset(&base, 0, sizeof base);
-    base.G = batch_grid(b, k);
-    base.B = b->bnd + c;
-    base.gamma = b->gamma32;
+    base.G = batch_grid<T>(b, k);
+    base.B = static_cast<const BoundaryDevT<T>*>(b->bnd) + c;
+    base.gamma = (T)state_gamma(b);
     base.axisymmetric = b->scheme.geometry == BS_GEOMETRY_AXISYMMETRIC;
     base.limiter = b->scheme.limiter;
-    base.cfl = (float)b->scheme.cfl;
-    base.dt = b->dt + k.cell;
+    base.cfl = (T)b->scheme.cfl;
+    base.dt = arena<T>(b->dt, k.cell);
     base.control = b->control + c;
     for (int cur = 0; cur < 3; cur++) {
       const int other[2] = {(cur + 1) % 3, (cur + 2) % 3};
       int prev = cur;
       for (int s = 0; s < 3; s++) {
-        TileArgs A = base;
+        TileArgsT<T> A = base;
         if (s < ns) {
           const int dst = (s == ns - 1 && s > 0) ? cur : other[s % 2];
-          A.u0 = batch_state(b, cur, k);
-          A.src = batch_state(b, prev, k);
-          A.dst = batch_state(b, dst, k);
+          A.u0 = batch_state<T>(b, cur, k);
+          A.src = batch_state<T>(b, prev, k);
+          A.dst = batch_state<T>(b, dst, k);
           A.a = ca[s];
           A.b = cb[s];
           A.dt_mode = s == 0 ? DT_COMPUTE_LOCAL : DT_READ;
@@ -218,12 +255,12 @@ bs_status batch_upload_tile_args(bs_batch* b) {
         }
         recs[(size_t)(ROLE_STAGE0 + 3 * cur + s) * n + c] = A;
       }
-      TileArgs R = base;
-      R.src = batch_state(b, cur, k);
+      TileArgsT<T> R = base;
+      R.src = batch_state<T>(b, cur, k);
       R.partials = b->partials;
       recs[(size_t)(ROLE_RESIDUAL + cur) * n + c] = R;
-      TileArgs P = base;
-      P.src = batch_state(b, cur, k);
+      TileArgsT<T> P = base;
+      P.src = batch_state<T>(b, cur, k);
       if (b->implicit_ready) {
         for (int q = 0; q < 4; q++) {
           P.res[q] = b->res + 4 * k.cell + (size_t)q * k.n;
@@ -236,25 +273,27 @@ bs_status batch_upload_tile_args(bs_batch* b) {
       recs[(size_t)(ROLE_PREPARE + cur) * n + c] = P;
     }
   }
-  HIP_TRY(hipMemcpy(b->tile_args, recs.data(), recs.size() * sizeof(TileArgs), hipMemcpyHostToDevice),
+  HIP_TRY(hipMemcpy(b->tile_args, recs.data(), recs.size() * sizeof(TileArgsT<T>), hipMemcpyHostToDevice),
           "batch records");
   return BS_OK;
 }
 
+template <typename T>
 bs_status batch_upload_implicit_args(bs_batch* b, double omega, double cfl_max, double growth) {
   const int n = b->count;
-  std::vector<RelaxArgs> rel((size_t)3 * n);
-  std::vector<UpdateArgs> upd((size_t)3 * n);
-  std::vector<TileArgs> prep((size_t)3 * n);
-  HIP_TRY(hipMemcpy(prep.data(), b->tile_args + (size_t)ROLE_PREPARE * n, prep.size() * sizeof(TileArgs),
+  std::vector<RelaxArgsT<T>> rel((size_t)3 * n);
+  std::vector<UpdateArgsT<T>> upd((size_t)3 * n);
+  std::vector<TileArgsT<T>> prep((size_t)3 * n);
+  TileArgsT<T>* tile_args = static_cast<TileArgsT<T>*>(b->tile_args);
+  HIP_TRY(hipMemcpy(prep.data(), tile_args + (size_t)ROLE_PREPARE * n, prep.size() * sizeof(TileArgsT<T>),
                     hipMemcpyDeviceToHost),
           "batch records");
   for (int c = 0; c < n; c++) {
     const BatchCase& k = b->cases[c];
     for (int cur = 0; cur < 3; cur++) {
-      RelaxArgs R;
-      R.G = batch_grid(b, k);
-      R.q = batch_state(b, cur, k);
+      RelaxArgsT<T> R;
+      R.G = batch_grid<T>(b, k);
+      R.q = batch_state<T>(b, cur, k);
       for (int q = 0; q < 4; q++) {
         R.res[q] = b->res + 4 * k.cell + (size_t)q * k.n;
         R.dq[q] = b->dq + 4 * k.cell + (size_t)q * k.n;
@@ -265,22 +304,24 @@ bs_status batch_upload_implicit_args(bs_batch* b, double omega, double cfl_max, 
       R.gamma = b->gamma32;
       R.omega = (float)omega;
       rel[(size_t)cur * n + c] = R;
-      UpdateArgs U;
-      U.src = batch_state(b, cur, k);
-      U.dst = batch_state(b, (cur + 1) % 3, k);
+      UpdateArgsT<T> U;
+      U.src = batch_state<T>(b, cur, k);
+      U.dst = batch_state<T>(b, (cur + 1) % 3, k);
       for (int q = 0; q < 4; q++) U.dq.q[q] = b->dq + 4 * k.cell + (size_t)q * k.n;
       U.n = k.n;
-      U.gamma = b->gamma32;
+      U.gamma = (T)state_gamma(b);
       U.control = b->control + c;
       U.cfl_max = cfl_max;
       U.growth = growth;
       upd[(size_t)cur * n + c] = U;
-      prep[(size_t)cur * n + c].omega = (float)omega;
+      prep[(size_t)cur * n + c].omega = (T)omega;
     }
   }
-  HIP_TRY(hipMemcpy(b->relax_args, rel.data(), rel.size() * sizeof(RelaxArgs), hipMemcpyHostToDevice), "records");
-  HIP_TRY(hipMemcpy(b->update_args, upd.data(), upd.size() * sizeof(UpdateArgs), hipMemcpyHostToDevice), "records");
-  HIP_TRY(hipMemcpy(b->tile_args + (size_t)ROLE_PREPARE * n, prep.data(), prep.size() * sizeof(TileArgs),
+  HIP_TRY(hipMemcpy(b->relax_args, rel.data(), rel.size() * sizeof(RelaxArgsT<T>), hipMemcpyHostToDevice),
+          "records");
+  HIP_TRY(hipMemcpy(b->update_args, upd.data(), upd.size() * sizeof(UpdateArgsT<T>), hipMemcpyHostToDevice),
+          "records");
+  HIP_TRY(hipMemcpy(tile_args + (size_t)ROLE_PREPARE * n, prep.data(), prep.size() * sizeof(TileArgsT<T>),
                     hipMemcpyHostToDevice),
           "records");
   return BS_OK;
@@ -348,48 +389,63 @@ bs_status batch_build_lists(bs_batch* b) {
   return BS_OK;
 }
 
+template <typename T>
 bs_status batch_launch_tiles(bs_batch* b, int mode, int role) {
-  const TileArgs* recs = b->tile_args + (size_t)role * b->count;
+  const TileArgsT<T>* recs = static_cast<const TileArgsT<T>*>(b->tile_args) + (size_t)role * b->count;
   for (int cl = 0; cl < 2; cl++) {
     if (b->n_tiles[cl] == 0) continue;
-    const int threads = cl == 0 ? BTX_L * BTY_L : BTX_S * BTY_S;
-    hipLaunchKernelGGL(cl == 0 ? batch_tile_fn_large(b->scheme, mode) : batch_tile_fn_small(b->scheme, mode), dim3(b->n_tiles[cl]), dim3(threads), 0, b->stream, recs,
+    const int threads = cl == 0 ? BatchTile<T>::lx * BatchTile<T>::ly : BatchTile<T>::sx * BatchTile<T>::sy;
+    hipLaunchKernelGGL(batch_tile_fn<T>(b->scheme, cl, mode), dim3(b->n_tiles[cl]), dim3(threads), 0, b->stream, recs,
                        b->work_tiles[cl]);
   }
   HIP_TRY(hipGetLastError(), "batch stencil launch");
   return BS_OK;
 }
 
+bs_status launch_tiles(bs_batch* b, int mode, int role) {
+  return b->fp64 ? batch_launch_tiles<double>(b, mode, role) : batch_launch_tiles<float>(b, mode, role);
+}
+
+template <typename T>
 bs_status batch_encode_explicit(bs_batch* b) {
-  const float *ca, *cb;
+  const T *ca, *cb;
   const int ns = rk_table(b->scheme.integrator, &ca, &cb);
   for (int s = 0; s < ns; s++) {
-    bs_status st = batch_launch_tiles(b, TILE_STAGE, ROLE_STAGE0 + 3 * b->cur + s);
+    bs_status st = batch_launch_tiles<T>(b, TILE_STAGE, ROLE_STAGE0 + 3 * b->cur + s);
     if (st) return st;
   }
   if (ns == 1) b->cur = (b->cur + 1) % 3;
   return BS_OK;
 }
 
+template <typename T>
 bs_status batch_encode_implicit(bs_batch* b, int sweeps) {
   const int cur = b->cur;
-  bs_status st = batch_launch_tiles(b, TILE_IMPLICIT, ROLE_PREPARE + cur);
+  bs_status st = batch_launch_tiles<T>(b, TILE_IMPLICIT, ROLE_PREPARE + cur);
   if (st) return st;
-  const RelaxArgs* rel = b->relax_args + (size_t)cur * b->count;
+  const RelaxArgsT<T>* rel = static_cast<const RelaxArgsT<T>*>(b->relax_args) + (size_t)cur * b->count;
   // The first red half-sweep (formed by the first launch) and the
   // half-sweeps after it, up to three per launch; the last launch writes
-  // the updated states.
-  const UpdateArgs* upd = b->update_args + (size_t)cur * b->count;
+  // the updated states (from the FP64 state in HBM on an FP64 batch).
+  const UpdateArgsT<T>* upd = static_cast<const UpdateArgsT<T>*>(b->update_args) + (size_t)cur * b->count;
+  using K = void (*)(const RelaxArgsT<T>*, const WorkItem*, int, int, const UpdateArgsT<T>*);
+  K kernels[2][3];
+  if constexpr (std::is_same<T, float>::value) {
+    kernels[0][0] = batch_relax_tile_kernel<T, 1, false>;
+    kernels[0][1] = batch_relax_tile_kernel<T, 2, false>;
+    kernels[0][2] = batch_relax_tile_kernel<T, 3, false>;
+  } else {
+    kernels[0][0] = kernels[0][1] = kernels[0][2] = nullptr;  // an FP64 state relaxes linearized only
+  }
+  kernels[1][0] = batch_relax_tile_kernel<T, 1, true>;
+  kernels[1][1] = batch_relax_tile_kernel<T, 2, true>;
+  kernels[1][2] = batch_relax_tile_kernel<T, 3, true>;
+  const int form = (b->relax_linear || !std::is_same<T, float>::value) ? 1 : 0;
   for (int done = 0, colour = 1, halves = 2 * sweeps - 1; done < halves && b->n_relax;) {
     const int nh = halves - done < 3 ? halves - done : 3;
     const int init = done == 0;
-    const UpdateArgs* u = done + nh == halves ? upd : nullptr;
-    const dim3 grid(b->n_relax), block(RTX * RTY);
-    using K = void (*)(const RelaxArgs*, const WorkItem*, int, int, const UpdateArgs*);
-    static const K kernels[2][3] = {
-        {batch_relax_tile_kernel<1, false>, batch_relax_tile_kernel<2, false>, batch_relax_tile_kernel<3, false>},
-        {batch_relax_tile_kernel<1, true>, batch_relax_tile_kernel<2, true>, batch_relax_tile_kernel<3, true>}};
-    hipLaunchKernelGGL(kernels[b->relax_linear ? 1 : 0][nh - 1], grid, block, 0, b->stream, rel, b->work_relax,
+    const UpdateArgsT<T>* u = done + nh == halves ? upd : nullptr;
+    hipLaunchKernelGGL(kernels[form][nh - 1], dim3(b->n_relax), dim3(RTX * RTY), 0, b->stream, rel, b->work_relax,
                        colour, init, u);
     colour = (colour + nh) & 1;
     done += nh;
@@ -429,6 +485,66 @@ bs_status batch_reset_control(bs_batch* b, int first, int count) {
   return BS_OK;
 }
 
+// Uploads `packed` (4 x n values of the case, conserved) as case c's
+// current state.
+template <typename T>
+bs_status upload_case_state(bs_batch* b, const BatchCase& k, const std::vector<T>& packed) {
+  HIP_TRY(hipMemcpy(arena<T>(b->state[b->cur], 4 * k.cell), packed.data(), packed.size() * sizeof(T),
+                    hipMemcpyHostToDevice),
+          "state upload");
+  return BS_OK;
+}
+
+template <typename T>
+bs_status batch_set_uniform(bs_batch* b, const double* states) {
+  for (int c = 0; c < b->count; c++) {
+    const BatchCase& k = b->cases[c];
+    const double* w = states + 4 * (size_t)c;
+    if (!physical(w)) return fail(BS_ERR_NONPHYSICAL, "non-physical initial state of case %d", c);
+    T q[4];
+    pack_cons(w, state_gamma(b), q);
+    std::vector<T> v(4 * (size_t)k.n);
+    for (int qq = 0; qq < 4; qq++) std::fill(v.begin() + (size_t)qq * k.n, v.begin() + (size_t)(qq + 1) * k.n, q[qq]);
+    bs_status st = upload_case_state<T>(b, k, v);
+    if (st) return st;
+  }
+  return batch_reset_control(b, 0, b->count);
+}
+
+template <typename T>
+bs_status batch_set_primitives(bs_batch* b, int index, const double* prim) {
+  const BatchCase& k = b->cases[index];
+  std::vector<T> v(4 * (size_t)k.n);
+  for (int c = 0; c < k.n; c++) {
+    const double* w = prim + 4 * (size_t)c;
+    if (!physical(w)) return fail(BS_ERR_NONPHYSICAL, "non-physical state in cell %d of case %d", c, index);
+    T q[4];
+    pack_cons(w, state_gamma(b), q);
+    for (int qq = 0; qq < 4; qq++) v[(size_t)qq * k.n + c] = q[qq];
+  }
+  bs_status st = upload_case_state<T>(b, k, v);
+  if (st) return st;
+  return batch_reset_control(b, index, 1);
+}
+
+// Reads case `index`'s state into four arrays of type V (null ones skipped).
+template <typename T, typename V>
+bs_status batch_read_state(bs_batch* b, int index, V* const dst[4]) {
+  const BatchCase& k = b->cases[index];
+  std::vector<T> v(k.n);
+  for (int q = 0; q < 4; q++) {
+    if (!dst[q]) continue;
+    const T* src = arena<T>(b->state[b->cur], 4 * k.cell + (size_t)q * k.n);
+    if (std::is_same<T, V>::value) {
+      HIP_TRY(hipMemcpy(dst[q], src, k.n * sizeof(T), hipMemcpyDeviceToHost), "state readback");
+    } else {
+      HIP_TRY(hipMemcpy(v.data(), src, k.n * sizeof(T), hipMemcpyDeviceToHost), "state readback");
+      for (int c = 0; c < k.n; c++) dst[q][c] = (V)v[c];
+    }
+  }
+  return BS_OK;
+}
+
 }  // namespace
 
 // The warm start's coarse fields to the cases' cells: case c's coarse
@@ -436,11 +552,13 @@ bs_status batch_reset_control(bs_batch* b, int first, int count) {
 // per variable, interpolated linearly between the cell centres of the two
 // uniform grids in (i / ni, j / nj), clamped at the ends; density and
 // pressure held at least `floor` (NaN kept, and flagged); stored conserved
-// in FP32 with the gamma the solver uses. The arithmetic is FP64 in the
-// order of the host's resampling, without contraction, so the state equals
-// the one set from the same field cell by cell on the host.
+// in the state's precision with the gamma the solver uses. The arithmetic
+// is FP64 in the order of the host's resampling, without contraction, so
+// the state equals the one set from the same field cell by cell on the
+// host.
+template <typename T>
 struct CoarseCase {
-  float* state;  // the case's 4 x n conserved values
+  T* state;  // the case's 4 x n conserved values
   int n, ni;
   int nj;
   const double* coarse;
@@ -456,7 +574,8 @@ __device__ __forceinline__ void coarse_axis(int k, int nd, int ns, int& a0, int&
   a1 = min(a0 + 1, ns - 1);
 }
 
-__global__ __launch_bounds__(256) void batch_coarse_kernel(CoarseCase K, int cni, int cnj, double floor_, double g,
+template <typename T>
+__global__ __launch_bounds__(256) void batch_coarse_kernel(CoarseCase<T> K, int cni, int cnj, double floor_, double g,
                                                            int* bad) {
 #pragma clang fp contract(off)
   const int c = blockIdx.x * 256 + threadIdx.x;
@@ -478,10 +597,10 @@ __global__ __launch_bounds__(256) void batch_coarse_kernel(CoarseCase K, int cni
   if (w[0] < floor_) w[0] = floor_;  // a NaN stays, and fails the check below
   if (w[3] < floor_) w[3] = floor_;
   if (!(w[0] > 0.0 && w[3] > 0.0 && isfinite(w[1]) && isfinite(w[2]))) atomicOr(bad, 1);
-  K.state[c] = (float)w[0];
-  K.state[(size_t)K.n + c] = (float)(w[0] * w[1]);
-  K.state[2 * (size_t)K.n + c] = (float)(w[0] * w[2]);
-  K.state[3 * (size_t)K.n + c] = (float)(w[3] / (g - 1) + 0.5 * w[0] * (w[1] * w[1] + w[2] * w[2]));
+  K.state[c] = (T)w[0];
+  K.state[(size_t)K.n + c] = (T)(w[0] * w[1]);
+  K.state[2 * (size_t)K.n + c] = (T)(w[0] * w[2]);
+  K.state[3 * (size_t)K.n + c] = (T)(w[3] / (g - 1) + 0.5 * w[0] * (w[1] * w[1] + w[2] * w[2]));
 }
 
 // ---- the allocation-free warm start (bs_batch_set_coarse_noalloc) --------
@@ -523,6 +642,7 @@ __global__ void clear_flag_kernel(int* flag) { *flag = 0; }
 // The coarse fields of every case through a staging area of `room` bytes
 // (whole cases at a time, in order), the fields' flag in `bad` (cleared
 // first by a kernel), then the flag read back; allocates nothing.
+template <typename T>
 bs_status coarse_through(bs_batch* b, const double* coarse, int cni, int cnj, const double* scale,
                          double floor_value, double* staging, size_t room, int* bad) {
   const size_t per = 4 * (size_t)cni * cnj;
@@ -540,15 +660,15 @@ bs_status coarse_through(bs_batch* b, const double* coarse, int cni, int cnj, co
             "coarse fields upload");
     for (int c = c0; c < c0 + n; c++) {
       const BatchCase& k = b->cases[c];
-      CoarseCase K;
-      K.state = b->state[b->cur] + 4 * k.cell;
+      CoarseCase<T> K;
+      K.state = arena<T>(b->state[b->cur], 4 * k.cell);
       K.n = k.n;
       K.ni = k.ni;
       K.nj = k.nj;
       K.coarse = staging + per * (c - c0);
       for (int q = 0; q < 4; q++) K.scale[q] = scale[4 * (size_t)c + q];
-      hipLaunchKernelGGL(batch_coarse_kernel, dim3((k.n + 255) / 256), dim3(256), 0, b->stream, K, cni, cnj,
-                         floor_value, (double)b->gamma32, bad);
+      hipLaunchKernelGGL(batch_coarse_kernel<T>, dim3((k.n + 255) / 256), dim3(256), 0, b->stream, K, cni, cnj,
+                         floor_value, state_gamma(b), bad);
     }
     HIP_TRY(hipGetLastError(), "coarse fields launch");
   }
@@ -559,11 +679,96 @@ bs_status coarse_through(bs_batch* b, const double* coarse, int cni, int cnj, co
   return batch_reset_control(b, 0, b->count);
 }
 
-}  // namespace
+bs_status coarse_into(bs_batch* b, const double* coarse, int cni, int cnj, const double* scale, double floor_value,
+                      double* staging, size_t room, int* bad) {
+  return b->fp64 ? coarse_through<double>(b, coarse, cni, cnj, scale, floor_value, staging, room, bad)
+                 : coarse_through<float>(b, coarse, cni, cnj, scale, floor_value, staging, room, bad);
+}
 
-extern "C" {
+// Metrics, boundary tables, controls and wall records of every case.
+template <typename T>
+bs_status batch_setup(bs_batch* b, const bs_config* configs) {
+  const int count = b->count;
+  std::vector<BoundaryDevT<T>> bnds(count);
+  std::vector<WallBoundaryT<T>> wbs(count);
+  std::vector<Control> ctl(count);
+  std::vector<WallArgsT<T>> wargs((size_t)3 * count);
+  for (int c = 0; c < count; c++) {
+    const bs_config& cf = configs[c];
+    const BatchCase& k = b->cases[c];
+    const int ni = k.ni, nj = k.nj;
+    const size_t fi = (size_t)(ni + 1) * nj, fj = (size_t)ni * (nj + 1);
+    std::vector<double> inx(fi), iny(fi), ia(fi), jnx(fj), jny(fj), ja(fj), vol(k.n), planar(k.n);
+    bs_status st = metrics(ni, nj, cf.x, cf.y, cf.geometry == BS_GEOMETRY_AXISYMMETRIC, inx.data(), iny.data(),
+                           ia.data(), jnx.data(), jny.data(), ja.data(), vol.data(), planar.data(), nullptr, nullptr);
+    if (st) return fail(st, "case %d: %s", c, bs_last_error());
+    auto put = [&](void* base, size_t off, const std::vector<double>& v) -> bs_status {
+      std::vector<T> f(v.begin(), v.end());
+      HIP_TRY(hipMemcpyAsync(arena<T>(base, off), f.data(), f.size() * sizeof(T), hipMemcpyHostToDevice, b->stream),
+              "metrics upload");
+      HIP_TRY(hipStreamSynchronize(b->stream), "metrics upload");
+      return BS_OK;
+    };
+    if ((st = put(b->inx, k.iface, inx)) || (st = put(b->iny, k.iface, iny)) || (st = put(b->ia, k.iface, ia)) ||
+        (st = put(b->jnx, k.jface, jnx)) || (st = put(b->jny, k.jface, jny)) || (st = put(b->ja, k.jface, ja)) ||
+        (st = put(b->vol, k.cell, vol)) || (st = put(b->planar, k.cell, planar)))
+      return st;
+    // The jMin side's FP64 face geometry (the body wall).
+    std::vector<double> geo(5 * (size_t)ni);
+    for (int t = 0; t < ni; t++) {
+      const int n0 = t, n1 = t + 1;
+      geo[5 * t + 0] = 0.5 * (cf.x[n0] + cf.x[n1]);
+      geo[5 * t + 1] = 0.5 * (cf.y[n0] + cf.y[n1]);
+      geo[5 * t + 2] = jnx[t];
+      geo[5 * t + 3] = jny[t];
+      geo[5 * t + 4] = ja[t];
+    }
+    HIP_TRY(hipMemcpy(b->wall_geo + 5 * k.wall, geo.data(), geo.size() * sizeof(double), hipMemcpyHostToDevice),
+            "wall geometry");
+    BoundaryDevT<T>& B = bnds[c];
+    WallBoundaryT<T>& W = wbs[c];
+    for (int sd = 0; sd < 4; sd++) {
+      const bs_boundary& bb = cf.boundaries[sd];
+      B.kind[sd] = bb.kind;
+      B.state[sd][0] = (T)bb.rho; B.state[sd][1] = (T)bb.u;
+      B.state[sd][2] = (T)bb.v; B.state[sd][3] = (T)bb.p;
+      B.face_cons[sd] = nullptr;
+      B.face_flux[sd] = nullptr;
+      W.kind[sd] = bb.kind;
+      W.state[sd][0] = bb.rho; W.state[sd][1] = bb.u; W.state[sd][2] = bb.v; W.state[sd][3] = bb.p;
+      W.cons[sd] = nullptr;
+    }
+    memset(&ctl[c], 0, sizeof(Control));
+    ctl[c].cfl = 1.0;
+    ctl[c].t_end = 3e38;
+    for (int cur = 0; cur < 3; cur++) {
+      WallArgsT<T>& A = wargs[(size_t)cur * count + c];
+      memset(&A, 0, sizeof A);
+      A.G = batch_grid<T>(b, k);
+      A.s = batch_state<T>(b, cur, k);
+      A.side = JMIN;
+      A.count = ni;
+      A.kind = cf.boundaries[JMIN].kind;
+      A.face_geo = b->wall_geo + 5 * k.wall;
+      A.gamma = cf.gamma;
+      A.gamma32 = state_gamma(b);
+      A.rec = cf.reconstruction;
+      A.limiter = cf.limiter;
+      A.flux = cf.flux;
+      A.partials = b->wall_partials;
+    }
+  }
+  HIP_TRY(hipMemcpy(b->bnd, bnds.data(), bnds.size() * sizeof(BoundaryDevT<T>), hipMemcpyHostToDevice),
+          "boundaries");
+  HIP_TRY(hipMemcpy(b->wall_bnd, wbs.data(), wbs.size() * sizeof(WallBoundaryT<T>), hipMemcpyHostToDevice),
+          "boundaries");
+  HIP_TRY(hipMemcpy(b->control, ctl.data(), ctl.size() * sizeof(Control), hipMemcpyHostToDevice), "control");
+  HIP_TRY(hipMemcpy(b->wall_args, wargs.data(), wargs.size() * sizeof(WallArgsT<T>), hipMemcpyHostToDevice),
+          "records");
+  return batch_upload_tile_args<T>(b);
+}
 
-bs_status bs_batch_create(bs_ctx* ctx, const bs_config* configs, int32_t count, bs_batch** out) {
+bs_status batch_create(bs_ctx* ctx, const bs_config* configs, int32_t count, bool fp64, bs_batch** out) {
   if (!ctx || !configs || !out || count < 1) return fail(BS_ERR_RANGE, "a batch needs at least one case");
   *out = nullptr;
   const bs_config& s0 = configs[0];
@@ -598,6 +803,8 @@ bs_status bs_batch_create(bs_ctx* ctx, const bs_config* configs, int32_t count, 
   b->scheme = s0;
   b->scheme.x = b->scheme.y = nullptr;
   b->gamma32 = (float)s0.gamma;
+  b->fp64 = fp64;
+  b->relax_linear = fp64;
   b->count = count;
   b->cases.resize(count);
   b->active.assign(count, 1);
@@ -607,6 +814,10 @@ bs_status bs_batch_create(bs_ctx* ctx, const bs_config* configs, int32_t count, 
     delete b;
     return code;
   };
+  const int lx = fp64 ? BatchTile<double>::lx : BatchTile<float>::lx;
+  const int ly = fp64 ? BatchTile<double>::ly : BatchTile<float>::ly;
+  const int sx = fp64 ? BatchTile<double>::sx : BatchTile<float>::sx;
+  const int sy = fp64 ? BatchTile<double>::sy : BatchTile<float>::sy;
   // Offsets.
   for (int c = 0; c < count; c++) {
     BatchCase& k = b->cases[c];
@@ -621,10 +832,10 @@ bs_status bs_batch_create(bs_ctx* ctx, const bs_config* configs, int32_t count, 
     b->ifaces += (size_t)(k.ni + 1) * k.nj;
     b->jfaces += (size_t)k.ni * (k.nj + 1);
     b->walls += k.ni;
-    k.tiles_x[0] = (k.ni + BTX_L - 1) / BTX_L;
-    k.tiles_y[0] = (k.nj + BTY_L - 1) / BTY_L;
-    k.tiles_x[1] = (k.ni + BTX_S - 1) / BTX_S;
-    k.tiles_y[1] = (k.nj + BTY_S - 1) / BTY_S;
+    k.tiles_x[0] = (k.ni + lx - 1) / lx;
+    k.tiles_y[0] = (k.nj + ly - 1) / ly;
+    k.tiles_x[1] = (k.ni + sx - 1) / sx;
+    k.tiles_y[1] = (k.nj + sy - 1) / sy;
     k.slot0 = b->slots;
     const int t0 = k.tiles_x[0] * k.tiles_y[0], t1 = k.tiles_x[1] * k.tiles_y[1];
     k.slots = t0 > t1 ? t0 : t1;
@@ -638,98 +849,71 @@ bs_status bs_batch_create(bs_ctx* ctx, const bs_config* configs, int32_t count, 
     return bail(fail(BS_ERR_EXECUTION, "hipStreamCreate failed"));
   bs_status st;
   for (int q = 0; q < 3; q++)
-    if ((st = dev_alloc(&b->state[q], 4 * b->cells, "batch state"))) return bail(st);
-  if ((st = dev_alloc(&b->inx, b->ifaces, "metrics")) || (st = dev_alloc(&b->iny, b->ifaces, "metrics")) ||
-      (st = dev_alloc(&b->ia, b->ifaces, "metrics")) || (st = dev_alloc(&b->jnx, b->jfaces, "metrics")) ||
-      (st = dev_alloc(&b->jny, b->jfaces, "metrics")) || (st = dev_alloc(&b->ja, b->jfaces, "metrics")) ||
-      (st = dev_alloc(&b->vol, b->cells, "metrics")) || (st = dev_alloc(&b->planar, b->cells, "metrics")) ||
-      (st = dev_alloc(&b->dt, b->cells, "time steps")) || (st = dev_alloc(&b->wall_geo, 5 * b->walls, "wall")) ||
+    if ((st = real_alloc(b, &b->state[q], 4 * b->cells, "batch state"))) return bail(st);
+  const size_t tile_bytes = fp64 ? sizeof(TileArgsT<double>) : sizeof(TileArgsT<float>);
+  const size_t relax_bytes = fp64 ? sizeof(RelaxArgsT<double>) : sizeof(RelaxArgsT<float>);
+  const size_t upd_bytes = fp64 ? sizeof(UpdateArgsT<double>) : sizeof(UpdateArgsT<float>);
+  const size_t wall_bytes = fp64 ? sizeof(WallArgsT<double>) : sizeof(WallArgsT<float>);
+  const size_t bnd_bytes = fp64 ? sizeof(BoundaryDevT<double>) : sizeof(BoundaryDevT<float>);
+  const size_t wb_bytes = fp64 ? sizeof(WallBoundaryT<double>) : sizeof(WallBoundaryT<float>);
+  if ((st = real_alloc(b, &b->inx, b->ifaces, "metrics")) || (st = real_alloc(b, &b->iny, b->ifaces, "metrics")) ||
+      (st = real_alloc(b, &b->ia, b->ifaces, "metrics")) || (st = real_alloc(b, &b->jnx, b->jfaces, "metrics")) ||
+      (st = real_alloc(b, &b->jny, b->jfaces, "metrics")) || (st = real_alloc(b, &b->ja, b->jfaces, "metrics")) ||
+      (st = real_alloc(b, &b->vol, b->cells, "metrics")) || (st = real_alloc(b, &b->planar, b->cells, "metrics")) ||
+      (st = real_alloc(b, &b->dt, b->cells, "time steps")) || (st = dev_alloc(&b->wall_geo, 5 * b->walls, "wall")) ||
       (st = dev_alloc(&b->partials, (size_t)b->slots, "partials")) ||
       (st = dev_alloc(&b->wall_partials, (size_t)b->wall_slots, "partials")) ||
       (st = dev_alloc(&b->sums, (size_t)count, "sums")) || (st = dev_alloc(&b->control, (size_t)count, "control")) ||
-      (st = dev_alloc(&b->bnd, (size_t)count, "boundaries")) ||
-      (st = dev_alloc(&b->wall_bnd, (size_t)count, "boundaries")) ||
-      (st = dev_alloc(&b->tile_args, (size_t)N_TILE_ROLES * count, "records")) ||
-      (st = dev_alloc(&b->relax_args, (size_t)3 * count, "records")) ||
-      (st = dev_alloc(&b->update_args, (size_t)3 * count, "records")) ||
-      (st = dev_alloc(&b->wall_args, (size_t)3 * count, "records")))
+      (st = byte_alloc(&b->bnd, bnd_bytes * count, "boundaries")) ||
+      (st = byte_alloc(&b->wall_bnd, wb_bytes * count, "boundaries")) ||
+      (st = byte_alloc(&b->tile_args, tile_bytes * N_TILE_ROLES * count, "records")) ||
+      (st = byte_alloc(&b->relax_args, relax_bytes * 3 * count, "records")) ||
+      (st = byte_alloc(&b->update_args, upd_bytes * 3 * count, "records")) ||
+      (st = byte_alloc(&b->wall_args, wall_bytes * 3 * count, "records")))
     return bail(st);
-  // Metrics per case (FP64 on the host), boundary tables, controls.
-  std::vector<BoundaryDev> bnds(count);
-  std::vector<WallBoundary> wbs(count);
-  std::vector<Control> ctl(count);
-  std::vector<WallArgs> wargs((size_t)3 * count);
-  for (int c = 0; c < count; c++) {
-    const bs_config& cf = configs[c];
-    const BatchCase& k = b->cases[c];
-    const int ni = k.ni, nj = k.nj;
-    const size_t fi = (size_t)(ni + 1) * nj, fj = (size_t)ni * (nj + 1);
-    std::vector<double> inx(fi), iny(fi), ia(fi), jnx(fj), jny(fj), ja(fj), vol(k.n), planar(k.n);
-    st = metrics(ni, nj, cf.x, cf.y, cf.geometry == BS_GEOMETRY_AXISYMMETRIC, inx.data(), iny.data(), ia.data(),
-                 jnx.data(), jny.data(), ja.data(), vol.data(), planar.data(), nullptr, nullptr);
-    if (st) return bail(fail(st, "case %d: %s", c, bs_last_error()));
-    auto put = [&](float* base, size_t off, const std::vector<double>& v) -> bs_status {
-      std::vector<float> f(v.begin(), v.end());
-      HIP_TRY(hipMemcpyAsync(base + off, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice, b->stream),
-              "metrics upload");
-      HIP_TRY(hipStreamSynchronize(b->stream), "metrics upload");
-      return BS_OK;
-    };
-    if ((st = put(b->inx, k.iface, inx)) || (st = put(b->iny, k.iface, iny)) || (st = put(b->ia, k.iface, ia)) ||
-        (st = put(b->jnx, k.jface, jnx)) || (st = put(b->jny, k.jface, jny)) || (st = put(b->ja, k.jface, ja)) ||
-        (st = put(b->vol, k.cell, vol)) || (st = put(b->planar, k.cell, planar)))
-      return bail(st);
-    // The jMin side's FP64 face geometry (the body wall).
-    std::vector<double> geo(5 * (size_t)ni);
-    for (int t = 0; t < ni; t++) {
-      const int n0 = t, n1 = t + 1;
-      geo[5 * t + 0] = 0.5 * (cf.x[n0] + cf.x[n1]);
-      geo[5 * t + 1] = 0.5 * (cf.y[n0] + cf.y[n1]);
-      geo[5 * t + 2] = jnx[t];
-      geo[5 * t + 3] = jny[t];
-      geo[5 * t + 4] = ja[t];
-    }
-    HIP_TRY(hipMemcpy(b->wall_geo + 5 * k.wall, geo.data(), geo.size() * sizeof(double), hipMemcpyHostToDevice),
-            "wall geometry");
-    BoundaryDev& B = bnds[c];
-    WallBoundary& W = wbs[c];
-    for (int sd = 0; sd < 4; sd++) {
-      const bs_boundary& bb = cf.boundaries[sd];
-      B.kind[sd] = bb.kind;
-      B.state[sd][0] = (float)bb.rho; B.state[sd][1] = (float)bb.u;
-      B.state[sd][2] = (float)bb.v; B.state[sd][3] = (float)bb.p;
-      B.face_cons[sd] = nullptr;
-      B.face_flux[sd] = nullptr;
-      W.kind[sd] = bb.kind;
-      W.state[sd][0] = bb.rho; W.state[sd][1] = bb.u; W.state[sd][2] = bb.v; W.state[sd][3] = bb.p;
-      W.cons[sd] = nullptr;
-    }
-    memset(&ctl[c], 0, sizeof(Control));
-    ctl[c].cfl = 1.0;
-    ctl[c].t_end = 3e38;
-    for (int cur = 0; cur < 3; cur++) {
-      WallArgs& A = wargs[(size_t)cur * count + c];
-      memset(&A, 0, sizeof A);
-      A.G = batch_grid(b, k);
-      A.s = batch_state(b, cur, k);
-      A.side = JMIN;
-      A.count = ni;
-      A.kind = cf.boundaries[JMIN].kind;
-      A.face_geo = b->wall_geo + 5 * k.wall;
-      A.gamma = cf.gamma;
-      A.gamma32 = (double)b->gamma32;
-      A.rec = cf.reconstruction;
-      A.limiter = cf.limiter;
-      A.flux = cf.flux;
-      A.partials = b->wall_partials;
-    }
-  }
-  HIP_TRY(hipMemcpy(b->bnd, bnds.data(), bnds.size() * sizeof(BoundaryDev), hipMemcpyHostToDevice), "boundaries");
-  HIP_TRY(hipMemcpy(b->wall_bnd, wbs.data(), wbs.size() * sizeof(WallBoundary), hipMemcpyHostToDevice), "boundaries");
-  HIP_TRY(hipMemcpy(b->control, ctl.data(), ctl.size() * sizeof(Control), hipMemcpyHostToDevice), "control");
-  HIP_TRY(hipMemcpy(b->wall_args, wargs.data(), wargs.size() * sizeof(WallArgs), hipMemcpyHostToDevice), "records");
-  if ((st = batch_upload_tile_args(b))) return bail(st);
+  if ((st = fp64 ? batch_setup<double>(b, configs) : batch_setup<float>(b, configs))) return bail(st);
   *out = b;
+  return BS_OK;
+}
+
+template <typename T>
+bs_status batch_drags(bs_batch* b, const double* p_inf, const double* q_inf, const double* base_radius, double* cd) {
+  // p_inf enters the wall kernel per case: set it in the records.
+  WallArgsT<T>* recs = static_cast<WallArgsT<T>*>(b->wall_args) + (size_t)b->cur * b->count;
+  std::vector<WallArgsT<T>> wargs((size_t)b->count);
+  HIP_TRY(hipMemcpy(wargs.data(), recs, wargs.size() * sizeof(WallArgsT<T>), hipMemcpyDeviceToHost), "records");
+  for (int c = 0; c < b->count; c++) wargs[c].p_inf = p_inf[c];
+  HIP_TRY(hipMemcpy(recs, wargs.data(), wargs.size() * sizeof(WallArgsT<T>), hipMemcpyHostToDevice), "records");
+  if (b->n_wall)
+    hipLaunchKernelGGL(batch_wall_fn<T>(b->scheme), dim3(b->n_wall), dim3(256), 0, b->stream, recs,
+                       static_cast<const WallBoundaryT<T>*>(b->wall_bnd), b->work_wall);
+  if (b->n_seg)
+    hipLaunchKernelGGL(segment_sums, dim3(b->n_seg), dim3(256), 0, b->stream, b->wall_partials, b->wall_first,
+                       b->wall_count, b->seg_cells, b->seg_cases, 0, b->sums);
+  std::vector<double> force(b->count);
+  HIP_TRY(hipMemcpyAsync(force.data(), b->sums, force.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream),
+          "drag readback");
+  HIP_TRY(hipStreamSynchronize(b->stream), "sync");
+  for (int c = 0; c < b->count; c++)
+    cd[c] = b->active[c] ? (2.0 * force[c]) / (q_inf[c] * base_radius[c] * base_radius[c]) : NAN;
+  return BS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+bs_status bs_batch_create(bs_ctx* ctx, const bs_config* configs, int32_t count, bs_batch** out) {
+  return batch_create(ctx, configs, count, false, out);
+}
+
+bs_status bs_batch_create_fp64(bs_ctx* ctx, const bs_config* configs, int32_t count, bs_batch** out) {
+  return batch_create(ctx, configs, count, true, out);
+}
+
+bs_status bs_batch_precision(bs_batch* b, int32_t* precision) {
+  if (!b || !precision) return fail(BS_ERR_RANGE, "null argument");
+  *precision = b->fp64 ? BS_PRECISION_FP64 : BS_PRECISION_FP32;
   return BS_OK;
 }
 
@@ -748,35 +932,13 @@ bs_status bs_batch_destroy(bs_batch* b) {
 bs_status bs_batch_set_uniform(bs_batch* b, const double* states) {
   BATCH_GUARD(b);
   if (!states) return fail(BS_ERR_RANGE, "no states");
-  for (int c = 0; c < b->count; c++) {
-    const BatchCase& k = b->cases[c];
-    const double* w = states + 4 * (size_t)c;
-    if (!physical(w)) return fail(BS_ERR_NONPHYSICAL, "non-physical initial state of case %d", c);
-    float q[4];
-    pack_cons(w, (double)b->gamma32, q);
-    std::vector<float> v(4 * (size_t)k.n);
-    for (int qq = 0; qq < 4; qq++) std::fill(v.begin() + (size_t)qq * k.n, v.begin() + (size_t)(qq + 1) * k.n, q[qq]);
-    HIP_TRY(hipMemcpy(b->state[b->cur] + 4 * k.cell, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice),
-            "state upload");
-  }
-  return batch_reset_control(b, 0, b->count);
+  return b->fp64 ? batch_set_uniform<double>(b, states) : batch_set_uniform<float>(b, states);
 }
 
 bs_status bs_batch_set_primitives(bs_batch* b, int32_t index, const double* prim) {
   BATCH_GUARD(b);
   if (index < 0 || index >= b->count || !prim) return fail(BS_ERR_RANGE, "no such case");
-  const BatchCase& k = b->cases[index];
-  std::vector<float> v(4 * (size_t)k.n);
-  for (int c = 0; c < k.n; c++) {
-    const double* w = prim + 4 * (size_t)c;
-    if (!physical(w)) return fail(BS_ERR_NONPHYSICAL, "non-physical state in cell %d of case %d", c, index);
-    float q[4];
-    pack_cons(w, (double)b->gamma32, q);
-    for (int qq = 0; qq < 4; qq++) v[(size_t)qq * k.n + c] = q[qq];
-  }
-  HIP_TRY(hipMemcpy(b->state[b->cur] + 4 * k.cell, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice),
-          "state upload");
-  return batch_reset_control(b, index, 1);
+  return b->fp64 ? batch_set_primitives<double>(b, index, prim) : batch_set_primitives<float>(b, index, prim);
 }
 
 bs_status bs_batch_set_coarse(bs_batch* b, const double* coarse, int32_t coarse_ni, int32_t coarse_nj,
@@ -784,45 +946,21 @@ bs_status bs_batch_set_coarse(bs_batch* b, const double* coarse, int32_t coarse_
   BATCH_GUARD(b);
   if (!coarse || !scale || coarse_ni < 1 || coarse_nj < 1) return fail(BS_ERR_RANGE, "no coarse fields");
   if (!(floor_value > 0)) return fail(BS_ERR_RANGE, "the floor must be positive");
-  const size_t per = 4 * (size_t)coarse_ni * coarse_nj;
+  const size_t bytes = 4 * (size_t)coarse_ni * coarse_nj * sizeof(double) * b->count;
   double* d_coarse = nullptr;
   int* d_bad = nullptr;
   auto release = [&]() {
     if (d_coarse) (void)hipFree(d_coarse);
     if (d_bad) (void)hipFree(d_bad);
   };
-  if (hipMalloc((void**)&d_coarse, per * b->count * sizeof(double)) != hipSuccess ||
-      hipMalloc((void**)&d_bad, sizeof(int)) != hipSuccess) {
+  if (hipMalloc((void**)&d_coarse, bytes) != hipSuccess || hipMalloc((void**)&d_bad, sizeof(int)) != hipSuccess) {
     release();
     return fail(BS_ERR_MEMORY, "coarse fields: out of device memory");
   }
-  bs_status st = BS_OK;
-  int bad = 0;
-  if (hipMemcpyAsync(d_coarse, coarse, per * b->count * sizeof(double), hipMemcpyHostToDevice, b->stream) !=
-          hipSuccess ||
-      hipMemsetAsync(d_bad, 0, sizeof(int), b->stream) != hipSuccess) {
-    st = fail(BS_ERR_EXECUTION, "coarse fields: upload failed");
-  }
-  for (int c = 0; st == BS_OK && c < b->count; c++) {
-    const BatchCase& k = b->cases[c];
-    CoarseCase K;
-    K.state = b->state[b->cur] + 4 * k.cell;
-    K.n = k.n;
-    K.ni = k.ni;
-    K.nj = k.nj;
-    K.coarse = d_coarse + per * c;
-    for (int q = 0; q < 4; q++) K.scale[q] = scale[4 * (size_t)c + q];
-    hipLaunchKernelGGL(batch_coarse_kernel, dim3((k.n + 255) / 256), dim3(256), 0, b->stream, K, (int)coarse_ni,
-                       (int)coarse_nj, floor_value, (double)b->gamma32, d_bad);
-    if (hipGetLastError() != hipSuccess) st = fail(BS_ERR_EXECUTION, "coarse fields: launch failed");
-  }
-  if (st == BS_OK && (hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
-                      hipStreamSynchronize(b->stream) != hipSuccess))
-    st = fail(BS_ERR_EXECUTION, "coarse fields: kernel failed");
+  // Every case's field in one group through the allocated staging.
+  const bs_status st = coarse_into(b, coarse, coarse_ni, coarse_nj, scale, floor_value, d_coarse, bytes, d_bad);
   release();
-  if (st) return st;
-  if (bad) return fail(BS_ERR_NONPHYSICAL, "a coarse field gives a non-physical state");
-  return batch_reset_control(b, 0, b->count);
+  return st;
 }
 
 bs_status bs_batch_set_coarse_noalloc(bs_batch* b, const double* coarse, int32_t coarse_ni, int32_t coarse_nj,
@@ -837,22 +975,23 @@ bs_status bs_batch_set_coarse_noalloc(bs_batch* b, const double* coarse, int32_t
   unsigned char* base = nullptr;
   bs_status st = scratch_address(b->device, &base);
   if (st) return st;
-  return coarse_through(b, coarse, coarse_ni, coarse_nj, scale, floor_value,
-                        reinterpret_cast<double*>(base + COARSE_FLAG_BYTES), COARSE_SCRATCH_BYTES - COARSE_FLAG_BYTES,
-                        reinterpret_cast<int*>(base));
+  return coarse_into(b, coarse, coarse_ni, coarse_nj, scale, floor_value,
+                     reinterpret_cast<double*>(base + COARSE_FLAG_BYTES), COARSE_SCRATCH_BYTES - COARSE_FLAG_BYTES,
+                     reinterpret_cast<int*>(base));
 }
 
 bs_status bs_batch_read_state(bs_batch* b, int32_t index, float* rho, float* mu, float* mv, float* energy) {
   BATCH_GUARD(b);
   if (index < 0 || index >= b->count) return fail(BS_ERR_RANGE, "no such case");
-  const BatchCase& k = b->cases[index];
-  float* dst[4] = {rho, mu, mv, energy};
-  for (int q = 0; q < 4; q++)
-    if (dst[q])
-      HIP_TRY(hipMemcpy(dst[q], b->state[b->cur] + 4 * k.cell + (size_t)q * k.n, k.n * sizeof(float),
-                        hipMemcpyDeviceToHost),
-              "state readback");
-  return BS_OK;
+  float* const dst[4] = {rho, mu, mv, energy};
+  return b->fp64 ? batch_read_state<double, float>(b, index, dst) : batch_read_state<float, float>(b, index, dst);
+}
+
+bs_status bs_batch_read_state_f64(bs_batch* b, int32_t index, double* rho, double* mu, double* mv, double* energy) {
+  BATCH_GUARD(b);
+  if (index < 0 || index >= b->count) return fail(BS_ERR_RANGE, "no such case");
+  double* const dst[4] = {rho, mu, mv, energy};
+  return b->fp64 ? batch_read_state<double, double>(b, index, dst) : batch_read_state<float, double>(b, index, dst);
 }
 
 bs_status bs_batch_set_active(bs_batch* b, const uint8_t* active, int32_t count) {
@@ -881,7 +1020,8 @@ bs_status bs_batch_iterate(bs_batch* b, int32_t steps) {
   if (steps < 0) return fail(BS_ERR_RANGE, "the step count cannot be negative");
   bs_status st;
   if (b->lists_dirty && (st = batch_build_lists(b))) return st;
-  return batch_run(b, b->explicit_graph, steps, [b]() { return batch_encode_explicit(b); });
+  if (b->fp64) return batch_run(b, b->explicit_graph, steps, [b]() { return batch_encode_explicit<double>(b); });
+  return batch_run(b, b->explicit_graph, steps, [b]() { return batch_encode_explicit<float>(b); });
 }
 
 bs_status bs_batch_set_implicit_cfl(bs_batch* b, double cfl) {
@@ -898,6 +1038,8 @@ bs_status bs_batch_set_relaxation(bs_batch* b, int32_t form) {
   BATCH_GUARD(b);
   if (form != BS_RELAX_FLUX_DIFFERENCE && form != BS_RELAX_LINEARIZED)
     return fail(BS_ERR_RANGE, "unknown relaxation form %d", form);
+  if (b->fp64 && form != BS_RELAX_LINEARIZED)
+    return fail(BS_ERR_UNSUPPORTED, "an FP64 batch relaxes with the linearized neighbour term only");
   const bool linear = form == BS_RELAX_LINEARIZED;
   if (linear != b->relax_linear) drop_graph(b->implicit_graph);
   b->relax_linear = linear;
@@ -930,18 +1072,21 @@ bs_status bs_batch_iterate_implicit(bs_batch* b, int32_t steps, double cfl_max, 
         (st = dev_alloc(&b->dq, 4 * b->cells, "implicit dq")))
       return st;
     b->implicit_ready = true;
-    if ((st = batch_upload_tile_args(b))) return st;
+    if ((st = b->fp64 ? batch_upload_tile_args<double>(b) : batch_upload_tile_args<float>(b))) return st;
     b->upd_cfl_max = -1;
   }
   StepGraph& g = b->implicit_graph;
   if (b->upd_cfl_max != cfl_max || b->upd_growth != growth || g.omega != omega || g.sweeps != sweeps) {
-    if ((st = batch_upload_implicit_args(b, omega, cfl_max, growth))) return st;
+    if ((st = b->fp64 ? batch_upload_implicit_args<double>(b, omega, cfl_max, growth)
+                      : batch_upload_implicit_args<float>(b, omega, cfl_max, growth)))
+      return st;
     b->upd_cfl_max = cfl_max;
     b->upd_growth = growth;
     drop_graph(g);
   }
   if (b->lists_dirty && (st = batch_build_lists(b))) return st;
-  st = batch_run(b, g, steps, [b, sweeps]() { return batch_encode_implicit(b, sweeps); });
+  if (b->fp64) st = batch_run(b, g, steps, [b, sweeps]() { return batch_encode_implicit<double>(b, sweeps); });
+  else st = batch_run(b, g, steps, [b, sweeps]() { return batch_encode_implicit<float>(b, sweeps); });
   g.sweeps = sweeps;
   g.omega = omega;
   return st;
@@ -952,7 +1097,7 @@ bs_status bs_batch_residual_norms(bs_batch* b, double* norms) {
   if (!norms) return fail(BS_ERR_RANGE, "no output");
   bs_status st;
   if (b->lists_dirty && (st = batch_build_lists(b))) return st;
-  if ((st = batch_launch_tiles(b, TILE_RESIDUAL, ROLE_RESIDUAL + b->cur))) return st;
+  if ((st = launch_tiles(b, TILE_RESIDUAL, ROLE_RESIDUAL + b->cur))) return st;
   if (b->n_seg)
     hipLaunchKernelGGL(segment_sums, dim3(b->n_seg), dim3(256), 0, b->stream, b->partials, b->seg_first, b->seg_count,
                        b->seg_cells, b->seg_cases, 1, b->sums);
@@ -970,28 +1115,8 @@ bs_status bs_batch_drags(bs_batch* b, const double* p_inf, const double* q_inf, 
   if (!p_inf || !q_inf || !base_radius || !cd) return fail(BS_ERR_RANGE, "null argument");
   bs_status st;
   if (b->lists_dirty && (st = batch_build_lists(b))) return st;
-  // p_inf enters the wall kernel per case: set it in the records.
-  std::vector<WallArgs> wargs((size_t)b->count);
-  HIP_TRY(hipMemcpy(wargs.data(), b->wall_args + (size_t)b->cur * b->count, wargs.size() * sizeof(WallArgs),
-                    hipMemcpyDeviceToHost),
-          "records");
-  for (int c = 0; c < b->count; c++) wargs[c].p_inf = p_inf[c];
-  HIP_TRY(hipMemcpy(b->wall_args + (size_t)b->cur * b->count, wargs.data(), wargs.size() * sizeof(WallArgs),
-                    hipMemcpyHostToDevice),
-          "records");
-  if (b->n_wall)
-    hipLaunchKernelGGL(batch_wall_fn(b->scheme), dim3(b->n_wall), dim3(256), 0, b->stream,
-                       b->wall_args + (size_t)b->cur * b->count, b->wall_bnd, b->work_wall);
-  if (b->n_seg)
-    hipLaunchKernelGGL(segment_sums, dim3(b->n_seg), dim3(256), 0, b->stream, b->wall_partials, b->wall_first,
-                       b->wall_count, b->seg_cells, b->seg_cases, 0, b->sums);
-  std::vector<double> force(b->count);
-  HIP_TRY(hipMemcpyAsync(force.data(), b->sums, force.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream),
-          "drag readback");
-  HIP_TRY(hipStreamSynchronize(b->stream), "sync");
-  for (int c = 0; c < b->count; c++)
-    cd[c] = b->active[c] ? (2.0 * force[c]) / (q_inf[c] * base_radius[c] * base_radius[c]) : NAN;
-  return BS_OK;
+  return b->fp64 ? batch_drags<double>(b, p_inf, q_inf, base_radius, cd)
+                 : batch_drags<float>(b, p_inf, q_inf, base_radius, cd);
 }
 
 bs_status bs_batch_status(bs_batch* b, uint32_t* status, int64_t* steps) {

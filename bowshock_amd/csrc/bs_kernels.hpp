@@ -547,13 +547,13 @@ struct WorkItem {
   int slot;  // FP64 partial slot
 };
 
-template <int TX, int TY, int REC, int LIM, int FLUX, int MODE>
-__global__ __launch_bounds__(TX * TY) void batch_tile_kernel(const TileArgs* __restrict__ args,
+template <typename T, int TX, int TY, int REC, int LIM, int FLUX, int MODE>
+__global__ __launch_bounds__(TX * TY) void batch_tile_kernel(const TileArgsT<T>* __restrict__ args,
                                                              const WorkItem* __restrict__ work) {
   // Work items are ordered by case and tile row; each XCD takes one
   // contiguous run of them (xcd_tile).
   const WorkItem w = work[xcd_tile(blockIdx.x, gridDim.x)];
-  tile_body<float, TX, TY, REC, LIM, FLUX, MODE>(args[w.args], w.bx, w.by, w.slot);
+  tile_body<T, TX, TY, REC, LIM, FLUX, MODE>(args[w.args], w.bx, w.by, w.slot);
 }
 
 // Deterministic FP64 sum of n partials by one block; writes sqrt(sum / cells)
@@ -885,13 +885,13 @@ __global__ __launch_bounds__(RTX * RTY) void relax_tile_kernel(RelaxArgsT<T> A, 
   relax_tile_body<T, NH, LIN>(A, t % tiles_x, t / tiles_x, colour0, init_red, U, fuse_update != 0);
 }
 
-template <int NH, bool LIN>
-__global__ __launch_bounds__(RTX * RTY) void batch_relax_tile_kernel(const RelaxArgs* __restrict__ args,
+template <typename T, int NH, bool LIN>
+__global__ __launch_bounds__(RTX * RTY) void batch_relax_tile_kernel(const RelaxArgsT<T>* __restrict__ args,
                                                                      const WorkItem* __restrict__ work, int colour0,
-                                                                     int init_red, const UpdateArgs* upd) {
+                                                                     int init_red, const UpdateArgsT<T>* upd) {
   const WorkItem w = work[xcd_tile(blockIdx.x, gridDim.x)];
-  if (upd) relax_tile_body<float, NH, LIN>(args[w.args], w.bx, w.by, colour0, init_red, upd[w.args], true);
-  else relax_tile_body<float, NH, LIN>(args[w.args], w.bx, w.by, colour0, init_red, UpdateArgs{}, false);
+  if (upd) relax_tile_body<T, NH, LIN>(args[w.args], w.bx, w.by, colour0, init_red, upd[w.args], true);
+  else relax_tile_body<T, NH, LIN>(args[w.args], w.bx, w.by, colour0, init_red, UpdateArgsT<T>{}, false);
 }
 
 

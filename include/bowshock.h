@@ -336,6 +336,14 @@ bs_status bs_kernel_timing(bs_solver *s, int32_t kernel, double *ms_per_launch, 
 typedef struct bs_batch bs_batch;
 
 bs_status bs_batch_create(bs_ctx *ctx, const bs_config *cases, int32_t count, bs_batch **out);
+/* The batch of FP64 solvers (bs_solver_create_fp64): state, metrics and
+ * time steps in FP64, the implicit steps' residual, diagonal and update in
+ * FP64 around an FP32 relaxation with the linearized neighbour term, for
+ * convergence below the FP32 residual floor (the reference's two-float
+ * engine's role, euler2d-two-float.ts:1-14, 379-448). */
+bs_status bs_batch_create_fp64(bs_ctx *ctx, const bs_config *cases, int32_t count, bs_batch **out);
+/* BS_PRECISION_FP32 or BS_PRECISION_FP64. */
+bs_status bs_batch_precision(bs_batch *b, int32_t *precision);
 bs_status bs_batch_destroy(bs_batch *b);
 /* Uniform start per case (setInitialCondition, euler2d-gpu.ts:412-437):
  * 4 primitives (rho, u, v, p) per case. */
@@ -368,6 +376,8 @@ bs_status bs_batch_set_coarse_noalloc(bs_batch *b, const double *coarse, int32_t
                                       const double *scale, double floor_value);
 /* readState (euler2d-gpu.ts:549-573) of one case; null outputs skipped. */
 bs_status bs_batch_read_state(bs_batch *b, int32_t index, float *rho, float *mu, float *mv, float *energy);
+/* The same with FP64 arrays: exact on an FP64 batch, widened from FP32. */
+bs_status bs_batch_read_state_f64(bs_batch *b, int32_t index, double *rho, double *mu, double *mv, double *energy);
 /* Cases with active[c] == 0 are skipped by later steps and reductions;
  * `count` must be the batch's case count. */
 bs_status bs_batch_set_active(bs_batch *b, const uint8_t *active, int32_t count);
@@ -375,7 +385,8 @@ bs_status bs_batch_set_active(bs_batch *b, const uint8_t *active, int32_t count)
  * on 64 x 8 tiles (set by the autotuner; 0 = all on 64 x 8). */
 bs_status bs_batch_set_tiling(bs_batch *b, int32_t small_tile_below_cells);
 /* bs_set_relaxation for every case of the batch (default
- * BS_RELAX_FLUX_DIFFERENCE). */
+ * BS_RELAX_FLUX_DIFFERENCE on an FP32 batch; an FP64 batch takes
+ * BS_RELAX_LINEARIZED only). */
 bs_status bs_batch_set_relaxation(bs_batch *b, int32_t form);
 /* `steps` explicit steps of every active case (iterate). */
 bs_status bs_batch_iterate(bs_batch *b, int32_t steps);

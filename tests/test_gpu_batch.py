@@ -218,3 +218,67 @@ def test_coarse_fields_set_on_the_gpu_equal_the_host_resampling(ctx):
         gpu.setCoarse(bad, scale, 0.05, shape=(W.COARSE_NJ, W.COARSE_NI))
     with pytest.raises(RangeError):
         gpu.setCoarse(targets[:, :-1], scale)
+
+
+# ---- FP64 batches (bs_batch_create_fp64) -----------------------------------
+
+
+def _singles64(ctx, cs):
+    solvers = []
+    for cfg, free, _, _ in cs:
+        s = GpuEuler2D.create(ctx, cfg, "fp64")
+        s.setCellPrimitives(np.array(free))
+        solvers.append(s)
+    return solvers
+
+
+@pytest.mark.parametrize("small_below", [0, 5000])
+def test_fp64_batch_matches_fp64_single_solves(ctx, small_below):
+    """Explicit and implicit steps of an FP64 batch: every case bitwise
+    equal to the FP64 solver's single solve (FP64 tiles, linearized FP32
+    relaxation, FP64 update), H and C grids, either tile class; FP64 state
+    read back as FP64."""
+    cs = _mixed_cases(limiter="vanAlbada", reconstruction="sharedScaled")
+    batch = GpuBatch2D(ctx, [c[0] for c in cs], precision="fp64")
+    batch.setTiling(small_below)
+    batch.setUniform([c[1] for c in cs])
+    singles = _singles64(ctx, cs)
+    batch.iterate(14)
+    for s in singles:
+        s.iterate(14)
+    batch.setImplicitCfl(1.0)
+    for s in singles:
+        s.setImplicitCfl(1.0)
+    for count in (25, 13):
+        batch.iterateImplicit(count, 10.0, 2)
+        for s in singles:
+            s.iterateImplicit(count, cfl=10.0, sweeps=2)
+        for k, s in enumerate(singles):
+            st = batch.readState(k)
+            assert st.rho.dtype == np.float64
+            assert np.array_equal(_state(st), _state(s.readState())), (count, k)
+    p = np.array([c[2] for c in cs])
+    q = np.array([c[3] for c in cs])
+    drags = batch.forebodyDrags(p, q, 0.5)
+    norms = batch.densityResidualNorms()
+    for k, s in enumerate(singles):
+        assert drags[k] == pytest.approx(s.forebodyDrag(p[k], q[k], 0.5), rel=1e-13, abs=1e-15)
+        assert norms[k] == pytest.approx(s.densityResidualNorm(), rel=1e-12, abs=0)
+    with pytest.raises(Exception):
+        batch.setRelaxation("fluxDifference")
+
+
+def test_fp64_batch_converges_below_the_fp32_floor(ctx):
+    """Population-style implicit solves (CFL 1 -> 10, 2 red-black sweeps)
+    of an FP64 batch reach a density residual of 1e-6 (absolute), which the
+    FP32 batch's state cannot (its floor), with drags settled to 1e-5."""
+    cs = _cases(limiter="vanAlbada", reconstruction="sharedScaled")
+    p = np.array([c[2] for c in cs])
+    q = np.array([c[3] for c in cs])
+    plan = SolvePlan(implicit=True, chunk=60, max_steps=6000, cfl=10.0, cfl_start=1.0, sweeps=2, residual=1e-6,
+                     drop=0.0, stall=10 ** 9)
+    batch = GpuBatch2D(ctx, [c[0] for c in cs], precision="fp64")
+    batch.setUniform([c[1] for c in cs])
+    res = solve(batch, p, q, 0.5, plan)
+    for k, r in enumerate(res):
+        assert r.converged and r.residual <= 1e-6, (k, r.reason, r.residual, r.steps)
