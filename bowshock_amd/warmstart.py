@@ -138,8 +138,9 @@ def resample(field: np.ndarray, ni: int, nj: int) -> np.ndarray:
 
 def primitives_from_state(state, ni: int, nj: int, gamma: float = 1.4) -> np.ndarray:
     """(4, nj, ni) primitives from a GpuState2D (pressure with the FP32
-    gamma, gpuPrimitive2D, euler2d-gpu.ts:136-146)."""
-    g = float(np.float32(gamma))
+    gamma for an FP32 state, gpuPrimitive2D, euler2d-gpu.ts:136-146; the
+    gamma itself for an FP64 state)."""
+    g = float(gamma) if state.rho.dtype == np.float64 else float(np.float32(gamma))
     rho = state.rho.astype(np.float64)
     u = state.mu / rho
     v = state.mv / rho
