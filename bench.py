@@ -357,11 +357,20 @@ class _CaseGroups:
 
         results = [None] * self.n
         t0 = time.perf_counter()
+        last = [t0]
+
+        def progress(steps, active):
+            # A line every 20 s of a long solve.
+            now = time.perf_counter()
+            if now - last[0] > 20:
+                last[0] = now
+                print(f"  {now - t0:.0f}s: step {steps}, {active} cases running", file=sys.stderr, flush=True)
+
         for ks, b in self.groups:
             offset = time.perf_counter() - t0
             pick = (lambda v: None if v is None else np.broadcast_to(np.asarray(v, dtype=np.float64), (self.n,))[ks])
             res = solve(b, np.asarray(p_inf)[ks], np.asarray(q_inf)[ks], 0.5, plan, reference=pick(reference),
-                        cfl_start=pick(cfl_start))
+                        cfl_start=pick(cfl_start), progress=progress)
             for k, r in zip(ks, res):
                 r.seconds += offset
                 results[k] = r
@@ -580,6 +589,68 @@ def run_warmstart(a):
         wall = time.perf_counter() - t
         return S, batch, results, wall, used
 
+    def run_sequenced(cs, coarse_cs, starts_fn):
+        """The cases with a coarser grid: solved there (to a density
+        residual of 1e-5) from the free stream or the network's prediction,
+        then each fine case started from its coarse solution at the final
+        CFL number and solved to the plan's target; the others as run()."""
+        seq = [k for k, cc in enumerate(coarse_cs) if cc is not None]
+        rest = [k for k, cc in enumerate(coarse_cs) if cc is None]
+        out = [None] * len(cs)
+        t0 = time.perf_counter()
+        Sc = _setup_cases([coarse_cs[k] for k in seq], implicit=True, precision=a.precision)
+        Sc["plan"].residual = 1e-5
+        gc = _CaseGroups(ctx, Sc)
+        gc.setUniform(Sc["frees"])
+        free_c = gc.densityResidualNorms()
+        used = np.zeros(len(seq), dtype=bool)
+        if starts_fn is not None:
+            targets, scale = starts_fn()
+            gc.setCoarse(targets[seq], scale[seq], (W.COARSE_NJ, W.COARSE_NI))
+            warm_c = gc.densityResidualNorms()
+            used = warm_c <= free_c
+            for j in np.nonzero(~used)[0]:
+                gc.setCellPrimitives(j, np.broadcast_to(np.array(Sc["frees"][j]), (Sc["ok"][j].cells, 4)))
+        plan_c = Sc["plan"]
+        cfl0 = np.where(used, plan_c.cfl, plan_c.cfl_start)
+        res_c = gc.solve(Sc["p_inf"], Sc["q_inf"], plan_c, reference=free_c, cfl_start=cfl0)
+        t_coarse = time.perf_counter() - t0
+        Sf = _setup_cases([cs[k] for k in seq], implicit=True, precision=a.precision)
+        gf = _CaseGroups(ctx, Sf)
+        free_f = np.empty(len(seq))
+        gf.setUniform(Sf["frees"])
+        free_f[:] = gf.densityResidualNorms()
+        for (ks, fb), (kc, cb) in zip(gf.groups, gc.groups):
+            assert ks == kc
+            fb.setFromCoarser(cb)
+        gc.destroy()
+        plan_f = Sf["plan"]
+        res_f = gf.solve(Sf["p_inf"], Sf["q_inf"], plan_f, reference=free_f,
+                         cfl_start=np.full(len(seq), plan_f.cfl))
+        gf.destroy()
+        for j, k in enumerate(seq):
+            out[k] = res_f[j]
+        if rest:
+            _, b2, res_r, _, _ = run([cs[k] for k in rest], None if starts_fn is None else
+                                     (lambda: tuple(v[rest] for v in starts_fn())))
+            b2.destroy()
+            for j, k in enumerate(rest):
+                out[k] = res_r[j]
+        wall = time.perf_counter() - t0
+        cells_f = np.array([cs[k].cells for k in seq])
+        cells_c = np.array([coarse_cs[k].cells for k in seq])
+        cu = float((cells_c * np.array([r.steps for r in res_c])).sum() +
+                   (cells_f * np.array([r.steps for r in res_f])).sum())
+        conv = sum(r.converged for r in out)
+        info = {"solves_per_s": conv / wall, "seconds": wall, "coarse_seconds": t_coarse, "converged": conv,
+                "sequenced_cases": len(seq), "coarse_converged": int(sum(r.converged for r in res_c)),
+                "median_coarse_steps": int(np.median([r.steps for r in res_c])) if seq else 0,
+                "median_fine_steps": int(np.median([r.steps for r in res_f])) if seq else 0,
+                "cell_updates": cu, "warm_start_used_on_coarse": int(used.sum())}
+        print(f"sequenced ({'network' if starts_fn else 'free stream'} on the coarser grids): {json.dumps(info)}",
+              file=sys.stderr, flush=True)
+        return out, info
+
     def fields(batch, cs, results):
         import math as _m
 
@@ -633,6 +704,20 @@ def run_warmstart(a):
     batch.destroy()
     steps_blunt = np.array([r.steps for r in res_blunt])
     conv_blunt = sum(r.converged for r in res_blunt)
+
+    # 5. Held-out rockets solved on the grids half as fine first, from the
+    # network's prediction or the free stream, then started from those
+    # solutions (solveTransonic's coarse-to-fine start, transonic-flow.ts:
+    # 157-168) at the final CFL number.
+    t = time.perf_counter()
+    coarse_cs = P.coarser_cases(held)
+    t_coarse_grids = time.perf_counter() - t
+    seq_plain = run_sequenced(held, coarse_cs, None)
+    seq_warm = run_sequenced(held, coarse_cs, starts)
+    def cd_change(results):
+        both = [k for k in range(len(held)) if res_plain[k].converged and results[k].converged]
+        return max((abs(results[k].cd - res_plain[k].cd) / abs(res_plain[k].cd) for k in both), default=float("nan"))
+
     pred_h = ws.predict_targets(Fh[rows_h])
     field_err = float(np.linalg.norm(pred_h - T_h) / np.linalg.norm(T_h))
 
@@ -699,6 +784,11 @@ def run_warmstart(a):
                    "speedup": sb / sp if sp > 0 else None, "warm_start_used": int(used_blunt.sum()),
                    "median_steps_warm": int(np.median(steps_blunt)), "outcomes_warm": _outcomes(res_blunt),
                    "max_rel_cd_change_warm_vs_plain": dcd_b,
+                   "coarser_grid_seconds": t_coarse_grids,
+                   "sequenced_free_stream": dict(seq_plain[1], speedup=seq_plain[1]["solves_per_s"] / sp if sp else None,
+                                                 max_rel_cd_change_vs_plain=cd_change(seq_plain[0])),
+                   "sequenced_warm": dict(seq_warm[1], speedup=seq_warm[1]["solves_per_s"] / sp if sp else None,
+                                          max_rel_cd_change_vs_plain=cd_change(seq_warm[0])),
                    "parallelism": "single GPU, one batch per flux (H grids HLLC, C grids Rusanov)"},
         "roofline": {"bound": "mfma", "achieved": tflops, "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": tflops / 2500.0, "traffic": None,

@@ -92,6 +92,7 @@ SIGNATURES = {
     "bs_batch_set_coarse_noalloc": (C.c_int, [_vp, _dp, C.c_int32, C.c_int32, _dp, C.c_double]),
     "bs_batch_read_state": (C.c_int, [_vp, C.c_int32, _fp, _fp, _fp, _fp]),
     "bs_batch_read_state_f64": (C.c_int, [_vp, C.c_int32, _dp, _dp, _dp, _dp]),
+    "bs_batch_set_from_coarser": (C.c_int, [_vp, _vp]),
     "bs_batch_set_active": (C.c_int, [_vp, C.POINTER(C.c_uint8), C.c_int32]),
     "bs_batch_set_tiling": (C.c_int, [_vp, C.c_int32]),
     "bs_batch_set_relaxation": (C.c_int, [_vp, C.c_int32]),

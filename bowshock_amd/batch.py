@@ -92,6 +92,14 @@ class GpuBatch2D:
             raise RangeError(f"coarse fields must be ({self.count}, {4 * cnj * cni})")
         check(N.lib().bs_batch_set_coarse(self._h, N.dptr(coarse), cni, cnj, N.dptr(scale), float(floor)))
 
+    def setFromCoarser(self, coarse: "GpuBatch2D"):
+        """solveTransonic's start from the coarser grid (transonic-flow.ts:
+        157-168), every case at once on the GPU: case k takes case k of
+        `coarse`, each cell the state of the coarse cell it lies in."""
+        if coarse.count != self.count:
+            raise RangeError("the batches must hold the same cases")
+        check(N.lib().bs_batch_set_from_coarser(self._h, coarse._h))
+
     def readState(self, index: int) -> GpuState2D:
         """Case `index`'s conserved state: FP32 arrays, FP64 on an FP64 batch."""
         ni, nj = self.shapes[index]
@@ -233,13 +241,14 @@ class _Tracker:
 
 
 def solve(batch: GpuBatch2D, p_inf, q_inf, base_radius, plan: Optional[SolvePlan] = None, reference=None,
-          cfl_start=None) -> List[CaseResult]:
+          cfl_start=None, progress=None) -> List[CaseResult]:
     """Run every case of the batch until it converges or fails; converged
     and failed cases leave the launches at the next check. ``reference``
     (per case) replaces the first reading as the residual the ``drop`` is
     measured from (the free stream's, so that differently started runs
     share one target); ``cfl_start`` (per case) replaces the plan's initial
-    implicit CFL number."""
+    implicit CFL number. ``progress(steps, active)`` is called after every
+    check."""
     plan = plan or SolvePlan()
     n = batch.count
     res = [CaseResult() for _ in range(n)]
@@ -277,6 +286,8 @@ def solve(batch: GpuBatch2D, p_inf, q_inf, base_radius, plan: Optional[SolvePlan
                 changed = True
         if changed and active.any():
             batch.setActive(active)
+        if progress is not None:
+            progress(done, int(active.sum()))
     return res
 
 

@@ -603,6 +603,28 @@ __global__ __launch_bounds__(256) void batch_coarse_kernel(CoarseCase<T> K, int 
   K.state[3 * (size_t)K.n + c] = (T)(w[3] / (g - 1) + 0.5 * w[0] * (w[1] * w[1] + w[2] * w[2]));
 }
 
+// The coarse-to-fine start (transonic-flow.ts:157-168): fine cell (i, j)
+// takes the state of the coarse cell whose index range holds its centre in
+// (i / ni, j / nj), i.e. (i / r, j / r) on grids that nest by r.
+template <typename TF, typename TC>
+struct InjectCase {
+  TF* fine;
+  const TC* coarse;
+  int ni, nj, n, cni, cnj, cn;
+};
+
+template <typename TF, typename TC>
+__global__ __launch_bounds__(256) void inject_kernel(InjectCase<TF, TC> K) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= K.n) return;
+  const int i = c % K.ni, j = c / K.ni;
+  const int ci = (int)(((long long)(2 * i + 1) * K.cni) / (2LL * K.ni));
+  const int cj = (int)(((long long)(2 * j + 1) * K.cnj) / (2LL * K.nj));
+  const int cc = ci + cj * K.cni;
+#pragma unroll
+  for (int q = 0; q < 4; q++) K.fine[(size_t)q * K.n + c] = (TF)K.coarse[(size_t)q * K.cn + cc];
+}
+
 // ---- the allocation-free warm start (bs_batch_set_coarse_noalloc) --------
 
 // Its staging: a file-scope device array, one copy per device the library
@@ -876,6 +898,24 @@ bs_status batch_create(bs_ctx* ctx, const bs_config* configs, int32_t count, boo
   return BS_OK;
 }
 
+template <typename TF, typename TC>
+bs_status batch_inject(bs_batch* b, bs_batch* from) {
+  HIP_TRY(hipStreamSynchronize(from->stream), "coarse batch");
+  for (int c = 0; c < b->count; c++) {
+    const BatchCase& k = b->cases[c];
+    const BatchCase& kc = from->cases[c];
+    InjectCase<TF, TC> K;
+    K.fine = arena<TF>(b->state[b->cur], 4 * k.cell);
+    K.coarse = arena<TC>(from->state[from->cur], 4 * kc.cell);
+    K.ni = k.ni; K.nj = k.nj; K.n = k.n;
+    K.cni = kc.ni; K.cnj = kc.nj; K.cn = kc.n;
+    hipLaunchKernelGGL((inject_kernel<TF, TC>), dim3((k.n + 255) / 256), dim3(256), 0, b->stream, K);
+  }
+  HIP_TRY(hipGetLastError(), "injection launch");
+  HIP_TRY(hipStreamSynchronize(b->stream), "injection");
+  return batch_reset_control(b, 0, b->count);
+}
+
 template <typename T>
 bs_status batch_drags(bs_batch* b, const double* p_inf, const double* q_inf, const double* base_radius, double* cd) {
   // p_inf enters the wall kernel per case: set it in the records.
@@ -978,6 +1018,21 @@ bs_status bs_batch_set_coarse_noalloc(bs_batch* b, const double* coarse, int32_t
   return coarse_into(b, coarse, coarse_ni, coarse_nj, scale, floor_value,
                      reinterpret_cast<double*>(base + COARSE_FLAG_BYTES), COARSE_SCRATCH_BYTES - COARSE_FLAG_BYTES,
                      reinterpret_cast<int*>(base));
+}
+
+bs_status bs_batch_set_from_coarser(bs_batch* b, bs_batch* coarse) {
+  BATCH_GUARD(b);
+  if (!coarse || coarse == b) return fail(BS_ERR_RANGE, "the start needs another batch");
+  BatchGuard cguard(coarse);
+  if (!cguard.ok) return fail(BS_ERR_BUSY, "bs_batch: await the coarse batch's previous operation first");
+  if (coarse->count != b->count) return fail(BS_ERR_RANGE, "the batches hold %d and %d cases", b->count, coarse->count);
+  if (coarse->device != b->device) return fail(BS_ERR_RANGE, "the batches are on different devices");
+  for (int c = 0; c < b->count; c++)
+    if (coarse->cases[c].ni > b->cases[c].ni || coarse->cases[c].nj > b->cases[c].nj)
+      return fail(BS_ERR_RANGE, "case %d: a start must come from a coarser grid (%d x %d over %d x %d)", c,
+                  coarse->cases[c].ni, coarse->cases[c].nj, b->cases[c].ni, b->cases[c].nj);
+  if (b->fp64) return coarse->fp64 ? batch_inject<double, double>(b, coarse) : batch_inject<double, float>(b, coarse);
+  return coarse->fp64 ? batch_inject<float, double>(b, coarse) : batch_inject<float, float>(b, coarse);
 }
 
 bs_status bs_batch_read_state(bs_batch* b, int32_t index, float* rho, float* mu, float* mv, float* energy) {

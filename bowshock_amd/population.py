@@ -117,6 +117,7 @@ class Case:
     grid: Optional[grids.StructuredGrid] = None
     i_tip: int = 0
     topology: str = "H"   # "C": the blunt-body grid, iMin the axis ahead of the tip
+    refinement: int = 1   # a C grid's refinement of its family's base grid
 
     @property
     def body(self) -> grids.Body:
@@ -170,21 +171,23 @@ def boundaries(case: Case, free) -> dict:
 
 
 def _grid_job(args):
-    body, ni, nj, mach, tol, blunt = args
+    body, ni, nj, mach, tol, blunt = args[:6]
+    refinement = args[6] if len(args) > 6 else 0
     if blunt:
         from .blunt_grid import blunt_body_grid
 
+        r = refinement or c_grid_refinement(ni)
         try:
-            bg = blunt_body_grid(body, C_GRID_BASE_NJ, mach, refinement=c_grid_refinement(ni))
+            bg = blunt_body_grid(body, C_GRID_BASE_NJ, mach, refinement=r)
         except ValueError as e:   # GridQualityError included
-            return "C grid: " + str(e), None, 0, "C"
-        return None, (bg.grid.x, bg.grid.y, bg.grid.ni, bg.grid.nj), 0, "C"
+            return "C grid: " + str(e), None, 0, "C", r
+        return None, (bg.grid.x, bg.grid.y, bg.grid.ni, bg.grid.nj), 0, "C", r
     try:
         shock = grids.tip_shock_angle(body, mach, GAMMA, tol)
     except ValueError as e:
-        return str(e), None, 0, "H"
+        return str(e), None, 0, "H", 1
     bg = grids.body_grid_array(body, ni, nj, shock)
-    return None, (bg.grid.x, bg.grid.y, ni, nj), bg.i_tip, "H"
+    return None, (bg.grid.x, bg.grid.y, ni, nj), bg.i_tip, "H", 1
 
 
 def build_grids(cases: List[Case], workers: int = 0, tolerance: float = 1e-10) -> List[Case]:
@@ -204,14 +207,50 @@ def build_grids(cases: List[Case], workers: int = 0, tolerance: float = 1e-10) -
             results[k] = r
     else:
         results = [_grid_job(j) for j in jobs]
-    for c, (refusal, xy, i_tip, topology) in zip(cases, results):
+    for c, (refusal, xy, i_tip, topology, refinement) in zip(cases, results):
         c.refused = refusal
         c.topology = topology
+        c.refinement = refinement
         if xy is not None:
             c.ni, c.nj = int(xy[2]), int(xy[3])
             c.grid = grids.StructuredGrid(c.ni, c.nj, xy[0], xy[1])
             c.i_tip = i_tip
     return cases
+
+
+def coarser_cases(cases: Sequence[Case], workers: int = 0, tolerance: float = 1e-10) -> List[Optional[Case]]:
+    """Each accepted case's grid half as fine, for the start of a solve
+    from a coarser grid (solveTransonic, transonic-flow.ts:157-168): a C
+    grid's family member of half the refinement (its grids nest), an H
+    grid with half the columns and rows over the same body and outer
+    boundary. None where the case has no grid, its C grid is its family's
+    base grid, or the coarser grid is refused."""
+    out: List[Optional[Case]] = []
+    jobs, where = [], []
+    for k, c in enumerate(cases):
+        if c.grid is None or (c.topology == "C" and c.refinement < 2):
+            out.append(None)
+            continue
+        blunt = c.topology == "C"
+        ni, nj = (c.ni, c.nj) if blunt else (max(2, c.ni // 2), max(2, (c.nj + 1) // 2))
+        out.append(Case(c.rocket, c.mach, ni, nj, c.cylinder))
+        jobs.append((c.body, ni, nj, c.mach, tolerance, blunt, c.refinement // 2 if blunt else 0))
+        where.append(k)
+    workers = workers or min(16, os.cpu_count() or 1)
+    if workers > 1 and len(jobs) > 8:
+        with ProcessPoolExecutor(workers) as ex:
+            results = list(ex.map(_grid_job, jobs, chunksize=2))
+    else:
+        results = [_grid_job(j) for j in jobs]
+    for k, (refusal, xy, i_tip, topology, refinement) in zip(where, results):
+        c = out[k]
+        if xy is None:
+            out[k] = None
+            continue
+        c.topology, c.refinement, c.i_tip = topology, refinement, i_tip
+        c.ni, c.nj = int(xy[2]), int(xy[3])
+        c.grid = grids.StructuredGrid(c.ni, c.nj, xy[0], xy[1])
+    return out
 
 
 def refusal_counts(cases: Sequence[Case]) -> dict:

@@ -374,6 +374,15 @@ bs_status bs_batch_set_coarse(bs_batch *b, const double *coarse, int32_t coarse_
 #define BS_COARSE_SCRATCH_BYTES (8u << 20)
 bs_status bs_batch_set_coarse_noalloc(bs_batch *b, const double *coarse, int32_t coarse_ni, int32_t coarse_nj,
                                       const double *scale, double floor_value);
+/* The coarse-to-fine start of solveTransonic (transonic-flow.ts:157-168)
+ * for every case at once: case c of `b` starts from the current state of
+ * case c of `coarse` (same case count and device, grids no finer than b's),
+ * each cell taking the state of the coarse cell whose index range holds its
+ * centre in (i / ni, j / nj): the cell it lies in, (i / r, j / r), on grids
+ * of a family that nest by r. Copies device to device, converting between
+ * FP32 and FP64 states; a new start (step counts and status reset, the
+ * implicit CFL numbers kept). */
+bs_status bs_batch_set_from_coarser(bs_batch *b, bs_batch *coarse);
 /* readState (euler2d-gpu.ts:549-573) of one case; null outputs skipped. */
 bs_status bs_batch_read_state(bs_batch *b, int32_t index, float *rho, float *mu, float *mv, float *energy);
 /* The same with FP64 arrays: exact on an FP64 batch, widened from FP32. */
