@@ -815,6 +815,8 @@ bs_status batch_create(bs_ctx* ctx, const bs_config* configs, int32_t count, boo
       return fail(BS_ERR_RANGE, "case %d: periodic boundaries come in pairs", c);
     if (!(k.cfl > 0 && k.cfl <= 1)) return fail(BS_ERR_RANGE, "cfl must be in (0, 1]");
     if (k.ni < 2 || k.nj < 2 || !k.x || !k.y) return fail(BS_ERR_RANGE, "case %d: the grid needs 2 x 2 cells", c);
+    if ((size_t)(k.ni + 1) * (size_t)(k.nj + 1) > ((size_t)1 << 28))
+      return fail(BS_ERR_RANGE, "case %d: the grid exceeds 2^28 nodes", c);
     if ((k.reconstruction == BS_RECON_SHARED || k.reconstruction == BS_RECON_SHARED_SCALED) &&
         k.limiter != BS_LIMITER_VAN_ALBADA)
       return fail(BS_ERR_RANGE, "shared reconstruction requires the 'vanAlbada' limiter");

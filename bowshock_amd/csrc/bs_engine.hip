@@ -508,6 +508,9 @@ static bs_status solver_create(bs_ctx* ctx, const bs_config* c, bool fp64, bs_so
   // config.ts:35-55 validateGrid2DConfig and the GPU engine's own checks.
   if (!(c->cfl > 0 && c->cfl <= 1)) return fail(BS_ERR_RANGE, "cfl must be in (0, 1]");
   if (c->ni < 2 || c->nj < 2) return fail(BS_ERR_RANGE, "the grid needs at least 2 x 2 cells");
+  // The stencil kernels address an array as base + 32-bit byte offset.
+  if ((size_t)(c->ni + 1) * (size_t)(c->nj + 1) > ((size_t)1 << 28))
+    return fail(BS_ERR_RANGE, "the grid exceeds 2^28 nodes");
   if ((c->reconstruction == BS_RECON_SHARED || c->reconstruction == BS_RECON_SHARED_SCALED) &&
       c->limiter != BS_LIMITER_VAN_ALBADA)
     return fail(BS_ERR_RANGE, "shared reconstruction requires the 'vanAlbada' limiter");

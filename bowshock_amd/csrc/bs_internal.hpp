@@ -34,10 +34,13 @@ bs_status hip_fail(hipError_t e, const char* what);
 constexpr int BATCH = 256;       // steps between status checks (time-loop.ts:13)
 constexpr int GRAPH_STEPS = 12;  // steps per captured graph (a multiple of 3)
 
-// Tile of the stencil kernels: TX cells along i (one wavefront row when 64)
-// by TY along j; set at build time.
+// Tile of the FP32 stencil kernels: TX cells along i by TY along j (a
+// wavefront covers 64 / TX rows of TX cells); set at build time. 32 x 8
+// measured fastest for the stage kernel at 1280 x 640 (15.7 us; 64 x 8
+// 16.1, 32 x 16 16.0, 32 x 12 16.4, 48 x 8 17.4, 32 x 4 18.8): its 3,200
+// tiles, eight to a CU, leave a finer last round than 1,600 of 64 x 8.
 #ifndef BS_TILE_X
-#define BS_TILE_X 64
+#define BS_TILE_X 32
 #endif
 #ifndef BS_TILE_Y
 #define BS_TILE_Y 8

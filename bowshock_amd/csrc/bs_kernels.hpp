@@ -215,9 +215,128 @@ constexpr EdgePlan edge_plan(int TX, int TY) {
   return {jw * 64, iw * 64 + ioff};
 }
 
-// The tile (bx, by) of one grid; its FP64 partial goes to A.partials[slot].
-template <typename T, int TX, int TY, int REC, int LIM, int FLUX, int MODE>
-__device__ __forceinline__ void tile_body(const TileArgsT<T>& A, int bx, int by, int slot) {
+// Element idx of a device array as the array's base plus a 32-bit byte
+// offset: with the base in SGPRs the access takes the scalar-base form,
+// one shared offset register for all arrays of the same index instead of a
+// 64-bit address formed per array. Grids are held to 2^28 nodes, so every
+// offset fits.
+template <typename T>
+__device__ __forceinline__ T& at(T* base, int idx) {
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (unsigned)idx * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ const T& at(const T* base, int idx) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (unsigned)idx * (unsigned)sizeof(T));
+}
+
+// Per-thread inputs of a tile: the metrics of the thread's own i- and
+// j-face (and, when asked, of its high-edge face), its cell's volume,
+// planar area, stored or global time step and u0, and, on tiles whose whole
+// window lies inside the grid, the conserved state of its cell and of its
+// halo cell. All are loaded up front, ahead of the staging, so their
+// latencies overlap and no later phase waits on a global load.
+template <typename T>
+struct TileInputs {
+  T mi[3], mj[3], me[3];
+  T vol, planar, dt, u0[4];
+  Vec4<T> own, halo;
+};
+
+// Tiles whose window [i0 - 2, i0 + TX + 2) x [j0 - 2, j0 + TY + 2) lies
+// inside the grid (all but a ring of them) read no ghost cell and have no
+// boundary face.
+template <int TX, int TY>
+__device__ __forceinline__ bool tile_is_inner(int ni, int nj, int i0, int j0) {
+  return i0 >= 2 && j0 >= 2 && i0 + TX + 2 <= ni && j0 + TY + 2 <= nj;
+}
+
+// Window position (wi, wj) of halo cell h < 4 (TX + TY): two rows below and
+// above the tile, then two columns left and right (corners excluded).
+template <int TX, int TY>
+__device__ __forceinline__ void halo_cell(int h, int& wi, int& wj) {
+  if (h < 4 * TX) {
+    const int r = h / TX;
+    wi = 2 + h % TX;
+    wj = r < 2 ? r : TY + r;
+  } else {
+    const int r = (h - 4 * TX) / TY;
+    wj = 2 + (h - 4 * TX) % TY;
+    wi = r < 2 ? r : TX + r;
+  }
+}
+
+// Loads a tile's TileInputs; EDGE: also the high-edge face metrics
+// (otherwise the edge faces load them when they are computed).
+template <typename T, int TX, int TY, int MODE, bool EDGE>
+__device__ __forceinline__ void tile_inputs(const TileArgsT<T>& A, int bx, int by, TileInputs<T>& in) {
+  constexpr int NT = TX * TY;
+  const GridDevT<T>& G = A.G;
+  const int ni = G.ni, nj = G.nj;
+  const int i0 = bx * TX, j0 = by * TY;
+  const int tid = threadIdx.x;
+  const int li = tid % TX, lj = tid / TX;
+  const int gi = i0 + li, gj = j0 + lj;
+  const bool valid = gi < ni && gj < nj;
+  const bool has_i = gi <= ni && gj < nj, has_j = gi < ni && gj <= nj;
+  // A value is loaded only where its consumer reads it (the face exists,
+  // the cell is in the grid, the tile is inner); the others stay unset: a
+  // zero written to a register whose load may still be in flight would make
+  // the compiler wait there for every outstanding load.
+  if (has_i) {
+    const int f = gi + gj * (ni + 1);
+    in.mi[0] = at(G.inx, f); in.mi[1] = at(G.iny, f); in.mi[2] = at(G.ia, f);
+  }
+  if (has_j) {
+    const int f = gi + gj * ni;
+    in.mj[0] = at(G.jnx, f); in.mj[1] = at(G.jny, f); in.mj[2] = at(G.ja, f);
+  }
+  if constexpr (EDGE) {
+    constexpr EdgePlan E = edge_plan(TX, TY);
+    static_assert(E.jstart + TX <= E.istart || E.istart + TY <= E.jstart, "edge ranges overlap");
+    if (tid >= E.jstart && tid < E.jstart + TX) {
+      const int ei = i0 + tid - E.jstart, ej = j0 + TY;
+      if (ei < ni && ej <= nj) {
+        const int f = ei + ej * ni;
+        in.me[0] = at(G.jnx, f); in.me[1] = at(G.jny, f); in.me[2] = at(G.ja, f);
+      }
+    } else if (tid >= E.istart && tid < E.istart + TY) {
+      const int ei = i0 + TX, ej = j0 + tid - E.istart;
+      if (ei <= ni && ej < nj) {
+        const int f = ei + ej * (ni + 1);
+        in.me[0] = at(G.inx, f); in.me[1] = at(G.iny, f); in.me[2] = at(G.ia, f);
+      }
+    }
+  }
+  if (valid) {
+    const int c = gi + gj * ni;
+    in.vol = at(G.vol, c);
+    in.planar = at(G.planar, c);
+    if (MODE == TILE_STAGE && A.dt_mode == DT_READ) in.dt = at(A.dt, c);
+    if (MODE == TILE_STAGE && A.dt_mode == DT_GLOBAL) in.dt = *A.dt_global;
+    // u0 of the SSP-RK combination; global stepping also keeps it for a
+    // zero time step (the state is left unchanged).
+    if (MODE == TILE_STAGE && (A.a != T(0.) || A.dt_mode == DT_GLOBAL)) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) in.u0[q] = at(A.u0.q[q], c);
+    }
+  }
+  if (tile_is_inner<TX, TY>(ni, nj, i0, j0)) {
+    const int c = gi + gj * ni;
+    in.own = {at(A.src.q[0], c), at(A.src.q[1], c), at(A.src.q[2], c), at(A.src.q[3], c)};
+    if (4 * (TX + TY) <= NT && tid < 4 * (TX + TY)) {
+      int wi, wj;
+      halo_cell<TX, TY>(tid, wi, wj);
+      const int h = (i0 + wi - 2) + (j0 + wj - 2) * ni;
+      in.halo = {at(A.src.q[0], h), at(A.src.q[1], h), at(A.src.q[2], h), at(A.src.q[3], h)};
+    }
+  }
+}
+
+// The tile (bx, by) of one grid from its inputs; its FP64 partial goes to
+// A.partials[slot]. EDGE: the high-edge faces' metrics come with the
+// inputs (otherwise those faces load them).
+template <typename T, int TX, int TY, int REC, int LIM, int FLUX, int MODE, bool EDGE>
+__device__ __forceinline__ void tile_compute(const TileArgsT<T>& A, int bx, int by, int slot, const TileInputs<T>& in) {
   constexpr int NT = TX * TY;
   constexpr int WX = TX + 4, WY = TY + 4;
   __shared__ T sp[4][WY][WX];        // primitives of the tile and its halo
@@ -241,53 +360,25 @@ __device__ __forceinline__ void tile_body(const TileArgsT<T>& A, int bx, int by,
   const int tid = threadIdx.x;
 
   // Phase 1: primitives of the window [i0-2, i0+TX+2) x [j0-2, j0+TY+2),
-  // corners excluded. Each thread first loads its own cell, whose conserved
-  // state it keeps in registers for the update; then the 4 (TX + TY) halo
-  // cells: two rows below and above the tile, two columns left and right.
+  // corners excluded: each thread's own cell, whose conserved state it
+  // keeps in registers for the update, then the 4 (TX + TY) halo cells.
+  // Inner tiles take the state from the inputs; the others evaluate ghost
+  // cells from the boundary conditions (load_cell).
   const int li = tid % TX, lj = tid / TX;
   const int gi = i0 + li, gj = j0 + lj;
   const bool valid = gi < ni && gj < nj;
   Vec4<T> own = {T(0.), T(0.), T(0.), T(0.)};
-  // Loads that do not depend on the staged primitives are issued first, so
-  // their latency overlaps the staging and the faces: the metrics of the
-  // thread's own i- and j-face, and the cell's volume, planar area, stored
-  // time step and u0.
-  const bool has_i = gi <= ni && gj < nj, has_j = gi < ni && gj <= nj;
-  T mi[3] = {T(0.), T(0.), T(0.)}, mj[3] = {T(0.), T(0.), T(0.)};
-  if (has_i) {
-    const int f = gi + gj * (ni + 1);
-    mi[0] = G.inx[f]; mi[1] = G.iny[f]; mi[2] = G.ia[f];
-  }
-  if (has_j) {
-    const int f = gi + gj * ni;
-    mj[0] = G.jnx[f]; mj[1] = G.jny[f]; mj[2] = G.ja[f];
-  }
-  T c_vol = T(1.), c_planar = T(0.), c_dt = T(0.), c_u0[4] = {T(0.), T(0.), T(0.), T(0.)};
-  if (valid) {
-    const int c = gi + gj * ni;
-    c_vol = G.vol[c];
-    c_planar = G.planar[c];
-    if (MODE == TILE_STAGE && A.dt_mode == DT_READ) c_dt = A.dt[c];
-    if (MODE == TILE_STAGE && A.a != T(0.)) {
-#pragma unroll
-      for (int q = 0; q < 4; q++) c_u0[q] = A.u0.q[q][c];
-    }
-  }
-  // Tiles whose whole window lies inside the grid (all but a ring of them)
-  // take the direct loads; the block-uniform test costs no divergence.
-  const bool inner = i0 >= 2 && j0 >= 2 && i0 + TX + 2 <= ni && j0 + TY + 2 <= nj;
-  auto fetch = [&](int i, int j) -> Vec4<T> {
-    if (inner) {
-      const int c = i + j * ni;
-      return {A.src.q[0][c], A.src.q[1][c], A.src.q[2][c], A.src.q[3][c]};
-    }
-    return load_cell(A.src, G, *A.B, i, j, g);
-  };
+  const T* mi = in.mi;
+  const T* mj = in.mj;
+  const T c_vol = in.vol, c_planar = in.planar, c_dt = in.dt;
+  const T* c_u0 = in.u0;
+  // The block-uniform test costs no divergence.
+  const bool inner = tile_is_inner<TX, TY>(ni, nj, i0, j0);
   // On a tile at the high edge of the grid the thread's position may be a
   // ghost cell (gi in [ni, ni + 1] or gj in [nj, nj + 1]) that the faces
   // near the edge still need.
   if (!(gi >= ni && gj >= nj) && gi <= ni + 1 && gj <= nj + 1) {
-    const Vec4<T> q = fetch(gi, gj);
+    const Vec4<T> q = inner ? in.own : load_cell(A.src, G, *A.B, gi, gj, g);
     if (valid) own = q;
     Prim<T> w = to_prim(q.x, q.y, q.z, q.w, g);
     sp[0][lj + 2][li + 2] = w.rho;
@@ -297,19 +388,18 @@ __device__ __forceinline__ void tile_body(const TileArgsT<T>& A, int bx, int by,
   }
   for (int h = tid; h < 4 * (TX + TY); h += NT) {
     int wi, wj;
-    if (h < 4 * TX) {
-      const int r = h / TX;
-      wi = 2 + h % TX;
-      wj = r < 2 ? r : TY + r;
-    } else {
-      const int r = (h - 4 * TX) / TY;
-      wj = 2 + (h - 4 * TX) % TY;
-      wi = r < 2 ? r : TX + r;
-    }
+    halo_cell<TX, TY>(h, wi, wj);
     const int hi = i0 + wi - 2, hj = j0 + wj - 2;
     const bool oi = hi < 0 || hi >= ni, oj = hj < 0 || hj >= nj;
     if ((oi && oj) || hi > ni + 1 || hj > nj + 1) continue;
-    Vec4<T> q = fetch(hi, hj);
+    Vec4<T> q;
+    if (inner && 4 * (TX + TY) <= NT) q = in.halo;
+    else if (inner) {
+      const int c = hi + hj * ni;
+      q = {A.src.q[0][c], A.src.q[1][c], A.src.q[2][c], A.src.q[3][c]};
+    } else {
+      q = load_cell(A.src, G, *A.B, hi, hj, g);
+    }
     Prim<T> w = to_prim(q.x, q.y, q.z, q.w, g);
     sp[0][wj][wi] = w.rho;
     sp[1][wj][wi] = w.u;
@@ -403,8 +493,9 @@ __device__ __forceinline__ void tile_body(const TileArgsT<T>& A, int bx, int by,
   auto faces = [&](auto inner_tag) {
     do_face(inner_tag, 0, li, lj, mi);
     do_face(inner_tag, 1, li, lj, mj);
-    if (tid >= E.jstart && tid < E.jstart + TX) do_face(inner_tag, 1, tid - E.jstart, TY, nullptr);
-    if (tid >= E.istart && tid < E.istart + TY) do_face(inner_tag, 0, TX, tid - E.istart, nullptr);
+    const T* me = EDGE ? in.me : nullptr;
+    if (tid >= E.jstart && tid < E.jstart + TX) do_face(inner_tag, 1, tid - E.jstart, TY, me);
+    if (tid >= E.istart && tid < E.istart + TY) do_face(inner_tag, 0, TX, tid - E.istart, me);
   };
   if (inner) faces(std::integral_constant<bool, true>());
   else faces(std::integral_constant<bool, false>());
@@ -450,11 +541,9 @@ __device__ __forceinline__ void tile_body(const TileArgsT<T>& A, int bx, int by,
         const T lam = bs_abs(w.u * six + w.v * siy) + a * bs_sqrt(six * six + siy * siy) +
                           bs_abs(w.u * sjx + w.v * sjy) + a * bs_sqrt(sjx * sjx + sjy * sjy);
         dt = A.cfl * vol * bs_rcp(lam);
-        A.dt[c] = dt;
-      } else if (A.dt_mode == DT_GLOBAL) {
-        dt = *A.dt_global;
+        at(A.dt, c) = dt;
       } else {
-        dt = c_dt;
+        dt = c_dt;  // the stored local step, or the global one
       }
       const T k = dt * bs_rcp(vol);
       Vec4<T> out;
@@ -478,10 +567,10 @@ __device__ __forceinline__ void tile_body(const TileArgsT<T>& A, int bx, int by,
         // A zero time step (global stepping already at t_end) leaves the
         // state unchanged (euler2d.wgsl update_cells).
 #pragma unroll
-        for (int q = 0; q < 4; q++) o[q] = A.u0.q[q][c];
+        for (int q = 0; q < 4; q++) o[q] = c_u0[q];
       }
 #pragma unroll
-      for (int q = 0; q < 4; q++) A.dst.q[q][c] = o[q];
+      for (int q = 0; q < 4; q++) at(A.dst.q[q], c) = o[q];
       out = {o[0], o[1], o[2], o[3]};
       flag_non_physical(!is_physical(out, g), A.control);
     } else if (MODE == TILE_IMPLICIT) {
@@ -492,21 +581,31 @@ __device__ __forceinline__ void tile_body(const TileArgsT<T>& A, int bx, int by,
       const T scale = (T)(0.5 * ((double)A.omega + 1.0 / cfl));
       T d = scale * (xi[0][lj][li] + xi[0][lj][li + 1] + xj[0][lj][li] + xj[0][lj + 1][li]);
       if (A.axisymmetric) d += planar * (bs_abs(w.v) + bs_sqrt(bs_div(g * w.p, w.rho)));
-      A.diag[c] = d;
+      at(A.diag, c) = d;
       // The first red half-sweep (dQ = R / D on red cells, 0 on black) is
       // formed by the relaxation's first launch from R and D.
 #pragma unroll
-      for (int q = 0; q < 4; q++) A.res[q][c] = r[q];
+      for (int q = 0; q < 4; q++) at(A.res[q], c) = r[q];
     } else if (A.rvec[0] != nullptr) {
 #pragma unroll
-      for (int q = 0; q < 4; q++) A.rvec[q][c] = r[q];
+      for (int q = 0; q < 4; q++) at(A.rvec[q], c) = r[q];
     }
   }
   if (A.partials != nullptr) {
     double s = block_sum<NT>(r2, red);
     if (tid == 0) A.partials[slot] = s;
   }
-  if (A.count_step && bx == 0 && by == 0 && tid == 0) A.control->steps += 1;
+  if (A.count_step && bx == 0 && by == 0 && tid == 0) atomicAdd((unsigned long long*)&A.control->steps, 1ull);
+}
+
+template <typename T, int TX, int TY, int REC, int LIM, int FLUX, int MODE>
+__device__ __forceinline__ void tile_body(const TileArgsT<T>& A, int bx, int by, int slot) {
+  // The FP32 tiles load the edge faces' metrics with the other inputs; the
+  // FP64 tiles, short of registers, when the edge faces are computed.
+  constexpr bool EDGE = std::is_same<T, float>::value;
+  TileInputs<T> in;
+  tile_inputs<T, TX, TY, MODE, EDGE>(A, bx, by, in);
+  tile_compute<T, TX, TY, REC, LIM, FLUX, MODE, EDGE>(A, bx, by, slot, in);
 }
 
 // Workgroups go to the 8 XCDs round-robin by their linear index, each XCD
@@ -528,8 +627,9 @@ __global__ __launch_bounds__(TX * TY) void tile_kernel(TileArgsT<T> A) {
   tile_body<T, TX, TY, REC, LIM, FLUX, MODE>(A, t % tiles_x, t / tiles_x, t);
 }
 
-// The FP32 stage tiles held to 64 VGPRs: eight waves per SIMD, so four
-// 512-thread tiles fit on a CU (their LDS allows four) instead of three.
+// The FP32 stage tiles held to 64 VGPRs: eight waves per SIMD, so a CU
+// holds as many tiles as their LDS allows (eight of 32 x 8, four of 64 x 8)
+// instead of three quarters of that.
 template <int TX, int TY, int REC, int LIM, int FLUX, int MODE>
 __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(8, 8))) void tile_kernel_occ8(
     TileArgsT<float> A) {
