@@ -14,7 +14,7 @@ namespace bs {
 // threads hold twice the registers), as the single FP64 solver's.
 template <typename T>
 struct BatchTile {
-  static constexpr int lx = 64, ly = 8;  // large tile
+  static constexpr int lx = 32, ly = 8;  // large tile (the single FP32 solver's)
   static constexpr int sx = 32, sy = 4;  // small tile
 };
 template <>
@@ -43,7 +43,11 @@ namespace {
 
 template <typename T, int TX_, int TY_, int REC, int LIM, int FLUX>
 BatchTileFnT<T> pick_batch_mode(int mode) {
-  if (mode == TILE_STAGE) return batch_tile_kernel<T, TX_, TY_, REC, LIM, FLUX, TILE_STAGE>;
+  if constexpr (std::is_same<T, float>::value) {
+    if (mode == TILE_STAGE) return batch_tile_kernel_occ8<TX_, TY_, REC, LIM, FLUX, TILE_STAGE>;
+  } else {
+    if (mode == TILE_STAGE) return batch_tile_kernel<T, TX_, TY_, REC, LIM, FLUX, TILE_STAGE>;
+  }
   if (mode == TILE_IMPLICIT) return batch_tile_kernel<T, TX_, TY_, REC, LIM, FLUX, TILE_IMPLICIT>;
   return batch_tile_kernel<T, TX_, TY_, REC, LIM, FLUX, TILE_RESIDUAL>;
 }

@@ -656,6 +656,14 @@ __global__ __launch_bounds__(TX * TY) void batch_tile_kernel(const TileArgsT<T>*
   tile_body<T, TX, TY, REC, LIM, FLUX, MODE>(args[w.args], w.bx, w.by, w.slot);
 }
 
+// The FP32 batched stage tiles held to 64 VGPRs, as tile_kernel_occ8.
+template <int TX, int TY, int REC, int LIM, int FLUX, int MODE>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(8, 8))) void batch_tile_kernel_occ8(
+    const TileArgsT<float>* __restrict__ args, const WorkItem* __restrict__ work) {
+  const WorkItem w = work[xcd_tile(blockIdx.x, gridDim.x)];
+  tile_body<float, TX, TY, REC, LIM, FLUX, MODE>(args[w.args], w.bx, w.by, w.slot);
+}
+
 // Deterministic FP64 sum of n partials by one block; writes sqrt(sum / cells)
 // when `rms` (euler2d.ts:480-491) or the plain sum.
 static __global__ __launch_bounds__(256) void finalize_sum(const double* partials, int n, double cells, int rms,
@@ -835,48 +843,92 @@ __device__ __forceinline__ void relax_tile_body(const RelaxArgsT<T>& A, int bx, 
   const int ni = G.ni, nj = G.nj;
   const int i0 = bx * RTX - NH, j0 = by * RTY - NH;
   const int tid = threadIdx.x;
-  for (int e = tid; e < WX * WY; e += NT) {
+  // Every global load of the staging is issued before the first LDS store,
+  // so their latencies overlap: a thread's window cells (q, R, D, and dQ
+  // unless it is formed here) and faces (normal, area, lambda A) first go
+  // to registers.
+  constexpr int NC = (WX * WY + NT - 1) / NT, NFI = ((CX + 1) * CY + NT - 1) / NT,
+                NFJ = (CX * (CY + 1) + NT - 1) / NT;
+  float cq[NC][4], cr[NC][4], cd[NC], cdq[NC][4], vi[NFI][4], vj[NFJ][4];
+  bool ok_c[NC], ok_i[NFI], ok_j[NFJ];
+#pragma unroll
+  for (int k = 0; k < NC; k++) {
+    const int e = tid + k * NT;
+    const int gi = i0 + e % WX, gj = j0 + e / WX;
+    ok_c[k] = e < WX * WY && gi >= 0 && gi < ni && gj >= 0 && gj < nj;
+    if (ok_c[k]) {
+      const int c = gi + gj * ni;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        cq[k][q] = (float)at(A.q.q[q], c);
+        cr[k][q] = at(A.res[q], c);
+        if (!init_red) cdq[k][q] = at(A.dq[q], c);
+      }
+      cd[k] = at(A.diag, c);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NFI; k++) {
+    const int e = tid + k * NT;
+    const int gi = i0 + 1 + e % (CX + 1), gj = j0 + 1 + e / (CX + 1);
+    ok_i[k] = e < (CX + 1) * CY && gi >= 0 && gi <= ni && gj >= 0 && gj < nj;
+    if (ok_i[k]) {
+      const int f = gi + gj * (ni + 1);
+      vi[k][0] = (float)at(G.inx, f);
+      vi[k][1] = (float)at(G.iny, f);
+      vi[k][2] = (float)at(G.ia, f);
+      vi[k][3] = at(A.lam_i, f);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NFJ; k++) {
+    const int e = tid + k * NT;
+    const int gi = i0 + 1 + e % CX, gj = j0 + 1 + e / CX;
+    ok_j[k] = e < CX * (CY + 1) && gi >= 0 && gi < ni && gj >= 0 && gj <= nj;
+    if (ok_j[k]) {
+      const int f = gi + gj * ni;
+      vj[k][0] = (float)at(G.jnx, f);
+      vj[k][1] = (float)at(G.jny, f);
+      vj[k][2] = (float)at(G.ja, f);
+      vj[k][3] = at(A.lam_j, f);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NC; k++) {
+    if (!ok_c[k]) continue;
+    const int e = tid + k * NT;
     const int wx = e % WX, wy = e / WX;
     const int gi = i0 + wx, gj = j0 + wy;
-    if (gi < 0 || gi >= ni || gj < 0 || gj >= nj) continue;
-    const int c = gi + gj * ni;
-    float r[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      sq[q][wy][wx] = (float)A.q.q[q][c];
-      r[q] = A.res[q][c];
-      sres[q][wy][wx] = r[q];
+      sq[q][wy][wx] = cq[k][q];
+      sres[q][wy][wx] = cr[k][q];
     }
-    const float d = A.diag[c];
-    sdiag[wy][wx] = d;
+    sdiag[wy][wx] = cd[k];
     if (init_red) {
       const bool red = ((gi + gj) & 1) == 0;
 #pragma unroll
-      for (int q = 0; q < 4; q++) sdq[q][wy][wx] = red ? bs_div(r[q], d) : 0.f;
+      for (int q = 0; q < 4; q++) sdq[q][wy][wx] = red ? bs_div(cr[k][q], cd[k]) : 0.f;
     } else {
 #pragma unroll
-      for (int q = 0; q < 4; q++) sdq[q][wy][wx] = A.dq[q][c];
+      for (int q = 0; q < 4; q++) sdq[q][wy][wx] = cdq[k][q];
     }
   }
-  for (int e = tid; e < (CX + 1) * CY; e += NT) {
+#pragma unroll
+  for (int k = 0; k < NFI; k++) {
+    if (!ok_i[k]) continue;
+    const int e = tid + k * NT;
     const int fx = 1 + e % (CX + 1), fy = 1 + e / (CX + 1);
-    const int gi = i0 + fx, gj = j0 + fy;
-    if (gi < 0 || gi > ni || gj < 0 || gj >= nj) continue;
-    const int f = gi + gj * (ni + 1);
-    sfi[0][fy - 1][fx - 1] = (float)G.inx[f];
-    sfi[1][fy - 1][fx - 1] = (float)G.iny[f];
-    sfi[2][fy - 1][fx - 1] = (float)G.ia[f];
-    sfi[3][fy - 1][fx - 1] = A.lam_i[f];
+#pragma unroll
+    for (int m = 0; m < 4; m++) sfi[m][fy - 1][fx - 1] = vi[k][m];
   }
-  for (int e = tid; e < CX * (CY + 1); e += NT) {
+#pragma unroll
+  for (int k = 0; k < NFJ; k++) {
+    if (!ok_j[k]) continue;
+    const int e = tid + k * NT;
     const int fx = 1 + e % CX, fy = 1 + e / CX;
-    const int gi = i0 + fx, gj = j0 + fy;
-    if (gi < 0 || gi >= ni || gj < 0 || gj > nj) continue;
-    const int f = gi + gj * ni;
-    sfj[0][fy - 1][fx - 1] = (float)G.jnx[f];
-    sfj[1][fy - 1][fx - 1] = (float)G.jny[f];
-    sfj[2][fy - 1][fx - 1] = (float)G.ja[f];
-    sfj[3][fy - 1][fx - 1] = A.lam_j[f];
+#pragma unroll
+    for (int m = 0; m < 4; m++) sfj[m][fy - 1][fx - 1] = vj[k][m];
   }
   __syncthreads();
   const float g = A.gamma;
@@ -961,16 +1013,16 @@ __device__ __forceinline__ void relax_tile_body(const RelaxArgsT<T>& A, int bx, 
         o = update_cell<float>(sq[0][wy][wx], sq[1][wy][wx], sq[2][wy][wx], sq[3][wy][wx], sdq[0][wy][wx],
                                sdq[1][wy][wx], sdq[2][wy][wx], sdq[3][wy][wx], U.gamma);
       else
-        o = update_cell<T>(U.src.q[0][c], U.src.q[1][c], U.src.q[2][c], U.src.q[3][c], (T)sdq[0][wy][wx],
-                           (T)sdq[1][wy][wx], (T)sdq[2][wy][wx], (T)sdq[3][wy][wx], U.gamma);
-      U.dst.q[0][c] = o.x;
-      U.dst.q[1][c] = o.y;
-      U.dst.q[2][c] = o.z;
-      U.dst.q[3][c] = o.w;
+        o = update_cell<T>(at(U.src.q[0], c), at(U.src.q[1], c), at(U.src.q[2], c), at(U.src.q[3], c),
+                           (T)sdq[0][wy][wx], (T)sdq[1][wy][wx], (T)sdq[2][wy][wx], (T)sdq[3][wy][wx], U.gamma);
+      at(U.dst.q[0], c) = o.x;
+      at(U.dst.q[1], c) = o.y;
+      at(U.dst.q[2], c) = o.z;
+      at(U.dst.q[3], c) = o.w;
       flag_non_physical(!is_physical(o, U.gamma), U.control);
     } else {
 #pragma unroll
-      for (int q = 0; q < 4; q++) A.dq[q][c] = sdq[q][wy][wx];
+      for (int q = 0; q < 4; q++) at(A.dq[q], c) = sdq[q][wy][wx];
     }
   }
   if (fuse && bx == 0 && by == 0 && tid == 0) advance_control(U);

@@ -3,8 +3,8 @@ load_cell in bowshock_amd/csrc/bs_kernels.hpp), replayed on the CPU for
 every thread of every tile: the state, metric, time-step and boundary
 arrays it reads and the arrays it writes, on grids whose last tiles are
 partial in both directions, with walls, face states and periodic sides,
-for the FP32 tile (32 x 8), the batch's large FP32 tile (64 x 8), the FP64
-tile (64 x 4) and the batch's small tile (32 x 4). The index arithmetic is
+for the FP32 tile (32 x 8, also the batch's large FP32 tile), the FP64
+tile (64 x 4), the batch's small tile (32 x 4) and a 64 x 8 tile. The index arithmetic is
 the kernel's, restated; each access is checked against its array's length. Among the grids is the
 412 x 80 blunt-body C grid (walls on iMin and jMin) of the fitted
 workload."""
