@@ -350,6 +350,29 @@ class _CaseGroups:
 
         return [autotune_tiling(b, steps=steps, implicit=implicit) for _, b in self.groups]
 
+    def kernel_sample(self, steps, implicit, cells):
+        """Per batch: `steps` steps with HIP-event timing of every pass (no
+        graphs), the mean microseconds per pass of each kernel and the
+        relaxation's (or stage's) fraction of the HBM roofline by the byte
+        model of the single-grid workloads. Advances the state."""
+        out = []
+        for ks, b in self.groups:
+            b.enableTiming(True)
+            if implicit:
+                b.iterateImplicit(steps, 10.0, 2)
+            else:
+                b.iterate(steps)
+            us = {k: 1e3 * b.kernelTiming(k)[0] for k in ("stage", "implicitPrepare", "implicitRelax")}
+            b.enableTiming(False)
+            n = float(cells[ks].sum())
+            if implicit:
+                frac = 3 * 58.0 * n / (us["implicitRelax"] * 1e-6) / HBM_PEAK
+            else:
+                frac = BYTES_PER_STAGE * n / (us["stage"] * 1e-6) / HBM_PEAK
+            out.append({"cases": len(ks), "cells": int(n), "us_per_pass": {k: v for k, v in us.items() if v > 0},
+                        ("relaxation" if implicit else "stage") + "_roofline_frac": frac})
+        return out
+
     def solve(self, p_inf, q_inf, plan, reference=None, cfl_start=None):
         import numpy as np
 
@@ -481,6 +504,8 @@ def run_batch(a):
     t_create = time.time() - t0
     batch.setUniform(S["frees"])
     tile = batch.autotune(steps=24 if sweep else 12, implicit=not sweep)
+    kernels = batch.kernel_sample(24 if sweep else 12, not sweep, cells)
+    print(f"kernel passes (HIP events, no graphs): {kernels}", file=sys.stderr, flush=True)
     batch.setUniform(S["frees"])
     print(f"{len(batch.groups)} batch(es) of {[len(g[0]) for g in batch.groups]} cases ({cells.sum():.3e} cells) "
           f"created in {t_create:.1f}s; small tile below {tile} cells", file=sys.stderr, flush=True)
@@ -523,7 +548,8 @@ def run_batch(a):
                    "cases": len(cases), "accepted": len(ok), "refusals": refusals, "converged": converged,
                    "outcomes": reasons, "cells": int(cells.sum()), "cell_updates": cu,
                    "cell_updates_per_s": cu / wall, "grid_seconds": t_grids, "create_seconds": t_create,
-                   "small_tile_below": tile, "converged_by_seconds": pct, "residual_at_most_1e-6": below,
+                   "small_tile_below": tile, "kernel_passes": kernels, "converged_by_seconds": pct,
+                   "residual_at_most_1e-6": below,
                    "steps_percentiles": {f"p{q}": float(np.percentile(steps, q)) for q in (50, 90, 99, 100)},
                    "batches": [len(g[0]) for g in batch.groups],
                    "parallelism": "single GPU, one batch per flux (H grids HLLC, C grids Rusanov)"},

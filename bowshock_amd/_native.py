@@ -103,6 +103,8 @@ SIGNATURES = {
     "bs_batch_residual_norms": (C.c_int, [_vp, _dp]),
     "bs_batch_drags": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
     "bs_batch_status": (C.c_int, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_int64)]),
+    "bs_batch_enable_timing": (C.c_int, [_vp, C.c_int32]),
+    "bs_batch_kernel_timing": (C.c_int, [_vp, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "bs_mlp_create": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(_vp)]),
     "bs_mlp_destroy": (C.c_int, [_vp]),
     "bs_mlp_param_count": (C.c_int64, [_vp]),

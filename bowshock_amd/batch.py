@@ -130,6 +130,20 @@ class GpuBatch2D:
     def iterate(self, steps: int):
         check(N.lib().bs_batch_iterate(self._h, int(steps)))
 
+    def enableTiming(self, on: bool = True):
+        """HIP-event timing of each pass over the active cases (the steps
+        then run without graphs); False stops."""
+        check(N.lib().bs_batch_enable_timing(self._h, 1 if on else 0))
+
+    def kernelTiming(self, kernel: str = "stage"):
+        """(ms per pass, passes) of "stage", "implicitPrepare" or
+        "implicitRelax" since the last call."""
+        ms = C.c_double()
+        L = C.c_int64()
+        check(N.lib().bs_batch_kernel_timing(self._h, {"stage": 0, "implicitPrepare": 1, "implicitRelax": 2}[kernel],
+                                             C.byref(ms), C.byref(L)))
+        return ms.value, L.value
+
     def setImplicitCfl(self, cfl: float):
         check(N.lib().bs_batch_set_implicit_cfl(self._h, float(cfl)))
 

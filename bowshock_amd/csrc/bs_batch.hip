@@ -132,6 +132,17 @@ struct bs_batch {
   double upd_cfl_max = -1, upd_growth = -1;
   int cur = 0;
   StepGraph explicit_graph, implicit_graph;
+  // bs_batch_enable_timing: HIP-event pairs around each stencil and
+  // relaxation launch (kind 0 stage, 1 prepare, 2 relaxation); the steps
+  // then run without graphs.
+  struct {
+    bool enabled = false;
+    std::vector<hipEvent_t> pool;
+    std::vector<int> kinds;
+    int used = 0;
+    double total_ms[3] = {0, 0, 0};
+    long long launches[3] = {0, 0, 0};
+  } timing;
   std::atomic<bool> busy{false};
 };
 
@@ -213,8 +224,45 @@ void batch_free(bs_batch* b) {
                  b->wall_count};
   for (void* p : all)
     if (p) (void)hipFree(p);
+  for (hipEvent_t e : b->timing.pool) (void)hipEventDestroy(e);
+  b->timing.pool.clear();
   if (b->stream) (void)hipStreamDestroy(b->stream);
   b->stream = nullptr;
+}
+
+// Index of a new event pair of `kind` with its start recorded, or -1 when
+// timing is off (or no event could be made).
+int timing_begin(bs_batch* b, int kind) {
+  auto& t = b->timing;
+  if (!t.enabled || kind < 0) return -1;
+  if ((size_t)(2 * t.used + 2) > t.pool.size()) {
+    for (int k = 0; k < 64; k++) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return -1;
+      t.pool.push_back(e);
+    }
+  }
+  t.kinds.resize(t.used + 1);
+  t.kinds[t.used] = kind;
+  (void)hipEventRecord(t.pool[2 * t.used], b->stream);
+  return t.used++;
+}
+
+void timing_end(bs_batch* b, int pair) {
+  if (pair >= 0) (void)hipEventRecord(b->timing.pool[2 * pair + 1], b->stream);
+}
+
+// After a synchronisation: the recorded pairs into the totals.
+void timing_collect(bs_batch* b) {
+  auto& t = b->timing;
+  for (int k = 0; k < t.used; k++) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, t.pool[2 * k], t.pool[2 * k + 1]) == hipSuccess) {
+      t.total_ms[t.kinds[k]] += ms;
+      t.launches[t.kinds[k]] += 1;
+    }
+  }
+  t.used = 0;
 }
 
 // Argument records of every role for every case (uploaded once; the
@@ -392,12 +440,15 @@ bs_status batch_build_lists(bs_batch* b) {
 template <typename T>
 bs_status batch_launch_tiles(bs_batch* b, int mode, int role) {
   const TileArgsT<T>* recs = static_cast<const TileArgsT<T>*>(b->tile_args) + (size_t)role * b->count;
+  // Both tile classes' launches count as one launch of the pass.
+  const int pair = timing_begin(b, mode == TILE_STAGE ? 0 : mode == TILE_IMPLICIT ? 1 : -1);
   for (int cl = 0; cl < 2; cl++) {
     if (b->n_tiles[cl] == 0) continue;
     const int threads = cl == 0 ? BatchTile<T>::lx * BatchTile<T>::ly : BatchTile<T>::sx * BatchTile<T>::sy;
     hipLaunchKernelGGL(batch_tile_fn<T>(b->scheme, cl, mode), dim3(b->n_tiles[cl]), dim3(threads), 0, b->stream, recs,
                        b->work_tiles[cl]);
   }
+  timing_end(b, pair);
   HIP_TRY(hipGetLastError(), "batch stencil launch");
   return BS_OK;
 }
@@ -445,8 +496,10 @@ bs_status batch_encode_implicit(bs_batch* b, int sweeps) {
     const int nh = halves - done < 3 ? halves - done : 3;
     const int init = done == 0;
     const UpdateArgsT<T>* u = done + nh == halves ? upd : nullptr;
+    const int pair = timing_begin(b, 2);
     hipLaunchKernelGGL(kernels[form][nh - 1], dim3(b->n_relax), dim3(RTX * RTY), 0, b->stream, rel, b->work_relax,
                        colour, init, u);
+    timing_end(b, pair);
     colour = (colour + nh) & 1;
     done += nh;
   }
@@ -457,7 +510,7 @@ bs_status batch_encode_implicit(bs_batch* b, int sweeps) {
 
 template <typename Encode>
 bs_status batch_run(bs_batch* b, StepGraph& g, int count, Encode encode) {
-  if (count >= GRAPH_STEPS) {
+  if (count >= GRAPH_STEPS && !b->timing.enabled) {
     bs_status st = replay_steps(b->stream, b->cur, g, count, encode);
     if (st) return st;
   } else {
@@ -467,6 +520,7 @@ bs_status batch_run(bs_batch* b, StepGraph& g, int count, Encode encode) {
     }
   }
   HIP_TRY(hipStreamSynchronize(b->stream), "batch steps");
+  timing_collect(b);
   return BS_OK;
 }
 
@@ -1062,6 +1116,27 @@ bs_status bs_batch_set_active(bs_batch* b, const uint8_t* active, int32_t count)
     b->active[c] = a;
   }
   if (changed) b->lists_dirty = true;
+  return BS_OK;
+}
+
+bs_status bs_batch_enable_timing(bs_batch* b, int32_t enable) {
+  BATCH_GUARD(b);
+  b->timing.enabled = enable != 0;
+  for (int k = 0; k < 3; k++) {
+    b->timing.total_ms[k] = 0;
+    b->timing.launches[k] = 0;
+  }
+  return BS_OK;
+}
+
+bs_status bs_batch_kernel_timing(bs_batch* b, int32_t kernel, double* ms_per_launch, int64_t* launches) {
+  BATCH_GUARD(b);
+  if (kernel < 0 || kernel > 2) return fail(BS_ERR_RANGE, "no such kernel");
+  const long long L = b->timing.launches[kernel];
+  if (ms_per_launch) *ms_per_launch = L > 0 ? b->timing.total_ms[kernel] / (double)L : 0.0;
+  if (launches) *launches = L;
+  b->timing.total_ms[kernel] = 0;
+  b->timing.launches[kernel] = 0;
   return BS_OK;
 }
 

@@ -806,15 +806,25 @@ __device__ __forceinline__ void advance_control(const UpdateArgsT<S>& U) {
 }
 
 
-// NH consecutive half-sweeps (colours colour0, colour0 ^ 1, ...) of a
-// 64 x 8 tile in one launch. Gauss-Seidel by colours: a cell's update reads
+// NH consecutive half-sweeps (colours colour0, colour0 ^ 1, ...) of an
+// RTX x RTY tile in one launch. Gauss-Seidel by colours: a cell's update reads
 // only cells of the other colour, so half-sweep s is exact on the tile grown
 // by NH - 1 - s cells, given q and dQ on the tile grown by NH cells; those
 // are staged in LDS once and the tile's dQ written back at the end. The
 // cell arithmetic is relax_body's (neighbour_term in the same order), so
 // the result is bitwise that of NH single half-sweeps; the grid's own
 // faces carry no neighbour term, as there.
-constexpr int RTX = 64, RTY = 8;
+// The relaxation tile: 32 x 16 measured fastest (1280 x 640, three
+// half-sweeps: 30.9 us; 64 x 8 31.7, 32 x 8 33.4, 64 x 16 37.7, 64 x 4
+// 51.4; the population of 500 rockets 1.5 % faster than at 64 x 8): its
+// 38 x 22 window stages fewer halo cells per cell than 70 x 14.
+#ifndef BS_RTX
+#define BS_RTX 32
+#endif
+#ifndef BS_RTY
+#define BS_RTY 16
+#endif
+constexpr int RTX = BS_RTX, RTY = BS_RTY;
 
 struct RelaxCtx {
   int i0, j0;  // global index of LDS window cell (0, 0)

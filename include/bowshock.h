@@ -391,7 +391,8 @@ bs_status bs_batch_read_state_f64(bs_batch *b, int32_t index, double *rho, doubl
  * `count` must be the batch's case count. */
 bs_status bs_batch_set_active(bs_batch *b, const uint8_t *active, int32_t count);
 /* Cases of fewer cells than the threshold run on 32 x 4 tiles, the others
- * on 64 x 8 tiles (set by the autotuner; 0 = all on 64 x 8). */
+ * on 32 x 8 tiles (64 x 4 on an FP64 batch) (set by the autotuner; 0 = all
+ * on the large tile). */
 bs_status bs_batch_set_tiling(bs_batch *b, int32_t small_tile_below_cells);
 /* bs_set_relaxation for every case of the batch (default
  * BS_RELAX_FLUX_DIFFERENCE on an FP32 batch; an FP64 batch takes
@@ -414,6 +415,13 @@ bs_status bs_batch_drags(bs_batch *b, const double *p_inf, const double *q_inf, 
                          double *cd);
 /* Status word (bit 0: non-physical) and step count per case. */
 bs_status bs_batch_status(bs_batch *b, uint32_t *status, int64_t *steps);
+/* bs_enable_timing / bs_kernel_timing for a batch: with timing enabled the
+ * steps run without graphs and a pair of HIP events brackets each pass over
+ * all active cases (kernel 0: an explicit stage, 1: the implicit prepare,
+ * 2: a relaxation launch); bs_batch_kernel_timing reports the mean
+ * milliseconds per pass since the last call. */
+bs_status bs_batch_enable_timing(bs_batch *b, int32_t enable);
+bs_status bs_batch_kernel_timing(bs_batch *b, int32_t kernel, double *ms_per_launch, int64_t *launches);
 
 /* ------------------------------------------------------------------------
  * Learned warm start. No counterpart in the reference, which contains no

@@ -112,6 +112,34 @@ def test_implicit_batch_matches_single_solves(ctx, small_below, form):
             assert np.array_equal(_state(batch.readState(k)), _state(s.readState())), (count, k)
 
 
+def test_timed_passes_run_the_same_steps(ctx):
+    """bs_batch_enable_timing: the steps run eagerly with an event pair per
+    pass, bitwise equal to the graph-replayed steps of an untimed batch, and
+    every pass is counted."""
+    cs = _cases(limiter="vanAlbada", reconstruction="sharedScaled")
+    timed, plain = (GpuBatch2D(ctx, [c[0] for c in cs]) for _ in range(2))
+    for b in (timed, plain):
+        b.setUniform([c[1] for c in cs])
+        b.setImplicitCfl(1.0)
+    timed.enableTiming(True)
+    for b in (timed, plain):
+        b.iterateImplicit(24, 10.0, 2)
+        b.iterate(12)
+    for k in range(len(cs)):
+        assert np.array_equal(_state(timed.readState(k)), _state(plain.readState(k))), k
+    prep, n_prep = timed.kernelTiming("implicitPrepare")
+    relax, n_relax = timed.kernelTiming("implicitRelax")
+    stage, n_stage = timed.kernelTiming("stage")
+    # Two sweeps: the prepare pass and one relaxation pass of three
+    # half-sweeps per step; SSP-RK3: three stage passes per step.
+    assert (n_prep, n_relax, n_stage) == (24, 24, 36)
+    assert prep > 0 and relax > 0 and stage > 0
+    assert timed.kernelTiming("implicitRelax") == (0.0, 0)
+    timed.enableTiming(False)
+    for b in (timed, plain):
+        b.destroy()
+
+
 def test_batch_solve_matches_single_converged_drag(ctx):
     """The driver loop retires each case once its residual and drag have
     settled; the drags equal a single solve run for the same step count."""
