@@ -33,6 +33,7 @@ Prints one JSON line.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import re
@@ -346,9 +347,9 @@ class _CaseGroups:
         return self.groups[g][1].readState(local)
 
     def autotune(self, steps, implicit):
-        from bowshock_amd.batch import autotune_tiling
+        from bowshock_amd.batch import autotune
 
-        return [autotune_tiling(b, steps=steps, implicit=implicit) for _, b in self.groups]
+        return [autotune(b, steps=steps, implicit=implicit) for _, b in self.groups]
 
     def kernel_sample(self, steps, implicit, cells):
         """Per batch: `steps` steps with HIP-event timing of every pass (no
@@ -503,12 +504,12 @@ def run_batch(a):
     batch = _CaseGroups(ctx, S)
     t_create = time.time() - t0
     batch.setUniform(S["frees"])
-    tile = batch.autotune(steps=24 if sweep else 12, implicit=not sweep)
+    tuned = [dataclasses.asdict(t) for t in batch.autotune(steps=24 if sweep else 12, implicit=not sweep)]
     kernels = batch.kernel_sample(24 if sweep else 12, not sweep, cells)
     print(f"kernel passes (HIP events, no graphs): {kernels}", file=sys.stderr, flush=True)
     batch.setUniform(S["frees"])
     print(f"{len(batch.groups)} batch(es) of {[len(g[0]) for g in batch.groups]} cases ({cells.sum():.3e} cells) "
-          f"created in {t_create:.1f}s; small tile below {tile} cells", file=sys.stderr, flush=True)
+          f"created in {t_create:.1f}s; launch tuning {tuned}", file=sys.stderr, flush=True)
     t = time.perf_counter()
     results = batch.solve(S["p_inf"], S["q_inf"], plan)
     wall = time.perf_counter() - t
@@ -548,7 +549,7 @@ def run_batch(a):
                    "cases": len(cases), "accepted": len(ok), "refusals": refusals, "converged": converged,
                    "outcomes": reasons, "cells": int(cells.sum()), "cell_updates": cu,
                    "cell_updates_per_s": cu / wall, "grid_seconds": t_grids, "create_seconds": t_create,
-                   "small_tile_below": tile, "kernel_passes": kernels, "converged_by_seconds": pct,
+                   "launch_tuning": tuned, "kernel_passes": kernels, "converged_by_seconds": pct,
                    "residual_at_most_1e-6": below,
                    "steps_percentiles": {f"p{q}": float(np.percentile(steps, q)) for q in (50, 90, 99, 100)},
                    "batches": [len(g[0]) for g in batch.groups],

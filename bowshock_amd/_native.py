@@ -95,6 +95,9 @@ SIGNATURES = {
     "bs_batch_set_from_coarser": (C.c_int, [_vp, _vp]),
     "bs_batch_set_active": (C.c_int, [_vp, C.POINTER(C.c_uint8), C.c_int32]),
     "bs_batch_set_tiling": (C.c_int, [_vp, C.c_int32]),
+    "bs_batch_set_persistent": (C.c_int, [_vp, C.c_int32]),
+    "bs_batch_set_relax_fusion": (C.c_int, [_vp, C.c_int32]),
+    "bs_batch_persistent_info": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "bs_batch_set_relaxation": (C.c_int, [_vp, C.c_int32]),
     "bs_batch_iterate": (C.c_int, [_vp, C.c_int32]),
     "bs_batch_set_implicit_cfl": (C.c_int, [_vp, C.c_double]),

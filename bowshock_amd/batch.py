@@ -57,6 +57,8 @@ class GpuBatch2D:
         self.shapes = [(c.grid.ni, c.grid.nj) for c in configs]
         self.active = np.ones(self.count, dtype=np.uint8)
         self.small_below = 0
+        self.persistent_tile = 0
+        self.relax_fusion = 3
 
     def destroy(self):
         if getattr(self, "_h", None):
@@ -126,6 +128,25 @@ class GpuBatch2D:
     def setTiling(self, small_tile_below_cells: int):
         check(N.lib().bs_batch_set_tiling(self._h, int(small_tile_below_cells)))
         self.small_below = int(small_tile_below_cells)
+
+    def setPersistent(self, tile: int):
+        """Explicit steps in persistent launches (bs_batch_set_persistent):
+        tile 1 (32 x 4), 2 (32 x 8), or 0 (one launch per stage)."""
+        check(N.lib().bs_batch_set_persistent(self._h, int(tile)))
+        self.persistent_tile = int(tile)
+
+    def persistentInfo(self):
+        """(last iterate was persistent, blocks of a launch over the active
+        set (0: does not fit), blocks the chip holds at once)."""
+        u, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        check(N.lib().bs_batch_persistent_info(self._h, C.byref(u), C.byref(b), C.byref(c)))
+        return bool(u.value), b.value, c.value
+
+    def setRelaxFusion(self, half_sweeps: int):
+        """At most this many red-black half-sweeps (1-3) per relaxation
+        launch (bs_batch_set_relax_fusion); the states do not change."""
+        check(N.lib().bs_batch_set_relax_fusion(self._h, int(half_sweeps)))
+        self.relax_fusion = int(half_sweeps)
 
     def iterate(self, steps: int):
         check(N.lib().bs_batch_iterate(self._h, int(steps)))
@@ -343,15 +364,36 @@ def _cache_path(path: Optional[str]) -> str:
     return os.path.join(base, "bowshock_amd", "tiling.json")
 
 
-def autotune_tiling(batch: GpuBatch2D, steps: int = 24, candidates: Sequence[int] = (0, 2048, 8192, 32768),
-                    cache: Optional[str] = None, implicit: bool = False) -> int:
-    """Pick the cell count below which cases run on the small tile by
-    timing ``steps`` steps of the batch at each candidate, and cache the
-    choice by the batch's size histogram (a JSON file). The steps advance
-    the batch's state: tune on a throwaway start."""
+@dataclass
+class Tuning:
+    """How a batch launches its kernels: the cell count below which cases
+    take the small stencil tile, the persistent launch's tile (explicit
+    FP32 batches; 0 = one launch per stage) and the half-sweeps per
+    relaxation launch (implicit)."""
+    small_below: int = 0
+    persistent: int = 0
+    relax_fusion: int = 3
+
+    def apply(self, batch: GpuBatch2D):
+        batch.setTiling(self.small_below)
+        if batch.precision == "fp32" or self.persistent == 0:
+            batch.setPersistent(self.persistent)
+        batch.setRelaxFusion(self.relax_fusion)
+
+
+def autotune(batch: GpuBatch2D, steps: int = 24, implicit: bool = False, cache: Optional[str] = None,
+             thresholds: Sequence[int] = (0, 2048, 8192, 32768)) -> Tuning:
+    """Time ``steps`` steps of the batch under each launch configuration
+    and keep the fastest, cached as JSON by the batch's size histogram,
+    precision and step kind. Explicit batches: the small-tile threshold,
+    then (FP32) persistent launches on either tile against the per-stage
+    launches. Implicit batches: the threshold, then the half-sweeps per
+    relaxation launch. The steps advance the batch's state: tune on a
+    throwaway start."""
     shapes = sorted(batch.shapes)
     hist = np.histogram([a * b for a, b in shapes], bins=[0, 1024, 2048, 4096, 8192, 16384, 32768, 65536, 1 << 40])[0]
-    key = f"{'implicit' if implicit else 'explicit'}:{','.join(map(str, hist.tolist()))}"
+    key = (f"v2:{batch.precision}:{'implicit' if implicit else 'explicit'}:"
+           f"{','.join(map(str, hist.tolist()))}")
     path = _cache_path(cache)
     table = {}
     if os.path.exists(path):
@@ -360,21 +402,43 @@ def autotune_tiling(batch: GpuBatch2D, steps: int = 24, candidates: Sequence[int
                 table = json.load(f)
         except (OSError, ValueError):
             table = {}
-    if key in table:
-        batch.setTiling(int(table[key]))
-        return int(table[key])
-    best, best_t = 0, math.inf
-    for cand in candidates:
-        batch.setTiling(cand)
+    if isinstance(table.get(key), dict):
+        t = Tuning(**table[key])
+        t.apply(batch)
+        return t
+
+    def timed(t: Tuning) -> float:
+        t.apply(batch)
         run = (lambda k: batch.iterateImplicit(k, 10.0, 2)) if implicit else batch.iterate
         run(12)  # capture the graphs outside the timing
         t0 = time.perf_counter()
         run(steps)
-        t = time.perf_counter() - t0
-        if t < best_t:
-            best, best_t = cand, t
-    batch.setTiling(best)
-    table[key] = best
+        return time.perf_counter() - t0
+
+    best, best_t = Tuning(), math.inf
+    for cand in thresholds:
+        t = Tuning(small_below=cand)
+        dt = timed(t)
+        if dt < best_t:
+            best, best_t = t, dt
+    if implicit:
+        for nh in (2, 1):
+            t = Tuning(small_below=best.small_below, relax_fusion=nh)
+            dt = timed(t)
+            if dt < best_t:
+                best, best_t = t, dt
+    elif batch.precision == "fp32":
+        for tile in (1, 2):
+            t = Tuning(small_below=best.small_below, persistent=tile)
+            t.apply(batch)
+            if batch.persistentInfo()[1] == 0:
+                continue  # the active cases do not fit on the chip at once
+            dt = timed(t)
+            if dt < best_t:
+                best, best_t = t, dt
+    best.apply(batch)
+    table[key] = {"small_below": best.small_below, "persistent": best.persistent,
+                  "relax_fusion": best.relax_fusion}
     try:
         os.makedirs(os.path.dirname(path), exist_ok=True)
         with open(path, "w") as f:

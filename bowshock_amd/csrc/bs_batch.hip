@@ -22,6 +22,7 @@
 // as HIP graphs; the work lists and graphs are rebuilt when the active set
 // or the tiling changes.
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -29,6 +30,7 @@
 #include <vector>
 
 #include "bs_batch_tiles.hpp"
+#include "bs_persistent.hpp"
 
 using namespace bs;
 
@@ -122,6 +124,7 @@ struct bs_batch {
   WorkItem* work_wall = nullptr;
   int n_tiles[2] = {0, 0}, n_relax = 0, n_wall = 0;
   bool relax_linear = false;  // BS_RELAX_LINEARIZED (always on an FP64 batch)
+  int relax_fusion = 3;       // half-sweeps per relaxation launch at most (bs_batch_set_relax_fusion)
   int *seg_first = nullptr, *seg_count = nullptr, *seg_cells = nullptr, *seg_cases = nullptr;
   int *wall_first = nullptr, *wall_count = nullptr;
   int n_seg = 0;
@@ -132,6 +135,20 @@ struct bs_batch {
   double upd_cfl_max = -1, upd_growth = -1;
   int cur = 0;
   StepGraph explicit_graph, implicit_graph;
+  // bs_batch_set_persistent: explicit steps in persistent launches of up to
+  // PERSIST_STEPS steps (bs_persistent.hpp) when the active cases' tiles fit
+  // on the chip at once; 0 off, 1 32 x 4 tiles, 2 32 x 8.
+  int persist_tile = 0;
+  bool persist_dirty = true;
+  int persist_blocks = 0;      // blocks of a launch; 0: the active set does not fit
+  int persist_stride = 0;      // the longest XCD list
+  int persist_capacity = 0;    // co-resident blocks per XCD
+  int persist_used = 0;        // the last bs_batch_iterate ran persistent launches
+  PersistItem* persist_items = nullptr;  // 8 lists of persist_stride tiles
+  int* persist_lens = nullptr;           // tiles per list
+  int* persist_claims = nullptr;         // per XCD: blocks that have taken a tile
+  int* persist_bars = nullptr;           // one barrier counter per case
+  unsigned* persist_fault = nullptr;
   // bs_batch_enable_timing: HIP-event pairs around each stencil and
   // relaxation launch (kind 0 stage, 1 prepare, 2 relaxation); the steps
   // then run without graphs.
@@ -139,6 +156,7 @@ struct bs_batch {
     bool enabled = false;
     std::vector<hipEvent_t> pool;
     std::vector<int> kinds;
+    std::vector<int> weights;  // passes per pair (a persistent launch holds many)
     int used = 0;
     double total_ms[3] = {0, 0, 0};
     long long launches[3] = {0, 0, 0};
@@ -221,7 +239,8 @@ void batch_free(bs_batch* b) {
                  b->wall_partials, b->sums, b->control, b->bnd, b->wall_bnd, b->tile_args, b->relax_args,
                  b->update_args, b->wall_args, b->work_tiles[0], b->work_tiles[1], b->work_relax,
                  b->work_wall, b->seg_first, b->seg_count, b->seg_cells, b->seg_cases, b->wall_first,
-                 b->wall_count};
+                 b->wall_count, b->persist_items, b->persist_lens, b->persist_claims, b->persist_bars,
+                 b->persist_fault};
   for (void* p : all)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : b->timing.pool) (void)hipEventDestroy(e);
@@ -232,7 +251,7 @@ void batch_free(bs_batch* b) {
 
 // Index of a new event pair of `kind` with its start recorded, or -1 when
 // timing is off (or no event could be made).
-int timing_begin(bs_batch* b, int kind) {
+int timing_begin(bs_batch* b, int kind, int weight = 1) {
   auto& t = b->timing;
   if (!t.enabled || kind < 0) return -1;
   if ((size_t)(2 * t.used + 2) > t.pool.size()) {
@@ -243,7 +262,9 @@ int timing_begin(bs_batch* b, int kind) {
     }
   }
   t.kinds.resize(t.used + 1);
+  t.weights.resize(t.used + 1);
   t.kinds[t.used] = kind;
+  t.weights[t.used] = weight;
   (void)hipEventRecord(t.pool[2 * t.used], b->stream);
   return t.used++;
 }
@@ -259,7 +280,7 @@ void timing_collect(bs_batch* b) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, t.pool[2 * k], t.pool[2 * k + 1]) == hipSuccess) {
       t.total_ms[t.kinds[k]] += ms;
-      t.launches[t.kinds[k]] += 1;
+      t.launches[t.kinds[k]] += t.weights[k];
     }
   }
   t.used = 0;
@@ -432,8 +453,108 @@ bs_status batch_build_lists(bs_batch* b) {
   b->n_wall = (int)wall.size();
   b->n_seg = (int)seg_cases.size();
   b->lists_dirty = false;
+  b->persist_dirty = true;
   drop_graph(b->explicit_graph);
   drop_graph(b->implicit_graph);
+  return BS_OK;
+}
+
+// The persistent launch's tile lists for the active cases: every case's
+// tiles on one XCD (cases dealt largest first to the XCD with the fewest
+// tiles). A launch takes 8 x (longest list) + 64 blocks, which the
+// dispatcher's round-robin spreads evenly over the XCDs. persist_blocks
+// stays 0 when an XCD would get more blocks than it holds at once (the
+// launch would then wait on blocks that cannot start).
+bs_status batch_build_persistent(bs_batch* b) {
+  b->persist_dirty = false;
+  b->persist_blocks = 0;
+  int tx, ty;
+  persistent_tile_shape(b->persist_tile, &tx, &ty);
+  if (b->persist_capacity == 0) {
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, b->device), "device properties");
+    int per_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, reinterpret_cast<const void*>(batch_persistent_fn(b->scheme, b->persist_tile)), tx * ty, 0),
+            "occupancy");
+    b->persist_capacity = prop.multiProcessorCount % PERSIST_XCDS == 0
+                              ? prop.multiProcessorCount / PERSIST_XCDS * per_cu : 0;
+  }
+  std::vector<int> order;
+  for (int c = 0; c < b->count; c++)
+    if (b->active[c]) order.push_back(c);
+  auto tiles = [&](int c) {
+    const BatchCase& k = b->cases[c];
+    return ((k.ni + tx - 1) / tx) * ((k.nj + ty - 1) / ty);
+  };
+  std::stable_sort(order.begin(), order.end(), [&](int p, int q) { return tiles(p) > tiles(q); });
+  std::vector<std::vector<PersistItem>> lists(PERSIST_XCDS);
+  for (int c : order) {
+    size_t x = 0;
+    for (size_t y = 1; y < lists.size(); y++)
+      if (lists[y].size() < lists[x].size()) x = y;
+    const BatchCase& k = b->cases[c];
+    const int nx = (k.ni + tx - 1) / tx, ny = (k.nj + ty - 1) / ty;
+    for (int by = 0; by < ny; by++)
+      for (int bx = 0; bx < nx; bx++) lists[x].push_back({c, bx, by, nx * ny});
+  }
+  size_t longest = 0;
+  for (auto& l : lists) longest = l.size() > longest ? l.size() : longest;
+  constexpr int SPARE = 64;  // blocks beyond 8 x longest
+  if (longest == 0 || (int)longest + SPARE / PERSIST_XCDS > b->persist_capacity) return BS_OK;
+  std::vector<PersistItem> items(PERSIST_XCDS * longest, PersistItem{-1, 0, 0, 0});
+  std::vector<int> lens(PERSIST_XCDS);
+  for (int x = 0; x < PERSIST_XCDS; x++) {
+    lens[x] = (int)lists[x].size();
+    for (size_t k = 0; k < lists[x].size(); k++) items[x * longest + k] = lists[x][k];
+  }
+  bs_status st;
+  if ((st = upload_vec(&b->persist_items, items, "persistent tiles")) ||
+      (st = upload_vec(&b->persist_lens, lens, "persistent tiles")))
+    return st;
+  b->persist_stride = (int)longest;
+  b->persist_blocks = PERSIST_XCDS * (int)longest + SPARE;
+  return BS_OK;
+}
+
+constexpr int PERSIST_STEPS = 60;        // steps per persistent launch
+constexpr int PERSIST_SPIN = 1 << 16;    // polls before a barrier gives up
+
+// `steps` explicit steps in persistent launches; the fault word is read
+// after each launch.
+bs_status batch_persistent_run(bs_batch* b, int steps) {
+  const float *ca, *cb;
+  const int ns = rk_table(b->scheme.integrator, &ca, &cb);
+  const PersistFn fn = batch_persistent_fn(b->scheme, b->persist_tile);
+  int tx, ty;
+  persistent_tile_shape(b->persist_tile, &tx, &ty);
+  const TileArgsT<float>* recs = static_cast<const TileArgsT<float>*>(b->tile_args) + (size_t)ROLE_STAGE0 * b->count;
+  for (int done = 0; done < steps;) {
+    const int n = steps - done < PERSIST_STEPS ? steps - done : PERSIST_STEPS;
+    const int cur0 = ns == 1 ? (b->cur + done) % 3 : b->cur;
+    hipLaunchKernelGGL(clear_counters, dim3((b->count + 255) / 256), dim3(256), 0, b->stream, b->persist_bars,
+                       b->count, b->persist_claims, b->persist_fault);
+    const int pair = timing_begin(b, 0, n * ns);
+    hipLaunchKernelGGL(fn, dim3(b->persist_blocks), dim3(tx * ty), 0, b->stream, recs, b->count, b->persist_items,
+                       b->persist_lens, b->persist_claims, b->persist_stride, n, ns, cur0, b->persist_bars,
+                       b->persist_fault, PERSIST_SPIN);
+    timing_end(b, pair);
+    HIP_TRY(hipGetLastError(), "persistent launch");
+    unsigned fault = 0;
+    HIP_TRY(hipMemcpyAsync(&fault, b->persist_fault, sizeof fault, hipMemcpyDeviceToHost, b->stream),
+            "persistent fault word");
+    HIP_TRY(hipStreamSynchronize(b->stream), "persistent launch");
+    if (fault) {
+      b->persist_tile = 0;
+      return fail(BS_ERR_EXECUTION,
+                  "persistent launch abandoned (%s); persistent launches are off for this batch and the states "
+                  "of its cases are indeterminate",
+                  fault & PERSIST_MISPLACED ? "a block read an XCD number out of range" : "a barrier timed out");
+    }
+    done += n;
+  }
+  if (ns == 1) b->cur = (b->cur + steps) % 3;
+  timing_collect(b);
   return BS_OK;
 }
 
@@ -493,7 +614,7 @@ bs_status batch_encode_implicit(bs_batch* b, int sweeps) {
   kernels[1][2] = batch_relax_tile_kernel<T, 3, true>;
   const int form = (b->relax_linear || !std::is_same<T, float>::value) ? 1 : 0;
   for (int done = 0, colour = 1, halves = 2 * sweeps - 1; done < halves && b->n_relax;) {
-    const int nh = halves - done < 3 ? halves - done : 3;
+    const int nh = halves - done < b->relax_fusion ? halves - done : b->relax_fusion;
     const int init = done == 0;
     const UpdateArgsT<T>* u = done + nh == halves ? upd : nullptr;
     const int pair = timing_begin(b, 2);
@@ -1147,11 +1268,59 @@ bs_status bs_batch_set_tiling(bs_batch* b, int32_t small_tile_below_cells) {
   return BS_OK;
 }
 
+bs_status bs_batch_set_relax_fusion(bs_batch* b, int32_t half_sweeps) {
+  BATCH_GUARD(b);
+  if (half_sweeps < 1 || half_sweeps > 3) return fail(BS_ERR_RANGE, "half-sweeps per launch: 1, 2 or 3");
+  if (half_sweeps != b->relax_fusion) drop_graph(b->implicit_graph);
+  b->relax_fusion = half_sweeps;
+  return BS_OK;
+}
+
+bs_status bs_batch_set_persistent(bs_batch* b, int32_t tile) {
+  BATCH_GUARD(b);
+  if (tile < 0 || tile > 2) return fail(BS_ERR_RANGE, "persistent tile: 0 (off), 1 (32 x 4) or 2 (32 x 8)");
+  if (tile != 0 && b->fp64) return fail(BS_ERR_RANGE, "persistent launches run FP32 batches");
+  if (tile != b->persist_tile) {
+    b->persist_tile = tile;
+    b->persist_dirty = true;
+    b->persist_capacity = 0;
+  }
+  if (tile != 0 && b->persist_fault == nullptr) {
+    bs_status st;
+    if ((st = dev_alloc(&b->persist_bars, (size_t)b->count * PERSIST_BAR_STRIDE, "barrier counters")) ||
+        (st = dev_alloc(&b->persist_claims, (size_t)PERSIST_XCDS, "claim counters")) ||
+        (st = dev_alloc(&b->persist_fault, (size_t)1, "fault word")))
+      return st;
+  }
+  return BS_OK;
+}
+
+bs_status bs_batch_persistent_info(bs_batch* b, int32_t* used, int32_t* blocks, int32_t* capacity) {
+  BATCH_GUARD(b);
+  bs_status st;
+  if (b->persist_tile && !b->fp64) {
+    if (b->lists_dirty && (st = batch_build_lists(b))) return st;
+    if (b->persist_dirty && (st = batch_build_persistent(b))) return st;
+  }
+  if (used) *used = b->persist_used;
+  if (blocks) *blocks = b->persist_tile ? b->persist_blocks : 0;
+  if (capacity) *capacity = b->persist_tile ? b->persist_capacity * PERSIST_XCDS : 0;
+  return BS_OK;
+}
+
 bs_status bs_batch_iterate(bs_batch* b, int32_t steps) {
   BATCH_GUARD(b);
   if (steps < 0) return fail(BS_ERR_RANGE, "the step count cannot be negative");
   bs_status st;
   if (b->lists_dirty && (st = batch_build_lists(b))) return st;
+  b->persist_used = 0;
+  if (b->persist_tile && !b->fp64) {
+    if (b->persist_dirty && (st = batch_build_persistent(b))) return st;
+    if (b->persist_blocks > 0) {
+      b->persist_used = 1;
+      return batch_persistent_run(b, steps);
+    }
+  }
   if (b->fp64) return batch_run(b, b->explicit_graph, steps, [b]() { return batch_encode_explicit<double>(b); });
   return batch_run(b, b->explicit_graph, steps, [b]() { return batch_encode_explicit<float>(b); });
 }

@@ -581,7 +581,10 @@ static bs_status solver_create(bs_ctx* ctx, const bs_config* c, bool fp64, bs_so
                       "boundary table")) ||
       (st = dev_alloc(&s->d.wall_out, 9 * (size_t)(ni > nj ? ni : nj), "wall fluxes")))
     return bail(st);
-  HIP_TRY(hipMemset(s->d.control, 0, sizeof(Control)), "control reset");
+  // On the solver's own (non-blocking) stream: a memset on the null stream
+  // is not ordered with its work.
+  HIP_TRY(hipMemsetAsync(s->d.control, 0, sizeof(Control), s->stream), "control reset");
+  HIP_TRY(hipStreamSynchronize(s->stream), "control reset");
   // Per-side FP64 face geometry: midpoint, normal towards increasing
   // index, area (euler2d.ts:1430-1469 boundaryFace).
   for (int side = 0; side < 4; side++) {
@@ -1483,7 +1486,8 @@ bs_status bs_set_fitted_shock(bs_solver* s, const double* free_stream) {
     if (!s->d.shock_out) {
       bs_status st = dev_alloc(&s->d.shock_out, 3 * (size_t)s->ni, "shock");
       if (st) return st;
-      HIP_TRY(hipMemset(s->d.shock_out, 0, 3 * (size_t)s->ni * sizeof(double)), "shock");
+      HIP_TRY(hipMemsetAsync(s->d.shock_out, 0, 3 * (size_t)s->ni * sizeof(double), s->stream), "shock");
+      HIP_TRY(hipStreamSynchronize(s->stream), "shock");
     }
     for (int k = 0; k < 4; k++) s->shock_free[k] = free_stream[k];
   }

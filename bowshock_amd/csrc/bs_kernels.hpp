@@ -60,18 +60,38 @@ struct StateDevT {
 };
 using StateDev = StateDevT<float>;
 
+// Element idx of a state array: a plain load, or (COH) a device-scope
+// relaxed atomic load, which reads the L2 and never a line the CU's L1
+// still holds from before another block's store.
+#ifndef BS_COH_NT
+#define BS_COH_NT 0
+#endif
+template <bool COH, typename T>
+__device__ __forceinline__ T ld(const T* base, int idx) {
+  if constexpr (COH) {
+    if constexpr (BS_COH_NT) return __builtin_nontemporal_load(base + idx);
+    else return __hip_atomic_load(const_cast<T*>(base) + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    return base[idx];
+  }
+}
+
 // The conserved state of cell (i, j), or of a ghost cell up to two layers
 // outside one side (never a corner): euler2d.wgsl load_cell, euler2d.ts
 // fillGhosts 1361-1403. Ghost layer g mirrors interior layer g (outflow,
 // wall with the momentum reflected about the boundary face normal), wraps
 // (periodic), or holds the inflow / face / far-field state.
-template <typename T>
+//
+// COH: the state is read with device-scope loads, which bypass the CU's L1
+// cache (the persistent batch kernel, whose blocks read states that other
+// blocks wrote during the same launch).
+template <typename T, bool COH = false>
 __device__ __forceinline__ Vec4<T> load_cell(const StateDevT<T>& s, const GridDevT<T>& G, const BoundaryDevT<T>& B,
                                              int i, int j, T g) {
   const int ni = G.ni, nj = G.nj;
   if (i >= 0 && i < ni && j >= 0 && j < nj) {
     int c = i + j * ni;
-    return {s.q[0][c], s.q[1][c], s.q[2][c], s.q[3][c]};
+    return {ld<COH>(s.q[0], c), ld<COH>(s.q[1], c), ld<COH>(s.q[2], c), ld<COH>(s.q[3], c)};
   }
   int side, t, mi, mj, wi, wj;
   T nx, ny;
@@ -104,13 +124,13 @@ __device__ __forceinline__ Vec4<T> load_cell(const StateDevT<T>& s, const GridDe
     int ii = side == IMIN ? 0 : side == IMAX ? ni - 1 : i;
     int jj = side == JMIN ? 0 : side == JMAX ? nj - 1 : j;
     int c = ii + jj * ni;
-    Prim<T> in = to_prim(s.q[0][c], s.q[1][c], s.q[2][c], s.q[3][c], g);
+    Prim<T> in = to_prim(ld<COH>(s.q[0], c), ld<COH>(s.q[1], c), ld<COH>(s.q[2], c), ld<COH>(s.q[3], c), g);
     Prim<T> fr = {B.state[side][0], B.state[side][1], B.state[side][2], B.state[side][3]};
     T o = (side == IMIN || side == JMIN) ? -T(1.) : T(1.);
     return to_cons(far_field_state(in, fr, o * nx, o * ny, g), g);
   }
   int c = kind == PERIODIC ? wi + wj * ni : mi + mj * ni;
-  Vec4<T> v = {s.q[0][c], s.q[1][c], s.q[2][c], s.q[3][c]};
+  Vec4<T> v = {ld<COH>(s.q[0], c), ld<COH>(s.q[1], c), ld<COH>(s.q[2], c), ld<COH>(s.q[3], c)};
   if (kind == WALL) {
     T mn = v.y * nx + v.z * ny;
     v.y -= T(2.) * mn * nx;
@@ -267,7 +287,7 @@ __device__ __forceinline__ void halo_cell(int h, int& wi, int& wj) {
 
 // Loads a tile's TileInputs; EDGE: also the high-edge face metrics
 // (otherwise the edge faces load them when they are computed).
-template <typename T, int TX, int TY, int MODE, bool EDGE>
+template <typename T, int TX, int TY, int MODE, bool EDGE, bool COH = false>
 __device__ __forceinline__ void tile_inputs(const TileArgsT<T>& A, int bx, int by, TileInputs<T>& in) {
   constexpr int NT = TX * TY;
   const GridDevT<T>& G = A.G;
@@ -322,12 +342,21 @@ __device__ __forceinline__ void tile_inputs(const TileArgsT<T>& A, int bx, int b
   }
   if (tile_is_inner<TX, TY>(ni, nj, i0, j0)) {
     const int c = gi + gj * ni;
-    in.own = {at(A.src.q[0], c), at(A.src.q[1], c), at(A.src.q[2], c), at(A.src.q[3], c)};
+    if constexpr (COH) {
+      in.own = {ld<true>(A.src.q[0], c), ld<true>(A.src.q[1], c), ld<true>(A.src.q[2], c), ld<true>(A.src.q[3], c)};
+    } else {
+      in.own = {at(A.src.q[0], c), at(A.src.q[1], c), at(A.src.q[2], c), at(A.src.q[3], c)};
+    }
     if (4 * (TX + TY) <= NT && tid < 4 * (TX + TY)) {
       int wi, wj;
       halo_cell<TX, TY>(tid, wi, wj);
       const int h = (i0 + wi - 2) + (j0 + wj - 2) * ni;
-      in.halo = {at(A.src.q[0], h), at(A.src.q[1], h), at(A.src.q[2], h), at(A.src.q[3], h)};
+      if constexpr (COH) {
+        in.halo = {ld<true>(A.src.q[0], h), ld<true>(A.src.q[1], h), ld<true>(A.src.q[2], h),
+                   ld<true>(A.src.q[3], h)};
+      } else {
+        in.halo = {at(A.src.q[0], h), at(A.src.q[1], h), at(A.src.q[2], h), at(A.src.q[3], h)};
+      }
     }
   }
 }
@@ -335,7 +364,7 @@ __device__ __forceinline__ void tile_inputs(const TileArgsT<T>& A, int bx, int b
 // The tile (bx, by) of one grid from its inputs; its FP64 partial goes to
 // A.partials[slot]. EDGE: the high-edge faces' metrics come with the
 // inputs (otherwise those faces load them).
-template <typename T, int TX, int TY, int REC, int LIM, int FLUX, int MODE, bool EDGE>
+template <typename T, int TX, int TY, int REC, int LIM, int FLUX, int MODE, bool EDGE, bool COH = false>
 __device__ __forceinline__ void tile_compute(const TileArgsT<T>& A, int bx, int by, int slot, const TileInputs<T>& in) {
   constexpr int NT = TX * TY;
   constexpr int WX = TX + 4, WY = TY + 4;
@@ -378,7 +407,7 @@ __device__ __forceinline__ void tile_compute(const TileArgsT<T>& A, int bx, int 
   // ghost cell (gi in [ni, ni + 1] or gj in [nj, nj + 1]) that the faces
   // near the edge still need.
   if (!(gi >= ni && gj >= nj) && gi <= ni + 1 && gj <= nj + 1) {
-    const Vec4<T> q = inner ? in.own : load_cell(A.src, G, *A.B, gi, gj, g);
+    const Vec4<T> q = inner ? in.own : load_cell<T, COH>(A.src, G, *A.B, gi, gj, g);
     if (valid) own = q;
     Prim<T> w = to_prim(q.x, q.y, q.z, q.w, g);
     sp[0][lj + 2][li + 2] = w.rho;
@@ -396,9 +425,9 @@ __device__ __forceinline__ void tile_compute(const TileArgsT<T>& A, int bx, int 
     if (inner && 4 * (TX + TY) <= NT) q = in.halo;
     else if (inner) {
       const int c = hi + hj * ni;
-      q = {A.src.q[0][c], A.src.q[1][c], A.src.q[2][c], A.src.q[3][c]};
+      q = {ld<COH>(A.src.q[0], c), ld<COH>(A.src.q[1], c), ld<COH>(A.src.q[2], c), ld<COH>(A.src.q[3], c)};
     } else {
-      q = load_cell(A.src, G, *A.B, hi, hj, g);
+      q = load_cell<T, COH>(A.src, G, *A.B, hi, hj, g);
     }
     Prim<T> w = to_prim(q.x, q.y, q.z, q.w, g);
     sp[0][wj][wi] = w.rho;
@@ -598,14 +627,14 @@ __device__ __forceinline__ void tile_compute(const TileArgsT<T>& A, int bx, int 
   if (A.count_step && bx == 0 && by == 0 && tid == 0) atomicAdd((unsigned long long*)&A.control->steps, 1ull);
 }
 
-template <typename T, int TX, int TY, int REC, int LIM, int FLUX, int MODE>
+template <typename T, int TX, int TY, int REC, int LIM, int FLUX, int MODE, bool COH = false>
 __device__ __forceinline__ void tile_body(const TileArgsT<T>& A, int bx, int by, int slot) {
   // The FP32 tiles load the edge faces' metrics with the other inputs; the
   // FP64 tiles, short of registers, when the edge faces are computed.
   constexpr bool EDGE = std::is_same<T, float>::value;
   TileInputs<T> in;
-  tile_inputs<T, TX, TY, MODE, EDGE>(A, bx, by, in);
-  tile_compute<T, TX, TY, REC, LIM, FLUX, MODE, EDGE>(A, bx, by, slot, in);
+  tile_inputs<T, TX, TY, MODE, EDGE, COH>(A, bx, by, in);
+  tile_compute<T, TX, TY, REC, LIM, FLUX, MODE, EDGE, COH>(A, bx, by, slot, in);
 }
 
 // Workgroups go to the 8 XCDs round-robin by their linear index, each XCD

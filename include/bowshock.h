@@ -394,6 +394,26 @@ bs_status bs_batch_set_active(bs_batch *b, const uint8_t *active, int32_t count)
  * on 32 x 8 tiles (64 x 4 on an FP64 batch) (set by the autotuner; 0 = all
  * on the large tile). */
 bs_status bs_batch_set_tiling(bs_batch *b, int32_t small_tile_below_cells);
+/* No counterpart in the reference (which solves its cases one after
+ * another). tile 1 (32 x 4) or 2 (32 x 8): bs_batch_iterate on an FP32
+ * batch runs its explicit steps in persistent launches of up to 60 steps,
+ * every stage of every active case in one launch, when the active cases'
+ * tiles fit on the chip at once (each case's tiles on one XCD); otherwise,
+ * and with tile 0 (the default), one launch per stage. The states are
+ * bitwise those of the per-stage launches. A launch whose device-side wait
+ * does not complete returns BS_ERR_EXECUTION and turns persistent launches
+ * off for the batch. FP64 batches refuse a non-zero tile. */
+bs_status bs_batch_set_persistent(bs_batch *b, int32_t tile);
+/* No counterpart in the reference. At most this many red-black half-sweeps
+ * (1, 2 or 3, the default) per relaxation launch of bs_batch_iterate_implicit:
+ * fused half-sweeps share one staging of the tile and its halo, and
+ * recompute a halo that shrinks by one cell per half-sweep. The states are
+ * bitwise the same for every setting (set by the autotuner). */
+bs_status bs_batch_set_relax_fusion(bs_batch *b, int32_t half_sweeps);
+/* Whether the last bs_batch_iterate ran persistent launches, the blocks a
+ * launch over the current active set takes (0: it does not fit, or
+ * persistent launches are off) and the blocks the chip holds at once. */
+bs_status bs_batch_persistent_info(bs_batch *b, int32_t *used, int32_t *blocks, int32_t *capacity);
 /* bs_set_relaxation for every case of the batch (default
  * BS_RELAX_FLUX_DIFFERENCE on an FP32 batch; an FP64 batch takes
  * BS_RELAX_LINEARIZED only). */

@@ -310,3 +310,124 @@ def test_fp64_batch_converges_below_the_fp32_floor(ctx):
     res = solve(batch, p, q, 0.5, plan)
     for k, r in enumerate(res):
         assert r.converged and r.residual <= 1e-6, (k, r.reason, r.residual, r.steps)
+
+
+@pytest.mark.parametrize("tile", [1, 2])
+@pytest.mark.parametrize("integrator", ["ssprk3", "ssprk2", "forwardEuler"])
+def test_persistent_launches_equal_per_stage_launches(ctx, tile, integrator):
+    """Persistent launches (every stage of every active case in one launch,
+    a case's blocks meeting at device-side barriers between stages) give
+    the states of one launch per stage bit for bit: H and C grids, runs of
+    steps that cross the 60-step launch boundary, a case retired between
+    runs (its state left as it was), and the same status words and step
+    counts."""
+    cs = _mixed_cases(timeIntegrator=integrator)
+    ref = GpuBatch2D(ctx, [c[0] for c in cs])
+    per = GpuBatch2D(ctx, [c[0] for c in cs])
+    for b in (ref, per):
+        b.setUniform([c[1] for c in cs])
+    per.setPersistent(tile)
+    for count in (37, 71):
+        ref.iterate(count)
+        per.iterate(count)
+        used, blocks, capacity = per.persistentInfo()
+        assert used and 0 < blocks <= capacity, (used, blocks, capacity)
+        for k in range(len(cs)):
+            assert np.array_equal(_state(per.readState(k)), _state(ref.readState(k))), (count, k)
+    mask = [1, 0, 1, 1]
+    for b in (ref, per):
+        b.setActive(mask)
+        b.iterate(25)
+    assert per.persistentInfo()[0]
+    for k in range(len(cs)):
+        assert np.array_equal(_state(per.readState(k)), _state(ref.readState(k))), ("retired", k)
+    s_ref, n_ref = ref.status()
+    s_per, n_per = per.status()
+    assert s_per.tolist() == s_ref.tolist() == [0] * len(cs)
+    assert n_per.tolist() == n_ref.tolist() == [133, 108, 133, 133]
+    ref.destroy()
+    per.destroy()
+
+
+def test_persistent_launches_fall_back_when_the_cases_do_not_fit(ctx):
+    """A case of 1280 x 640 cells on 32 x 4 tiles needs more blocks than the
+    chip holds at once: the batch runs its per-stage launches instead, with
+    the same states."""
+    grid, free, _, _, bcs = cases.cone_case(1280, 640, mach=3.0)
+    cfg = Euler2DConfig(grid=grid, boundaries=bcs, geometry="axisymmetric", gamma=cases.GAMMA, flux="hllc",
+                        limiter="minmod", reconstruction="perVariable", timeIntegrator="ssprk3", cfl=0.5,
+                        timeStepping="local")
+    ref = GpuBatch2D(ctx, [cfg])
+    per = GpuBatch2D(ctx, [cfg])
+    for b in (ref, per):
+        b.setUniform([free])
+    per.setPersistent(1)
+    used, blocks, capacity = per.persistentInfo()
+    assert blocks == 0 and capacity > 0
+    ref.iterate(6)
+    per.iterate(6)
+    assert not per.persistentInfo()[0]
+    assert np.array_equal(_state(per.readState(0)), _state(ref.readState(0)))
+    ref.destroy()
+    per.destroy()
+
+
+def test_fp64_batches_refuse_persistent_launches(ctx):
+    cs = _cases()
+    b = GpuBatch2D(ctx, [c[0] for c in cs], "fp64")
+    with pytest.raises(RangeError):
+        b.setPersistent(1)
+    b.destroy()
+
+
+@pytest.mark.parametrize("fusion,sweeps", [(1, 2), (2, 2), (1, 3), (2, 3)])
+def test_relaxation_fusion_keeps_the_states(ctx, fusion, sweeps):
+    """Fewer red-black half-sweeps per relaxation launch (more launches)
+    give the states of the default three per launch bit for bit."""
+    cs = _mixed_cases(limiter="vanAlbada", reconstruction="sharedScaled")
+    ref = GpuBatch2D(ctx, [c[0] for c in cs])
+    fused = GpuBatch2D(ctx, [c[0] for c in cs])
+    fused.setRelaxFusion(fusion)
+    for b in (ref, fused):
+        b.setUniform([c[1] for c in cs])
+        b.setImplicitCfl(1.0)
+        b.iterateImplicit(13, 10.0, sweeps)
+    for k in range(len(cs)):
+        assert np.array_equal(_state(fused.readState(k)), _state(ref.readState(k))), k
+    with pytest.raises(RangeError):
+        fused.setRelaxFusion(4)
+    ref.destroy()
+    fused.destroy()
+
+
+@pytest.mark.parametrize("implicit", [False, True])
+def test_the_autotuner_picks_a_launch_plan_with_the_same_states(ctx, tmp_path, implicit):
+    """autotune times the launch plans on a throwaway batch, caches the
+    fastest by the batch's size histogram, and a batch run under the plan
+    (read back from the cache) gives the states of the default plan."""
+    from bowshock_amd.batch import Tuning, autotune
+
+    scheme = dict(limiter="vanAlbada", reconstruction="sharedScaled") if implicit else {}
+    cs = _mixed_cases(**scheme)
+    cache = str(tmp_path / "tiling.json")
+    probe = GpuBatch2D(ctx, [c[0] for c in cs])
+    probe.setUniform([c[1] for c in cs])
+    if implicit:
+        probe.setImplicitCfl(1.0)
+    t = autotune(probe, steps=12, implicit=implicit, cache=cache)
+    assert isinstance(t, Tuning) and t.relax_fusion in (1, 2, 3) and t.persistent in (0, 1, 2)
+    probe.destroy()
+    tuned = GpuBatch2D(ctx, [c[0] for c in cs])
+    assert autotune(tuned, implicit=implicit, cache=cache) == t  # from the cache
+    ref = GpuBatch2D(ctx, [c[0] for c in cs])
+    for b in (ref, tuned):
+        b.setUniform([c[1] for c in cs])
+        if implicit:
+            b.setImplicitCfl(1.0)
+            b.iterateImplicit(20, 10.0, 2)
+        else:
+            b.iterate(40)
+    for k in range(len(cs)):
+        assert np.array_equal(_state(tuned.readState(k)), _state(ref.readState(k))), k
+    ref.destroy()
+    tuned.destroy()

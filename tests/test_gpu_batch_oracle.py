@@ -83,16 +83,21 @@ def _batch(ctx, accepted, scheme, precision="fp32"):
     return b
 
 
-@pytest.mark.parametrize("precision", ["fp32", "fp64"])
-def test_the_mach_sweep_as_one_batch_against_the_oracle(ctx, precision):
-    """FP64 batches to the FP64 parity bar (L1 1e-12, Linf 1e-10)."""
+@pytest.mark.parametrize("precision,persistent", [("fp32", 0), ("fp64", 0), ("fp32", 1), ("fp32", 2)])
+def test_the_mach_sweep_as_one_batch_against_the_oracle(ctx, precision, persistent):
+    """FP64 batches to the FP64 parity bar (L1 1e-12, Linf 1e-10); FP32
+    batches also in persistent launches (both tile shapes)."""
     sweep = P.build_grids(P.sweep_cases(), workers=1)
     accepted = [c for c in sweep if c.grid is not None]
     assert len(sweep) == 39 and len(accepted) >= 30
     scheme = dict(flux="hllc", limiter="minmod", reconstruction="perVariable", timeIntegrator="ssprk3", cfl=0.5)
     batch = _batch(ctx, accepted, scheme, precision)
     batch.setTiling(8192)  # the sweep's grids take the small tile
+    if persistent:
+        batch.setPersistent(persistent)
     batch.iterate(30)
+    if persistent:
+        assert batch.persistentInfo()[0]
     _check(batch, accepted, scheme, 30, implicit=False, bars=(1e-12, 1e-10) if precision == "fp64" else (5e-6, 2e-4))
     batch.destroy()
 
