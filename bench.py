@@ -79,6 +79,13 @@ def parse():
                     help="warmstart: predicted cells within this of the free stream start as the free stream")
     ap.add_argument("--rockets", type=int, default=2000, help="population: rockets drawn (4 Mach numbers each)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="batched CPU baseline: seconds of oracle work")
+    ap.add_argument("--no-solves", action="store_true",
+                    help="single: skip the converged-solves leg (the seeded Mach sweep solved to convergence)")
+    ap.add_argument("--persistent", choices=("auto", "0", "1", "2"), default="auto",
+                    help="explicit batches: the autotuner's launch plan, or persistent launches on 32 x 4 (1) or "
+                         "32 x 8 (2) tiles, or per-stage launches (0)")
+    ap.add_argument("--no-fp64-sample", action="store_true",
+                    help="batched workloads: skip the FP32-vs-FP64 drag sample (FP64 single solves)")
     return ap.parse_args()
 
 
@@ -114,7 +121,8 @@ def main():
         sys.exit(2)
 
     if a.workload == "warmstart":
-        return run_warmstart(a)
+        print(json.dumps(run_warmstart(a)), flush=True)
+        return None
     if a.workload == "ladder":
         return run_ladder(a)
     if a.workload == "implicit":
@@ -199,6 +207,10 @@ def main():
                          f"FP64 C oracle (gcc -O3), {ce:.1f}s", "host": host_info()}
         print(f"cpu baseline {cpu['value']:.3e} CU/s ({ce:.1f}s)", file=sys.stderr, flush=True)
 
+    solves = None
+    if not a.no_solves:
+        solves = {"plain_sweep": converged_sweep(ctx), "warm_start_sample": warm_start_sample(a)}
+
     line = {
         "metric": "Euler cell-updates/s, %HBM roofline; converged CD solves/s warm-start vs plain",
         "value": value,
@@ -224,9 +236,73 @@ def main():
                      "us_per_launch": ms_launch * 1e3},
         "cpu_baseline": cpu,
         "drag": cd,
+        "converged_solves": solves,
     }
     print(json.dumps(line), flush=True)
     gpu.destroy()
+
+
+def converged_sweep(ctx):
+    """The metric's solves/s half on the batched Mach sweep (the seeded
+    nose at Mach 1.2 ... 5.0, SSP-RK3, FP32): one batch, launch plan from
+    the autotuner, solved from the free stream until every case has
+    converged (the SolvePlan stopping rule), wall time from the first
+    step to the last check."""
+    import numpy as np
+
+    from bowshock_amd import population as P
+
+    t0 = time.time()
+    cases = P.build_grids(P.sweep_cases(), workers=1)
+    S = _setup_cases(cases, implicit=False)
+    cells = np.array([c.cells for c in S["ok"]])
+    batch = _CaseGroups(ctx, S)
+    batch.setUniform(S["frees"])
+    tuned = [dataclasses.asdict(t) for t in batch.autotune(steps=24, implicit=False)]
+    batch.setUniform(S["frees"])
+    t = time.perf_counter()
+    results = batch.solve(S["p_inf"], S["q_inf"], S["plan"])
+    wall = time.perf_counter() - t
+    batch.destroy()
+    steps = np.array([r.steps for r in results])
+    converged = sum(r.converged for r in results)
+    cu = float((cells * steps).sum())
+    out = {"workload": "batched Mach sweep 1.2:0.1:5.0 of one seeded nose (mulberry32, seed 20261020), SSP-RK3 "
+                       "FP32, free-stream start, to the SolvePlan stopping rule",
+           "cases": len(cases), "accepted": len(S["ok"]), "converged": converged, "seconds": wall,
+           "solves_per_s": converged / wall, "cell_updates_per_s": cu / wall,
+           "hbm_roofline_frac": BYTES_PER_CU_RK3 * cu / wall / HBM_PEAK,
+           "median_steps": int(np.median(steps)), "launch_tuning": tuned, "setup_seconds": time.time() - t0 - wall}
+    print(f"converged sweep: {converged}/{len(S['ok'])} in {wall:.3f}s = {converged / wall:.1f} solves/s, "
+          f"{cu / wall:.3e} CU/s, tuning {tuned}", file=sys.stderr, flush=True)
+    return out
+
+
+WARM_SAMPLE_ROCKETS = 120
+WARM_SAMPLE_TRAIN_STEPS = 1500
+
+
+def warm_start_sample(a):
+    """The metric's warm-start half on a reduced population: the
+    warmstart workload at 120 rockets (480 cases; the network trained for
+    1,500 Adam steps on the converged fields of the training rockets), its
+    held-out quarter solved from the free stream and from the warm start.
+    The full population (2,000 rockets, 6,000 training fields) takes
+    minutes: bench.py --workload warmstart."""
+    b = argparse.Namespace(**vars(a))
+    b.rockets, b.train_steps, b.cpu_seconds, b.precision = WARM_SAMPLE_ROCKETS, WARM_SAMPLE_TRAIN_STEPS, 0.0, "fp32"
+    t0 = time.time()
+    line = run_warmstart(b)
+    c = line["config"]
+    out = {"workload": c["workload"], "held_out_cases": c["held_out_cases"], "training_fields": c["training_fields"],
+           "train_steps": c["train_steps"], "solves_per_s_plain": c["solves_per_s_plain"],
+           "solves_per_s_warm": c["solves_per_s_warm"], "speedup": c["speedup"],
+           "median_steps_plain": c["median_steps_plain"], "median_steps_warm": c["median_steps_warm"],
+           "max_rel_cd_change_warm_vs_plain": c["max_rel_cd_change_warm_vs_plain"],
+           "speedup_coarser_grid_first_from_network": c["sequenced_warm"]["speedup"],
+           "held_out_field_rel_l2": c["held_out_field_rel_l2"], "seconds": time.time() - t0}
+    print(f"warm-start sample: {json.dumps(out)}", file=sys.stderr, flush=True)
+    return out
 
 
 # Algorithmic bytes per cell-update of an implicit step with s red-black
@@ -505,6 +581,10 @@ def run_batch(a):
     t_create = time.time() - t0
     batch.setUniform(S["frees"])
     tuned = [dataclasses.asdict(t) for t in batch.autotune(steps=24 if sweep else 12, implicit=not sweep)]
+    if a.persistent != "auto" and a.precision == "fp32":
+        for t, (_, b) in zip(tuned, batch.groups):
+            b.setPersistent(int(a.persistent))
+            t["persistent"] = int(a.persistent)
     kernels = batch.kernel_sample(24 if sweep else 12, not sweep, cells)
     print(f"kernel passes (HIP events, no graphs): {kernels}", file=sys.stderr, flush=True)
     batch.setUniform(S["frees"])
@@ -525,7 +605,7 @@ def run_batch(a):
     print(f"{converged}/{len(ok)} converged in {wall:.2f}s: {solves:.2f} solves/s, {cu / wall:.3e} CU/s; "
           f"{reasons}; steps median {int(np.median(steps))}, p99 {int(np.percentile(steps, 99))}; "
           f"converged by (s) {pct}", file=sys.stderr, flush=True)
-    fp64 = _fp64_sample(ctx, S, results, plan) if a.precision == "fp32" else None
+    fp64 = _fp64_sample(ctx, S, results, plan) if a.precision == "fp32" and not a.no_fp64_sample else None
     below = sum(r.converged and r.residual <= 1e-6 for r in results)
     print(f"{below}/{len(ok)} cases at a density residual <= 1e-6", file=sys.stderr, flush=True)
     cpu = _cpu_baseline(S, not sweep, a.cpu_seconds, cu / max(1, converged)) if a.cpu_seconds > 0 else None
@@ -823,7 +903,7 @@ def run_warmstart(a):
                      "us_per_launch": ms_gemm * 1e3},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def run_fitted(a):
