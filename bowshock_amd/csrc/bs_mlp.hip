@@ -29,6 +29,7 @@ namespace {
 typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int GM = 64, GN = 64, GK = 32, GPAD = 8;
 
@@ -275,6 +276,139 @@ __global__ __launch_bounds__(256) void gemm128_kernel(GemmArgs g, int split, flo
       }
 }
 
+// 64 x 64 block tile for operands that are both K-contiguous (A [m][k],
+// B [n][k]): four wavefronts of 32 x 32 as gemm_bf16_kernel, K steps of 64
+// through two LDS buffers, and the next step's operands loaded into
+// registers while the current step is multiplied, so that one barrier per
+// step suffices and the global loads' latency overlaps the MFMAs. Each
+// thread moves two 16-byte vectors of A and two of B per step.
+constexpr int PK = 64, PPAD = 8;
+
+//
+// A one-dimensional grid of (M / 64) (N / 64) blocks placed XCD-aware
+// (xcd_tile): the tiles of one XCD are a contiguous run with the M tiles of
+// an N column adjacent, so the blocks that share a column of B run on one
+// XCD together and all but the first read it from that XCD's L2.
+template <int EPI>
+__global__ __launch_bounds__(256) void gemm_pipe_kernel(GemmArgs g) {
+  __shared__ __attribute__((aligned(16))) bf16 As[2][GM][PK + PPAD];
+  __shared__ __attribute__((aligned(16))) bf16 Bs[2][GN][PK + PPAD];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x), mt = g.M / GM;
+  const long m0 = (long)(tile % mt) * GM, n0 = (long)(tile / mt) * GN;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  // Rows t / 8 and t / 8 + 32, k offset 8 (t % 8).
+  const int r = t >> 3, kq = (t & 7) * 8;
+  const bf16* a0 = g.A + (m0 + r) * g.lda_m + kq;
+  const bf16* a1 = a0 + 32 * g.lda_m;
+  const bf16* b0 = g.B + (n0 + r) * g.ldb_n + kq;
+  const bf16* b1 = b0 + 32 * g.ldb_n;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  uint4 ra0 = *(const uint4*)a0, ra1 = *(const uint4*)a1, rb0 = *(const uint4*)b0, rb1 = *(const uint4*)b1;
+  const int steps = g.K / PK;
+  for (int s = 0; s < steps; s++) {
+    const int buf = s & 1;
+    *(uint4*)&As[buf][r][kq] = ra0;
+    *(uint4*)&As[buf][r + 32][kq] = ra1;
+    *(uint4*)&Bs[buf][r][kq] = rb0;
+    *(uint4*)&Bs[buf][r + 32][kq] = rb1;
+    __syncthreads();
+    if (s + 1 < steps) {
+      const long o = (long)(s + 1) * PK;
+      ra0 = *(const uint4*)(a0 + o);
+      ra1 = *(const uint4*)(a1 + o);
+      rb0 = *(const uint4*)(b0 + o);
+      rb1 = *(const uint4*)(b1 + o);
+    }
+#pragma unroll
+    for (int kk = 0; kk < PK; kk += 32) {
+      bf16x8 a[2], b[2];
+#pragma unroll
+      for (int i = 0; i < 2; i++) a[i] = *(const bf16x8*)&As[buf][wm + i * 16 + (lane & 15)][kk + 8 * (lane >> 4)];
+#pragma unroll
+      for (int j = 0; j < 2; j++) b[j] = *(const bf16x8*)&Bs[buf][wn + j * 16 + (lane & 15)][kk + 8 * (lane >> 4)];
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        gemm_store<EPI>(g, m0 + wm + i * 16 + 4 * (lane >> 4) + q, n0 + wn + j * 16 + (lane & 15), acc[i][j][q]);
+}
+
+// The same tile for K = 64 STEPS with STEPS small: every step's operands
+// are loaded into registers at the start, so the block waits for global
+// memory once instead of once per step (a 256 x 512 x 12800 GEMM spends
+// most of a step waiting on its loads otherwise).
+template <int EPI, int STEPS>
+__global__ __launch_bounds__(256) void gemm_allk_kernel(GemmArgs g) {
+  __shared__ __attribute__((aligned(16))) bf16 As[2][GM][PK + PPAD];
+  __shared__ __attribute__((aligned(16))) bf16 Bs[2][GN][PK + PPAD];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x), mt = g.M / GM;
+  const long m0 = (long)(tile % mt) * GM, n0 = (long)(tile / mt) * GN;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int r = t >> 3, kq = (t & 7) * 8;
+  const bf16* a0 = g.A + (m0 + r) * g.lda_m + kq;
+  const bf16* a1 = a0 + 32 * g.lda_m;
+  const bf16* b0 = g.B + (n0 + r) * g.ldb_n + kq;
+  const bf16* b1 = b0 + 32 * g.ldb_n;
+  // Native vector types: arrays of them stay in registers.
+  u32x4 ra0[STEPS], ra1[STEPS], rb0[STEPS], rb1[STEPS];
+#pragma unroll
+  for (int s = 0; s < STEPS; s++) {
+    ra0[s] = *(const u32x4*)(a0 + s * PK);
+    ra1[s] = *(const u32x4*)(a1 + s * PK);
+    rb0[s] = *(const u32x4*)(b0 + s * PK);
+    rb1[s] = *(const u32x4*)(b1 + s * PK);
+  }
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int s = 0; s < STEPS; s++) {
+    const int buf = s & 1;
+    *(u32x4*)&As[buf][r][kq] = ra0[s];
+    *(u32x4*)&As[buf][r + 32][kq] = ra1[s];
+    *(u32x4*)&Bs[buf][r][kq] = rb0[s];
+    *(u32x4*)&Bs[buf][r + 32][kq] = rb1[s];
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < PK; kk += 32) {
+      bf16x8 a[2], b[2];
+#pragma unroll
+      for (int i = 0; i < 2; i++) a[i] = *(const bf16x8*)&As[buf][wm + i * 16 + (lane & 15)][kk + 8 * (lane >> 4)];
+#pragma unroll
+      for (int j = 0; j < 2; j++) b[j] = *(const bf16x8*)&Bs[buf][wn + j * 16 + (lane & 15)][kk + 8 * (lane >> 4)];
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        gemm_store<EPI>(g, m0 + wm + i * 16 + 4 * (lane >> 4) + q, n0 + wn + j * 16 + (lane & 15), acc[i][j][q]);
+}
+
 // Sum of the split partial products in a fixed order, then the epilogue.
 template <int EPI>
 __global__ __launch_bounds__(256) void gemm_split_reduce(GemmArgs g, int split, const float* __restrict__ ws) {
@@ -353,6 +487,24 @@ __global__ void to_bf16(const float* __restrict__ w, bf16* __restrict__ wb, long
   if (e < n) wb[e] = (bf16)w[e];
 }
 
+// The transposed BF16 mirror of one layer's weights: wt[n][k] = w[k][n]
+// for w of rows x cols (both multiples of 64), through a 64 x 64 LDS tile
+// so that the loads and the stores are both coalesced.
+__global__ __launch_bounds__(256) void to_bf16_transposed(const float* __restrict__ w, bf16* __restrict__ wt,
+                                                          int rows, int cols) {
+  __shared__ float tile[64][65];
+  const long r0 = (long)blockIdx.y * 64, c0 = (long)blockIdx.x * 64;
+  for (int k = threadIdx.x; k < 64 * 64; k += 256) {
+    const int r = k >> 6, c = k & 63;
+    tile[r][c] = w[(r0 + r) * cols + c0 + c];
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < 64 * 64; k += 256) {
+    const int c = k >> 6, r = k & 63;
+    wt[(c0 + c) * rows + r0 + r] = (bf16)tile[r][c];
+  }
+}
+
 int round64(int v) { return (v + 63) / 64 * 64; }
 
 }  // namespace
@@ -366,7 +518,8 @@ struct bs_mlp {
   std::vector<long> w_off, b_off; // offsets into the flat parameter vector
   long n_params = 0;              // padded flat count
   float *w = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr;
-  bf16* wb = nullptr;
+  bf16* wb = nullptr;   // BF16 mirror of w: per layer [in][out]
+  bf16* wbt = nullptr;  // per layer [out][in], the forward GEMMs' K-contiguous operand
   bf16* x = nullptr;                      // [max_batch x in_pad]
   std::vector<float*> z;                  // per hidden layer [max_batch x hidden], and the output
   std::vector<bf16*> act;                 // per hidden layer
@@ -404,7 +557,7 @@ struct MlpGuard {
   (void)hipSetDevice((p)->device)
 
 void mlp_free(bs_mlp* p) {
-  std::vector<void*> all = {p->w, p->grad, p->m, p->v, p->wb, p->x, p->dzf[0], p->dzf[1], p->dzb[0], p->dzb[1],
+  std::vector<void*> all = {p->w, p->grad, p->m, p->v, p->wb, p->wbt, p->x, p->dzf[0], p->dzf[1], p->dzb[0], p->dzb[1],
                             p->partials, p->idx, p->data_x, p->data_y, p->stage_x, p->ws};
   for (float* q : p->z) all.push_back(q);
   for (bf16* q : p->act) all.push_back(q);
@@ -436,8 +589,17 @@ void launch_gemm128(bs_mlp* p, const GemmArgs& g) {
 
 template <int EPI>
 void launch_gemm(bs_mlp* p, const GemmArgs& g, bool a_kc, bool b_nc) {
-  // The 128 x 128 tile where it fills the chip (or its K is long enough to
-  // split); otherwise the 64 x 64 tile, which gives four times the blocks.
+  // Both operands K-contiguous (the forward GEMMs on the transposed weight
+  // mirror, dZ W^T): the pipelined 64 x 64 tile. Otherwise the 128 x 128
+  // tile where it fills the chip (or its K is long enough to split), else
+  // the 64 x 64 tile, which gives four times the blocks.
+  if (a_kc && !b_nc && g.K % PK == 0 && g.K < 4096) {
+    const dim3 grid((g.N / GN) * (g.M / GM));
+    if (g.K == 8 * PK) hipLaunchKernelGGL((gemm_allk_kernel<EPI, 8>), grid, dim3(256), 0, p->stream, g);
+    else if (g.K == PK) hipLaunchKernelGGL((gemm_allk_kernel<EPI, 1>), grid, dim3(256), 0, p->stream, g);
+    else hipLaunchKernelGGL(gemm_pipe_kernel<EPI>, grid, dim3(256), 0, p->stream, g);
+    return;
+  }
   const long tiles128 = (long)(g.M / BM) * (g.N / BN);
   if (g.M % BM == 0 && g.N % BN == 0 && g.K % BK == 0 && (tiles128 >= 256 || g.K >= 4096)) {
     if (a_kc && b_nc) launch_gemm128<EPI, true, true>(p, g);
@@ -453,6 +615,17 @@ void launch_gemm(bs_mlp* p, const GemmArgs& g, bool a_kc, bool b_nc) {
   else hipLaunchKernelGGL((gemm_bf16_kernel<EPI, false, false>), grid, dim3(256), 0, p->stream, g);
 }
 
+// The BF16 mirrors of the FP32 master weights (wb is also written by the
+// Adam kernel; the transposed mirror is refreshed here).
+void mlp_mirror(bs_mlp* p, bool plain) {
+  if (plain)
+    hipLaunchKernelGGL(to_bf16, dim3((unsigned)((p->n_params + 255) / 256)), dim3(256), 0, p->stream, p->w, p->wb,
+                       p->n_params);
+  for (int l = 0; l < p->layers; l++)
+    hipLaunchKernelGGL(to_bf16_transposed, dim3(p->lout[l] / 64, p->lin[l] / 64), dim3(256), 0, p->stream,
+                       p->w + p->w_off[l], p->wbt + p->w_off[l], p->lin[l], p->lout[l]);
+}
+
 // Forward pass of `rows` staged input rows (p->x, padded to batch_pad).
 void mlp_forward(bs_mlp* p, int batch_pad) {
   const bf16* a = p->x;
@@ -461,9 +634,9 @@ void mlp_forward(bs_mlp* p, int batch_pad) {
     g.A = a;
     g.lda_m = p->lin[l];
     g.lda_k = 1;
-    g.B = p->wb + p->w_off[l];
-    g.ldb_k = p->lout[l];
-    g.ldb_n = 1;
+    g.B = p->wbt + p->w_off[l];  // B[k][n] = W[k][n], K contiguous
+    g.ldb_k = 1;
+    g.ldb_n = p->lin[l];
     g.C = p->z[l];
     g.ldc = p->lout[l];
     g.bias = p->w + p->b_off[l];
@@ -472,10 +645,10 @@ void mlp_forward(bs_mlp* p, int batch_pad) {
     g.K = p->lin[l];
     if (l + 1 < p->layers) {
       g.Cb = p->act[l];
-      launch_gemm<EPI_BIAS_SILU>(p, g, true, true);
+      launch_gemm<EPI_BIAS_SILU>(p, g, true, false);
       a = p->act[l];
     } else {
-      launch_gemm<EPI_BIAS>(p, g, true, true);
+      launch_gemm<EPI_BIAS>(p, g, true, false);
     }
   }
 }
@@ -594,7 +767,8 @@ bs_status bs_mlp_create(bs_ctx* ctx, int32_t inputs, int32_t hidden, int32_t dep
   bs_status st;
   if ((st = dev_alloc(&p->w, p->n_params, "weights")) || (st = dev_alloc(&p->grad, p->n_params, "gradients")) ||
       (st = dev_alloc(&p->m, p->n_params, "moments")) || (st = dev_alloc(&p->v, p->n_params, "moments")) ||
-      (st = dev_alloc(&p->wb, p->n_params, "weights")) || (st = dev_alloc(&p->x, (size_t)max_batch * p->in_pad, "inputs")) ||
+      (st = dev_alloc(&p->wb, p->n_params, "weights")) || (st = dev_alloc(&p->wbt, p->n_params, "weights")) ||
+      (st = dev_alloc(&p->x, (size_t)max_batch * p->in_pad, "inputs")) ||
       (st = dev_alloc(&p->dzf[0], (size_t)max_batch * wide, "gradients")) ||
       (st = dev_alloc(&p->dzf[1], (size_t)max_batch * wide, "gradients")) ||
       (st = dev_alloc(&p->dzb[0], (size_t)max_batch * wide, "gradients")) ||
@@ -620,6 +794,7 @@ bs_status bs_mlp_create(bs_ctx* ctx, int32_t inputs, int32_t hidden, int32_t dep
   if (hipMemsetAsync(p->w, 0, bytes, p->stream) != hipSuccess || hipMemsetAsync(p->m, 0, bytes, p->stream) != hipSuccess ||
       hipMemsetAsync(p->v, 0, bytes, p->stream) != hipSuccess ||
       hipMemsetAsync(p->wb, 0, p->n_params * sizeof(bf16), p->stream) != hipSuccess ||
+      hipMemsetAsync(p->wbt, 0, p->n_params * sizeof(bf16), p->stream) != hipSuccess ||
       hipStreamSynchronize(p->stream) != hipSuccess)
     return bail(fail(BS_ERR_EXECUTION, "network initialisation failed"));
   *out = p;
@@ -654,8 +829,7 @@ bs_status bs_mlp_set_params(bs_mlp* p, const float* params, int64_t count) {
     src += out_w;
   }
   HIP_TRY(hipMemcpyAsync(p->w, flat.data(), flat.size() * sizeof(float), hipMemcpyHostToDevice, p->stream), "upload");
-  hipLaunchKernelGGL(to_bf16, dim3((unsigned)((p->n_params + 255) / 256)), dim3(256), 0, p->stream, p->w, p->wb,
-                     p->n_params);
+  mlp_mirror(p, true);
   HIP_TRY(hipMemsetAsync(p->m, 0, p->n_params * sizeof(float), p->stream), "reset");
   HIP_TRY(hipMemsetAsync(p->v, 0, p->n_params * sizeof(float), p->stream), "reset");
   HIP_TRY(hipStreamSynchronize(p->stream), "upload");
@@ -724,6 +898,7 @@ bs_status bs_mlp_train_batch(bs_mlp* p, const int32_t* idx, int32_t batch, doubl
   const float c1 = 1.f - powf(b1, (float)p->step), c2 = 1.f - powf(b2, (float)p->step);
   hipLaunchKernelGGL(adam_step, dim3((unsigned)((p->n_params + 255) / 256)), dim3(256), 0, p->stream, p->w, p->grad,
                      p->m, p->v, p->wb, p->n_params, (float)lr, b1, b2, 1e-8f, c1, c2);
+  mlp_mirror(p, false);
   HIP_TRY(hipGetLastError(), "adam launch");
   HIP_TRY(hipStreamSynchronize(p->stream), "adam step");
   return BS_OK;
@@ -761,18 +936,18 @@ bs_status bs_mlp_time_gemm(bs_mlp* p, int32_t layer, int32_t reps, double* ms_pe
   g.A = layer == 0 ? p->x : p->act[layer - 1];
   g.lda_m = p->lin[layer];
   g.lda_k = 1;
-  g.B = p->wb + p->w_off[layer];
-  g.ldb_k = p->lout[layer];
-  g.ldb_n = 1;
+  g.B = p->wbt + p->w_off[layer];
+  g.ldb_k = 1;
+  g.ldb_n = p->lin[layer];
   g.C = p->z[layer];
   g.ldc = p->lout[layer];
   g.bias = p->w + p->b_off[layer];
   g.M = p->max_batch;
   g.N = p->lout[layer];
   g.K = p->lin[layer];
-  launch_gemm<EPI_BIAS>(p, g, true, true);
+  launch_gemm<EPI_BIAS>(p, g, true, false);
   (void)hipEventRecord(e0, p->stream);
-  for (int r = 0; r < reps; r++) launch_gemm<EPI_BIAS>(p, g, true, true);
+  for (int r = 0; r < reps; r++) launch_gemm<EPI_BIAS>(p, g, true, false);
   (void)hipEventRecord(e1, p->stream);
   HIP_TRY(hipEventSynchronize(e1), "gemm timing");
   float ms = 0.f;
