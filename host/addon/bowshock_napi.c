@@ -700,13 +700,21 @@ static napi_value js_batch_set_active(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
-/* batchSetInt(b, what, value): 0 tiling threshold, 1 relaxation form. */
+/* batchSetInt(b, what, value): 0 tiling threshold, 1 relaxation form,
+ * 2 persistent tile, 3 half-sweeps per relaxation launch. */
 static napi_value js_batch_set_int(napi_env env, napi_callback_info info) {
   napi_value argv[3];
   if (!get_args(env, info, 3, argv)) return NULL;
   bs_batch *b = (bs_batch *)get_ptr(env, argv[0]);
   const int32_t v = i32(env, argv[2]);
-  bs_status st = i32(env, argv[1]) == 0 ? bs_batch_set_tiling(b, v) : bs_batch_set_relaxation(b, v);
+  bs_status st;
+  switch (i32(env, argv[1])) {
+    case 0: st = bs_batch_set_tiling(b, v); break;
+    case 1: st = bs_batch_set_relaxation(b, v); break;
+    case 2: st = bs_batch_set_persistent(b, v); break;
+    case 3: st = bs_batch_set_relax_fusion(b, v); break;
+    default: st = BS_ERR_RANGE; break;
+  }
   if (st) return throw_status(env, st);
   return NULL;
 }

@@ -195,6 +195,10 @@ export class GpuBatch2D {
   setActive(active: readonly boolean[]): void;
   setTiling(smallTileBelowCells: number): void;
   setRelaxation(form: 'fluxDifference' | 'linearized'): void;
+  /** Explicit FP32 steps in persistent launches: 1 (32 x 4 tiles), 2 (32 x 8) or 0 (per-stage launches). */
+  setPersistent(tile: 0 | 1 | 2): void;
+  /** At most this many red-black half-sweeps per relaxation launch (states unchanged). */
+  setRelaxFusion(halfSweeps: 1 | 2 | 3): void;
   iterate(steps: number): Promise<void>;
   setImplicitCfl(cfl: number | readonly number[]): Promise<void>;
   iterateImplicit(

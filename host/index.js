@@ -636,6 +636,18 @@ class GpuBatch2D {
     call(() => native().batchSetInt(this.handle, 1, code));
   }
 
+  /** Explicit FP32 steps in persistent launches: tile 1 (32 x 4), 2 (32 x 8) or 0 (one launch per stage). */
+  setPersistent(tile) {
+    this.exclusive.assertIdle();
+    call(() => native().batchSetInt(this.handle, 2, tile));
+  }
+
+  /** At most this many red-black half-sweeps (1-3) per relaxation launch. */
+  setRelaxFusion(halfSweeps) {
+    this.exclusive.assertIdle();
+    call(() => native().batchSetInt(this.handle, 3, halfSweeps));
+  }
+
   async iterate(steps) {
     await this.exclusive.run(() => native().batchIterate(this.handle, steps));
   }

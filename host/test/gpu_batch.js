@@ -1,9 +1,10 @@
 'use strict';
 // Runs a batch through the Node facade on the GPU and prints the results
-// as JSON: `node gpu_batch.js case.json`. The case holds the grids, their
-// Mach numbers and boundary kinds (an H grid's inflow iMin or a C grid's
-// axis); the calling test compares the states and the solve with the
-// Python facade on the same batch.
+// as JSON: `node gpu_batch.js case.json` (one half-sweep per relaxation
+// launch for the implicit steps, persistent launches for the solve). The
+// case holds the grids, their Mach numbers and boundary kinds (an H grid's
+// inflow iMin or a C grid's axis); the calling test compares the states
+// and the solve with the Python facade on the same batch.
 const fs = require('fs');
 const bs = require('..');
 
@@ -30,6 +31,8 @@ const bs = require('..');
   // Implicit steps of every case, the states read back.
   const batch = await bs.GpuBatch2D.create(device, configs);
   batch.setUniform(frees.map((f) => f.state));
+  // One half-sweep per relaxation launch: the states do not change.
+  batch.setRelaxFusion(1);
   await batch.setImplicitCfl(1);
   await batch.iterateImplicit(c.steps, { cfl: 10, sweeps: 2 });
   out.rho = [];
@@ -55,6 +58,8 @@ const bs = require('..');
   // The stopping rule, explicit steps from the free stream.
   const solveBatch = await bs.GpuBatch2D.create(device, configs);
   solveBatch.setUniform(frees.map((f) => f.state));
+  // Persistent launches: the same states, so the same outcomes.
+  solveBatch.setPersistent(2);
   out.solve = await bs.solveBatch(solveBatch, frees, 0.5, { chunk: 240, maxSteps: c.maxSteps });
   solveBatch.destroy();
   // faceStates sides are refused.
