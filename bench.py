@@ -68,8 +68,6 @@ def parse():
     ap.add_argument("--mach", type=float, default=3.0)
     ap.add_argument("--cpu-steps", type=int, default=40, help="oracle steps timed for the CPU baseline (0: skip)")
     ap.add_argument("--no-graphs", action="store_true", help="launch kernel by kernel instead of HIP graphs")
-    ap.add_argument("--dataflow", action="store_true",
-                    help="single: explicit steps in dataflow launches (bs_set_dataflow) instead of one launch per stage")
     ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32",
                     help="the FP32 solver (batches), or the FP64 one (state, metrics and residual in FP64; the "
                          "batched workloads then stop on a density residual of 1e-6)")
@@ -157,8 +155,6 @@ def main():
     gpu.setCellPrimitives(np.array(free))
     if a.no_graphs:
         gpu.enableGraphs(False)
-    if a.dataflow:
-        gpu.setDataflow(True)
     # Untimed: the pre-warm, one call of the timed region's length (which
     # captures the graphs it replays), then the `--warmup` steps.
     gpu.iterate(PREWARM_STEPS)
@@ -236,9 +232,7 @@ def main():
                    "untimed_steps_before_warmup": PREWARM_STEPS + a.steps},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "kernel": ("dataflow_stage_kernel (time per stage within dataflow launches)" if a.dataflow
-                                else "tile_kernel (explicit stage)"),
-                     "bytes_per_cell_update": width * BYTES_PER_CU_RK3,
+                     "kernel": "tile_kernel (explicit stage)", "bytes_per_cell_update": width * BYTES_PER_CU_RK3,
                      "us_per_launch": ms_launch * 1e3},
         "cpu_baseline": cpu,
         "drag": cd,

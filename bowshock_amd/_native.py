@@ -65,7 +65,6 @@ SIGNATURES = {
     "bs_iterate_implicit": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_int32, C.c_double, C.c_double,
                                       C.POINTER(Report)]),
     "bs_enable_graphs": (C.c_int, [_vp, C.c_int32]),
-    "bs_set_dataflow": (C.c_int, [_vp, C.c_int32]),
     "bs_set_relaxation": (C.c_int, [_vp, C.c_int32]),
     "bs_set_fitted_shock": (C.c_int, [_vp, _dp]),
     "bs_read_shock": (C.c_int, [_vp, _dp, _dp, _dp, C.c_int32]),

@@ -34,7 +34,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-Wall", "-
 
 SOURCES = ["bs_engine.hip", "bs_tiles_f32.hip", "bs_tiles_f64.hip", "bs_batch.hip", "bs_batch_tiles_large.hip",
            "bs_batch_tiles_small.hip", "bs_batch_tiles_large_f64.hip", "bs_batch_tiles_small_f64.hip", "bs_mlp.hip",
-           "bs_batch_persistent.hip", "bs_tiles_dataflow.hip"]
+           "bs_batch_persistent.hip"]
 
 
 def _run(cmd):

@@ -18,7 +18,6 @@
 #include <vector>
 
 #include "bs_internal.hpp"
-#include "bs_dataflow.hpp"
 #include "bs_ledger.hpp"
 #include "bs_anderson.hpp"
 #include "bs_linegrid.hpp"
@@ -68,11 +67,6 @@ struct DeviceBuffers {
   double *line = nullptr, *eta = nullptr, *height = nullptr, *move = nullptr;
   // implicit work
   float *res = nullptr, *diag = nullptr, *lam_i = nullptr, *lam_j = nullptr, *dq = nullptr;
-  // dataflow launches (bs_set_dataflow): the stage records, per-tile flags,
-  // the work-item counter and the fault word
-  void* df_recs = nullptr;
-  int *df_flags = nullptr, *df_next = nullptr;
-  unsigned* df_fault = nullptr;
 };
 
 struct Timing {
@@ -111,8 +105,6 @@ struct bs_solver {
   StepGraph explicit_graph, implicit_graph, anderson_graph, fitted_graph;
   long long anderson_for = 0;  // id of the history the anderson graph was captured with
   int fitted_for = 0;          // steps per move the fitted graph was captured with
-  bool dataflow = false;       // explicit local steps in dataflow launches (bs_set_dataflow)
-  int df_blocks = 0;           // blocks of a dataflow launch
 };
 
 namespace {
@@ -326,104 +318,6 @@ bs_status encode_step_t(bs_solver* s) {
 
 bs_status encode_step(bs_solver* s) { return s->fp64 ? encode_step_t<double>(s) : encode_step_t<float>(s); }
 
-// ---- dataflow launches -------------------------------------------------------
-
-// Dataflow launches run the explicit local steps of an FP32 solver without
-// a fitted shock (whose boundary kernel runs between steps).
-bool dataflow_ok(const bs_solver* s) {
-  return s->dataflow && !s->fp64 && !s->fitted && s->cfg.time_stepping == BS_STEPPING_LOCAL;
-}
-
-// The stage records of every current buffer (the records encode_step_t
-// passes by value), the flags and the counter; the launch's block count
-// from the kernel's occupancy.
-bs_status dataflow_setup(bs_solver* s) {
-  const float *ca, *cb;
-  const int ns = rk_table(s->cfg.integrator, &ca, &cb);
-  TileArgsT<float> recs[9];
-  memset(recs, 0, sizeof recs);
-  for (int cur = 0; cur < 3; cur++) {
-    const int other[2] = {(cur + 1) % 3, (cur + 2) % 3};
-    int prev = cur;
-    for (int k = 0; k < ns; k++) {
-      const int dst = (k == ns - 1 && k > 0) ? cur : other[k % 2];
-      TileArgsT<float> A = tile_args<float>(s);
-      A.u0 = state_of<float>(s, cur);
-      A.src = state_of<float>(s, prev);
-      A.dst = state_of<float>(s, dst);
-      A.a = ca[k];
-      A.b = cb[k];
-      A.dt_mode = k == 0 ? DT_COMPUTE_LOCAL : DT_READ;
-      A.count_step = k == ns - 1 ? 1 : 0;
-      recs[3 * cur + k] = A;
-      prev = dst;
-    }
-  }
-  bs_status st;
-  const int tiles = s->tiles_x * s->tiles_y;
-  if (!s->d.df_recs && ((st = dev_alloc((char**)&s->d.df_recs, sizeof recs, "dataflow records")) ||
-                        (st = dev_alloc(&s->d.df_flags, (size_t)tiles * DATAFLOW_FLAG_STRIDE, "dataflow flags")) ||
-                        (st = dev_alloc(&s->d.df_next, (size_t)1, "dataflow counter")) ||
-                        (st = dev_alloc(&s->d.df_fault, (size_t)1, "dataflow fault word"))))
-    return st;
-  HIP_TRY(hipMemcpyAsync(s->d.df_recs, recs, sizeof recs, hipMemcpyHostToDevice, s->stream), "dataflow records");
-  HIP_TRY(hipStreamSynchronize(s->stream), "dataflow records");
-  hipDeviceProp_t prop;
-  HIP_TRY(hipGetDeviceProperties(&prop, s->device), "device properties");
-  int per_cu = 0;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(dataflow_fn(s->cfg)),
-                                                       Tile<float>::x * Tile<float>::y, 0),
-          "occupancy");
-  s->df_blocks = prop.multiProcessorCount * (per_cu > 0 ? per_cu : 1);
-  if (s->df_blocks > tiles) s->df_blocks = tiles;
-  return BS_OK;
-}
-
-// `count` explicit local steps in dataflow launches (at most 2^29 work
-// items each); the fault word is read after each launch.
-bs_status dataflow_run(bs_solver* s, int count) {
-  const float *ca, *cb;
-  const int ns = rk_table(s->cfg.integrator, &ca, &cb);
-  const int tiles = s->tiles_x * s->tiles_y;
-  const long long per_launch = ((long long)1 << 29) / ((long long)ns * tiles);
-  const int max_steps = per_launch < 1 ? 1 : (per_launch > count ? count : (int)per_launch);
-  DataflowArgs D;
-  D.recs = static_cast<const TileArgsT<float>*>(s->d.df_recs);
-  D.tiles_x = s->tiles_x;
-  D.tiles_y = s->tiles_y;
-  D.ns = ns;
-  D.wrap_i = s->cfg.boundaries[IMIN].kind == BS_BC_PERIODIC;
-  D.wrap_j = s->cfg.boundaries[JMIN].kind == BS_BC_PERIODIC;
-  D.flags = s->d.df_flags;
-  D.next = s->d.df_next;
-  D.fault = s->d.df_fault;
-  D.spin_limit = 1 << 18;
-  const DataflowFn fn = dataflow_fn(s->cfg);
-  for (int done = 0; done < count;) {
-    const int n = count - done < max_steps ? count - done : max_steps;
-    D.steps = n;
-    D.cur0 = s->cur;
-    hipLaunchKernelGGL(dataflow_clear, dim3((tiles + 255) / 256), dim3(256), 0, s->stream, s->d.df_flags, tiles,
-                       s->d.df_next, s->d.df_fault);
-    hipEvent_t* ev = timing_pair(s, 0, (long long)n * ns);
-    hipLaunchKernelGGL(fn, dim3(s->df_blocks), dim3(Tile<float>::x * Tile<float>::y), 0, s->stream, D);
-    timing_end(s, ev);
-    HIP_TRY(hipGetLastError(), "dataflow launch");
-    unsigned fault = 0;
-    HIP_TRY(hipMemcpyAsync(&fault, s->d.df_fault, sizeof fault, hipMemcpyDeviceToHost, s->stream), "fault word");
-    HIP_TRY(hipStreamSynchronize(s->stream), "dataflow launch");
-    if (fault) {
-      s->dataflow = false;
-      return fail(BS_ERR_EXECUTION,
-                  "dataflow launch abandoned (a tile's wait for its neighbours timed out); dataflow launches are off "
-                  "for this solver and its state is indeterminate");
-    }
-    if (ns == 1) s->cur = (s->cur + n) % 3;
-    done += n;
-  }
-  return BS_OK;
-}
-
 // `count` steps: captured graphs (replay_steps) for a call of at least
 // GRAPH_STEPS steps, a shorter call launched kernel by kernel. Per-launch
 // kernel timing needs the events between launches, so it runs without
@@ -482,7 +376,7 @@ void free_all(bs_solver* s) {
   void* all[] = {d.state[0], d.state[1], d.state[2], d.inx, d.iny, d.ia, d.jnx, d.jny, d.ja, d.vol, d.planar,
                  d.dt, d.dt_global, d.dt_partials, d.partials, d.scalar, d.control, d.bnd, d.wall_out, d.shock_out,
                  d.line, d.eta, d.height, d.move, d.res,
-                 d.diag, d.lam_i, d.lam_j, d.dq, d.df_recs, d.df_flags, d.df_next, d.df_fault};
+                 d.diag, d.lam_i, d.lam_j, d.dq};
   for (void* p : all)
     if (p) (void)hipFree(p);
   for (int k = 0; k < 4; k++) {
@@ -916,17 +810,11 @@ bs_status bs_iterate(bs_solver* s, int32_t count, bs_report* report) {
   int remaining = count;
   while (remaining > 0) {
     const int batch = remaining < BATCH ? remaining : BATCH;
-    bs_status st;
-    if (dataflow_ok(s)) {
-      // Dataflow launches record their own event pairs when timing is on.
-      st = dataflow_run(s, batch);
-    } else {
-      hipEvent_t* pair = region ? timing_pair(s, 0, (long long)batch * ns) : nullptr;
-      s->timing.suppress = region;
-      st = run_steps(s, s->explicit_graph, batch, [s]() { return encode_step(s); });
-      s->timing.suppress = false;
-      timing_end(s, pair);
-    }
+    hipEvent_t* pair = region ? timing_pair(s, 0, (long long)batch * ns) : nullptr;
+    s->timing.suppress = region;
+    bs_status st = run_steps(s, s->explicit_graph, batch, [s]() { return encode_step(s); });
+    s->timing.suppress = false;
+    timing_end(s, pair);
     if (st) return st;
     st = read_control(s, &c);
     if (st) return st;
@@ -1811,14 +1699,6 @@ bs_status bs_set_relaxation(bs_solver* s, int32_t form) {
     drop_graph(s->anderson_graph);
   }
   s->relax_linear = linear;
-  return BS_OK;
-}
-
-bs_status bs_set_dataflow(bs_solver* s, int32_t enable) {
-  SOLVER_GUARD(s);
-  if (enable && s->fp64) return fail(BS_ERR_RANGE, "dataflow launches run FP32 solvers");
-  s->dataflow = enable != 0;
-  if (s->dataflow) return dataflow_setup(s);
   return BS_OK;
 }
 

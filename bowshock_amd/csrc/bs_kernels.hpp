@@ -60,33 +60,13 @@ struct StateDevT {
 };
 using StateDev = StateDevT<float>;
 
-// Coherence policy of the stencil tiles (template parameter COH):
-// COH_NONE: plain loads and stores (one launch per stage: the kernel
-//   boundary makes every store visible);
-// COH_XCD: states read with device-scope loads, which read the L2 and never
-//   a line the CU's L1 still holds from before another block's store (the
-//   persistent batch kernel: every block of a case on one XCD, one L2);
-// COH_DEVICE: also u0 and the time steps read that way, and the new states
-//   and time steps stored with device-scope stores, which write through
-//   the XCD's L2 (the dataflow kernel: a tile's neighbours may run on other
-//   XCDs, whose L2s are not coherent with this one).
-constexpr int COH_NONE = 0, COH_XCD = 1, COH_DEVICE = 2;
-
 // Element idx of a state array: a plain load, or (COH) a device-scope
-// relaxed atomic load.
-template <int COH, typename T>
+// relaxed atomic load, which reads the L2 and never a line the CU's L1
+// still holds from before another block's store.
+template <bool COH, typename T>
 __device__ __forceinline__ T ld(const T* base, int idx) {
-  if constexpr (COH != COH_NONE) return __hip_atomic_load(const_cast<T*>(base) + idx, __ATOMIC_RELAXED,
-                                                          __HIP_MEMORY_SCOPE_AGENT);
+  if constexpr (COH) return __hip_atomic_load(const_cast<T*>(base) + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   else return base[idx];
-}
-
-// Store of element idx: plain, or (COH_DEVICE) a device-scope relaxed
-// atomic store.
-template <int COH, typename T>
-__device__ __forceinline__ void st(T* base, int idx, T v) {
-  if constexpr (COH == COH_DEVICE) __hip_atomic_store(base + idx, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  else *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (unsigned)idx * (unsigned)sizeof(T)) = v;
 }
 
 // The conserved state of cell (i, j), or of a ghost cell up to two layers
@@ -98,7 +78,7 @@ __device__ __forceinline__ void st(T* base, int idx, T v) {
 // COH: the state is read with device-scope loads, which bypass the CU's L1
 // cache (the persistent batch kernel, whose blocks read states that other
 // blocks wrote during the same launch).
-template <typename T, int COH = COH_NONE>
+template <typename T, bool COH = false>
 __device__ __forceinline__ Vec4<T> load_cell(const StateDevT<T>& s, const GridDevT<T>& G, const BoundaryDevT<T>& B,
                                              int i, int j, T g) {
   const int ni = G.ni, nj = G.nj;
@@ -300,7 +280,7 @@ __device__ __forceinline__ void halo_cell(int h, int& wi, int& wj) {
 
 // Loads a tile's TileInputs; EDGE: also the high-edge face metrics
 // (otherwise the edge faces load them when they are computed).
-template <typename T, int TX, int TY, int MODE, bool EDGE, int COH = COH_NONE>
+template <typename T, int TX, int TY, int MODE, bool EDGE, bool COH = false>
 __device__ __forceinline__ void tile_inputs(const TileArgsT<T>& A, int bx, int by, TileInputs<T>& in) {
   constexpr int NT = TX * TY;
   const GridDevT<T>& G = A.G;
@@ -344,19 +324,19 @@ __device__ __forceinline__ void tile_inputs(const TileArgsT<T>& A, int bx, int b
     const int c = gi + gj * ni;
     in.vol = at(G.vol, c);
     in.planar = at(G.planar, c);
-    if (MODE == TILE_STAGE && A.dt_mode == DT_READ) in.dt = COH == COH_DEVICE ? ld<COH>(A.dt, c) : at(A.dt, c);
+    if (MODE == TILE_STAGE && A.dt_mode == DT_READ) in.dt = at(A.dt, c);
     if (MODE == TILE_STAGE && A.dt_mode == DT_GLOBAL) in.dt = *A.dt_global;
     // u0 of the SSP-RK combination; global stepping also keeps it for a
     // zero time step (the state is left unchanged).
     if (MODE == TILE_STAGE && (A.a != T(0.) || A.dt_mode == DT_GLOBAL)) {
 #pragma unroll
-      for (int q = 0; q < 4; q++) in.u0[q] = COH == COH_DEVICE ? ld<COH>(A.u0.q[q], c) : at(A.u0.q[q], c);
+      for (int q = 0; q < 4; q++) in.u0[q] = at(A.u0.q[q], c);
     }
   }
   if (tile_is_inner<TX, TY>(ni, nj, i0, j0)) {
     const int c = gi + gj * ni;
     if constexpr (COH) {
-      in.own = {ld<COH>(A.src.q[0], c), ld<COH>(A.src.q[1], c), ld<COH>(A.src.q[2], c), ld<COH>(A.src.q[3], c)};
+      in.own = {ld<true>(A.src.q[0], c), ld<true>(A.src.q[1], c), ld<true>(A.src.q[2], c), ld<true>(A.src.q[3], c)};
     } else {
       in.own = {at(A.src.q[0], c), at(A.src.q[1], c), at(A.src.q[2], c), at(A.src.q[3], c)};
     }
@@ -365,8 +345,8 @@ __device__ __forceinline__ void tile_inputs(const TileArgsT<T>& A, int bx, int b
       halo_cell<TX, TY>(tid, wi, wj);
       const int h = (i0 + wi - 2) + (j0 + wj - 2) * ni;
       if constexpr (COH) {
-        in.halo = {ld<COH>(A.src.q[0], h), ld<COH>(A.src.q[1], h), ld<COH>(A.src.q[2], h),
-                   ld<COH>(A.src.q[3], h)};
+        in.halo = {ld<true>(A.src.q[0], h), ld<true>(A.src.q[1], h), ld<true>(A.src.q[2], h),
+                   ld<true>(A.src.q[3], h)};
       } else {
         in.halo = {at(A.src.q[0], h), at(A.src.q[1], h), at(A.src.q[2], h), at(A.src.q[3], h)};
       }
@@ -377,7 +357,7 @@ __device__ __forceinline__ void tile_inputs(const TileArgsT<T>& A, int bx, int b
 // The tile (bx, by) of one grid from its inputs; its FP64 partial goes to
 // A.partials[slot]. EDGE: the high-edge faces' metrics come with the
 // inputs (otherwise those faces load them).
-template <typename T, int TX, int TY, int REC, int LIM, int FLUX, int MODE, bool EDGE, int COH = COH_NONE>
+template <typename T, int TX, int TY, int REC, int LIM, int FLUX, int MODE, bool EDGE, bool COH = false>
 __device__ __forceinline__ void tile_compute(const TileArgsT<T>& A, int bx, int by, int slot, const TileInputs<T>& in) {
   constexpr int NT = TX * TY;
   constexpr int WX = TX + 4, WY = TY + 4;
@@ -583,7 +563,7 @@ __device__ __forceinline__ void tile_compute(const TileArgsT<T>& A, int bx, int 
         const T lam = bs_abs(w.u * six + w.v * siy) + a * bs_sqrt(six * six + siy * siy) +
                           bs_abs(w.u * sjx + w.v * sjy) + a * bs_sqrt(sjx * sjx + sjy * sjy);
         dt = A.cfl * vol * bs_rcp(lam);
-        st<COH>(A.dt, c, dt);
+        at(A.dt, c) = dt;
       } else {
         dt = c_dt;  // the stored local step, or the global one
       }
@@ -612,7 +592,7 @@ __device__ __forceinline__ void tile_compute(const TileArgsT<T>& A, int bx, int 
         for (int q = 0; q < 4; q++) o[q] = c_u0[q];
       }
 #pragma unroll
-      for (int q = 0; q < 4; q++) st<COH>(A.dst.q[q], c, o[q]);
+      for (int q = 0; q < 4; q++) at(A.dst.q[q], c) = o[q];
       out = {o[0], o[1], o[2], o[3]};
       flag_non_physical(!is_physical(out, g), A.control);
     } else if (MODE == TILE_IMPLICIT) {
@@ -640,7 +620,7 @@ __device__ __forceinline__ void tile_compute(const TileArgsT<T>& A, int bx, int 
   if (A.count_step && bx == 0 && by == 0 && tid == 0) atomicAdd((unsigned long long*)&A.control->steps, 1ull);
 }
 
-template <typename T, int TX, int TY, int REC, int LIM, int FLUX, int MODE, int COH = COH_NONE>
+template <typename T, int TX, int TY, int REC, int LIM, int FLUX, int MODE, bool COH = false>
 __device__ __forceinline__ void tile_body(const TileArgsT<T>& A, int bx, int by, int slot) {
   // The FP32 tiles load the edge faces' metrics with the other inputs; the
   // FP64 tiles, short of registers, when the edge faces are computed.

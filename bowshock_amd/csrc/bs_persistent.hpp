@@ -112,7 +112,7 @@ __global__ __launch_bounds__(TX * TY) void batch_persistent_kernel(const TileArg
     const int cur = ns == 1 ? (cur0 + step) % 3 : cur0;
     for (int s = 0; s < ns; s++) {
       if (epoch > 0 && !group_barrier(ctr, epoch * it.members, fault, spin_limit)) return;
-      tile_body<float, TX, TY, REC, LIM, FLUX, TILE_STAGE, COH_XCD>(recs[(size_t)(3 * cur + s) * count + it.c], it.bx,
+      tile_body<float, TX, TY, REC, LIM, FLUX, TILE_STAGE, true>(recs[(size_t)(3 * cur + s) * count + it.c], it.bx,
                                                                  it.by, 0);
       epoch++;
     }

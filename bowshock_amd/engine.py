@@ -387,12 +387,6 @@ class GpuEuler2D:
         """Run steps as captured HIP graphs (default on)."""
         check(N.lib().bs_enable_graphs(self._h, 1 if on else 0))
 
-    def setDataflow(self, on: bool = True):
-        """Explicit local steps of an FP32 solver in dataflow launches
-        (bs_set_dataflow): no launch or barrier between stages, the same
-        states."""
-        check(N.lib().bs_set_dataflow(self._h, 1 if on else 0))
-
     def enableTiming(self, on=True):
         """HIP-event timing of the solver's kernels: True (or "launch") times
         every launch, without graphs; "region" times whole batches of

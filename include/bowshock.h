@@ -309,17 +309,6 @@ bs_status bs_iterate_anderson(bs_solver *s, bs_anderson *a, int32_t count, doubl
  * reference, whose WebGPU engine encodes command buffers instead. */
 bs_status bs_enable_graphs(bs_solver *s, int32_t enable);
 
-/* No counterpart in the reference. Explicit local steps of an FP32 solver
- * (without a fitted shock) in dataflow launches: each launch runs up to a
- * status batch of steps, every (stage, tile) pair a work item taken in
- * order from a device counter, a tile's stage waiting only until its own
- * and its eight neighbours' previous stage have been stored, with no
- * barrier or launch between stages. The states are bitwise those of one
- * launch per stage. A wait that does not complete returns
- * BS_ERR_EXECUTION and turns dataflow launches off for the solver. Off by
- * default; an FP64 solver refuses. */
-bs_status bs_set_dataflow(bs_solver *s, int32_t enable);
-
 /* Mean device time per launch of the solver's kernels since the last call,
  * for throughput reporting: kernel-name index (0 stage, 1 implicit prepare,
  * 2 implicit relaxation, whose last launch of a step also applies the
