@@ -231,3 +231,29 @@ def train(ctx: Context, features: np.ndarray, targets: np.ndarray, *, steps: int
         if log and (s % 500 == 0 or s == steps - 1):
             log(s, loss)
     return WarmStart(net, xs, ys)
+
+
+def time_gemms(ctx: Context, batch: int = 256, reps: int = 50, inputs: int = 9, hidden: int = 512, depth: int = 4,
+               outputs: int = 12800):
+    """Device time per forward GEMM of each layer of the warm-start network
+    (bs_mlp_time_gemm, HIP events around `reps` back-to-back launches):
+    [(layer, microseconds, TFLOP/s)]."""
+    net = Mlp(ctx, inputs, hidden, depth, outputs, max_batch=batch)
+    net.init_params(0)
+    out = []
+    for layer in range(depth + 1):
+        ms, flops = net.time_gemm(layer, reps)
+        out.append((layer, ms * 1e3, flops / (ms * 1e-3) / 1e12))
+    net.destroy()
+    return out
+
+
+if __name__ == "__main__":
+    import argparse
+
+    ap = argparse.ArgumentParser(description="Time the warm-start network's forward GEMMs on the GPU.")
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--reps", type=int, default=50)
+    args = ap.parse_args()
+    for layer, us, tflops in time_gemms(Context(0), args.batch, args.reps):
+        print(f"batch {args.batch} layer {layer}: {us:.2f} us, {tflops:.1f} TFLOP/s", flush=True)
