@@ -618,7 +618,7 @@ bs_status batch_encode_implicit(bs_batch* b, int sweeps) {
     const int init = done == 0;
     const UpdateArgsT<T>* u = done + nh == halves ? upd : nullptr;
     const int pair = timing_begin(b, 2);
-    hipLaunchKernelGGL(kernels[form][nh - 1], dim3(b->n_relax), dim3(RTX * RTY), 0, b->stream, rel, b->work_relax,
+    hipLaunchKernelGGL(kernels[form][nh - 1], dim3(b->n_relax), dim3(RNT), 0, b->stream, rel, b->work_relax,
                        colour, init, u);
     timing_end(b, pair);
     colour = (colour + nh) & 1;

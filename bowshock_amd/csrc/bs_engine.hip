@@ -1167,7 +1167,7 @@ static bs_status encode_implicit(bs_solver* s, const StateDevT<float>& res, cons
     const int nh = halves - done < 3 ? halves - done : 3;
     const int init = done == 0, last = fused && done + nh == halves;
     hipEvent_t* ev = timing_begin(s, 2);
-    hipLaunchKernelGGL(kernels[form][nh - 1], rgrid, dim3(RTX * RTY), 0, s->stream, R, colour, init, U, last);
+    hipLaunchKernelGGL(kernels[form][nh - 1], rgrid, dim3(RNT), 0, s->stream, R, colour, init, U, last);
     timing_end(s, ev);
     colour = (colour + nh) & 1;
     done += nh;

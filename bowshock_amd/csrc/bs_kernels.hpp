@@ -852,6 +852,18 @@ __device__ __forceinline__ void advance_control(const UpdateArgsT<S>& U) {
 #define BS_RTY 16
 #endif
 constexpr int RTX = BS_RTX, RTY = BS_RTY;
+// Threads per relaxation block; every loop of the tile is strided by it.
+// The half-sweeps of a 32 x 16 tile update 360, 306 and 256 cells, so 512
+// threads leave lanes idle, yet fewer threads are slower (1280 x 640,
+// three half-sweeps: 384 threads 49.0 us, 448 34.0, 512 30.8) and more
+// gain nothing beyond noise (768 29.9, 1024 30.2; the 500-rocket
+// population within 3 %): the tile is latency-bound, and waves in flight
+// matter more than busy lanes.
+#ifndef BS_RNT
+#define BS_RNT (BS_RTX * BS_RTY)
+#endif
+constexpr int RNT = BS_RNT;
+static_assert(RNT % 64 == 0 && RNT <= 1024, "relaxation block size");
 
 struct RelaxCtx {
   int i0, j0;  // global index of LDS window cell (0, 0)
@@ -862,7 +874,7 @@ __device__ __forceinline__ void relax_tile_body(const RelaxArgsT<T>& A, int bx, 
                                                 const UpdateArgsT<T>& U, bool fuse_update) {
   static_assert(LIN || std::is_same<T, float>::value, "an FP64 state relaxes with the linearized term");
   const bool fuse = fuse_update;
-  constexpr int WX = RTX + 2 * NH, WY = RTY + 2 * NH, NT = RTX * RTY;
+  constexpr int WX = RTX + 2 * NH, WY = RTY + 2 * NH, NT = RNT;
   // Window cells [0, WX) x [0, WY) hold q, dQ, the residual and the
   // diagonal; the cells updated by some half-sweep lie in [1, WX - 1) x
   // [1, WY - 1), and their faces (normal, area, lambda A) are staged too,
@@ -1066,7 +1078,7 @@ __device__ __forceinline__ void relax_tile_body(const RelaxArgsT<T>& A, int bx, 
 }
 
 template <typename T, int NH, bool LIN>
-__global__ __launch_bounds__(RTX * RTY) void relax_tile_kernel(RelaxArgsT<T> A, int colour0, int init_red,
+__global__ __launch_bounds__(RNT) void relax_tile_kernel(RelaxArgsT<T> A, int colour0, int init_red,
                                                              UpdateArgsT<T> U, int fuse_update) {
   // One-dimensional grid, tiles placed per XCD as in tile_kernel.
   const int tiles_x = (A.G.ni + RTX - 1) / RTX;
@@ -1075,7 +1087,7 @@ __global__ __launch_bounds__(RTX * RTY) void relax_tile_kernel(RelaxArgsT<T> A, 
 }
 
 template <typename T, int NH, bool LIN>
-__global__ __launch_bounds__(RTX * RTY) void batch_relax_tile_kernel(const RelaxArgsT<T>* __restrict__ args,
+__global__ __launch_bounds__(RNT) void batch_relax_tile_kernel(const RelaxArgsT<T>* __restrict__ args,
                                                                      const WorkItem* __restrict__ work, int colour0,
                                                                      int init_red, const UpdateArgsT<T>* upd) {
   const WorkItem w = work[xcd_tile(blockIdx.x, gridDim.x)];
