@@ -600,26 +600,31 @@ bs_status batch_encode_implicit(bs_batch* b, int sweeps) {
   // half-sweeps after it, up to three per launch; the last launch writes
   // the updated states (from the FP64 state in HBM on an FP64 batch).
   const UpdateArgsT<T>* upd = static_cast<const UpdateArgsT<T>*>(b->update_args) + (size_t)cur * b->count;
-  using K = void (*)(const RelaxArgsT<T>*, const WorkItem*, int, int, const UpdateArgsT<T>*);
-  K kernels[2][3];
-  if constexpr (std::is_same<T, float>::value) {
-    kernels[0][0] = batch_relax_tile_kernel<T, 1, false>;
-    kernels[0][1] = batch_relax_tile_kernel<T, 2, false>;
-    kernels[0][2] = batch_relax_tile_kernel<T, 3, false>;
-  } else {
-    kernels[0][0] = kernels[0][1] = kernels[0][2] = nullptr;  // an FP64 state relaxes linearized only
+  // [form][half-sweeps - 1][first launch]
+  using K = void (*)(const RelaxArgsT<T>*, const WorkItem*, int, const UpdateArgsT<T>*);
+  K kernels[2][3][2] = {};
+  if constexpr (std::is_same<T, float>::value) {  // an FP64 state relaxes linearized only
+    kernels[0][0][0] = batch_relax_tile_kernel<T, 1, false, false>;
+    kernels[0][1][0] = batch_relax_tile_kernel<T, 2, false, false>;
+    kernels[0][2][0] = batch_relax_tile_kernel<T, 3, false, false>;
+    kernels[0][0][1] = batch_relax_tile_kernel<T, 1, false, true>;
+    kernels[0][1][1] = batch_relax_tile_kernel<T, 2, false, true>;
+    kernels[0][2][1] = batch_relax_tile_kernel<T, 3, false, true>;
   }
-  kernels[1][0] = batch_relax_tile_kernel<T, 1, true>;
-  kernels[1][1] = batch_relax_tile_kernel<T, 2, true>;
-  kernels[1][2] = batch_relax_tile_kernel<T, 3, true>;
+  kernels[1][0][0] = batch_relax_tile_kernel<T, 1, true, false>;
+  kernels[1][1][0] = batch_relax_tile_kernel<T, 2, true, false>;
+  kernels[1][2][0] = batch_relax_tile_kernel<T, 3, true, false>;
+  kernels[1][0][1] = batch_relax_tile_kernel<T, 1, true, true>;
+  kernels[1][1][1] = batch_relax_tile_kernel<T, 2, true, true>;
+  kernels[1][2][1] = batch_relax_tile_kernel<T, 3, true, true>;
   const int form = (b->relax_linear || !std::is_same<T, float>::value) ? 1 : 0;
   for (int done = 0, colour = 1, halves = 2 * sweeps - 1; done < halves && b->n_relax;) {
     const int nh = halves - done < b->relax_fusion ? halves - done : b->relax_fusion;
     const int init = done == 0;
     const UpdateArgsT<T>* u = done + nh == halves ? upd : nullptr;
     const int pair = timing_begin(b, 2);
-    hipLaunchKernelGGL(kernels[form][nh - 1], dim3(b->n_relax), dim3(RNT), 0, b->stream, rel, b->work_relax,
-                       colour, init, u);
+    hipLaunchKernelGGL(kernels[form][nh - 1][init], dim3(b->n_relax), dim3(RNT), 0, b->stream, rel, b->work_relax,
+                       colour, u);
     timing_end(b, pair);
     colour = (colour + nh) & 1;
     done += nh;

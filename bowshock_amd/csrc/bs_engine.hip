@@ -1149,25 +1149,30 @@ static bs_status encode_implicit(bs_solver* s, const StateDevT<float>& res, cons
   const bool fused = !dq_only;
   // The 2 sweeps - 1 half-sweeps after the first red one, up to three per
   // launch (relax_tile_kernel), starting with black.
-  using K = void (*)(RelaxArgsT<T>, int, int, UpdateArgsT<T>, int);
-  K kernels[2][3];
-  if constexpr (std::is_same<T, float>::value) {
-    kernels[0][0] = relax_tile_kernel<T, 1, false>;
-    kernels[0][1] = relax_tile_kernel<T, 2, false>;
-    kernels[0][2] = relax_tile_kernel<T, 3, false>;
-  } else {
-    kernels[0][0] = kernels[0][1] = kernels[0][2] = nullptr;  // an FP64 state relaxes linearized only
+  // [form][half-sweeps - 1][first launch]
+  using K = void (*)(RelaxArgsT<T>, int, UpdateArgsT<T>, int);
+  K kernels[2][3][2] = {};
+  if constexpr (std::is_same<T, float>::value) {  // an FP64 state relaxes linearized only
+    kernels[0][0][0] = relax_tile_kernel<T, 1, false, false>;
+    kernels[0][1][0] = relax_tile_kernel<T, 2, false, false>;
+    kernels[0][2][0] = relax_tile_kernel<T, 3, false, false>;
+    kernels[0][0][1] = relax_tile_kernel<T, 1, false, true>;
+    kernels[0][1][1] = relax_tile_kernel<T, 2, false, true>;
+    kernels[0][2][1] = relax_tile_kernel<T, 3, false, true>;
   }
-  kernels[1][0] = relax_tile_kernel<T, 1, true>;
-  kernels[1][1] = relax_tile_kernel<T, 2, true>;
-  kernels[1][2] = relax_tile_kernel<T, 3, true>;
+  kernels[1][0][0] = relax_tile_kernel<T, 1, true, false>;
+  kernels[1][1][0] = relax_tile_kernel<T, 2, true, false>;
+  kernels[1][2][0] = relax_tile_kernel<T, 3, true, false>;
+  kernels[1][0][1] = relax_tile_kernel<T, 1, true, true>;
+  kernels[1][1][1] = relax_tile_kernel<T, 2, true, true>;
+  kernels[1][2][1] = relax_tile_kernel<T, 3, true, true>;
   const int form = (s->relax_linear || !std::is_same<T, float>::value) ? 1 : 0;
   dim3 rgrid(((s->ni + RTX - 1) / RTX) * ((s->nj + RTY - 1) / RTY));
   for (int done = 0, colour = 1, halves = 2 * sweeps - 1; done < halves;) {
     const int nh = halves - done < 3 ? halves - done : 3;
     const int init = done == 0, last = fused && done + nh == halves;
     hipEvent_t* ev = timing_begin(s, 2);
-    hipLaunchKernelGGL(kernels[form][nh - 1], rgrid, dim3(RNT), 0, s->stream, R, colour, init, U, last);
+    hipLaunchKernelGGL(kernels[form][nh - 1][init], rgrid, dim3(RNT), 0, s->stream, R, colour, U, last);
     timing_end(s, ev);
     colour = (colour + nh) & 1;
     done += nh;

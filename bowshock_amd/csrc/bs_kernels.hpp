@@ -864,28 +864,37 @@ constexpr int RTX = BS_RTX, RTY = BS_RTY;
 #endif
 constexpr int RNT = BS_RNT;
 static_assert(RNT % 64 == 0 && RNT <= 1024, "relaxation block size");
+// Waves per SIMD the relaxation's registers must allow: three 512-thread
+// tiles to a CU (their LDS allows three) need six, 80 VGPRs at most.
+#ifndef BS_RELAX_WAVES
+#define BS_RELAX_WAVES 6
+#endif
+constexpr int RELAX_WAVES = BS_RELAX_WAVES;
 
 struct RelaxCtx {
   int i0, j0;  // global index of LDS window cell (0, 0)
 };
 
-template <typename T, int NH, bool LIN>
-__device__ __forceinline__ void relax_tile_body(const RelaxArgsT<T>& A, int bx, int by, int colour0, int init_red,
+// INIT: the launch forms the first red half-sweep (a template parameter,
+// so that an instance keeps registers for only one of the two stagings).
+template <typename T, int NH, bool LIN, bool INIT>
+__device__ __forceinline__ void relax_tile_body(const RelaxArgsT<T>& A, int bx, int by, int colour0,
                                                 const UpdateArgsT<T>& U, bool fuse_update) {
   static_assert(LIN || std::is_same<T, float>::value, "an FP64 state relaxes with the linearized term");
   const bool fuse = fuse_update;
   constexpr int WX = RTX + 2 * NH, WY = RTY + 2 * NH, NT = RNT;
-  // Window cells [0, WX) x [0, WY) hold q, dQ, the residual and the
-  // diagonal; the cells updated by some half-sweep lie in [1, WX - 1) x
-  // [1, WY - 1), and their faces (normal, area, lambda A) are staged too,
-  // so the half-sweeps read LDS only. With init_red the first red
-  // half-sweep (dQ = R / D on red cells, 0 on black, as the prepare
-  // kernel defines it) is formed here instead of read.
+  // Window cells [0, WX) x [0, WY) hold q and dQ; the cells updated by
+  // some half-sweep lie in [1, WX - 1) x [1, WY - 1), and their faces
+  // (normal, area, lambda A) are staged too. The residual and diagonal of
+  // the cells a thread updates in each half-sweep go to its registers
+  // instead of LDS: 16.7 KB less LDS at 32 x 16 and three half-sweeps
+  // (50,688 B instead of 67,408), three tiles to a CU instead of two. With
+  // INIT the first red half-sweep (dQ = R / D on red cells, 0 on
+  // black, as the prepare kernel defines it) is formed here instead of
+  // read.
   constexpr int CX = WX - 2, CY = WY - 2;
   __shared__ float sq[4][WY][WX];
   __shared__ float sdq[4][WY][WX];
-  __shared__ float sres[4][WY][WX];
-  __shared__ float sdiag[WY][WX];
   __shared__ float sfi[4][CY][CX + 1];  // i-face x (between cells x - 1 and x) of row y: [y - 1][x - 1]
   __shared__ float sfj[4][CY + 1][CX];  // j-face y of column x: [y - 1][x - 1]
   const GridDevT<T>& G = A.G;
@@ -908,12 +917,46 @@ __device__ __forceinline__ void relax_tile_body(const RelaxArgsT<T>& A, int bx, 
     if (ok_c[k]) {
       const int c = gi + gj * ni;
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        cq[k][q] = (float)at(A.q.q[q], c);
-        cr[k][q] = at(A.res[q], c);
-        if (!init_red) cdq[k][q] = at(A.dq[q], c);
+      for (int q = 0; q < 4; q++) cq[k][q] = (float)at(A.q.q[q], c);
+      if constexpr (INIT) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) cr[k][q] = at(A.res[q], c);
+        cd[k] = at(A.diag, c);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; q++) cdq[k][q] = at(A.dq[q], c);
       }
-      cd[k] = at(A.diag, c);
+    }
+  }
+  // Half-sweep s updates the cells of one colour in the tile grown by
+  // r = NH - 1 - s cells, enumerated row by row, (RTX + 2 r) / 2 to a row;
+  // thread tid takes the cells tid, tid + NT, ... of that list (HK at
+  // most). hs_cell places entry t: window (wx, wy), grid (gi, gj).
+  constexpr int HS0 = (RTX + 2 * (NH - 1)) / 2 * (RTY + 2 * (NH - 1)), HK = (HS0 + NT - 1) / NT;
+  auto hs_cell = [&](int s, int t, int& wx, int& wy, int& gi, int& gj) {
+    const int colour = (colour0 + s) & 1;
+    const int r = NH - 1 - s;
+    const int half_w = (RTX + 2 * r) / 2;
+    const int row = t / half_w, k = t % half_w;
+    wy = NH - r + row;
+    gj = j0 + wy;
+    const int gx0 = i0 + NH - r;
+    gi = gx0 + 2 * k + ((colour - gx0 - gj) & 1);
+    wx = gi - i0;
+    return t < half_w * (RTY + 2 * r) && gi >= 0 && gi < ni && gj >= 0 && gj < nj;
+  };
+  float hres[NH][HK][4], hdiag[NH][HK];
+#pragma unroll
+  for (int s = 0; s < NH; s++) {
+#pragma unroll
+    for (int k = 0; k < HK; k++) {
+      int wx, wy, gi, gj;
+      if (hs_cell(s, tid + k * NT, wx, wy, gi, gj)) {
+        const int c = gi + gj * ni;
+#pragma unroll
+        for (int q = 0; q < 4; q++) hres[s][k][q] = at(A.res[q], c);
+        hdiag[s][k] = at(A.diag, c);
+      }
     }
   }
 #pragma unroll
@@ -949,12 +992,8 @@ __device__ __forceinline__ void relax_tile_body(const RelaxArgsT<T>& A, int bx, 
     const int wx = e % WX, wy = e / WX;
     const int gi = i0 + wx, gj = j0 + wy;
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      sq[q][wy][wx] = cq[k][q];
-      sres[q][wy][wx] = cr[k][q];
-    }
-    sdiag[wy][wx] = cd[k];
-    if (init_red) {
+    for (int q = 0; q < 4; q++) sq[q][wy][wx] = cq[k][q];
+    if constexpr (INIT) {
       const bool red = ((gi + gj) & 1) == 0;
 #pragma unroll
       for (int q = 0; q < 4; q++) sdq[q][wy][wx] = red ? bs_div(cr[k][q], cd[k]) : 0.f;
@@ -1019,18 +1058,10 @@ __device__ __forceinline__ void relax_tile_body(const RelaxArgsT<T>& A, int bx, 
   };
 #pragma unroll
   for (int s = 0; s < NH; s++) {
-    const int colour = (colour0 + s) & 1;
-    const int r = NH - 1 - s;                 // halo of this half-sweep's region
-    const int rw = RTX + 2 * r, rh = RTY + 2 * r;
-    const int half_w = rw / 2;
-    for (int t = tid; t < half_w * rh; t += NT) {
-      const int row = t / half_w, k = t % half_w;
-      const int wy = NH - r + row;
-      const int gj = j0 + wy;
-      const int gx0 = i0 + NH - r;
-      const int gi = gx0 + 2 * k + ((colour - gx0 - gj) & 1);
-      const int wx = gi - i0;
-      if (gi < 0 || gi >= ni || gj < 0 || gj >= nj) continue;
+#pragma unroll
+    for (int hk = 0; hk < HK; hk++) {
+      int wx, wy, gi, gj;
+      if (!hs_cell(s, tid + hk * NT, wx, wy, gi, gj)) continue;
       float sum[4] = {0.f, 0.f, 0.f, 0.f};
       const int cx = wx - 1, cy = wy - 1;
       if (gi > 0) term(wx - 1, wy, -sfi[0][cy][cx], -sfi[1][cy][cx], sfi[2][cy][cx], sfi[3][cy][cx], sum);
@@ -1039,9 +1070,9 @@ __device__ __forceinline__ void relax_tile_body(const RelaxArgsT<T>& A, int bx, 
       if (gj > 0) term(wx, wy - 1, -sfj[0][cy][cx], -sfj[1][cy][cx], sfj[2][cy][cx], sfj[3][cy][cx], sum);
       if (gj < nj - 1)
         term(wx, wy + 1, sfj[0][cy + 1][cx], sfj[1][cy + 1][cx], sfj[2][cy + 1][cx], sfj[3][cy + 1][cx], sum);
-      const float id = bs_rcp(sdiag[wy][wx]);
+      const float id = bs_rcp(hdiag[s][hk]);
 #pragma unroll
-      for (int q = 0; q < 4; q++) sdq[q][wy][wx] = (sres[q][wy][wx] - sum[q]) * id;
+      for (int q = 0; q < 4; q++) sdq[q][wy][wx] = (hres[s][hk][q] - sum[q]) * id;
     }
     __syncthreads();
   }
@@ -1077,22 +1108,22 @@ __device__ __forceinline__ void relax_tile_body(const RelaxArgsT<T>& A, int bx, 
   if (fuse && bx == 0 && by == 0 && tid == 0) advance_control(U);
 }
 
-template <typename T, int NH, bool LIN>
-__global__ __launch_bounds__(RNT) void relax_tile_kernel(RelaxArgsT<T> A, int colour0, int init_red,
-                                                             UpdateArgsT<T> U, int fuse_update) {
+template <typename T, int NH, bool LIN, bool INIT>
+__global__ __launch_bounds__(RNT) __attribute__((amdgpu_waves_per_eu(RELAX_WAVES, 8))) void relax_tile_kernel(
+    RelaxArgsT<T> A, int colour0, UpdateArgsT<T> U, int fuse_update) {
   // One-dimensional grid, tiles placed per XCD as in tile_kernel.
   const int tiles_x = (A.G.ni + RTX - 1) / RTX;
   const int t = xcd_tile(blockIdx.x, gridDim.x);
-  relax_tile_body<T, NH, LIN>(A, t % tiles_x, t / tiles_x, colour0, init_red, U, fuse_update != 0);
+  relax_tile_body<T, NH, LIN, INIT>(A, t % tiles_x, t / tiles_x, colour0, U, fuse_update != 0);
 }
 
-template <typename T, int NH, bool LIN>
-__global__ __launch_bounds__(RNT) void batch_relax_tile_kernel(const RelaxArgsT<T>* __restrict__ args,
-                                                                     const WorkItem* __restrict__ work, int colour0,
-                                                                     int init_red, const UpdateArgsT<T>* upd) {
+template <typename T, int NH, bool LIN, bool INIT>
+__global__ __launch_bounds__(RNT) __attribute__((amdgpu_waves_per_eu(RELAX_WAVES, 8))) void batch_relax_tile_kernel(
+    const RelaxArgsT<T>* __restrict__ args, const WorkItem* __restrict__ work, int colour0,
+    const UpdateArgsT<T>* upd) {
   const WorkItem w = work[xcd_tile(blockIdx.x, gridDim.x)];
-  if (upd) relax_tile_body<T, NH, LIN>(args[w.args], w.bx, w.by, colour0, init_red, upd[w.args], true);
-  else relax_tile_body<T, NH, LIN>(args[w.args], w.bx, w.by, colour0, init_red, UpdateArgsT<T>{}, false);
+  if (upd) relax_tile_body<T, NH, LIN, INIT>(args[w.args], w.bx, w.by, colour0, upd[w.args], true);
+  else relax_tile_body<T, NH, LIN, INIT>(args[w.args], w.bx, w.by, colour0, UpdateArgsT<T>{}, false);
 }
 
 
