@@ -379,6 +379,15 @@ __device__ __forceinline__ void tile_compute(const TileArgsT<T>& A, int bx, int 
   __shared__ T xi[XP][TY][TX + 1];    // i-face area vector (stage) or lambda A (implicit)
   __shared__ T xj[XP][TY + 1][TX];
   __shared__ double red[NT / 64];
+  // The FP32 implicit tiles with a shared reconstruction form each cell's
+  // limiter coefficient once per direction (shared_coeff) instead of once
+  // per face side: i-coefficients of window cells x in [1, TX + 3), rows
+  // [2, TY + 2), at ci[y - 2][x - 1]; j-coefficients of columns
+  // [2, TX + 2), rows [1, TY + 3), at cj[y - 1][x - 2]. 2,368 B of LDS at
+  // 32 x 8, which keeps eight tiles to a CU.
+  constexpr bool COEF = MODE == TILE_IMPLICIT && REC != PER_VARIABLE && std::is_same<T, float>::value;
+  __shared__ T ci[COEF ? TY : 1][COEF ? TX + 2 : 1];
+  __shared__ T cj[COEF ? TY + 2 : 1][COEF ? TX : 1];
 
   const GridDevT<T>& G = A.G;
   const int ni = G.ni, nj = G.nj;
@@ -438,6 +447,22 @@ __device__ __forceinline__ void tile_compute(const TileArgsT<T>& A, int bx, int 
   auto prim_at = [&](int lj, int li) -> Prim<T> {
     return {sp[0][lj][li], sp[1][lj][li], sp[2][lj][li], sp[3][lj][li]};
   };
+  if constexpr (COEF) {
+    // Window cells outside what the tile staged give values no face reads:
+    // a face reads the coefficients of its two cells, whose stencils are
+    // the four cells face_flux reads.
+    constexpr int NI = TY * (TX + 2), NJ = (TY + 2) * TX;
+    for (int e = tid; e < NI + NJ; e += NT) {
+      if (e < NI) {
+        const int y = e / (TX + 2), x = e % (TX + 2);
+        ci[y][x] = shared_coeff<T, REC>(prim_at(y + 2, x), prim_at(y + 2, x + 1), prim_at(y + 2, x + 2), g);
+      } else {
+        const int y = (e - NI) / TX, x = (e - NI) % TX;
+        cj[y][x] = shared_coeff<T, REC>(prim_at(y, x + 2), prim_at(y + 1, x + 2), prim_at(y + 2, x + 2), g);
+      }
+    }
+    __syncthreads();
+  }
 
   // Phase 2: faces. Face f of direction d at local (fx, fy). Inner tiles
   // (IN) have no boundary face and no face outside the grid, so their
@@ -460,8 +485,12 @@ __device__ __forceinline__ void tile_compute(const TileArgsT<T>& A, int bx, int 
         f = {b[0], b[1], b[2], b[3]};
       } else {
         const int wall = side >= 0 && A.B->kind[side < 0 ? 0 : side] == WALL ? (side == IMIN ? -1 : 1) : 0;
-        f = face_flux<T, REC, FLUX>(LIM, prim_at(fy + 2, fx), l0, r0, prim_at(fy + 2, fx + 3), nx, ny,
-                                        wall, g);
+        if constexpr (COEF)
+          f = face_flux_coeff<T, REC, FLUX>(prim_at(fy + 2, fx), l0, r0, prim_at(fy + 2, fx + 3), ci[fy][fx],
+                                            ci[fy][fx + 1], nx, ny, wall, g);
+        else
+          f = face_flux<T, REC, FLUX>(LIM, prim_at(fy + 2, fx), l0, r0, prim_at(fy + 2, fx + 3), nx, ny,
+                                      wall, g);
       }
       if (MODE == TILE_IMPLICIT) {
         T rl = gi > 0 ? bs_abs(l0.u * nx + l0.v * ny) + bs_sqrt(bs_div(g * l0.p, l0.rho)) : T(0.);
@@ -491,8 +520,12 @@ __device__ __forceinline__ void tile_compute(const TileArgsT<T>& A, int bx, int 
         f = {b[0], b[1], b[2], b[3]};
       } else {
         const int wall = side >= 0 && A.B->kind[side < 0 ? 0 : side] == WALL ? (side == JMIN ? -1 : 1) : 0;
-        f = face_flux<T, REC, FLUX>(LIM, prim_at(fy, fx + 2), l0, r0, prim_at(fy + 3, fx + 2), nx, ny,
-                                        wall, g);
+        if constexpr (COEF)
+          f = face_flux_coeff<T, REC, FLUX>(prim_at(fy, fx + 2), l0, r0, prim_at(fy + 3, fx + 2), cj[fy][fx],
+                                            cj[fy + 1][fx], nx, ny, wall, g);
+        else
+          f = face_flux<T, REC, FLUX>(LIM, prim_at(fy, fx + 2), l0, r0, prim_at(fy + 3, fx + 2), nx, ny,
+                                      wall, g);
       }
       if (MODE == TILE_IMPLICIT) {
         T rl = gj > 0 ? bs_abs(l0.u * nx + l0.v * ny) + bs_sqrt(bs_div(g * l0.p, l0.rho)) : T(0.);
@@ -844,7 +877,10 @@ __device__ __forceinline__ void advance_control(const UpdateArgsT<S>& U) {
 // The relaxation tile: 32 x 16 measured fastest (1280 x 640, three
 // half-sweeps: 30.9 us; 64 x 8 31.7, 32 x 8 33.4, 64 x 16 37.7, 64 x 4
 // 51.4; the population of 500 rockets 1.5 % faster than at 64 x 8): its
-// 38 x 22 window stages fewer halo cells per cell than 70 x 14.
+// 38 x 22 window stages fewer halo cells per cell than 70 x 14. With the
+// residual and diagonal in registers (three 32 x 16 tiles to a CU), 32 x 8
+// (five to a CU) measured 28.0 us against 26.6 and the 500-rocket
+// population 136.3 against 143.2 solves/s.
 #ifndef BS_RTX
 #define BS_RTX 32
 #endif

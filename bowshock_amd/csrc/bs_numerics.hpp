@@ -95,20 +95,11 @@ __device__ __forceinline__ T limit(int kind, T a, T b) {
   }
 }
 
-// reconstruction.ts:50-64 (per variable), 70-103 (shared), 110-151 (shared
-// scaled): the centre cell's state at its +1 (side > 0) or -1 face.
+// reconstruction.ts:70-103 (shared), 110-151 (shared scaled): the one
+// limiter coefficient of the centre cell c between neighbours p and n,
+// which its two faces in that direction share (shared_apply).
 template <typename T, int REC>
-__device__ __forceinline__ Prim<T> reconstruct(int lim, const Prim<T>& p, const Prim<T>& c, const Prim<T>& n,
-                                               T side, T g) {
-  Prim<T> o;
-  if (REC == PER_VARIABLE) {
-    T h = T(0.5) * side;
-    o.rho = c.rho + h * limit<T>(lim, c.rho - p.rho, n.rho - c.rho);
-    o.u = c.u + h * limit<T>(lim, c.u - p.u, n.u - c.u);
-    o.v = c.v + h * limit<T>(lim, c.v - p.v, n.v - c.v);
-    o.p = c.p + h * limit<T>(lim, c.p - p.p, n.p - c.p);
-    return o;
-  }
+__device__ __forceinline__ T shared_coeff(const Prim<T>& p, const Prim<T>& c, const Prim<T>& n, T g) {
   T ar = c.rho - p.rho, au = c.u - p.u, av = c.v - p.v, ap = c.p - p.p;
   T br = n.rho - c.rho, bu = n.u - c.u, bv = n.v - c.v, bp = n.p - c.p;
   if (REC == SHARED_SCALED) {
@@ -123,13 +114,37 @@ __device__ __forceinline__ Prim<T> reconstruct(int lim, const Prim<T>& p, const 
   T aa = ar * ar + au * au + av * av + ap * ap;
   T bb = br * br + bu * bu + bv * bv + bp * bp;
   T f = bs_div(T(2) * (ab + e), aa + bb + T(2) * e);
-  f = f < T(0) ? T(0) : f;
+  return f < T(0) ? T(0) : f;
+}
+
+// The shared reconstructions' state of cell c at its +1 (side > 0) or -1
+// face from its coefficient f.
+template <typename T>
+__device__ __forceinline__ Prim<T> shared_apply(const Prim<T>& p, const Prim<T>& c, const Prim<T>& n, T side, T f) {
+  Prim<T> o;
   T k = T(0.25) * side * f;
   o.rho = c.rho + k * (n.rho - p.rho);
   o.u = c.u + k * (n.u - p.u);
   o.v = c.v + k * (n.v - p.v);
   o.p = c.p + k * (n.p - p.p);
   return o;
+}
+
+// reconstruction.ts:50-64 (per variable), 70-103 (shared), 110-151 (shared
+// scaled): the centre cell's state at its +1 (side > 0) or -1 face.
+template <typename T, int REC>
+__device__ __forceinline__ Prim<T> reconstruct(int lim, const Prim<T>& p, const Prim<T>& c, const Prim<T>& n,
+                                               T side, T g) {
+  if (REC == PER_VARIABLE) {
+    Prim<T> o;
+    T h = T(0.5) * side;
+    o.rho = c.rho + h * limit<T>(lim, c.rho - p.rho, n.rho - c.rho);
+    o.u = c.u + h * limit<T>(lim, c.u - p.u, n.u - c.u);
+    o.v = c.v + h * limit<T>(lim, c.v - p.v, n.v - c.v);
+    o.p = c.p + h * limit<T>(lim, c.p - p.p, n.p - c.p);
+    return o;
+  }
+  return shared_apply(p, c, n, side, shared_coeff<T, REC>(p, c, n, g));
 }
 
 // flux.ts:34-36 energyOf.
@@ -209,10 +224,30 @@ __device__ __forceinline__ FaceState<T> rotate_in(const Prim<T>& w, T nx, T ny) 
 // `wall` is -1 for a wall face on a minimum side, +1 on a maximum side,
 // else 0.
 template <typename T, int REC, int FLUX>
+__device__ __forceinline__ Vec4<T> face_flux_of(int lim, Prim<T> left, Prim<T> right, const Prim<T>& l0,
+                                                const Prim<T>& r0, T nx, T ny, int wall, T g);
+
+template <typename T, int REC, int FLUX>
 __device__ __forceinline__ Vec4<T> face_flux(int lim, const Prim<T>& ll, const Prim<T>& l0, const Prim<T>& r0,
                                              const Prim<T>& rr, T nx, T ny, int wall, T g) {
-  Prim<T> left = reconstruct<T, REC>(lim, ll, l0, r0, T(1), g);
-  Prim<T> right = reconstruct<T, REC>(lim, l0, r0, rr, T(-1), g);
+  return face_flux_of<T, REC, FLUX>(lim, reconstruct<T, REC>(lim, ll, l0, r0, T(1), g),
+                                    reconstruct<T, REC>(lim, l0, r0, rr, T(-1), g), l0, r0, nx, ny, wall, g);
+}
+
+// face_flux with the shared reconstructions' coefficients of the two cells
+// (fl of l0, fr of r0) already formed.
+template <typename T, int REC, int FLUX>
+__device__ __forceinline__ Vec4<T> face_flux_coeff(const Prim<T>& ll, const Prim<T>& l0, const Prim<T>& r0,
+                                                   const Prim<T>& rr, T fl, T fr, T nx, T ny, int wall, T g) {
+  static_assert(REC != PER_VARIABLE, "per-variable reconstructions have no shared coefficient");
+  return face_flux_of<T, REC, FLUX>(0, shared_apply(ll, l0, r0, T(1), fl), shared_apply(l0, r0, rr, T(-1), fr), l0,
+                                    r0, nx, ny, wall, g);
+}
+
+// The flux from the two reconstructed face states.
+template <typename T, int REC, int FLUX>
+__device__ __forceinline__ Vec4<T> face_flux_of(int lim, Prim<T> left, Prim<T> right, const Prim<T>& l0,
+                                                const Prim<T>& r0, T nx, T ny, int wall, T g) {
   // Per variable with minmod (or first order) the face value of a positive
   // quantity lies between the cell's value and the mean of the cell and its
   // neighbour, so positive density and pressure stay positive and the
