@@ -10,8 +10,8 @@
 
 namespace bs {
 
-// Tile classes per state precision: an FP64 tile is half as tall (its
-// threads hold twice the registers), as the single FP64 solver's.
+// Tile classes per state precision; the large FP64 tile is the single
+// FP64 solver's.
 template <typename T>
 struct BatchTile {
   static constexpr int lx = 32, ly = 8;  // large tile (the single FP32 solver's)
@@ -19,7 +19,7 @@ struct BatchTile {
 };
 template <>
 struct BatchTile<double> {
-  static constexpr int lx = 64, ly = 4;
+  static constexpr int lx = 32, ly = 8;
   static constexpr int sx = 32, sy = 4;
 };
 

@@ -48,15 +48,17 @@ constexpr int GRAPH_STEPS = 12;  // steps per captured graph (a multiple of 3)
 constexpr int TX = BS_TILE_X, TY = BS_TILE_Y;
 static_assert(TX * TY <= 1024 && TX * TY >= TX + TY && (TX * TY) % 64 == 0, "tile shape");
 
-// Tile per state precision: FP64 halves the tile height so the doubled
-// register footprint of a thread still fits without spilling.
+// Tile per state precision. The FP64 tile has the FP32 tile's 256
+// threads; at 32 x 8 it stages fewer halo cells per cell than at 64 x 4
+// (1280 x 640, FP64 implicit prepare 49.7 against 54.5 us). Both hold
+// four FP64 tiles to a CU.
 template <typename T>
 struct Tile {
   static constexpr int x = TX, y = TY;
 };
 template <>
 struct Tile<double> {
-  static constexpr int x = 64, y = 4;
+  static constexpr int x = 32, y = 8;
 };
 
 template <typename T>

@@ -4,7 +4,8 @@ every thread of every tile: the state, metric, time-step and boundary
 arrays it reads and the arrays it writes, on grids whose last tiles are
 partial in both directions, with walls, face states and periodic sides,
 for the FP32 tile (32 x 8, also the batch's large FP32 tile), the FP64
-tile (64 x 4), the batch's small tile (32 x 4) and a 64 x 8 tile. The index arithmetic is
+tile (32 x 8, with the FP64 stage's far-face metric reads; also 64 x 4),
+the batch's small tile (32 x 4) and a 64 x 8 tile. The index arithmetic is
 the kernel's, restated; each access is checked against its array's length. Among the grids is the
 412 x 80 blunt-body C grid (walls on iMin and jMin) of the fitted
 workload."""
@@ -131,7 +132,7 @@ GRIDS = [
 
 
 @pytest.mark.parametrize("ni,nj,kinds", GRIDS)
-@pytest.mark.parametrize("TX,TY,fp64", [(32, 8, False), (64, 8, False), (64, 4, True), (32, 4, False)])
+@pytest.mark.parametrize("TX,TY,fp64", [(32, 8, False), (64, 8, False), (32, 8, True), (64, 4, True), (32, 4, False)])
 def test_every_index_of_the_tile_is_in_bounds(ni, nj, kinds, TX, TY, fp64):
     acc, blocks = tile_accesses(ni, nj, TX, TY, kinds, fp64)
     sizes = {"state": ni * nj, "cell": ni * nj, "iface": (ni + 1) * nj, "jface": ni * (nj + 1),
