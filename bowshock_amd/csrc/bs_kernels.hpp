@@ -379,13 +379,14 @@ __device__ __forceinline__ void tile_compute(const TileArgsT<T>& A, int bx, int 
   __shared__ T xi[XP][TY][TX + 1];    // i-face area vector (stage) or lambda A (implicit)
   __shared__ T xj[XP][TY + 1][TX];
   __shared__ double red[NT / 64];
-  // The FP32 implicit tiles with a shared reconstruction form each cell's
+  // The implicit tiles with a shared reconstruction form each cell's
   // limiter coefficient once per direction (shared_coeff) instead of once
   // per face side: i-coefficients of window cells x in [1, TX + 3), rows
   // [2, TY + 2), at ci[y - 2][x - 1]; j-coefficients of columns
-  // [2, TX + 2), rows [1, TY + 3), at cj[y - 1][x - 2]. 2,368 B of LDS at
-  // 32 x 8, which keeps eight tiles to a CU.
-  constexpr bool COEF = MODE == TILE_IMPLICIT && REC != PER_VARIABLE && std::is_same<T, float>::value;
+  // [2, TX + 2), rows [1, TY + 3), at cj[y - 1][x - 2]. At 32 x 8 that is
+  // 2,368 B of LDS in FP32 (eight tiles to a CU still) and 4,736 B in FP64
+  // (four still: 40,672 B).
+  constexpr bool COEF = MODE == TILE_IMPLICIT && REC != PER_VARIABLE;
   __shared__ T ci[COEF ? TY : 1][COEF ? TX + 2 : 1];
   __shared__ T cj[COEF ? TY + 2 : 1][COEF ? TX : 1];
 
