@@ -286,15 +286,19 @@ def warm_start_sample(a):
     """The metric's warm-start half on a reduced population: the
     warmstart workload at 120 rockets (480 cases; the network trained for
     1,500 Adam steps on the converged fields of the training rockets), its
-    held-out quarter solved from the free stream and from the warm start.
-    The full population (2,000 rockets, 6,000 training fields) takes
-    minutes: bench.py --workload warmstart."""
+    held-out quarter solved from the free stream and from the warm start,
+    as FP64 batches to a density residual of 1e-6, where the drags of the
+    two starts can be compared at the 1e-5 bar (FP32 batches stop at their
+    residual floor, where they differ by more). The full population
+    (2,000 rockets, 6,000 training fields) takes minutes: bench.py
+    --workload warmstart."""
     b = argparse.Namespace(**vars(a))
-    b.rockets, b.train_steps, b.cpu_seconds, b.precision = WARM_SAMPLE_ROCKETS, WARM_SAMPLE_TRAIN_STEPS, 0.0, "fp32"
+    b.rockets, b.train_steps, b.cpu_seconds, b.precision = WARM_SAMPLE_ROCKETS, WARM_SAMPLE_TRAIN_STEPS, 0.0, "fp64"
     t0 = time.time()
     line = run_warmstart(b)
     c = line["config"]
-    out = {"workload": c["workload"], "held_out_cases": c["held_out_cases"], "training_fields": c["training_fields"],
+    out = {"workload": c["workload"], "precision": "fp64 batches to a density residual of 1e-6",
+           "held_out_cases": c["held_out_cases"], "training_fields": c["training_fields"],
            "train_steps": c["train_steps"], "solves_per_s_plain": c["solves_per_s_plain"],
            "solves_per_s_warm": c["solves_per_s_warm"], "speedup": c["speedup"],
            "median_steps_plain": c["median_steps_plain"], "median_steps_warm": c["median_steps_warm"],
