@@ -384,7 +384,8 @@ class Tuning:
 def autotune(batch: GpuBatch2D, steps: int = 24, implicit: bool = False, cache: Optional[str] = None,
              thresholds: Sequence[int] = (0, 2048, 8192, 32768)) -> Tuning:
     """Time ``steps`` steps of the batch under each launch configuration
-    and keep the fastest, cached as JSON by the batch's size histogram,
+    (the best of three when that takes under 20 ms) and keep the fastest,
+    cached as JSON by the batch's size histogram,
     precision and step kind. Explicit batches: the small-tile threshold,
     then (FP32) persistent launches on either tile against the per-stage
     launches. Implicit batches: the threshold, then the half-sweeps per
@@ -411,9 +412,17 @@ def autotune(batch: GpuBatch2D, steps: int = 24, implicit: bool = False, cache: 
         t.apply(batch)
         run = (lambda k: batch.iterateImplicit(k, 10.0, 2)) if implicit else batch.iterate
         run(12)  # capture the graphs outside the timing
-        t0 = time.perf_counter()
-        run(steps)
-        return time.perf_counter() - t0
+        # A timing of a few milliseconds or less (small batches) varies
+        # with the clocks from run to run: the best of three is kept.
+        best_dt = math.inf
+        for _ in range(3):
+            t0 = time.perf_counter()
+            run(steps)
+            dt = time.perf_counter() - t0
+            best_dt = min(best_dt, dt)
+            if dt > 0.02:
+                break
+        return best_dt
 
     best, best_t = Tuning(), math.inf
     for cand in thresholds:
