@@ -27,20 +27,54 @@ template <typename T> struct Vec4 { T x, y, z, w; };
 // Reciprocal, square root and division. FP32 (the stencil kernels): the
 // hardware v_rcp_f32 / v_rsq_f32 / v_sqrt_f32 (1 ulp), a / b as a * rcp(b);
 // the compiler's own division and square root wrap each in range-scaling
-// sequences of 6-10 instructions, and a face needs about a dozen. FP64 (the
-// wall-flux and drag evaluation): IEEE operations, as the oracle.
+// sequences of 6-10 instructions, and a face needs about a dozen.
+// FP64 (the FP64 solver's stencils, the wall-flux and drag evaluation): the
+// hardware v_rcp_f64 / v_rsq_f64 estimates refined by iteration, instead of
+// the compiler's IEEE sequences (div_scale / div_fmas / div_fixup around
+// the reciprocal; range scaling and class tests around the square root):
+// - reciprocal: two Newton-Raphson steps r += r (1 - x r), each squaring
+//   the relative error;
+// - quotient: q = a r(b), then one residual correction q += r (a - b q)
+//   (Markstein's last step);
+// - square root: from y = rsq(x), one step of Goldschmidt's coupled
+//   iteration on (g, h) = (x y, y / 2), then two Newton corrections
+//   g += h (x - g g); 0 for x = 0;
+// - reciprocal square root: two Newton-Raphson steps y += y (1 - x y y) / 2.
+// The results are within an ulp or two of the IEEE ones for the finite,
+// normal operands a physical state produces, not correctly rounded.
 template <typename T> __device__ __forceinline__ T bs_sqrt(T x);
 template <> __device__ __forceinline__ float bs_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
-template <> __device__ __forceinline__ double bs_sqrt<double>(double x) { return __builtin_sqrt(x); }
+template <> __device__ __forceinline__ double bs_sqrt<double>(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  const double r = __builtin_fma(-g, h, 0.5);
+  g = __builtin_fma(g, r, g);
+  h = __builtin_fma(h, r, h);
+  g = __builtin_fma(__builtin_fma(-g, g, x), h, g);
+  g = __builtin_fma(__builtin_fma(-g, g, x), h, g);
+  return x == 0.0 ? x : g;
+}
 template <typename T> __device__ __forceinline__ T bs_rcp(T x);
 template <> __device__ __forceinline__ float bs_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
-template <> __device__ __forceinline__ double bs_rcp<double>(double x) { return 1.0 / x; }
+template <> __device__ __forceinline__ double bs_rcp<double>(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+  return __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+}
 template <typename T> __device__ __forceinline__ T bs_rsqrt(T x);
 template <> __device__ __forceinline__ float bs_rsqrt<float>(float x) { return __builtin_amdgcn_rsqf(x); }
-template <> __device__ __forceinline__ double bs_rsqrt<double>(double x) { return 1.0 / __builtin_sqrt(x); }
+template <> __device__ __forceinline__ double bs_rsqrt<double>(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  y = __builtin_fma(0.5 * y, __builtin_fma(-x * y, y, 1.0), y);
+  return __builtin_fma(0.5 * y, __builtin_fma(-x * y, y, 1.0), y);
+}
 template <typename T> __device__ __forceinline__ T bs_div(T a, T b);
 template <> __device__ __forceinline__ float bs_div<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
-template <> __device__ __forceinline__ double bs_div<double>(double a, double b) { return a / b; }
+template <> __device__ __forceinline__ double bs_div<double>(double a, double b) {
+  const double r = bs_rcp(b);
+  const double q = a * r;
+  return __builtin_fma(r, __builtin_fma(-b, q, a), q);
+}
 template <typename T> __device__ __forceinline__ T bs_abs(T x) { return x < T(0) ? -x : x; }
 template <typename T> __device__ __forceinline__ T bs_min(T a, T b) { return a < b ? a : b; }
 template <typename T> __device__ __forceinline__ T bs_max(T a, T b) { return a > b ? a : b; }
