@@ -384,7 +384,8 @@ class Tuning:
 def autotune(batch: GpuBatch2D, steps: int = 24, implicit: bool = False, cache: Optional[str] = None,
              thresholds: Sequence[int] = (0, 2048, 8192, 32768)) -> Tuning:
     """Time ``steps`` steps of the batch under each launch configuration
-    (the best of three when that takes under 20 ms) and keep the fastest,
+    (stretched to about 5 ms, best of three, when that is shorter than
+    20 ms) and keep the fastest per step,
     cached as JSON by the batch's size histogram,
     precision and step kind. Explicit batches: the small-tile threshold,
     then (FP32) persistent launches on either tile against the per-stage
@@ -409,20 +410,25 @@ def autotune(batch: GpuBatch2D, steps: int = 24, implicit: bool = False, cache: 
         return t
 
     def timed(t: Tuning) -> float:
+        """Seconds per step of the launch plan t."""
         t.apply(batch)
         run = (lambda k: batch.iterateImplicit(k, 10.0, 2)) if implicit else batch.iterate
         run(12)  # capture the graphs outside the timing
-        # A timing of a few milliseconds or less (small batches) varies
-        # with the clocks from run to run: the best of three is kept.
-        best_dt = math.inf
-        for _ in range(3):
-            t0 = time.perf_counter()
-            run(steps)
-            dt = time.perf_counter() - t0
-            best_dt = min(best_dt, dt)
-            if dt > 0.02:
-                break
-        return best_dt
+        # A small batch's `steps` take well under a millisecond, where the
+        # host's clock and the launch latency vary by a tenth: such a
+        # timing is stretched to about 5 ms (whole multiples of `steps`)
+        # and the best of three is kept.
+        t0 = time.perf_counter()
+        run(steps)
+        dt = time.perf_counter() - t0
+        n = steps * min(16, max(1, math.ceil(0.005 / max(dt, 1e-6))))
+        best = dt / steps
+        if n > steps or dt < 0.02:
+            for _ in range(3):
+                t0 = time.perf_counter()
+                run(n)
+                best = min(best, (time.perf_counter() - t0) / n)
+        return best
 
     best, best_t = Tuning(), math.inf
     for cand in thresholds:
