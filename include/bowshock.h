@@ -391,8 +391,8 @@ bs_status bs_batch_read_state_f64(bs_batch *b, int32_t index, double *rho, doubl
  * `count` must be the batch's case count. */
 bs_status bs_batch_set_active(bs_batch *b, const uint8_t *active, int32_t count);
 /* Cases of fewer cells than the threshold run on 32 x 4 tiles, the others
- * on 32 x 8 tiles (64 x 4 on an FP64 batch) (set by the autotuner; 0 = all
- * on the large tile). */
+ * on 32 x 8 tiles, in either precision (set by the autotuner; 0 = all on
+ * the large tile). */
 bs_status bs_batch_set_tiling(bs_batch *b, int32_t small_tile_below_cells);
 /* No counterpart in the reference (which solves its cases one after
  * another). tile 1 (32 x 4) or 2 (32 x 8): bs_batch_iterate on an FP32
