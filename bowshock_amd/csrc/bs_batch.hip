@@ -26,6 +26,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -92,7 +93,8 @@ __global__ __launch_bounds__(256) void segment_sums(const double* __restrict__ p
 
 struct bs_batch {
   int device;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;  // every kernel and copy of the batch
+  std::map<const void*, size_t> capacity;  // elements allocated per work-list array
   bs_config scheme;
   float gamma32;
   bool fp64 = false;  // state, metrics and time steps in FP64
@@ -342,7 +344,7 @@ bs_status batch_upload_tile_args(bs_batch* b) {
       recs[(size_t)(ROLE_PREPARE + cur) * n + c] = P;
     }
   }
-  HIP_TRY(hipMemcpy(b->tile_args, recs.data(), recs.size() * sizeof(TileArgsT<T>), hipMemcpyHostToDevice),
+  HIP_TRY(copy_on(b->stream, b->tile_args, recs.data(), recs.size() * sizeof(TileArgsT<T>), hipMemcpyHostToDevice),
           "batch records");
   return BS_OK;
 }
@@ -354,7 +356,7 @@ bs_status batch_upload_implicit_args(bs_batch* b, double omega, double cfl_max, 
   std::vector<UpdateArgsT<T>> upd((size_t)3 * n);
   std::vector<TileArgsT<T>> prep((size_t)3 * n);
   TileArgsT<T>* tile_args = static_cast<TileArgsT<T>*>(b->tile_args);
-  HIP_TRY(hipMemcpy(prep.data(), tile_args + (size_t)ROLE_PREPARE * n, prep.size() * sizeof(TileArgsT<T>),
+  HIP_TRY(copy_on(b->stream, prep.data(), tile_args + (size_t)ROLE_PREPARE * n, prep.size() * sizeof(TileArgsT<T>),
                     hipMemcpyDeviceToHost),
           "batch records");
   for (int c = 0; c < n; c++) {
@@ -386,25 +388,33 @@ bs_status batch_upload_implicit_args(bs_batch* b, double omega, double cfl_max, 
       prep[(size_t)cur * n + c].omega = (T)omega;
     }
   }
-  HIP_TRY(hipMemcpy(b->relax_args, rel.data(), rel.size() * sizeof(RelaxArgsT<T>), hipMemcpyHostToDevice),
+  HIP_TRY(copy_on(b->stream, b->relax_args, rel.data(), rel.size() * sizeof(RelaxArgsT<T>), hipMemcpyHostToDevice),
           "records");
-  HIP_TRY(hipMemcpy(b->update_args, upd.data(), upd.size() * sizeof(UpdateArgsT<T>), hipMemcpyHostToDevice),
+  HIP_TRY(copy_on(b->stream, b->update_args, upd.data(), upd.size() * sizeof(UpdateArgsT<T>), hipMemcpyHostToDevice),
           "records");
-  HIP_TRY(hipMemcpy(tile_args + (size_t)ROLE_PREPARE * n, prep.data(), prep.size() * sizeof(TileArgsT<T>),
+  HIP_TRY(copy_on(b->stream, tile_args + (size_t)ROLE_PREPARE * n, prep.data(), prep.size() * sizeof(TileArgsT<T>),
                     hipMemcpyHostToDevice),
           "records");
   return BS_OK;
 }
 
+// Uploads a host list into one of the batch's device arrays on the batch's
+// stream. The array is reallocated only when the list outgrows it: the work
+// lists shrink as cases converge, so a batch driven from its own thread
+// frees no memory while another batch's thread captures a graph.
 template <typename T>
-bs_status upload_vec(T** dst, const std::vector<T>& v, const char* what) {
-  if (*dst) {
-    (void)hipFree(*dst);
+bs_status upload_vec(bs_batch* b, T** dst, const std::vector<T>& v, const char* what) {
+  size_t& cap = b->capacity[(const void*)dst];
+  if (!*dst || cap < v.size()) {
+    if (*dst) (void)hipFree(*dst);
     *dst = nullptr;
+    cap = 0;
+    const size_t n = v.size() ? v.size() : 1;
+    bs_status st = dev_alloc(dst, n, what);
+    if (st) return st;
+    cap = n;
   }
-  bs_status st = dev_alloc(dst, v.size() ? v.size() : 1, what);
-  if (st) return st;
-  if (!v.empty()) HIP_TRY(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), what);
+  if (!v.empty()) HIP_TRY(copy_on(b->stream, *dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), what);
   return BS_OK;
 }
 
@@ -440,12 +450,12 @@ bs_status batch_build_lists(bs_batch* b) {
   }
   bs_status st;
   for (int cl = 0; cl < 2; cl++)
-    if ((st = upload_vec(&b->work_tiles[cl], tiles[cl], "work list"))) return st;
-  if ((st = upload_vec(&b->work_relax, relax, "work list")) ||
-      (st = upload_vec(&b->work_wall, wall, "work list")) || (st = upload_vec(&b->seg_first, first, "segments")) ||
-      (st = upload_vec(&b->seg_count, cnt, "segments")) || (st = upload_vec(&b->seg_cells, cells, "segments")) ||
-      (st = upload_vec(&b->seg_cases, seg_cases, "segments")) || (st = upload_vec(&b->wall_first, wfirst, "segments")) ||
-      (st = upload_vec(&b->wall_count, wcnt, "segments")))
+    if ((st = upload_vec(b, &b->work_tiles[cl], tiles[cl], "work list"))) return st;
+  if ((st = upload_vec(b, &b->work_relax, relax, "work list")) ||
+      (st = upload_vec(b, &b->work_wall, wall, "work list")) || (st = upload_vec(b, &b->seg_first, first, "segments")) ||
+      (st = upload_vec(b, &b->seg_count, cnt, "segments")) || (st = upload_vec(b, &b->seg_cells, cells, "segments")) ||
+      (st = upload_vec(b, &b->seg_cases, seg_cases, "segments")) || (st = upload_vec(b, &b->wall_first, wfirst, "segments")) ||
+      (st = upload_vec(b, &b->wall_count, wcnt, "segments")))
     return st;
   b->n_tiles[0] = (int)tiles[0].size();
   b->n_tiles[1] = (int)tiles[1].size();
@@ -509,8 +519,8 @@ bs_status batch_build_persistent(bs_batch* b) {
     for (size_t k = 0; k < lists[x].size(); k++) items[x * longest + k] = lists[x][k];
   }
   bs_status st;
-  if ((st = upload_vec(&b->persist_items, items, "persistent tiles")) ||
-      (st = upload_vec(&b->persist_lens, lens, "persistent tiles")))
+  if ((st = upload_vec(b, &b->persist_items, items, "persistent tiles")) ||
+      (st = upload_vec(b, &b->persist_lens, lens, "persistent tiles")))
     return st;
   b->persist_stride = (int)longest;
   b->persist_blocks = PERSIST_XCDS * (int)longest + SPARE;
@@ -654,14 +664,14 @@ bs_status batch_run(bs_batch* b, StepGraph& g, int count, Encode encode) {
 // keeping their implicit CFL number (a new start, as bs_set_primitives).
 bs_status batch_reset_control(bs_batch* b, int first, int count) {
   std::vector<Control> ctl(count);
-  HIP_TRY(hipMemcpy(ctl.data(), b->control + first, count * sizeof(Control), hipMemcpyDeviceToHost), "control");
+  HIP_TRY(copy_on(b->stream, ctl.data(), b->control + first, count * sizeof(Control), hipMemcpyDeviceToHost), "control");
   for (auto& c : ctl) {
     const double cfl = c.cfl;
     memset(&c, 0, sizeof c);
     c.cfl = cfl > 0 ? cfl : 1.0;
     c.t_end = 3e38;
   }
-  HIP_TRY(hipMemcpy(b->control + first, ctl.data(), count * sizeof(Control), hipMemcpyHostToDevice), "control");
+  HIP_TRY(copy_on(b->stream, b->control + first, ctl.data(), count * sizeof(Control), hipMemcpyHostToDevice), "control");
   return BS_OK;
 }
 
@@ -669,7 +679,7 @@ bs_status batch_reset_control(bs_batch* b, int first, int count) {
 // current state.
 template <typename T>
 bs_status upload_case_state(bs_batch* b, const BatchCase& k, const std::vector<T>& packed) {
-  HIP_TRY(hipMemcpy(arena<T>(b->state[b->cur], 4 * k.cell), packed.data(), packed.size() * sizeof(T),
+  HIP_TRY(copy_on(b->stream, arena<T>(b->state[b->cur], 4 * k.cell), packed.data(), packed.size() * sizeof(T),
                     hipMemcpyHostToDevice),
           "state upload");
   return BS_OK;
@@ -716,9 +726,9 @@ bs_status batch_read_state(bs_batch* b, int index, V* const dst[4]) {
     if (!dst[q]) continue;
     const T* src = arena<T>(b->state[b->cur], 4 * k.cell + (size_t)q * k.n);
     if (std::is_same<T, V>::value) {
-      HIP_TRY(hipMemcpy(dst[q], src, k.n * sizeof(T), hipMemcpyDeviceToHost), "state readback");
+      HIP_TRY(copy_on(b->stream, dst[q], src, k.n * sizeof(T), hipMemcpyDeviceToHost), "state readback");
     } else {
-      HIP_TRY(hipMemcpy(v.data(), src, k.n * sizeof(T), hipMemcpyDeviceToHost), "state readback");
+      HIP_TRY(copy_on(b->stream, v.data(), src, k.n * sizeof(T), hipMemcpyDeviceToHost), "state readback");
       for (int c = 0; c < k.n; c++) dst[q][c] = (V)v[c];
     }
   }
@@ -925,7 +935,7 @@ bs_status batch_setup(bs_batch* b, const bs_config* configs) {
       geo[5 * t + 3] = jny[t];
       geo[5 * t + 4] = ja[t];
     }
-    HIP_TRY(hipMemcpy(b->wall_geo + 5 * k.wall, geo.data(), geo.size() * sizeof(double), hipMemcpyHostToDevice),
+    HIP_TRY(copy_on(b->stream, b->wall_geo + 5 * k.wall, geo.data(), geo.size() * sizeof(double), hipMemcpyHostToDevice),
             "wall geometry");
     BoundaryDevT<T>& B = bnds[c];
     WallBoundaryT<T>& W = wbs[c];
@@ -960,12 +970,12 @@ bs_status batch_setup(bs_batch* b, const bs_config* configs) {
       A.partials = b->wall_partials;
     }
   }
-  HIP_TRY(hipMemcpy(b->bnd, bnds.data(), bnds.size() * sizeof(BoundaryDevT<T>), hipMemcpyHostToDevice),
+  HIP_TRY(copy_on(b->stream, b->bnd, bnds.data(), bnds.size() * sizeof(BoundaryDevT<T>), hipMemcpyHostToDevice),
           "boundaries");
-  HIP_TRY(hipMemcpy(b->wall_bnd, wbs.data(), wbs.size() * sizeof(WallBoundaryT<T>), hipMemcpyHostToDevice),
+  HIP_TRY(copy_on(b->stream, b->wall_bnd, wbs.data(), wbs.size() * sizeof(WallBoundaryT<T>), hipMemcpyHostToDevice),
           "boundaries");
-  HIP_TRY(hipMemcpy(b->control, ctl.data(), ctl.size() * sizeof(Control), hipMemcpyHostToDevice), "control");
-  HIP_TRY(hipMemcpy(b->wall_args, wargs.data(), wargs.size() * sizeof(WallArgsT<T>), hipMemcpyHostToDevice),
+  HIP_TRY(copy_on(b->stream, b->control, ctl.data(), ctl.size() * sizeof(Control), hipMemcpyHostToDevice), "control");
+  HIP_TRY(copy_on(b->stream, b->wall_args, wargs.data(), wargs.size() * sizeof(WallArgsT<T>), hipMemcpyHostToDevice),
           "records");
   return batch_upload_tile_args<T>(b);
 }
@@ -1103,9 +1113,9 @@ bs_status batch_drags(bs_batch* b, const double* p_inf, const double* q_inf, con
   // p_inf enters the wall kernel per case: set it in the records.
   WallArgsT<T>* recs = static_cast<WallArgsT<T>*>(b->wall_args) + (size_t)b->cur * b->count;
   std::vector<WallArgsT<T>> wargs((size_t)b->count);
-  HIP_TRY(hipMemcpy(wargs.data(), recs, wargs.size() * sizeof(WallArgsT<T>), hipMemcpyDeviceToHost), "records");
+  HIP_TRY(copy_on(b->stream, wargs.data(), recs, wargs.size() * sizeof(WallArgsT<T>), hipMemcpyDeviceToHost), "records");
   for (int c = 0; c < b->count; c++) wargs[c].p_inf = p_inf[c];
-  HIP_TRY(hipMemcpy(recs, wargs.data(), wargs.size() * sizeof(WallArgsT<T>), hipMemcpyHostToDevice), "records");
+  HIP_TRY(copy_on(b->stream, recs, wargs.data(), wargs.size() * sizeof(WallArgsT<T>), hipMemcpyHostToDevice), "records");
   if (b->n_wall)
     hipLaunchKernelGGL(batch_wall_fn<T>(b->scheme), dim3(b->n_wall), dim3(256), 0, b->stream, recs,
                        static_cast<const WallBoundaryT<T>*>(b->wall_bnd), b->work_wall);
@@ -1334,9 +1344,9 @@ bs_status bs_batch_set_implicit_cfl(bs_batch* b, double cfl) {
   BATCH_GUARD(b);
   if (!(cfl > 0)) return fail(BS_ERR_RANGE, "the CFL number must be positive");
   std::vector<Control> ctl(b->count);
-  HIP_TRY(hipMemcpy(ctl.data(), b->control, ctl.size() * sizeof(Control), hipMemcpyDeviceToHost), "control");
+  HIP_TRY(copy_on(b->stream, ctl.data(), b->control, ctl.size() * sizeof(Control), hipMemcpyDeviceToHost), "control");
   for (auto& c : ctl) c.cfl = cfl;
-  HIP_TRY(hipMemcpy(b->control, ctl.data(), ctl.size() * sizeof(Control), hipMemcpyHostToDevice), "control");
+  HIP_TRY(copy_on(b->stream, b->control, ctl.data(), ctl.size() * sizeof(Control), hipMemcpyHostToDevice), "control");
   return BS_OK;
 }
 
@@ -1358,9 +1368,9 @@ bs_status bs_batch_set_implicit_cfls(bs_batch* b, const double* cfl) {
   for (int c = 0; c < b->count; c++)
     if (!(cfl[c] > 0)) return fail(BS_ERR_RANGE, "the CFL number of case %d must be positive", c);
   std::vector<Control> ctl(b->count);
-  HIP_TRY(hipMemcpy(ctl.data(), b->control, ctl.size() * sizeof(Control), hipMemcpyDeviceToHost), "control");
+  HIP_TRY(copy_on(b->stream, ctl.data(), b->control, ctl.size() * sizeof(Control), hipMemcpyDeviceToHost), "control");
   for (int c = 0; c < b->count; c++) ctl[c].cfl = cfl[c];
-  HIP_TRY(hipMemcpy(b->control, ctl.data(), ctl.size() * sizeof(Control), hipMemcpyHostToDevice), "control");
+  HIP_TRY(copy_on(b->stream, b->control, ctl.data(), ctl.size() * sizeof(Control), hipMemcpyHostToDevice), "control");
   return BS_OK;
 }
 
@@ -1428,7 +1438,7 @@ bs_status bs_batch_drags(bs_batch* b, const double* p_inf, const double* q_inf, 
 bs_status bs_batch_status(bs_batch* b, uint32_t* status, int64_t* steps) {
   BATCH_GUARD(b);
   std::vector<Control> ctl(b->count);
-  HIP_TRY(hipMemcpy(ctl.data(), b->control, ctl.size() * sizeof(Control), hipMemcpyDeviceToHost), "control");
+  HIP_TRY(copy_on(b->stream, ctl.data(), b->control, ctl.size() * sizeof(Control), hipMemcpyDeviceToHost), "control");
   for (int c = 0; c < b->count; c++) {
     if (status) status[c] = ctl[c].status;
     if (steps) steps[c] = ctl[c].steps;

@@ -357,12 +357,12 @@ bs_status account(bs_solver* s, const Control& c, long long submitted) {
 
 bs_status reset_control(bs_solver* s) {
   Control c;
-  HIP_TRY(hipMemcpy(&c, s->d.control, sizeof c, hipMemcpyDeviceToHost), "control readback");
+  HIP_TRY(copy_on(s->stream, &c, s->d.control, sizeof c, hipMemcpyDeviceToHost), "control readback");
   double cfl = c.cfl;
   memset(&c, 0, sizeof c);
   c.cfl = cfl > 0 ? cfl : 1.0;
   c.t_end = 3e38;
-  HIP_TRY(hipMemcpy(s->d.control, &c, sizeof c, hipMemcpyHostToDevice), "control upload");
+  HIP_TRY(copy_on(s->stream, s->d.control, &c, sizeof c, hipMemcpyHostToDevice), "control upload");
   s->steps_accounted = 0;
   return BS_OK;
 }
@@ -405,11 +405,11 @@ bs_status real_alloc(const bs_solver* s, void** p, size_t count, const char* wha
 // Copies FP64 host values to `dst` in the solver's state precision.
 bs_status copy_real(const bs_solver* s, void* dst, const double* src, size_t count, const char* what) {
   if (s->fp64) {
-    HIP_TRY(hipMemcpy(dst, src, count * sizeof(double), hipMemcpyHostToDevice), what);
+    HIP_TRY(copy_on(s->stream, dst, src, count * sizeof(double), hipMemcpyHostToDevice), what);
     return BS_OK;
   }
   std::vector<float> f(src, src + count);
-  HIP_TRY(hipMemcpy(dst, f.data(), count * sizeof(float), hipMemcpyHostToDevice), what);
+  HIP_TRY(copy_on(s->stream, dst, f.data(), count * sizeof(float), hipMemcpyHostToDevice), what);
   return BS_OK;
 }
 
@@ -613,7 +613,7 @@ static bs_status solver_create(bs_ctx* ctx, const bs_config* c, bool fp64, bs_so
       geo[5 * t + 4] = far;
     }
     if ((st = dev_alloc(&s->d.face_geo[side], geo.size(), "boundary geometry"))) return bail(st);
-    HIP_TRY(hipMemcpy(s->d.face_geo[side], geo.data(), geo.size() * sizeof(double), hipMemcpyHostToDevice),
+    HIP_TRY(copy_on(s->stream, s->d.face_geo[side], geo.data(), geo.size() * sizeof(double), hipMemcpyHostToDevice),
             "boundary geometry");
     if (c->boundaries[side].kind == BS_BC_FACE_STATES) {
       if ((st = real_alloc(s, &s->d.face_cons[side], 4 * (size_t)m, "face states")) ||
@@ -630,7 +630,7 @@ static bs_status solver_create(bs_ctx* ctx, const bs_config* c, bool fp64, bs_so
   memset(&init, 0, sizeof init);
   init.cfl = 1.0;
   init.t_end = 3e38;
-  HIP_TRY(hipMemcpy(s->d.control, &init, sizeof init, hipMemcpyHostToDevice), "control init");
+  HIP_TRY(copy_on(s->stream, s->d.control, &init, sizeof init, hipMemcpyHostToDevice), "control init");
   if ((st = upload_boundary(s))) return bail(st);
   *out = s;
   return BS_OK;
@@ -694,13 +694,13 @@ static bs_status set_state_from(bs_solver* s, const V* const src[4]) {
   for (int q = 0; q < 4; q++) {
     void* dst = base + q * n * real_size(s);
     if (same) {
-      HIP_TRY(hipMemcpy(dst, src[q], n * sizeof(V), hipMemcpyHostToDevice), "state upload");
+      HIP_TRY(copy_on(s->stream, dst, src[q], n * sizeof(V), hipMemcpyHostToDevice), "state upload");
     } else if (s->fp64) {
       std::vector<double> v(src[q], src[q] + n);
-      HIP_TRY(hipMemcpy(dst, v.data(), n * sizeof(double), hipMemcpyHostToDevice), "state upload");
+      HIP_TRY(copy_on(s->stream, dst, v.data(), n * sizeof(double), hipMemcpyHostToDevice), "state upload");
     } else {
       std::vector<float> v(src[q], src[q] + n);
-      HIP_TRY(hipMemcpy(dst, v.data(), n * sizeof(float), hipMemcpyHostToDevice), "state upload");
+      HIP_TRY(copy_on(s->stream, dst, v.data(), n * sizeof(float), hipMemcpyHostToDevice), "state upload");
     }
   }
   return reset_control(s);
@@ -935,12 +935,12 @@ static constexpr double RESIDUAL_PRECISION = 1e-8;
 extern "C++" {
 // Copies `count` values of an FP32 device array into an FP64 one through
 // the host (a check made once per solve).
-static bs_status widen_copy(const void* src, void* dst, size_t count, const char* what) {
+static bs_status widen_copy(hipStream_t stream, const void* src, void* dst, size_t count, const char* what) {
   std::vector<float> f(count);
   std::vector<double> d(count);
-  HIP_TRY(hipMemcpy(f.data(), src, count * sizeof(float), hipMemcpyDeviceToHost), what);
+  HIP_TRY(copy_on(stream, f.data(), src, count * sizeof(float), hipMemcpyDeviceToHost), what);
   for (size_t k = 0; k < count; k++) d[k] = f[k];
-  HIP_TRY(hipMemcpy(dst, d.data(), count * sizeof(double), hipMemcpyHostToDevice), what);
+  HIP_TRY(copy_on(stream, dst, d.data(), count * sizeof(double), hipMemcpyHostToDevice), what);
   return BS_OK;
 }
 
@@ -959,15 +959,15 @@ static bs_status double_residual(bs_solver* s, std::vector<double>& out) {
   };
   HIP_TRY(hipStreamSynchronize(s->stream), "precision check");
   const size_t n = s->n;
-  if ((st = widen_copy(s->d.state[s->cur], twin->d.state[0], 4 * n, "precision check state"))) return done(st);
+  if ((st = widen_copy(s->stream, s->d.state[s->cur], twin->d.state[0], 4 * n, "precision check state"))) return done(st);
   twin->cur = 0;
   for (int side = 0; side < 4; side++) {
     if (s->cfg.boundaries[side].kind != BS_BC_FACE_STATES) continue;
     const size_t m = 4 * (size_t)along(s, side);
-    if ((st = widen_copy(s->d.face_cons[side], twin->d.face_cons[side], m, "precision check faces")))
+    if ((st = widen_copy(s->stream, s->d.face_cons[side], twin->d.face_cons[side], m, "precision check faces")))
       return done(st);
     if (s->imposed[side]) {
-      if ((st = widen_copy(s->d.face_flux[side], twin->d.face_flux[side], m, "precision check fluxes")))
+      if ((st = widen_copy(s->stream, s->d.face_flux[side], twin->d.face_flux[side], m, "precision check fluxes")))
         return done(st);
       twin->imposed[side] = true;
     }
@@ -1360,7 +1360,7 @@ bs_status bs_anderson_create(bs_ctx* ctx, int64_t size, int32_t depth, bs_anders
   AndersonCtl c;
   memset(&c, 0, sizeof c);
   c.best = 1.0e308;
-  if (hipMemcpy(a->ctl, &c, sizeof c, hipMemcpyHostToDevice) != hipSuccess)
+  if (copy_on(a->stream, a->ctl, &c, sizeof c, hipMemcpyHostToDevice) != hipSuccess)
     return bail(fail(BS_ERR_EXECUTION, "anderson control upload"));
   *out = a;
   return BS_OK;
@@ -1419,7 +1419,7 @@ bs_status bs_anderson_state(bs_anderson* a, int32_t* columns, int32_t* used, int
   if (!g.ok) return fail(BS_ERR_BUSY, "bs_anderson: await the previous operation first");
   (void)hipSetDevice(a->device);
   AndersonCtl c;
-  HIP_TRY(hipMemcpy(&c, a->ctl, sizeof c, hipMemcpyDeviceToHost), "anderson state");
+  HIP_TRY(copy_on(a->stream, &c, a->ctl, sizeof c, hipMemcpyDeviceToHost), "anderson state");
   if (columns) *columns = c.count;
   if (used) *used = c.used;
   if (restarts) *restarts = c.restarts;
@@ -1592,8 +1592,8 @@ bs_status bs_set_line_grid(bs_solver* s, const double* lines, const double* frac
                      (st = dev_alloc(&s->d.height, nl, "heights")) || (st = dev_alloc(&s->d.move, nl, "moves"))))
     return st;
   drop_graph(s->fitted_graph);  // the symmetry flag is captured with the moves
-  HIP_TRY(hipMemcpy(s->d.line, lines, 4 * nl * sizeof(double), hipMemcpyHostToDevice), "lines");
-  HIP_TRY(hipMemcpy(s->d.eta, fractions, nl * (nj + 1) * sizeof(double), hipMemcpyHostToDevice), "fractions");
+  HIP_TRY(copy_on(s->stream, s->d.line, lines, 4 * nl * sizeof(double), hipMemcpyHostToDevice), "lines");
+  HIP_TRY(copy_on(s->stream, s->d.eta, fractions, nl * (nj + 1) * sizeof(double), hipMemcpyHostToDevice), "fractions");
   s->lines = true;
   s->symmetry = first_node_symmetry != 0;
   return BS_OK;
@@ -1608,8 +1608,8 @@ bs_status bs_set_heights(bs_solver* s, const double* heights, int32_t count) {
   for (size_t i = 0; i < nl; i++)
     if (!(heights[i] > 0)) return fail(BS_ERR_RANGE, "height %zu is not positive", i);
   std::vector<double> zero(nl, 0.0);
-  HIP_TRY(hipMemcpy(s->d.height, heights, nl * sizeof(double), hipMemcpyHostToDevice), "heights");
-  HIP_TRY(hipMemcpy(s->d.move, zero.data(), nl * sizeof(double), hipMemcpyHostToDevice), "moves");
+  HIP_TRY(copy_on(s->stream, s->d.height, heights, nl * sizeof(double), hipMemcpyHostToDevice), "heights");
+  HIP_TRY(copy_on(s->stream, s->d.move, zero.data(), nl * sizeof(double), hipMemcpyHostToDevice), "moves");
   line_metrics(s);
   HIP_TRY(hipGetLastError(), "line metrics");
   HIP_TRY(hipStreamSynchronize(s->stream), "line metrics");
@@ -1639,8 +1639,8 @@ bs_status bs_move_grid(bs_solver* s, const double* heights, int32_t count) {
     if (!(heights[i] > 0)) return fail(BS_ERR_RANGE, "height %zu is not positive", i);
     move[i] = heights[i] - now[i];
   }
-  HIP_TRY(hipMemcpy(s->d.height, heights, nl * sizeof(double), hipMemcpyHostToDevice), "heights");
-  HIP_TRY(hipMemcpy(s->d.move, move.data(), nl * sizeof(double), hipMemcpyHostToDevice), "moves");
+  HIP_TRY(copy_on(s->stream, s->d.height, heights, nl * sizeof(double), hipMemcpyHostToDevice), "heights");
+  HIP_TRY(copy_on(s->stream, s->d.move, move.data(), nl * sizeof(double), hipMemcpyHostToDevice), "moves");
   line_remap(s);
   line_metrics(s);
   HIP_TRY(hipGetLastError(), "grid move");

@@ -98,6 +98,16 @@ inline void drop_tail(StepGraph& g) {
   g.tail_cur = g.tail_end = -1;
 }
 
+// A blocking host-device copy on the object's own stream: ordered after the
+// work already queued there, and allowed while another thread captures a
+// graph on its stream (a copy on the legacy stream is refused then, even
+// with thread-local capture and non-blocking streams).
+inline hipError_t copy_on(hipStream_t stream, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, stream);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(stream);
+}
+
 inline void drop_graph(StepGraph& g) {
   drop_tail(g);
   if (g.exec) (void)hipGraphExecDestroy(g.exec);

@@ -870,8 +870,8 @@ bs_status bs_mlp_load_data(bs_mlp* p, const float* x, const float* y, int32_t ro
   if ((st = dev_alloc(&p->data_x, (size_t)rows * p->inputs, "data set")) ||
       (st = dev_alloc(&p->data_y, (size_t)rows * p->outputs, "data set")))
     return st;
-  HIP_TRY(hipMemcpy(p->data_x, x, (size_t)rows * p->inputs * sizeof(float), hipMemcpyHostToDevice), "data upload");
-  HIP_TRY(hipMemcpy(p->data_y, y, (size_t)rows * p->outputs * sizeof(float), hipMemcpyHostToDevice), "data upload");
+  HIP_TRY(copy_on(p->stream, p->data_x, x, (size_t)rows * p->inputs * sizeof(float), hipMemcpyHostToDevice), "data upload");
+  HIP_TRY(copy_on(p->stream, p->data_y, y, (size_t)rows * p->outputs * sizeof(float), hipMemcpyHostToDevice), "data upload");
   p->data_rows = rows;
   return BS_OK;
 }

@@ -431,3 +431,43 @@ def test_the_autotuner_picks_a_launch_plan_with_the_same_states(ctx, tmp_path, i
         assert np.array_equal(_state(tuned.readState(k)), _state(ref.readState(k))), k
     ref.destroy()
     tuned.destroy()
+
+
+def test_two_batches_driven_from_two_threads_equal_serial_runs(ctx):
+    """Each batch owns a stream for its kernels and copies. Two batches
+    advanced from two host threads at once (graph captures, work-list
+    uploads after a case retires, status and state reads, all overlapping)
+    end in the same states, bit for bit, as the same batches advanced one
+    after the other."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    cs = _cases()
+    halves = (cs[:2], cs[2:])
+
+    def run(batch, group, implicit):
+        batch.setUniform([c[1] for c in group])
+        for chunk, active in ((40, (1, 1)), (80, (1, 1)), (60, (0, 1)), (50, (1, 1))):
+            batch.setActive(np.array(active, dtype=np.uint8))
+            if implicit:
+                batch.iterateImplicit(chunk, 10.0, 2)
+            else:
+                batch.iterate(chunk)
+            batch.status()
+            batch.densityResidualNorms()
+        return [_state(batch.readState(k)) for k in range(batch.count)]
+
+    serial = []
+    for implicit, group in zip((False, True), halves):
+        b = GpuBatch2D(ctx, [c[0] for c in group])
+        serial.append(run(b, group, implicit))
+        b.destroy()
+    batches = [GpuBatch2D(ctx, [c[0] for c in group]) for group in halves]
+    with ThreadPoolExecutor(2) as ex:
+        futures = [ex.submit(run, b, group, implicit) for b, group, implicit in zip(batches, halves, (False, True))]
+        concurrent = [f.result() for f in futures]
+    for b in batches:
+        b.destroy()
+    for s, c in zip(serial, concurrent):
+        for a, d in zip(s, c):
+            assert np.all(np.isfinite(a))
+            assert np.array_equal(a, d)
