@@ -18,7 +18,9 @@
  * made while another call on the same solver is running returns BS_ERR_BUSY
  * (the reference's ExclusiveRunner, src/engine-gpu/gpu-common.ts:122-140);
  * so do bs_solver_destroy, bs_batch_destroy, bs_anderson_destroy and
- * bs_mlp_destroy, which then free nothing.
+ * bs_mlp_destroy, which then free nothing. Distinct objects may be driven
+ * from distinct threads at once: each object's kernels and copies are
+ * ordered on its own stream.
  *
  * Conventions (src/grid/structured.ts:1-8): node (i, j) at i + j (ni + 1);
  * cell (i, j) at i + j ni; sides in the order iMin, iMax, jMin, jMax.
