@@ -468,7 +468,16 @@ __device__ __forceinline__ void tile_compute(const TileArgsT<T>& A, int bx, int 
   // Phase 2: faces. Face f of direction d at local (fx, fy). Inner tiles
   // (IN) have no boundary face and no face outside the grid, so their
   // instance drops the side tests and the wall mirror.
-  auto do_face = [&](auto inner_tag, int d, int fx, int fy, const T* pre) {
+  // The implicit spectral radii need each cell's sound speed on both of its
+  // low faces and in its diagonal: the thread's own cell computes it once.
+  // Read only where the own cell lies in the grid (the faces' gi < ni and
+  // gj < nj tests, the valid cells).
+  T c_own = T(0.);
+  if constexpr (MODE == TILE_IMPLICIT) {
+    const Prim<T> w = prim_at(lj + 2, li + 2);
+    c_own = bs_sqrt(bs_div(g * w.p, w.rho));
+  }
+  auto do_face = [&](auto inner_tag, int d, int fx, int fy, const T* pre, const T* c_right) {
     constexpr bool IN = decltype(inner_tag)::value;
     const int gi = i0 + fx, gj = j0 + fy;
     T nx, ny, area;
@@ -495,7 +504,8 @@ __device__ __forceinline__ void tile_compute(const TileArgsT<T>& A, int bx, int 
       }
       if (MODE == TILE_IMPLICIT) {
         T rl = gi > 0 ? bs_abs(l0.u * nx + l0.v * ny) + bs_sqrt(bs_div(g * l0.p, l0.rho)) : T(0.);
-        T rr = gi < ni ? bs_abs(r0.u * nx + r0.v * ny) + bs_sqrt(bs_div(g * r0.p, r0.rho)) : T(0.);
+        T rr = gi < ni ? bs_abs(r0.u * nx + r0.v * ny) + (c_right ? *c_right : bs_sqrt(bs_div(g * r0.p, r0.rho)))
+                       : T(0.);
         lam = bs_max(rl, rr) * area;
         A.lam_i[fidx] = lam;
       }
@@ -530,7 +540,8 @@ __device__ __forceinline__ void tile_compute(const TileArgsT<T>& A, int bx, int 
       }
       if (MODE == TILE_IMPLICIT) {
         T rl = gj > 0 ? bs_abs(l0.u * nx + l0.v * ny) + bs_sqrt(bs_div(g * l0.p, l0.rho)) : T(0.);
-        T rr = gj < nj ? bs_abs(r0.u * nx + r0.v * ny) + bs_sqrt(bs_div(g * r0.p, r0.rho)) : T(0.);
+        T rr = gj < nj ? bs_abs(r0.u * nx + r0.v * ny) + (c_right ? *c_right : bs_sqrt(bs_div(g * r0.p, r0.rho)))
+                       : T(0.);
         lam = bs_max(rl, rr) * area;
         A.lam_j[fidx] = lam;
       }
@@ -552,11 +563,13 @@ __device__ __forceinline__ void tile_compute(const TileArgsT<T>& A, int bx, int 
   constexpr EdgePlan E = edge_plan(TX, TY);
   static_assert(E.jstart + TX <= NT && E.istart + TY <= NT, "edge faces outside the block");
   auto faces = [&](auto inner_tag) {
-    do_face(inner_tag, 0, li, lj, mi);
-    do_face(inner_tag, 1, li, lj, mj);
+    // The right cell of the thread's own two faces is its own cell.
+    const T* own = MODE == TILE_IMPLICIT ? &c_own : nullptr;
+    do_face(inner_tag, 0, li, lj, mi, own);
+    do_face(inner_tag, 1, li, lj, mj, own);
     const T* me = EDGE ? in.me : nullptr;
-    if (tid >= E.jstart && tid < E.jstart + TX) do_face(inner_tag, 1, tid - E.jstart, TY, me);
-    if (tid >= E.istart && tid < E.istart + TY) do_face(inner_tag, 0, TX, tid - E.istart, me);
+    if (tid >= E.jstart && tid < E.jstart + TX) do_face(inner_tag, 1, tid - E.jstart, TY, me, nullptr);
+    if (tid >= E.istart && tid < E.istart + TY) do_face(inner_tag, 0, TX, tid - E.istart, me, nullptr);
   };
   if (inner) faces(std::integral_constant<bool, true>());
   else faces(std::integral_constant<bool, false>());
@@ -641,7 +654,7 @@ __device__ __forceinline__ void tile_compute(const TileArgsT<T>& A, int bx, int 
       const double cfl = A.control->cfl;
       const T scale = (T)(0.5 * ((double)A.omega + 1.0 / cfl));
       T d = scale * (xi[0][lj][li] + xi[0][lj][li + 1] + xj[0][lj][li] + xj[0][lj + 1][li]);
-      if (A.axisymmetric) d += planar * (bs_abs(w.v) + bs_sqrt(bs_div(g * w.p, w.rho)));
+      if (A.axisymmetric) d += planar * (bs_abs(w.v) + c_own);
       at(A.diag, c) = d;
       // The first red half-sweep (dQ = R / D on red cells, 0 on black) is
       // formed by the relaxation's first launch from R and D.
