@@ -68,9 +68,10 @@ def parse():
     ap.add_argument("--mach", type=float, default=3.0)
     ap.add_argument("--cpu-steps", type=int, default=40, help="oracle steps timed for the CPU baseline (0: skip)")
     ap.add_argument("--no-graphs", action="store_true", help="launch kernel by kernel instead of HIP graphs")
-    ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32",
+    ap.add_argument("--precision", choices=("fp32", "fp64", "mixed"), default="fp32",
                     help="the FP32 solver (batches), or the FP64 one (state, metrics and residual in FP64; the "
-                         "batched workloads then stop on a density residual of 1e-6)")
+                         "batched workloads then stop on a density residual of 1e-6); population only: mixed, "
+                         "FP32 batches to their residual floor, then FP64 batches from their states to 1e-6")
     ap.add_argument("--traffic", default=None, help="rocprofv3 PMC summary of the stage kernel")
     ap.add_argument("--workload", choices=("single", "implicit", "sweep", "population", "warmstart", "ladder", "verified",
                                            "fitted"), default="single")
@@ -119,6 +120,8 @@ def main():
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}); "
               "the batched solves would shard by case, one process per GPU, with no exchange.", file=sys.stderr)
         sys.exit(2)
+    if a.precision == "mixed" and a.workload != "population":
+        sys.exit("--precision mixed applies to the population workload")
 
     if a.workload == "warmstart":
         print(json.dumps(run_warmstart(a)), flush=True)
@@ -487,6 +490,35 @@ class _CaseGroups:
                 results[k] = r
         return results
 
+    def solve_mixed(self, first, frees, p_inf, q_inf, plan32, plan64):
+        """`first` (FP32 batches of the same cases) to its stopping rule,
+        then these FP64 batches from its states to plan64's target
+        (batch.solve_mixed), one flux's batches after the other; steps and
+        seconds count both phases."""
+        import numpy as np
+
+        from bowshock_amd.batch import solve_mixed
+
+        results = [None] * self.n
+        t0 = time.perf_counter()
+        last = [t0]
+
+        def progress(steps, active):
+            now = time.perf_counter()
+            if now - last[0] > 20:
+                last[0] = now
+                print(f"  {now - t0:.0f}s: step {steps}, {active} cases running", file=sys.stderr, flush=True)
+
+        for (ks, b64), (ks32, b32) in zip(self.groups, first.groups):
+            assert ks == ks32
+            offset = time.perf_counter() - t0
+            res = solve_mixed(b32, b64, [frees[k] for k in ks], np.asarray(p_inf)[ks], np.asarray(q_inf)[ks], 0.5,
+                              plan32, plan64, progress=progress)
+            for k, r in zip(ks, res):
+                r.seconds += offset
+                results[k] = r
+        return results
+
     def destroy(self):
         for _, b in self.groups:
             b.destroy()
@@ -583,15 +615,23 @@ def run_batch(a):
     t_grids = time.time() - t0
     refusals = P.refusal_counts(cases)
     print(f"{len(cases)} cases, grids in {t_grids:.1f}s: {refusals}", file=sys.stderr, flush=True)
-    S = _setup_cases(cases, implicit=not sweep, precision=a.precision)
+    mixed = a.precision == "mixed"
+    if mixed and sweep:
+        sys.exit("--precision mixed applies to the population workload")
+    S = _setup_cases(cases, implicit=not sweep, precision="fp64" if mixed else a.precision)
+    S32 = _setup_cases(cases, implicit=True, precision="fp32") if mixed else None
     ok, plan = S["ok"], S["plan"]
     cells = np.array([c.cells for c in ok])
     ctx = Context(0)
     t0 = time.time()
     batch = _CaseGroups(ctx, S)
+    first = _CaseGroups(ctx, S32) if mixed else None
     t_create = time.time() - t0
     batch.setUniform(S["frees"])
     tuned = [dataclasses.asdict(t) for t in batch.autotune(steps=24 if sweep else 12, implicit=not sweep)]
+    if mixed:
+        first.setUniform(S32["frees"])
+        tuned += [dataclasses.asdict(t) for t in first.autotune(steps=12, implicit=True)]
     if a.persistent != "auto" and a.precision == "fp32":
         for t, (_, b) in zip(tuned, batch.groups):
             b.setPersistent(int(a.persistent))
@@ -601,9 +641,16 @@ def run_batch(a):
     batch.setUniform(S["frees"])
     print(f"{len(batch.groups)} batch(es) of {[len(g[0]) for g in batch.groups]} cases ({cells.sum():.3e} cells) "
           f"created in {t_create:.1f}s; launch tuning {tuned}", file=sys.stderr, flush=True)
+    if mixed:
+        first.setUniform(S32["frees"])
     t = time.perf_counter()
-    results = batch.solve(S["p_inf"], S["q_inf"], plan)
+    if mixed:
+        results = batch.solve_mixed(first, S["frees"], S["p_inf"], S["q_inf"], S32["plan"], plan)
+    else:
+        results = batch.solve(S["p_inf"], S["q_inf"], plan)
     wall = time.perf_counter() - t
+    if mixed:
+        first.destroy()
     steps = np.array([r.steps for r in results])
     converged = sum(r.converged for r in results)
     cu = float((cells * steps).sum())
@@ -632,11 +679,14 @@ def run_batch(a):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": a.precision,
+        "dtype": "fp32 then fp64" if mixed else a.precision,
         "data": "synthetic: seeded bodies (mulberry32, seed 20261020), generated H grids (sharp tips) and C grids (blunt tips), free-stream start",
         "config": {"workload": ("Mach sweep 1.2:0.1:5.0 of one seeded nose on a 4-caliber cylinder, SSP-RK3"
                                 if sweep else f"population of {a.rockets} seeded rockets x M 1.5/2/3/4, "
-                                              "LU-SGS red-black 2 sweeps, CFL 1 -> 10 x1.2"),
+                                              "LU-SGS red-black 2 sweeps, CFL 1 -> 10 x1.2"
+                                              + (", FP32 batches to their residual floor, then FP64 batches "
+                                                 "from their states to a density residual of 1e-6"
+                                                 if mixed else "")),
                    "cases": len(cases), "accepted": len(ok), "refusals": refusals, "converged": converged,
                    "outcomes": reasons, "cells": int(cells.sum()), "cell_updates": cu,
                    "cell_updates_per_s": cu / wall, "grid_seconds": t_grids, "create_seconds": t_create,

@@ -326,6 +326,40 @@ def solve(batch: GpuBatch2D, p_inf, q_inf, base_radius, plan: Optional[SolvePlan
     return res
 
 
+def solve_mixed(fp32: GpuBatch2D, fp64: GpuBatch2D, frees, p_inf, q_inf, base_radius, plan32: SolvePlan,
+                plan64: SolvePlan, progress=None) -> List[CaseResult]:
+    """FP32 steps while they pay, then FP64 steps to the residual target.
+    The FP32 batch, started by the caller, runs ``plan32`` (its stopping
+    rule accepts the FP32 residual floor once the drag has settled); every
+    case's state is then widened into the FP64 batch of the same grids
+    (bs_batch_set_from_coarser: on equal grids each cell takes its own
+    state), and the FP64 batch runs ``plan64`` from there, implicit cases
+    at plan64's final CFL number. A case whose FP32 state went non-physical
+    restarts in FP64 from its free stream ``frees[k]`` with the CFL ramp.
+    The results are the FP64 phase's, with the steps, residual history and
+    seconds of both phases."""
+    if fp32.precision != "fp32" or fp64.precision != "fp64":
+        raise RangeError("solve_mixed takes an FP32 batch and an FP64 batch")
+    if fp32.count != fp64.count or list(fp32.shapes) != list(fp64.shapes):
+        raise RangeError("the two batches must hold the same grids")
+    start = time.perf_counter()
+    first = solve(fp32, p_inf, q_inf, base_radius, plan32, progress=progress)
+    fp64.setFromCoarser(fp32)
+    cfl = np.full(fp64.count, plan64.cfl)
+    for k, r in enumerate(first):
+        if r.reason == "non-physical state":
+            fp64.setCellPrimitives(k, frees[k])
+            cfl[k] = plan64.cfl_start
+    offset = time.perf_counter() - start
+    res = solve(fp64, p_inf, q_inf, base_radius, plan64, cfl_start=cfl if plan64.implicit else None,
+                progress=progress)
+    for a, r in zip(first, res):
+        r.steps += a.steps
+        r.seconds += offset
+        r.history = a.history + r.history
+    return res
+
+
 def solve_single(gpu, p_inf: float, q_inf: float, base_radius: float, plan: Optional[SolvePlan] = None,
                  reference: Optional[float] = None) -> CaseResult:
     """The same stopping rule for one GpuEuler2D solve."""

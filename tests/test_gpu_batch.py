@@ -471,3 +471,36 @@ def test_two_batches_driven_from_two_threads_equal_serial_runs(ctx):
         for a, d in zip(s, c):
             assert np.all(np.isfinite(a))
             assert np.array_equal(a, d)
+
+
+def test_mixed_precision_solve_reaches_the_fp64_target(ctx):
+    """FP32 steps to the FP32 stopping rule, then FP64 steps from the
+    widened states: every case reaches the FP64 residual target, with the
+    drag of the FP64 solve from the free stream within the drag tolerance,
+    and the steps of both phases counted."""
+    from bowshock_amd.batch import solve_mixed
+
+    cs = _cases()
+    cfgs = [c[0] for c in cs]
+    frees = [c[1] for c in cs]
+    p = np.array([c[2] for c in cs])
+    q = np.array([c[3] for c in cs])
+    plan32 = SolvePlan(implicit=True, chunk=60, max_steps=6000, cfl=10.0, cfl_start=1.0, sweeps=2)
+    plan64 = SolvePlan(implicit=True, chunk=60, max_steps=12000, cfl=10.0, cfl_start=1.0, sweeps=2, residual=1e-6,
+                       drop=0.0, stall=10 ** 9)
+    b64 = GpuBatch2D(ctx, cfgs, "fp64")
+    b64.setUniform(frees)
+    plain = solve(b64, p, q, 0.5, plan64)
+    b32 = GpuBatch2D(ctx, cfgs)
+    b32.setUniform(frees)
+    b64.setUniform(frees)
+    with pytest.raises(RangeError):
+        solve_mixed(b64, b32, frees, p, q, 0.5, plan32, plan64)
+    mixed = solve_mixed(b32, b64, frees, p, q, 0.5, plan32, plan64)
+    for k, (a, m) in enumerate(zip(plain, mixed)):
+        assert a.converged and m.converged, (k, a.reason, m.reason)
+        assert m.residual <= 1e-6
+        assert m.cd == pytest.approx(a.cd, rel=1e-5)
+        assert len(m.history) * plan64.chunk == m.steps
+    b32.destroy()
+    b64.destroy()
